@@ -1,0 +1,134 @@
+/*
+ * k2g.h -- C ABI of the MI355X Kraken 2 classify core (libk2g.so).
+ *
+ * Plain C types only; no HIP or torch types cross this boundary.  Every
+ * function returns 0 on success or a sysexits-style status (64 usage, 65 data
+ * error, 66 no input, 70 internal error, 71 OS error, 74 I/O error); the
+ * message of the last failure on the calling thread is in k2g_last_error().
+ * Nothing here calls exit().
+ *
+ * Threading: a k2g_db is read-only after k2g_db_open and may be shared by
+ * threads (the reference's KeyValueStore is const and shared the same way,
+ * src/classify.cc:500-587).  A k2g_batch owns one HIP stream and its buffers
+ * and must be used by one thread at a time.
+ */
+#ifndef K2G_H_
+#define K2G_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define K2G_OK 0
+#define K2G_EUSAGE 64
+#define K2G_EDATA 65
+#define K2G_ENOINPUT 66
+#define K2G_ESOFTWARE 70
+#define K2G_EOS 71
+#define K2G_EIO 74
+
+/* Hitlist run codes above any internal taxid. */
+#define K2G_CODE_MATE 0xFFFFFFFFu  /* "|:|" mate-pair border   (src/classify.cc:33) */
+#define K2G_CODE_FRAME 0xFFFFFFFEu /* "-:-" reading-frame border (src/classify.cc:34) */
+#define K2G_CODE_AMBIG 0xFFFFFFFDu /* "A:n" ambiguous span     (src/classify.cc:35) */
+
+typedef struct k2g_db k2g_db;
+typedef struct k2g_batch k2g_batch;
+
+typedef struct {
+  uint64_t capacity, size, key_bits, value_bits; /* hash.k2d header */
+  uint64_t node_count;                           /* taxo.k2d */
+  uint64_t k, l, spaced_seed_mask, toggle_mask, minimum_acceptable_hash_value;
+  int32_t dna_db, revcom_version; /* opts.k2d */
+  uint64_t device_bytes;          /* HBM held by the resident DB */
+} k2g_db_info;
+
+/* Replaces load_index (src/classify.cc:269-302): reads hash.k2d, taxo.k2d
+ * and opts.k2d, uploads the cell array and the parent array to HBM and keeps
+ * them resident.  Rejects 40-bit cells (src/kv_store.h:35-57), taxonomies
+ * whose ids are not parent-before-child, and cells holding a taxid that is
+ * not a node. */
+int k2g_db_open(const char *hash_k2d, const char *taxo_k2d, const char *opts_k2d, k2g_db **out);
+int k2g_db_close(k2g_db *db);
+int k2g_db_get_info(const k2g_db *db, k2g_db_info *info);
+/* External (NCBI) taxid of each internal id, node_count entries. */
+int k2g_db_external_ids(const k2g_db *db, uint64_t *out, uint64_t n);
+const char *k2g_last_error(void);
+
+/* KeyValueStore::GetBatch (src/kv_store.h:26-33; src/compact_hash.h:354-392):
+ * out[i] = value of keys[i] or 0.  Host arrays; synchronous. */
+int k2g_get_batch(k2g_db *db, const uint64_t *keys, uint32_t *out, size_t n);
+/* Same on device arrays, enqueued on a HIP stream (NULL = default stream). */
+int k2g_get_batch_device(k2g_db *db, const uint64_t *d_keys, uint32_t *d_out, size_t n,
+                         void *hip_stream);
+
+/* ---- classify: the ProcessFiles -> ClassifySequence seam
+ * (src/classify.cc:479-708 -> 773-948) */
+typedef struct {
+  double confidence;        /* -T, in [0, 1] */
+  int32_t min_hit_groups;   /* -g */
+  int32_t quick;            /* -q */
+  int32_t paired;           /* -P / -S: every fragment is a pair (unless per-fragment flags) */
+  int32_t mode;             /* 0: plain, one launch per batch; 1: persistent, longest first */
+  int32_t want_kmer_counts; /* -R: per-taxon LOOKUP-hit counts */
+  int32_t want_hll;         /* -K: per-hit (taxon, encoded HLL hash) records */
+  int32_t table_slots;      /* per-fragment LDS hit table (power of 2 in [2,256]; 0 = 256) */
+} k2g_classify_params;
+
+/* Fragment i: mate 1 = seq[off[2i], off[2i+1]), mate 2 = seq[off[2i+1],
+ * off[2i+2]) (empty for single reads).  paired may be NULL (then
+ * params.paired applies to all) or give a 0/1 flag per fragment. */
+typedef struct {
+  const char *seq;
+  const uint64_t *off; /* 2n+1 entries */
+  const uint8_t *paired;
+  uint64_t n;
+} k2g_fragments;
+
+/* Results of the last k2g_classify on a batch, in host memory owned by the
+ * batch (valid until the next call on it).  Runs of fragment i are the
+ * (code, count) pairs runs[2*run_off[i] .. 2*run_off[i+1]); codes are
+ * internal taxids (0 = no hit) or K2G_CODE_*.  HLL records likewise at
+ * hll_off.  kmer_counts has node_count entries. */
+typedef struct {
+  uint64_t n;
+  const uint32_t *call;        /* internal taxid, 0 = unclassified */
+  const uint32_t *hit_groups;  /* minimizer_hit_groups */
+  const uint32_t *total_kmers; /* confidence denominator */
+  const uint64_t *run_off;     /* n+1 */
+  const uint32_t *runs;
+  const uint64_t *hll_off;     /* n+1, or NULL */
+  const uint32_t *hll;         /* (taxon, encoded) pairs, or NULL */
+  const uint64_t *kmer_counts; /* or NULL */
+  uint64_t node_count;
+  uint64_t redone;             /* fragments that took the dense-table path */
+  double kernel_ms;            /* device time of the classify kernels */
+} k2g_results;
+
+int k2g_batch_create(k2g_db *db, k2g_batch **out);
+int k2g_batch_destroy(k2g_batch *b);
+
+/* Host fragments -> host results (uploads, classifies, downloads; sync). */
+int k2g_classify(k2g_batch *b, const k2g_classify_params *p, const k2g_fragments *in,
+                 k2g_results *out);
+
+/* Device-resident fragments (all pointers in HBM): classifies on the batch's
+ * stream and leaves results in the batch's device buffers; call
+ * k2g_batch_fetch to synchronise and copy them to host memory.  order may be
+ * NULL; otherwise the persistent kernel visits fragments in that order. */
+int k2g_classify_device(k2g_batch *b, const k2g_classify_params *p, const char *d_seq,
+                        const uint64_t *d_off, const uint8_t *d_paired, uint64_t n,
+                        const uint32_t *d_order);
+int k2g_batch_sync(k2g_batch *b, double *kernel_ms);
+int k2g_batch_fetch(k2g_batch *b, k2g_results *out);
+/* HIP stream of the batch (a hipStream_t), for callers that enqueue their
+ * own work around k2g_classify_device. */
+void *k2g_batch_stream(k2g_batch *b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* K2G_H_ */

@@ -1,0 +1,22 @@
+"""MI355X-native Kraken 2 classify core.
+
+The hot path (minimizer scan, compact-hash probe, hit-count/LCA resolution)
+runs as HIP kernels for gfx950 inside libk2g.so; this package is the thin
+Python binding used by tests and the benchmark.  The C ABI is
+include/k2g.h; the classify command-line front-end is bin/k2g_classify.
+"""
+from ._lib import (  # noqa: F401
+    BIN_DIR,
+    CODE_AMBIG,
+    CODE_FRAME,
+    CODE_MATE,
+    EXPORTED_SYMBOLS,
+    LIB_PATH,
+    Batch,
+    ClassifyOutput,
+    Db,
+    K2GError,
+    format_hitlist,
+    load_library,
+    pack_fragments,
+)

@@ -1,0 +1,295 @@
+"""ctypes bindings of libk2g.so (declared in include/k2g.h).
+
+The shared library is built in-tree (kraken2_amd/lib/libk2g.so); nothing here
+falls back to a CPU path: if the library or a GPU is missing, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libk2g.so")
+BIN_DIR = os.path.join(PKG_DIR, "bin")
+
+CODE_MATE = 0xFFFFFFFF
+CODE_FRAME = 0xFFFFFFFE
+CODE_AMBIG = 0xFFFFFFFD
+
+EXPORTED_SYMBOLS = (
+    "k2g_db_open",
+    "k2g_db_close",
+    "k2g_db_get_info",
+    "k2g_db_external_ids",
+    "k2g_last_error",
+    "k2g_get_batch",
+    "k2g_get_batch_device",
+    "k2g_batch_create",
+    "k2g_batch_destroy",
+    "k2g_classify",
+    "k2g_classify_device",
+    "k2g_batch_sync",
+    "k2g_batch_fetch",
+    "k2g_batch_stream",
+)
+
+
+class K2GError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"k2g error {code}: {msg}")
+        self.code = code
+
+
+class DbInfo(C.Structure):
+    _fields_ = [
+        ("capacity", C.c_uint64),
+        ("size", C.c_uint64),
+        ("key_bits", C.c_uint64),
+        ("value_bits", C.c_uint64),
+        ("node_count", C.c_uint64),
+        ("k", C.c_uint64),
+        ("l", C.c_uint64),
+        ("spaced_seed_mask", C.c_uint64),
+        ("toggle_mask", C.c_uint64),
+        ("minimum_acceptable_hash_value", C.c_uint64),
+        ("dna_db", C.c_int32),
+        ("revcom_version", C.c_int32),
+        ("device_bytes", C.c_uint64),
+    ]
+
+
+class ClassifyParams(C.Structure):
+    _fields_ = [
+        ("confidence", C.c_double),
+        ("min_hit_groups", C.c_int32),
+        ("quick", C.c_int32),
+        ("paired", C.c_int32),
+        ("mode", C.c_int32),
+        ("want_kmer_counts", C.c_int32),
+        ("want_hll", C.c_int32),
+        ("table_slots", C.c_int32),
+    ]
+
+
+class Fragments(C.Structure):
+    _fields_ = [
+        ("seq", C.c_void_p),
+        ("off", C.c_void_p),
+        ("paired", C.c_void_p),
+        ("n", C.c_uint64),
+    ]
+
+
+class Results(C.Structure):
+    _fields_ = [
+        ("n", C.c_uint64),
+        ("call", C.POINTER(C.c_uint32)),
+        ("hit_groups", C.POINTER(C.c_uint32)),
+        ("total_kmers", C.POINTER(C.c_uint32)),
+        ("run_off", C.POINTER(C.c_uint64)),
+        ("runs", C.POINTER(C.c_uint32)),
+        ("hll_off", C.POINTER(C.c_uint64)),
+        ("hll", C.POINTER(C.c_uint32)),
+        ("kmer_counts", C.POINTER(C.c_uint64)),
+        ("node_count", C.c_uint64),
+        ("redone", C.c_uint64),
+        ("kernel_ms", C.c_double),
+    ]
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Loads libk2g.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise K2GError(70, f"{path} is not built (run __graft_entry__.build())")
+    lib = C.CDLL(path)
+    vp, u64, i32 = C.c_void_p, C.c_uint64, C.c_int
+    lib.k2g_db_open.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(vp)]
+    lib.k2g_db_close.argtypes = [vp]
+    lib.k2g_db_get_info.argtypes = [vp, C.POINTER(DbInfo)]
+    lib.k2g_db_external_ids.argtypes = [vp, vp, u64]
+    lib.k2g_last_error.restype = C.c_char_p
+    lib.k2g_get_batch.argtypes = [vp, vp, vp, C.c_size_t]
+    lib.k2g_get_batch_device.argtypes = [vp, vp, vp, C.c_size_t, vp]
+    lib.k2g_batch_create.argtypes = [vp, C.POINTER(vp)]
+    lib.k2g_batch_destroy.argtypes = [vp]
+    lib.k2g_classify.argtypes = [vp, C.POINTER(ClassifyParams), C.POINTER(Fragments), C.POINTER(Results)]
+    lib.k2g_classify_device.argtypes = [vp, C.POINTER(ClassifyParams), vp, vp, vp, u64, vp]
+    lib.k2g_batch_sync.argtypes = [vp, C.POINTER(C.c_double)]
+    lib.k2g_batch_fetch.argtypes = [vp, C.POINTER(Results)]
+    lib.k2g_batch_stream.argtypes = [vp]
+    lib.k2g_batch_stream.restype = vp
+    for name in EXPORTED_SYMBOLS:
+        if name not in ("k2g_last_error", "k2g_batch_stream"):
+            getattr(lib, name).restype = i32
+    _lib = lib
+    return lib
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise K2GError(rc, _lib.k2g_last_error().decode())
+
+
+@dataclass
+class ClassifyOutput:
+    call: np.ndarray          # internal taxid per fragment
+    hit_groups: np.ndarray
+    total_kmers: np.ndarray
+    run_off: np.ndarray       # n+1
+    runs: np.ndarray          # (m, 2) uint32: code, count
+    hll_off: np.ndarray | None
+    hll: np.ndarray | None    # (r, 2) uint32: taxon, encoded hash
+    kmer_counts: np.ndarray | None
+    redone: int
+    kernel_ms: float
+
+
+class Db:
+    """A resident database (hash.k2d + taxo.k2d + opts.k2d) on the current GPU."""
+
+    def __init__(self, hash_k2d: str, taxo_k2d: str, opts_k2d: str):
+        lib = load_library()
+        h = C.c_void_p()
+        _check(lib.k2g_db_open(hash_k2d.encode(), taxo_k2d.encode(), opts_k2d.encode(), C.byref(h)))
+        self._h = h
+        info = DbInfo()
+        _check(lib.k2g_db_get_info(h, C.byref(info)))
+        self.info = info
+        ext = np.zeros(info.node_count, dtype=np.uint64)
+        _check(lib.k2g_db_external_ids(h, ext.ctypes.data, info.node_count))
+        self.external_ids = ext
+
+    @classmethod
+    def from_dir(cls, d: str) -> "Db":
+        return cls(os.path.join(d, "hash.k2d"), os.path.join(d, "taxo.k2d"), os.path.join(d, "opts.k2d"))
+
+    def close(self):
+        if self._h:
+            _lib.k2g_db_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_batch(self, keys: np.ndarray) -> np.ndarray:
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        out = np.zeros(keys.shape[0], dtype=np.uint32)
+        _check(_lib.k2g_get_batch(self._h, keys.ctypes.data, out.ctypes.data, keys.shape[0]))
+        return out
+
+
+def pack_fragments(mates1, mates2=None):
+    """Concatenates reads into (seq bytes, 2n+1 offsets).  mates2 None = single-end."""
+    n = len(mates1)
+    parts = []
+    off = np.zeros(2 * n + 1, dtype=np.uint64)
+    pos = 0
+    for i in range(n):
+        a = mates1[i].encode() if isinstance(mates1[i], str) else mates1[i]
+        off[2 * i] = pos
+        parts.append(a)
+        pos += len(a)
+        off[2 * i + 1] = pos
+        if mates2 is not None:
+            b = mates2[i].encode() if isinstance(mates2[i], str) else mates2[i]
+            parts.append(b)
+            pos += len(b)
+    off[2 * n] = pos
+    return b"".join(parts), off
+
+
+class Batch:
+    def __init__(self, db: Db):
+        self.db = db
+        h = C.c_void_p()
+        _check(_lib.k2g_batch_create(db._h, C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            _lib.k2g_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def classify(self, seq: bytes, off: np.ndarray, *, paired=False, paired_flags=None,
+                 confidence=0.0, min_hit_groups=0, quick=False, mode=0, kmer_counts=False,
+                 hll=False, table_slots=0) -> ClassifyOutput:
+        off = np.ascontiguousarray(off, dtype=np.uint64)
+        n = (off.shape[0] - 1) // 2
+        p = ClassifyParams(confidence, min_hit_groups, int(quick), int(paired), mode,
+                           int(kmer_counts), int(hll), table_slots)
+        sbuf = np.frombuffer(seq, dtype=np.uint8) if len(seq) else np.zeros(1, np.uint8)
+        flags = None
+        if paired_flags is not None:
+            flags = np.ascontiguousarray(paired_flags, dtype=np.uint8)
+        fr = Fragments(sbuf.ctypes.data, off.ctypes.data, flags.ctypes.data if flags is not None else None, n)
+        res = Results()
+        _check(_lib.k2g_classify(self._h, C.byref(p), C.byref(fr), C.byref(res)))
+        return _copy_results(res, hll, kmer_counts)
+
+
+def _arr(ptr, n, dtype):
+    if n == 0 or not ptr:
+        return np.zeros(0, dtype=dtype)
+    return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype, copy=True)
+
+
+def _copy_results(res: Results, hll: bool, kmer_counts: bool) -> ClassifyOutput:
+    n = res.n
+    run_off = _arr(res.run_off, n + 1, np.uint64)
+    m = int(run_off[-1]) if n else 0
+    runs = _arr(res.runs, 2 * m, np.uint32).reshape(-1, 2)
+    hoff = hrec = None
+    if hll and res.hll_off:
+        hoff = _arr(res.hll_off, n + 1, np.uint64)
+        hrec = _arr(res.hll, 2 * int(hoff[-1]), np.uint32).reshape(-1, 2)
+    kc = None
+    if kmer_counts and res.kmer_counts:
+        kc = _arr(res.kmer_counts, res.node_count, np.uint64)
+    return ClassifyOutput(
+        call=_arr(res.call, n, np.uint32),
+        hit_groups=_arr(res.hit_groups, n, np.uint32),
+        total_kmers=_arr(res.total_kmers, n, np.uint32),
+        run_off=run_off,
+        runs=runs,
+        hll_off=hoff,
+        hll=hrec,
+        kmer_counts=kc,
+        redone=int(res.redone),
+        kernel_ms=float(res.kernel_ms),
+    )
+
+
+def format_hitlist(runs: np.ndarray, external_ids: np.ndarray) -> str:
+    """Kraken hitlist text of one fragment's runs (src/classify.cc:950-991)."""
+    if runs.shape[0] == 0:
+        return "0:0"
+    out = []
+    last = runs.shape[0] - 1
+    for i, (code, cnt) in enumerate(runs.tolist()):
+        if code == CODE_MATE:
+            out.append("|:|" if i == last else "|:| ")
+        elif code == CODE_FRAME:
+            out.append("-:-" if i == last else "-:- ")
+        elif code == CODE_AMBIG:
+            out.append(f"A:{cnt} ")
+        else:
+            out.append(f"{int(external_ids[code])}:{cnt}" + ("" if i == last else " "))
+    return "".join(out)

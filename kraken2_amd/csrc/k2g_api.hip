@@ -1,0 +1,751 @@
+// Host side of libk2g.so: resident-DB management, batch buffers and the
+// launch sequence of the classify core.  Implements include/k2g.h.
+#include "k2g.h"
+
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cerrno>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fcntl.h>
+#include <string>
+#include <sys/stat.h>
+#include <unistd.h>
+#include <vector>
+
+#include "k2g_device.h"
+#include "k2g_kernels.h"
+
+namespace k2g {
+__global__ void k_classify_plain(DbView, ClassifyParams, FragBatch, FragOut);
+__global__ void k_classify_persistent(DbView, ClassifyParams, FragBatch, FragOut,
+                                      unsigned long long *);
+__global__ void k_classify_dense(DbView, ClassifyParams, FragBatch, FragOut, const uint32_t *,
+                                 uint32_t, DenseScratch);
+__global__ void k_get_batch(DbView, const uint64_t *, uint32_t *, uint64_t);
+__global__ void k_max_value(DbView, unsigned int *);
+}  // namespace k2g
+
+using namespace k2g;
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string &msg) {
+  g_error = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return fail(K2G_ESOFTWARE, std::string(#expr) + " failed: " + hipGetErrorString(e_)); \
+  } while (0)
+
+// Growable device buffer.
+template <class T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t cap = 0;
+  hipError_t reserve(size_t n) {
+    if (n <= cap) return hipSuccess;
+    if (p) (void) hipFree(p);
+    p = nullptr;
+    cap = 0;
+    size_t want = std::max<size_t>(n, 64);
+    hipError_t e = hipMalloc((void **) &p, want * sizeof(T));
+    if (e == hipSuccess) cap = want;
+    return e;
+  }
+  void release() {
+    if (p) (void) hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+bool read_fully(int fd, void *buf, size_t n, off_t off) {
+  char *p = (char *) buf;
+  while (n) {
+    ssize_t r = pread(fd, p, n, off);
+    if (r < 0 && errno == EINTR) continue;
+    if (r <= 0) return false;
+    p += r;
+    n -= (size_t) r;
+    off += r;
+  }
+  return true;
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------------
+struct k2g_db {
+  DbView view{};
+  uint32_t *d_cells = nullptr;
+  uint32_t *d_parent = nullptr;
+  std::vector<uint64_t> external_ids;
+  k2g_db_info info{};
+  int device = 0;
+  int num_cus = 0;
+};
+
+struct k2g_batch {
+  k2g_db *db = nullptr;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // inputs (host-path uploads)
+  DevBuf<char> seq;
+  DevBuf<uint64_t> off;
+  DevBuf<uint8_t> paired;
+  // per-fragment device outputs
+  DevBuf<uint32_t> call, groups, total, n_runs, status, n_recs, order, redo;
+  DevBuf<uint64_t> run_off, bounds, comp_off, comp_rec_off;
+  DevBuf<uint32_t> runs, recs, comp_runs, comp_recs;
+  DevBuf<unsigned long long> kmer_counts, cursor;
+  DevBuf<uint32_t> redo_count;
+  DevBuf<unsigned char> temp;
+  DevBuf<uint32_t> sort_keys, sort_keys_out, sort_vals;
+  DevBuf<uint32_t> dense_counts, dense_lists, dense_lens;
+  uint32_t dense_slots = 0;
+  // host copies of the last results
+  std::vector<uint32_t> h_call, h_groups, h_total, h_runs, h_recs;
+  std::vector<uint64_t> h_run_off, h_rec_off, h_kmer_counts;
+  // last launch
+  uint64_t n = 0;
+  k2g_classify_params params{};
+  uint64_t redone = 0;
+  double kernel_ms = 0;
+  int persistent_blocks = 0;
+};
+
+extern "C" const char *k2g_last_error(void) { return g_error.c_str(); }
+
+// ----------------------------------------------------------------------
+// DB open: src/classify.cc:269-302, src/compact_hash.h:278-341,
+// src/taxonomy.cc:195-238
+extern "C" int k2g_db_open(const char *hash_k2d, const char *taxo_k2d, const char *opts_k2d,
+                           k2g_db **out) {
+  if (!hash_k2d || !taxo_k2d || !opts_k2d || !out) return fail(K2G_EUSAGE, "null argument");
+  *out = nullptr;
+  // opts.k2d: raw IndexOptions (src/kraken2_data.h:15-25), read up to the
+  // file size into a zeroed struct (src/classify.cc:272-279).
+  struct RawOpts {
+    uint64_t k, l, spaced, toggle;
+    uint8_t dna, pad[7];
+    uint64_t min_hash;
+    int32_t revcom, db_version, db_type, pad2;
+  } ro;
+  static_assert(sizeof(RawOpts) == 64, "opts.k2d layout");
+  std::memset(&ro, 0, sizeof(ro));
+  {
+    int fd = open(opts_k2d, O_RDONLY);
+    if (fd < 0) return fail(K2G_EOS, std::string("unable to open ") + opts_k2d);
+    struct stat sb;
+    fstat(fd, &sb);
+    size_t want = std::min<size_t>(sizeof(ro), (size_t) sb.st_size);
+    bool ok = read_fully(fd, &ro, want, 0);
+    close(fd);
+    if (!ok) return fail(K2G_EIO, std::string("error reading ") + opts_k2d);
+  }
+  const bool dna = ro.dna != 0;
+  const uint32_t bits = dna ? 2 : 4;
+  if (ro.l == 0 || ro.l > (uint64_t) (63 / bits))
+    return fail(K2G_ESOFTWARE, std::string("l exceeds size limits for minimizer ") +
+                                   (dna ? "nucleotide" : "protein") + " scanner");
+  if (ro.k < ro.l) return fail(K2G_EDATA, "opts.k2d: k < l");
+  if (ro.k - ro.l > 63) return fail(K2G_EDATA, "opts.k2d: k - l > 63 is not supported by the GPU scan");
+  if (ro.revcom > 1 || ro.revcom < 0) return fail(K2G_EDATA, "opts.k2d: unknown revcom version");
+
+  // taxo.k2d
+  std::vector<uint32_t> parent;
+  std::vector<uint64_t> ext;
+  {
+    FILE *f = fopen(taxo_k2d, "rb");
+    if (!f) return fail(K2G_EOS, std::string("unable to open ") + taxo_k2d);
+    char magic[8];
+    uint64_t hdr[3];
+    if (fread(magic, 1, 8, f) != 8 || std::memcmp(magic, "K2TAXDAT", 8) != 0) {
+      fclose(f);
+      return fail(K2G_EDATA, std::string("malformed taxonomy file ") + taxo_k2d);
+    }
+    if (fread(hdr, 8, 3, f) != 3 || hdr[0] == 0 || hdr[0] >= (1ULL << 31)) {
+      fclose(f);
+      return fail(K2G_EDATA, std::string("read exhausted taxonomy information in ") + taxo_k2d);
+    }
+    const uint64_t nodes = hdr[0];
+    std::vector<uint64_t> raw(nodes * 7);
+    size_t got = fread(raw.data(), 56, nodes, f);
+    fclose(f);
+    if (got != nodes)
+      return fail(K2G_EDATA, std::string("read exhausted taxonomy information in ") + taxo_k2d);
+    parent.resize(nodes);
+    ext.resize(nodes);
+    for (uint64_t i = 0; i < nodes; i++) {
+      const uint64_t pid = raw[i * 7 + 0];
+      // IsAAncestorOfB / LowestCommonAncestor rely on parent < child
+      // (src/taxonomy.cc:249-272); anything else could walk forever.
+      if (i > 0 && pid >= i)
+        return fail(K2G_EDATA, "taxonomy node " + std::to_string(i) + " has parent id " +
+                                   std::to_string(pid) + " (parents must precede children)");
+      parent[i] = (uint32_t) (i == 0 ? 0 : pid);
+      ext[i] = raw[i * 7 + 5];
+    }
+  }
+
+  // hash.k2d header
+  int fd = open(hash_k2d, O_RDONLY);
+  if (fd < 0) return fail(K2G_EOS, std::string("unable to open ") + hash_k2d);
+  uint64_t hh[4];
+  if (!read_fully(fd, hh, sizeof(hh), 0)) {
+    close(fd);
+    return fail(K2G_EDATA, std::string("unexpected end of file reading ") + hash_k2d);
+  }
+  const uint64_t capacity = hh[0], kb = hh[2], vb = hh[3];
+  if (kb + vb == 40) {
+    close(fd);
+    return fail(K2G_EDATA, "40-bit compact hash cells are not supported");
+  }
+  if (kb + vb != 32 || kb == 0 || vb == 0 || vb > 31 || capacity == 0) {
+    close(fd);
+    return fail(K2G_EDATA, "Unable to determine width of compact hash cell");
+  }
+  struct stat sb;
+  fstat(fd, &sb);
+  if ((uint64_t) sb.st_size < 32 + capacity * 4) {
+    close(fd);
+    return fail(K2G_EDATA, std::string("unexpected end of file reading ") + hash_k2d);
+  }
+
+  auto *db = new k2g_db();
+  hipError_t e = hipGetDevice(&db->device);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&db->num_cus, hipDeviceAttributeMultiprocessorCount, db->device);
+  if (e != hipSuccess) {
+    close(fd);
+    delete db;
+    return fail(K2G_ESOFTWARE, std::string("no HIP device: ") + hipGetErrorString(e));
+  }
+  const uint64_t padded = (capacity + 63) / 64 * 64;
+  e = hipMalloc((void **) &db->d_cells, padded * 4);
+  if (e == hipSuccess) e = hipMemset(db->d_cells + capacity, 0, (padded - capacity) * 4);
+  if (e == hipSuccess) e = hipMalloc((void **) &db->d_parent, parent.size() * 4);
+  if (e == hipSuccess)
+    e = hipMemcpy(db->d_parent, parent.data(), parent.size() * 4, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    close(fd);
+    k2g_db_close(db);
+    return fail(K2G_EOS, std::string("unable to allocate device memory for the table: ") +
+                             hipGetErrorString(e));
+  }
+  // stream the cell array through a pinned staging buffer
+  {
+    const size_t chunk = 64ull << 20;
+    char *stage[2] = {nullptr, nullptr};
+    hipStream_t s;
+    e = hipStreamCreate(&s);
+    if (e == hipSuccess) e = hipHostMalloc((void **) &stage[0], chunk, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void **) &stage[1], chunk, hipHostMallocDefault);
+    hipEvent_t done[2];
+    hipEventCreate(&done[0]);
+    hipEventCreate(&done[1]);
+    const uint64_t bytes = capacity * 4;
+    bool ok = e == hipSuccess;
+    int which = 0;
+    for (uint64_t pos = 0; ok && pos < bytes; pos += chunk, which ^= 1) {
+      const size_t n = (size_t) std::min<uint64_t>(chunk, bytes - pos);
+      hipEventSynchronize(done[which]);
+      if (!read_fully(fd, stage[which], n, (off_t) (32 + pos))) {
+        ok = false;
+        break;
+      }
+      if (hipMemcpyAsync((char *) db->d_cells + pos, stage[which], n, hipMemcpyHostToDevice, s) !=
+              hipSuccess ||
+          hipEventRecord(done[which], s) != hipSuccess)
+        ok = false;
+    }
+    hipStreamSynchronize(s);
+    hipEventDestroy(done[0]);
+    hipEventDestroy(done[1]);
+    if (stage[0]) hipHostFree(stage[0]);
+    if (stage[1]) hipHostFree(stage[1]);
+    hipStreamDestroy(s);
+    close(fd);
+    if (!ok) {
+      k2g_db_close(db);
+      return fail(K2G_EIO, std::string("error reading ") + hash_k2d);
+    }
+  }
+
+  DbView &v = db->view;
+  v.cells = db->d_cells;
+  v.capacity = capacity;
+  v.barrett = capacity > 1 ? (uint64_t) (((unsigned __int128) 1 << 64) / capacity) : 0;
+  v.key_bits = (uint32_t) kb;
+  v.value_bits = (uint32_t) vb;
+  v.parent = db->d_parent;
+  v.node_count = (uint32_t) parent.size();
+  v.k = (uint32_t) ro.k;
+  v.l = (uint32_t) ro.l;
+  v.dna = dna ? 1 : 0;
+  v.revcom_version = (uint32_t) ro.revcom;
+  v.lmer_mask = (1ULL << (ro.l * bits)) - 1;
+  v.spaced_mask = ro.spaced;
+  v.toggle_mask = ro.toggle & v.lmer_mask;
+  v.min_hash = ro.min_hash;
+
+  // every stored value must be a node of the taxonomy
+  {
+    unsigned int *d_max = nullptr, h_max = 0;
+    e = hipMalloc((void **) &d_max, sizeof(unsigned int));
+    if (e == hipSuccess) e = hipMemset(d_max, 0, sizeof(unsigned int));
+    if (e == hipSuccess) {
+      const unsigned grid = (unsigned) std::min<uint64_t>((capacity + 255) / 256, 65536);
+      hipLaunchKernelGGL(k_max_value, dim3(grid), dim3(256), 0, 0, v, d_max);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(&h_max, d_max, sizeof(h_max), hipMemcpyDeviceToHost);
+    if (d_max) hipFree(d_max);
+    if (e != hipSuccess) {
+      k2g_db_close(db);
+      return fail(K2G_ESOFTWARE, std::string("table check failed: ") + hipGetErrorString(e));
+    }
+    if (h_max >= v.node_count) {
+      k2g_db_close(db);
+      return fail(K2G_EDATA, "hash table holds taxid " + std::to_string(h_max) +
+                                 " but the taxonomy has only " + std::to_string(v.node_count) +
+                                 " nodes");
+    }
+  }
+
+  db->external_ids = std::move(ext);
+  k2g_db_info &inf = db->info;
+  inf.capacity = capacity;
+  inf.size = hh[1];
+  inf.key_bits = kb;
+  inf.value_bits = vb;
+  inf.node_count = parent.size();
+  inf.k = ro.k;
+  inf.l = ro.l;
+  inf.spaced_seed_mask = ro.spaced;
+  inf.toggle_mask = ro.toggle;
+  inf.minimum_acceptable_hash_value = ro.min_hash;
+  inf.dna_db = dna ? 1 : 0;
+  inf.revcom_version = ro.revcom;
+  inf.device_bytes = padded * 4 + parent.size() * 4;
+  *out = db;
+  return K2G_OK;
+}
+
+extern "C" int k2g_db_close(k2g_db *db) {
+  if (!db) return K2G_OK;
+  if (db->d_cells) hipFree(db->d_cells);
+  if (db->d_parent) hipFree(db->d_parent);
+  delete db;
+  return K2G_OK;
+}
+
+extern "C" int k2g_db_get_info(const k2g_db *db, k2g_db_info *info) {
+  if (!db || !info) return fail(K2G_EUSAGE, "null argument");
+  *info = db->info;
+  return K2G_OK;
+}
+
+extern "C" int k2g_db_external_ids(const k2g_db *db, uint64_t *out, uint64_t n) {
+  if (!db || !out) return fail(K2G_EUSAGE, "null argument");
+  if (n < db->external_ids.size()) return fail(K2G_EUSAGE, "output too small for node_count ids");
+  std::memcpy(out, db->external_ids.data(), db->external_ids.size() * 8);
+  return K2G_OK;
+}
+
+// ----------------------------------------------------------------------
+extern "C" int k2g_get_batch_device(k2g_db *db, const uint64_t *d_keys, uint32_t *d_out, size_t n,
+                                    void *hip_stream) {
+  if (!db) return fail(K2G_EUSAGE, "null db");
+  if (n == 0) return K2G_OK;
+  if (!d_keys || !d_out) return fail(K2G_EUSAGE, "null array");
+  const uint64_t blocks = (n + 255) / 256;
+  hipLaunchKernelGGL(k_get_batch, dim3((unsigned) blocks), dim3(256), 0,
+                     (hipStream_t) hip_stream, db->view, d_keys, d_out, (uint64_t) n);
+  HIP_TRY(hipGetLastError());
+  return K2G_OK;
+}
+
+extern "C" int k2g_get_batch(k2g_db *db, const uint64_t *keys, uint32_t *out, size_t n) {
+  if (!db) return fail(K2G_EUSAGE, "null db");
+  if (n == 0) return K2G_OK;
+  if (!keys || !out) return fail(K2G_EUSAGE, "null array");
+  uint64_t *dk = nullptr;
+  uint32_t *dv = nullptr;
+  HIP_TRY(hipMalloc((void **) &dk, n * 8));
+  hipError_t e = hipMalloc((void **) &dv, n * 4);
+  if (e == hipSuccess) e = hipMemcpy(dk, keys, n * 8, hipMemcpyHostToDevice);
+  int rc = K2G_OK;
+  if (e == hipSuccess) {
+    rc = k2g_get_batch_device(db, dk, dv, n, nullptr);
+    if (rc == K2G_OK) e = hipMemcpy(out, dv, n * 4, hipMemcpyDeviceToHost);
+  }
+  hipFree(dk);
+  if (dv) hipFree(dv);
+  if (e != hipSuccess) return fail(K2G_ESOFTWARE, std::string("get_batch: ") + hipGetErrorString(e));
+  return rc;
+}
+
+// ----------------------------------------------------------------------
+// batch
+namespace k2g {
+
+// Exact element count of a fragment: one per k-mer end of each scanned
+// stream plus border markers (src/classify.cc:797-838); runs and HLL
+// records never outnumber it.
+__global__ void k_bounds(const uint64_t *off, const uint8_t *paired, uint32_t paired_all,
+                         uint64_t n, uint32_t k, uint32_t dna, uint64_t *bounds, uint32_t *keys) {
+  const uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > n) return;
+  if (i == n) {
+    bounds[n] = 0;
+    return;
+  }
+  const bool pr = paired ? paired[i] != 0 : paired_all != 0;
+  uint64_t e = 0;
+  for (int m = 0; m < (pr ? 2 : 1); m++) {
+    const uint64_t L = off[2 * i + m + 1] - off[2 * i + m];
+    if (dna) {
+      e += L + 1 > k ? L + 1 - k : 0;
+    } else {
+      for (int fr = 0; fr < 6; fr++) {
+        uint64_t fl;
+        if (L < 3) {
+          fl = 1;
+        } else {
+          const uint64_t first = (fr % 3) == 0 ? 3 : ((fr % 3) == 1 ? 4 : 2);
+          fl = (L - 1 >= first) ? (L - 1 - first) / 3 + 1 : 0;
+        }
+        e += fl + 1 > k ? fl + 1 - k : 0;
+      }
+      e += 5;
+    }
+  }
+  if (pr) e += 1;
+  bounds[i] = e;
+  if (keys) keys[i] = (uint32_t) (e < 0xFFFFFFFFull ? e : 0xFFFFFFFFull);
+}
+
+__global__ void k_iota(uint32_t *v, uint64_t n) {
+  const uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = (uint32_t) i;
+}
+
+__global__ void k_collect_redo(const uint32_t *status, uint64_t n, uint32_t *list,
+                               uint32_t *count) {
+  const uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && (status[i] & 1u)) list[atomicAdd(count, 1u)] = (uint32_t) i;
+}
+
+// Copies each fragment's runs (2 words each) into a dense array.
+__global__ __launch_bounds__(64) void k_compact(const uint64_t *src_off, const uint32_t *counts,
+                                                const uint64_t *dst_off, const uint32_t *src,
+                                                uint32_t *dst, uint64_t n) {
+  for (uint64_t f = blockIdx.x; f < n; f += gridDim.x) {
+    const uint64_t s = src_off[f], d = dst_off[f];
+    const uint32_t c = counts[f];
+    for (uint32_t j = threadIdx.x; j < 2 * c; j += 64) dst[2 * d + j] = src[2 * s + j];
+  }
+}
+
+__global__ void k_widen(const uint32_t *in, uint64_t *out, uint64_t n) {
+  const uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = in[i];
+  if (i == n) out[n] = 0;
+}
+
+}  // namespace k2g
+
+extern "C" int k2g_batch_create(k2g_db *db, k2g_batch **out) {
+  if (!db || !out) return fail(K2G_EUSAGE, "null argument");
+  auto *b = new k2g_batch();
+  b->db = db;
+  hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&b->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+  int per_cu = 0;
+  if (e == hipSuccess)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_classify_persistent, 64, 0);
+  if (e != hipSuccess) {
+    k2g_batch_destroy(b);
+    return fail(K2G_ESOFTWARE, std::string("batch setup: ") + hipGetErrorString(e));
+  }
+  b->persistent_blocks = std::max(1, per_cu) * std::max(1, db->num_cus);
+  *out = b;
+  return K2G_OK;
+}
+
+extern "C" int k2g_batch_destroy(k2g_batch *b) {
+  if (!b) return K2G_OK;
+  if (b->stream) hipStreamSynchronize(b->stream);
+  for (auto *buf : {&b->seq}) buf->release();
+  b->off.release();
+  b->paired.release();
+  for (auto *buf : {&b->call, &b->groups, &b->total, &b->n_runs, &b->status, &b->n_recs, &b->order,
+                    &b->redo, &b->runs, &b->recs, &b->comp_runs, &b->comp_recs, &b->redo_count,
+                    &b->sort_keys, &b->sort_keys_out, &b->sort_vals, &b->dense_counts,
+                    &b->dense_lists, &b->dense_lens})
+    buf->release();
+  for (auto *buf : {&b->run_off, &b->bounds, &b->comp_off, &b->comp_rec_off}) buf->release();
+  b->kmer_counts.release();
+  b->cursor.release();
+  b->temp.release();
+  if (b->ev0) hipEventDestroy(b->ev0);
+  if (b->ev1) hipEventDestroy(b->ev1);
+  if (b->stream) hipStreamDestroy(b->stream);
+  delete b;
+  return K2G_OK;
+}
+
+extern "C" void *k2g_batch_stream(k2g_batch *b) { return b ? (void *) b->stream : nullptr; }
+
+static int check_params(const k2g_classify_params *p) {
+  if (!p) return fail(K2G_EUSAGE, "null params");
+  if (!(p->confidence >= 0 && p->confidence <= 1))
+    return fail(K2G_EUSAGE, "confidence threshold must be in [0, 1]");
+  if (p->mode != 0 && p->mode != 1) return fail(K2G_EUSAGE, "mode must be 0 (plain) or 1 (persistent)");
+  const int s = p->table_slots;
+  if (s != 0 && (s < 2 || s > 256 || (s & (s - 1)) != 0))
+    return fail(K2G_EUSAGE, "table_slots must be a power of two in [2, 256]");
+  return K2G_OK;
+}
+
+extern "C" int k2g_classify_device(k2g_batch *b, const k2g_classify_params *p, const char *d_seq,
+                                   const uint64_t *d_off, const uint8_t *d_paired, uint64_t n,
+                                   const uint32_t *d_order) {
+  if (!b) return fail(K2G_EUSAGE, "null batch");
+  int rc = check_params(p);
+  if (rc) return rc;
+  k2g_db *db = b->db;
+  const DbView &v = db->view;
+  b->n = n;
+  b->params = *p;
+  b->redone = 0;
+  b->kernel_ms = 0;
+  if (n == 0) return K2G_OK;
+  if (n >= (1ULL << 32)) return fail(K2G_EUSAGE, "batch too large (fragments must fit 32 bits)");
+  if (!d_seq || !d_off) return fail(K2G_EUSAGE, "null fragment arrays");
+  hipStream_t s = b->stream;
+
+  HIP_TRY(b->call.reserve(n));
+  HIP_TRY(b->groups.reserve(n));
+  HIP_TRY(b->total.reserve(n));
+  HIP_TRY(b->n_runs.reserve(n));
+  HIP_TRY(b->status.reserve(n));
+  HIP_TRY(b->n_recs.reserve(n));
+  HIP_TRY(b->run_off.reserve(n + 1));
+  HIP_TRY(b->bounds.reserve(n + 1));
+  HIP_TRY(b->sort_keys.reserve(n));
+  HIP_TRY(b->redo_count.reserve(1));
+  HIP_TRY(b->cursor.reserve(1));
+
+  // element bounds -> run offsets
+  const unsigned g1 = (unsigned) ((n + 1 + 255) / 256);
+  hipLaunchKernelGGL(k_bounds, dim3(g1), dim3(256), 0, s, d_off, d_paired,
+                     (uint32_t) (p->paired != 0), n, v.k, v.dna, b->bounds.p,
+                     p->mode == 1 && !d_order ? b->sort_keys.p : nullptr);
+  HIP_TRY(hipGetLastError());
+  size_t tbytes = 0;
+  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tbytes, b->bounds.p, b->run_off.p, n + 1, s));
+  HIP_TRY(b->temp.reserve(tbytes));
+  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(b->temp.p, tbytes, b->bounds.p, b->run_off.p, n + 1, s));
+  uint64_t total_elems = 0;
+  HIP_TRY(hipMemcpyAsync(&total_elems, b->run_off.p + n, 8, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  HIP_TRY(b->runs.reserve(2 * std::max<uint64_t>(total_elems, 1)));
+  if (p->want_hll) HIP_TRY(b->recs.reserve(2 * std::max<uint64_t>(total_elems, 1)));
+  if (p->want_kmer_counts) {
+    HIP_TRY(b->kmer_counts.reserve(v.node_count));
+    HIP_TRY(hipMemsetAsync(b->kmer_counts.p, 0, v.node_count * 8, s));
+  }
+
+  // persistent order: longest fragments first
+  const uint32_t *order = d_order;
+  if (p->mode == 1 && !order) {
+    HIP_TRY(b->order.reserve(n));
+    HIP_TRY(b->sort_vals.reserve(n));
+    HIP_TRY(b->sort_keys_out.reserve(n));
+    const unsigned gi = (unsigned) ((n + 255) / 256);
+    hipLaunchKernelGGL(k_iota, dim3(gi), dim3(256), 0, s, b->sort_vals.p, n);
+    size_t sbytes = 0;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, sbytes, b->sort_keys.p,
+                                                         b->sort_keys_out.p, b->sort_vals.p,
+                                                         b->order.p, (int) n, 0, 32, s));
+    HIP_TRY(b->temp.reserve(sbytes));
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairsDescending(b->temp.p, sbytes, b->sort_keys.p,
+                                                         b->sort_keys_out.p, b->sort_vals.p,
+                                                         b->order.p, (int) n, 0, 32, s));
+    order = b->order.p;
+  }
+
+  ClassifyParams cp;
+  cp.confidence = p->confidence;
+  cp.min_hit_groups = p->min_hit_groups;
+  cp.quick = p->quick ? 1u : 0u;
+  cp.tab_cap = p->table_slots ? (uint32_t) p->table_slots : 256u;
+  FragBatch in{d_seq, d_off, d_paired, (uint32_t) (p->paired != 0), n, order};
+  FragOut out{b->call.p,    b->groups.p,  b->total.p, b->n_runs.p,
+              b->status.p,  b->n_recs.p,  b->run_off.p, b->runs.p,
+              p->want_hll ? b->recs.p : nullptr,
+              p->want_kmer_counts ? b->kmer_counts.p : nullptr};
+
+  HIP_TRY(hipEventRecord(b->ev0, s));
+  if (p->mode == 0) {
+    hipLaunchKernelGGL(k_classify_plain, dim3((unsigned) n), dim3(64), 0, s, v, cp, in, out);
+  } else {
+    HIP_TRY(hipMemsetAsync(b->cursor.p, 0, 8, s));
+    const unsigned grid = (unsigned) std::min<uint64_t>(b->persistent_blocks, n);
+    hipLaunchKernelGGL(k_classify_persistent, dim3(grid), dim3(64), 0, s, v, cp, in, out,
+                       b->cursor.p);
+  }
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(b->ev1, s));
+
+  // fragments whose hit table overflowed: classify again with a dense table
+  HIP_TRY(b->redo.reserve(n));
+  HIP_TRY(hipMemsetAsync(b->redo_count.p, 0, 4, s));
+  hipLaunchKernelGGL(k_collect_redo, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, s,
+                     b->status.p, n, b->redo.p, b->redo_count.p);
+  HIP_TRY(hipGetLastError());
+  uint32_t n_redo = 0;
+  HIP_TRY(hipMemcpyAsync(&n_redo, b->redo_count.p, 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (n_redo) {
+    const uint32_t slots = std::min<uint32_t>(n_redo, 256);
+    if (b->dense_slots < slots) {
+      HIP_TRY(b->dense_counts.reserve((uint64_t) slots * v.node_count));
+      HIP_TRY(b->dense_lists.reserve((uint64_t) slots * v.node_count));
+      HIP_TRY(b->dense_lens.reserve(slots));
+      HIP_TRY(hipMemsetAsync(b->dense_counts.p, 0, (uint64_t) slots * v.node_count * 4, s));
+      HIP_TRY(hipMemsetAsync(b->dense_lens.p, 0, slots * 4, s));
+      b->dense_slots = slots;
+    }
+    DenseScratch ds{b->dense_counts.p, b->dense_lists.p, b->dense_lens.p, slots};
+    FragOut out2 = out;
+    out2.kmer_counts = nullptr;  // already counted by the first pass
+    hipLaunchKernelGGL(k_classify_dense, dim3(slots), dim3(64), 0, s, v, cp, in, out2, b->redo.p,
+                       n_redo, ds);
+    HIP_TRY(hipGetLastError());
+    b->redone = n_redo;
+  }
+  return K2G_OK;
+}
+
+extern "C" int k2g_batch_sync(k2g_batch *b, double *kernel_ms) {
+  if (!b) return fail(K2G_EUSAGE, "null batch");
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  float ms = 0;
+  if (b->n) HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+  b->kernel_ms = ms;
+  if (kernel_ms) *kernel_ms = ms;
+  return K2G_OK;
+}
+
+extern "C" int k2g_batch_fetch(k2g_batch *b, k2g_results *r) {
+  if (!b || !r) return fail(K2G_EUSAGE, "null argument");
+  int rc = k2g_batch_sync(b, nullptr);
+  if (rc) return rc;
+  const uint64_t n = b->n;
+  hipStream_t s = b->stream;
+  std::memset(r, 0, sizeof(*r));
+  r->n = n;
+  r->node_count = b->db->view.node_count;
+  r->redone = b->redone;
+  r->kernel_ms = b->kernel_ms;
+  b->h_call.resize(n);
+  b->h_groups.resize(n);
+  b->h_total.resize(n);
+  b->h_run_off.assign(n + 1, 0);
+  if (n) {
+    std::vector<uint32_t> st(n);
+    HIP_TRY(hipMemcpy(st.data(), b->status.p, n * 4, hipMemcpyDeviceToHost));
+    for (uint64_t i = 0; i < n; i++)
+      if (st[i] & 2u) return fail(K2G_ESOFTWARE, "internal error: hitlist exceeded its bound");
+    HIP_TRY(hipMemcpy(b->h_call.data(), b->call.p, n * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(b->h_groups.data(), b->groups.p, n * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(b->h_total.data(), b->total.p, n * 4, hipMemcpyDeviceToHost));
+    // compact runs: exclusive scan of counts, then gather
+    auto compact = [&](const uint32_t *d_counts, const uint32_t *d_src, DevBuf<uint64_t> &off,
+                       DevBuf<uint32_t> &dst, std::vector<uint64_t> &h_off,
+                       std::vector<uint32_t> &h_dst) -> int {
+      HIP_TRY(off.reserve(n + 1));
+      HIP_TRY(b->bounds.reserve(n + 1));
+      hipLaunchKernelGGL(k_widen, dim3((unsigned) ((n + 1 + 255) / 256)), dim3(256), 0, s,
+                         d_counts, b->bounds.p, n);
+      size_t tb = 0;
+      HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, b->bounds.p, off.p, n + 1, s));
+      HIP_TRY(b->temp.reserve(tb));
+      HIP_TRY(hipcub::DeviceScan::ExclusiveSum(b->temp.p, tb, b->bounds.p, off.p, n + 1, s));
+      h_off.resize(n + 1);
+      HIP_TRY(hipMemcpyAsync(h_off.data(), off.p, (n + 1) * 8, hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipStreamSynchronize(s));
+      const uint64_t tot = h_off[n];
+      HIP_TRY(dst.reserve(2 * std::max<uint64_t>(tot, 1)));
+      const unsigned grid = (unsigned) std::min<uint64_t>(n, 65536);
+      hipLaunchKernelGGL(k_compact, dim3(grid), dim3(64), 0, s, b->run_off.p, d_counts, off.p,
+                         d_src, dst.p, n);
+      HIP_TRY(hipGetLastError());
+      h_dst.resize(2 * tot);
+      if (tot)
+        HIP_TRY(hipMemcpyAsync(h_dst.data(), dst.p, tot * 8, hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipStreamSynchronize(s));
+      return K2G_OK;
+    };
+    rc = compact(b->n_runs.p, b->runs.p, b->comp_off, b->comp_runs, b->h_run_off, b->h_runs);
+    if (rc) return rc;
+    if (b->params.want_hll) {
+      rc = compact(b->n_recs.p, b->recs.p, b->comp_rec_off, b->comp_recs, b->h_rec_off, b->h_recs);
+      if (rc) return rc;
+      r->hll_off = b->h_rec_off.data();
+      r->hll = b->h_recs.data();
+    }
+    if (b->params.want_kmer_counts) {
+      b->h_kmer_counts.resize(b->db->view.node_count);
+      HIP_TRY(hipMemcpy(b->h_kmer_counts.data(), b->kmer_counts.p,
+                        b->db->view.node_count * 8, hipMemcpyDeviceToHost));
+      r->kmer_counts = b->h_kmer_counts.data();
+    }
+  }
+  r->call = b->h_call.data();
+  r->hit_groups = b->h_groups.data();
+  r->total_kmers = b->h_total.data();
+  r->run_off = b->h_run_off.data();
+  r->runs = b->h_runs.data();
+  return K2G_OK;
+}
+
+extern "C" int k2g_classify(k2g_batch *b, const k2g_classify_params *p, const k2g_fragments *in,
+                            k2g_results *out) {
+  if (!b || !in || !out) return fail(K2G_EUSAGE, "null argument");
+  int rc = check_params(p);
+  if (rc) return rc;
+  const uint64_t n = in->n;
+  if (n && (!in->seq || !in->off)) return fail(K2G_EUSAGE, "null fragment arrays");
+  const uint64_t bases = n ? in->off[2 * n] : 0;
+  for (uint64_t i = 0; i < 2 * n; i++)
+    if (in->off[i + 1] < in->off[i]) return fail(K2G_EUSAGE, "fragment offsets must not decrease");
+  hipStream_t s = b->stream;
+  if (n) {
+    HIP_TRY(b->seq.reserve(bases + 1));
+    HIP_TRY(b->off.reserve(2 * n + 1));
+    if (bases) HIP_TRY(hipMemcpyAsync(b->seq.p, in->seq, bases, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(b->off.p, in->off, (2 * n + 1) * 8, hipMemcpyHostToDevice, s));
+    if (in->paired) {
+      HIP_TRY(b->paired.reserve(n));
+      HIP_TRY(hipMemcpyAsync(b->paired.p, in->paired, n, hipMemcpyHostToDevice, s));
+    }
+  }
+  rc = k2g_classify_device(b, p, b->seq.p, b->off.p, in->paired ? b->paired.p : nullptr, n,
+                           nullptr);
+  if (rc) return rc;
+  return k2g_batch_fetch(b, out);
+}

@@ -1,0 +1,153 @@
+// Device-side building blocks of the MI355X classify core: the compact-hash
+// probe, taxonomy walks, and the DB view every kernel receives by value.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace k2g {
+
+constexpr int kWave = 64;
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// Special element codes in a hitlist run (never valid internal taxids, which
+// are < 2^31 because value_bits <= 31).
+constexpr uint32_t kCodeMate = 0xFFFFFFFFu;   // "|:|"
+constexpr uint32_t kCodeFrame = 0xFFFFFFFEu;  // "-:-"
+constexpr uint32_t kCodeAmbig = 0xFFFFFFFDu;  // "A:n"
+
+constexpr uint64_t kNoCand = ~0ULL;  // ring sentinel: no valid l-mer
+
+// Everything a kernel needs to read the resident DB.
+struct DbView {
+  const uint32_t *cells;   // capacity cells, zero padded to a multiple of 64
+  uint64_t capacity;
+  uint64_t barrett;        // floor(2^64 / capacity) (0 when capacity == 1)
+  uint32_t key_bits, value_bits;
+  const uint32_t *parent;  // internal parent id per node (node 0 -> 0)
+  uint32_t node_count;
+  // scanner parameters (opts.k2d)
+  uint32_t k, l;
+  uint32_t dna;            // 1 = nucleotide DB, 0 = protein (translated search)
+  uint32_t revcom_version;
+  uint64_t spaced_mask;
+  uint64_t toggle_mask;    // already masked to the l-mer width
+  uint64_t lmer_mask;
+  uint64_t min_hash;       // minimum_acceptable_hash_value
+};
+
+// MurmurHash3 64-bit finalizer (src/kv_store.h:59-67)
+__device__ __forceinline__ uint64_t fmix64(uint64_t k) {
+  k ^= k >> 33;
+  k *= 0xff51afd7ed558ccdULL;
+  k ^= k >> 33;
+  k *= 0xc4ceb9fe1a85ec53ULL;
+  k ^= k >> 33;
+  return k;
+}
+
+// Exact x % d with a precomputed m = floor(2^64 / d): q = mulhi(x, m) is
+// floor(x / d) or one less, so at most one correction step remains.
+__device__ __forceinline__ uint64_t mod_capacity(uint64_t x, uint64_t d, uint64_t m) {
+  if (d == 1) return 0;
+  uint64_t q = __umul64hi(x, m);
+  uint64_t r = x - q * d;
+  while (r >= d) r -= d;
+  return r;
+}
+
+// CompactHashTable::Get with linear probing (src/compact_hash.h:354-374,
+// 493-500).  The probe walks 16-byte aligned groups of four cells so one
+// load covers up to four probe steps.
+__device__ __forceinline__ uint32_t table_get(const DbView &db, uint64_t key) {
+  const uint64_t hc = fmix64(key);
+  const uint32_t ck = (uint32_t) (hc >> (64 - db.key_bits));
+  const uint32_t vb = db.value_bits;
+  const uint32_t vmask = (1u << vb) - 1u;
+  uint64_t idx = mod_capacity(hc, db.capacity, db.barrett);
+  uint64_t left = db.capacity;  // cells still to examine before a full wrap
+  for (;;) {
+    const uint64_t base = idx & ~3ULL;
+    const u32x4 v = *(const u32x4 *) (db.cells + base);
+    const uint32_t c4[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      if ((uint64_t) j < idx - base) continue;
+      const uint32_t c = c4[j];
+      if ((c & vmask) == 0) return 0;
+      if ((c >> vb) == ck) return c & vmask;
+      if (--left == 0) return 0;
+      if (base + j + 1 == db.capacity) break;  // wrap to cell 0
+    }
+    idx = base + 4;
+    if (idx >= db.capacity) idx = 0;
+  }
+}
+
+// src/taxonomy.cc:251-258 (ids are BFS ordered: parent < child)
+__device__ __forceinline__ bool is_ancestor(const uint32_t *parent, uint32_t a, uint32_t b) {
+  if (a == 0 || b == 0) return false;
+  while (b > a) b = parent[b];
+  return b == a;
+}
+
+// src/taxonomy.cc:262-272
+__device__ __forceinline__ uint32_t lca(const uint32_t *parent, uint32_t a, uint32_t b) {
+  if (a == 0 || b == 0) return a ? a : b;
+  while (a != b) {
+    if (a > b)
+      a = parent[a];
+    else
+      b = parent[b];
+  }
+  return a;
+}
+
+__device__ __forceinline__ uint64_t lanemask_lt(int lane) { return (1ULL << lane) - 1ULL; }
+__device__ __forceinline__ uint64_t lanemask_le(int lane) { return ~0ULL >> (63 - lane); }
+__device__ __forceinline__ int top_lane(uint64_t m) { return 63 - __clzll((long long) m); }
+
+// Spread the low 32 bits so bit t lands on bit 2t.
+__device__ __forceinline__ uint64_t spread2(uint64_t x) {
+  x &= 0xFFFFFFFFULL;
+  x = (x | (x << 16)) & 0x0000FFFF0000FFFFULL;
+  x = (x | (x << 8)) & 0x00FF00FF00FF00FFULL;
+  x = (x | (x << 4)) & 0x0F0F0F0F0F0F0F0FULL;
+  x = (x | (x << 2)) & 0x3333333333333333ULL;
+  x = (x | (x << 1)) & 0x5555555555555555ULL;
+  return x;
+}
+
+// Spread the low 16 bits so bit t lands on bit 4t.
+__device__ __forceinline__ uint64_t spread4(uint64_t x) {
+  x &= 0xFFFFULL;
+  x = (x | (x << 24)) & 0x000000FF000000FFULL;
+  x = (x | (x << 12)) & 0x000F000F000F000FULL;
+  x = (x | (x << 6)) & 0x0303030303030303ULL;
+  x = (x | (x << 3)) & 0x1111111111111111ULL;
+  return x;
+}
+
+// 2-bit groups of x in reverse order (src/mmscanner.cc:193-215 before the
+// complement step).
+__device__ __forceinline__ uint64_t reverse_pairs(uint64_t x) {
+  uint64_t r = __builtin_bitreverse64(x);
+  return ((r >> 1) & 0x5555555555555555ULL) | ((r & 0x5555555555555555ULL) << 1);
+}
+
+// Canonical l-mer (src/mmscanner.cc:193-220): min of the l-mer and its
+// reverse complement in the reference's two revcom conventions.
+__device__ __forceinline__ uint64_t canonical_lmer(uint64_t x, uint32_t l, uint32_t version) {
+  const uint64_t mask = (l >= 32) ? ~0ULL : ((1ULL << (2 * l)) - 1ULL);
+  const uint64_t r = ~reverse_pairs(x);
+  const uint64_t rc = version == 0 ? (r & mask) : ((r >> (64 - 2 * l)) & mask);
+  return x < rc ? x : rc;
+}
+
+// Bits [s, s+width) of the 128-bit value hi:lo (s in [1, 127]).
+__device__ __forceinline__ uint64_t window128(uint64_t lo, uint64_t hi, uint32_t s) {
+  if (s >= 64) return hi >> (s - 64);
+  return (lo >> s) | (hi << (64 - s));
+}
+
+}  // namespace k2g

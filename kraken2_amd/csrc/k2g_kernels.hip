@@ -1,0 +1,543 @@
+// HIP kernels of the MI355X classify core (gfx950, wave64).
+//
+// One wavefront owns one fragment (a read or a read pair) and walks it in
+// chunks of 64 k-mer end positions.  Per chunk every lane:
+//   * loads one base (coalesced) and the wave turns the chunk into 2-bit
+//     planes with ballots;
+//   * builds the l-mer ending at its position from the planes, takes the
+//     canonical form, applies the spaced-seed mask and the toggle
+//     (src/mmscanner.cc:150-153);
+//   * takes the window minimum over the k-l+1 candidates kept in an LDS ring
+//     (src/mmscanner.cc:159-187; after an ambiguous base only the l-mers
+//     starting past it count, src/mmscanner.cc:128-146);
+//   * decides LOOKUP / REPEAT / AMBIG against the previous non-ambiguous
+//     minimizer (src/classify.cc:797-838) and probes the table for LOOKUPs
+//     (src/compact_hash.h:354-374);
+// then the wave run-length encodes the per-k-mer taxa into hitlist runs,
+// accumulates hit counts in an LDS table, and finally resolves the call
+// (src/classify.cc:710-762, 890-906).  Nothing here is a dense contraction,
+// so no MFMA is involved: the probe is a random 16-byte gather into HBM.
+#include "k2g_device.h"
+#include "k2g_kernels.h"
+
+namespace k2g {
+
+// Genetic code (A,G,C,T ordering, first base in the high bits) mapped
+// straight to the 15-letter reduced protein alphabet codes of the protein
+// scanner (src/aa_translate.cc:37; src/mmscanner.cc:41-95).
+__constant__ uint8_t c_codon_code[64] = {
+    9, 9, 2, 2, 11, 11, 2, 2, 13, 13, 13, 13, 8,  12, 8,  8,  4, 4, 4, 4, 6, 6,
+    6, 6, 1, 1, 1,  1,  12, 12, 12, 12, 2,  2,  7,  7,  11, 11, 11, 11, 10, 10,
+    10, 10, 8, 8, 8, 8, 0, 0, 15, 15, 0, 14, 3, 3, 2, 2, 2, 2, 8, 8, 5, 5};
+
+__device__ __forceinline__ uint32_t dna_code(uint8_t ch) {
+  switch (ch | 0x20) {
+    case 'a': return 0;
+    case 'c': return 1;
+    case 'g': return 2;
+    case 't': return 3;
+  }
+  return 0xFFu;
+}
+
+// A=0 G=1 C=2 T=3 (codon index order of the translation table)
+__device__ __forceinline__ uint32_t agct_code(uint8_t ch) {
+  switch (ch | 0x20) {
+    case 'a': return 0;
+    case 'g': return 1;
+    case 'c': return 2;
+    case 't': return 3;
+  }
+  return 0xFFu;
+}
+
+// Length of reading frame fr (0..5) of a DNA mate of length L
+// (src/aa_translate.cc:41-100: codons end at i >= 2 with i % 3 == fr % 3;
+// a mate shorter than 3 leaves every frame as one blank character).
+__device__ __forceinline__ uint64_t frame_len(uint64_t L, int fr) {
+  if (L < 3) return 1;
+  const uint64_t first = (fr % 3) == 0 ? 3 : ((fr % 3) == 1 ? 4 : 2);
+  return (L - 1 >= first) ? (L - 1 - first) / 3 + 1 : 0;
+}
+
+// Reduced-alphabet code of position q of reading frame fr, 0xFF = ambiguous.
+__device__ __forceinline__ uint32_t frame_code(const char *m, uint64_t L, int fr, uint64_t q) {
+  if (L < 3) return 0xFFu;
+  const uint64_t first = (fr % 3) == 0 ? 3 : ((fr % 3) == 1 ? 4 : 2);
+  const uint64_t nfr = frame_len(L, fr);
+  const uint64_t j = fr < 3 ? q : nfr - 1 - q;
+  const uint64_t i = first + 3 * j;
+  const uint32_t a = agct_code((uint8_t) m[i - 2]);
+  const uint32_t b = agct_code((uint8_t) m[i - 1]);
+  const uint32_t c = agct_code((uint8_t) m[i]);
+  if ((a | b | c) == 0xFFu || a > 3 || b > 3 || c > 3) return 0xFFu;
+  const uint32_t idx = fr < 3 ? ((a << 4) | (b << 2) | c) : (((3 - c) << 4) | ((3 - b) << 2) | (3 - a));
+  return c_codon_code[idx];
+}
+
+// 32-bit sparse encoding of the HLL hash fmix64(m + 1) (p = 12, p' = 25;
+// src/hyperloglogplus.cc:166-190, 816-824).
+__device__ __forceinline__ uint32_t hll_encode(uint64_t m) {
+  const uint64_t h = fmix64(m + 1);
+  const uint32_t idx = (uint32_t) ((h >> 39) << 7);
+  if ((uint32_t) (idx << 12) == 0) {
+    const uint64_t rest = h << 25;
+    const uint32_t rank = (rest ? (uint32_t) __clzll((long long) rest) : 39u) + 1u;
+    return idx | (rank << 1) | 1u;
+  }
+  return idx;
+}
+
+__device__ __forceinline__ uint64_t shfl64(uint64_t v, int src) {
+  return (uint64_t) __shfl((unsigned long long) v, src);
+}
+__device__ __forceinline__ uint32_t shfl32(uint32_t v, int src) {
+  return (uint32_t) __shfl((unsigned int) v, src);
+}
+
+// ----------------------------------------------------------------------
+// Per-fragment hit-count tables
+
+struct LdsTable {  // open addressing in LDS; taxon 0 marks an empty slot
+  uint32_t *key, *cnt;
+  uint32_t cap;
+  __device__ void clear(int lane) {
+    for (uint32_t i = lane; i < cap; i += kWave) {
+      key[i] = 0;
+      cnt[i] = 0;
+    }
+  }
+  __device__ bool add(uint32_t t, uint32_t c) {
+    uint32_t h = (t * 2654435761u) & (cap - 1);
+    for (uint32_t n = 0; n < cap; n++) {
+      const uint32_t old = atomicCAS(&key[h], 0u, t);
+      if (old == 0u || old == t) {
+        atomicAdd(&cnt[h], c);
+        return true;
+      }
+      h = (h + 1) & (cap - 1);
+    }
+    return false;
+  }
+  __device__ uint32_t get(uint32_t t) const {
+    uint32_t h = (t * 2654435761u) & (cap - 1);
+    for (uint32_t n = 0; n < cap; n++) {
+      const uint32_t k = key[h];
+      if (k == t) return cnt[h];
+      if (k == 0) return 0;
+      h = (h + 1) & (cap - 1);
+    }
+    return 0;
+  }
+  __device__ uint32_t slots() const { return cap; }
+  __device__ uint32_t slot_key(uint32_t i) const { return key[i]; }
+  __device__ uint32_t slot_cnt(uint32_t i) const { return cnt[i]; }
+  __device__ void release(int) {}
+};
+
+// Loads that bypass a possibly stale vector L1 line: the dense table is
+// updated with L2 atomics by every lane of the block.
+__device__ __forceinline__ uint32_t load_agent(const uint32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void store_agent(uint32_t *p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+struct DenseTable {  // overflow path: dense counts per taxon in global memory
+  uint32_t *counts, *list, *len;
+  __device__ void clear(int) {}
+  __device__ bool add(uint32_t t, uint32_t c) {
+    if (atomicAdd(&counts[t], c) == 0u) store_agent(&list[atomicAdd(len, 1u)], t);
+    return true;
+  }
+  __device__ uint32_t get(uint32_t t) const { return load_agent(&counts[t]); }
+  __device__ uint32_t slots() const { return load_agent(len); }
+  __device__ uint32_t slot_key(uint32_t i) const { return load_agent(&list[i]); }
+  __device__ uint32_t slot_cnt(uint32_t i) const { return load_agent(&counts[load_agent(&list[i])]); }
+  __device__ void release(int lane) {
+    const uint32_t n = load_agent(len);
+    for (uint32_t i = lane; i < n; i += kWave) store_agent(&counts[load_agent(&list[i])], 0u);
+    __syncthreads();
+    if (lane == 0) store_agent(len, 0u);
+    __syncthreads();
+  }
+};
+
+// ----------------------------------------------------------------------
+// Fragment state carried across chunks and streams (wave-uniform).
+struct RunState {
+  uint32_t run_code = 0, run_count = 0;
+  uint32_t n_runs = 0, nrec = 0, groups = 0;
+  uint64_t elems = 0;
+  bool done = false;
+};
+
+template <class Table>
+__device__ __forceinline__ void close_run(const FragOut &out, uint64_t base, uint64_t cap,
+                                          uint32_t idx, uint32_t code, uint32_t count,
+                                          Table &tab, int *bad) {
+  if (idx < cap) {
+    out.runs[2 * (base + idx)] = code;
+    out.runs[2 * (base + idx) + 1] = count;
+  } else {
+    atomicOr(bad, 2);
+  }
+  if (code != 0 && code < kCodeAmbig) {
+    if (!tab.add(code, count)) atomicOr(bad, 1);
+  }
+}
+
+// Appends a border marker element (mate "|:|" or frame "-:-").
+template <class Table>
+__device__ void push_marker(const FragOut &out, uint64_t base, uint64_t cap, RunState &st,
+                            uint32_t code, Table &tab, int *bad, int lane) {
+  if (st.run_count > 0) {
+    if (lane == 0) close_run(out, base, cap, st.n_runs, st.run_code, st.run_count, tab, bad);
+    st.n_runs++;
+  }
+  st.run_code = code;
+  st.run_count = 1;
+  st.elems++;
+}
+
+// Scans one stream (a DNA mate, or one reading frame of a mate) and appends
+// its k-mer elements to the fragment's runs.
+template <class Table>
+__device__ void scan_stream(const DbView &db, const ClassifyParams &P, const FragOut &out,
+                            uint64_t base, uint64_t cap, const char *m, uint64_t L, int fr,
+                            RunState &st, Table &tab, uint64_t *ring, int *bad, int lane) {
+  const bool dna = db.dna != 0;
+  const uint64_t n = dna ? L : frame_len(L, fr);
+  const uint32_t k = db.k, l = db.l;
+  const uint64_t lbits = (1ULL << l) - 1ULL;
+  int64_t last_amb = -1;
+  uint64_t last_min = ~0ULL;
+  uint32_t last_taxon = 0;
+  uint64_t pp0 = 0, pp1 = 0, pp2 = 0, pp3 = 0;  // previous chunk's planes
+  for (uint64_t c = 0; c < n; c += kWave) {
+    const uint64_t p = c + lane;
+    const bool inb = p < n;
+    uint32_t code = 0xFFu;
+    if (inb) code = dna ? dna_code((uint8_t) m[p]) : frame_code(m, L, fr, p);
+    const bool amb = inb && code == 0xFFu;
+    const bool good = inb && !amb;
+    const uint64_t P0 = __ballot(good && (code & 1u));
+    const uint64_t P1 = __ballot(good && (code & 2u));
+    const uint64_t P2 = dna ? 0 : __ballot(good && (code & 4u));
+    const uint64_t P3 = dna ? 0 : __ballot(good && (code & 8u));
+    const uint64_t A = __ballot(amb);
+    const uint64_t mA = A & lanemask_le(lane);
+    const int64_t lastN = mA ? (int64_t) (c + top_lane(mA)) : last_amb;
+
+    // candidate of the l-mer ending at p
+    const bool lvalid = inb && p + 1 >= l && lastN < (int64_t) (p + 1 - l);
+    uint64_t cand = kNoCand;
+    if (lvalid) {
+      const uint32_t s = (uint32_t) lane + 65u - l;
+      const uint64_t w0 = window128(pp0, P0, s) & lbits;
+      const uint64_t w1 = window128(pp1, P1, s) & lbits;
+      const uint64_t r0 = __builtin_bitreverse64(w0) >> (64 - l);
+      const uint64_t r1 = __builtin_bitreverse64(w1) >> (64 - l);
+      uint64_t lmer;
+      if (dna) {
+        lmer = (spread2(r1) << 1) | spread2(r0);
+        cand = canonical_lmer(lmer, l, db.revcom_version);
+      } else {
+        const uint64_t w2 = window128(pp2, P2, s) & lbits;
+        const uint64_t w3 = window128(pp3, P3, s) & lbits;
+        const uint64_t r2 = __builtin_bitreverse64(w2) >> (64 - l);
+        const uint64_t r3 = __builtin_bitreverse64(w3) >> (64 - l);
+        lmer = (spread4(r3) << 3) | (spread4(r2) << 2) | (spread4(r1) << 1) | spread4(r0);
+        cand = lmer;
+      }
+      if (db.spaced_mask) cand &= db.spaced_mask;
+      cand ^= db.toggle_mask;
+    }
+    ring[p & 127] = cand;
+    __syncthreads();
+
+    // k-mer ending at p
+    const bool kvalid = inb && p + 1 >= k;
+    bool nonamb = false;
+    uint64_t mn = ~0ULL;
+    if (kvalid) {
+      nonamb = (k == l) ? lvalid : (lastN < (int64_t) (p + 2 - k));
+      if (nonamb) {
+        for (uint32_t j = 0; j <= k - l; j++) {
+          const uint64_t v = ring[(p - j) & 127];
+          mn = v < mn ? v : mn;
+        }
+        mn ^= db.toggle_mask;
+      }
+    }
+    __syncthreads();
+
+    // token decision against the previous non-ambiguous minimizer
+    const uint64_t NA = __ballot(nonamb);
+    const uint64_t prevm = NA & lanemask_lt(lane);
+    const uint64_t before = shfl64(mn, prevm ? top_lane(prevm) : lane);
+    const uint64_t prev_min = prevm ? before : last_min;
+    const bool lookup = nonamb && mn != prev_min;
+    const bool skip = lookup && db.min_hash != 0 && fmix64(mn) < db.min_hash;
+    uint32_t taxon = 0;
+    if (lookup && !skip) taxon = table_get(db, mn);
+    const uint64_t LK = __ballot(lookup);
+    const uint64_t srcm = LK & lanemask_le(lane);
+    const uint32_t srct = shfl32(taxon, srcm ? top_lane(srcm) : lane);
+    const uint32_t tax_i = nonamb ? (srcm ? srct : last_taxon) : 0u;
+    uint64_t HIT = __ballot(lookup && !skip && taxon != 0);
+    uint64_t E = __ballot(kvalid);
+
+    if (P.quick) {  // src/classify.cc:880-884: stop at the first qualifying hit
+      const uint32_t g_le = st.groups + __popcll(HIT & lanemask_le(lane));
+      const uint64_t Q = __ballot(kvalid && tax_i != 0 && (int32_t) g_le >= P.min_hit_groups);
+      if (Q) {
+        const int cut = __ffsll((long long) Q) - 1;
+        E &= lanemask_lt(cut);
+        HIT &= lanemask_le(cut);
+        st.done = true;
+      }
+    }
+
+    // per-taxon k-mer counts and HLL records for LOOKUP hits
+    if ((HIT >> lane) & 1ULL) {
+      if (out.kmer_counts) atomicAdd(&out.kmer_counts[taxon], 1ULL);
+      if (out.recs) {
+        const uint32_t r = st.nrec + __popcll(HIT & lanemask_lt(lane));
+        if (r < cap) {
+          out.recs[2 * (base + r)] = taxon;
+          out.recs[2 * (base + r) + 1] = hll_encode(mn);
+        } else {
+          atomicOr(bad, 2);
+        }
+      }
+    }
+    st.nrec += __popcll(HIT);
+    st.groups += __popcll(HIT);
+
+    // run-length encode the elements of this chunk
+    if (E) {
+      const bool isE = (E >> lane) & 1ULL;
+      const uint32_t code_i = nonamb ? tax_i : kCodeAmbig;
+      const uint32_t prev_code = shfl32(code_i, lane > 0 ? lane - 1 : 0);
+      const int lo = __ffsll((long long) E) - 1;
+      const bool boundary =
+          isE && (lane == lo ? (st.run_count == 0 || code_i != st.run_code) : code_i != prev_code);
+      const uint64_t B = __ballot(boundary);
+      const uint64_t pb = B & lanemask_lt(lane);
+      const int pbl = pb ? top_lane(pb) : lane;
+      const uint32_t pb_code = shfl32(code_i, pbl);
+      const uint32_t drop = (st.run_count == 0) ? 1u : 0u;
+      if (boundary) {
+        uint32_t cc, cn;
+        bool has = true;
+        if (pb) {
+          cc = pb_code;
+          cn = (uint32_t) (lane - pbl);
+        } else {
+          cc = st.run_code;
+          cn = st.run_count + (uint32_t) (lane - lo);
+          has = st.run_count > 0;
+        }
+        if (has) {
+          const uint32_t idx = st.n_runs + __popcll(pb) - drop;
+          close_run(out, base, cap, idx, cc, cn, tab, bad);
+        }
+      }
+      st.n_runs += (uint32_t) __popcll(B) - (B ? drop : 0u);
+      const int hi = top_lane(E);
+      if (B) {
+        const int lb = top_lane(B);
+        st.run_code = shfl32(code_i, lb);
+        st.run_count = (uint32_t) (hi - lb + 1);
+      } else {
+        st.run_count += (uint32_t) __popcll(E);
+      }
+      st.elems += (uint64_t) __popcll(E);
+    }
+
+    if (A) last_amb = (int64_t) (c + top_lane(A));
+    if (NA) last_min = shfl64(mn, top_lane(NA));
+    if (LK) last_taxon = shfl32(taxon, top_lane(LK));
+    pp0 = P0;
+    pp1 = P1;
+    pp2 = P2;
+    pp3 = P3;
+    if (st.done) break;
+  }
+}
+
+// Wave-wide (score, taxon) reduction: higher score wins, ties take the LCA
+// (src/classify.cc:718-740).
+__device__ __forceinline__ void reduce_best(const uint32_t *parent, uint32_t &score,
+                                            uint32_t &taxon) {
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const uint32_t os = (uint32_t) __shfl_xor((unsigned int) score, d);
+    const uint32_t ot = (uint32_t) __shfl_xor((unsigned int) taxon, d);
+    if (os > score) {
+      score = os;
+      taxon = ot;
+    } else if (os == score) {
+      taxon = lca(parent, taxon, ot);
+    }
+  }
+}
+
+__device__ __forceinline__ uint32_t reduce_sum(uint32_t v) {
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) v += (uint32_t) __shfl_xor((unsigned int) v, d);
+  return v;
+}
+
+template <class Table>
+__device__ void classify_fragment(const DbView &db, const ClassifyParams &P, const FragBatch &in,
+                                  const FragOut &out, uint64_t f, Table &tab, uint64_t *ring,
+                                  int *bad, int lane) {
+  const uint64_t o0 = in.off[2 * f], o1 = in.off[2 * f + 1], o2 = in.off[2 * f + 2];
+  const bool paired = in.paired ? in.paired[f] != 0 : in.paired_all != 0;
+  const bool tx = db.dna == 0;
+  const int nframes = tx ? 6 : 1;
+  const uint64_t base = out.run_off[f];
+  const uint64_t cap = out.run_off[f + 1] - base;
+  tab.clear(lane);
+  if (lane == 0) *bad = 0;
+  __syncthreads();
+
+  RunState st;
+  for (int mate = 0; mate < (paired ? 2 : 1) && !st.done; mate++) {
+    const char *m = in.seq + (mate ? o1 : o0);
+    const uint64_t L = mate ? o2 - o1 : o1 - o0;
+    for (int fr = 0; fr < nframes; fr++) {
+      scan_stream(db, P, out, base, cap, m, L, fr, st, tab, ring, bad, lane);
+      if (st.done) break;
+      if (tx && fr != 5) push_marker(out, base, cap, st, kCodeFrame, tab, bad, lane);
+    }
+    if (st.done) break;
+    if (paired && mate == 0) push_marker(out, base, cap, st, kCodeMate, tab, bad, lane);
+  }
+  if (st.run_count > 0) {
+    if (lane == 0) close_run(out, base, cap, st.n_runs, st.run_code, st.run_count, tab, bad);
+    st.n_runs++;
+  }
+  __syncthreads();
+
+  // ResolveTree (src/classify.cc:710-762)
+  uint64_t total = st.elems;
+  if (paired) total--;
+  if (tx) total -= paired ? 4 : 2;
+  const uint32_t required = (uint32_t) ceil(P.confidence * (double) total);
+  uint32_t best = 0, best_t = 0;
+  const uint32_t ns = tab.slots();
+  for (uint32_t i = lane; i < ns; i += kWave) {
+    const uint32_t t = tab.slot_key(i);
+    if (t == 0) continue;
+    uint32_t s = 0;
+    for (uint32_t a = t; a != 0; a = db.parent[a]) s += tab.get(a);
+    if (s > best) {
+      best = s;
+      best_t = t;
+    } else if (s == best) {
+      best_t = lca(db.parent, best_t, t);
+    }
+  }
+  reduce_best(db.parent, best, best_t);
+  uint32_t call = best_t;
+  uint32_t score = call ? tab.get(call) : 0;
+  while (call && score < required) {
+    uint32_t part = 0;
+    for (uint32_t i = lane; i < ns; i += kWave) {
+      const uint32_t t = tab.slot_key(i);
+      if (t != 0 && is_ancestor(db.parent, call, t)) part += tab.slot_cnt(i);
+    }
+    score = reduce_sum(part);
+    if (score >= required) break;
+    call = db.parent[call];
+  }
+  if (call && (int64_t) st.groups < (int64_t) P.min_hit_groups) call = 0;
+  __syncthreads();
+  if (lane == 0) {
+    out.call[f] = call;
+    out.groups[f] = st.groups;
+    out.total[f] = (uint32_t) total;
+    out.n_runs[f] = st.n_runs;
+    out.n_recs[f] = st.nrec;
+    out.status[f] = (uint32_t) *bad;
+  }
+  tab.release(lane);
+}
+
+struct WaveLds {
+  uint64_t ring[128];
+  uint32_t tkey[256];
+  uint32_t tcnt[256];
+  int bad;
+};
+
+// Plain path: one wavefront per fragment, one launch per batch.
+__global__ __launch_bounds__(64) void k_classify_plain(DbView db, ClassifyParams P, FragBatch in,
+                                                        FragOut out) {
+  __shared__ WaveLds lds;
+  const uint64_t f = blockIdx.x;
+  if (f >= in.n) return;
+  LdsTable tab{lds.tkey, lds.tcnt, P.tab_cap};
+  classify_fragment(db, P, in, out, f, tab, lds.ring, &lds.bad, threadIdx.x);
+}
+
+// Persistent path: resident wavefronts pull fragments from a shared cursor in
+// the given order (longest first), so long and short reads share one launch.
+__global__ __launch_bounds__(64) void k_classify_persistent(DbView db, ClassifyParams P,
+                                                             FragBatch in, FragOut out,
+                                                             unsigned long long *cursor) {
+  __shared__ WaveLds lds;
+  LdsTable tab{lds.tkey, lds.tcnt, P.tab_cap};
+  const int lane = threadIdx.x;
+  for (;;) {
+    unsigned long long i = 0;
+    if (lane == 0) i = atomicAdd(cursor, 1ULL);
+    i = __shfl(i, 0);
+    if (i >= in.n) break;
+    const uint64_t f = in.order ? in.order[i] : i;
+    classify_fragment(db, P, in, out, f, tab, lds.ring, &lds.bad, lane);
+  }
+}
+
+// Overflow path: fragments whose distinct hit taxa exceeded the LDS table are
+// classified again with a dense per-taxon count array (one per block).
+__global__ __launch_bounds__(64) void k_classify_dense(DbView db, ClassifyParams P, FragBatch in,
+                                                        FragOut out, const uint32_t *redo,
+                                                        uint32_t n_redo, DenseScratch scratch) {
+  __shared__ WaveLds lds;
+  const int lane = threadIdx.x;
+  const uint32_t slot = blockIdx.x;
+  DenseTable tab{scratch.counts + (uint64_t) slot * db.node_count,
+                 scratch.lists + (uint64_t) slot * db.node_count, scratch.lens + slot};
+  for (uint32_t i = blockIdx.x; i < n_redo; i += gridDim.x)
+    classify_fragment(db, P, in, out, redo[i], tab, lds.ring, &lds.bad, lane);
+}
+
+// KeyValueStore::GetBatch on the device (one lane per key).
+__global__ __launch_bounds__(256) void k_get_batch(DbView db, const uint64_t *keys, uint32_t *vals,
+                                                    uint64_t n) {
+  const uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) vals[i] = table_get(db, keys[i]);
+}
+
+// Largest cell value (used to check that every stored taxid is a node).
+__global__ __launch_bounds__(256) void k_max_value(DbView db, unsigned int *out) {
+  const uint32_t vmask = (1u << db.value_bits) - 1u;
+  uint32_t best = 0;
+  for (uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x; i < db.capacity;
+       i += (uint64_t) gridDim.x * blockDim.x) {
+    const uint32_t v = db.cells[i] & vmask;
+    best = v > best ? v : best;
+  }
+  for (int d = 1; d < kWave; d <<= 1) {
+    const uint32_t o = (uint32_t) __shfl_xor((unsigned int) best, d);
+    best = o > best ? o : best;
+  }
+  if ((threadIdx.x & (kWave - 1)) == 0) atomicMax(out, best);
+}
+
+}  // namespace k2g

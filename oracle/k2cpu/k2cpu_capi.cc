@@ -111,13 +111,13 @@ void k2cpu_probe_lines(void *h, const uint64_t *keys, uint32_t *out, long n) {
 uint64_t k2cpu_db_capacity(void *h) { return ((K2cpuDb *) h)->idx.table.capacity; }
 
 // Classifies n fragments.  Mate sequences are concatenated in seq; mate 1
-// of fragment i is seq[off1[i], off1[i+1]), mate 2 seq[off2[i], off2[i+1])
+// of fragment i is seq[off[2i], off[2i+1]), mate 2 seq[off[2i+1], off[2i+2])
 // when paired.  Per fragment: call (internal id), hit groups, total k-mers,
 // lookups and probe lines; the kraken hitlist field (text after the length
 // column) goes NUL-separated into text (cap bytes).  Returns bytes used or
 // -1 when text is too small.
 long k2cpu_classify(void *h, int paired, int quick, double confidence, int min_hit_groups,
-                    const char *seq, const uint64_t *off1, const uint64_t *off2, long n,
+                    const char *seq, const uint64_t *off, long n,
                     uint32_t *call, uint32_t *groups, uint32_t *total, uint32_t *lookups,
                     uint64_t *lines, char *text, long cap) {
   auto *d = (K2cpuDb *) h;
@@ -131,8 +131,8 @@ long k2cpu_classify(void *h, int paired, int quick, double confidence, int min_h
   long pos = 0;
   std::string s1, s2, line;
   for (long i = 0; i < n; i++) {
-    s1.assign(seq + off1[i], seq + off1[i + 1]);
-    if (paired) s2.assign(seq + off2[i], seq + off2[i + 1]);
+    s1.assign(seq + off[2 * i], seq + off[2 * i + 1]);
+    if (paired) s2.assign(seq + off[2 * i + 1], seq + off[2 * i + 2]);
     classify_fragment(d->idx, o, s1, paired ? &s2 : nullptr, r, nullptr, lines != nullptr);
     call[i] = (uint32_t) r.call;
     groups[i] = (uint32_t) r.hit_groups;

@@ -1,0 +1,24 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+TESTS = os.path.dirname(os.path.abspath(__file__))
+if TESTS not in sys.path:
+    sys.path.insert(0, TESTS)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an AMD GPU (MI355X) and the built libk2g.so")
+
+
+@pytest.fixture(scope="session")
+def synth_db(tmp_path_factory):
+    """A small synthetic nucleotide DB built by the oracle's deterministic build."""
+    from helpers import make_synthetic_db
+    d = tmp_path_factory.mktemp("synth_db")
+    make_synthetic_db(str(d), seed=11, levels="2,2,3", genome_len=20000)
+    return str(d)

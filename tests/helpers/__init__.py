@@ -1,0 +1,154 @@
+"""Test helpers: synthetic DB construction and the CPU oracle's ctypes binding.
+
+The oracle (oracle/k2cpu) is the checker only; product code never imports it.
+"""
+import ctypes as C
+import os
+import random
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+ORACLE_BIN = os.path.join(ROOT, "oracle", "bin")
+K2SYNTH = os.path.join(ROOT, "kraken2_amd", "bin", "k2synth")
+DNA_SEED_TEMPLATE = "1" * 17 + "01" * 7  # kraken2-build default for l=31, 7 spaces
+
+
+def run(cmd, **kw):
+    r = subprocess.run(cmd, capture_output=True, text=True, **kw)
+    if r.returncode != 0:
+        raise RuntimeError(f"{cmd} failed ({r.returncode}): {r.stderr}")
+    return r
+
+
+def make_synthetic_db(d, seed=1, levels="2,2,3", ranks=None, genome_len=20000, sub=0.05,
+                      k=35, l=31, template=DNA_SEED_TEMPLATE, load_factor=0.7,
+                      block=16384, subblock=2048):
+    """Writes taxonomy + genomes with k2synth and builds hash/taxo/opts with the oracle."""
+    nlev = len(levels.split(","))
+    if ranks is None:
+        ranks = ",".join((["phylum", "class", "order", "family", "genus", "species"])[-nlev:])
+    run([K2SYNTH, "db-inputs", "--out", d, "--seed", str(seed), "--levels", levels,
+         "--ranks", ranks, "--genome-len", str(genome_len), "--sub", str(sub)])
+    lib = os.path.join(d, "library.fna")
+    with open(lib, "rb") as f:
+        est = int(run([os.path.join(ORACLE_BIN, "k2cpu_estimate_capacity"), "-k", str(k), "-l", str(l),
+                       "-S", template], stdin=f).stdout.strip())
+    cap = int((est + 8192) / load_factor)
+    with open(lib, "rb") as f:
+        run([os.path.join(ORACLE_BIN, "k2cpu_build_db"), "-k", str(k), "-l", str(l), "-S", template,
+             "-H", os.path.join(d, "hash.k2d"), "-t", os.path.join(d, "taxo.k2d"),
+             "-o", os.path.join(d, "opts.k2d"), "-n", os.path.join(d, "taxonomy"),
+             "-m", os.path.join(d, "seqid2taxid.map"), "-c", str(cap), "-B", str(block),
+             "-b", str(subblock), "-r", "0"], stdin=f)
+    return d
+
+
+def read_fasta(path):
+    names, seqs = [], []
+    with open(path) as f:
+        for line in f:
+            line = line.rstrip("\n")
+            if line.startswith(">"):
+                names.append(line[1:].split()[0])
+                seqs.append([])
+            elif seqs:
+                seqs[-1].append(line)
+    return names, ["".join(s) for s in seqs]
+
+
+def revcomp(s):
+    t = {"A": "T", "C": "G", "G": "C", "T": "A"}
+    return "".join(t.get(c, "N") for c in reversed(s))
+
+
+def edge_reads(genomes, n, seed, min_len=0, max_len=400):
+    """Reads sampled from genomes with the parity edge cases mixed in:
+    N / IUPAC / lowercase bases, lengths around k, reverse strands, random reads."""
+    rng = random.Random(seed)
+    out = []
+    for i in range(n):
+        kind = rng.random()
+        L = rng.choice([0, 1, 2, 3, 30, 34, 35, 36, 40, 64, 65, 100, 150, 151, 250, 400,
+                        rng.randint(min_len, max_len)])
+        if kind < 0.1:
+            s = "".join(rng.choice("ACGT") for _ in range(L))
+        else:
+            g = genomes[rng.randrange(len(genomes))]
+            L = min(L, len(g))
+            p = rng.randrange(len(g) - L + 1)
+            s = g[p:p + L]
+            if rng.random() < 0.5:
+                s = revcomp(s)
+            s = list(s)
+            for j in range(len(s)):
+                r = rng.random()
+                if r < 0.01:
+                    s[j] = rng.choice("ACGT")
+                elif r < 0.015:
+                    s[j] = rng.choice("NNNRYKMSWnx-")
+                elif r < 0.03:
+                    s[j] = s[j].lower()
+            s = "".join(s)
+        out.append(s)
+    return out
+
+
+class Oracle:
+    """ctypes binding of oracle/bin/libk2cpu.so (the CPU restatement)."""
+
+    def __init__(self):
+        lib = C.CDLL(os.path.join(ORACLE_BIN, "libk2cpu.so"))
+        vp = C.c_void_p
+        lib.k2cpu_db_open.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p]
+        lib.k2cpu_db_open.restype = vp
+        lib.k2cpu_db_close.argtypes = [vp]
+        lib.k2cpu_get_batch.argtypes = [vp, vp, vp, C.c_long]
+        lib.k2cpu_probe_lines.argtypes = [vp, vp, vp, C.c_long]
+        lib.k2cpu_db_capacity.argtypes = [vp]
+        lib.k2cpu_db_capacity.restype = C.c_uint64
+        lib.k2cpu_classify.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_int, vp, vp, C.c_long,
+                                       vp, vp, vp, vp, vp, vp, C.c_long]
+        lib.k2cpu_classify.restype = C.c_long
+        lib.k2cpu_scan.argtypes = [C.c_long, C.c_long, C.c_uint64, C.c_uint64, C.c_int, C.c_long, C.c_long,
+                                   C.c_char_p, C.c_long, vp, vp, C.c_long]
+        lib.k2cpu_scan.restype = C.c_long
+        lib.k2cpu_translate.argtypes = [C.c_char_p, C.c_long, vp, C.c_long]
+        lib.k2cpu_translate.restype = C.c_long
+        lib.k2cpu_hll_script.argtypes = [C.c_char_p, vp, C.c_long]
+        lib.k2cpu_hll_script.restype = C.c_long
+        self.lib = lib
+
+    def open(self, d):
+        h = self.lib.k2cpu_db_open(os.path.join(d, "hash.k2d").encode(), os.path.join(d, "taxo.k2d").encode(),
+                                   os.path.join(d, "opts.k2d").encode())
+        if not h:
+            raise RuntimeError(f"oracle could not open DB in {d}")
+        return h
+
+    def get_batch(self, h, keys):
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        out = np.zeros(keys.shape[0], dtype=np.uint32)
+        self.lib.k2cpu_get_batch(h, keys.ctypes.data, out.ctypes.data, keys.shape[0])
+        return out
+
+    def classify(self, h, seq, off, paired=False, quick=False, confidence=0.0, min_hit_groups=0,
+                 lines=False):
+        """Per fragment: call, groups, total, lookups, lines and the hitlist text."""
+        off = np.ascontiguousarray(off, dtype=np.uint64)
+        n = (off.shape[0] - 1) // 2
+        res = {k: np.zeros(n, np.uint32) for k in ("call", "groups", "total", "lookups")}
+        res["lines"] = np.zeros(n, np.uint64)
+        cap = 16 * int(off[-1]) + 64 * n + 64
+        text = C.create_string_buffer(cap)
+        sbuf = np.frombuffer(seq if len(seq) else b"\0", dtype=np.uint8)
+        used = self.lib.k2cpu_classify(h, int(paired), int(quick), confidence, min_hit_groups,
+                                       sbuf.ctypes.data, off.ctypes.data, n,
+                                       res["call"].ctypes.data, res["groups"].ctypes.data,
+                                       res["total"].ctypes.data, res["lookups"].ctypes.data,
+                                       res["lines"].ctypes.data if lines else None, text, cap)
+        if used < 0:
+            raise RuntimeError("oracle text buffer too small")
+        res["text"] = [t.decode() for t in text.raw[:used].split(b"\0")[:n]]
+        return res

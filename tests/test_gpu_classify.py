@@ -1,0 +1,116 @@
+"""GPU parity: the HIP classify core against the CPU oracle on the same DB.
+
+Bit-exact comparison of call, minimizer hit groups, confidence denominator
+and the hitlist text of every fragment (src/classify.cc:773-991)."""
+import os
+import random
+
+import numpy as np
+import pytest
+
+from helpers import Oracle, edge_reads, read_fasta
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def env(synth_db):
+    import kraken2_amd as k2
+    db = k2.Db.from_dir(synth_db)
+    batch = k2.Batch(db)
+    orc = Oracle()
+    h = orc.open(synth_db)
+    names, genomes = read_fasta(os.path.join(synth_db, "library.fna"))
+    yield k2, db, batch, orc, h, genomes
+    batch.close()
+    db.close()
+
+
+def _compare(k2, db, out, ref, n):
+    ext = db.external_ids
+    for i in range(n):
+        runs = out.runs[out.run_off[i]:out.run_off[i + 1]]
+        got = k2.format_hitlist(runs, ext)
+        assert (int(out.call[i]), int(out.hit_groups[i]), int(out.total_kmers[i]), got) == \
+               (int(ref["call"][i]), int(ref["groups"][i]), int(ref["total"][i]), ref["text"][i]), \
+               f"fragment {i}"
+
+
+def test_get_batch_all_cells_and_misses(env, synth_db):
+    k2, db, batch, orc, h, genomes = env
+    rng = np.random.default_rng(5)
+    keys = rng.integers(0, 2**62, size=200000, dtype=np.uint64)
+    assert np.array_equal(db.get_batch(keys), orc.get_batch(h, keys))
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("paired", [False, True])
+def test_classify_edge_reads(env, mode, paired):
+    k2, db, batch, orc, h, genomes = env
+    n = 3000
+    m1 = edge_reads(genomes, n, seed=100 + mode + 2 * paired)
+    m2 = edge_reads(genomes, n, seed=200 + mode) if paired else None
+    seq, off = k2.pack_fragments(m1, m2)
+    out = batch.classify(seq, off, paired=paired, mode=mode, min_hit_groups=2)
+    ref = orc.classify(h, seq, off, paired=paired, min_hit_groups=2)
+    _compare(k2, db, out, ref, n)
+
+
+@pytest.mark.parametrize("conf,g,quick", [(0.0, 0, False), (0.3, 2, False), (0.75, 1, False),
+                                          (1.0, 3, False), (0.0, 2, True), (0.5, 0, True)])
+def test_classify_confidence_and_quick(env, conf, g, quick):
+    k2, db, batch, orc, h, genomes = env
+    n = 2000
+    m1 = edge_reads(genomes, n, seed=300, min_len=100, max_len=300)
+    seq, off = k2.pack_fragments(m1)
+    out = batch.classify(seq, off, confidence=conf, min_hit_groups=g, quick=quick, mode=1)
+    ref = orc.classify(h, seq, off, confidence=conf, min_hit_groups=g, quick=quick)
+    if quick:
+        for i in range(n):
+            assert (int(out.call[i]), int(out.hit_groups[i])) == (int(ref["call"][i]), int(ref["groups"][i]))
+    else:
+        _compare(k2, db, out, ref, n)
+
+
+def test_long_reads_and_dense_table(env):
+    """Long reads cross many chunks; a 2-slot LDS table forces the dense path."""
+    k2, db, batch, orc, h, genomes = env
+    rng = random.Random(7)
+    m1 = []
+    for i in range(60):
+        g = genomes[rng.randrange(len(genomes))]
+        L = rng.randint(500, min(len(g), 15000))
+        p = rng.randrange(len(g) - L + 1)
+        s = list(g[p:p + L])
+        for j in range(len(s)):
+            if rng.random() < 0.03:
+                s[j] = rng.choice("ACGTN")
+        m1.append("".join(s))
+    seq, off = k2.pack_fragments(m1)
+    ref = orc.classify(h, seq, off, confidence=0.1, min_hit_groups=2)
+    for slots, mode in ((256, 0), (2, 1), (4, 0)):
+        out = batch.classify(seq, off, confidence=0.1, min_hit_groups=2, mode=mode, table_slots=slots)
+        _compare(k2, db, out, ref, len(m1))
+        if slots == 2:
+            assert out.redone > 0
+
+
+def test_mixed_paired_flags(env):
+    k2, db, batch, orc, h, genomes = env
+    n = 1000
+    m1 = edge_reads(genomes, n, seed=400, min_len=50, max_len=250)
+    m2 = edge_reads(genomes, n, seed=401, min_len=50, max_len=250)
+    flags = np.array([i % 3 == 0 for i in range(n)], dtype=np.uint8)
+    m2x = [m2[i] if flags[i] else "" for i in range(n)]
+    seq, off = k2.pack_fragments(m1, m2x)
+    out = batch.classify(seq, off, paired_flags=flags, mode=1)
+    for pflag in (0, 1):
+        idx = [i for i in range(n) if flags[i] == pflag]
+        sub1 = [m1[i] for i in idx]
+        sub2 = [m2x[i] for i in idx] if pflag else None
+        s2, o2 = k2.pack_fragments(sub1, sub2)
+        ref = orc.classify(h, s2, o2, paired=bool(pflag))
+        for j, i in enumerate(idx):
+            runs = out.runs[out.run_off[i]:out.run_off[i + 1]]
+            assert (int(out.call[i]), k2.format_hitlist(runs, db.external_ids)) == \
+                   (int(ref["call"][j]), ref["text"][j])
