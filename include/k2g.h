@@ -128,6 +128,42 @@ int k2g_batch_fetch(k2g_batch *b, k2g_results *out);
  * own work around k2g_classify_device. */
 void *k2g_batch_stream(k2g_batch *b);
 
+/* ---- build side: the table construction of build_db
+ * (src/build_db.h:186-326; CompareAndSet src/compact_hash.h:416-462)
+ * as lock-free atomicCAS with an in-kernel LCA merge.  Sequences are scanned
+ * in the deterministic build's windows (blocks of block_size bases split
+ * into subblocks of subblock_size, each overlapping the next by k-1), so the
+ * set of (minimizer, taxon) pairs inserted equals the reference's; cell
+ * placement may differ from a CPU build (it is order dependent there too). */
+typedef struct k2g_builder k2g_builder;
+
+typedef struct {
+  uint64_t k, l;
+  uint64_t spaced_seed_mask;  /* already expanded (src/utilities.cc:14-27) */
+  uint64_t toggle_mask;
+  uint64_t min_clear_hash_value;
+  uint64_t block_size, subblock_size; /* build_db -B / -b */
+  int32_t dna_db;
+  int32_t revcom_version; /* 1 for new databases */
+} k2g_build_params;
+
+/* capacity cells; value_bits = 0 picks the smallest width that holds every
+ * node id (src/build_db.cc:66-75); key_bits = 32 - value_bits. */
+int k2g_build_create(const char *taxo_k2d, uint64_t capacity, uint64_t value_bits,
+                     const k2g_build_params *p, k2g_builder **out);
+/* Sequence i = seq[off[i], off[i+1]) belongs to internal taxon taxid[i]. */
+int k2g_build_insert(k2g_builder *b, const char *seq, const uint64_t *off, const uint32_t *taxid,
+                     uint64_t n);
+/* Same with seq in HBM (off and taxid stay in host memory). */
+int k2g_build_insert_device(k2g_builder *b, const char *d_seq, const uint64_t *off,
+                            const uint32_t *taxid, uint64_t n);
+int k2g_build_get_info(k2g_builder *b, uint64_t *capacity, uint64_t *size, uint64_t *key_bits,
+                       uint64_t *value_bits);
+int k2g_build_download(k2g_builder *b, uint32_t *cells, uint64_t capacity);
+/* hash.k2d in the reference's WriteTable layout (src/compact_hash.h:343-352). */
+int k2g_build_write(k2g_builder *b, const char *hash_k2d);
+int k2g_build_destroy(k2g_builder *b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -13,9 +13,12 @@ from ._lib import (  # noqa: F401
     EXPORTED_SYMBOLS,
     LIB_PATH,
     Batch,
+    Builder,
+    BuildParams,
     ClassifyOutput,
     Db,
     K2GError,
+    expand_spaced_seed_mask,
     format_hitlist,
     load_library,
     pack_fragments,

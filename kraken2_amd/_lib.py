@@ -34,6 +34,13 @@ EXPORTED_SYMBOLS = (
     "k2g_batch_sync",
     "k2g_batch_fetch",
     "k2g_batch_stream",
+    "k2g_build_create",
+    "k2g_build_insert",
+    "k2g_build_insert_device",
+    "k2g_build_get_info",
+    "k2g_build_download",
+    "k2g_build_write",
+    "k2g_build_destroy",
 )
 
 
@@ -293,3 +300,84 @@ def format_hitlist(runs: np.ndarray, external_ids: np.ndarray) -> str:
         else:
             out.append(f"{int(external_ids[code])}:{cnt}" + ("" if i == last else " "))
     return "".join(out)
+
+
+class BuildParams(C.Structure):
+    _fields_ = [
+        ("k", C.c_uint64),
+        ("l", C.c_uint64),
+        ("spaced_seed_mask", C.c_uint64),
+        ("toggle_mask", C.c_uint64),
+        ("min_clear_hash_value", C.c_uint64),
+        ("block_size", C.c_uint64),
+        ("subblock_size", C.c_uint64),
+        ("dna_db", C.c_int32),
+        ("revcom_version", C.c_int32),
+    ]
+
+
+def expand_spaced_seed_mask(template: str, bits_per_char: int) -> int:
+    """Template bit string -> mask with bits_per_char bits per template bit
+    (src/utilities.cc:14-27)."""
+    t = int(template, 2)
+    out = 0
+    ones = (1 << bits_per_char) - 1
+    for i in range(64 // bits_per_char - 1, -1, -1):
+        out <<= bits_per_char
+        if (t >> i) & 1:
+            out |= ones
+    return out & 0xFFFFFFFFFFFFFFFF
+
+
+class Builder:
+    """GPU table construction (atomicCAS + LCA) into a resident cell array."""
+
+    def __init__(self, taxo_k2d: str, capacity: int, *, k=35, l=31, spaced_seed_mask=0,
+                 toggle_mask=0xE37E28C4271B5A2D, min_clear_hash_value=0, block_size=10 * 1024 * 1024,
+                 subblock_size=1024, dna=True, value_bits=0):
+        lib = load_library()
+        lib.k2g_build_create.argtypes = [C.c_char_p, C.c_uint64, C.c_uint64, C.POINTER(BuildParams),
+                                         C.POINTER(C.c_void_p)]
+        lib.k2g_build_insert.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]
+        lib.k2g_build_insert_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]
+        lib.k2g_build_get_info.argtypes = [C.c_void_p] + [C.POINTER(C.c_uint64)] * 4
+        lib.k2g_build_download.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64]
+        lib.k2g_build_write.argtypes = [C.c_void_p, C.c_char_p]
+        lib.k2g_build_destroy.argtypes = [C.c_void_p]
+        p = BuildParams(k, l, spaced_seed_mask, toggle_mask, min_clear_hash_value, block_size,
+                        subblock_size, int(dna), 1)
+        h = C.c_void_p()
+        _check(lib.k2g_build_create(taxo_k2d.encode(), capacity, value_bits, C.byref(p), C.byref(h)))
+        self._h = h
+        self.capacity = capacity
+
+    def insert(self, seq: bytes, off: np.ndarray, taxid: np.ndarray):
+        off = np.ascontiguousarray(off, dtype=np.uint64)
+        taxid = np.ascontiguousarray(taxid, dtype=np.uint32)
+        buf = np.frombuffer(seq, np.uint8) if len(seq) else np.zeros(1, np.uint8)
+        _check(_lib.k2g_build_insert(self._h, buf.ctypes.data, off.ctypes.data, taxid.ctypes.data,
+                                     taxid.shape[0]))
+
+    def info(self):
+        v = [C.c_uint64() for _ in range(4)]
+        _check(_lib.k2g_build_get_info(self._h, *[C.byref(x) for x in v]))
+        return dict(zip(("capacity", "size", "key_bits", "value_bits"), [x.value for x in v]))
+
+    def cells(self) -> np.ndarray:
+        out = np.zeros(self.capacity, np.uint32)
+        _check(_lib.k2g_build_download(self._h, out.ctypes.data, self.capacity))
+        return out
+
+    def write(self, path: str):
+        _check(_lib.k2g_build_write(self._h, path.encode()))
+
+    def close(self):
+        if self._h:
+            _lib.k2g_build_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,511 @@
+// k2g_classify: the classify front-end over libk2g.
+//
+// Same command line as the reference's classify binary (getopt string of
+// src/classify.cc:1076), so scripts/kraken2 and scripts/k2 can exec it
+// unchanged.  Input is cut into the reference's blocks (3 MiB of bases for
+// single reads, 10 000 pairs from two files, 20 000 interleaved records;
+// src/classify.cc:523-543); several blocks go to the GPU as one batch, and
+// the output, statistics and per-taxon counters are produced block by block
+// in input order exactly as the reference's ordered output does.
+#include <getopt.h>
+#include <sys/time.h>
+#include <unistd.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <iostream>
+#include <string>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "k2g.h"
+#include "k2g_host.h"
+
+using k2host::Record;
+
+namespace {
+
+struct Options {
+  std::string index_filename, taxonomy_filename, options_filename, report_filename,
+      classified_output_filename, unclassified_output_filename, kraken_output_filename;
+  bool mpa_style_report = false, report_kmer_data = false, quick_mode = false,
+       report_zero_counts = false, print_scientific_name = false, paired_end_processing = false,
+       single_file_pairs = false;
+  double confidence_threshold = 0;
+  int num_threads = 1, minimum_quality_score = 0, minimum_hit_groups = 0;
+  int gpu_mode = 1;                 // K2G_MODE: 0 plain, 1 persistent
+  size_t batch_bases = 256u << 20;  // bases per GPU batch (several blocks)
+  std::vector<std::string> filenames;
+};
+
+struct Stats {
+  uint64_t total_sequences = 0, total_bases = 0, total_classified = 0;
+};
+
+void usage(int code) {
+  std::cerr << "Usage: classify [options] <fasta/fastq file(s)>\n\n"
+               "Options: (*mandatory)\n"
+               "* -H filename      Kraken 2 index filename\n"
+               "* -t filename      Kraken 2 taxonomy filename\n"
+               "* -o filename      Kraken 2 options filename\n"
+               "  -q               Quick mode\n"
+               "  -M               Use memory mapping to access hash & taxonomy\n"
+               "  -T NUM           Confidence score threshold (def. 0)\n"
+               "  -p NUM           Number of host threads (def. 1)\n"
+               "  -Q NUM           Minimum quality score (FASTQ only, def. 0)\n"
+               "  -P               Process pairs of reads\n"
+               "  -S               Process pairs with mates in same file\n"
+               "  -R filename      Print report to filename\n"
+               "  -m               In comb. w/ -R, use mpa-style report\n"
+               "  -z               In comb. w/ -R, report taxa w/ 0 count\n"
+               "  -n               Print scientific name instead of taxid in Kraken output\n"
+               "  -g NUM           Minimum number of hit groups needed for call\n"
+               "  -C filename      Filename/format to have classified sequences\n"
+               "  -U filename      Filename/format to have unclassified sequences\n"
+               "  -O filename      Output file for normal Kraken output\n"
+               "  -K               In comb. w/ -R, provide minimizer information in report\n";
+  exit(code);
+}
+
+void die(int code, const std::string &msg) {
+  std::cerr << "classify: " << msg << "\n";
+  exit(code);
+}
+
+void parse(int argc, char **argv, Options &o) {
+  int opt;
+  while ((opt = getopt(argc, argv, "h?H:t:o:T:p:R:C:U:O:Q:g:nmzqPSMKD")) != -1) {
+    switch (opt) {
+      case 'h': case '?': usage(0); break;
+      case 'H': o.index_filename = optarg; break;
+      case 't': o.taxonomy_filename = optarg; break;
+      case 'T':
+        o.confidence_threshold = std::stod(optarg);
+        if (o.confidence_threshold < 0 || o.confidence_threshold > 1)
+          die(64, "confidence threshold must be in [0, 1]");
+        break;
+      case 'o': o.options_filename = optarg; break;
+      case 'q': o.quick_mode = true; break;
+      case 'p':
+        o.num_threads = atoi(optarg);
+        if (o.num_threads < 1) die(64, "number of threads can't be less than 1");
+        break;
+      case 'g': o.minimum_hit_groups = atoi(optarg); break;
+      case 'P': o.paired_end_processing = true; break;
+      case 'S': o.paired_end_processing = o.single_file_pairs = true; break;
+      case 'm': o.mpa_style_report = true; break;
+      case 'K': o.report_kmer_data = true; break;
+      case 'R': o.report_filename = optarg; break;
+      case 'z': o.report_zero_counts = true; break;
+      case 'C': o.classified_output_filename = optarg; break;
+      case 'U': o.unclassified_output_filename = optarg; break;
+      case 'O': o.kraken_output_filename = optarg; break;
+      case 'n': o.print_scientific_name = true; break;
+      case 'Q': o.minimum_quality_score = atoi(optarg); break;
+      case 'M': break;  // the table is resident in HBM either way
+      case 'D': die(64, "daemon mode is served by the k2g request loop, not this front-end");
+    }
+  }
+  if (o.index_filename.empty() || o.taxonomy_filename.empty() || o.options_filename.empty()) {
+    std::cerr << "classify: mandatory filename missing\n";
+    usage(64);
+  }
+  if (o.mpa_style_report && o.report_filename.empty()) {
+    std::cerr << "classify: -m requires -R be used\n";
+    usage(64);
+  }
+  if (o.mpa_style_report) die(64, "mpa-style reports are not provided by this front-end");
+  for (int i = optind; i < argc; i++) o.filenames.push_back(argv[i]);
+  if (const char *m = getenv("K2G_MODE")) o.gpu_mode = atoi(m) ? 1 : 0;
+  if (const char *b = getenv("K2G_BATCH_BASES")) o.batch_bases = strtoull(b, nullptr, 10);
+}
+
+struct Outputs {
+  FILE *kraken = stdout, *c1 = nullptr, *c2 = nullptr, *u1 = nullptr, *u2 = nullptr;
+};
+
+FILE *open_out(const std::string &path) {
+  FILE *f = fopen(path.c_str(), "w");
+  if (!f) {
+    std::cerr << "\rUnable to open file: " << path << ", reason: " << strerror(errno) << "\n";
+    exit(EXIT_FAILURE);
+  }
+  return f;
+}
+
+// src/classify.cc:1008-1059 (paired names expand '#' to _1/_2)
+void open_outputs(const Options &o, Outputs &out) {
+  auto split = [&](const std::string &fmt, FILE *&a, FILE *&b) {
+    std::vector<std::string> parts;
+    size_t p = 0;
+    for (;;) {
+      size_t q = fmt.find('#', p);
+      if (q == std::string::npos || parts.size() == 2) {
+        parts.push_back(fmt.substr(p));
+        break;
+      }
+      parts.push_back(fmt.substr(p, q - p));
+      p = q + 1;
+    }
+    if (parts.size() < 2) die(65, "Paired filename format missing # character: " + fmt);
+    if (parts.size() > 2) die(65, "Paired filename format has >1 # character: " + fmt);
+    a = open_out(parts[0] + "_1" + parts[1]);
+    b = open_out(parts[0] + "_2" + parts[1]);
+  };
+  if (!o.classified_output_filename.empty()) {
+    if (o.paired_end_processing)
+      split(o.classified_output_filename, out.c1, out.c2);
+    else
+      out.c1 = open_out(o.classified_output_filename);
+  }
+  if (!o.unclassified_output_filename.empty()) {
+    if (o.paired_end_processing)
+      split(o.unclassified_output_filename, out.u1, out.u2);
+    else
+      out.u1 = open_out(o.unclassified_output_filename);
+  }
+  if (!o.kraken_output_filename.empty())
+    out.kraken = o.kraken_output_filename == "-" ? nullptr : open_out(o.kraken_output_filename);
+}
+
+std::string trim_pair_info(const std::string &id) {  // src/classify.cc:764-771
+  const size_t n = id.size();
+  if (n <= 2) return id;
+  if (id[n - 2] == '/' && (id[n - 1] == '1' || id[n - 1] == '2')) return id.substr(0, n - 2);
+  return id;
+}
+
+void mask_low_quality(Record &r, int q) {  // src/classify.cc:1061-1071
+  if (!r.fastq) return;
+  if (r.seq.size() != r.quals.size()) {
+    std::cerr << "classify: " << r.header << ": Sequence length (" << r.seq.size()
+              << ") != Quality string length (" << r.quals.size() << ")\n";
+    exit(65);
+  }
+  for (size_t i = 0; i < r.seq.size(); i++)
+    if ((r.quals[i] - '!') < q) r.seq[i] = 'x';
+}
+
+struct Block {
+  std::vector<Record> a, b;
+  size_t frags = 0;   // fragments in this block
+  size_t first = 0;   // index of its first fragment in the GPU batch
+  std::string kraken, c1, c2, u1, u2;
+  Stats stats;
+};
+
+struct Ctx {
+  Options o;
+  k2host::Taxonomy tax;
+  k2g_db *db = nullptr;
+  k2g_batch *batch = nullptr;
+  bool translated = false;
+  Stats stats;
+  k2host::TaxonCounters counters;
+  std::vector<uint64_t> kmer_totals;
+};
+
+void append_uint(std::string &s, uint64_t v) {
+  char buf[24];
+  int n = 0;
+  do {
+    buf[n++] = (char) ('0' + v % 10);
+    v /= 10;
+  } while (v);
+  while (n) s.push_back(buf[--n]);
+}
+
+// One kraken output line (src/classify.cc:908-945, 950-991).
+void format_line(const Ctx &c, const Record &s1, const Record *s2, uint32_t call,
+                 const uint32_t *runs, uint64_t nruns, std::string &out) {
+  const auto &tax = c.tax;
+  out += call ? "C\t" : "U\t";
+  if (s2)
+    out += trim_pair_info(s1.header);
+  else
+    out += s1.header;
+  out.push_back('\t');
+  const uint64_t ext = tax.nodes[call].external_id;
+  if (c.o.print_scientific_name) {
+    out += call ? tax.name(call) : "unclassified";
+    out += " (taxid ";
+    append_uint(out, ext);
+    out += ")";
+  } else {
+    append_uint(out, ext);
+  }
+  out.push_back('\t');
+  append_uint(out, s1.seq.size());
+  if (s2) {
+    out.push_back('|');
+    append_uint(out, s2->seq.size());
+  }
+  out.push_back('\t');
+  if (c.o.quick_mode) {
+    append_uint(out, ext);
+    out += ":Q";
+  } else if (nruns == 0) {
+    out += "0:0";
+  } else {
+    for (uint64_t i = 0; i < nruns; i++) {
+      const uint32_t code = runs[2 * i], cnt = runs[2 * i + 1];
+      const bool last = i + 1 == nruns;
+      if (code == K2G_CODE_MATE) {
+        out += last ? "|:|" : "|:| ";
+      } else if (code == K2G_CODE_FRAME) {
+        out += last ? "-:-" : "-:- ";
+      } else if (code == K2G_CODE_AMBIG) {
+        out += "A:";
+        append_uint(out, cnt);
+        out.push_back(' ');
+      } else {
+        append_uint(out, tax.nodes[code].external_id);
+        out.push_back(':');
+        append_uint(out, cnt);
+        if (!last) out.push_back(' ');
+      }
+    }
+  }
+  out.push_back('\n');
+}
+
+void run_batch(Ctx &c, std::vector<Block> &blocks, Outputs &out) {
+  const Options &o = c.o;
+  const bool paired = o.paired_end_processing;
+  // fragment arrays
+  size_t nfrag = 0, bases = 0;
+  for (auto &b : blocks) {
+    b.first = nfrag;
+    if (!paired)
+      b.frags = b.a.size();
+    else if (o.single_file_pairs)
+      b.frags = b.a.size() / 2;
+    else
+      b.frags = std::min(b.a.size(), b.b.size());
+    nfrag += b.frags;
+    for (auto &r : b.a) bases += r.seq.size();
+    for (auto &r : b.b) bases += r.seq.size();
+  }
+  std::string seq;
+  seq.reserve(bases);
+  std::vector<uint64_t> off(2 * nfrag + 1, 0);
+  size_t f = 0;
+  for (auto &b : blocks) {
+    for (size_t i = 0; i < b.frags; i++, f++) {
+      Record *r1, *r2 = nullptr;
+      if (!paired) {
+        r1 = &b.a[i];
+      } else if (o.single_file_pairs) {
+        r1 = &b.a[2 * i];
+        r2 = &b.a[2 * i + 1];
+      } else {
+        r1 = &b.a[i];
+        r2 = &b.b[i];
+      }
+      if (o.minimum_quality_score > 0) {
+        mask_low_quality(*r1, o.minimum_quality_score);
+        if (r2) mask_low_quality(*r2, o.minimum_quality_score);
+      }
+      off[2 * f] = seq.size();
+      seq += r1->seq;
+      off[2 * f + 1] = seq.size();
+      if (r2) seq += r2->seq;
+    }
+  }
+  off[2 * nfrag] = seq.size();
+
+  k2g_classify_params p{};
+  p.confidence = o.confidence_threshold;
+  p.min_hit_groups = o.minimum_hit_groups;
+  p.quick = o.quick_mode;
+  p.paired = paired;
+  p.mode = o.gpu_mode;
+  p.want_kmer_counts = !o.report_filename.empty();
+  p.want_hll = !o.report_filename.empty() && o.report_kmer_data;
+  k2g_fragments fr{seq.data(), off.data(), nullptr, nfrag};
+  k2g_results res;
+  int rc = k2g_classify(c.batch, &p, &fr, &res);
+  if (rc) die(rc, k2g_last_error());
+
+  // format blocks in parallel, emit in order
+#pragma omp parallel for schedule(dynamic, 1) num_threads(o.num_threads)
+  for (size_t bi = 0; bi < blocks.size(); bi++) {
+    Block &b = blocks[bi];
+    b.kraken.clear();
+    for (size_t i = 0; i < b.frags; i++) {
+      const size_t g = b.first + i;
+      Record *r1, *r2 = nullptr;
+      if (!paired) {
+        r1 = &b.a[i];
+      } else if (o.single_file_pairs) {
+        r1 = &b.a[2 * i];
+        r2 = &b.a[2 * i + 1];
+      } else {
+        r1 = &b.a[i];
+        r2 = &b.b[i];
+      }
+      const uint32_t call = res.call[g];
+      b.stats.total_sequences++;
+      if (call) b.stats.total_classified++;
+      b.stats.total_bases += r1->seq.size() + (r2 ? r2->seq.size() : 0);
+      if (out.kraken)
+        format_line(c, *r1, r2, call, res.runs + 2 * res.run_off[g], res.run_off[g + 1] - res.run_off[g],
+                    b.kraken);
+      if (call) {
+        if (out.c1) {
+          char suffix[48];
+          snprintf(suffix, sizeof(suffix), " kraken:taxid|%llu",
+                   (unsigned long long) c.tax.nodes[call].external_id);
+          r1->append_text(b.c1, suffix);
+          if (r2) r2->append_text(b.c2, suffix);
+        }
+      } else if (out.u1) {
+        r1->append_text(b.u1, nullptr);
+        if (r2) r2->append_text(b.u2, nullptr);
+      }
+    }
+  }
+  for (auto &b : blocks) {
+    if (out.kraken) fwrite(b.kraken.data(), 1, b.kraken.size(), out.kraken);
+    if (out.c1) fwrite(b.c1.data(), 1, b.c1.size(), out.c1);
+    if (out.c2) fwrite(b.c2.data(), 1, b.c2.size(), out.c2);
+    if (out.u1) fwrite(b.u1.data(), 1, b.u1.size(), out.u1);
+    if (out.u2) fwrite(b.u2.data(), 1, b.u2.size(), out.u2);
+    c.stats.total_sequences += b.stats.total_sequences;
+    c.stats.total_bases += b.stats.total_bases;
+    c.stats.total_classified += b.stats.total_classified;
+  }
+  if (!o.report_filename.empty()) {
+    // read counts per called taxon; k-mer counters per block (HLL state
+    // depends on the block a hit falls in, src/classify.cc:554, 640-645)
+    for (auto &b : blocks)
+      for (size_t i = 0; i < b.frags; i++)
+        if (res.call[b.first + i]) c.counters[res.call[b.first + i]].n_reads++;
+    if (o.report_kmer_data) {
+      std::vector<std::pair<uint32_t, uint32_t>> recs;
+      for (auto &b : blocks) {
+        recs.clear();
+        for (size_t i = 0; i < b.frags; i++) {
+          const size_t g = b.first + i;
+          for (uint64_t j = res.hll_off[g]; j < res.hll_off[g + 1]; j++)
+            recs.emplace_back(res.hll[2 * j], res.hll[2 * j + 1]);
+        }
+        k2host::fold_block_records(recs, c.counters);
+      }
+    } else {
+      for (uint64_t t = 0; t < res.node_count; t++)
+        if (res.kmer_counts[t]) c.counters[t].n_kmers += res.kmer_counts[t];
+    }
+  }
+}
+
+int process_files(Ctx &c, const char *f1, const char *f2, Outputs &out) {
+  const Options &o = c.o;
+  k2host::Reader r1, r2;
+  if (!r1.open(f1)) die(66, std::string("unable to open ") + (f1 ? f1 : "stdin"));
+  if (f2 && !r2.open(f2)) die(66, std::string("unable to open ") + f2);
+  bool done = false;
+  std::vector<Block> blocks;
+  while (!done) {
+    blocks.clear();
+    size_t bases = 0;
+    while (bases < o.batch_bases && blocks.size() < 4096) {
+      blocks.emplace_back();
+      Block &b = blocks.back();
+      bool ok;
+      if (!o.paired_end_processing) {
+        ok = r1.load_block(3 * 1024 * 1024, b.a) > 0;
+      } else if (!o.single_file_pairs) {
+        ok = r1.load_batch(10000, b.a) > 0;
+        if (ok) ok = r2.load_batch(10000, b.b) > 0;
+      } else {
+        ok = r1.load_batch(20000, b.a) > 0;
+      }
+      if (!ok) {
+        blocks.pop_back();
+        done = true;
+        break;
+      }
+      for (auto &r : b.a) bases += r.seq.size();
+      for (auto &r : b.b) bases += r.seq.size();
+    }
+    if (!blocks.empty()) run_batch(c, blocks, out);
+  }
+  return 0;
+}
+
+void report_stats(double seconds, const Stats &s) {  // src/classify.cc:447-477
+  const uint64_t unc = s.total_sequences - s.total_classified;
+  if (isatty(fileno(stderr))) std::cerr << "\r";
+  fprintf(stderr, "%llu sequences (%.2f Mbp) processed in %.3fs (%.1f Kseq/m, %.2f Mbp/m).\n",
+          (unsigned long long) s.total_sequences, s.total_bases / 1.0e6, seconds,
+          s.total_sequences / 1.0e3 / (seconds / 60), s.total_bases / 1.0e6 / (seconds / 60));
+  fprintf(stderr, "  %llu sequences classified (%.2f%%)\n", (unsigned long long) s.total_classified,
+          s.total_classified * 100.0 / s.total_sequences);
+  fprintf(stderr, "  %llu sequences unclassified (%.2f%%)\n", (unsigned long long) unc,
+          unc * 100.0 / s.total_sequences);
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+  Ctx c;
+  parse(argc, argv, c.o);
+  std::cerr << "Loading database information...";
+  std::string err;
+  if (!c.tax.load(c.o.taxonomy_filename, err)) die(65, err);
+  int rc = k2g_db_open(c.o.index_filename.c_str(), c.o.taxonomy_filename.c_str(),
+                       c.o.options_filename.c_str(), &c.db);
+  if (rc) {
+    std::cerr << "\n";
+    die(rc, k2g_last_error());
+  }
+  k2g_db_info info;
+  k2g_db_get_info(c.db, &info);
+  c.translated = !info.dna_db;
+  rc = k2g_batch_create(c.db, &c.batch);
+  if (rc) die(rc, k2g_last_error());
+  std::cerr << " done." << std::endl;
+
+  Outputs out;
+  open_outputs(c.o, out);
+  struct timeval t1, t2;
+  gettimeofday(&t1, nullptr);
+  const Options &o = c.o;
+  if (o.filenames.empty()) {
+    if (o.paired_end_processing && !o.single_file_pairs)
+      die(64, "paired end processing used with no files specified");
+    process_files(c, nullptr, nullptr, out);
+  } else {
+    for (size_t i = 0; i < o.filenames.size(); i++) {
+      if (o.paired_end_processing && !o.single_file_pairs) {
+        if (i + 1 == o.filenames.size()) die(64, "paired end processing used with unpaired file");
+        process_files(c, o.filenames[i].c_str(), o.filenames[i + 1].c_str(), out);
+        i++;
+      } else {
+        process_files(c, o.filenames[i].c_str(), nullptr, out);
+      }
+    }
+  }
+  for (FILE *f : {out.kraken, out.c1, out.c2, out.u1, out.u2})
+    if (f) fflush(f);
+  gettimeofday(&t2, nullptr);
+  const double secs = (t2.tv_sec - t1.tv_sec) + (t2.tv_usec - t1.tv_usec) / 1e6;
+  report_stats(secs, c.stats);
+  if (!o.report_filename.empty()) {
+    if (!k2host::report_kraken_style(o.report_filename, o.report_zero_counts, o.report_kmer_data,
+                                     c.tax, c.counters, c.stats.total_sequences,
+                                     c.stats.total_sequences - c.stats.total_classified))
+      die(74, "unable to write report " + o.report_filename);
+  }
+  for (FILE *f : {out.c1, out.c2, out.u1, out.u2})
+    if (f) fclose(f);
+  if (out.kraken && out.kraken != stdout) fclose(out.kraken);
+  k2g_batch_destroy(c.batch);
+  k2g_db_close(c.db);
+  return 0;
+}

@@ -1,0 +1,82 @@
+// Host-side pieces of the classify front-end around libk2g: sequence input,
+// taxonomy names/ranks, distinct-minimizer estimates and the kreport.
+#pragma once
+#include <cstddef>
+#include <cstdint>
+#include <map>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+namespace k2host {
+
+// One FASTA/FASTQ record with kseq field semantics (src/seqreader.h:29-61).
+struct Record {
+  bool fastq = false;
+  std::string header, comment, seq, quals;
+  void append_text(std::string &out, const char *header_suffix) const;  // src/seqreader.cc:20-49
+};
+
+// Buffered reader (src/kseq.h record rules: '>'/'@' headers, multi-line
+// FASTA, '+' separated FASTQ, one trailing '\r' dropped per line).
+class Reader {
+ public:
+  bool open(const char *path);  // nullptr = stdin
+  ~Reader();
+  bool next(Record &r);
+  // src/seqreader.h:101-119 and 146-163
+  size_t load_block(size_t bases, std::vector<Record> &out);
+  size_t load_batch(size_t records, std::vector<Record> &out);
+
+ private:
+  int get();
+  void line_into(std::string &dst, bool append, int *last);
+  int fd_ = -1;
+  bool own_ = false, eof_ = false;
+  std::vector<char> buf_;
+  size_t pos_ = 0, end_ = 0;
+  int pending_ = 0;
+};
+
+// taxo.k2d for names, ranks, parents and external ids (src/taxonomy.cc:195-238).
+struct Taxonomy {
+  struct Node {
+    uint64_t parent_id, first_child, child_count, name_offset, rank_offset, external_id,
+        godparent_id;
+  };
+  std::vector<Node> nodes;
+  std::vector<char> names, ranks;
+  bool load(const std::string &path, std::string &err);
+  const char *name(uint64_t id) const { return names.data() + nodes[id].name_offset; }
+  const char *rank(uint64_t id) const { return ranks.data() + nodes[id].rank_offset; }
+};
+
+// Distinct-minimizer state of one taxon or clade.  The reference's
+// HyperLogLogPlusMinus (p=12, sparse p'=25) ends in a state fixed by the set
+// of sparse-encoded hashes it saw and whether any per-block sketch switched
+// to registers (src/hyperloglogplus.cc:471-509, 572-651); both are kept here.
+struct Distinct {
+  uint64_t n_observed = 0;
+  bool dense = false;
+  std::vector<uint32_t> encoded;  // sorted, unique
+  void merge(const Distinct &o);
+  uint64_t estimate() const;      // Ertl estimator, src/hyperloglogplus.cc:708-739
+};
+
+struct TaxonCounter {
+  uint64_t n_reads = 0, n_kmers = 0;
+  Distinct distinct;
+};
+using TaxonCounters = std::map<uint64_t, TaxonCounter>;
+
+// Per-block HLL bookkeeping: records (taxon, encoded) in insertion order for
+// one input block (src/classify.cc:554, 640-645).
+void fold_block_records(const std::vector<std::pair<uint32_t, uint32_t>> &recs,
+                        TaxonCounters &total);
+
+// src/reports.cc:241-259
+bool report_kraken_style(const std::string &path, bool report_zeros, bool kmer_data,
+                         const Taxonomy &tax, const TaxonCounters &calls, uint64_t total_seqs,
+                         uint64_t total_unclassified);
+
+}  // namespace k2host

@@ -27,6 +27,7 @@ __global__ void k_classify_dense(DbView, ClassifyParams, FragBatch, FragOut, con
                                  uint32_t, DenseScratch);
 __global__ void k_get_batch(DbView, const uint64_t *, uint32_t *, uint64_t);
 __global__ void k_max_value(DbView, unsigned int *);
+__global__ void k_build(BuildView, const char *, const SubBlock *, uint64_t);
 }  // namespace k2g
 
 using namespace k2g;
@@ -748,4 +749,247 @@ extern "C" int k2g_classify(k2g_batch *b, const k2g_classify_params *p, const k2
                            nullptr);
   if (rc) return rc;
   return k2g_batch_fetch(b, out);
+}
+
+// ----------------------------------------------------------------------
+// build
+struct k2g_builder {
+  BuildView view{};
+  uint32_t *d_cells = nullptr, *d_parent = nullptr;
+  unsigned long long *d_size = nullptr;
+  unsigned int *d_error = nullptr;
+  uint64_t capacity = 0, node_count = 0;
+  k2g_build_params params{};
+  hipStream_t stream = nullptr;
+  DevBuf<char> seq;
+  DevBuf<SubBlock> items;
+  int num_cus = 0;
+};
+
+static int read_parents(const char *taxo_k2d, std::vector<uint32_t> &parent) {
+  FILE *f = fopen(taxo_k2d, "rb");
+  if (!f) return fail(K2G_EOS, std::string("unable to open ") + taxo_k2d);
+  char magic[8];
+  uint64_t hdr[3];
+  if (fread(magic, 1, 8, f) != 8 || std::memcmp(magic, "K2TAXDAT", 8) != 0 ||
+      fread(hdr, 8, 3, f) != 3 || hdr[0] == 0 || hdr[0] >= (1ULL << 31)) {
+    fclose(f);
+    return fail(K2G_EDATA, std::string("malformed taxonomy file ") + taxo_k2d);
+  }
+  std::vector<uint64_t> raw(hdr[0] * 7);
+  const size_t got = fread(raw.data(), 56, hdr[0], f);
+  fclose(f);
+  if (got != hdr[0]) return fail(K2G_EDATA, std::string("read exhausted taxonomy information in ") + taxo_k2d);
+  parent.resize(hdr[0]);
+  for (uint64_t i = 0; i < hdr[0]; i++) {
+    const uint64_t pid = raw[7 * i];
+    if (i > 0 && pid >= i)
+      return fail(K2G_EDATA, "taxonomy node " + std::to_string(i) + " has parent id " +
+                                 std::to_string(pid) + " (parents must precede children)");
+    parent[i] = (uint32_t) (i ? pid : 0);
+  }
+  return K2G_OK;
+}
+
+extern "C" int k2g_build_create(const char *taxo_k2d, uint64_t capacity, uint64_t value_bits,
+                                const k2g_build_params *p, k2g_builder **out) {
+  if (!taxo_k2d || !p || !out) return fail(K2G_EUSAGE, "null argument");
+  *out = nullptr;
+  if (capacity == 0) return fail(K2G_EUSAGE, "capacity must be positive integer");
+  const uint32_t bits = p->dna_db ? 2 : 4;
+  if (p->l == 0 || p->l > 63 / bits) return fail(K2G_EUSAGE, "l exceeds size limits for the minimizer scanner");
+  if (p->k < p->l) return fail(K2G_EUSAGE, "k cannot be less than l");
+  if (p->k - p->l > 63) return fail(K2G_EUSAGE, "k - l > 63 is not supported by the GPU scan");
+  if (p->subblock_size == 0 || p->block_size < p->subblock_size)
+    return fail(K2G_EUSAGE, "block size cannot be less than subblock size");
+  std::vector<uint32_t> parent;
+  int rc = read_parents(taxo_k2d, parent);
+  if (rc) return rc;
+  uint64_t vb = 1;
+  while ((1ULL << vb) < parent.size()) vb++;
+  if (value_bits) {
+    if (value_bits < vb) return fail(K2G_EDATA, "more bits required for storing taxid");
+    vb = value_bits;
+  }
+  if (vb > 31) return fail(K2G_EUSAGE, "can't have more than 31 bits of storage for taxid");
+  auto *b = new k2g_builder();
+  b->capacity = capacity;
+  b->node_count = parent.size();
+  b->params = *p;
+  const uint64_t padded = (capacity + 63) / 64 * 64;
+  hipError_t e = hipMalloc((void **) &b->d_cells, padded * 4);
+  if (e == hipSuccess) e = hipMemset(b->d_cells, 0, padded * 4);
+  if (e == hipSuccess) e = hipMalloc((void **) &b->d_parent, parent.size() * 4);
+  if (e == hipSuccess) e = hipMemcpy(b->d_parent, parent.data(), parent.size() * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMalloc((void **) &b->d_size, 8);
+  if (e == hipSuccess) e = hipMemset(b->d_size, 0, 8);
+  if (e == hipSuccess) e = hipMalloc((void **) &b->d_error, 4);
+  if (e == hipSuccess) e = hipMemset(b->d_error, 0, 4);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  int dev = 0;
+  if (e == hipSuccess) e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&b->num_cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e != hipSuccess) {
+    k2g_build_destroy(b);
+    return fail(K2G_EOS, std::string("unable to allocate hash table memory: ") + hipGetErrorString(e));
+  }
+  BuildView &v = b->view;
+  v.cells = b->d_cells;
+  v.capacity = capacity;
+  v.barrett = capacity > 1 ? (uint64_t) (((unsigned __int128) 1 << 64) / capacity) : 0;
+  v.key_bits = (uint32_t) (32 - vb);
+  v.value_bits = (uint32_t) vb;
+  v.parent = b->d_parent;
+  v.k = (uint32_t) p->k;
+  v.l = (uint32_t) p->l;
+  v.dna = p->dna_db ? 1 : 0;
+  v.revcom_version = (uint32_t) p->revcom_version;
+  const uint64_t lmask = (1ULL << (p->l * bits)) - 1;
+  v.spaced_mask = p->spaced_seed_mask;
+  v.toggle_mask = p->toggle_mask & lmask;
+  v.min_clear = p->min_clear_hash_value;
+  v.size = b->d_size;
+  v.error = b->d_error;
+  *out = b;
+  return K2G_OK;
+}
+
+extern "C" int k2g_build_destroy(k2g_builder *b) {
+  if (!b) return K2G_OK;
+  if (b->stream) (void) hipStreamSynchronize(b->stream);
+  if (b->d_cells) (void) hipFree(b->d_cells);
+  if (b->d_parent) (void) hipFree(b->d_parent);
+  if (b->d_size) (void) hipFree(b->d_size);
+  if (b->d_error) (void) hipFree(b->d_error);
+  b->seq.release();
+  b->items.release();
+  if (b->stream) (void) hipStreamDestroy(b->stream);
+  delete b;
+  return K2G_OK;
+}
+
+// Windows of the deterministic build (src/build_db.h:224-244): blocks of
+// block_size, each extended by k-1, cut into subblocks of subblock_size,
+// each extended by k-1 within its block.
+static void build_windows(const k2g_build_params &p, const uint64_t *off, const uint32_t *taxid,
+                          uint64_t n, uint64_t base, std::vector<SubBlock> &items) {
+  const uint64_t k = p.k, B = p.block_size, b = p.subblock_size;
+  for (uint64_t s = 0; s < n; s++) {
+    const uint64_t L = off[s + 1] - off[s];
+    if (taxid[s] == 0) continue;
+    for (uint64_t j = 0; j < L; j += B) {
+      const uint64_t bfin = std::min(j + B + k - 1, L);
+      for (uint64_t i = j; i < bfin; i += b) {
+        const uint64_t fin = std::min(i + b + k - 1, bfin);
+        SubBlock it;
+        it.start = off[s] - base + i;
+        it.len = (uint32_t) (fin - i);
+        it.taxid = taxid[s];
+        it.first_e = (uint32_t) (i == 0 ? k - 1 : k - 2);
+        it.pad = 0;
+        items.push_back(it);
+      }
+    }
+  }
+}
+
+static int build_launch(k2g_builder *b, const char *d_seq, const std::vector<SubBlock> &items) {
+  if (items.empty()) return K2G_OK;
+  HIP_TRY(b->items.reserve(items.size()));
+  HIP_TRY(hipMemcpyAsync(b->items.p, items.data(), items.size() * sizeof(SubBlock),
+                         hipMemcpyHostToDevice, b->stream));
+  const uint64_t grid = std::min<uint64_t>(items.size(), (uint64_t) std::max(1, b->num_cus) * 32);
+  hipLaunchKernelGGL(k_build, dim3((unsigned) grid), dim3(64), 0, b->stream, b->view, d_seq,
+                     b->items.p, (uint64_t) items.size());
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  unsigned int err = 0;
+  HIP_TRY(hipMemcpy(&err, b->d_error, 4, hipMemcpyDeviceToHost));
+  if (err) return fail(K2G_ESOFTWARE, "compact hash table capacity exceeded");
+  return K2G_OK;
+}
+
+static int check_window_lengths(const k2g_build_params &p) {
+  if (p.subblock_size + p.k > 0xFFFFFFFFull) return fail(K2G_EUSAGE, "subblock size too large");
+  return K2G_OK;
+}
+
+extern "C" int k2g_build_insert_device(k2g_builder *b, const char *d_seq, const uint64_t *off,
+                                       const uint32_t *taxid, uint64_t n) {
+  if (!b || (n && (!d_seq || !off || !taxid))) return fail(K2G_EUSAGE, "null argument");
+  int rc = check_window_lengths(b->params);
+  if (rc) return rc;
+  for (uint64_t i = 0; i < n; i++) {
+    if (off[i + 1] < off[i]) return fail(K2G_EUSAGE, "sequence offsets must not decrease");
+    if (taxid[i] >= b->node_count) return fail(K2G_EDATA, "taxid is not a node of the taxonomy");
+  }
+  // windows in chunks so the item list stays modest
+  std::vector<SubBlock> items;
+  uint64_t s = 0;
+  while (s < n) {
+    items.clear();
+    uint64_t e = s;
+    while (e < n && items.size() < (1u << 22)) {
+      build_windows(b->params, off + e, taxid + e, 1, 0, items);
+      e++;
+    }
+    rc = build_launch(b, d_seq, items);
+    if (rc) return rc;
+    s = e;
+  }
+  return K2G_OK;
+}
+
+extern "C" int k2g_build_insert(k2g_builder *b, const char *seq, const uint64_t *off,
+                                const uint32_t *taxid, uint64_t n) {
+  if (!b || (n && (!seq || !off || !taxid))) return fail(K2G_EUSAGE, "null argument");
+  if (n == 0) return K2G_OK;
+  const uint64_t base = off[0], bytes = off[n] - off[0];
+  HIP_TRY(b->seq.reserve(bytes + 1));
+  HIP_TRY(hipMemcpy(b->seq.p, seq + base, bytes, hipMemcpyHostToDevice));
+  std::vector<uint64_t> rel(n + 1);
+  for (uint64_t i = 0; i <= n; i++) rel[i] = off[i] - base;
+  return k2g_build_insert_device(b, b->seq.p, rel.data(), taxid, n);
+}
+
+extern "C" int k2g_build_get_info(k2g_builder *b, uint64_t *capacity, uint64_t *size,
+                                  uint64_t *key_bits, uint64_t *value_bits) {
+  if (!b) return fail(K2G_EUSAGE, "null builder");
+  unsigned long long sz = 0;
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  HIP_TRY(hipMemcpy(&sz, b->d_size, 8, hipMemcpyDeviceToHost));
+  if (capacity) *capacity = b->capacity;
+  if (size) *size = sz;
+  if (key_bits) *key_bits = b->view.key_bits;
+  if (value_bits) *value_bits = b->view.value_bits;
+  return K2G_OK;
+}
+
+extern "C" int k2g_build_download(k2g_builder *b, uint32_t *cells, uint64_t capacity) {
+  if (!b || !cells) return fail(K2G_EUSAGE, "null argument");
+  if (capacity < b->capacity) return fail(K2G_EUSAGE, "output smaller than the table");
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  HIP_TRY(hipMemcpy(cells, b->d_cells, b->capacity * 4, hipMemcpyDeviceToHost));
+  return K2G_OK;
+}
+
+extern "C" int k2g_build_write(k2g_builder *b, const char *hash_k2d) {
+  if (!b || !hash_k2d) return fail(K2G_EUSAGE, "null argument");
+  uint64_t hdr[4];
+  int rc = k2g_build_get_info(b, &hdr[0], &hdr[1], &hdr[2], &hdr[3]);
+  if (rc) return rc;
+  FILE *f = fopen(hash_k2d, "wb");
+  if (!f) return fail(K2G_EOS, std::string("unable to write ") + hash_k2d);
+  bool ok = fwrite(hdr, 8, 4, f) == 4;
+  const uint64_t chunk = 64ull << 20;
+  std::vector<uint32_t> buf;
+  for (uint64_t pos = 0; ok && pos < b->capacity; pos += chunk) {
+    const uint64_t m = std::min(chunk, b->capacity - pos);
+    buf.resize(m);
+    if (hipMemcpy(buf.data(), b->d_cells + pos, m * 4, hipMemcpyDeviceToHost) != hipSuccess) ok = false;
+    if (ok) ok = fwrite(buf.data(), 4, m, f) == m;
+  }
+  if (fclose(f) != 0) ok = false;
+  if (!ok) return fail(K2G_EIO, std::string("error writing ") + hash_k2d);
+  return K2G_OK;
 }

@@ -44,3 +44,33 @@ struct DenseScratch {
 };
 
 }  // namespace k2g
+
+namespace k2g {
+
+// ---- build side (src/build_db.h:186-326, src/compact_hash.h:416-462)
+struct BuildView {
+  uint32_t *cells;
+  uint64_t capacity;
+  uint64_t barrett;
+  uint32_t key_bits, value_bits;
+  const uint32_t *parent;
+  uint32_t k, l, dna, revcom_version;
+  uint64_t spaced_mask, toggle_mask, min_clear;
+  unsigned long long *size;  // cells filled
+  unsigned int *error;       // 1: capacity exceeded
+};
+
+// One scanner window of the deterministic build: sequence bytes
+// [start, start + len) scanned with a fresh scanner.  first_e is the first
+// window-relative k-mer end the reference's scanner reports: k-1 at the
+// start of a sequence, k-2 at any later subblock start (src/mmscanner.cc:
+// 114-189 returns once str_pos_ >= k, which a start > 0 already satisfies).
+struct SubBlock {
+  uint64_t start;
+  uint32_t len;
+  uint32_t taxid;
+  uint32_t first_e;
+  uint32_t pad;
+};
+
+}  // namespace k2g

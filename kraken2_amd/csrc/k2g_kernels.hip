@@ -541,3 +541,150 @@ __global__ __launch_bounds__(256) void k_max_value(DbView db, unsigned int *out)
 }
 
 }  // namespace k2g
+
+namespace k2g {
+
+// Reduced amino-acid alphabet of the protein scanner (src/mmscanner.cc:
+// 41-95), 0xFF for every other byte.
+__device__ __forceinline__ uint32_t prot_code(uint8_t ch) {
+  const uint8_t c = (ch >= 'a' && ch <= 'z') ? (uint8_t) (ch - 32) : ch;
+  switch (c) {
+    case '*': case 'U': case 'O': return 0;
+    case 'A': return 1;
+    case 'N': case 'Q': case 'S': return 2;
+    case 'C': return 3;
+    case 'D': case 'E': return 4;
+    case 'F': return 5;
+    case 'G': return 6;
+    case 'H': return 7;
+    case 'I': case 'L': return 8;
+    case 'K': return 9;
+    case 'P': return 10;
+    case 'R': return 11;
+    case 'M': case 'V': return 12;
+    case 'T': return 13;
+    case 'W': return 14;
+    case 'Y': return 15;
+  }
+  return 0xFFu;
+}
+
+// CompareAndSet + SetMinimizerLCA as one lock-free loop: the first cell of
+// the linear probe that is empty or carries the same compacted key receives
+// LCA(old value, taxid) (src/build_db.h:186-194, src/compact_hash.h:416-462).
+__device__ void table_insert(const BuildView &b, uint64_t key, uint32_t taxid) {
+  const uint64_t hc = fmix64(key);
+  const uint32_t ck = (uint32_t) (hc >> (64 - b.key_bits));
+  const uint32_t vb = b.value_bits;
+  const uint32_t vmask = (1u << vb) - 1u;
+  uint64_t idx = mod_capacity(hc, b.capacity, b.barrett);
+  uint64_t left = b.capacity;
+  uint32_t c = load_agent(&b.cells[idx]);
+  for (;;) {
+    const uint32_t v = c & vmask;
+    if (v == 0 || (c >> vb) == ck) {
+      const uint32_t nv = v == 0 ? taxid : lca(b.parent, v, taxid);
+      if (v != 0 && nv == v) return;
+      const uint32_t want = (ck << vb) | nv;
+      const uint32_t old = atomicCAS(&b.cells[idx], c, want);
+      if (old == c) {
+        if (v == 0) atomicAdd(b.size, 1ULL);
+        return;
+      }
+      c = old;  // lost a race: look at the same cell again
+      continue;
+    }
+    if (--left == 0) {
+      atomicOr(b.error, 1u);
+      return;
+    }
+    if (++idx == b.capacity) idx = 0;
+    c = load_agent(&b.cells[idx]);
+  }
+}
+
+// Scans one build window and inserts its non-ambiguous minimizers.
+__device__ void build_window(const BuildView &b, const char *m, uint64_t n, uint32_t first_e,
+                             uint32_t taxid, uint64_t *ring, int lane) {
+  const bool dna = b.dna != 0;
+  const uint32_t k = b.k, l = b.l;
+  const uint64_t lbits = (1ULL << l) - 1ULL;
+  int64_t last_amb = -1;
+  uint64_t last_min = ~0ULL;
+  uint64_t pp0 = 0, pp1 = 0, pp2 = 0, pp3 = 0;
+  for (uint64_t c = 0; c < n; c += kWave) {
+    const uint64_t p = c + lane;
+    const bool inb = p < n;
+    uint32_t code = 0xFFu;
+    if (inb) code = dna ? dna_code((uint8_t) m[p]) : prot_code((uint8_t) m[p]);
+    const bool amb = inb && code == 0xFFu;
+    const bool good = inb && !amb;
+    const uint64_t P0 = __ballot(good && (code & 1u));
+    const uint64_t P1 = __ballot(good && (code & 2u));
+    const uint64_t P2 = dna ? 0 : __ballot(good && (code & 4u));
+    const uint64_t P3 = dna ? 0 : __ballot(good && (code & 8u));
+    const uint64_t A = __ballot(amb);
+    const uint64_t mA = A & lanemask_le(lane);
+    const int64_t lastN = mA ? (int64_t) (c + top_lane(mA)) : last_amb;
+    const bool lvalid = inb && p + 1 >= l && lastN < (int64_t) (p + 1 - l);
+    uint64_t cand = kNoCand;
+    if (lvalid) {
+      const uint32_t s = (uint32_t) lane + 65u - l;
+      const uint64_t r0 = __builtin_bitreverse64(window128(pp0, P0, s) & lbits) >> (64 - l);
+      const uint64_t r1 = __builtin_bitreverse64(window128(pp1, P1, s) & lbits) >> (64 - l);
+      if (dna) {
+        cand = canonical_lmer((spread2(r1) << 1) | spread2(r0), l, b.revcom_version);
+      } else {
+        const uint64_t r2 = __builtin_bitreverse64(window128(pp2, P2, s) & lbits) >> (64 - l);
+        const uint64_t r3 = __builtin_bitreverse64(window128(pp3, P3, s) & lbits) >> (64 - l);
+        cand = (spread4(r3) << 3) | (spread4(r2) << 2) | (spread4(r1) << 1) | spread4(r0);
+      }
+      if (b.spaced_mask) cand &= b.spaced_mask;
+      cand ^= b.toggle_mask;
+    }
+    ring[p & 127] = cand;
+    __syncthreads();
+    const bool kvalid = inb && p >= first_e;
+    bool nonamb = false;
+    uint64_t mn = ~0ULL;
+    if (kvalid) {
+      nonamb = (k == l) ? lvalid : (lastN < (int64_t) (p + 2) - (int64_t) k);
+      if (nonamb) {
+        for (uint32_t j = 0; j <= k - l && j <= p; j++) {
+          const uint64_t v = ring[(p - j) & 127];
+          mn = v < mn ? v : mn;
+        }
+        nonamb = mn != kNoCand;
+        mn ^= b.toggle_mask;
+      }
+    }
+    __syncthreads();
+    const uint64_t NA = __ballot(nonamb);
+    const uint64_t prevm = NA & lanemask_lt(lane);
+    const uint64_t before = shfl64(mn, prevm ? top_lane(prevm) : lane);
+    const uint64_t prev_min = prevm ? before : last_min;
+    // consecutive repeats insert the same (minimizer, taxid) pair again,
+    // which leaves the cell unchanged; insert each run once
+    if (nonamb && mn != prev_min) {
+      if (!(b.min_clear && fmix64(mn) < b.min_clear)) table_insert(b, mn, taxid);
+    }
+    if (A) last_amb = (int64_t) (c + top_lane(A));
+    if (NA) last_min = shfl64(mn, top_lane(NA));
+    pp0 = P0;
+    pp1 = P1;
+    pp2 = P2;
+    pp3 = P3;
+  }
+}
+
+__global__ __launch_bounds__(64) void k_build(BuildView b, const char *seq, const SubBlock *items,
+                                               uint64_t n_items) {
+  __shared__ uint64_t ring[128];
+  for (uint64_t i = blockIdx.x; i < n_items; i += gridDim.x) {
+    const SubBlock it = items[i];
+    build_window(b, seq + it.start, it.len, it.first_e, it.taxid, ring, threadIdx.x);
+    __syncthreads();
+  }
+}
+
+}  // namespace k2g

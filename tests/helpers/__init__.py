@@ -140,7 +140,7 @@ class Oracle:
         n = (off.shape[0] - 1) // 2
         res = {k: np.zeros(n, np.uint32) for k in ("call", "groups", "total", "lookups")}
         res["lines"] = np.zeros(n, np.uint64)
-        cap = 16 * int(off[-1]) + 64 * n + 64
+        cap = 6 * int(off[-1]) + 64 * n + 64
         text = C.create_string_buffer(cap)
         sbuf = np.frombuffer(seq if len(seq) else b"\0", dtype=np.uint8)
         used = self.lib.k2cpu_classify(h, int(paired), int(quick), confidence, min_hit_groups,
