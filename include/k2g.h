@@ -164,6 +164,16 @@ int k2g_build_download(k2g_builder *b, uint32_t *cells, uint64_t capacity);
 int k2g_build_write(k2g_builder *b, const char *hash_k2d);
 int k2g_build_destroy(k2g_builder *b);
 
+/* estimate_capacity (src/estimate_capacity.cc:66-100, 185-203): exact count
+ * of the distinct minimizers whose MurmurHash3 falls in the first n_ranges
+ * of 1024 hash ranges, +1, scaled by 1024 / n_ranges.  Whole sequences are
+ * scanned (block and subblock sizes are ignored). */
+int k2g_estimate_capacity(const k2g_build_params *p, const char *seq, const uint64_t *off,
+                          uint64_t n, uint32_t n_ranges, uint64_t *estimate);
+int k2g_estimate_capacity_device(const k2g_build_params *p, const char *d_seq,
+                                 const uint64_t *off, uint64_t n, uint32_t n_ranges,
+                                 uint64_t *estimate);
+
 #ifdef __cplusplus
 }
 #endif

@@ -41,6 +41,8 @@ EXPORTED_SYMBOLS = (
     "k2g_build_download",
     "k2g_build_write",
     "k2g_build_destroy",
+    "k2g_estimate_capacity",
+    "k2g_estimate_capacity_device",
 )
 
 

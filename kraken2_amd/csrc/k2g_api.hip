@@ -251,15 +251,21 @@ extern "C" int k2g_db_open(const char *hash_k2d, const char *taxo_k2d, const cha
     e = hipStreamCreate(&s);
     if (e == hipSuccess) e = hipHostMalloc((void **) &stage[0], chunk, hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void **) &stage[1], chunk, hipHostMallocDefault);
-    hipEvent_t done[2];
-    hipEventCreate(&done[0]);
-    hipEventCreate(&done[1]);
+    hipEvent_t done[2] = {nullptr, nullptr};
+    if (e == hipSuccess) e = hipEventCreate(&done[0]);
+    if (e == hipSuccess) e = hipEventCreate(&done[1]);
+    // record both once so every later wait is on an event that was recorded
+    if (e == hipSuccess) e = hipEventRecord(done[0], s);
+    if (e == hipSuccess) e = hipEventRecord(done[1], s);
     const uint64_t bytes = capacity * 4;
     bool ok = e == hipSuccess;
     int which = 0;
     for (uint64_t pos = 0; ok && pos < bytes; pos += chunk, which ^= 1) {
       const size_t n = (size_t) std::min<uint64_t>(chunk, bytes - pos);
-      hipEventSynchronize(done[which]);
+      if (hipEventSynchronize(done[which]) != hipSuccess) {
+        ok = false;
+        break;
+      }
       if (!read_fully(fd, stage[which], n, (off_t) (32 + pos))) {
         ok = false;
         break;
@@ -269,12 +275,12 @@ extern "C" int k2g_db_open(const char *hash_k2d, const char *taxo_k2d, const cha
           hipEventRecord(done[which], s) != hipSuccess)
         ok = false;
     }
-    hipStreamSynchronize(s);
-    hipEventDestroy(done[0]);
-    hipEventDestroy(done[1]);
-    if (stage[0]) hipHostFree(stage[0]);
-    if (stage[1]) hipHostFree(stage[1]);
-    hipStreamDestroy(s);
+    (void) hipStreamSynchronize(s);
+    if (done[0]) (void) hipEventDestroy(done[0]);
+    if (done[1]) (void) hipEventDestroy(done[1]);
+    if (stage[0]) (void) hipHostFree(stage[0]);
+    if (stage[1]) (void) hipHostFree(stage[1]);
+    (void) hipStreamDestroy(s);
     close(fd);
     if (!ok) {
       k2g_db_close(db);
@@ -992,4 +998,115 @@ extern "C" int k2g_build_write(k2g_builder *b, const char *hash_k2d) {
   if (fclose(f) != 0) ok = false;
   if (!ok) return fail(K2G_EIO, std::string("error writing ") + hash_k2d);
   return K2G_OK;
+}
+
+// ----------------------------------------------------------------------
+// capacity estimate (src/estimate_capacity.cc:66-100, 185-203): the exact
+// number of distinct minimizers among those whose hash falls in the first
+// n_ranges of 1024 ranges, scaled by 1024 / n_ranges.
+extern "C" int k2g_estimate_capacity_device(const k2g_build_params *p, const char *d_seq,
+                                            const uint64_t *off, uint64_t n, uint32_t n_ranges,
+                                            uint64_t *estimate) {
+  if (!p || !estimate || (n && (!d_seq || !off))) return fail(K2G_EUSAGE, "null argument");
+  if (n_ranges < 1 || n_ranges > 1024) return fail(K2G_EUSAGE, "n must be in [1, 1024]");
+  const uint32_t bits = p->dna_db ? 2 : 4;
+  if (p->l == 0 || p->l > 63 / bits || p->k < p->l || p->k - p->l > 63)
+    return fail(K2G_EUSAGE, "unsupported k/l for the GPU scan");
+  // whole-sequence scans cut into windows overlapping by k-1; a window that
+  // continues a sequence reports from its (k-1)-th base on, like one long scan
+  const uint64_t W = 1u << 16;
+  std::vector<SubBlock> items;
+  uint64_t bases = 0;
+  for (uint64_t s = 0; s < n; s++) {
+    if (off[s + 1] < off[s]) return fail(K2G_EUSAGE, "sequence offsets must not decrease");
+    const uint64_t L = off[s + 1] - off[s];
+    bases += L;
+    for (uint64_t i = 0; i < L; i += W) {
+      SubBlock it;
+      it.start = off[s] + i;
+      it.len = (uint32_t) std::min<uint64_t>(W + p->k - 1, L - i);
+      it.taxid = 0;
+      it.first_e = (uint32_t) (i == 0 ? p->k - 1 : (p->k - 1));
+      it.pad = 0;
+      if (i > 0) {
+        // resume: the window starts k-1 bases back so its first report is
+        // the k-mer ending at position i of the sequence
+        it.start -= p->k - 1;
+        it.len = (uint32_t) std::min<uint64_t>(W + p->k - 1, L - i + p->k - 1);
+      }
+      items.push_back(it);
+    }
+  }
+  const uint64_t cap = bases * n_ranges / 1024 + 1024 * 1024;
+  DevBuf<SubBlock> d_items;
+  DevBuf<uint64_t> sample, sorted;
+  DevBuf<unsigned long long> cnt;
+  DevBuf<unsigned char> temp;
+  HIP_TRY(d_items.reserve(std::max<size_t>(items.size(), 1)));
+  HIP_TRY(sample.reserve(cap));
+  HIP_TRY(sorted.reserve(cap));
+  HIP_TRY(cnt.reserve(2));
+  HIP_TRY(hipMemset(cnt.p, 0, 16));
+  if (!items.empty())
+    HIP_TRY(hipMemcpy(d_items.p, items.data(), items.size() * sizeof(SubBlock), hipMemcpyHostToDevice));
+  BuildView v{};
+  v.k = (uint32_t) p->k;
+  v.l = (uint32_t) p->l;
+  v.dna = p->dna_db ? 1 : 0;
+  v.revcom_version = (uint32_t) p->revcom_version;
+  v.spaced_mask = p->spaced_seed_mask;
+  v.toggle_mask = p->toggle_mask & ((1ULL << (p->l * bits)) - 1);
+  v.sample = sample.p;
+  v.n_sample = cnt.p;
+  v.sample_cap = cap;
+  v.sample_ranges = n_ranges;
+  int dev = 0, cus = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  if (!items.empty()) {
+    const uint64_t grid = std::min<uint64_t>(items.size(), (uint64_t) std::max(1, cus) * 32);
+    hipLaunchKernelGGL(k_build, dim3((unsigned) grid), dim3(64), 0, 0, v, d_seq, d_items.p,
+                       (uint64_t) items.size());
+    HIP_TRY(hipGetLastError());
+  }
+  unsigned long long m = 0;
+  HIP_TRY(hipMemcpy(&m, cnt.p, 8, hipMemcpyDeviceToHost));
+  if (m > cap) return fail(K2G_ESOFTWARE, "capacity sample overflow");
+  uint64_t distinct = 0;
+  if (m) {
+    size_t tb = 0;
+    HIP_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, sample.p, sorted.p, (int) m));
+    HIP_TRY(temp.reserve(tb));
+    HIP_TRY(hipcub::DeviceRadixSort::SortKeys(temp.p, tb, sample.p, sorted.p, (int) m));
+    size_t tb2 = 0;
+    HIP_TRY(hipcub::DeviceSelect::Unique(nullptr, tb2, sorted.p, sample.p, cnt.p + 1, (int) m));
+    HIP_TRY(temp.reserve(tb2));
+    HIP_TRY(hipcub::DeviceSelect::Unique(temp.p, tb2, sorted.p, sample.p, cnt.p + 1, (int) m));
+    unsigned long long u = 0;
+    HIP_TRY(hipMemcpy(&u, cnt.p + 1, 8, hipMemcpyDeviceToHost));
+    distinct = u;
+  }
+  d_items.release();
+  sample.release();
+  sorted.release();
+  cnt.release();
+  temp.release();
+  const uint64_t sum = distinct + 1;
+  *estimate = (uint64_t) (sum * 1024 * 1.0 / n_ranges);
+  return K2G_OK;
+}
+
+extern "C" int k2g_estimate_capacity(const k2g_build_params *p, const char *seq,
+                                     const uint64_t *off, uint64_t n, uint32_t n_ranges,
+                                     uint64_t *estimate) {
+  if (!p || !estimate || (n && (!seq || !off))) return fail(K2G_EUSAGE, "null argument");
+  const uint64_t base = n ? off[0] : 0, bytes = n ? off[n] - off[0] : 0;
+  DevBuf<char> d;
+  HIP_TRY(d.reserve(bytes + 1));
+  if (bytes) HIP_TRY(hipMemcpy(d.p, seq + base, bytes, hipMemcpyHostToDevice));
+  std::vector<uint64_t> rel(n + 1);
+  for (uint64_t i = 0; i <= n; i++) rel[i] = off[i] - base;
+  int rc = k2g_estimate_capacity_device(p, d.p, rel.data(), n, n_ranges, estimate);
+  d.release();
+  return rc;
 }

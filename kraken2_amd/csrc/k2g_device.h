@@ -103,6 +103,20 @@ __device__ __forceinline__ uint32_t lca(const uint32_t *parent, uint32_t a, uint
   return a;
 }
 
+// Wave-uniform values must live in scalar registers: a loop or branch whose
+// condition the compiler sees as per-lane (e.g. a shuffled value) is
+// structurized for divergent lanes, and around a lane-0-only side effect
+// that structure can leave lanes looping without lane 0.  readfirstlane makes
+// the uniformity explicit.
+__device__ __forceinline__ uint32_t uniform32(uint32_t v) {
+  return (uint32_t) __builtin_amdgcn_readfirstlane((int) v);
+}
+__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
+  const uint32_t lo = (uint32_t) __builtin_amdgcn_readfirstlane((int) (uint32_t) v);
+  const uint32_t hi = (uint32_t) __builtin_amdgcn_readfirstlane((int) (uint32_t) (v >> 32));
+  return ((uint64_t) hi << 32) | lo;
+}
+
 __device__ __forceinline__ uint64_t lanemask_lt(int lane) { return (1ULL << lane) - 1ULL; }
 __device__ __forceinline__ uint64_t lanemask_le(int lane) { return ~0ULL >> (63 - lane); }
 __device__ __forceinline__ int top_lane(uint64_t m) { return 63 - __clzll((long long) m); }

@@ -58,6 +58,13 @@ struct BuildView {
   uint64_t spaced_mask, toggle_mask, min_clear;
   unsigned long long *size;  // cells filled
   unsigned int *error;       // 1: capacity exceeded
+  // capacity estimation mode (cells == null): minimizers whose fmix64 has
+  // its low 10 bits below sample_ranges are appended to sample[]
+  // (src/estimate_capacity.cc:185-203)
+  uint64_t *sample;
+  unsigned long long *n_sample;
+  uint64_t sample_cap;
+  uint32_t sample_ranges;
 };
 
 // One scanner window of the deterministic build: sequence bytes

@@ -192,13 +192,17 @@ __device__ __forceinline__ void close_run(const FragOut &out, uint64_t base, uin
 template <class Table>
 __device__ void push_marker(const FragOut &out, uint64_t base, uint64_t cap, RunState &st,
                             uint32_t code, Table &tab, int *bad, int lane) {
+  st.elems++;
+  if (st.run_count > 0 && st.run_code == code) {  // adjacent markers (empty frames) share a run
+    st.run_count++;
+    return;
+  }
   if (st.run_count > 0) {
     if (lane == 0) close_run(out, base, cap, st.n_runs, st.run_code, st.run_count, tab, bad);
     st.n_runs++;
   }
   st.run_code = code;
   st.run_count = 1;
-  st.elems++;
 }
 
 // Scans one stream (a DNA mate, or one reading frame of a mate) and appends
@@ -349,7 +353,7 @@ __device__ void scan_stream(const DbView &db, const ClassifyParams &P, const Fra
       const int hi = top_lane(E);
       if (B) {
         const int lb = top_lane(B);
-        st.run_code = shfl32(code_i, lb);
+        st.run_code = uniform32(shfl32(code_i, lb));
         st.run_count = (uint32_t) (hi - lb + 1);
       } else {
         st.run_count += (uint32_t) __popcll(E);
@@ -358,8 +362,8 @@ __device__ void scan_stream(const DbView &db, const ClassifyParams &P, const Fra
     }
 
     if (A) last_amb = (int64_t) (c + top_lane(A));
-    if (NA) last_min = shfl64(mn, top_lane(NA));
-    if (LK) last_taxon = shfl32(taxon, top_lane(LK));
+    if (NA) last_min = uniform64(shfl64(mn, top_lane(NA)));
+    if (LK) last_taxon = uniform32(shfl32(taxon, top_lane(LK)));
     pp0 = P0;
     pp1 = P1;
     pp2 = P2;
@@ -395,12 +399,13 @@ template <class Table>
 __device__ void classify_fragment(const DbView &db, const ClassifyParams &P, const FragBatch &in,
                                   const FragOut &out, uint64_t f, Table &tab, uint64_t *ring,
                                   int *bad, int lane) {
-  const uint64_t o0 = in.off[2 * f], o1 = in.off[2 * f + 1], o2 = in.off[2 * f + 2];
-  const bool paired = in.paired ? in.paired[f] != 0 : in.paired_all != 0;
+  const uint64_t o0 = uniform64(in.off[2 * f]), o1 = uniform64(in.off[2 * f + 1]),
+                 o2 = uniform64(in.off[2 * f + 2]);
+  const bool paired = uniform32(in.paired ? in.paired[f] : in.paired_all) != 0;
   const bool tx = db.dna == 0;
   const int nframes = tx ? 6 : 1;
-  const uint64_t base = out.run_off[f];
-  const uint64_t cap = out.run_off[f + 1] - base;
+  const uint64_t base = uniform64(out.run_off[f]);
+  const uint64_t cap = uniform64(out.run_off[f + 1]) - base;
   tab.clear(lane);
   if (lane == 0) *bad = 0;
   __syncthreads();
@@ -443,17 +448,17 @@ __device__ void classify_fragment(const DbView &db, const ClassifyParams &P, con
     }
   }
   reduce_best(db.parent, best, best_t);
-  uint32_t call = best_t;
-  uint32_t score = call ? tab.get(call) : 0;
+  uint32_t call = uniform32(best_t);
+  uint32_t score = uniform32(call ? tab.get(call) : 0);
   while (call && score < required) {
     uint32_t part = 0;
     for (uint32_t i = lane; i < ns; i += kWave) {
       const uint32_t t = tab.slot_key(i);
       if (t != 0 && is_ancestor(db.parent, call, t)) part += tab.slot_cnt(i);
     }
-    score = reduce_sum(part);
+    score = uniform32(reduce_sum(part));
     if (score >= required) break;
-    call = db.parent[call];
+    call = uniform32(db.parent[call]);
   }
   if (call && (int64_t) st.groups < (int64_t) P.min_hit_groups) call = 0;
   __syncthreads();
@@ -494,11 +499,12 @@ __global__ __launch_bounds__(64) void k_classify_persistent(DbView db, ClassifyP
   LdsTable tab{lds.tkey, lds.tcnt, P.tab_cap};
   const int lane = threadIdx.x;
   for (;;) {
-    unsigned long long i = 0;
-    if (lane == 0) i = atomicAdd(cursor, 1ULL);
-    i = __shfl(i, 0);
+    // every lane issues the atomic (lane 0 adds 1, the others 0) so no
+    // lane-0 branch precedes the loop exit; the index is then a scalar
+    const unsigned long long got = atomicAdd(cursor, lane == 0 ? 1ULL : 0ULL);
+    const uint64_t i = uniform64((uint64_t) __shfl(got, 0));
     if (i >= in.n) break;
-    const uint64_t f = in.order ? in.order[i] : i;
+    const uint64_t f = uniform64(in.order ? (uint64_t) in.order[i] : i);
     classify_fragment(db, P, in, out, f, tab, lds.ring, &lds.bad, lane);
   }
 }
@@ -514,7 +520,7 @@ __global__ __launch_bounds__(64) void k_classify_dense(DbView db, ClassifyParams
   DenseTable tab{scratch.counts + (uint64_t) slot * db.node_count,
                  scratch.lists + (uint64_t) slot * db.node_count, scratch.lens + slot};
   for (uint32_t i = blockIdx.x; i < n_redo; i += gridDim.x)
-    classify_fragment(db, P, in, out, redo[i], tab, lds.ring, &lds.bad, lane);
+    classify_fragment(db, P, in, out, uniform32(redo[i]), tab, lds.ring, &lds.bad, lane);
 }
 
 // KeyValueStore::GetBatch on the device (one lane per key).
@@ -666,10 +672,17 @@ __device__ void build_window(const BuildView &b, const char *m, uint64_t n, uint
     // consecutive repeats insert the same (minimizer, taxid) pair again,
     // which leaves the cell unchanged; insert each run once
     if (nonamb && mn != prev_min) {
-      if (!(b.min_clear && fmix64(mn) < b.min_clear)) table_insert(b, mn, taxid);
+      if (b.sample) {
+        if ((fmix64(mn) & 1023u) < b.sample_ranges) {
+          const unsigned long long at = atomicAdd(b.n_sample, 1ULL);
+          if (at < b.sample_cap) b.sample[at] = mn;
+        }
+      } else if (!(b.min_clear && fmix64(mn) < b.min_clear)) {
+        table_insert(b, mn, taxid);
+      }
     }
     if (A) last_amb = (int64_t) (c + top_lane(A));
-    if (NA) last_min = shfl64(mn, top_lane(NA));
+    if (NA) last_min = uniform64(shfl64(mn, top_lane(NA)));
     pp0 = P0;
     pp1 = P1;
     pp2 = P2;
@@ -682,7 +695,8 @@ __global__ __launch_bounds__(64) void k_build(BuildView b, const char *seq, cons
   __shared__ uint64_t ring[128];
   for (uint64_t i = blockIdx.x; i < n_items; i += gridDim.x) {
     const SubBlock it = items[i];
-    build_window(b, seq + it.start, it.len, it.first_e, it.taxid, ring, threadIdx.x);
+    build_window(b, seq + uniform64(it.start), uniform32(it.len), uniform32(it.first_e),
+                 uniform32(it.taxid), ring, threadIdx.x);
     __syncthreads();
   }
 }

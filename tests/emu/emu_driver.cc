@@ -1,0 +1,82 @@
+// Host driver of the SIMT emulation: runs the classify kernels of
+// kraken2_amd/csrc/k2g_kernels.hip block by block with 64 host threads.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "../../kraken2_amd/csrc/k2g_kernels.hip"
+
+namespace emu {
+thread_local Dim3 threadIdx_, blockIdx_;
+Dim3 blockDim_, gridDim_;
+std::barrier<> *bar = nullptr;
+unsigned long long vote_slot[64];
+uint64_t shfl_slot[64];
+std::atomic<unsigned long long> collectives{0};
+}  // namespace emu
+
+template <class F>
+static void run_grid(unsigned grid, F body) {
+  std::barrier<> b(64);
+  emu::bar = &b;
+  emu::gridDim_.x = grid;
+  emu::blockDim_.x = 64;
+  for (unsigned blk = 0; blk < grid; blk++) {
+    std::vector<std::thread> ts;
+    for (unsigned t = 0; t < 64; t++)
+      ts.emplace_back([&, t, blk] {
+        emu::threadIdx_.x = t;
+        emu::blockIdx_.x = blk;
+        body();
+      });
+    for (auto &t : ts) t.join();
+  }
+}
+
+extern "C" {
+
+// Classifies n fragments with the device code.  cells padded to a multiple
+// of 64; parent has node_count entries.  mode 0 plain, 1 persistent with the
+// given grid.  run_off (n+1) and runs/recs buffers prepared by the caller.
+int emu_classify(const uint32_t *cells, uint64_t capacity, uint32_t kb, uint32_t vb, const uint32_t *parent,
+                 uint32_t node_count, uint32_t k, uint32_t l, uint32_t dna, uint64_t spaced, uint64_t toggle,
+                 uint64_t min_hash, double conf, int32_t min_groups, uint32_t quick, uint32_t tab_cap,
+                 const char *seq, const uint64_t *off, const uint8_t *paired, uint32_t paired_all, uint64_t n,
+                 const uint32_t *order, int mode, unsigned grid, const uint64_t *run_off, uint32_t *call,
+                 uint32_t *groups, uint32_t *total, uint32_t *n_runs, uint32_t *status, uint32_t *n_recs,
+                 uint32_t *runs, uint32_t *recs, unsigned long long *kmer_counts) {
+  using namespace k2g;
+  DbView v{};
+  v.cells = cells;
+  v.capacity = capacity;
+  v.barrett = capacity > 1 ? (uint64_t) (((unsigned __int128) 1 << 64) / capacity) : 0;
+  v.key_bits = kb;
+  v.value_bits = vb;
+  v.parent = parent;
+  v.node_count = node_count;
+  v.k = k;
+  v.l = l;
+  v.dna = dna;
+  v.revcom_version = 1;
+  const uint32_t bits = dna ? 2 : 4;
+  v.lmer_mask = (1ULL << (l * bits)) - 1;
+  v.spaced_mask = spaced;
+  v.toggle_mask = toggle & v.lmer_mask;
+  v.min_hash = min_hash;
+  ClassifyParams P{conf, min_groups, quick, tab_cap};
+  FragBatch in{seq, off, paired, paired_all, n, order};
+  FragOut out{call, groups, total, n_runs, status, n_recs, run_off, runs, recs, kmer_counts};
+  if (mode == 0) {
+    run_grid((unsigned) n, [&] { k_classify_plain(v, P, in, out); });
+  } else {
+    unsigned long long cursor = 0;
+    run_grid(grid, [&] { k_classify_persistent(v, P, in, out, &cursor); });
+  }
+  return 0;
+}
+
+unsigned long long emu_collectives() { return emu::collectives.load(); }
+
+}  // extern "C"

@@ -1,0 +1,93 @@
+"""The classify kernels' device code, executed on the CPU by the SIMT
+emulation (tests/emu: 64 host threads per wave, barrier-synchronised
+ballots and shuffles), against the CPU oracle.  Covers the kernel logic
+without a GPU; the same code runs on the MI355X in tests/test_gpu_*.py."""
+import os
+import random
+import subprocess
+
+import numpy as np
+import pytest
+
+from helpers import Oracle, edge_reads, read_fasta
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+G = os.path.join(HERE, "golden")
+
+
+@pytest.fixture(scope="module")
+def emu_lib():
+    r = subprocess.run(["make", "-C", os.path.join(HERE, "emu")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    import sys
+    sys.path.insert(0, os.path.join(HERE, "emu"))
+    import emu
+    return emu, emu.load()
+
+
+def _check(emu, lib, d, seq, off, pflags, *, conf=0.0, g=0, quick=False, tab=256, mode=1, hll=False):
+    import kraken2_amd as k2
+    db = emu.read_db(d)
+    out = emu.classify(lib, db, seq, off, pflags, conf=conf, min_groups=g, quick=quick, tab_cap=tab, mode=mode,
+                       hll=hll, kmer_counts=hll)
+    orc = Oracle()
+    h = orc.open(d)
+    n = len(pflags)
+    paired_all = bool(pflags.all())
+    assert pflags.all() or not pflags.any()
+    ref = orc.classify(h, seq, off, paired=paired_all, quick=quick, confidence=conf, min_hit_groups=g)
+    for i in range(n):
+        ro = int(out["run_off"][i])
+        runs = out["runs"][2 * ro:2 * (ro + int(out["n_runs"][i]))].reshape(-1, 2)
+        got = (int(out["call"][i]), int(out["groups"][i]))
+        exp = (int(ref["call"][i]), int(ref["groups"][i]))
+        assert got == exp, i
+        if not quick:
+            assert int(out["total"][i]) == int(ref["total"][i])
+            assert k2.format_hitlist(runs, db["ext"]) == ref["text"][i], i
+    return out
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_emulated_single_end(emu_lib, synth_db, mode):
+    import kraken2_amd as k2
+    emu, lib = emu_lib
+    _, genomes = read_fasta(os.path.join(synth_db, "library.fna"))
+    reads = edge_reads(genomes, 60, seed=31 + mode)
+    seq, off = k2.pack_fragments(reads)
+    _check(emu, lib, synth_db, seq, off, np.zeros(len(reads), np.uint8), g=2, mode=mode)
+
+
+def test_emulated_paired_confidence(emu_lib, synth_db):
+    import kraken2_amd as k2
+    emu, lib = emu_lib
+    _, genomes = read_fasta(os.path.join(synth_db, "library.fna"))
+    m1 = edge_reads(genomes, 40, seed=41)
+    m2 = edge_reads(genomes, 40, seed=42)
+    seq, off = k2.pack_fragments(m1, m2)
+    _check(emu, lib, synth_db, seq, off, np.ones(40, np.uint8), conf=0.3, g=2)
+
+
+def test_emulated_quick_and_dense_table(emu_lib, synth_db):
+    import kraken2_amd as k2
+    emu, lib = emu_lib
+    _, genomes = read_fasta(os.path.join(synth_db, "library.fna"))
+    reads = edge_reads(genomes, 40, seed=51, min_len=100, max_len=300)
+    seq, off = k2.pack_fragments(reads)
+    _check(emu, lib, synth_db, seq, off, np.zeros(40, np.uint8), g=2, quick=True)
+    out = _check(emu, lib, synth_db, seq, off, np.zeros(40, np.uint8), g=0, tab=2)
+    # a 2-slot table overflows for reads with 3+ distinct taxa: flagged for the dense pass
+    assert (out["status"] & 1).any()
+
+
+def test_emulated_translated(emu_lib):
+    import kraken2_amd as k2
+    emu, lib = emu_lib
+    d = os.path.join(G, "prot")
+    names, seqs = [], []
+    for line in open(os.path.join(d, "reads_1.fa")):
+        if not line.startswith(">"):
+            seqs.append(line.strip())
+    seqs = seqs[:40]
+    seq, off = k2.pack_fragments(seqs)
+    _check(emu, lib, d, seq, off, np.zeros(len(seqs), np.uint8), g=0)
