@@ -499,10 +499,11 @@ __global__ __launch_bounds__(64) void k_classify_persistent(DbView db, ClassifyP
   LdsTable tab{lds.tkey, lds.tcnt, P.tab_cap};
   const int lane = threadIdx.x;
   for (;;) {
-    // every lane issues the atomic (lane 0 adds 1, the others 0) so no
-    // lane-0 branch precedes the loop exit; the index is then a scalar
-    const unsigned long long got = atomicAdd(cursor, lane == 0 ? 1ULL : 0ULL);
-    const uint64_t i = uniform64((uint64_t) __shfl(got, 0));
+    // every lane adds 1, so each fetch advances the cursor by exactly 64 and
+    // any lane's result / 64 is the wave's fragment; no lane-0 branch precedes
+    // the loop exit and the index is a scalar
+    const unsigned long long got = atomicAdd(cursor, 1ULL);
+    const uint64_t i = uniform64((uint64_t) got) >> 6;
     if (i >= in.n) break;
     const uint64_t f = uniform64(in.order ? (uint64_t) in.order[i] : i);
     classify_fragment(db, P, in, out, f, tab, lds.ring, &lds.bad, lane);
