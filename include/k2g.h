@@ -76,6 +76,7 @@ typedef struct {
   int32_t want_kmer_counts; /* -R: per-taxon LOOKUP-hit counts */
   int32_t want_hll;         /* -K: per-hit (taxon, encoded HLL hash) records */
   int32_t table_slots;      /* per-fragment LDS hit table (power of 2 in [2,256]; 0 = 256) */
+  int32_t want_probe_stats; /* per-fragment probes and distinct 128-byte table lines touched */
 } k2g_classify_params;
 
 /* Fragment i: mate 1 = seq[off[2i], off[2i+1]), mate 2 = seq[off[2i+1],
@@ -103,6 +104,8 @@ typedef struct {
   const uint64_t *hll_off;     /* n+1, or NULL */
   const uint32_t *hll;         /* (taxon, encoded) pairs, or NULL */
   const uint64_t *kmer_counts; /* or NULL */
+  const uint32_t *lookups;     /* per fragment, with want_probe_stats, else NULL */
+  const uint64_t *probe_lines; /* per fragment, with want_probe_stats, else NULL */
   uint64_t node_count;
   uint64_t redone;             /* fragments that took the dense-table path */
   double kernel_ms;            /* device time of the classify kernels */

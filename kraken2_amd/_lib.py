@@ -80,6 +80,7 @@ class ClassifyParams(C.Structure):
         ("want_kmer_counts", C.c_int32),
         ("want_hll", C.c_int32),
         ("table_slots", C.c_int32),
+        ("want_probe_stats", C.c_int32),
     ]
 
 
@@ -103,6 +104,8 @@ class Results(C.Structure):
         ("hll_off", C.POINTER(C.c_uint64)),
         ("hll", C.POINTER(C.c_uint32)),
         ("kmer_counts", C.POINTER(C.c_uint64)),
+        ("lookups", C.POINTER(C.c_uint32)),
+        ("probe_lines", C.POINTER(C.c_uint64)),
         ("node_count", C.c_uint64),
         ("redone", C.c_uint64),
         ("kernel_ms", C.c_double),
@@ -158,6 +161,8 @@ class ClassifyOutput:
     hll_off: np.ndarray | None
     hll: np.ndarray | None    # (r, 2) uint32: taxon, encoded hash
     kmer_counts: np.ndarray | None
+    lookups: np.ndarray | None
+    probe_lines: np.ndarray | None
     redone: int
     kernel_ms: float
 
@@ -239,11 +244,11 @@ class Batch:
 
     def classify(self, seq: bytes, off: np.ndarray, *, paired=False, paired_flags=None,
                  confidence=0.0, min_hit_groups=0, quick=False, mode=0, kmer_counts=False,
-                 hll=False, table_slots=0) -> ClassifyOutput:
+                 hll=False, table_slots=0, probe_stats=False) -> ClassifyOutput:
         off = np.ascontiguousarray(off, dtype=np.uint64)
         n = (off.shape[0] - 1) // 2
         p = ClassifyParams(confidence, min_hit_groups, int(quick), int(paired), mode,
-                           int(kmer_counts), int(hll), table_slots)
+                           int(kmer_counts), int(hll), table_slots, int(probe_stats))
         sbuf = np.frombuffer(seq, dtype=np.uint8) if len(seq) else np.zeros(1, np.uint8)
         flags = None
         if paired_flags is not None:
@@ -272,6 +277,8 @@ def _copy_results(res: Results, hll: bool, kmer_counts: bool) -> ClassifyOutput:
     kc = None
     if kmer_counts and res.kmer_counts:
         kc = _arr(res.kmer_counts, res.node_count, np.uint64)
+    lk = _arr(res.lookups, n, np.uint32) if res.lookups else None
+    ln = _arr(res.probe_lines, n, np.uint64) if res.probe_lines else None
     return ClassifyOutput(
         call=_arr(res.call, n, np.uint32),
         hit_groups=_arr(res.hit_groups, n, np.uint32),
@@ -281,6 +288,8 @@ def _copy_results(res: Results, hll: bool, kmer_counts: bool) -> ClassifyOutput:
         hll_off=hoff,
         hll=hrec,
         kmer_counts=kc,
+        lookups=lk,
+        probe_lines=ln,
         redone=int(res.redone),
         kernel_ms=float(res.kernel_ms),
     )

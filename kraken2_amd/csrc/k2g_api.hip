@@ -108,7 +108,10 @@ struct k2g_batch {
   DevBuf<uint32_t> call, groups, total, n_runs, status, n_recs, order, redo;
   DevBuf<uint64_t> run_off, bounds, comp_off, comp_rec_off;
   DevBuf<uint32_t> runs, recs, comp_runs, comp_recs;
-  DevBuf<unsigned long long> kmer_counts, cursor;
+  DevBuf<unsigned long long> kmer_counts, cursor, lines;
+  DevBuf<uint32_t> lookups;
+  std::vector<uint32_t> h_lookups;
+  std::vector<uint64_t> h_lines;
   DevBuf<uint32_t> redo_count;
   DevBuf<unsigned char> temp;
   DevBuf<uint32_t> sort_keys, sort_keys_out, sort_vals;
@@ -505,6 +508,8 @@ extern "C" int k2g_batch_destroy(k2g_batch *b) {
   for (auto *buf : {&b->run_off, &b->bounds, &b->comp_off, &b->comp_rec_off}) buf->release();
   b->kmer_counts.release();
   b->cursor.release();
+  b->lines.release();
+  b->lookups.release();
   b->temp.release();
   if (b->ev0) hipEventDestroy(b->ev0);
   if (b->ev1) hipEventDestroy(b->ev1);
@@ -600,10 +605,16 @@ extern "C" int k2g_classify_device(k2g_batch *b, const k2g_classify_params *p, c
   cp.quick = p->quick ? 1u : 0u;
   cp.tab_cap = p->table_slots ? (uint32_t) p->table_slots : 256u;
   FragBatch in{d_seq, d_off, d_paired, (uint32_t) (p->paired != 0), n, order};
+  if (p->want_probe_stats) {
+    HIP_TRY(b->lookups.reserve(n));
+    HIP_TRY(b->lines.reserve(n));
+  }
   FragOut out{b->call.p,    b->groups.p,  b->total.p, b->n_runs.p,
               b->status.p,  b->n_recs.p,  b->run_off.p, b->runs.p,
               p->want_hll ? b->recs.p : nullptr,
-              p->want_kmer_counts ? b->kmer_counts.p : nullptr};
+              p->want_kmer_counts ? b->kmer_counts.p : nullptr,
+              p->want_probe_stats ? b->lookups.p : nullptr,
+              p->want_probe_stats ? b->lines.p : nullptr};
 
   HIP_TRY(hipEventRecord(b->ev0, s));
   if (p->mode == 0) {
@@ -639,6 +650,8 @@ extern "C" int k2g_classify_device(k2g_batch *b, const k2g_classify_params *p, c
     DenseScratch ds{b->dense_counts.p, b->dense_lists.p, b->dense_lens.p, slots};
     FragOut out2 = out;
     out2.kmer_counts = nullptr;  // already counted by the first pass
+    out2.lookups = nullptr;
+    out2.lines = nullptr;
     hipLaunchKernelGGL(k_classify_dense, dim3(slots), dim3(64), 0, s, v, cp, in, out2, b->redo.p,
                        n_redo, ds);
     HIP_TRY(hipGetLastError());
@@ -714,6 +727,14 @@ extern "C" int k2g_batch_fetch(k2g_batch *b, k2g_results *r) {
       if (rc) return rc;
       r->hll_off = b->h_rec_off.data();
       r->hll = b->h_recs.data();
+    }
+    if (b->params.want_probe_stats) {
+      b->h_lookups.resize(n);
+      b->h_lines.resize(n);
+      HIP_TRY(hipMemcpy(b->h_lookups.data(), b->lookups.p, n * 4, hipMemcpyDeviceToHost));
+      HIP_TRY(hipMemcpy(b->h_lines.data(), b->lines.p, n * 8, hipMemcpyDeviceToHost));
+      r->lookups = b->h_lookups.data();
+      r->probe_lines = b->h_lines.data();
     }
     if (b->params.want_kmer_counts) {
       b->h_kmer_counts.resize(b->db->view.node_count);

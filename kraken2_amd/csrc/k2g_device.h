@@ -84,6 +84,28 @@ __device__ __forceinline__ uint32_t table_get(const DbView &db, uint64_t key) {
   }
 }
 
+// Distinct 128-byte lines of the cell array that table_get(key) reads, from
+// the home cell through the terminating cell (measurement only).
+__device__ __forceinline__ uint32_t table_lines(const DbView &db, uint64_t key) {
+  const uint64_t hc = fmix64(key);
+  const uint32_t ck = (uint32_t) (hc >> (64 - db.key_bits));
+  const uint32_t vb = db.value_bits;
+  const uint32_t vmask = (1u << vb) - 1u;
+  uint64_t idx = mod_capacity(hc, db.capacity, db.barrett);
+  uint64_t left = db.capacity, line = idx >> 5;
+  uint32_t lines = 1;
+  for (;;) {
+    const uint32_t c = db.cells[idx];
+    if ((c & vmask) == 0 || (c >> vb) == ck) return lines;
+    if (--left == 0) return lines;
+    if (++idx == db.capacity) idx = 0;
+    if ((idx >> 5) != line) {
+      line = idx >> 5;
+      lines++;
+    }
+  }
+}
+
 // src/taxonomy.cc:251-258 (ids are BFS ordered: parent < child)
 __device__ __forceinline__ bool is_ancestor(const uint32_t *parent, uint32_t a, uint32_t b) {
   if (a == 0 || b == 0) return false;

@@ -33,6 +33,8 @@ struct FragOut {
   uint32_t *runs;          // (code, count) pairs
   uint32_t *recs;          // (taxon, sparse-encoded HLL hash) pairs, or null
   unsigned long long *kmer_counts;  // per internal taxid, or null
+  uint32_t *lookups;       // per fragment: table probes issued, or null
+  unsigned long long *lines;  // per fragment: distinct 128-byte lines the probes touched
 };
 
 // Per-slot scratch of the overflow path: a dense count array per taxon.

@@ -171,6 +171,9 @@ struct RunState {
   uint32_t n_runs = 0, nrec = 0, groups = 0;
   uint64_t elems = 0;
   bool done = false;
+  // per-lane measurement counters (only with out.lookups)
+  uint32_t my_lookups = 0;
+  uint64_t my_lines = 0;
 };
 
 template <class Table>
@@ -285,7 +288,13 @@ __device__ void scan_stream(const DbView &db, const ClassifyParams &P, const Fra
     const bool lookup = nonamb && mn != prev_min;
     const bool skip = lookup && db.min_hash != 0 && fmix64(mn) < db.min_hash;
     uint32_t taxon = 0;
-    if (lookup && !skip) taxon = table_get(db, mn);
+    if (lookup && !skip) {
+      taxon = table_get(db, mn);
+      if (out.lookups) {
+        st.my_lookups++;
+        st.my_lines += table_lines(db, mn);
+      }
+    }
     const uint64_t LK = __ballot(lookup);
     const uint64_t srcm = LK & lanemask_le(lane);
     const uint32_t srct = shfl32(taxon, srcm ? top_lane(srcm) : lane);
@@ -462,6 +471,17 @@ __device__ void classify_fragment(const DbView &db, const ClassifyParams &P, con
   }
   if (call && (int64_t) st.groups < (int64_t) P.min_hit_groups) call = 0;
   __syncthreads();
+  if (out.lookups) {
+    uint64_t lk = st.my_lookups, ln = st.my_lines;
+    for (int d = 1; d < kWave; d <<= 1) {
+      lk += shfl64(lk, lane ^ d);
+      ln += shfl64(ln, lane ^ d);
+    }
+    if (lane == 0) {
+      out.lookups[f] = (uint32_t) lk;
+      out.lines[f] = ln;
+    }
+  }
   if (lane == 0) {
     out.call[f] = call;
     out.groups[f] = st.groups;
@@ -481,7 +501,7 @@ struct WaveLds {
 };
 
 // Plain path: one wavefront per fragment, one launch per batch.
-__global__ __launch_bounds__(64) void k_classify_plain(DbView db, ClassifyParams P, FragBatch in,
+__global__ __launch_bounds__(64, 8) void k_classify_plain(DbView db, ClassifyParams P, FragBatch in,
                                                         FragOut out) {
   __shared__ WaveLds lds;
   const uint64_t f = blockIdx.x;
@@ -492,21 +512,23 @@ __global__ __launch_bounds__(64) void k_classify_plain(DbView db, ClassifyParams
 
 // Persistent path: resident wavefronts pull fragments from a shared cursor in
 // the given order (longest first), so long and short reads share one launch.
-__global__ __launch_bounds__(64) void k_classify_persistent(DbView db, ClassifyParams P,
+__global__ __launch_bounds__(64, 8) void k_classify_persistent(DbView db, ClassifyParams P,
                                                              FragBatch in, FragOut out,
                                                              unsigned long long *cursor) {
   __shared__ WaveLds lds;
   LdsTable tab{lds.tkey, lds.tcnt, P.tab_cap};
   const int lane = threadIdx.x;
-  for (;;) {
-    // every lane adds 1, so each fetch advances the cursor by exactly 64 and
-    // any lane's result / 64 is the wave's fragment; no lane-0 branch precedes
-    // the loop exit and the index is a scalar
-    const unsigned long long got = atomicAdd(cursor, 1ULL);
-    const uint64_t i = uniform64((uint64_t) got) >> 6;
-    if (i >= in.n) break;
+  // Every lane adds 1, so each fetch advances the cursor by exactly 64 and
+  // any lane's result / 64 is the wave's next fragment; no lane-0 branch
+  // precedes the loop exit and the index is a scalar.  The next fetch is
+  // issued before the current fragment is classified so its latency hides
+  // behind that work.
+  uint64_t i = uniform64((uint64_t) atomicAdd(cursor, 1ULL)) >> 6;
+  while (i < in.n) {
+    const unsigned long long next = atomicAdd(cursor, 1ULL);
     const uint64_t f = uniform64(in.order ? (uint64_t) in.order[i] : i);
     classify_fragment(db, P, in, out, f, tab, lds.ring, &lds.bad, lane);
+    i = uniform64((uint64_t) next) >> 6;
   }
 }
 

@@ -67,7 +67,7 @@ int emu_classify(const uint32_t *cells, uint64_t capacity, uint32_t kb, uint32_t
   v.min_hash = min_hash;
   ClassifyParams P{conf, min_groups, quick, tab_cap};
   FragBatch in{seq, off, paired, paired_all, n, order};
-  FragOut out{call, groups, total, n_runs, status, n_recs, run_off, runs, recs, kmer_counts};
+  FragOut out{call, groups, total, n_runs, status, n_recs, run_off, runs, recs, kmer_counts, nullptr, nullptr};
   if (mode == 0) {
     run_grid((unsigned) n, [&] { k_classify_plain(v, P, in, out); });
   } else {
