@@ -1,19 +1,30 @@
 #!/usr/bin/env python3
 """Benchmark: reads/s of the batched persistent GPU classify against the plain
 one-launch-per-batch GPU classify on the same device-resident reads, both
-checked bit-exact against the CPU oracle, with the reference CPU classify
-timed on this host beside them.
+checked bit-exact (against each other on every read, and against the CPU
+oracle on a sample), with the reference CPU classify timed on this host's
+cores beside them.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c1]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3]
 
-A step is one classify pass over the whole read set already resident in
-HBM.  Prints one JSON line (rank 0, N=1).
+Workloads (all synthetic and seeded):
+  c3  batched uneven-length classify: 4000 genomes (lognormal, median 2.5 Mb)
+      built into a ~2.5e9-cell table on the GPU, 10M reads mixed 40% single
+      150 bp / 30% pairs 2x250 / 30% long (lognormal median 8 kb, cap 50 kb,
+      2.5% substitutions + 2.5% 1-3 bp indels), shuffled.
+  c2  plain classify at real shape: the same table, 50M pairs 2x150 with
+      20% off-target reads.
+  c1  the small CPU-runnable case: 64 genomes x 200 kb, 100k x 150 bp.
+
+A step is one classify pass over the whole read set already resident in HBM.
+Prints one JSON line.
 """
 import argparse
 import ctypes as C
 import json
 import os
 import re
+import shutil
 import subprocess
 import sys
 import tempfile
@@ -26,54 +37,125 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "reads/sec batched vs plain GPU classify (bit-exact) + p50/p99 request latency"
+BLOCK_BASES = 3 * 1024 * 1024       # reference block for single reads (src/classify.cc:527)
+BLOCK_PAIRS = 10000                 # reference block for pairs (src/classify.cc:31, 531)
 
 WORKLOADS = {
-    # single-end 150 bp reads against a 64-genome, 86-node DB (k=35, l=31)
-    "c1": dict(desc="plain classify, 64 genomes x 200 kb, 100k x 150 bp single-end, -T 0 -g 2",
-               levels="4,4,4", genome_len=200000, n_reads=100000, read_len=150, seed=1,
-               confidence=0.0, min_hit_groups=2, paired=False),
+    "c1": dict(kind="host", desc="c1: plain classify, 64 genomes x 200 kb, 100k x 150 bp single-end, -T 0 -g 2",
+               levels="4,4,4", genome_len=200000, n_reads=100000, read_len=150, seed=1, confidence=0.0,
+               min_hit_groups=2),
+    "c3": dict(kind="gpu", desc="c3: batched uneven-length classify, 10M reads (40% 150 bp single, 30% 2x250 "
+               "pairs, 30% long lognormal median 8 kb cap 50 kb with 5% errors) against the c2 table "
+               "(4000 genomes, ~2.5e9 cells), -T 0.1 -g 2",
+               seed=3, n_reads=10_000_000, f_single=0.4, f_paired=0.3, paired_len=250, off_frac=0.0,
+               confidence=0.1, min_hit_groups=2),
+    "c2": dict(kind="gpu", desc="c2: plain classify at real shape, 50M pairs 2x150 (20% off-target GC 0.41) "
+               "against 4000 genomes / ~2.5e9 cells, -T 0.1 -g 2 -P",
+               seed=2, n_reads=50_000_000, f_single=0.0, f_paired=1.0, paired_len=150, off_frac=0.2,
+               confidence=0.1, min_hit_groups=2),
 }
 
 
-class Hip:
-    """Minimal HIP runtime access for device-resident inputs (plumbing only)."""
-
-    def __init__(self):
-        self.lib = C.CDLL("libamdhip64.so")
-        self.lib.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
-        self.lib.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
-        self.lib.hipFree.argtypes = [C.c_void_p]
-        self.lib.hipDeviceSynchronize.argtypes = []
-
-    def upload(self, arr: np.ndarray) -> C.c_void_p:
-        p = C.c_void_p()
-        n = max(1, arr.nbytes)
-        if self.lib.hipMalloc(C.byref(p), n) != 0:
-            raise RuntimeError("hipMalloc failed")
-        if arr.nbytes and self.lib.hipMemcpy(p, arr.ctypes.data, arr.nbytes, 1) != 0:
-            raise RuntimeError("hipMemcpy failed")
-        return p
-
-    def free(self, p):
-        self.lib.hipFree(p)
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
 
 
-def prepare(work: dict, d: str):
+def scratch_dir(prefix):
+    for base in ("/dev/shm", None):
+        try:
+            if base and shutil.disk_usage(base).free < (40 << 30):
+                continue
+            return tempfile.mkdtemp(prefix=prefix, dir=base)
+        except OSError:
+            continue
+    return tempfile.mkdtemp(prefix=prefix)
+
+
+def run_space(off, paired, k, pct=25):
+    """Hitlist space the library reserves per fragment: min(e, e * pct / 100 + 64)
+    of its exact k-mer-element count e (nucleotide databases)."""
+    l1 = (off[1::2][:len(paired)] - off[0::2][:len(paired)]).astype(np.int64)
+    l2 = (off[2::2] - off[1::2][:len(paired)]).astype(np.int64)
+    e = np.maximum(l1 - k + 1, 0) + np.where(paired == 1, np.maximum(l2 - k + 1, 0) + 1, 0)
+    return np.minimum(e, e * pct // 100 + 64).astype(np.int64)
+
+
+def split_chunks(bounds, limit):
+    """Contiguous fragment ranges whose reserved hitlist space stays under limit."""
+    csum = np.cumsum(bounds)
+    chunks, s, base = [], 0, 0
+    n = len(bounds)
+    while s < n:
+        e = int(np.searchsorted(csum, base + limit, side="right"))
+        e = max(e, s + 1)
+        chunks.append((s, e))
+        base = int(csum[e - 1])
+        s = e
+    return chunks
+
+
+def reference_blocks(off, paired):
+    """The reference's input blocks over the fragment stream (src/classify.cc:
+    523-543): unpaired reads until 3 MiB of bases, pairs 10 000 at a time; a
+    block never mixes the two (they come from separate classify runs)."""
+    n = len(paired)
+    lens = (off[2::2] - off[0::2][:n]).astype(np.int64)
+    blocks, s = [], 0
+    while s < n:
+        e = s
+        if paired[s]:
+            while e < n and paired[e] and e - s < BLOCK_PAIRS:
+                e += 1
+        else:
+            bases = 0
+            while e < n and not paired[e] and bases < BLOCK_BASES:
+                bases += int(lens[e])
+                e += 1
+        blocks.append((s, e))
+        s = e
+    return blocks
+
+
+class Runner:
+    """Drives k2g_classify_device over ranges of device-resident fragments on one batch."""
+
+    def __init__(self, k2, db, d_seq, d_off, d_paired, work):
+        self.k2 = k2
+        self.lib = k2.load_library()
+        self.batch = k2.Batch(db)
+        self.d_seq, self.d_off, self.d_paired = d_seq, d_off, d_paired
+        self.work = work
+
+    def run(self, s, e, mode, stats=False):
+        lib = self.lib
+        p = self.k2._lib.ClassifyParams(self.work["confidence"], self.work["min_hit_groups"], 0, 0, mode, 0, 0, 0,
+                                        int(stats), 0)
+        pp = C.c_void_p(self.d_paired.value + s) if self.d_paired is not None else None
+        rc = lib.k2g_classify_device(self.batch._h, C.byref(p), self.d_seq,
+                                     C.c_void_p(self.d_off.value + 16 * s), pp, e - s, None)
+        ms = C.c_double()
+        rc = rc or lib.k2g_batch_sync(self.batch._h, C.byref(ms))
+        if rc:
+            raise self.k2.K2GError(rc, lib.k2g_last_error().decode())
+        return ms.value
+
+    def close(self):
+        self.batch.close()
+
+
+def run_host_workload(a, work, k2):
+    """c1: generated on the host with k2synth, built by the deterministic CPU build."""
     from helpers import K2SYNTH, make_synthetic_db, run
-    make_synthetic_db(d, seed=work["seed"], levels=work["levels"], genome_len=work["genome_len"])
-    reads = os.path.join(d, "reads.fa")
-    run([K2SYNTH, "reads", "--lib", os.path.join(d, "library.fna"), "--out", reads, "--seed",
+    from kraken2_amd.synth import Hip
+    tmp = a.keep or scratch_dir("k2g_c1_")
+    os.makedirs(tmp, exist_ok=True)
+    make_synthetic_db(tmp, seed=work["seed"], levels=work["levels"], genome_len=work["genome_len"])
+    reads = os.path.join(tmp, "reads.fa")
+    run([K2SYNTH, "reads", "--lib", os.path.join(tmp, "library.fna"), "--out", reads, "--seed",
          str(work["seed"]), "--n", str(work["n_reads"]), "--len", str(work["read_len"]), "--sub", "0.01",
          "--random-frac", "0.1"])
-    return reads
-
-
-def load_reads(path):
-    seqs = []
-    with open(path, "rb") as f:
-        for line in f:
-            if not line.startswith(b">"):
-                seqs.append(line.rstrip(b"\n"))
+    seqs = [l.rstrip(b"\n") for l in open(reads, "rb") if not l.startswith(b">")]
     n = len(seqs)
     lens = np.array([len(s) for s in seqs], np.uint64)
     ends = np.cumsum(lens, dtype=np.uint64)
@@ -81,140 +163,271 @@ def load_reads(path):
     off[0:2 * n:2] = np.concatenate([np.zeros(1, np.uint64), ends[:-1]])
     off[1:2 * n:2] = ends
     off[2 * n] = ends[-1]
-    return b"".join(seqs), off, seqs
+    hip = Hip()
+    seq = b"".join(seqs)
+    d_seq = hip.upload(np.frombuffer(seq, np.uint8))
+    d_off = hip.upload(off)
+    paired = np.zeros(n, np.uint8)
+    host = dict(seq=seq, off=off, reads_fa=reads, pairs_fa=None)
+    return tmp, hip, d_seq, d_off, None, off, paired, host
 
 
-def cpu_baseline(work, d, reads, threads):
+def run_gpu_workload(a, work, k2):
+    from kraken2_amd.synth import Workload
+    tmp = a.keep or scratch_dir(f"k2g_{a.workload}_")
+    wl = Workload(tmp, seed=2, n_genomes=max(5, int(4000 * a.scale)), log=log)
+    t0 = time.time()
+    rs = wl.make_reads(max(100, int(work["n_reads"] * a.scale)), f_single=work["f_single"], f_paired=work["f_paired"],
+                       paired_len=work["paired_len"], off_frac=work["off_frac"], seed=work["seed"],
+                       group_pairs=True)
+    log(f"# reads: {rs.n} fragments, {rs.bases / 1e9:.2f} Gbp in {time.time() - t0:.1f}s")
+    wl.free_genomes()
+    return tmp, wl.hip, rs.d_seq, rs.d_off, rs.d_paired, rs.off, rs.paired, dict(readset=rs, build_s=wl.build_s,
+                                                                               capacity=wl.capacity,
+                                                                               estimate=wl.estimate)
+
+
+def write_sample_fasta(path1, path2, frags):
+    """frags: list of (seq1, seq2 or None, name)."""
+    with open(path1, "w") as f1:
+        f2 = open(path2, "w") if path2 else None
+        for s1, s2, name in frags:
+            if s2 is None:
+                f1.write(f">{name}\n{s1}\n")
+            else:
+                f1.write(f">{name}/1\n{s1}\n")
+                f2.write(f">{name}/2\n{s2}\n")
+        if f2:
+            f2.close()
+
+
+def cpu_baseline(work, d, files, threads):
     """The reference classify (compiled from its sources into oracle/_ref) when
-    present, else the oracle restatement, on all host threads."""
+    present, else the oracle restatement, on all host threads.  Single reads
+    and pairs run as two invocations (one run cannot mix them); times add."""
     ref = os.path.join(ROOT, "oracle", "_ref", "classify")
     port = os.path.join(ROOT, "oracle", "bin", "k2cpu_classify")
     exe, kind = (ref, "reference") if os.path.exists(ref) else (port, "port")
-    cmd = [exe, "-H", f"{d}/hash.k2d", "-t", f"{d}/taxo.k2d", "-o", f"{d}/opts.k2d", "-p", str(threads),
-           "-T", str(work["confidence"]), "-g", str(work["min_hit_groups"]), "-O", "/dev/null", reads]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        return None
-    m = re.search(r"(\d+) sequences \(([\d.]+) Mbp\) processed in ([\d.]+)s", r.stderr)
-    n, secs = int(m.group(1)), float(m.group(3))
-    return {"value": n / secs, "unit": "reads/s", "cores": threads, "kind": kind,
-            "sample": f"all {n} reads of the workload, timed interval of the binary itself "
-                      f"(after DB load, through parse, classify and output), {secs:.3f} s"}
+    total_n, total_s, total_bp = 0, 0.0, 0.0
+    for args in files:
+        cmd = [exe, "-H", f"{d}/hash.k2d", "-t", f"{d}/taxo.k2d", "-o", f"{d}/opts.k2d", "-p", str(threads),
+               "-T", str(work["confidence"]), "-g", str(work["min_hit_groups"]), "-O", "/dev/null", *args]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            log(f"# cpu baseline failed: {r.stderr[-300:]}")
+            return None
+        m = re.search(r"(\d+) sequences \(([\d.]+) Mbp\) processed in ([\d.]+)s", r.stderr)
+        total_n += int(m.group(1))
+        total_bp += float(m.group(2))
+        total_s += float(m.group(3))
+    return {"value": round(total_n / total_s, 1), "unit": "reads/s", "cores": threads, "kind": kind,
+            "sample": f"{total_n} fragments ({total_bp:.1f} Mbp) drawn evenly from the workload; the binary's own "
+                      f"timed interval (after DB load, through parse, classify and output), {total_s:.2f} s"}
+
+
+def host_range(hip, d_seq, off, paired, s, e):
+    """(seq bytes, rebased 2m+1 offsets, paired flags) of fragments [s, e)."""
+    a0, a2 = int(off[2 * s]), int(off[2 * e])
+    seq = hip.download(d_seq, a2 - a0, a0)
+    return seq, (off[2 * s:2 * e + 1] - off[2 * s]).astype(np.uint64), paired[s:e]
+
+
+def fragment_text(seq, off, j):
+    a0, a1, a2 = (int(x) for x in off[2 * j:2 * j + 3])
+    return seq[a0:a1], seq[a1:a2]
+
+
+def oracle_check(k2, db, tmp, seq, off, pflags, work):
+    """Classifies the fragments with the library (one batch) and with the CPU
+    oracle; returns (mismatches, library digests)."""
+    from helpers import Oracle
+    n = len(pflags)
+    batch = k2.Batch(db)
+    batch.classify(seq, off, paired_flags=pflags, confidence=work["confidence"],
+                   min_hit_groups=work["min_hit_groups"], mode=1)
+    dig = batch.digests(n)
+    out = batch.classify(seq, off, paired_flags=pflags, confidence=work["confidence"],
+                         min_hit_groups=work["min_hit_groups"], mode=1)
+    batch.close()
+    orc = Oracle()
+    h = orc.open(tmp)
+    mism = 0
+    for pflag in (0, 1):
+        sel = [j for j in range(n) if pflags[j] == pflag]
+        if not sel:
+            continue
+        parts, o = [], np.zeros(2 * len(sel) + 1, np.uint64)
+        pos = 0
+        for jj, j in enumerate(sel):
+            m1, m2 = fragment_text(seq, off, j)
+            parts += [m1, m2]
+            o[2 * jj] = pos
+            o[2 * jj + 1] = pos + len(m1)
+            pos += len(m1) + len(m2)
+        o[-1] = pos
+        ref = orc.classify(h, b"".join(parts), o, paired=bool(pflag), confidence=work["confidence"],
+                           min_hit_groups=work["min_hit_groups"])
+        for jj, j in enumerate(sel):
+            runs = out.runs[out.run_off[j]:out.run_off[j + 1]]
+            got = (int(out.call[j]), int(out.hit_groups[j]), int(out.total_kmers[j]),
+                   k2.format_hitlist(runs, db.external_ids))
+            exp = (int(ref["call"][jj]), int(ref["groups"][jj]), int(ref["total"][jj]), ref["text"][jj])
+            mism += got != exp
+    orc.lib.k2cpu_db_close(h)
+    return mism, dig
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c1", choices=sorted(WORKLOADS))
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--keep", default=None, help="directory for the generated workload")
+    ap.add_argument("--no-verify", action="store_true")
+    ap.add_argument("--keep", default=None)
+    ap.add_argument("--chunk-space", type=float, default=1.2e10,
+                    help="hitlist runs reserved per persistent launch (8 bytes each)")
+    ap.add_argument("--cpu-sample", type=int, default=None, help="fragments timed on the CPU")
+    ap.add_argument("--gather-probes", type=float, default=2 ** 30)
+    ap.add_argument("--scale", type=float, default=1.0,
+                    help="scale genomes and reads down for a quick functional run (not a bench number)")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py measures one GPU: --gpus {a.gpus} is not supported (this build shards "
-              "reads across GPUs only by running one process per GPU)", file=sys.stderr)
+        log(f"bench.py measures one GPU: --gpus {a.gpus} is not supported (reads would shard across GPUs "
+            "as one process per GPU with a replicated table; this build runs on one GPU)")
         sys.exit(2)
-    work = WORKLOADS[a.workload]
     import kraken2_amd as k2
-    from helpers import Oracle
 
-    tmp = a.keep or tempfile.mkdtemp(prefix="k2g_bench_")
-    os.makedirs(tmp, exist_ok=True)
-    t0 = time.time()
-    reads = prepare(work, tmp)
-    seq, off, seqs = load_reads(reads)
-    n = len(seqs)
-    print(f"# workload {a.workload}: {n} reads ready in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
-
+    work = WORKLOADS[a.workload]
+    t_start = time.time()
+    if work["kind"] == "host":
+        tmp, hip, d_seq, d_off, d_paired, off, paired, host = run_host_workload(a, work, k2)
+    else:
+        tmp, hip, d_seq, d_off, d_paired, off, paired, host = run_gpu_workload(a, work, k2)
+    n = len(paired)
+    total_bases = int(off[-1])
     db = k2.Db.from_dir(tmp)
-    lib = k2.load_library()
-    hip = Hip()
-    d_seq = hip.upload(np.frombuffer(seq, np.uint8))
-    d_off = hip.upload(off)
-    batch_p = k2.Batch(db)
-    batch_b = k2.Batch(db)
-    P = k2._lib.ClassifyParams
-    params_b = P(work["confidence"], work["min_hit_groups"], 0, int(work["paired"]), 1, 0, 0, 0)
-    params_p = P(work["confidence"], work["min_hit_groups"], 0, int(work["paired"]), 0, 0, 0, 0)
+    k = int(db.info.k)
+    log(f"# DB resident: {db.info.capacity} cells, {db.info.device_bytes / 1e9:.2f} GB; "
+        f"{n} fragments, {total_bases / 1e9:.2f} Gbp; setup {time.time() - t_start:.1f}s")
 
-    # the reference's blocks: 3 MiB of bases per block for single reads
-    blocks = []
-    start, bases = 0, 0
-    lens = np.diff(off)[0::2]
-    for i in range(n):
-        bases += int(lens[i])
-        if bases >= 3 * 1024 * 1024 or i == n - 1:
-            blocks.append((start, i + 1))
-            start, bases = i + 1, 0
-    off_ptr = d_off.value
+    chunks = split_chunks(run_space(off, paired, k), int(a.chunk_space))
+    blocks = reference_blocks(off, paired)
+    runner = Runner(k2, db, d_seq, d_off, d_paired, work)
 
-    def step_batched():
-        rc = lib.k2g_classify_device(batch_b._h, C.byref(params_b), d_seq, d_off, None, n, None)
-        ms = C.c_double()
-        rc |= lib.k2g_batch_sync(batch_b._h, C.byref(ms))
-        if rc:
-            raise k2.K2GError(rc, lib.k2g_last_error().decode())
-        return ms.value
+    passes = {
+        "plain": (blocks, 0),             # one plain launch per reference input block
+        "plain_one_launch": (chunks, 0),  # the plain kernel over the whole batch at once
+        "batched": (chunks, 1),           # persistent, longest first
+    }
 
-    def step_plain():
-        tot = 0.0
-        for (s, e) in blocks:
-            rc = lib.k2g_classify_device(batch_p._h, C.byref(params_p), d_seq, C.c_void_p(off_ptr + 16 * s),
-                                         None, e - s, None)
-            ms = C.c_double()
-            rc |= lib.k2g_batch_sync(batch_p._h, C.byref(ms))
-            if rc:
-                raise k2.K2GError(rc, lib.k2g_last_error().decode())
-            tot += ms.value
-        return tot
+    def run_pass(name):
+        ranges, mode = passes[name]
+        return sum(runner.run(s, e, mode) for s, e in ranges)
 
-    for _ in range(a.warmup):
-        step_plain()
-        step_batched()
-    hip.lib.hipDeviceSynchronize()
-    t = time.perf_counter()
-    kms_p = [step_plain() for _ in range(a.steps)]
-    hip.lib.hipDeviceSynchronize()
-    plain_s = (time.perf_counter() - t) / a.steps
-    t = time.perf_counter()
-    kms_b = [step_batched() for _ in range(a.steps)]
-    hip.lib.hipDeviceSynchronize()
-    batched_s = (time.perf_counter() - t) / a.steps
-    print(f"# plain {plain_s * 1e3:.2f} ms/step, batched {batched_s * 1e3:.2f} ms/step", file=sys.stderr,
-          flush=True)
+    timing = {}
+    for name in passes:
+        for _ in range(a.warmup):
+            run_pass(name)
+        t = time.perf_counter()
+        kms = []
+        for _ in range(a.steps):
+            kms.append(run_pass(name))
+        wall = (time.perf_counter() - t) / a.steps
+        timing[name] = (wall, float(np.mean(kms)))
+        log(f"# {name}: {wall * 1e3:.1f} ms/step ({len(passes[name][0])} launches), kernel {np.mean(kms):.1f} ms")
+    batched_s, batched_kms = timing["batched"]
+    plain_s, plain_kms = timing["plain"]
 
-    # parity of the last batched pass against the oracle, every read
-    res = k2._lib.Results()
-    rc = lib.k2g_batch_fetch(batch_b._h, C.byref(res))
-    if rc:
-        raise k2.K2GError(rc, lib.k2g_last_error().decode())
-    out = k2._lib._copy_results(res, False, False)
-    orc = Oracle()
-    h = orc.open(tmp)
-    ref = orc.classify(h, seq, off, confidence=work["confidence"], min_hit_groups=work["min_hit_groups"],
-                       lines=True)
-    mismatches = 0
-    for i in range(n):
-        runs = out.runs[out.run_off[i]:out.run_off[i + 1]]
-        if int(out.call[i]) != int(ref["call"][i]) or k2.format_hitlist(runs, db.external_ids) != ref["text"][i]:
-            mismatches += 1
+    # ---- verification and measurement passes (outside the timed region)
+    digests = {}
+    lookups = lines = n_runs_total = 0
+    space_redone = dense_redone = 0
+    for name in passes:
+        if a.no_verify and name != "batched":
+            continue
+        ranges, mode = passes[name]
+        dig = np.zeros(n, np.uint64)
+        for s, e in ranges:
+            stats = name == "batched"
+            runner.run(s, e, mode, stats=stats)
+            dig[s:e] = runner.batch.digests(e - s)
+            if stats:
+                c = runner.batch.fetch_counts(e - s, probe_stats=True)
+                lookups += int(c["lookups"].sum())
+                lines += int(c["probe_lines"].sum())
+                n_runs_total += int(c["n_runs"].astype(np.int64).sum())
+                sr, dr = runner.batch.redo_counts()
+                space_redone += sr
+                dense_redone += dr
+        digests[name] = dig
+        log(f"# digests of {name} done")
+    modes_agree = a.no_verify or (np.array_equal(digests["plain"], digests["batched"]) and
+                                  np.array_equal(digests["plain_one_launch"], digests["batched"]))
 
-    # roofline of the persistent kernel: algorithmic bytes per launch =
-    # input bases + 8 B offsets per read + 128 B per distinct probed line +
-    # outputs (call, groups, total, n_runs, status, n_recs = 24 B per read,
-    # 8 B per hitlist run)
-    lines = int(ref["lines"].sum())
-    lookups = int(ref["lookups"].sum())
-    n_runs = int(out.run_off[-1])
-    alg_bytes = len(seq) + 16 * n + 128 * lines + 24 * n + 8 * n_runs
-    kernel_s = float(np.mean(kms_b)) / 1e3
+    rng = np.random.default_rng(7)
+    sample = np.sort(rng.choice(n, size=min(n, 4000), replace=False))
+    if work["kind"] == "host":
+        seq_all = host["seq"]
+        parts, so = [], np.zeros(2 * len(sample) + 1, np.uint64)
+        pos = 0
+        for j, i in enumerate(sample.tolist()):
+            a0, a2 = int(off[2 * i]), int(off[2 * i + 2])
+            parts.append(seq_all[a0:a2])
+            so[2 * j] = pos
+            so[2 * j + 1] = pos + int(off[2 * i + 1]) - a0
+            pos += a2 - a0
+        so[-1] = pos
+        s_seq = b"".join(parts)
+    else:
+        s_seq, so, _ = host["readset"].host_fragments(sample)
+    mism, s_dig = oracle_check(k2, db, tmp, s_seq, so, paired[sample], work)
+    sample_digest_agree = bool(np.array_equal(s_dig, digests["batched"][sample]))
+    log(f"# verify: modes agree on all {n} fragments: {modes_agree}; oracle sample mismatches: "
+        f"{mism}/{len(sample)}; sample digests equal the bench run's: {sample_digest_agree}")
+
+    # roofline of the persistent classify kernel: algorithmic bytes per launch
+    #   = bases + per fragment (16 B offsets + 1 B pair flag + 4 B order + 16 B
+    #     run offset/space + 24 B scalar outputs) + 128 B per distinct table
+    #     line a probe touches + 8 B per hitlist run
+    alg_bytes = total_bases + 61 * n + 128 * lines + 8 * n_runs_total
+    kernel_s = batched_kms / 1e3
     achieved = alg_bytes / kernel_s / 1e9
+    ceil_rate, ceil_ms = db.gather_ceiling(int(a.gather_probes))
+    probe_rate = lines / kernel_s
+    log(f"# gather ceiling {ceil_rate / 1e9:.2f} G lines/s ({ceil_ms:.1f} ms); classify "
+        f"{probe_rate / 1e9:.2f} G lines/s")
 
-    cpu = None if a.no_cpu else cpu_baseline(work, tmp, reads, os.cpu_count() or 1)
+    cpu = None
+    if not a.no_cpu:
+        if work["kind"] == "host":
+            files = [[host["reads_fa"]]]
+        else:
+            m = min(n, a.cpu_sample or 400000)
+            w = min(1000, m)
+            starts = np.linspace(0, n - w, max(1, m // w)).astype(np.int64)
+            singles, pairs = [], []
+            for s0 in starts.tolist():
+                sq, so2, pf = host_range(hip, d_seq, off, paired, s0, s0 + w)
+                for j in range(w):
+                    m1, m2 = fragment_text(sq, so2, j)
+                    (pairs if pf[j] else singles).append((m1.decode(), m2.decode() if pf[j] else None,
+                                                          f"f{s0 + j}"))
+            files = []
+            if singles:
+                write_sample_fasta(os.path.join(tmp, "cpu_single.fa"), None, singles)
+                files.append([os.path.join(tmp, "cpu_single.fa")])
+            if pairs:
+                write_sample_fasta(os.path.join(tmp, "cpu_1.fa"), os.path.join(tmp, "cpu_2.fa"), pairs)
+                files.append(["-P", os.path.join(tmp, "cpu_1.fa"), os.path.join(tmp, "cpu_2.fa")])
+            log(f"# cpu sample: {len(singles)} unpaired + {len(pairs)} pairs in {len(starts)} windows")
+        cpu = cpu_baseline(work, tmp, files, os.cpu_count() or 1)
+        log(f"# cpu baseline: {cpu}")
+
     value = n / batched_s
     line = {
-        "metric": "reads/sec batched vs plain GPU classify (bit-exact) + p50/p99 request latency",
+        "metric": METRIC,
         "value": round(value, 1),
         "unit": "reads/s",
         "n_gpus": 1,
@@ -225,27 +438,45 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u64/u32 integer",
-        "data": "synthetic (seeded generator; DB built by the deterministic build)",
-        "config": {"workload": a.workload + ": " + work["desc"], "reads": n, "bases": len(seq),
-                   "table_cells": int(db.info.capacity), "parallelism": "single GPU"},
+        "data": "synthetic (seeded generators; table built on the GPU)" if work["kind"] == "gpu"
+                else "synthetic (seeded generator; table built by the deterministic CPU build)",
+        "config": {"workload": work["desc"] + ("" if a.scale == 1.0 else f" (scaled x{a.scale})"), "fragments": n,
+                   "bases": total_bases, "table_cells": int(db.info.capacity), "parallelism": "single GPU"},
+        "mbp_per_s": round(total_bases / batched_s / 1e6, 1),
+        "kernel_ms_per_step": round(batched_kms, 3),
+        "persistent_launches_per_step": len(chunks),
         "plain": {"reads_per_s": round(n / plain_s, 1), "ms_per_step": round(plain_s * 1e3, 3),
-                  "kernel_ms_per_step": round(float(np.mean(kms_p)), 3), "launches_per_step": len(blocks)},
+                  "kernel_ms_per_step": round(plain_kms, 3), "launches_per_step": len(blocks),
+                  "batches": "reference input blocks: 3 MiB of unpaired bases or 10 000 pairs"},
+        "plain_one_launch": {"reads_per_s": round(n / timing["plain_one_launch"][0], 1),
+                             "ms_per_step": round(timing["plain_one_launch"][0] * 1e3, 3),
+                             "kernel_ms_per_step": round(timing["plain_one_launch"][1], 3),
+                             "launches_per_step": len(chunks)},
         "batched_vs_plain": round(plain_s / batched_s, 3),
-        "kernel_ms_per_step": round(float(np.mean(kms_b)), 3),
-        "bit_exact": mismatches == 0,
-        "mismatches": mismatches,
+        "batched_vs_plain_one_launch": round(timing["plain_one_launch"][0] / batched_s, 3),
+        "bit_exact": bool(modes_agree and mism == 0 and sample_digest_agree),
+        "verify": {"all_fragments_equal_across_modes": bool(modes_agree), "oracle_sample": int(len(sample)),
+                   "oracle_mismatches": int(mism), "sample_digests_match_bench_run": sample_digest_agree},
         "lookups": lookups,
         "probe_lines": lines,
+        "hitlist_runs": n_runs_total,
+        "redone": {"run_space": space_redone, "dense_table": dense_redone},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+        "gather": {"classify_lines_per_s": round(probe_rate, 1), "ceiling_lines_per_s": round(ceil_rate, 1),
+                   "frac_of_ceiling": round(probe_rate / ceil_rate, 4) if ceil_rate else None,
+                   "ceiling": "independent 16-byte loads at random 128-byte lines of the resident table"},
         "cpu_baseline": cpu,
     }
+    if cpu:
+        line["vs_cpu"] = round(value / cpu["value"], 2)
+    if work["kind"] == "gpu":
+        line["config"].update({"table_estimate": host["estimate"], "gpu_build_s": round(host["build_s"], 2)})
     print(json.dumps(line), flush=True)
-    hip.free(d_seq)
-    hip.free(d_off)
-    batch_b.close()
-    batch_p.close()
+    runner.close()
     db.close()
+    if not a.keep:
+        shutil.rmtree(tmp, ignore_errors=True)
 
 
 if __name__ == "__main__":

@@ -77,6 +77,9 @@ typedef struct {
   int32_t want_hll;         /* -K: per-hit (taxon, encoded HLL hash) records */
   int32_t table_slots;      /* per-fragment LDS hit table (power of 2 in [2,256]; 0 = 256) */
   int32_t want_probe_stats; /* per-fragment probes and distinct 128-byte table lines touched */
+  int32_t run_cap_pct;      /* hitlist space reserved per fragment, in percent of its element
+                             * count (plus 64); fragments that need more are classified again
+                             * into exact space.  0 = 25; 100 = always exact (never redone). */
 } k2g_classify_params;
 
 /* Fragment i: mate 1 = seq[off[2i], off[2i+1]), mate 2 = seq[off[2i+1],
@@ -130,6 +133,31 @@ int k2g_batch_fetch(k2g_batch *b, k2g_results *out);
 /* HIP stream of the batch (a hipStream_t), for callers that enqueue their
  * own work around k2g_classify_device. */
 void *k2g_batch_stream(k2g_batch *b);
+/* Per-fragment scalars of the last classify call, without hitlists: each
+ * non-NULL array receives n entries (n must equal the call's fragment
+ * count).  lookups / probe_lines need want_probe_stats on that call.  For
+ * callers that need calls and counters only (a report, a digest pass). */
+int k2g_batch_fetch_counts(k2g_batch *b, uint32_t *call, uint32_t *hit_groups,
+                           uint32_t *total_kmers, uint32_t *n_runs, uint32_t *lookups,
+                           uint64_t *probe_lines, uint64_t n);
+/* Fragments of the last classify call that were classified a second time:
+ * into exact hitlist space (run_cap_pct overflow) and with the dense
+ * per-taxon table (more distinct hit taxa than table_slots). */
+int k2g_batch_redo_counts(k2g_batch *b, uint64_t *run_space, uint64_t *dense_table);
+/* One 64-bit digest per fragment of the last classify call (call, hit
+ * groups, total k-mers and every hitlist run, in order), copied to
+ * out[0..n).  Comparing digests checks two runs over the same fragments for
+ * equality without moving hitlists to the host.  No reference counterpart:
+ * a verification aid for large batches. */
+int k2g_batch_digests(k2g_batch *b, uint64_t *out, uint64_t n);
+
+/* Random-gather ceiling of the resident table: n_probes independent 16-byte
+ * loads at uniformly random 128-byte-aligned offsets of the table's cells
+ * (the access a hash probe makes), timed on the device.  Returns the rate in
+ * distinct lines per second.  No reference counterpart: the measured
+ * ceiling that probe throughput is compared with. */
+int k2g_db_gather_ceiling(k2g_db *db, uint64_t n_probes, uint64_t seed, double *lines_per_s,
+                          double *kernel_ms);
 
 /* ---- build side: the table construction of build_db
  * (src/build_db.h:186-326; CompareAndSet src/compact_hash.h:416-462)

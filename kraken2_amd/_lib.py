@@ -34,6 +34,10 @@ EXPORTED_SYMBOLS = (
     "k2g_batch_sync",
     "k2g_batch_fetch",
     "k2g_batch_stream",
+    "k2g_batch_fetch_counts",
+    "k2g_batch_redo_counts",
+    "k2g_batch_digests",
+    "k2g_db_gather_ceiling",
     "k2g_build_create",
     "k2g_build_insert",
     "k2g_build_insert_device",
@@ -81,6 +85,7 @@ class ClassifyParams(C.Structure):
         ("want_hll", C.c_int32),
         ("table_slots", C.c_int32),
         ("want_probe_stats", C.c_int32),
+        ("run_cap_pct", C.c_int32),
     ]
 
 
@@ -138,6 +143,10 @@ def load_library(path: str = LIB_PATH):
     lib.k2g_batch_sync.argtypes = [vp, C.POINTER(C.c_double)]
     lib.k2g_batch_fetch.argtypes = [vp, C.POINTER(Results)]
     lib.k2g_batch_stream.argtypes = [vp]
+    lib.k2g_batch_fetch_counts.argtypes = [vp] + [vp] * 6 + [u64]
+    lib.k2g_batch_redo_counts.argtypes = [vp, C.POINTER(u64), C.POINTER(u64)]
+    lib.k2g_batch_digests.argtypes = [vp, vp, u64]
+    lib.k2g_db_gather_ceiling.argtypes = [vp, u64, u64, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     lib.k2g_batch_stream.restype = vp
     for name in EXPORTED_SYMBOLS:
         if name not in ("k2g_last_error", "k2g_batch_stream"):
@@ -197,6 +206,12 @@ class Db:
         except Exception:
             pass
 
+    def gather_ceiling(self, n_probes: int, seed: int = 1) -> tuple[float, float]:
+        """(lines/s, kernel ms) of random 16-byte loads at 128-byte-aligned table offsets."""
+        rate, ms = C.c_double(), C.c_double()
+        _check(_lib.k2g_db_gather_ceiling(self._h, n_probes, seed, C.byref(rate), C.byref(ms)))
+        return rate.value, ms.value
+
     def get_batch(self, keys: np.ndarray) -> np.ndarray:
         keys = np.ascontiguousarray(keys, dtype=np.uint64)
         out = np.zeros(keys.shape[0], dtype=np.uint32)
@@ -244,11 +259,11 @@ class Batch:
 
     def classify(self, seq: bytes, off: np.ndarray, *, paired=False, paired_flags=None,
                  confidence=0.0, min_hit_groups=0, quick=False, mode=0, kmer_counts=False,
-                 hll=False, table_slots=0, probe_stats=False) -> ClassifyOutput:
+                 hll=False, table_slots=0, probe_stats=False, run_cap_pct=0) -> ClassifyOutput:
         off = np.ascontiguousarray(off, dtype=np.uint64)
         n = (off.shape[0] - 1) // 2
         p = ClassifyParams(confidence, min_hit_groups, int(quick), int(paired), mode,
-                           int(kmer_counts), int(hll), table_slots, int(probe_stats))
+                           int(kmer_counts), int(hll), table_slots, int(probe_stats), run_cap_pct)
         sbuf = np.frombuffer(seq, dtype=np.uint8) if len(seq) else np.zeros(1, np.uint8)
         flags = None
         if paired_flags is not None:
@@ -257,6 +272,33 @@ class Batch:
         res = Results()
         _check(_lib.k2g_classify(self._h, C.byref(p), C.byref(fr), C.byref(res)))
         return _copy_results(res, hll, kmer_counts)
+
+    def digests(self, n: int) -> np.ndarray:
+        """Per-fragment 64-bit digests of the last classify call's results."""
+        out = np.zeros(max(1, n), dtype=np.uint64)
+        _check(_lib.k2g_batch_digests(self._h, out.ctypes.data, n))
+        return out[:n]
+
+    def fetch_counts(self, n: int, probe_stats: bool = False) -> dict:
+        """Per-fragment call, hit_groups, total_kmers, n_runs (and lookups,
+        probe_lines) of the last call, without hitlists."""
+        out = {k: np.zeros(max(1, n), np.uint32) for k in ("call", "hit_groups", "total_kmers", "n_runs")}
+        lk = np.zeros(max(1, n), np.uint32) if probe_stats else None
+        ln = np.zeros(max(1, n), np.uint64) if probe_stats else None
+        _check(_lib.k2g_batch_fetch_counts(self._h, *[out[k].ctypes.data for k in
+                                                      ("call", "hit_groups", "total_kmers", "n_runs")],
+                                           lk.ctypes.data if lk is not None else None,
+                                           ln.ctypes.data if ln is not None else None, n))
+        res = {k: v[:n] for k, v in out.items()}
+        if probe_stats:
+            res["lookups"], res["probe_lines"] = lk[:n], ln[:n]
+        return res
+
+    def redo_counts(self) -> tuple[int, int]:
+        """(fragments redone into exact hitlist space, fragments redone with the dense table)."""
+        a, b = C.c_uint64(), C.c_uint64()
+        _check(_lib.k2g_batch_redo_counts(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
 
 def _arr(ptr, n, dtype):

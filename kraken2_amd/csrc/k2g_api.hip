@@ -70,6 +70,25 @@ struct DevBuf {
   }
 };
 
+// Grows buf to n elements keeping its first `keep`.
+template <class T>
+hipError_t grow_keep(DevBuf<T> &buf, size_t n, size_t keep, hipStream_t s) {
+  if (n <= buf.cap) return hipSuccess;
+  T *q = nullptr;
+  hipError_t e = hipMalloc((void **) &q, n * sizeof(T));
+  if (e != hipSuccess) return e;
+  if (keep) e = hipMemcpyAsync(q, buf.p, keep * sizeof(T), hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    (void) hipFree(q);
+    return e;
+  }
+  if (buf.p) (void) hipFree(buf.p);
+  buf.p = q;
+  buf.cap = n;
+  return hipSuccess;
+}
+
 bool read_fully(int fd, void *buf, size_t n, off_t off) {
   char *p = (char *) buf;
   while (n) {
@@ -106,13 +125,14 @@ struct k2g_batch {
   DevBuf<uint8_t> paired;
   // per-fragment device outputs
   DevBuf<uint32_t> call, groups, total, n_runs, status, n_recs, order, redo;
-  DevBuf<uint64_t> run_off, bounds, comp_off, comp_rec_off;
+  DevBuf<uint64_t> run_off, run_cap, bounds, comp_off, comp_rec_off, space_bounds, space_off,
+      digest;
   DevBuf<uint32_t> runs, recs, comp_runs, comp_recs;
   DevBuf<unsigned long long> kmer_counts, cursor, lines;
   DevBuf<uint32_t> lookups;
   std::vector<uint32_t> h_lookups;
   std::vector<uint64_t> h_lines;
-  DevBuf<uint32_t> redo_count;
+  DevBuf<uint32_t> redo_count, space, space_only;
   DevBuf<unsigned char> temp;
   DevBuf<uint32_t> sort_keys, sort_keys_out, sort_vals;
   DevBuf<uint32_t> dense_counts, dense_lists, dense_lens;
@@ -123,7 +143,7 @@ struct k2g_batch {
   // last launch
   uint64_t n = 0;
   k2g_classify_params params{};
-  uint64_t redone = 0;
+  uint64_t redone = 0, space_redone = 0;
   double kernel_ms = 0;
   int persistent_blocks = 0;
 };
@@ -412,12 +432,16 @@ namespace k2g {
 // Exact element count of a fragment: one per k-mer end of each scanned
 // stream plus border markers (src/classify.cc:797-838); runs and HLL
 // records never outnumber it.
+// The space reserved for a fragment's runs is cap = min(e, e * pct / 100 + 64)
+// of its e elements; a fragment that needs more is classified again into e.
 __global__ void k_bounds(const uint64_t *off, const uint8_t *paired, uint32_t paired_all,
-                         uint64_t n, uint32_t k, uint32_t dna, uint64_t *bounds, uint32_t *keys) {
+                         uint64_t n, uint32_t k, uint32_t dna, uint32_t pct, uint64_t *bounds,
+                         uint64_t *caps, uint32_t *keys) {
   const uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
   if (i > n) return;
   if (i == n) {
     bounds[n] = 0;
+    caps[n] = 0;
     return;
   }
   const bool pr = paired ? paired[i] != 0 : paired_all != 0;
@@ -442,6 +466,8 @@ __global__ void k_bounds(const uint64_t *off, const uint8_t *paired, uint32_t pa
   }
   if (pr) e += 1;
   bounds[i] = e;
+  const uint64_t c = e * pct / 100 + 64;
+  caps[i] = c < e ? c : e;
   if (keys) keys[i] = (uint32_t) (e < 0xFFFFFFFFull ? e : 0xFFFFFFFFull);
 }
 
@@ -450,10 +476,73 @@ __global__ void k_iota(uint32_t *v, uint64_t n) {
   if (i < n) v[i] = (uint32_t) i;
 }
 
-__global__ void k_collect_redo(const uint32_t *status, uint64_t n, uint32_t *list,
-                               uint32_t *count) {
+// Fragments to classify again: count[0] dense-table list (status bit 0),
+// count[1] all fragments that ran out of hitlist space (bit 1), count[2]
+// those of them whose hit table did not overflow (status == 2).
+__global__ void k_collect_redo(const uint32_t *status, uint64_t n, uint32_t *dense,
+                               uint32_t *space, uint32_t *space_only, uint32_t *count) {
   const uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n && (status[i] & 1u)) list[atomicAdd(count, 1u)] = (uint32_t) i;
+  if (i >= n) return;
+  const uint32_t st = status[i];
+  if (st & 1u) dense[atomicAdd(&count[0], 1u)] = (uint32_t) i;
+  if (st & 2u) space[atomicAdd(&count[1], 1u)] = (uint32_t) i;
+  if (st == 2u) space_only[atomicAdd(&count[2], 1u)] = (uint32_t) i;
+}
+
+__global__ void k_gather_bounds(const uint32_t *list, uint32_t m, const uint64_t *bounds,
+                                uint64_t *out) {
+  const uint64_t j = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < m) out[j] = bounds[list[j]];
+  if (j == m) out[m] = 0;
+}
+
+// Moves the listed fragments' runs to exact-size space after `base`.
+__global__ void k_repoint(const uint32_t *list, uint32_t m, const uint64_t *tail_off,
+                          uint64_t base, const uint64_t *bounds, uint64_t *run_off,
+                          uint64_t *run_cap) {
+  const uint64_t j = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= m) return;
+  const uint32_t f = list[j];
+  run_off[f] = base + tail_off[j];
+  run_cap[f] = bounds[f];
+}
+
+// One digest per fragment over its call, counters and runs, in order.
+__global__ void k_digest(const uint32_t *call, const uint32_t *groups, const uint32_t *total,
+                         const uint32_t *n_runs, const uint64_t *run_off, const uint32_t *runs,
+                         uint64_t n, uint64_t *out) {
+  const uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint64_t h = 0x6A09E667F3BCC909ull;
+  auto mix = [&h](uint64_t w) { h = fmix64(h ^ (w + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2))); };
+  mix(call[i]);
+  mix(((uint64_t) groups[i] << 32) | total[i]);
+  const uint32_t m = n_runs[i];
+  mix(m);
+  const uint32_t *r = runs + 2 * run_off[i];
+  for (uint32_t j = 0; j < m; j++) mix(((uint64_t) r[2 * j] << 32) | r[2 * j + 1]);
+  out[i] = h;
+}
+
+// Random-gather ceiling: each lane makes `per_lane` independent 16-byte loads
+// at uniformly random 128-byte lines of the cell array.
+__global__ __launch_bounds__(256) void k_gather_ceiling(const uint32_t *cells, uint64_t n_lines,
+                                                        uint32_t per_lane, uint64_t seed,
+                                                        uint32_t *sink) {
+  const uint64_t t = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t acc = 0;
+  for (uint32_t j = 0; j < per_lane; j += 8) {
+    uint4 v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const uint64_t h = fmix64(seed + (t * per_lane + j + u) * 0x9E3779B97F4A7C15ull);
+      const uint64_t line = __umul64hi(h, n_lines);
+      v[u] = *reinterpret_cast<const uint4 *>(cells + line * 32);
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) acc += v[u].x ^ v[u].y ^ v[u].z ^ v[u].w;
+  }
+  if (acc == 0x5EEDF00Du) sink[t & 63] = acc;
 }
 
 // Copies each fragment's runs (2 words each) into a dense array.
@@ -505,7 +594,11 @@ extern "C" int k2g_batch_destroy(k2g_batch *b) {
                     &b->sort_keys, &b->sort_keys_out, &b->sort_vals, &b->dense_counts,
                     &b->dense_lists, &b->dense_lens})
     buf->release();
-  for (auto *buf : {&b->run_off, &b->bounds, &b->comp_off, &b->comp_rec_off}) buf->release();
+  for (auto *buf : {&b->run_off, &b->run_cap, &b->bounds, &b->comp_off, &b->comp_rec_off,
+                    &b->space_bounds, &b->space_off, &b->digest})
+    buf->release();
+  b->space.release();
+  b->space_only.release();
   b->kmer_counts.release();
   b->cursor.release();
   b->lines.release();
@@ -528,6 +621,8 @@ static int check_params(const k2g_classify_params *p) {
   const int s = p->table_slots;
   if (s != 0 && (s < 2 || s > 256 || (s & (s - 1)) != 0))
     return fail(K2G_EUSAGE, "table_slots must be a power of two in [2, 256]");
+  if (p->run_cap_pct < 0 || p->run_cap_pct > 100)
+    return fail(K2G_EUSAGE, "run_cap_pct must be in [0, 100]");
   return K2G_OK;
 }
 
@@ -542,6 +637,7 @@ extern "C" int k2g_classify_device(k2g_batch *b, const k2g_classify_params *p, c
   b->n = n;
   b->params = *p;
   b->redone = 0;
+  b->space_redone = 0;
   b->kernel_ms = 0;
   if (n == 0) return K2G_OK;
   if (n >= (1ULL << 32)) return fail(K2G_EUSAGE, "batch too large (fragments must fit 32 bits)");
@@ -555,26 +651,28 @@ extern "C" int k2g_classify_device(k2g_batch *b, const k2g_classify_params *p, c
   HIP_TRY(b->status.reserve(n));
   HIP_TRY(b->n_recs.reserve(n));
   HIP_TRY(b->run_off.reserve(n + 1));
+  HIP_TRY(b->run_cap.reserve(n + 1));
   HIP_TRY(b->bounds.reserve(n + 1));
   HIP_TRY(b->sort_keys.reserve(n));
-  HIP_TRY(b->redo_count.reserve(1));
+  HIP_TRY(b->redo_count.reserve(3));
   HIP_TRY(b->cursor.reserve(1));
 
-  // element bounds -> run offsets
+  // element bounds -> reserved run space -> run offsets
+  const uint32_t pct = p->run_cap_pct ? (uint32_t) p->run_cap_pct : 25u;
   const unsigned g1 = (unsigned) ((n + 1 + 255) / 256);
   hipLaunchKernelGGL(k_bounds, dim3(g1), dim3(256), 0, s, d_off, d_paired,
-                     (uint32_t) (p->paired != 0), n, v.k, v.dna, b->bounds.p,
+                     (uint32_t) (p->paired != 0), n, v.k, v.dna, pct, b->bounds.p, b->run_cap.p,
                      p->mode == 1 && !d_order ? b->sort_keys.p : nullptr);
   HIP_TRY(hipGetLastError());
   size_t tbytes = 0;
-  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tbytes, b->bounds.p, b->run_off.p, n + 1, s));
+  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tbytes, b->run_cap.p, b->run_off.p, n + 1, s));
   HIP_TRY(b->temp.reserve(tbytes));
-  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(b->temp.p, tbytes, b->bounds.p, b->run_off.p, n + 1, s));
-  uint64_t total_elems = 0;
-  HIP_TRY(hipMemcpyAsync(&total_elems, b->run_off.p + n, 8, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(b->temp.p, tbytes, b->run_cap.p, b->run_off.p, n + 1, s));
+  uint64_t total_space = 0;
+  HIP_TRY(hipMemcpyAsync(&total_space, b->run_off.p + n, 8, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
-  HIP_TRY(b->runs.reserve(2 * std::max<uint64_t>(total_elems, 1)));
-  if (p->want_hll) HIP_TRY(b->recs.reserve(2 * std::max<uint64_t>(total_elems, 1)));
+  HIP_TRY(b->runs.reserve(2 * std::max<uint64_t>(total_space, 1)));
+  if (p->want_hll) HIP_TRY(b->recs.reserve(2 * std::max<uint64_t>(total_space, 1)));
   if (p->want_kmer_counts) {
     HIP_TRY(b->kmer_counts.reserve(v.node_count));
     HIP_TRY(hipMemsetAsync(b->kmer_counts.p, 0, v.node_count * 8, s));
@@ -609,9 +707,9 @@ extern "C" int k2g_classify_device(k2g_batch *b, const k2g_classify_params *p, c
     HIP_TRY(b->lookups.reserve(n));
     HIP_TRY(b->lines.reserve(n));
   }
-  FragOut out{b->call.p,    b->groups.p,  b->total.p, b->n_runs.p,
-              b->status.p,  b->n_recs.p,  b->run_off.p, b->runs.p,
-              p->want_hll ? b->recs.p : nullptr,
+  FragOut out{b->call.p,    b->groups.p,  b->total.p,   b->n_runs.p,
+              b->status.p,  b->n_recs.p,  b->run_off.p, b->run_cap.p,
+              b->runs.p,    p->want_hll ? b->recs.p : nullptr,
               p->want_kmer_counts ? b->kmer_counts.p : nullptr,
               p->want_probe_stats ? b->lookups.p : nullptr,
               p->want_probe_stats ? b->lines.p : nullptr};
@@ -628,16 +726,60 @@ extern "C" int k2g_classify_device(k2g_batch *b, const k2g_classify_params *p, c
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(b->ev1, s));
 
-  // fragments whose hit table overflowed: classify again with a dense table
+  // fragments to classify again: out of hitlist space, or hit table overflowed
   HIP_TRY(b->redo.reserve(n));
-  HIP_TRY(hipMemsetAsync(b->redo_count.p, 0, 4, s));
+  HIP_TRY(b->space.reserve(n));
+  HIP_TRY(b->space_only.reserve(n));
+  HIP_TRY(hipMemsetAsync(b->redo_count.p, 0, 12, s));
   hipLaunchKernelGGL(k_collect_redo, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, s,
-                     b->status.p, n, b->redo.p, b->redo_count.p);
+                     b->status.p, n, b->redo.p, b->space.p, b->space_only.p, b->redo_count.p);
   HIP_TRY(hipGetLastError());
-  uint32_t n_redo = 0;
-  HIP_TRY(hipMemcpyAsync(&n_redo, b->redo_count.p, 4, hipMemcpyDeviceToHost, s));
+  uint32_t counts[3] = {0, 0, 0};
+  HIP_TRY(hipMemcpyAsync(counts, b->redo_count.p, 12, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
+  const uint32_t n_redo = counts[0], n_space = counts[1], n_space_only = counts[2];
+  FragOut out2 = out;
+  out2.kmer_counts = nullptr;  // the first pass counted every fragment
+  out2.lookups = nullptr;
+  out2.lines = nullptr;
+  if (n_space) {
+    // exact space after the reserved space, then classify those fragments again
+    HIP_TRY(b->space_bounds.reserve(n_space + 1));
+    HIP_TRY(b->space_off.reserve(n_space + 1));
+    const unsigned gs = (unsigned) ((n_space + 1 + 255) / 256);
+    hipLaunchKernelGGL(k_gather_bounds, dim3(gs), dim3(256), 0, s, b->space.p, n_space,
+                       b->bounds.p, b->space_bounds.p);
+    HIP_TRY(hipGetLastError());
+    size_t sb = 0;
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, sb, b->space_bounds.p, b->space_off.p,
+                                             n_space + 1, s));
+    HIP_TRY(b->temp.reserve(sb));
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(b->temp.p, sb, b->space_bounds.p, b->space_off.p,
+                                             n_space + 1, s));
+    uint64_t tail = 0;
+    HIP_TRY(hipMemcpyAsync(&tail, b->space_off.p + n_space, 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(grow_keep(b->runs, 2 * (total_space + tail), 2 * total_space, s));
+    if (p->want_hll) HIP_TRY(grow_keep(b->recs, 2 * (total_space + tail), 2 * total_space, s));
+    hipLaunchKernelGGL(k_repoint, dim3(gs), dim3(256), 0, s, b->space.p, n_space, b->space_off.p,
+                       total_space, b->bounds.p, b->run_off.p, b->run_cap.p);
+    HIP_TRY(hipGetLastError());
+    out2.runs = b->runs.p;
+    out2.recs = p->want_hll ? b->recs.p : nullptr;
+    if (n_space_only) {
+      FragBatch in2 = in;
+      in2.n = n_space_only;
+      in2.order = b->space_only.p;
+      HIP_TRY(hipMemsetAsync(b->cursor.p, 0, 8, s));
+      const unsigned grid = (unsigned) std::min<uint64_t>(b->persistent_blocks, n_space_only);
+      hipLaunchKernelGGL(k_classify_persistent, dim3(grid), dim3(64), 0, s, v, cp, in2, out2,
+                         b->cursor.p);
+      HIP_TRY(hipGetLastError());
+    }
+    b->space_redone = n_space;
+  }
   if (n_redo) {
+    // hit table overflowed: classify again with a dense per-taxon table
     const uint32_t slots = std::min<uint32_t>(n_redo, 256);
     if (b->dense_slots < slots) {
       HIP_TRY(b->dense_counts.reserve((uint64_t) slots * v.node_count));
@@ -648,15 +790,55 @@ extern "C" int k2g_classify_device(k2g_batch *b, const k2g_classify_params *p, c
       b->dense_slots = slots;
     }
     DenseScratch ds{b->dense_counts.p, b->dense_lists.p, b->dense_lens.p, slots};
-    FragOut out2 = out;
-    out2.kmer_counts = nullptr;  // already counted by the first pass
-    out2.lookups = nullptr;
-    out2.lines = nullptr;
     hipLaunchKernelGGL(k_classify_dense, dim3(slots), dim3(64), 0, s, v, cp, in, out2, b->redo.p,
                        n_redo, ds);
     HIP_TRY(hipGetLastError());
     b->redone = n_redo;
   }
+  return K2G_OK;
+}
+
+extern "C" int k2g_batch_fetch_counts(k2g_batch *b, uint32_t *call, uint32_t *hit_groups,
+                                      uint32_t *total_kmers, uint32_t *n_runs, uint32_t *lookups,
+                                      uint64_t *probe_lines, uint64_t n) {
+  if (!b) return fail(K2G_EUSAGE, "null batch");
+  if (n != b->n) return fail(K2G_EUSAGE, "count differs from the last batch's fragments");
+  if ((lookups || probe_lines) && !b->params.want_probe_stats)
+    return fail(K2G_EUSAGE, "probe statistics were not requested for the last batch");
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  if (!n) return K2G_OK;
+  std::vector<uint32_t> st(n);
+  HIP_TRY(hipMemcpy(st.data(), b->status.p, n * 4, hipMemcpyDeviceToHost));
+  for (uint64_t i = 0; i < n; i++)
+    if (st[i] & 2u) return fail(K2G_ESOFTWARE, "internal error: hitlist exceeded its bound");
+  if (call) HIP_TRY(hipMemcpy(call, b->call.p, n * 4, hipMemcpyDeviceToHost));
+  if (hit_groups) HIP_TRY(hipMemcpy(hit_groups, b->groups.p, n * 4, hipMemcpyDeviceToHost));
+  if (total_kmers) HIP_TRY(hipMemcpy(total_kmers, b->total.p, n * 4, hipMemcpyDeviceToHost));
+  if (n_runs) HIP_TRY(hipMemcpy(n_runs, b->n_runs.p, n * 4, hipMemcpyDeviceToHost));
+  if (lookups) HIP_TRY(hipMemcpy(lookups, b->lookups.p, n * 4, hipMemcpyDeviceToHost));
+  if (probe_lines) HIP_TRY(hipMemcpy(probe_lines, b->lines.p, n * 8, hipMemcpyDeviceToHost));
+  return K2G_OK;
+}
+
+extern "C" int k2g_batch_redo_counts(k2g_batch *b, uint64_t *run_space, uint64_t *dense_table) {
+  if (!b) return fail(K2G_EUSAGE, "null batch");
+  if (run_space) *run_space = b->space_redone;
+  if (dense_table) *dense_table = b->redone;
+  return K2G_OK;
+}
+
+extern "C" int k2g_batch_digests(k2g_batch *b, uint64_t *out, uint64_t n) {
+  if (!b || (!out && n)) return fail(K2G_EUSAGE, "null argument");
+  if (n != b->n) return fail(K2G_EUSAGE, "digest count differs from the last batch's fragments");
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  if (!n) return K2G_OK;
+  HIP_TRY(b->digest.reserve(n));
+  hipLaunchKernelGGL(k_digest, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, b->stream,
+                     b->call.p, b->groups.p, b->total.p, b->n_runs.p, b->run_off.p, b->runs.p, n,
+                     b->digest.p);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(out, b->digest.p, n * 8, hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
   return K2G_OK;
 }
 
@@ -1130,4 +1312,46 @@ extern "C" int k2g_estimate_capacity(const k2g_build_params *p, const char *seq,
   int rc = k2g_estimate_capacity_device(p, d.p, rel.data(), n, n_ranges, estimate);
   d.release();
   return rc;
+}
+
+// ----------------------------------------------------------------------
+// random-gather ceiling of the resident table
+extern "C" int k2g_db_gather_ceiling(k2g_db *db, uint64_t n_probes, uint64_t seed,
+                                     double *lines_per_s, double *kernel_ms) {
+  if (!db || !lines_per_s) return fail(K2G_EUSAGE, "null argument");
+  const uint64_t n_lines = db->view.capacity / 32;  // 128-byte lines of whole cells
+  if (n_lines == 0) return fail(K2G_EUSAGE, "table smaller than one 128-byte line");
+  constexpr uint32_t per_lane = 32;
+  constexpr unsigned block = 256;
+  const uint64_t blocks64 = std::max<uint64_t>(1, (n_probes + per_lane * block - 1) / (per_lane * block));
+  if (blocks64 > 0x7FFFFFFFull) return fail(K2G_EUSAGE, "too many probes");
+  const unsigned grid = (unsigned) blocks64;
+  hipStream_t s = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  uint32_t *sink = nullptr;
+  hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = hipMalloc((void **) &sink, 64 * 4);
+  float ms = 0;
+  for (int rep = 0; rep < 2 && e == hipSuccess; rep++) {  // first launch warms up
+    e = hipEventRecord(e0, s);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_gather_ceiling, dim3(grid), dim3(block), 0, s, db->view.cells, n_lines,
+                         per_lane, seed + rep, sink);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(e1, s);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  }
+  if (sink) (void) hipFree(sink);
+  if (e0) (void) hipEventDestroy(e0);
+  if (e1) (void) hipEventDestroy(e1);
+  if (s) (void) hipStreamDestroy(s);
+  if (e != hipSuccess) return fail(K2G_ESOFTWARE, std::string("gather ceiling: ") + hipGetErrorString(e));
+  const double probes = (double) grid * block * per_lane;
+  *lines_per_s = ms > 0 ? probes / (ms * 1e-3) : 0.0;
+  if (kernel_ms) *kernel_ms = ms;
+  return K2G_OK;
 }

@@ -27,9 +27,11 @@ struct FragOut {
   uint32_t *groups;        // minimizer hit groups
   uint32_t *total;         // total_kmers denominator (src/classify.cc:890-896)
   uint32_t *n_runs;        // hitlist runs written
-  uint32_t *status;        // 0 ok, 1 hit-table overflow (redo with the dense table), 2 run bound
+  uint32_t *status;        // bit 0: hit-table overflow (redo with the dense table);
+                           // bit 1: more runs or records than run_cap (redo with the exact bound)
   uint32_t *n_recs;        // HLL records written
-  const uint64_t *run_off; // n+1 bounds: runs/recs of fragment i live at [run_off[i], run_off[i+1])
+  const uint64_t *run_off; // runs/recs of fragment i start at run_off[i] ...
+  const uint64_t *run_cap; // ... and hold at most run_cap[i] entries
   uint32_t *runs;          // (code, count) pairs
   uint32_t *recs;          // (taxon, sparse-encoded HLL hash) pairs, or null
   unsigned long long *kmer_counts;  // per internal taxid, or null

@@ -414,7 +414,7 @@ __device__ void classify_fragment(const DbView &db, const ClassifyParams &P, con
   const bool tx = db.dna == 0;
   const int nframes = tx ? 6 : 1;
   const uint64_t base = uniform64(out.run_off[f]);
-  const uint64_t cap = uniform64(out.run_off[f + 1]) - base;
+  const uint64_t cap = uniform64(out.run_cap[f]);
   tab.clear(lane);
   if (lane == 0) *bad = 0;
   __syncthreads();

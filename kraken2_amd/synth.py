@@ -169,12 +169,23 @@ class Workload:
 
     def make_reads(self, n, *, f_single, f_paired, single_len=150, paired_len=150, long_median=8000,
                    long_sigma=0.8, long_cap=50000, sub=0.01, long_sub=0.025, long_indel=0.025,
-                   insert_mean=350, insert_sd=50, off_frac=0.0, off_gc=0.41, seed=None):
+                   insert_mean=350, insert_sd=50, off_frac=0.0, off_gc=0.41, seed=None, group_pairs=False):
+        """group_pairs puts every unpaired read (short and long, still interleaved)
+        before every pair: the two input streams a CPU classify run takes."""
         seed = self.seed if seed is None else seed
         off = np.zeros(2 * n + 1, np.uint64)
         kind = np.zeros(n, np.uint8)
         self.synth.k2s_read_layout(seed, n, f_single, f_paired, single_len, paired_len, long_median,
                                    long_sigma, long_cap, off.ctypes.data, kind.ctypes.data)
+        if group_pairs:
+            perm = np.argsort(kind == 1, kind="stable")
+            l1 = (off[1:2 * n:2] - off[0:2 * n:2])[perm]
+            l2 = (off[2::2] - off[1:2 * n:2])[perm]
+            kind = np.ascontiguousarray(kind[perm])
+            lens = np.empty(2 * n, np.uint64)
+            lens[0::2], lens[1::2] = l1, l2
+            off = np.zeros(2 * n + 1, np.uint64)
+            off[1:] = np.cumsum(lens)
         bases = int(off[-1])
         d_seq = self.hip.malloc(bases)
         rc = self.synth.k2s_reads(seed, n, kind.ctypes.data, off.ctypes.data, self.d_genomes,

@@ -13,7 +13,7 @@ def load():
     vp = C.c_void_p
     u32, u64, i32, dbl = C.c_uint32, C.c_uint64, C.c_int32, C.c_double
     lib.emu_classify.argtypes = [vp, u64, u32, u32, vp, u32, u32, u32, u32, u64, u64, u64, dbl, i32, u32, u32,
-                                 vp, vp, vp, u32, u64, vp, C.c_int, C.c_uint, vp] + [vp] * 9
+                                 vp, vp, vp, u32, u64, vp, C.c_int, C.c_uint, vp, vp] + [vp] * 9
     lib.emu_collectives.restype = C.c_ulonglong
     return lib
 
@@ -62,13 +62,16 @@ def bounds(db, off, paired_flags):
 
 
 def classify(lib, db, seq, off, paired_flags, *, conf=0.0, min_groups=0, quick=False, tab_cap=256, mode=1,
-             grid=4, hll=False, kmer_counts=False):
+             grid=4, hll=False, kmer_counts=False, cap_pct=None):
+    """cap_pct None reserves each fragment's exact element count for its runs;
+    otherwise min(e, e * cap_pct // 100 + 64) as the library does."""
     n = (len(off) - 1) // 2
     off = np.ascontiguousarray(off, np.uint64)
     pf = np.ascontiguousarray(paired_flags, np.uint8)
     b = bounds(db, off, pf)
+    caps = b.copy() if cap_pct is None else np.minimum(b, b * np.uint64(cap_pct) // np.uint64(100) + np.uint64(64))
     run_off = np.zeros(n + 1, np.uint64)
-    run_off[1:] = np.cumsum(b)
+    run_off[1:] = np.cumsum(caps)
     tot = int(run_off[-1])
     order = np.argsort(-b.astype(np.int64), kind="stable").astype(np.uint32)
     outs = {k: np.zeros(n, np.uint32) for k in ("call", "groups", "total", "n_runs", "status", "n_recs")}
@@ -79,7 +82,7 @@ def classify(lib, db, seq, off, paired_flags, *, conf=0.0, min_groups=0, quick=F
     lib.emu_classify(db["cells"].ctypes.data, db["capacity"], db["kb"], db["vb"], db["parent"].ctypes.data,
                      len(db["parent"]), db["k"], db["l"], db["dna"], db["mask"], db["toggle"], db["min_hash"],
                      conf, min_groups, int(quick), tab_cap, sb.ctypes.data, off.ctypes.data, pf.ctypes.data, 0,
-                     n, order.ctypes.data, mode, grid, run_off.ctypes.data,
+                     n, order.ctypes.data, mode, grid, run_off.ctypes.data, caps.ctypes.data,
                      *[outs[k].ctypes.data for k in ("call", "groups", "total", "n_runs", "status", "n_recs")],
                      runs.ctypes.data, recs.ctypes.data if recs is not None else None,
                      kc.ctypes.data if kc is not None else None)

@@ -39,12 +39,13 @@ extern "C" {
 
 // Classifies n fragments with the device code.  cells padded to a multiple
 // of 64; parent has node_count entries.  mode 0 plain, 1 persistent with the
-// given grid.  run_off (n+1) and runs/recs buffers prepared by the caller.
+// given grid.  run_off/run_cap (n) and runs/recs buffers prepared by the caller.
 int emu_classify(const uint32_t *cells, uint64_t capacity, uint32_t kb, uint32_t vb, const uint32_t *parent,
                  uint32_t node_count, uint32_t k, uint32_t l, uint32_t dna, uint64_t spaced, uint64_t toggle,
                  uint64_t min_hash, double conf, int32_t min_groups, uint32_t quick, uint32_t tab_cap,
                  const char *seq, const uint64_t *off, const uint8_t *paired, uint32_t paired_all, uint64_t n,
-                 const uint32_t *order, int mode, unsigned grid, const uint64_t *run_off, uint32_t *call,
+                 const uint32_t *order, int mode, unsigned grid, const uint64_t *run_off,
+                 const uint64_t *run_cap, uint32_t *call,
                  uint32_t *groups, uint32_t *total, uint32_t *n_runs, uint32_t *status, uint32_t *n_recs,
                  uint32_t *runs, uint32_t *recs, unsigned long long *kmer_counts) {
   using namespace k2g;
@@ -67,7 +68,8 @@ int emu_classify(const uint32_t *cells, uint64_t capacity, uint32_t kb, uint32_t
   v.min_hash = min_hash;
   ClassifyParams P{conf, min_groups, quick, tab_cap};
   FragBatch in{seq, off, paired, paired_all, n, order};
-  FragOut out{call, groups, total, n_runs, status, n_recs, run_off, runs, recs, kmer_counts, nullptr, nullptr};
+  FragOut out{call,    groups, total, n_runs, status,      n_recs,  run_off,
+              run_cap, runs,   recs,  kmer_counts, nullptr, nullptr};
   if (mode == 0) {
     run_grid((unsigned) n, [&] { k_classify_plain(v, P, in, out); });
   } else {

@@ -63,6 +63,20 @@ def revcomp(s):
     return "".join(t.get(c, "N") for c in reversed(s))
 
 
+def chimeric_reads(genomes, n, seed, seg=40, length=2000):
+    """Reads stitched from short segments of random genomes: many hitlist runs."""
+    rng = random.Random(seed)
+    out = []
+    for _ in range(n):
+        parts = []
+        while sum(map(len, parts)) < length:
+            g = genomes[rng.randrange(len(genomes))]
+            p = rng.randrange(len(g) - seg)
+            parts.append(g[p:p + seg])
+        out.append("".join(parts)[:length])
+    return out
+
+
 def edge_reads(genomes, n, seed, min_len=0, max_len=400):
     """Reads sampled from genomes with the parity edge cases mixed in:
     N / IUPAC / lowercase bases, lengths around k, reverse strands, random reads."""

@@ -9,7 +9,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from helpers import Oracle, edge_reads, read_fasta
+from helpers import Oracle, chimeric_reads, edge_reads, read_fasta
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 G = os.path.join(HERE, "golden")
@@ -25,11 +25,12 @@ def emu_lib():
     return emu, emu.load()
 
 
-def _check(emu, lib, d, seq, off, pflags, *, conf=0.0, g=0, quick=False, tab=256, mode=1, hll=False):
+def _check(emu, lib, d, seq, off, pflags, *, conf=0.0, g=0, quick=False, tab=256, mode=1, hll=False,
+           cap_pct=None):
     import kraken2_amd as k2
     db = emu.read_db(d)
     out = emu.classify(lib, db, seq, off, pflags, conf=conf, min_groups=g, quick=quick, tab_cap=tab, mode=mode,
-                       hll=hll, kmer_counts=hll)
+                       hll=hll, kmer_counts=hll, cap_pct=cap_pct)
     orc = Oracle()
     h = orc.open(d)
     n = len(pflags)
@@ -44,8 +45,12 @@ def _check(emu, lib, d, seq, off, pflags, *, conf=0.0, g=0, quick=False, tab=256
         assert got == exp, i
         if not quick:
             assert int(out["total"][i]) == int(ref["total"][i])
+            if out["status"][i] & 2:  # out of run space: runs truncated, counted in full
+                assert int(out["n_runs"][i]) > int(out["run_off"][i + 1] - out["run_off"][i])
+                continue
             assert k2.format_hitlist(runs, db["ext"]) == ref["text"][i], i
     return out
+
 
 
 @pytest.mark.parametrize("mode", [0, 1])
@@ -91,3 +96,16 @@ def test_emulated_translated(emu_lib):
     seqs = seqs[:40]
     seq, off = k2.pack_fragments(seqs)
     _check(emu, lib, d, seq, off, np.zeros(len(seqs), np.uint8), g=0)
+
+
+def test_emulated_run_space_overflow(emu_lib, synth_db):
+    """With little run space reserved, fragments with more runs are flagged
+    (status bit 1) while their call and counters stay exact; the rest match
+    the oracle in full."""
+    import kraken2_amd as k2
+    emu, lib = emu_lib
+    _, genomes = read_fasta(os.path.join(synth_db, "library.fna"))
+    reads = chimeric_reads(genomes, 12, seed=61) + edge_reads(genomes, 12, seed=62)
+    seq, off = k2.pack_fragments(reads)
+    out = _check(emu, lib, synth_db, seq, off, np.zeros(len(reads), np.uint8), g=2, cap_pct=1)
+    assert (out["status"] & 2).any() and not (out["status"] & 2).all()

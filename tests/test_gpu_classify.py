@@ -8,7 +8,7 @@ import random
 import numpy as np
 import pytest
 
-from helpers import Oracle, edge_reads, read_fasta
+from helpers import Oracle, chimeric_reads, edge_reads, read_fasta
 
 pytestmark = pytest.mark.gpu
 
@@ -114,3 +114,49 @@ def test_mixed_paired_flags(env):
             runs = out.runs[out.run_off[i]:out.run_off[i + 1]]
             assert (int(out.call[i]), k2.format_hitlist(runs, db.external_ids)) == \
                    (int(ref["call"][j]), ref["text"][j])
+
+
+@pytest.mark.parametrize("mode,slots,hll", [(1, 256, False), (0, 256, False), (1, 2, False), (1, 256, True)])
+def test_run_space_overflow_redo(env, mode, slots, hll):
+    """With 1% of the element count reserved for runs, fragments that need
+    more are classified again into exact space; results equal the oracle and
+    the exact-space run, HLL records included."""
+    k2, db, batch, orc, h, genomes = env
+    reads = chimeric_reads(genomes, 150, seed=500) + edge_reads(genomes, 300, seed=501)
+    seq, off = k2.pack_fragments(reads)
+    n = len(reads)
+    ref = orc.classify(h, seq, off, min_hit_groups=2)
+    exact = batch.classify(seq, off, min_hit_groups=2, mode=mode, table_slots=slots, hll=hll,
+                           kmer_counts=hll, run_cap_pct=100)
+    assert batch.redo_counts()[0] == 0
+    d_exact = batch.digests(n)
+    out = batch.classify(seq, off, min_hit_groups=2, mode=mode, table_slots=slots, hll=hll,
+                         kmer_counts=hll, run_cap_pct=1)
+    space, dense = batch.redo_counts()
+    assert space > 0
+    if slots == 2:
+        assert dense > 0
+    _compare(k2, db, out, ref, n)
+    assert np.array_equal(batch.digests(n), d_exact)
+    if hll:
+        assert np.array_equal(out.kmer_counts, exact.kmer_counts)
+        for i in range(n):
+            a = out.hll[out.hll_off[i]:out.hll_off[i + 1]]
+            b = exact.hll[exact.hll_off[i]:exact.hll_off[i + 1]]
+            assert np.array_equal(a, b), i
+
+
+def test_digests_plain_vs_persistent(env):
+    k2, db, batch, orc, h, genomes = env
+    reads = edge_reads(genomes, 2000, seed=600)
+    seq, off = k2.pack_fragments(reads)
+    batch.classify(seq, off, min_hit_groups=2, mode=0)
+    d0 = batch.digests(len(reads))
+    batch.classify(seq, off, min_hit_groups=2, mode=1)
+    d1 = batch.digests(len(reads))
+    assert np.array_equal(d0, d1)
+    # a digest depends on the fragment's results: reads with different calls differ
+    out = batch.classify(seq, off, min_hit_groups=2, mode=1)
+    calls = out.call
+    i, j = int(np.argmax(calls != calls[0])), 0
+    assert calls[i] != calls[j] and d1[i] != d1[j]
