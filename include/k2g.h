@@ -159,6 +159,40 @@ int k2g_batch_digests(k2g_batch *b, uint64_t *out, uint64_t n);
 int k2g_db_gather_ceiling(k2g_db *db, uint64_t n_probes, uint64_t seed, double *lines_per_s,
                           double *kernel_ms);
 
+/* ---- serving: the resident-index request loop.  The reference keeps its
+ * index loaded in ClassifyDaemon (src/classify.cc:373-430) and runs whole
+ * classify() calls one after another.  Here requests are admitted in arrival
+ * order whenever a slot is free and the device memory a request needs fits in
+ * what is left of a fixed budget (one allocation made by k2g_server_create,
+ * carved per request); admitted requests run concurrently, each on its own
+ * slot's HIP stream, while the table stays resident.  A request's results
+ * equal those of k2g_classify on the same fragments alone. */
+typedef struct k2g_server k2g_server;
+
+typedef struct {
+  uint64_t hbm_budget; /* device bytes for request state: reads, hitlists, HLL records, scratch */
+  int32_t slots;       /* requests on the GPU at once, in [1, 64] */
+} k2g_server_params;
+
+int k2g_server_create(k2g_db *db, const k2g_server_params *p, k2g_server **out);
+/* Queues a request (FIFO).  The fragment arrays must stay valid until
+ * k2g_server_wait returns for it.  Hitlist space is always exact
+ * (params->run_cap_pct is ignored).  K2G_EUSAGE if the request alone needs
+ * more than the budget. */
+int k2g_server_submit(k2g_server *s, const k2g_classify_params *p, const k2g_fragments *in,
+                      uint64_t *ticket);
+/* Blocks until the request is done.  out (may be NULL) points into memory
+ * that stays valid until k2g_server_release; times_ms (may be NULL)
+ * receives {queued, on the GPU, total} milliseconds. */
+int k2g_server_wait(k2g_server *s, uint64_t ticket, k2g_results *out, double *times_ms);
+int k2g_server_release(k2g_server *s, uint64_t ticket);
+/* Highest device memory held by admitted requests, most requests on the GPU
+ * at once, and the budget. */
+int k2g_server_stats(k2g_server *s, uint64_t *peak_bytes, int32_t *max_concurrent,
+                     uint64_t *budget);
+/* Finishes queued requests, then frees the server. */
+int k2g_server_destroy(k2g_server *s);
+
 /* ---- build side: the table construction of build_db
  * (src/build_db.h:186-326; CompareAndSet src/compact_hash.h:416-462)
  * as lock-free atomicCAS with an in-kernel LCA merge.  Sequences are scanned

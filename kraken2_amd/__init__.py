@@ -18,6 +18,7 @@ from ._lib import (  # noqa: F401
     ClassifyOutput,
     Db,
     K2GError,
+    Server,
     expand_spaced_seed_mask,
     format_hitlist,
     load_library,

@@ -38,6 +38,12 @@ EXPORTED_SYMBOLS = (
     "k2g_batch_redo_counts",
     "k2g_batch_digests",
     "k2g_db_gather_ceiling",
+    "k2g_server_create",
+    "k2g_server_submit",
+    "k2g_server_wait",
+    "k2g_server_release",
+    "k2g_server_stats",
+    "k2g_server_destroy",
     "k2g_build_create",
     "k2g_build_insert",
     "k2g_build_insert_device",
@@ -86,6 +92,13 @@ class ClassifyParams(C.Structure):
         ("table_slots", C.c_int32),
         ("want_probe_stats", C.c_int32),
         ("run_cap_pct", C.c_int32),
+    ]
+
+
+class ServerParams(C.Structure):
+    _fields_ = [
+        ("hbm_budget", C.c_uint64),
+        ("slots", C.c_int32),
     ]
 
 
@@ -146,6 +159,12 @@ def load_library(path: str = LIB_PATH):
     lib.k2g_batch_fetch_counts.argtypes = [vp] + [vp] * 6 + [u64]
     lib.k2g_batch_redo_counts.argtypes = [vp, C.POINTER(u64), C.POINTER(u64)]
     lib.k2g_batch_digests.argtypes = [vp, vp, u64]
+    lib.k2g_server_create.argtypes = [vp, C.POINTER(ServerParams), C.POINTER(vp)]
+    lib.k2g_server_submit.argtypes = [vp, C.POINTER(ClassifyParams), C.POINTER(Fragments), C.POINTER(u64)]
+    lib.k2g_server_wait.argtypes = [vp, u64, C.POINTER(Results), C.POINTER(C.c_double)]
+    lib.k2g_server_release.argtypes = [vp, u64]
+    lib.k2g_server_stats.argtypes = [vp, C.POINTER(u64), C.POINTER(C.c_int32), C.POINTER(u64)]
+    lib.k2g_server_destroy.argtypes = [vp]
     lib.k2g_db_gather_ceiling.argtypes = [vp, u64, u64, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     lib.k2g_batch_stream.restype = vp
     for name in EXPORTED_SYMBOLS:
@@ -428,6 +447,63 @@ class Builder:
         if self._h:
             _lib.k2g_build_destroy(self._h)
             self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """The request server: requests admitted FIFO against a device-memory
+    budget, run concurrently on `slots` streams, the table resident."""
+
+    def __init__(self, db: Db, hbm_budget: int, slots: int = 4):
+        self.db = db
+        h = C.c_void_p()
+        _check(_lib.k2g_server_create(db._h, C.byref(ServerParams(int(hbm_budget), int(slots))), C.byref(h)))
+        self._h = h
+        self._keep = {}
+
+    def submit(self, seq: bytes, off: np.ndarray, *, paired=False, paired_flags=None, confidence=0.0,
+               min_hit_groups=0, quick=False, mode=1, kmer_counts=False, hll=False, table_slots=0) -> int:
+        off = np.ascontiguousarray(off, dtype=np.uint64)
+        n = (off.shape[0] - 1) // 2
+        p = ClassifyParams(confidence, min_hit_groups, int(quick), int(paired), mode,
+                           int(kmer_counts), int(hll), table_slots, 0, 100)
+        sbuf = np.frombuffer(seq, dtype=np.uint8) if len(seq) else np.zeros(1, np.uint8)
+        flags = None if paired_flags is None else np.ascontiguousarray(paired_flags, dtype=np.uint8)
+        fr = Fragments(sbuf.ctypes.data, off.ctypes.data, flags.ctypes.data if flags is not None else None, n)
+        t = C.c_uint64()
+        _check(_lib.k2g_server_submit(self._h, C.byref(p), C.byref(fr), C.byref(t)))
+        self._keep[t.value] = (sbuf, off, flags, hll, kmer_counts)
+        return t.value
+
+    def wait(self, ticket: int):
+        """(ClassifyOutput, {"queued_ms", "gpu_ms", "total_ms"}); releases the request."""
+        res = Results()
+        times = (C.c_double * 3)()
+        rc = _lib.k2g_server_wait(self._h, ticket, C.byref(res), times)
+        try:
+            _check(rc)
+            _, _, _, hll, kc = self._keep[ticket]
+            out = _copy_results(res, hll, kc)
+        finally:
+            _lib.k2g_server_release(self._h, ticket)
+            self._keep.pop(ticket, None)
+        return out, {"queued_ms": times[0], "gpu_ms": times[1], "total_ms": times[2]}
+
+    def stats(self) -> dict:
+        peak, conc, budget = C.c_uint64(), C.c_int32(), C.c_uint64()
+        _check(_lib.k2g_server_stats(self._h, C.byref(peak), C.byref(conc), C.byref(budget)))
+        return {"peak_bytes": peak.value, "max_concurrent": conc.value, "budget": budget.value}
+
+    def close(self):
+        if self._h:
+            _lib.k2g_server_destroy(self._h)
+            self._h = None
+            self._keep.clear()
 
     def __del__(self):
         try:

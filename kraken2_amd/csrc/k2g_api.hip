@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "k2g_device.h"
+#include "k2g_internal.h"
 #include "k2g_kernels.h"
 
 namespace k2g {
@@ -32,48 +33,34 @@ __global__ void k_build(BuildView, const char *, const SubBlock *, uint64_t);
 
 using namespace k2g;
 
-namespace {
+namespace k2g {
 
+namespace {
 thread_local std::string g_error;
+}  // namespace
 
 int fail(int code, const std::string &msg) {
   g_error = msg;
   return code;
 }
 
-#define HIP_TRY(expr)                                                                   \
-  do {                                                                                  \
-    hipError_t e_ = (expr);                                                             \
-    if (e_ != hipSuccess)                                                               \
-      return fail(K2G_ESOFTWARE, std::string(#expr) + " failed: " + hipGetErrorString(e_)); \
-  } while (0)
-
-// Growable device buffer.
-template <class T>
-struct DevBuf {
-  T *p = nullptr;
-  size_t cap = 0;
-  hipError_t reserve(size_t n) {
-    if (n <= cap) return hipSuccess;
-    if (p) (void) hipFree(p);
-    p = nullptr;
-    cap = 0;
-    size_t want = std::max<size_t>(n, 64);
-    hipError_t e = hipMalloc((void **) &p, want * sizeof(T));
-    if (e == hipSuccess) cap = want;
-    return e;
-  }
-  void release() {
-    if (p) (void) hipFree(p);
-    p = nullptr;
-    cap = 0;
-  }
-};
+const std::string &last_error() { return g_error; }
 
 // Grows buf to n elements keeping its first `keep`.
 template <class T>
 hipError_t grow_keep(DevBuf<T> &buf, size_t n, size_t keep, hipStream_t s) {
   if (n <= buf.cap) return hipSuccess;
+  if (buf.region) {  // fresh space in the region, then copy
+    DevBuf<T> q;
+    q.region = buf.region;
+    hipError_t e = q.reserve(n);
+    if (e == hipSuccess && keep)
+      e = hipMemcpyAsync(q.p, buf.p, keep * sizeof(T), hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) return e;
+    buf.p = q.p;
+    buf.cap = q.cap;
+    return hipSuccess;
+  }
   T *q = nullptr;
   hipError_t e = hipMalloc((void **) &q, n * sizeof(T));
   if (e != hipSuccess) return e;
@@ -88,6 +75,10 @@ hipError_t grow_keep(DevBuf<T> &buf, size_t n, size_t keep, hipStream_t s) {
   buf.cap = n;
   return hipSuccess;
 }
+
+}  // namespace k2g
+
+namespace {
 
 bool read_fully(int fd, void *buf, size_t n, off_t off) {
   char *p = (char *) buf;
@@ -104,51 +95,7 @@ bool read_fully(int fd, void *buf, size_t n, off_t off) {
 
 }  // namespace
 
-// ----------------------------------------------------------------------
-struct k2g_db {
-  DbView view{};
-  uint32_t *d_cells = nullptr;
-  uint32_t *d_parent = nullptr;
-  std::vector<uint64_t> external_ids;
-  k2g_db_info info{};
-  int device = 0;
-  int num_cus = 0;
-};
-
-struct k2g_batch {
-  k2g_db *db = nullptr;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  // inputs (host-path uploads)
-  DevBuf<char> seq;
-  DevBuf<uint64_t> off;
-  DevBuf<uint8_t> paired;
-  // per-fragment device outputs
-  DevBuf<uint32_t> call, groups, total, n_runs, status, n_recs, order, redo;
-  DevBuf<uint64_t> run_off, run_cap, bounds, comp_off, comp_rec_off, space_bounds, space_off,
-      digest;
-  DevBuf<uint32_t> runs, recs, comp_runs, comp_recs;
-  DevBuf<unsigned long long> kmer_counts, cursor, lines;
-  DevBuf<uint32_t> lookups;
-  std::vector<uint32_t> h_lookups;
-  std::vector<uint64_t> h_lines;
-  DevBuf<uint32_t> redo_count, space, space_only;
-  DevBuf<unsigned char> temp;
-  DevBuf<uint32_t> sort_keys, sort_keys_out, sort_vals;
-  DevBuf<uint32_t> dense_counts, dense_lists, dense_lens;
-  uint32_t dense_slots = 0;
-  // host copies of the last results
-  std::vector<uint32_t> h_call, h_groups, h_total, h_runs, h_recs;
-  std::vector<uint64_t> h_run_off, h_rec_off, h_kmer_counts;
-  // last launch
-  uint64_t n = 0;
-  k2g_classify_params params{};
-  uint64_t redone = 0, space_redone = 0;
-  double kernel_ms = 0;
-  int persistent_blocks = 0;
-};
-
-extern "C" const char *k2g_last_error(void) { return g_error.c_str(); }
+extern "C" const char *k2g_last_error(void) { return k2g::last_error().c_str(); }
 
 // ----------------------------------------------------------------------
 // DB open: src/classify.cc:269-302, src/compact_hash.h:278-341,
@@ -339,7 +286,7 @@ extern "C" int k2g_db_open(const char *hash_k2d, const char *taxo_k2d, const cha
       e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpy(&h_max, d_max, sizeof(h_max), hipMemcpyDeviceToHost);
-    if (d_max) hipFree(d_max);
+    if (d_max) (void) hipFree(d_max);
     if (e != hipSuccess) {
       k2g_db_close(db);
       return fail(K2G_ESOFTWARE, std::string("table check failed: ") + hipGetErrorString(e));
@@ -373,8 +320,8 @@ extern "C" int k2g_db_open(const char *hash_k2d, const char *taxo_k2d, const cha
 
 extern "C" int k2g_db_close(k2g_db *db) {
   if (!db) return K2G_OK;
-  if (db->d_cells) hipFree(db->d_cells);
-  if (db->d_parent) hipFree(db->d_parent);
+  if (db->d_cells) (void) hipFree(db->d_cells);
+  if (db->d_parent) (void) hipFree(db->d_parent);
   delete db;
   return K2G_OK;
 }
@@ -419,8 +366,8 @@ extern "C" int k2g_get_batch(k2g_db *db, const uint64_t *keys, uint32_t *out, si
     rc = k2g_get_batch_device(db, dk, dv, n, nullptr);
     if (rc == K2G_OK) e = hipMemcpy(out, dv, n * 4, hipMemcpyDeviceToHost);
   }
-  hipFree(dk);
-  if (dv) hipFree(dv);
+  (void) hipFree(dk);
+  if (dv) (void) hipFree(dv);
   if (e != hipSuccess) return fail(K2G_ESOFTWARE, std::string("get_batch: ") + hipGetErrorString(e));
   return rc;
 }
@@ -429,11 +376,8 @@ extern "C" int k2g_get_batch(k2g_db *db, const uint64_t *keys, uint32_t *out, si
 // batch
 namespace k2g {
 
-// Exact element count of a fragment: one per k-mer end of each scanned
-// stream plus border markers (src/classify.cc:797-838); runs and HLL
-// records never outnumber it.
-// The space reserved for a fragment's runs is cap = min(e, e * pct / 100 + 64)
-// of its e elements; a fragment that needs more is classified again into e.
+// Element count e and reserved hitlist space of every fragment; a fragment
+// that needs more than its reserved space is classified again into e.
 __global__ void k_bounds(const uint64_t *off, const uint8_t *paired, uint32_t paired_all,
                          uint64_t n, uint32_t k, uint32_t dna, uint32_t pct, uint64_t *bounds,
                          uint64_t *caps, uint32_t *keys) {
@@ -445,29 +389,10 @@ __global__ void k_bounds(const uint64_t *off, const uint8_t *paired, uint32_t pa
     return;
   }
   const bool pr = paired ? paired[i] != 0 : paired_all != 0;
-  uint64_t e = 0;
-  for (int m = 0; m < (pr ? 2 : 1); m++) {
-    const uint64_t L = off[2 * i + m + 1] - off[2 * i + m];
-    if (dna) {
-      e += L + 1 > k ? L + 1 - k : 0;
-    } else {
-      for (int fr = 0; fr < 6; fr++) {
-        uint64_t fl;
-        if (L < 3) {
-          fl = 1;
-        } else {
-          const uint64_t first = (fr % 3) == 0 ? 3 : ((fr % 3) == 1 ? 4 : 2);
-          fl = (L - 1 >= first) ? (L - 1 - first) / 3 + 1 : 0;
-        }
-        e += fl + 1 > k ? fl + 1 - k : 0;
-      }
-      e += 5;
-    }
-  }
-  if (pr) e += 1;
+  const uint64_t e = fragment_elements(off[2 * i + 1] - off[2 * i],
+                                       pr ? off[2 * i + 2] - off[2 * i + 1] : 0, pr, k, dna != 0);
   bounds[i] = e;
-  const uint64_t c = e * pct / 100 + 64;
-  caps[i] = c < e ? c : e;
+  caps[i] = run_space(e, pct);
   if (keys) keys[i] = (uint32_t) (e < 0xFFFFFFFFull ? e : 0xFFFFFFFFull);
 }
 
@@ -585,33 +510,34 @@ extern "C" int k2g_batch_create(k2g_db *db, k2g_batch **out) {
 
 extern "C" int k2g_batch_destroy(k2g_batch *b) {
   if (!b) return K2G_OK;
-  if (b->stream) hipStreamSynchronize(b->stream);
-  for (auto *buf : {&b->seq}) buf->release();
-  b->off.release();
-  b->paired.release();
-  for (auto *buf : {&b->call, &b->groups, &b->total, &b->n_runs, &b->status, &b->n_recs, &b->order,
-                    &b->redo, &b->runs, &b->recs, &b->comp_runs, &b->comp_recs, &b->redo_count,
-                    &b->sort_keys, &b->sort_keys_out, &b->sort_vals, &b->dense_counts,
-                    &b->dense_lists, &b->dense_lens})
-    buf->release();
-  for (auto *buf : {&b->run_off, &b->run_cap, &b->bounds, &b->comp_off, &b->comp_rec_off,
-                    &b->space_bounds, &b->space_off, &b->digest})
-    buf->release();
-  b->space.release();
-  b->space_only.release();
-  b->kmer_counts.release();
-  b->cursor.release();
-  b->lines.release();
-  b->lookups.release();
-  b->temp.release();
-  if (b->ev0) hipEventDestroy(b->ev0);
-  if (b->ev1) hipEventDestroy(b->ev1);
-  if (b->stream) hipStreamDestroy(b->stream);
+  if (b->stream) (void) hipStreamSynchronize(b->stream);
+  b->for_each_device_buf([](auto &buf) { buf.release(); });
+  b->h_runs.release();
+  b->h_recs.release();
+  if (b->ev0) (void) hipEventDestroy(b->ev0);
+  if (b->ev1) (void) hipEventDestroy(b->ev1);
+  if (b->stream) (void) hipStreamDestroy(b->stream);
   delete b;
   return K2G_OK;
 }
 
 extern "C" void *k2g_batch_stream(k2g_batch *b) { return b ? (void *) b->stream : nullptr; }
+
+namespace k2g {
+// Largest hipcub scratch a classify call over n fragments asks for (scans of
+// n + 1 words; the longest-first sort of n keys).
+static size_t temp_bytes_for(uint64_t n, bool sort) {
+  size_t a = 0, c = 0;
+  uint64_t *u = nullptr;
+  (void) hipcub::DeviceScan::ExclusiveSum(nullptr, a, u, u, n + 1, (hipStream_t) 0);
+  if (sort) {
+    uint32_t *k = nullptr;
+    (void) hipcub::DeviceRadixSort::SortPairsDescending(nullptr, c, k, k, k, k, (int) n, 0, 32,
+                                                        (hipStream_t) 0);
+  }
+  return std::max(a, c);
+}
+}  // namespace k2g
 
 static int check_params(const k2g_classify_params *p) {
   if (!p) return fail(K2G_EUSAGE, "null params");
@@ -656,6 +582,8 @@ extern "C" int k2g_classify_device(k2g_batch *b, const k2g_classify_params *p, c
   HIP_TRY(b->sort_keys.reserve(n));
   HIP_TRY(b->redo_count.reserve(3));
   HIP_TRY(b->cursor.reserve(1));
+  // in a bounded region, the largest scratch any step needs is taken once
+  if (b->temp.region) HIP_TRY(b->temp.reserve(temp_bytes_for(n, p->mode == 1 && !d_order)));
 
   // element bounds -> reserved run space -> run offsets
   const uint32_t pct = p->run_cap_pct ? (uint32_t) p->run_cap_pct : 25u;
@@ -875,10 +803,10 @@ extern "C" int k2g_batch_fetch(k2g_batch *b, k2g_results *r) {
     HIP_TRY(hipMemcpy(b->h_call.data(), b->call.p, n * 4, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(b->h_groups.data(), b->groups.p, n * 4, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(b->h_total.data(), b->total.p, n * 4, hipMemcpyDeviceToHost));
-    // compact runs: exclusive scan of counts, then gather
+    // compact runs: exclusive scan of counts, then a gather that writes
+    // straight into pinned host memory
     auto compact = [&](const uint32_t *d_counts, const uint32_t *d_src, DevBuf<uint64_t> &off,
-                       DevBuf<uint32_t> &dst, std::vector<uint64_t> &h_off,
-                       std::vector<uint32_t> &h_dst) -> int {
+                       PinnedBuf<uint32_t> &dst, std::vector<uint64_t> &h_off) -> int {
       HIP_TRY(off.reserve(n + 1));
       HIP_TRY(b->bounds.reserve(n + 1));
       hipLaunchKernelGGL(k_widen, dim3((unsigned) ((n + 1 + 255) / 256)), dim3(256), 0, s,
@@ -892,23 +820,22 @@ extern "C" int k2g_batch_fetch(k2g_batch *b, k2g_results *r) {
       HIP_TRY(hipStreamSynchronize(s));
       const uint64_t tot = h_off[n];
       HIP_TRY(dst.reserve(2 * std::max<uint64_t>(tot, 1)));
+      uint32_t *d_dst = nullptr;
+      HIP_TRY(hipHostGetDevicePointer((void **) &d_dst, dst.p, 0));
       const unsigned grid = (unsigned) std::min<uint64_t>(n, 65536);
       hipLaunchKernelGGL(k_compact, dim3(grid), dim3(64), 0, s, b->run_off.p, d_counts, off.p,
-                         d_src, dst.p, n);
+                         d_src, d_dst, n);
       HIP_TRY(hipGetLastError());
-      h_dst.resize(2 * tot);
-      if (tot)
-        HIP_TRY(hipMemcpyAsync(h_dst.data(), dst.p, tot * 8, hipMemcpyDeviceToHost, s));
       HIP_TRY(hipStreamSynchronize(s));
       return K2G_OK;
     };
-    rc = compact(b->n_runs.p, b->runs.p, b->comp_off, b->comp_runs, b->h_run_off, b->h_runs);
+    rc = compact(b->n_runs.p, b->runs.p, b->comp_off, b->h_runs, b->h_run_off);
     if (rc) return rc;
     if (b->params.want_hll) {
-      rc = compact(b->n_recs.p, b->recs.p, b->comp_rec_off, b->comp_recs, b->h_rec_off, b->h_recs);
+      rc = compact(b->n_recs.p, b->recs.p, b->comp_rec_off, b->h_recs, b->h_rec_off);
       if (rc) return rc;
       r->hll_off = b->h_rec_off.data();
-      r->hll = b->h_recs.data();
+      r->hll = b->h_recs.p;
     }
     if (b->params.want_probe_stats) {
       b->h_lookups.resize(n);
@@ -929,7 +856,7 @@ extern "C" int k2g_batch_fetch(k2g_batch *b, k2g_results *r) {
   r->hit_groups = b->h_groups.data();
   r->total_kmers = b->h_total.data();
   r->run_off = b->h_run_off.data();
-  r->runs = b->h_runs.data();
+  r->runs = b->h_runs.p;
   return K2G_OK;
 }
 
@@ -1354,4 +1281,53 @@ extern "C" int k2g_db_gather_ceiling(k2g_db *db, uint64_t n_probes, uint64_t see
   *lines_per_s = ms > 0 ? probes / (ms * 1e-3) : 0.0;
   if (kernel_ms) *kernel_ms = ms;
   return K2G_OK;
+}
+
+// ----------------------------------------------------------------------
+// device memory plan of one k2g_classify call in a bound region (mirrors
+// the reservations of k2g_classify, k2g_classify_device and k2g_batch_fetch)
+uint64_t k2g::plan_classify_bytes(const k2g_db *db, const k2g_classify_params *p,
+                                  const k2g_fragments *in) {
+  const uint64_t n = in->n;
+  if (n == 0) return 0;
+  const DbView &v = db->view;
+  const uint64_t bases = in->off[2 * n];
+  const uint32_t pct = p->run_cap_pct ? (uint32_t) p->run_cap_pct : 25u;
+  uint64_t space = 0;
+  for (uint64_t i = 0; i < n; i++) {
+    const bool pr = in->paired ? in->paired[i] != 0 : p->paired != 0;
+    const uint64_t e = fragment_elements(in->off[2 * i + 1] - in->off[2 * i],
+                                         pr ? in->off[2 * i + 2] - in->off[2 * i + 1] : 0, pr,
+                                         v.k, v.dna != 0);
+    space += run_space(e, pct);
+  }
+  const bool sort = p->mode == 1;
+  uint64_t b = 0;
+  b += region_bytes(bases + 1, 1) + region_bytes(2 * n + 1, 8);  // seq, off
+  if (in->paired) b += region_bytes(n, 1);
+  b += 6 * region_bytes(n, 4);                                    // call .. n_recs
+  b += 3 * region_bytes(n + 1, 8);                                // run_off, run_cap, bounds
+  b += region_bytes(n, 4) + region_bytes(3, 4) + region_bytes(1, 8);  // sort keys, counters, cursor
+  b += region_bytes(temp_bytes_for(n, sort), 1);
+  b += region_bytes(2 * std::max<uint64_t>(space, 1), 4);         // runs
+  if (p->want_hll) b += region_bytes(2 * std::max<uint64_t>(space, 1), 4);
+  if (p->want_kmer_counts) b += region_bytes(v.node_count, 8);
+  if (sort) b += 3 * region_bytes(n, 4);                          // order, sort values, sorted keys
+  if (p->want_probe_stats) b += region_bytes(n, 4) + region_bytes(n, 8);
+  b += 3 * region_bytes(n, 4);                                    // redo lists
+  const uint64_t slots = std::min<uint64_t>(n, 256);              // dense-table redo, worst case
+  b += 2 * region_bytes(slots * v.node_count, 4) + region_bytes(slots, 4);
+  if (pct < 100) {                                                // exact-space redo, worst case
+    b += 2 * region_bytes(n + 1, 8);
+    uint64_t exact = 0;
+    for (uint64_t i = 0; i < n; i++) {
+      const bool pr = in->paired ? in->paired[i] != 0 : p->paired != 0;
+      exact += fragment_elements(in->off[2 * i + 1] - in->off[2 * i],
+                                 pr ? in->off[2 * i + 2] - in->off[2 * i + 1] : 0, pr, v.k,
+                                 v.dna != 0);
+    }
+    b += region_bytes(2 * (space + exact), 4) * (p->want_hll ? 2 : 1);
+  }
+  b += region_bytes(n + 1, 8) * (p->want_hll ? 2 : 1);           // fetch offsets
+  return b;
 }

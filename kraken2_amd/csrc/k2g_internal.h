@@ -1,0 +1,173 @@
+// Host-side internals of libk2g.so shared by the classify driver
+// (k2g_api.hip) and the request server (k2g_server.hip): device and pinned
+// buffers, the resident database and the batch state.
+#pragma once
+#include "k2g.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "k2g_device.h"
+#include "k2g_kernels.h"
+
+namespace k2g {
+
+// Sets the calling thread's last error message and returns code.
+int fail(int code, const std::string &msg);
+// Last error message of the calling thread.
+const std::string &last_error();
+
+#define HIP_TRY(expr)                                                                       \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return ::k2g::fail(K2G_ESOFTWARE, std::string(#expr) + " failed: " + hipGetErrorString(e_)); \
+  } while (0)
+
+// A device range that a batch carves its buffers from by bumping `used`
+// (never freed piecemeal).  Lets a caller bound a batch's device memory to
+// a range it admitted in advance.
+struct Region {
+  char *base = nullptr;
+  size_t size = 0;
+  size_t used = 0;
+};
+
+constexpr size_t kRegionAlign = 256;
+
+inline size_t region_bytes(size_t n, size_t elem) {
+  const size_t b = std::max<size_t>(n, 64) * elem;
+  return (b + kRegionAlign - 1) / kRegionAlign * kRegionAlign;
+}
+
+// Growable device buffer: hipMalloc'd, or carved from a Region when one is
+// bound (then growth takes new space in the region and release is a no-op).
+template <class T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t cap = 0;
+  Region *region = nullptr;
+  hipError_t reserve(size_t n) {
+    if (n <= cap) return hipSuccess;
+    const size_t want = std::max<size_t>(n, 64);
+    if (region) {
+      const size_t off = (region->used + kRegionAlign - 1) / kRegionAlign * kRegionAlign;
+      const size_t bytes = region_bytes(want, sizeof(T));
+      if (off + bytes > region->size) return hipErrorOutOfMemory;
+      p = reinterpret_cast<T *>(region->base + off);
+      cap = want;
+      region->used = off + bytes;
+      return hipSuccess;
+    }
+    if (p) (void) hipFree(p);
+    p = nullptr;
+    cap = 0;
+    hipError_t e = hipMalloc((void **) &p, want * sizeof(T));
+    if (e == hipSuccess) cap = want;
+    return e;
+  }
+  void release() {
+    if (p && !region) (void) hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+  // Drops the current storage (freeing it if owned) and allocates from r
+  // (or with hipMalloc when r is null) from now on.
+  void bind(Region *r) {
+    release();
+    region = r;
+  }
+};
+
+// Growable pinned host buffer, also addressed by kernels (zero-copy writes).
+template <class T>
+struct PinnedBuf {
+  T *p = nullptr;
+  size_t cap = 0;
+  hipError_t reserve(size_t n) {
+    if (n <= cap) return hipSuccess;
+    if (p) (void) hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+    const size_t want = std::max<size_t>(n + n / 2, 1024);
+    hipError_t e = hipHostMalloc((void **) &p, want * sizeof(T), hipHostMallocMapped);
+    if (e == hipSuccess) cap = want;
+    return e;
+  }
+  void release() {
+    if (p) (void) hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+}  // namespace k2g
+
+struct k2g_db {
+  k2g::DbView view{};
+  uint32_t *d_cells = nullptr;
+  uint32_t *d_parent = nullptr;
+  std::vector<uint64_t> external_ids;
+  k2g_db_info info{};
+  int device = 0;
+  int num_cus = 0;
+};
+
+struct k2g_batch {
+  k2g_db *db = nullptr;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // inputs (host-path uploads)
+  k2g::DevBuf<char> seq;
+  k2g::DevBuf<uint64_t> off;
+  k2g::DevBuf<uint8_t> paired;
+  // per-fragment device outputs
+  k2g::DevBuf<uint32_t> call, groups, total, n_runs, status, n_recs, order, redo;
+  k2g::DevBuf<uint64_t> run_off, run_cap, bounds, comp_off, comp_rec_off, space_bounds, space_off,
+      digest;
+  k2g::DevBuf<uint32_t> runs, recs;
+  k2g::DevBuf<unsigned long long> kmer_counts, cursor, lines;
+  k2g::DevBuf<uint32_t> lookups;
+  k2g::DevBuf<uint32_t> redo_count, space, space_only;
+  k2g::DevBuf<unsigned char> temp;
+  k2g::DevBuf<uint32_t> sort_keys, sort_keys_out, sort_vals;
+  k2g::DevBuf<uint32_t> dense_counts, dense_lists, dense_lens;
+  uint32_t dense_slots = 0;
+  // host copies of the last results (hitlists and HLL records are written
+  // by the compaction kernel straight into pinned memory)
+  std::vector<uint32_t> h_call, h_groups, h_total, h_lookups;
+  std::vector<uint64_t> h_run_off, h_rec_off, h_kmer_counts, h_lines;
+  k2g::PinnedBuf<uint32_t> h_runs, h_recs;
+  // last launch
+  uint64_t n = 0;
+  k2g_classify_params params{};
+  uint64_t redone = 0, space_redone = 0;
+  double kernel_ms = 0;
+  int persistent_blocks = 0;
+
+  template <class F>
+  void for_each_device_buf(F f) {
+    f(seq), f(off), f(paired);
+    f(call), f(groups), f(total), f(n_runs), f(status), f(n_recs), f(order), f(redo);
+    f(run_off), f(run_cap), f(bounds), f(comp_off), f(comp_rec_off), f(space_bounds),
+        f(space_off), f(digest);
+    f(runs), f(recs), f(kmer_counts), f(cursor), f(lines), f(lookups);
+    f(redo_count), f(space), f(space_only), f(temp), f(sort_keys), f(sort_keys_out), f(sort_vals);
+    f(dense_counts), f(dense_lists), f(dense_lens);
+  }
+  // Allocates every device buffer from r from now on (null: hipMalloc).
+  void bind_region(k2g::Region *r) {
+    for_each_device_buf([r](auto &buf) { buf.bind(r); });
+    dense_slots = 0;
+  }
+};
+
+namespace k2g {
+// Device bytes a k2g_classify call (host fragments, results fetched) takes
+// from a bound Region, as an upper bound; requires run_cap_pct == 100.
+uint64_t plan_classify_bytes(const k2g_db *db, const k2g_classify_params *p,
+                             const k2g_fragments *in);
+}  // namespace k2g

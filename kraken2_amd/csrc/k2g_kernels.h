@@ -39,6 +39,42 @@ struct FragOut {
   unsigned long long *lines;  // per fragment: distinct 128-byte lines the probes touched
 };
 
+// Exact element count of a fragment: one per k-mer end of each scanned
+// stream plus border markers (src/classify.cc:797-838); a fragment's
+// hitlist runs and HLL records never outnumber it.  L2 is ignored unless
+// paired.  Translated databases scan six frames of each mate
+// (src/aa_translate.cc:41-100) with five frame markers between them.
+__host__ __device__ inline uint64_t fragment_elements(uint64_t L1, uint64_t L2, bool paired,
+                                                      uint32_t k, bool dna) {
+  uint64_t e = 0;
+  for (int m = 0; m < (paired ? 2 : 1); m++) {
+    const uint64_t L = m ? L2 : L1;
+    if (dna) {
+      e += L + 1 > k ? L + 1 - k : 0;
+    } else {
+      for (int fr = 0; fr < 6; fr++) {
+        uint64_t fl;
+        if (L < 3) {
+          fl = 1;
+        } else {
+          const uint64_t first = (fr % 3) == 0 ? 3 : ((fr % 3) == 1 ? 4 : 2);
+          fl = (L - 1 >= first) ? (L - 1 - first) / 3 + 1 : 0;
+        }
+        e += fl + 1 > k ? fl + 1 - k : 0;
+      }
+      e += 5;
+    }
+  }
+  if (paired) e += 1;
+  return e;
+}
+
+// Hitlist space reserved for a fragment of e elements: min(e, e * pct / 100 + 64).
+__host__ __device__ inline uint64_t run_space(uint64_t e, uint32_t pct) {
+  const uint64_t c = e * pct / 100 + 64;
+  return c < e ? c : e;
+}
+
 // Per-slot scratch of the overflow path: a dense count array per taxon.
 struct DenseScratch {
   uint32_t *counts;  // slots * node_count
