@@ -21,6 +21,7 @@ Prints one JSON line.
 """
 import argparse
 import ctypes as C
+import hashlib
 import json
 import os
 import re
@@ -28,6 +29,7 @@ import shutil
 import subprocess
 import sys
 import tempfile
+import threading
 import time
 
 import numpy as np
@@ -181,10 +183,151 @@ def run_gpu_workload(a, work, k2):
                        paired_len=work["paired_len"], off_frac=work["off_frac"], seed=work["seed"],
                        group_pairs=True)
     log(f"# reads: {rs.n} fragments, {rs.bases / 1e9:.2f} Gbp in {time.time() - t0:.1f}s")
+    pools = None if a.no_serve else make_pools(wl, a)
     wl.free_genomes()
     return tmp, wl.hip, rs.d_seq, rs.d_off, rs.d_paired, rs.off, rs.paired, dict(readset=rs, build_s=wl.build_s,
                                                                                capacity=wl.capacity,
-                                                                               estimate=wl.estimate)
+                                                                               estimate=wl.estimate, pools=pools)
+
+
+def make_pools(wl, a):
+    """Host read pools that serving requests are sliced from: the paired
+    2x150 mix with 20% off-target reads, and the uneven-length mix (single,
+    pairs and long reads interleaved), both generated on the GPU."""
+    pools = {}
+    m = max(2000, int(a.serve_pool * min(1.0, a.scale * 20)))
+    for name, key, seed in (("pairs 2x150", "c2", 52), ("uneven mix", "c3", 53)):
+        w = WORKLOADS[key]
+        rs = wl.make_reads(m, f_single=w["f_single"], f_paired=w["f_paired"], paired_len=w["paired_len"],
+                           off_frac=w["off_frac"], seed=seed)
+        seq = np.frombuffer(rs.hip.download(rs.d_seq, rs.bases), np.uint8)
+        pools[name] = dict(seq=seq, off=rs.off.copy(), paired=rs.paired.copy(), n=rs.n)
+        rs.free()
+    log(f"# serving pools: " + ", ".join(f"{k} {v['n']} fragments {len(v['seq']) / 1e6:.0f} Mbp"
+                                          for k, v in pools.items()))
+    return pools
+
+
+def serve_requests(pools, n_req, max_reads, rng):
+    """Request sizes log-uniform in [1, max_reads] fragments, each a random
+    slice of one of the pools."""
+    names = sorted(pools)
+    reqs = []
+    for _ in range(n_req):
+        name = names[int(rng.integers(len(names)))]
+        pool = pools[name]
+        size = min(pool["n"], int(np.exp(rng.uniform(0.0, np.log(max_reads)))))
+        s0 = int(rng.integers(0, pool["n"] - size + 1))
+        off = pool["off"][2 * s0:2 * (s0 + size) + 1]
+        a0, a2 = int(off[0]), int(off[-1])
+        reqs.append(dict(seq=pool["seq"][a0:a2], off=(off - off[0]).astype(np.uint64),
+                         paired=pool["paired"][s0:s0 + size], pool=name, size=size, bases=a2 - a0))
+    return reqs
+
+
+def result_digest(out):
+    h = hashlib.sha1()
+    for arr in (out.call, out.hit_groups, out.total_kmers, out.run_off, out.runs, out.hll_off, out.hll,
+                out.kmer_counts):
+        h.update(b"-" if arr is None else np.ascontiguousarray(arr).tobytes())
+    return h.hexdigest()
+
+
+def serve_at_rate(k2, db, reqs, rate, budget, slots, work, rng):
+    """Poisson arrivals at `rate` requests/s into a fresh server; returns
+    per-request (digest, times) and the server's statistics."""
+    srv = k2.Server(db, budget, slots)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, len(reqs))) if rate else np.zeros(len(reqs))
+    tickets = [None] * len(reqs)
+    results = [None] * len(reqs)
+    ready = threading.Semaphore(0)
+    errors = []
+
+    def waiter():
+        try:
+            for i in range(len(reqs)):
+                ready.acquire()
+                out, times = srv.wait(tickets[i])
+                results[i] = (result_digest(out), times)
+        except Exception as e:  # surfaced after join
+            errors.append(e)
+
+    th = threading.Thread(target=waiter)
+    th.start()
+    t0 = time.perf_counter()
+    try:
+        for i, r in enumerate(reqs):
+            delay = t0 + arrivals[i] - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            tickets[i] = srv.submit(r["seq"], r["off"], paired_flags=r["paired"], confidence=work["confidence"],
+                                    min_hit_groups=work["min_hit_groups"], kmer_counts=True, hll=True)
+            ready.release()
+    finally:
+        th.join()
+    wall = time.perf_counter() - t0
+    stats = srv.stats()
+    srv.close()
+    if errors:
+        raise errors[0]
+    return results, wall, stats
+
+
+def run_serving(k2, db, tmp, pools, a, work):
+    """Config 5: the resident table serves requests of log-uniform size with
+    -R -K (per-taxon counts and HLL records) under a device-memory budget."""
+    rng = np.random.default_rng(5)
+    max_reads = max(10, int(200_000 * min(1.0, a.scale * 20)))
+    budget = int(a.serve_budget_gb * (1 << 30))
+    # service time of one request at a time (one slot, closed loop)
+    probe = serve_requests(pools, 30, max_reads, rng)
+    res, wall, _ = serve_at_rate(k2, db, probe, 0, budget, 1, work, rng)
+    seq_ms = float(np.mean([t["gpu_ms"] for _, t in res]))
+    mu = 1000.0 / seq_ms
+    log(f"# serving: one request at a time takes {seq_ms:.1f} ms on average ({mu:.1f} req/s)")
+    rates_out = []
+    all_reqs, all_dig = [], []
+    for mult in (0.5, 1.0, 2.0):
+        reqs = serve_requests(pools, a.serve_requests, max_reads, rng)
+        rate = mult * mu
+        res, wall, st = serve_at_rate(k2, db, reqs, rate, budget, a.serve_slots, work, rng)
+        lat = np.array([t["total_ms"] for _, t in res])
+        qd = np.array([t["queued_ms"] for _, t in res])
+        reads = sum(r["size"] for r in reqs)
+        bases = sum(r["bases"] for r in reqs)
+        row = {"offered_req_per_s": round(rate, 2), "completed_req_per_s": round(len(reqs) / wall, 2),
+               "requests": len(reqs), "reads_per_s": round(reads / wall, 1), "mbp_per_s": round(bases / wall / 1e6, 1),
+               "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+               "queued_p50_ms": round(float(np.percentile(qd, 50)), 3),
+               "max_concurrent": st["max_concurrent"], "peak_bytes": st["peak_bytes"]}
+        rates_out.append(row)
+        log(f"# serving at {rate:.1f} req/s: {row}")
+        all_reqs += reqs
+        all_dig += [d for d, _ in res]
+    # every request alone (one batch, one request at a time) must give the same results
+    batch = k2.Batch(db)
+    alone_equal = True
+    for r, d in zip(all_reqs, all_dig):
+        out = batch.classify(r["seq"].tobytes(), r["off"], paired_flags=r["paired"], confidence=work["confidence"],
+                             min_hit_groups=work["min_hit_groups"], kmer_counts=True, hll=True, mode=1)
+        alone_equal &= result_digest(out) == d
+    batch.close()
+    # and the CPU oracle on the smaller requests (calls and hitlists)
+    small = sorted(range(len(all_reqs)), key=lambda i: all_reqs[i]["size"])[:40]
+    mism = checked = 0
+    for i in small:
+        r = all_reqs[i]
+        m, _ = oracle_check(k2, db, tmp, r["seq"].tobytes(), r["off"], r["paired"], work)
+        mism += m
+        checked += r["size"]
+    log(f"# serving verify: all {len(all_reqs)} requests equal alone: {alone_equal}; oracle mismatches "
+        f"{mism} over {checked} fragments of {len(small)} requests")
+    return {"workload": "c5: requests of log-uniform size in [1, %d] fragments sliced from the pairs 2x150 and "
+                        "uneven-length mixes, Poisson arrivals, -R -K (counts + HLL records), against the resident "
+                        "table" % max_reads,
+            "budget_bytes": budget, "slots": a.serve_slots, "sequential_service_ms": round(seq_ms, 3),
+            "rates": rates_out, "bit_exact_vs_alone": bool(alone_equal), "oracle_requests": len(small),
+            "oracle_fragments": checked, "oracle_mismatches": int(mism)}
 
 
 def write_sample_fasta(path1, path2, frags):
@@ -290,6 +433,13 @@ def main():
                     help="hitlist runs reserved per persistent launch (8 bytes each)")
     ap.add_argument("--cpu-sample", type=int, default=None, help="fragments timed on the CPU")
     ap.add_argument("--gather-probes", type=float, default=2 ** 30)
+    ap.add_argument("--plain-steps", type=int, default=1, help="timed steps of the plain per-block pass")
+    ap.add_argument("--full-verify", action="store_true", help="digest every plain block, not an even spread")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--serve-requests", type=int, default=300, help="requests per offered rate")
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-budget-gb", type=float, default=32.0)
+    ap.add_argument("--serve-pool", type=int, default=300_000, help="fragments per serving pool")
     ap.add_argument("--scale", type=float, default=1.0,
                     help="scale genomes and reads down for a quick functional run (not a bench number)")
     a = ap.parse_args()
@@ -326,30 +476,44 @@ def main():
         ranges, mode = passes[name]
         return sum(runner.run(s, e, mode) for s, e in ranges)
 
+    # The persistent pass is the headline: W warmup steps, then K timed steps.
+    # The plain per-block pass (thousands of launches at full size) is timed
+    # over min(K, plain_steps) steps after a warmup over its first blocks.
     timing = {}
     for name in passes:
+        steps = a.steps if name != "plain" else max(1, min(a.steps, a.plain_steps))
         for _ in range(a.warmup):
-            run_pass(name)
+            if name == "plain":
+                for s0, e0 in blocks[:16]:
+                    runner.run(s0, e0, 0)
+            else:
+                run_pass(name)
         t = time.perf_counter()
         kms = []
-        for _ in range(a.steps):
+        for _ in range(steps):
             kms.append(run_pass(name))
-        wall = (time.perf_counter() - t) / a.steps
-        timing[name] = (wall, float(np.mean(kms)))
+            log(f"# {name}: step {len(kms)}/{steps} kernel {kms[-1]:.1f} ms")
+        wall = (time.perf_counter() - t) / steps
+        timing[name] = (wall, float(np.mean(kms)), steps)
         log(f"# {name}: {wall * 1e3:.1f} ms/step ({len(passes[name][0])} launches), kernel {np.mean(kms):.1f} ms")
-    batched_s, batched_kms = timing["batched"]
-    plain_s, plain_kms = timing["plain"]
+    batched_s, batched_kms, _ = timing["batched"]
+    plain_s, plain_kms, plain_steps = timing["plain"]
 
     # ---- verification and measurement passes (outside the timed region)
     digests = {}
     lookups = lines = n_runs_total = 0
     space_redone = dense_redone = 0
+    checked = {}
     for name in passes:
         if a.no_verify and name != "batched":
             continue
         ranges, mode = passes[name]
+        if name == "plain" and not a.full_verify and len(ranges) > 256:
+            ranges = ranges[::max(1, len(ranges) // 256)]  # an even spread of the blocks
         dig = np.zeros(n, np.uint64)
+        mask = np.zeros(n, bool)
         for s, e in ranges:
+            mask[s:e] = True
             stats = name == "batched"
             runner.run(s, e, mode, stats=stats)
             dig[s:e] = runner.batch.digests(e - s)
@@ -362,9 +526,15 @@ def main():
                 space_redone += sr
                 dense_redone += dr
         digests[name] = dig
-        log(f"# digests of {name} done")
-    modes_agree = a.no_verify or (np.array_equal(digests["plain"], digests["batched"]) and
-                                  np.array_equal(digests["plain_one_launch"], digests["batched"]))
+        checked[name] = mask
+        log(f"# digests of {name} done ({int(mask.sum())} fragments)")
+    if a.no_verify:
+        modes_agree, plain_checked = True, 0
+    else:
+        pm = checked["plain"]
+        plain_checked = int(pm.sum())
+        modes_agree = (np.array_equal(digests["plain"][pm], digests["batched"][pm]) and
+                       np.array_equal(digests["plain_one_launch"], digests["batched"]))
 
     rng = np.random.default_rng(7)
     sample = np.sort(rng.choice(n, size=min(n, 4000), replace=False))
@@ -384,7 +554,7 @@ def main():
         s_seq, so, _ = host["readset"].host_fragments(sample)
     mism, s_dig = oracle_check(k2, db, tmp, s_seq, so, paired[sample], work)
     sample_digest_agree = bool(np.array_equal(s_dig, digests["batched"][sample]))
-    log(f"# verify: modes agree on all {n} fragments: {modes_agree}; oracle sample mismatches: "
+    log(f"# verify: modes agree ({plain_checked} plain / {n} one-launch fragments): {modes_agree}; oracle sample mismatches: "
         f"{mism}/{len(sample)}; sample digests equal the bench run's: {sample_digest_agree}")
 
     # roofline of the persistent classify kernel: algorithmic bytes per launch
@@ -425,6 +595,12 @@ def main():
         cpu = cpu_baseline(work, tmp, files, os.cpu_count() or 1)
         log(f"# cpu baseline: {cpu}")
 
+    serving = None
+    if work["kind"] == "gpu" and host.get("pools"):
+        runner.close()
+        host["readset"].free()
+        serving = run_serving(k2, db, tmp, host["pools"], a, work)
+
     value = n / batched_s
     line = {
         "metric": METRIC,
@@ -447,6 +623,7 @@ def main():
         "persistent_launches_per_step": len(chunks),
         "plain": {"reads_per_s": round(n / plain_s, 1), "ms_per_step": round(plain_s * 1e3, 3),
                   "kernel_ms_per_step": round(plain_kms, 3), "launches_per_step": len(blocks),
+                  "timed_steps": plain_steps,
                   "batches": "reference input blocks: 3 MiB of unpaired bases or 10 000 pairs"},
         "plain_one_launch": {"reads_per_s": round(n / timing["plain_one_launch"][0], 1),
                              "ms_per_step": round(timing["plain_one_launch"][0] * 1e3, 3),
@@ -455,7 +632,8 @@ def main():
         "batched_vs_plain": round(plain_s / batched_s, 3),
         "batched_vs_plain_one_launch": round(timing["plain_one_launch"][0] / batched_s, 3),
         "bit_exact": bool(modes_agree and mism == 0 and sample_digest_agree),
-        "verify": {"all_fragments_equal_across_modes": bool(modes_agree), "oracle_sample": int(len(sample)),
+        "verify": {"modes_equal": bool(modes_agree), "plain_fragments_compared": plain_checked,
+                   "one_launch_fragments_compared": n, "oracle_sample": int(len(sample)),
                    "oracle_mismatches": int(mism), "sample_digests_match_bench_run": sample_digest_agree},
         "lookups": lookups,
         "probe_lines": lines,
@@ -468,12 +646,18 @@ def main():
                    "ceiling": "independent 16-byte loads at random 128-byte lines of the resident table"},
         "cpu_baseline": cpu,
     }
+    if serving:
+        mid = serving["rates"][1]
+        line["latency_ms"] = {"p50": mid["p50_ms"], "p99": mid["p99_ms"],
+                              "offered_req_per_s": mid["offered_req_per_s"]}
+        line["serving"] = serving
     if cpu:
         line["vs_cpu"] = round(value / cpu["value"], 2)
     if work["kind"] == "gpu":
         line["config"].update({"table_estimate": host["estimate"], "gpu_build_s": round(host["build_s"], 2)})
     print(json.dumps(line), flush=True)
-    runner.close()
+    if not serving:
+        runner.close()
     db.close()
     if not a.keep:
         shutil.rmtree(tmp, ignore_errors=True)

@@ -92,6 +92,7 @@ class ClassifyParams(C.Structure):
         ("table_slots", C.c_int32),
         ("want_probe_stats", C.c_int32),
         ("run_cap_pct", C.c_int32),
+        ("unit_elements", C.c_int32),
     ]
 
 
@@ -278,11 +279,12 @@ class Batch:
 
     def classify(self, seq: bytes, off: np.ndarray, *, paired=False, paired_flags=None,
                  confidence=0.0, min_hit_groups=0, quick=False, mode=0, kmer_counts=False,
-                 hll=False, table_slots=0, probe_stats=False, run_cap_pct=0) -> ClassifyOutput:
+                 hll=False, table_slots=0, probe_stats=False, run_cap_pct=0, unit_elements=0) -> ClassifyOutput:
         off = np.ascontiguousarray(off, dtype=np.uint64)
         n = (off.shape[0] - 1) // 2
         p = ClassifyParams(confidence, min_hit_groups, int(quick), int(paired), mode,
-                           int(kmer_counts), int(hll), table_slots, int(probe_stats), run_cap_pct)
+                           int(kmer_counts), int(hll), table_slots, int(probe_stats), run_cap_pct,
+                           unit_elements)
         sbuf = np.frombuffer(seq, dtype=np.uint8) if len(seq) else np.zeros(1, np.uint8)
         flags = None
         if paired_flags is not None:
@@ -472,7 +474,10 @@ class Server:
         n = (off.shape[0] - 1) // 2
         p = ClassifyParams(confidence, min_hit_groups, int(quick), int(paired), mode,
                            int(kmer_counts), int(hll), table_slots, 0, 100)
-        sbuf = np.frombuffer(seq, dtype=np.uint8) if len(seq) else np.zeros(1, np.uint8)
+        if isinstance(seq, np.ndarray):
+            sbuf = np.ascontiguousarray(seq, dtype=np.uint8) if len(seq) else np.zeros(1, np.uint8)
+        else:
+            sbuf = np.frombuffer(seq, dtype=np.uint8) if len(seq) else np.zeros(1, np.uint8)
         flags = None if paired_flags is None else np.ascontiguousarray(paired_flags, dtype=np.uint8)
         fr = Fragments(sbuf.ctypes.data, off.ctypes.data, flags.ctypes.data if flags is not None else None, n)
         t = C.c_uint64()

@@ -396,6 +396,18 @@ __global__ void k_bounds(const uint64_t *off, const uint8_t *paired, uint32_t pa
   if (keys) keys[i] = (uint32_t) (e < 0xFFFFFFFFull ? e : 0xFFFFFFFFull);
 }
 
+// Work units over the longest-first order: position j starts a unit when its
+// element prefix enters a new window of T elements.
+__global__ void k_unit_flags(const uint64_t *prefix, uint64_t n, uint64_t T, uint8_t *flag) {
+  const uint64_t j = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  flag[j] = (j == 0 || prefix[j] / T != prefix[j - 1] / T) ? 1 : 0;
+}
+
+__global__ void k_unit_end(uint32_t *unit_start, const uint32_t *n_units, uint64_t n) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) unit_start[*n_units] = (uint32_t) n;
+}
+
 __global__ void k_iota(uint32_t *v, uint64_t n) {
   const uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) v[i] = (uint32_t) i;
@@ -525,17 +537,21 @@ extern "C" void *k2g_batch_stream(k2g_batch *b) { return b ? (void *) b->stream 
 
 namespace k2g {
 // Largest hipcub scratch a classify call over n fragments asks for (scans of
-// n + 1 words; the longest-first sort of n keys).
+// n + 1 words; the longest-first sort of n keys and its unit selection).
 static size_t temp_bytes_for(uint64_t n, bool sort) {
   size_t a = 0, c = 0;
   uint64_t *u = nullptr;
   (void) hipcub::DeviceScan::ExclusiveSum(nullptr, a, u, u, n + 1, (hipStream_t) 0);
+  size_t d = 0;
   if (sort) {
     uint32_t *k = nullptr;
+    uint8_t *fl = nullptr;
     (void) hipcub::DeviceRadixSort::SortPairsDescending(nullptr, c, k, k, k, k, (int) n, 0, 32,
                                                         (hipStream_t) 0);
+    (void) hipcub::DeviceSelect::Flagged(nullptr, d, hipcub::CountingInputIterator<uint32_t>(0), fl,
+                                         k, k, (int) n, (hipStream_t) 0);
   }
-  return std::max(a, c);
+  return std::max(a, std::max(c, d));
 }
 }  // namespace k2g
 
@@ -608,6 +624,7 @@ extern "C" int k2g_classify_device(k2g_batch *b, const k2g_classify_params *p, c
 
   // persistent order: longest fragments first
   const uint32_t *order = d_order;
+  const uint32_t *units = nullptr;
   if (p->mode == 1 && !order) {
     HIP_TRY(b->order.reserve(n));
     HIP_TRY(b->sort_vals.reserve(n));
@@ -623,6 +640,30 @@ extern "C" int k2g_classify_device(k2g_batch *b, const k2g_classify_params *p, c
                                                          b->sort_keys_out.p, b->sort_vals.p,
                                                          b->order.p, (int) n, 0, 32, s));
     order = b->order.p;
+    // work units: windows of about T elements of the sorted order
+    const uint64_t T = p->unit_elements > 0 ? (uint64_t) p->unit_elements : 4096u;
+    HIP_TRY(b->unit_pos.reserve(n + 1));
+    HIP_TRY(b->unit_scan.reserve(n + 1));
+    HIP_TRY(b->unit_flag.reserve(n));
+    HIP_TRY(b->unit_start.reserve(n + 1));
+    HIP_TRY(b->n_units.reserve(1));
+    hipLaunchKernelGGL(k_widen, dim3((unsigned) ((n + 1 + 255) / 256)), dim3(256), 0, s,
+                       b->sort_keys_out.p, b->unit_pos.p, n);
+    size_t ub = 0;
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, ub, b->unit_pos.p, b->unit_scan.p, n + 1, s));
+    HIP_TRY(b->temp.reserve(ub));
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(b->temp.p, ub, b->unit_pos.p, b->unit_scan.p, n + 1, s));
+    hipLaunchKernelGGL(k_unit_flags, dim3(gi), dim3(256), 0, s, b->unit_scan.p, n, T,
+                       b->unit_flag.p);
+    size_t fb = 0;
+    HIP_TRY(hipcub::DeviceSelect::Flagged(nullptr, fb, hipcub::CountingInputIterator<uint32_t>(0),
+                                          b->unit_flag.p, b->unit_start.p, b->n_units.p, (int) n, s));
+    HIP_TRY(b->temp.reserve(fb));
+    HIP_TRY(hipcub::DeviceSelect::Flagged(b->temp.p, fb, hipcub::CountingInputIterator<uint32_t>(0),
+                                          b->unit_flag.p, b->unit_start.p, b->n_units.p, (int) n, s));
+    hipLaunchKernelGGL(k_unit_end, dim3(1), dim3(64), 0, s, b->unit_start.p, b->n_units.p, n);
+    HIP_TRY(hipGetLastError());
+    units = b->unit_start.p;
   }
 
   ClassifyParams cp;
@@ -630,7 +671,8 @@ extern "C" int k2g_classify_device(k2g_batch *b, const k2g_classify_params *p, c
   cp.min_hit_groups = p->min_hit_groups;
   cp.quick = p->quick ? 1u : 0u;
   cp.tab_cap = p->table_slots ? (uint32_t) p->table_slots : 256u;
-  FragBatch in{d_seq, d_off, d_paired, (uint32_t) (p->paired != 0), n, order};
+  FragBatch in{d_seq, d_off,  d_paired, (uint32_t) (p->paired != 0), n, order, units,
+               units ? b->n_units.p : nullptr};
   if (p->want_probe_stats) {
     HIP_TRY(b->lookups.reserve(n));
     HIP_TRY(b->lines.reserve(n));
@@ -698,6 +740,8 @@ extern "C" int k2g_classify_device(k2g_batch *b, const k2g_classify_params *p, c
       FragBatch in2 = in;
       in2.n = n_space_only;
       in2.order = b->space_only.p;
+      in2.unit_start = nullptr;
+      in2.n_units = nullptr;
       HIP_TRY(hipMemsetAsync(b->cursor.p, 0, 8, s));
       const unsigned grid = (unsigned) std::min<uint64_t>(b->persistent_blocks, n_space_only);
       hipLaunchKernelGGL(k_classify_persistent, dim3(grid), dim3(64), 0, s, v, cp, in2, out2,
@@ -1312,7 +1356,11 @@ uint64_t k2g::plan_classify_bytes(const k2g_db *db, const k2g_classify_params *p
   b += region_bytes(2 * std::max<uint64_t>(space, 1), 4);         // runs
   if (p->want_hll) b += region_bytes(2 * std::max<uint64_t>(space, 1), 4);
   if (p->want_kmer_counts) b += region_bytes(v.node_count, 8);
-  if (sort) b += 3 * region_bytes(n, 4);                          // order, sort values, sorted keys
+  if (sort) {
+    b += 3 * region_bytes(n, 4);                                  // order, sort values, sorted keys
+    b += 2 * region_bytes(n + 1, 8) + region_bytes(n, 1);         // unit prefix, scan, flags
+    b += region_bytes(n + 1, 4) + region_bytes(1, 4);             // unit starts, unit count
+  }
   if (p->want_probe_stats) b += region_bytes(n, 4) + region_bytes(n, 8);
   b += 3 * region_bytes(n, 4);                                    // redo lists
   const uint64_t slots = std::min<uint64_t>(n, 256);              // dense-table redo, worst case

@@ -135,6 +135,9 @@ struct k2g_batch {
   k2g::DevBuf<unsigned char> temp;
   k2g::DevBuf<uint32_t> sort_keys, sort_keys_out, sort_vals;
   k2g::DevBuf<uint32_t> dense_counts, dense_lists, dense_lens;
+  k2g::DevBuf<uint64_t> unit_pos, unit_scan;
+  k2g::DevBuf<uint8_t> unit_flag;
+  k2g::DevBuf<uint32_t> unit_start, n_units;
   uint32_t dense_slots = 0;
   // host copies of the last results (hitlists and HLL records are written
   // by the compaction kernel straight into pinned memory)
@@ -157,6 +160,7 @@ struct k2g_batch {
     f(runs), f(recs), f(kmer_counts), f(cursor), f(lines), f(lookups);
     f(redo_count), f(space), f(space_only), f(temp), f(sort_keys), f(sort_keys_out), f(sort_vals);
     f(dense_counts), f(dense_lists), f(dense_lens);
+    f(unit_pos), f(unit_scan), f(unit_flag), f(unit_start), f(n_units);
   }
   // Allocates every device buffer from r from now on (null: hipMalloc).
   void bind_region(k2g::Region *r) {

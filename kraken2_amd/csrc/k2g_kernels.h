@@ -20,6 +20,11 @@ struct FragBatch {
   uint32_t paired_all;
   uint64_t n;
   const uint32_t *order;  // processing order for the persistent kernel (null = 0..n-1)
+  // Work units of the persistent kernel: unit u is order[unit_start[u] ..
+  // unit_start[u+1]), *n_units units in all (device memory).  Null: one
+  // fragment per unit.
+  const uint32_t *unit_start;
+  const uint32_t *n_units;
 };
 
 struct FragOut {

@@ -510,25 +510,31 @@ __global__ __launch_bounds__(64, 8) void k_classify_plain(DbView db, ClassifyPar
   classify_fragment(db, P, in, out, f, tab, lds.ring, &lds.bad, threadIdx.x);
 }
 
-// Persistent path: resident wavefronts pull fragments from a shared cursor in
-// the given order (longest first), so long and short reads share one launch.
+// Persistent path: resident wavefronts pull work units (runs of fragments of
+// similar length, longest first) from a shared cursor, so long and short
+// reads share one launch and the cursor sees one atomic per unit.
 __global__ __launch_bounds__(64, 8) void k_classify_persistent(DbView db, ClassifyParams P,
                                                              FragBatch in, FragOut out,
                                                              unsigned long long *cursor) {
   __shared__ WaveLds lds;
   LdsTable tab{lds.tkey, lds.tcnt, P.tab_cap};
   const int lane = threadIdx.x;
+  const uint64_t n_units = in.unit_start ? uniform32(*in.n_units) : in.n;
   // Every lane adds 1, so each fetch advances the cursor by exactly 64 and
-  // any lane's result / 64 is the wave's next fragment; no lane-0 branch
+  // any lane's result / 64 is the wave's next unit; no lane-0 branch
   // precedes the loop exit and the index is a scalar.  The next fetch is
-  // issued before the current fragment is classified so its latency hides
+  // issued before the current unit is classified so its latency hides
   // behind that work.
-  uint64_t i = uniform64((uint64_t) atomicAdd(cursor, 1ULL)) >> 6;
-  while (i < in.n) {
+  uint64_t u = uniform64((uint64_t) atomicAdd(cursor, 1ULL)) >> 6;
+  while (u < n_units) {
     const unsigned long long next = atomicAdd(cursor, 1ULL);
-    const uint64_t f = uniform64(in.order ? (uint64_t) in.order[i] : i);
-    classify_fragment(db, P, in, out, f, tab, lds.ring, &lds.bad, lane);
-    i = uniform64((uint64_t) next) >> 6;
+    const uint64_t lo = in.unit_start ? uniform32(in.unit_start[u]) : u;
+    const uint64_t hi = in.unit_start ? uniform32(in.unit_start[u + 1]) : u + 1;
+    for (uint64_t j = lo; j < hi; j++) {
+      const uint64_t f = uniform64(in.order ? (uint64_t) in.order[j] : j);
+      classify_fragment(db, P, in, out, f, tab, lds.ring, &lds.bad, lane);
+    }
+    u = uniform64((uint64_t) next) >> 6;
   }
 }
 

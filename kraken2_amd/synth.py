@@ -228,3 +228,4 @@ class ReadSet:
     def free(self):
         for p in (self.d_seq, self.d_off, self.d_paired):
             self.hip.free(p)
+        self.d_seq = self.d_off = self.d_paired = None

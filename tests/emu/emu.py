@@ -13,7 +13,7 @@ def load():
     vp = C.c_void_p
     u32, u64, i32, dbl = C.c_uint32, C.c_uint64, C.c_int32, C.c_double
     lib.emu_classify.argtypes = [vp, u64, u32, u32, vp, u32, u32, u32, u32, u64, u64, u64, dbl, i32, u32, u32,
-                                 vp, vp, vp, u32, u64, vp, C.c_int, C.c_uint, vp, vp] + [vp] * 9
+                                 vp, vp, vp, u32, u64, vp, C.c_int, C.c_uint, vp, vp, vp, vp] + [vp] * 9
     lib.emu_collectives.restype = C.c_ulonglong
     return lib
 
@@ -62,7 +62,7 @@ def bounds(db, off, paired_flags):
 
 
 def classify(lib, db, seq, off, paired_flags, *, conf=0.0, min_groups=0, quick=False, tab_cap=256, mode=1,
-             grid=4, hll=False, kmer_counts=False, cap_pct=None):
+             grid=4, hll=False, kmer_counts=False, cap_pct=None, unit_elements=None):
     """cap_pct None reserves each fragment's exact element count for its runs;
     otherwise min(e, e * cap_pct // 100 + 64) as the library does."""
     n = (len(off) - 1) // 2
@@ -74,6 +74,13 @@ def classify(lib, db, seq, off, paired_flags, *, conf=0.0, min_groups=0, quick=F
     run_off[1:] = np.cumsum(caps)
     tot = int(run_off[-1])
     order = np.argsort(-b.astype(np.int64), kind="stable").astype(np.uint32)
+    units = n_units = None
+    if unit_elements:  # windows of unit_elements elements over the longest-first order
+        pre = np.concatenate([[0], np.cumsum(np.minimum(b[order], 0xFFFFFFFF).astype(np.uint64))[:-1]])
+        w = pre // np.uint64(unit_elements)
+        starts = np.flatnonzero(np.concatenate([[True], w[1:] != w[:-1]])) if n else np.zeros(0, np.int64)
+        units = np.concatenate([starts, [n]]).astype(np.uint32)
+        n_units = np.array([len(starts)], np.uint32)
     outs = {k: np.zeros(n, np.uint32) for k in ("call", "groups", "total", "n_runs", "status", "n_recs")}
     runs = np.zeros(2 * max(tot, 1), np.uint32)
     recs = np.zeros(2 * max(tot, 1), np.uint32) if hll else None
@@ -83,10 +90,13 @@ def classify(lib, db, seq, off, paired_flags, *, conf=0.0, min_groups=0, quick=F
                      len(db["parent"]), db["k"], db["l"], db["dna"], db["mask"], db["toggle"], db["min_hash"],
                      conf, min_groups, int(quick), tab_cap, sb.ctypes.data, off.ctypes.data, pf.ctypes.data, 0,
                      n, order.ctypes.data, mode, grid, run_off.ctypes.data, caps.ctypes.data,
+                     units.ctypes.data if units is not None else None,
+                     n_units.ctypes.data if n_units is not None else None,
                      *[outs[k].ctypes.data for k in ("call", "groups", "total", "n_runs", "status", "n_recs")],
                      runs.ctypes.data, recs.ctypes.data if recs is not None else None,
                      kc.ctypes.data if kc is not None else None)
     outs["run_off"] = run_off
+    outs["units"] = units
     outs["runs"] = runs
     outs["recs"] = recs
     outs["kmer_counts"] = kc

@@ -45,7 +45,8 @@ int emu_classify(const uint32_t *cells, uint64_t capacity, uint32_t kb, uint32_t
                  uint64_t min_hash, double conf, int32_t min_groups, uint32_t quick, uint32_t tab_cap,
                  const char *seq, const uint64_t *off, const uint8_t *paired, uint32_t paired_all, uint64_t n,
                  const uint32_t *order, int mode, unsigned grid, const uint64_t *run_off,
-                 const uint64_t *run_cap, uint32_t *call,
+                 const uint64_t *run_cap, const uint32_t *unit_start, const uint32_t *n_units,
+                 uint32_t *call,
                  uint32_t *groups, uint32_t *total, uint32_t *n_runs, uint32_t *status, uint32_t *n_recs,
                  uint32_t *runs, uint32_t *recs, unsigned long long *kmer_counts) {
   using namespace k2g;
@@ -67,7 +68,7 @@ int emu_classify(const uint32_t *cells, uint64_t capacity, uint32_t kb, uint32_t
   v.toggle_mask = toggle & v.lmer_mask;
   v.min_hash = min_hash;
   ClassifyParams P{conf, min_groups, quick, tab_cap};
-  FragBatch in{seq, off, paired, paired_all, n, order};
+  FragBatch in{seq, off, paired, paired_all, n, order, unit_start, n_units};
   FragOut out{call,    groups, total, n_runs, status,      n_recs,  run_off,
               run_cap, runs,   recs,  kmer_counts, nullptr, nullptr};
   if (mode == 0) {

@@ -26,11 +26,11 @@ def emu_lib():
 
 
 def _check(emu, lib, d, seq, off, pflags, *, conf=0.0, g=0, quick=False, tab=256, mode=1, hll=False,
-           cap_pct=None):
+           cap_pct=None, unit_elements=None):
     import kraken2_amd as k2
     db = emu.read_db(d)
     out = emu.classify(lib, db, seq, off, pflags, conf=conf, min_groups=g, quick=quick, tab_cap=tab, mode=mode,
-                       hll=hll, kmer_counts=hll, cap_pct=cap_pct)
+                       hll=hll, kmer_counts=hll, cap_pct=cap_pct, unit_elements=unit_elements)
     orc = Oracle()
     h = orc.open(d)
     n = len(pflags)
@@ -53,14 +53,19 @@ def _check(emu, lib, d, seq, off, pflags, *, conf=0.0, g=0, quick=False, tab=256
 
 
 
-@pytest.mark.parametrize("mode", [0, 1])
-def test_emulated_single_end(emu_lib, synth_db, mode):
+@pytest.mark.parametrize("mode,units", [(0, None), (1, None), (1, 300)])
+def test_emulated_single_end(emu_lib, synth_db, mode, units):
+    """Plain, persistent with one fragment per claim, and persistent with
+    work units of about 300 elements (several short reads per claim)."""
     import kraken2_amd as k2
     emu, lib = emu_lib
     _, genomes = read_fasta(os.path.join(synth_db, "library.fna"))
     reads = edge_reads(genomes, 60, seed=31 + mode)
     seq, off = k2.pack_fragments(reads)
-    _check(emu, lib, synth_db, seq, off, np.zeros(len(reads), np.uint8), g=2, mode=mode)
+    out = _check(emu, lib, synth_db, seq, off, np.zeros(len(reads), np.uint8), g=2, mode=mode,
+                 unit_elements=units)
+    if units:
+        assert 1 < len(out["units"]) - 1 < len(reads)
 
 
 def test_emulated_paired_confidence(emu_lib, synth_db):

@@ -160,3 +160,14 @@ def test_digests_plain_vs_persistent(env):
     calls = out.call
     i, j = int(np.argmax(calls != calls[0])), 0
     assert calls[i] != calls[j] and d1[i] != d1[j]
+
+
+@pytest.mark.parametrize("unit_elements", [1, 200, 4096, 1 << 30])
+def test_work_unit_sizes(env, unit_elements):
+    """Persistent work units from one fragment each up to the whole batch."""
+    k2, db, batch, orc, h, genomes = env
+    reads = edge_reads(genomes, 400, seed=700) + chimeric_reads(genomes, 20, seed=701, length=5000)
+    seq, off = k2.pack_fragments(reads)
+    out = batch.classify(seq, off, min_hit_groups=2, mode=1, unit_elements=unit_elements)
+    ref = orc.classify(h, seq, off, min_hit_groups=2)
+    _compare(k2, db, out, ref, len(reads))
