@@ -21,11 +21,33 @@
 #include "k2g_kernels.h"
 
 namespace k2g {
-__global__ void k_classify_plain(DbView, ClassifyParams, FragBatch, FragOut);
-__global__ void k_classify_persistent(DbView, ClassifyParams, FragBatch, FragOut,
-                                      unsigned long long *);
-__global__ void k_classify_dense(DbView, ClassifyParams, FragBatch, FragOut, const uint32_t *,
-                                 uint32_t, DenseScratch);
+#define K2G_DECLARE_VARIANT(SUFFIX)                                                      \
+  __global__ void k_classify_plain##SUFFIX(DbView, ClassifyParams, FragBatch, FragOut);   \
+  __global__ void k_classify_persistent##SUFFIX(DbView, ClassifyParams, FragBatch, FragOut, \
+                                                unsigned long long *);
+K2G_DECLARE_VARIANT(_dna)
+K2G_DECLARE_VARIANT(_dna_x)
+K2G_DECLARE_VARIANT(_prot)
+#undef K2G_DECLARE_VARIANT
+__global__ void k_classify_dense_dna(DbView, ClassifyParams, FragBatch, FragOut, const uint32_t *,
+                                     uint32_t, DenseScratch);
+__global__ void k_classify_dense_prot(DbView, ClassifyParams, FragBatch, FragOut, const uint32_t *,
+                                      uint32_t, DenseScratch);
+
+// The kernel variants a batch launches: nucleotide without per-hit extras
+// (the hot path), nucleotide with extras, translated search.
+struct ClassifyKernels {
+  void (*plain)(DbView, ClassifyParams, FragBatch, FragOut);
+  void (*persistent)(DbView, ClassifyParams, FragBatch, FragOut, unsigned long long *);
+  void (*dense)(DbView, ClassifyParams, FragBatch, FragOut, const uint32_t *, uint32_t,
+                DenseScratch);
+};
+inline int variant_index(bool dna, bool extras) { return dna ? (extras ? 1 : 0) : 2; }
+const ClassifyKernels kVariants[3] = {
+    {k_classify_plain_dna, k_classify_persistent_dna, k_classify_dense_dna},
+    {k_classify_plain_dna_x, k_classify_persistent_dna_x, k_classify_dense_dna},
+    {k_classify_plain_prot, k_classify_persistent_prot, k_classify_dense_prot},
+};
 __global__ void k_get_batch(DbView, const uint64_t *, uint32_t *, uint64_t);
 __global__ void k_max_value(DbView, unsigned int *);
 __global__ void k_build(BuildView, const char *, const SubBlock *, uint64_t);
@@ -508,14 +530,15 @@ extern "C" int k2g_batch_create(k2g_db *db, k2g_batch **out) {
   hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&b->ev0);
   if (e == hipSuccess) e = hipEventCreate(&b->ev1);
-  int per_cu = 0;
-  if (e == hipSuccess)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_classify_persistent, 64, 0);
+  for (int vi = 0; vi < 3 && e == hipSuccess; vi++) {
+    int per_cu = 0;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kVariants[vi].persistent, 64, 0);
+    b->persistent_blocks[vi] = std::max(1, per_cu) * std::max(1, db->num_cus);
+  }
   if (e != hipSuccess) {
     k2g_batch_destroy(b);
     return fail(K2G_ESOFTWARE, std::string("batch setup: ") + hipGetErrorString(e));
   }
-  b->persistent_blocks = std::max(1, per_cu) * std::max(1, db->num_cus);
   *out = b;
   return K2G_OK;
 }
@@ -684,14 +707,16 @@ extern "C" int k2g_classify_device(k2g_batch *b, const k2g_classify_params *p, c
               p->want_probe_stats ? b->lookups.p : nullptr,
               p->want_probe_stats ? b->lines.p : nullptr};
 
+  const int vi = variant_index(v.dna != 0, p->want_hll || p->want_kmer_counts ||
+                                               p->want_probe_stats);
+  const ClassifyKernels &K = kVariants[vi];
   HIP_TRY(hipEventRecord(b->ev0, s));
   if (p->mode == 0) {
-    hipLaunchKernelGGL(k_classify_plain, dim3((unsigned) n), dim3(64), 0, s, v, cp, in, out);
+    hipLaunchKernelGGL(K.plain, dim3((unsigned) n), dim3(64), 0, s, v, cp, in, out);
   } else {
     HIP_TRY(hipMemsetAsync(b->cursor.p, 0, 8, s));
-    const unsigned grid = (unsigned) std::min<uint64_t>(b->persistent_blocks, n);
-    hipLaunchKernelGGL(k_classify_persistent, dim3(grid), dim3(64), 0, s, v, cp, in, out,
-                       b->cursor.p);
+    const unsigned grid = (unsigned) std::min<uint64_t>(b->persistent_blocks[vi], n);
+    hipLaunchKernelGGL(K.persistent, dim3(grid), dim3(64), 0, s, v, cp, in, out, b->cursor.p);
   }
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(b->ev1, s));
@@ -743,8 +768,8 @@ extern "C" int k2g_classify_device(k2g_batch *b, const k2g_classify_params *p, c
       in2.unit_start = nullptr;
       in2.n_units = nullptr;
       HIP_TRY(hipMemsetAsync(b->cursor.p, 0, 8, s));
-      const unsigned grid = (unsigned) std::min<uint64_t>(b->persistent_blocks, n_space_only);
-      hipLaunchKernelGGL(k_classify_persistent, dim3(grid), dim3(64), 0, s, v, cp, in2, out2,
+      const unsigned grid = (unsigned) std::min<uint64_t>(b->persistent_blocks[vi], n_space_only);
+      hipLaunchKernelGGL(K.persistent, dim3(grid), dim3(64), 0, s, v, cp, in2, out2,
                          b->cursor.p);
       HIP_TRY(hipGetLastError());
     }
@@ -762,7 +787,7 @@ extern "C" int k2g_classify_device(k2g_batch *b, const k2g_classify_params *p, c
       b->dense_slots = slots;
     }
     DenseScratch ds{b->dense_counts.p, b->dense_lists.p, b->dense_lens.p, slots};
-    hipLaunchKernelGGL(k_classify_dense, dim3(slots), dim3(64), 0, s, v, cp, in, out2, b->redo.p,
+    hipLaunchKernelGGL(K.dense, dim3(slots), dim3(64), 0, s, v, cp, in, out2, b->redo.p,
                        n_redo, ds);
     HIP_TRY(hipGetLastError());
     b->redone = n_redo;

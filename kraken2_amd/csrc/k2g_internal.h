@@ -149,7 +149,7 @@ struct k2g_batch {
   k2g_classify_params params{};
   uint64_t redone = 0, space_redone = 0;
   double kernel_ms = 0;
-  int persistent_blocks = 0;
+  int persistent_blocks[3] = {0, 0, 0};  // per kernel variant: resident waves
 
   template <class F>
   void for_each_device_buf(F f) {

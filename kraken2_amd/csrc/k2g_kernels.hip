@@ -209,12 +209,14 @@ __device__ void push_marker(const FragOut &out, uint64_t base, uint64_t cap, Run
 }
 
 // Scans one stream (a DNA mate, or one reading frame of a mate) and appends
-// its k-mer elements to the fragment's runs.
-template <class Table>
+// its k-mer elements to the fragment's runs.  DNA: a nucleotide database
+// (else translated search); EXTRAS: per-hit HLL records, per-taxon counts or
+// probe statistics may be requested (their code is compiled out otherwise).
+template <bool DNA, bool EXTRAS, class Table>
 __device__ void scan_stream(const DbView &db, const ClassifyParams &P, const FragOut &out,
                             uint64_t base, uint64_t cap, const char *m, uint64_t L, int fr,
                             RunState &st, Table &tab, uint64_t *ring, int *bad, int lane) {
-  const bool dna = db.dna != 0;
+  constexpr bool dna = DNA;
   const uint64_t n = dna ? L : frame_len(L, fr);
   const uint32_t k = db.k, l = db.l;
   const uint64_t lbits = (1ULL << l) - 1ULL;
@@ -290,7 +292,7 @@ __device__ void scan_stream(const DbView &db, const ClassifyParams &P, const Fra
     uint32_t taxon = 0;
     if (lookup && !skip) {
       taxon = table_get(db, mn);
-      if (out.lookups) {
+      if (EXTRAS && out.lookups) {
         st.my_lookups++;
         st.my_lines += table_lines(db, mn);
       }
@@ -314,7 +316,7 @@ __device__ void scan_stream(const DbView &db, const ClassifyParams &P, const Fra
     }
 
     // per-taxon k-mer counts and HLL records for LOOKUP hits
-    if ((HIT >> lane) & 1ULL) {
+    if (EXTRAS && ((HIT >> lane) & 1ULL)) {
       if (out.kmer_counts) atomicAdd(&out.kmer_counts[taxon], 1ULL);
       if (out.recs) {
         const uint32_t r = st.nrec + __popcll(HIT & lanemask_lt(lane));
@@ -404,14 +406,14 @@ __device__ __forceinline__ uint32_t reduce_sum(uint32_t v) {
   return v;
 }
 
-template <class Table>
+template <bool DNA, bool EXTRAS, class Table>
 __device__ void classify_fragment(const DbView &db, const ClassifyParams &P, const FragBatch &in,
                                   const FragOut &out, uint64_t f, Table &tab, uint64_t *ring,
                                   int *bad, int lane) {
   const uint64_t o0 = uniform64(in.off[2 * f]), o1 = uniform64(in.off[2 * f + 1]),
                  o2 = uniform64(in.off[2 * f + 2]);
   const bool paired = uniform32(in.paired ? in.paired[f] : in.paired_all) != 0;
-  const bool tx = db.dna == 0;
+  constexpr bool tx = !DNA;
   const int nframes = tx ? 6 : 1;
   const uint64_t base = uniform64(out.run_off[f]);
   const uint64_t cap = uniform64(out.run_cap[f]);
@@ -424,7 +426,7 @@ __device__ void classify_fragment(const DbView &db, const ClassifyParams &P, con
     const char *m = in.seq + (mate ? o1 : o0);
     const uint64_t L = mate ? o2 - o1 : o1 - o0;
     for (int fr = 0; fr < nframes; fr++) {
-      scan_stream(db, P, out, base, cap, m, L, fr, st, tab, ring, bad, lane);
+      scan_stream<DNA, EXTRAS>(db, P, out, base, cap, m, L, fr, st, tab, ring, bad, lane);
       if (st.done) break;
       if (tx && fr != 5) push_marker(out, base, cap, st, kCodeFrame, tab, bad, lane);
     }
@@ -471,7 +473,7 @@ __device__ void classify_fragment(const DbView &db, const ClassifyParams &P, con
   }
   if (call && (int64_t) st.groups < (int64_t) P.min_hit_groups) call = 0;
   __syncthreads();
-  if (out.lookups) {
+  if (EXTRAS && out.lookups) {
     uint64_t lk = st.my_lookups, ln = st.my_lines;
     for (int d = 1; d < kWave; d <<= 1) {
       lk += shfl64(lk, lane ^ d);
@@ -501,21 +503,23 @@ struct WaveLds {
 };
 
 // Plain path: one wavefront per fragment, one launch per batch.
-__global__ __launch_bounds__(64, 8) void k_classify_plain(DbView db, ClassifyParams P, FragBatch in,
-                                                        FragOut out) {
+template <bool DNA, bool EXTRAS>
+__device__ __forceinline__ void plain_body(const DbView &db, const ClassifyParams &P,
+                                           const FragBatch &in, const FragOut &out) {
   __shared__ WaveLds lds;
   const uint64_t f = blockIdx.x;
   if (f >= in.n) return;
   LdsTable tab{lds.tkey, lds.tcnt, P.tab_cap};
-  classify_fragment(db, P, in, out, f, tab, lds.ring, &lds.bad, threadIdx.x);
+  classify_fragment<DNA, EXTRAS>(db, P, in, out, f, tab, lds.ring, &lds.bad, threadIdx.x);
 }
 
 // Persistent path: resident wavefronts pull work units (runs of fragments of
 // similar length, longest first) from a shared cursor, so long and short
 // reads share one launch and the cursor sees one atomic per unit.
-__global__ __launch_bounds__(64, 8) void k_classify_persistent(DbView db, ClassifyParams P,
-                                                             FragBatch in, FragOut out,
-                                                             unsigned long long *cursor) {
+template <bool DNA, bool EXTRAS>
+__device__ __forceinline__ void persistent_body(const DbView &db, const ClassifyParams &P,
+                                                const FragBatch &in, const FragOut &out,
+                                                unsigned long long *cursor) {
   __shared__ WaveLds lds;
   LdsTable tab{lds.tkey, lds.tcnt, P.tab_cap};
   const int lane = threadIdx.x;
@@ -532,7 +536,7 @@ __global__ __launch_bounds__(64, 8) void k_classify_persistent(DbView db, Classi
     const uint64_t hi = in.unit_start ? uniform32(in.unit_start[u + 1]) : u + 1;
     for (uint64_t j = lo; j < hi; j++) {
       const uint64_t f = uniform64(in.order ? (uint64_t) in.order[j] : j);
-      classify_fragment(db, P, in, out, f, tab, lds.ring, &lds.bad, lane);
+      classify_fragment<DNA, EXTRAS>(db, P, in, out, f, tab, lds.ring, &lds.bad, lane);
     }
     u = uniform64((uint64_t) next) >> 6;
   }
@@ -540,16 +544,47 @@ __global__ __launch_bounds__(64, 8) void k_classify_persistent(DbView db, Classi
 
 // Overflow path: fragments whose distinct hit taxa exceeded the LDS table are
 // classified again with a dense per-taxon count array (one per block).
-__global__ __launch_bounds__(64) void k_classify_dense(DbView db, ClassifyParams P, FragBatch in,
-                                                        FragOut out, const uint32_t *redo,
-                                                        uint32_t n_redo, DenseScratch scratch) {
+template <bool DNA>
+__device__ __forceinline__ void dense_body(const DbView &db, const ClassifyParams &P,
+                                           const FragBatch &in, const FragOut &out,
+                                           const uint32_t *redo, uint32_t n_redo,
+                                           const DenseScratch &scratch) {
   __shared__ WaveLds lds;
   const int lane = threadIdx.x;
   const uint32_t slot = blockIdx.x;
   DenseTable tab{scratch.counts + (uint64_t) slot * db.node_count,
                  scratch.lists + (uint64_t) slot * db.node_count, scratch.lens + slot};
   for (uint32_t i = blockIdx.x; i < n_redo; i += gridDim.x)
-    classify_fragment(db, P, in, out, uniform32(redo[i]), tab, lds.ring, &lds.bad, lane);
+    classify_fragment<DNA, true>(db, P, in, out, uniform32(redo[i]), tab, lds.ring, &lds.bad, lane);
+}
+
+// Kernel variants: _dna (nucleotide, no per-hit extras: the hot path),
+// _dna_x (nucleotide with HLL records / counts / probe statistics), _prot
+// (translated search, extras available).
+#define K2G_CLASSIFY_VARIANT(SUFFIX, DNA, EXTRAS)                                              \
+  __global__ __launch_bounds__(64, 8) void k_classify_plain##SUFFIX(                          \
+      DbView db, ClassifyParams P, FragBatch in, FragOut out) {                                \
+    plain_body<DNA, EXTRAS>(db, P, in, out);                                                   \
+  }                                                                                            \
+  __global__ __launch_bounds__(64, 8) void k_classify_persistent##SUFFIX(                     \
+      DbView db, ClassifyParams P, FragBatch in, FragOut out, unsigned long long *cursor) {    \
+    persistent_body<DNA, EXTRAS>(db, P, in, out, cursor);                                      \
+  }
+K2G_CLASSIFY_VARIANT(_dna, true, false)
+K2G_CLASSIFY_VARIANT(_dna_x, true, true)
+K2G_CLASSIFY_VARIANT(_prot, false, true)
+#undef K2G_CLASSIFY_VARIANT
+
+__global__ __launch_bounds__(64) void k_classify_dense_dna(DbView db, ClassifyParams P, FragBatch in,
+                                                            FragOut out, const uint32_t *redo,
+                                                            uint32_t n_redo, DenseScratch scratch) {
+  dense_body<true>(db, P, in, out, redo, n_redo, scratch);
+}
+__global__ __launch_bounds__(64) void k_classify_dense_prot(DbView db, ClassifyParams P,
+                                                             FragBatch in, FragOut out,
+                                                             const uint32_t *redo, uint32_t n_redo,
+                                                             DenseScratch scratch) {
+  dense_body<false>(db, P, in, out, redo, n_redo, scratch);
 }
 
 // KeyValueStore::GetBatch on the device (one lane per key).

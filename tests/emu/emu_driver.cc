@@ -71,11 +71,21 @@ int emu_classify(const uint32_t *cells, uint64_t capacity, uint32_t kb, uint32_t
   FragBatch in{seq, off, paired, paired_all, n, order, unit_start, n_units};
   FragOut out{call,    groups, total, n_runs, status,      n_recs,  run_off,
               run_cap, runs,   recs,  kmer_counts, nullptr, nullptr};
+  // the variant the library would launch for this database and these outputs
+  const bool extras = recs || kmer_counts;
   if (mode == 0) {
-    run_grid((unsigned) n, [&] { k_classify_plain(v, P, in, out); });
+    run_grid((unsigned) n, [&] {
+      if (!dna) k_classify_plain_prot(v, P, in, out);
+      else if (extras) k_classify_plain_dna_x(v, P, in, out);
+      else k_classify_plain_dna(v, P, in, out);
+    });
   } else {
     unsigned long long cursor = 0;
-    run_grid(grid, [&] { k_classify_persistent(v, P, in, out, &cursor); });
+    run_grid(grid, [&] {
+      if (!dna) k_classify_persistent_prot(v, P, in, out, &cursor);
+      else if (extras) k_classify_persistent_dna_x(v, P, in, out, &cursor);
+      else k_classify_persistent_dna(v, P, in, out, &cursor);
+    });
   }
   return 0;
 }
