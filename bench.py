@@ -420,6 +420,19 @@ def oracle_check(k2, db, tmp, seq, off, pflags, work):
     return mism, dig
 
 
+def profiled_traffic(workload):
+    """HBM bytes per launch of the persistent kernel measured by counters for
+    this workload (profiles/traffic.txt), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 4 and not line.startswith("#") and f[0] == workload:
+            return int(f[2]) + int(f[3])
+    return None
+
+
 def copy_calibration():
     """1 GiB device-to-device copies through torch's copy kernel: the read and
     write byte counters of these dispatches, divided by 1 GiB, give the
@@ -664,7 +677,9 @@ def main():
         "hitlist_runs": n_runs_total,
         "redone": {"run_space": space_redone, "dense_table": dense_redone},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": profiled_traffic(a.workload) if a.scale == 1.0 else None,
+                     "algorithmic_bytes": int(alg_bytes)},
         "gather": {"classify_lines_per_s": round(probe_rate, 1), "ceiling_lines_per_s": round(ceil_rate, 1),
                    "frac_of_ceiling": round(probe_rate / ceil_rate, 4) if ceil_rate else None,
                    "ceiling": "independent 16-byte loads at random 128-byte lines of the resident table"},
