@@ -81,7 +81,8 @@ typedef struct {
                              * count (plus 64); fragments that need more are classified again
                              * into exact space.  0 = 25; 100 = always exact (never redone). */
   int32_t unit_elements;    /* persistent mode: fragments (longest first) are handed to wavefronts
-                             * in units of about this many k-mer elements; 0 = 4096 */
+                             * in units of about this many k-mer elements (fewer when a batch is
+                             * too small to give every resident wave four units); 0 = 4096 */
 } k2g_classify_params;
 
 /* Fragment i: mate 1 = seq[off[2i], off[2i+1]), mate 2 = seq[off[2i+1],

@@ -419,10 +419,15 @@ __global__ void k_bounds(const uint64_t *off, const uint8_t *paired, uint32_t pa
 }
 
 // Work units over the longest-first order: position j starts a unit when its
-// element prefix enters a new window of T elements.
-__global__ void k_unit_flags(const uint64_t *prefix, uint64_t n, uint64_t T, uint8_t *flag) {
+// element prefix enters a new window of T elements.  T is the requested unit
+// size, lowered for small batches so that every resident wave gets about four
+// units (prefix[n] is the batch's total).
+__global__ void k_unit_flags(const uint64_t *prefix, uint64_t n, uint64_t T_max, uint64_t waves,
+                             uint8_t *flag) {
   const uint64_t j = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
+  uint64_t T = prefix[n] / (4 * waves);
+  T = T < 1 ? 1 : (T > T_max ? T_max : T);
   flag[j] = (j == 0 || prefix[j] / T != prefix[j - 1] / T) ? 1 : 0;
 }
 
@@ -676,8 +681,10 @@ extern "C" int k2g_classify_device(k2g_batch *b, const k2g_classify_params *p, c
     HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, ub, b->unit_pos.p, b->unit_scan.p, n + 1, s));
     HIP_TRY(b->temp.reserve(ub));
     HIP_TRY(hipcub::DeviceScan::ExclusiveSum(b->temp.p, ub, b->unit_pos.p, b->unit_scan.p, n + 1, s));
+    const int vi_units = variant_index(v.dna != 0, p->want_hll || p->want_kmer_counts ||
+                                                         p->want_probe_stats);
     hipLaunchKernelGGL(k_unit_flags, dim3(gi), dim3(256), 0, s, b->unit_scan.p, n, T,
-                       b->unit_flag.p);
+                       (uint64_t) b->persistent_blocks[vi_units], b->unit_flag.p);
     size_t fb = 0;
     HIP_TRY(hipcub::DeviceSelect::Flagged(nullptr, fb, hipcub::CountingInputIterator<uint32_t>(0),
                                           b->unit_flag.p, b->unit_start.p, b->n_units.p, (int) n, s));

@@ -143,6 +143,20 @@ __device__ __forceinline__ uint64_t lanemask_lt(int lane) { return (1ULL << lane
 __device__ __forceinline__ uint64_t lanemask_le(int lane) { return ~0ULL >> (63 - lane); }
 __device__ __forceinline__ int top_lane(uint64_t m) { return 63 - __clzll((long long) m); }
 
+// Each bit of a wave-uniform x twice: bit i -> bits 2i and 2i+1 (one scalar
+// instruction on the device).
+__device__ __forceinline__ uint64_t replicate_bits(uint32_t x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint64_t r;
+  asm("s_bitreplicate_b64_b32 %0, %1" : "=s"(r) : "s"(x));
+  return r;
+#else
+  uint64_t r = 0;
+  for (int i = 0; i < 32; i++) r |= (uint64_t) ((x >> i) & 1u) * (3ULL << (2 * i));
+  return r;
+#endif
+}
+
 // Spread the low 32 bits so bit t lands on bit 2t.
 __device__ __forceinline__ uint64_t spread2(uint64_t x) {
   x &= 0xFFFFFFFFULL;

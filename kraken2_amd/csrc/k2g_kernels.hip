@@ -208,177 +208,313 @@ __device__ void push_marker(const FragOut &out, uint64_t base, uint64_t cap, Run
   st.run_count = 1;
 }
 
+// A workgroup is one wavefront, whose LDS operations complete in program
+// order: ordering the lanes' LDS accesses only needs a compiler barrier.
+__device__ __forceinline__ void wave_sync() { __builtin_amdgcn_wave_barrier(); }
+
+constexpr uint64_t kEvenBits = 0x5555555555555555ULL;
+constexpr uint64_t kOddBits = 0xAAAAAAAAAAAAAAAAULL;
+
+// Per-wave LDS.  Lookups of up to kBatchChunks chunks are queued (qkey) and
+// probed together (qval), so every lane of the wave probes instead of the
+// ~1/3 of lanes whose k-mer starts a new minimizer.
+constexpr int kQueue = 192;       // queued lookups per probe batch (>= 64 + 64)
+constexpr int kBatchChunks = 8;   // chunks per probe batch
+
+struct WaveLds {
+  uint64_t ring[128];             // window candidates by position & 127
+  uint64_t qkey[kQueue];          // minimizers to look up, in k-mer order
+  uint32_t qval[kQueue];          // their table values
+  uint64_t cmask[kBatchChunks][3];  // per queued chunk: non-ambiguous, LOOKUP, k-mer-end masks
+  uint32_t cqb[kBatchChunks];     // per queued chunk: queue index of its first lookup
+  uint32_t tkey[256];             // LDS hit table
+  uint32_t tcnt[256];
+  int bad;
+};
+
+// CompactHashTable::Get (src/compact_hash.h:354-374, 493-500), written for a
+// wave where every lane probes: each step loads a 16-byte group of four cells
+// and decides all four without branching; lanes leave the loop at different
+// steps.  Keys whose hash is below min_hash are not probed (src/classify.cc:
+// 819-823) and read as 0.
+__device__ __forceinline__ uint32_t probe_key(const DbView &db, uint64_t key) {
+  const uint64_t hc = fmix64(key);
+  if (hc < db.min_hash) return 0u;
+  const uint32_t vb = db.value_bits;
+  const uint32_t vmask = (1u << vb) - 1u;
+  const uint32_t ck = (uint32_t) (hc >> (64 - db.key_bits));
+  uint64_t idx = mod_capacity(hc, db.capacity, db.barrett);
+  uint64_t left = db.capacity;  // cells still to examine before a full wrap
+  for (;;) {
+    const uint64_t group = idx & ~3ULL;
+    const u32x4 v = *(const u32x4 *) (db.cells + group);
+    const uint32_t off = (uint32_t) (idx - group);
+    uint64_t avail = 4u - off;                     // cells of this group from idx on
+    if (db.capacity - idx < avail) avail = db.capacity - idx;
+    if (left < avail) avail = left;
+    const uint32_t c4[4] = {v.x, v.y, v.z, v.w};
+    uint32_t stop = 0, val = 0;
+#pragma unroll
+    for (int j = 3; j >= 0; j--) {  // the first stopping cell in probe order wins
+      const uint32_t c = c4[j];
+      const bool in = (uint32_t) j >= off && (uint64_t) (j - off) < avail;
+      const bool empty = (c & vmask) == 0u;
+      const bool hit = (c >> vb) == ck;
+      const bool st = in && (empty || hit);
+      val = st ? (empty ? 0u : (c & vmask)) : val;
+      stop |= st ? 1u : 0u;
+    }
+    if (stop) return val;
+    left -= avail;
+    if (left == 0) return 0u;
+    idx += avail;
+    if (idx >= db.capacity) idx = 0;
+  }
+}
+
 // Scans one stream (a DNA mate, or one reading frame of a mate) and appends
 // its k-mer elements to the fragment's runs.  DNA: a nucleotide database
 // (else translated search); EXTRAS: per-hit HLL records, per-taxon counts or
 // probe statistics may be requested (their code is compiled out otherwise).
+//
+// The stream is processed in probe batches of up to kBatchChunks chunks of
+// 64 k-mer end positions:
+//   A. each chunk's minimizers and LOOKUP/REPEAT/AMBIG decisions
+//      (src/classify.cc:797-838) are computed and its LOOKUP minimizers
+//      appended to the wave's queue, with the chunk's masks;
+//   B. every lane probes queued keys (64 per round);
+//   C. the chunks are replayed in order: each k-mer takes the value of the
+//      latest LOOKUP at or before it (src/classify.cc:846-888) and the wave
+//      run-length encodes the elements and counts hits.
 template <bool DNA, bool EXTRAS, class Table>
 __device__ void scan_stream(const DbView &db, const ClassifyParams &P, const FragOut &out,
                             uint64_t base, uint64_t cap, const char *m, uint64_t L, int fr,
-                            RunState &st, Table &tab, uint64_t *ring, int *bad, int lane) {
+                            RunState &st, Table &tab, WaveLds &lds, int *bad, int lane) {
   constexpr bool dna = DNA;
   const uint64_t n = dna ? L : frame_len(L, fr);
   const uint32_t k = db.k, l = db.l;
+  const uint32_t w = k - l + 1;  // l-mers per k-mer window
   const uint64_t lbits = (1ULL << l) - 1ULL;
   int64_t last_amb = -1;
   uint64_t last_min = ~0ULL;
   uint32_t last_taxon = 0;
-  uint64_t pp0 = 0, pp1 = 0, pp2 = 0, pp3 = 0;  // previous chunk's planes
-  for (uint64_t c = 0; c < n; c += kWave) {
-    const uint64_t p = c + lane;
-    const bool inb = p < n;
-    uint32_t code = 0xFFu;
-    if (inb) code = dna ? dna_code((uint8_t) m[p]) : frame_code(m, L, fr, p);
-    const bool amb = inb && code == 0xFFu;
-    const bool good = inb && !amb;
-    const uint64_t P0 = __ballot(good && (code & 1u));
-    const uint64_t P1 = __ballot(good && (code & 2u));
-    const uint64_t P2 = dna ? 0 : __ballot(good && (code & 4u));
-    const uint64_t P3 = dna ? 0 : __ballot(good && (code & 8u));
-    const uint64_t A = __ballot(amb);
-    const uint64_t mA = A & lanemask_le(lane);
-    const int64_t lastN = mA ? (int64_t) (c + top_lane(mA)) : last_amb;
+  uint64_t pp0 = 0, pp1 = 0, pp2 = 0, pp3 = 0;  // previous chunk's planes (translated)
+  uint64_t pBEh = 0, pLEh = 0;                   // previous chunk's upper stream words (DNA)
+  const uint64_t lmask2 = (1ULL << (2 * l)) - 1ULL;
+  uint64_t c = 0;
+  while (c < n) {
+    // ---- A: scan chunks, queue their lookups
+    uint32_t qn = 0, g = 0;
+    while (c < n && g < (uint32_t) kBatchChunks && qn + kWave <= (uint32_t) kQueue) {
+      const uint64_t p = c + lane;
+      const bool inb = p < n;
+      uint32_t code = 0xFFu;
+      if (inb) code = dna ? dna_code((uint8_t) m[p]) : frame_code(m, L, fr, p);
+      const bool amb = inb && code == 0xFFu;
+      const bool good = inb && !amb;
+      const uint64_t P0 = __ballot(good && (code & 1u));
+      const uint64_t P1 = __ballot(good && (code & 2u));
+      const uint64_t P2 = dna ? 0 : __ballot(good && (code & 4u));
+      const uint64_t P3 = dna ? 0 : __ballot(good && (code & 8u));
+      const uint64_t A = __ballot(amb);
+      const uint64_t mA = A & lanemask_le(lane);
+      const int64_t lastN = mA ? (int64_t) (c + top_lane(mA)) : last_amb;
 
-    // candidate of the l-mer ending at p
-    const bool lvalid = inb && p + 1 >= l && lastN < (int64_t) (p + 1 - l);
-    uint64_t cand = kNoCand;
-    if (lvalid) {
-      const uint32_t s = (uint32_t) lane + 65u - l;
-      const uint64_t w0 = window128(pp0, P0, s) & lbits;
-      const uint64_t w1 = window128(pp1, P1, s) & lbits;
-      const uint64_t r0 = __builtin_bitreverse64(w0) >> (64 - l);
-      const uint64_t r1 = __builtin_bitreverse64(w1) >> (64 - l);
-      uint64_t lmer;
+      // candidate of the l-mer ending at p
+      const bool lvalid = inb && p + 1 >= l && lastN < (int64_t) (p + 1 - l);
+      uint64_t cand = kNoCand;
       if (dna) {
-        lmer = (spread2(r1) << 1) | spread2(r0);
-        cand = canonical_lmer(lmer, l, db.revcom_version);
-      } else {
-        const uint64_t w2 = window128(pp2, P2, s) & lbits;
-        const uint64_t w3 = window128(pp3, P3, s) & lbits;
+        // The chunk's bases as packed 2-bit streams (wave-uniform): big-endian
+        // (first base in the top bits of its 32-base word) for forward l-mers,
+        // little-endian for reverse complements; each lane then cuts its
+        // l-mer out of two adjacent words.
+        const uint64_t r0l = replicate_bits((uint32_t) P0), r0h = replicate_bits((uint32_t) (P0 >> 32));
+        const uint64_t r1l = replicate_bits((uint32_t) P1), r1h = replicate_bits((uint32_t) (P1 >> 32));
+        const uint64_t LEl = (r0l & kEvenBits) | (r1l & kOddBits);
+        const uint64_t LEh = (r0h & kEvenBits) | (r1h & kOddBits);
+        const uint64_t BEl = __builtin_bitreverse64((r1l & kEvenBits) | (r0l & kOddBits));
+        const uint64_t BEh = __builtin_bitreverse64((r1h & kEvenBits) | (r0h & kOddBits));
+        if (lvalid) {
+          const bool up = lane >= 32;
+          const uint32_t s = 62u - 2u * (uint32_t) (lane & 31);
+          const uint64_t flo = up ? BEh : BEl, fhi = up ? BEl : pBEh;
+          const uint64_t fwd = ((flo >> s) | ((fhi << 1) << (63u - s))) & lmask2;
+          const uint64_t top = up ? LEh : LEl, bot = up ? LEl : pLEh;
+          const uint64_t t = (top << s) | ((bot >> 1) >> (63u - s));
+          const uint64_t rc = (~t) >> (64u - 2u * l);
+          uint64_t canon;
+          if (db.revcom_version == 0) {  // src/mmscanner.cc:193-215, old databases
+            const uint32_t d = 2u * (32u - l);
+            const uint64_t rc0 = ((rc << d) | ((1ULL << d) - 1ULL)) & lmask2;
+            canon = fwd < rc0 ? fwd : rc0;
+          } else {
+            canon = fwd < rc ? fwd : rc;
+          }
+          if (db.spaced_mask) canon &= db.spaced_mask;
+          cand = canon ^ db.toggle_mask;
+        }
+        pBEh = BEh;
+        pLEh = LEh;
+      } else if (lvalid) {
+        const uint32_t sh = (uint32_t) lane + 65u - l;
+        const uint64_t w0 = window128(pp0, P0, sh) & lbits;
+        const uint64_t w1 = window128(pp1, P1, sh) & lbits;
+        const uint64_t w2 = window128(pp2, P2, sh) & lbits;
+        const uint64_t w3 = window128(pp3, P3, sh) & lbits;
+        const uint64_t r0 = __builtin_bitreverse64(w0) >> (64 - l);
+        const uint64_t r1 = __builtin_bitreverse64(w1) >> (64 - l);
         const uint64_t r2 = __builtin_bitreverse64(w2) >> (64 - l);
         const uint64_t r3 = __builtin_bitreverse64(w3) >> (64 - l);
-        lmer = (spread4(r3) << 3) | (spread4(r2) << 2) | (spread4(r1) << 1) | spread4(r0);
-        cand = lmer;
+        cand = (spread4(r3) << 3) | (spread4(r2) << 2) | (spread4(r1) << 1) | spread4(r0);
+        if (db.spaced_mask) cand &= db.spaced_mask;
+        cand ^= db.toggle_mask;
       }
-      if (db.spaced_mask) cand &= db.spaced_mask;
-      cand ^= db.toggle_mask;
-    }
-    ring[p & 127] = cand;
-    __syncthreads();
+      lds.ring[p & 127] = cand;
+      wave_sync();
 
-    // k-mer ending at p
-    const bool kvalid = inb && p + 1 >= k;
-    bool nonamb = false;
-    uint64_t mn = ~0ULL;
-    if (kvalid) {
-      nonamb = (k == l) ? lvalid : (lastN < (int64_t) (p + 2 - k));
-      if (nonamb) {
-        for (uint32_t j = 0; j <= k - l; j++) {
-          const uint64_t v = ring[(p - j) & 127];
+      // k-mer ending at p: minimum over its window of l-mer candidates
+      const bool kvalid = inb && p + 1 >= k;
+      const bool nonamb =
+          kvalid && ((k == l) ? lvalid : (lastN < (int64_t) (p + 2 - k)));
+      uint64_t mn = ~0ULL;
+#pragma unroll
+      for (uint32_t j = 0; j < 8; j++) {
+        if (j < w) {
+          const uint64_t v = lds.ring[(p - j) & 127];
           mn = v < mn ? v : mn;
         }
-        mn ^= db.toggle_mask;
       }
-    }
-    __syncthreads();
+      for (uint32_t j = 8; j < w; j++) {
+        const uint64_t v = lds.ring[(p - j) & 127];
+        mn = v < mn ? v : mn;
+      }
+      mn ^= db.toggle_mask;
 
-    // token decision against the previous non-ambiguous minimizer
-    const uint64_t NA = __ballot(nonamb);
-    const uint64_t prevm = NA & lanemask_lt(lane);
-    const uint64_t before = shfl64(mn, prevm ? top_lane(prevm) : lane);
-    const uint64_t prev_min = prevm ? before : last_min;
-    const bool lookup = nonamb && mn != prev_min;
-    const bool skip = lookup && db.min_hash != 0 && fmix64(mn) < db.min_hash;
-    uint32_t taxon = 0;
-    if (lookup && !skip) {
-      taxon = table_get(db, mn);
-      if (EXTRAS && out.lookups) {
-        st.my_lookups++;
-        st.my_lines += table_lines(db, mn);
+      // token decision against the previous non-ambiguous minimizer
+      const uint64_t NA = __ballot(nonamb);
+      const uint64_t prevm = NA & lanemask_lt(lane);
+      const uint64_t before = shfl64(mn, prevm ? top_lane(prevm) : lane);
+      const uint64_t prev_min = prevm ? before : last_min;
+      const bool lookup = nonamb && mn != prev_min;
+      const uint64_t LK = __ballot(lookup);
+      const uint64_t E = __ballot(kvalid);
+      if (lookup) lds.qkey[qn + __popcll(LK & lanemask_lt(lane))] = mn;
+      if (lane == 0) {
+        lds.cmask[g][0] = NA;
+        lds.cmask[g][1] = LK;
+        lds.cmask[g][2] = E;
+        lds.cqb[g] = qn;
       }
-    }
-    const uint64_t LK = __ballot(lookup);
-    const uint64_t srcm = LK & lanemask_le(lane);
-    const uint32_t srct = shfl32(taxon, srcm ? top_lane(srcm) : lane);
-    const uint32_t tax_i = nonamb ? (srcm ? srct : last_taxon) : 0u;
-    uint64_t HIT = __ballot(lookup && !skip && taxon != 0);
-    uint64_t E = __ballot(kvalid);
-
-    if (P.quick) {  // src/classify.cc:880-884: stop at the first qualifying hit
-      const uint32_t g_le = st.groups + __popcll(HIT & lanemask_le(lane));
-      const uint64_t Q = __ballot(kvalid && tax_i != 0 && (int32_t) g_le >= P.min_hit_groups);
-      if (Q) {
-        const int cut = __ffsll((long long) Q) - 1;
-        E &= lanemask_lt(cut);
-        HIT &= lanemask_le(cut);
-        st.done = true;
-      }
+      qn += (uint32_t) __popcll(LK);
+      g++;
+      if (A) last_amb = (int64_t) (c + top_lane(A));
+      if (NA) last_min = uniform64(shfl64(mn, top_lane(NA)));
+      pp0 = P0;
+      pp1 = P1;
+      pp2 = P2;
+      pp3 = P3;
+      c += kWave;
+      wave_sync();  // the ring slots of this chunk are read by the next one
     }
 
-    // per-taxon k-mer counts and HLL records for LOOKUP hits
-    if (EXTRAS && ((HIT >> lane) & 1ULL)) {
-      if (out.kmer_counts) atomicAdd(&out.kmer_counts[taxon], 1ULL);
-      if (out.recs) {
-        const uint32_t r = st.nrec + __popcll(HIT & lanemask_lt(lane));
-        if (r < cap) {
-          out.recs[2 * (base + r)] = taxon;
-          out.recs[2 * (base + r) + 1] = hll_encode(mn);
+    // ---- B: probe the queued minimizers, every lane busy
+    for (uint32_t q0 = 0; q0 < qn; q0 += kWave) {
+      const uint32_t q = q0 + (uint32_t) lane;
+      if (q < qn) {
+        const uint64_t key = lds.qkey[q];
+        lds.qval[q] = probe_key(db, key);
+        if (EXTRAS && out.lookups && fmix64(key) >= db.min_hash) {
+          st.my_lookups++;
+          st.my_lines += table_lines(db, key);
+        }
+      }
+    }
+    wave_sync();
+
+    // ---- C: replay the chunks in order
+    for (uint32_t gi = 0; gi < g && !st.done; gi++) {
+      const uint64_t NA = uniform64(lds.cmask[gi][0]);
+      const uint64_t LK = uniform64(lds.cmask[gi][1]);
+      uint64_t E = uniform64(lds.cmask[gi][2]);
+      const uint32_t qb = uniform32(lds.cqb[gi]);
+      const bool nonamb = (NA >> lane) & 1ULL;
+      const bool lookup = (LK >> lane) & 1ULL;
+      const bool kvalid = (E >> lane) & 1ULL;
+      const uint64_t srcm = LK & lanemask_le(lane);
+      const uint32_t tq = srcm ? lds.qval[qb + __popcll(srcm) - 1] : last_taxon;
+      const uint32_t tax_i = nonamb ? tq : 0u;
+      uint64_t HIT = __ballot(lookup && tq != 0);
+
+      if (P.quick) {  // src/classify.cc:880-884: stop at the first qualifying hit
+        const uint32_t g_le = st.groups + __popcll(HIT & lanemask_le(lane));
+        const uint64_t Q = __ballot(kvalid && tax_i != 0 && (int32_t) g_le >= P.min_hit_groups);
+        if (Q) {
+          const int cut = __ffsll((long long) Q) - 1;
+          E &= lanemask_lt(cut);
+          HIT &= lanemask_le(cut);
+          st.done = true;
+        }
+      }
+
+      // per-taxon k-mer counts and HLL records for LOOKUP hits
+      if (EXTRAS && ((HIT >> lane) & 1ULL)) {
+        if (out.kmer_counts) atomicAdd(&out.kmer_counts[tq], 1ULL);
+        if (out.recs) {
+          const uint32_t r = st.nrec + __popcll(HIT & lanemask_lt(lane));
+          if (r < cap) {
+            out.recs[2 * (base + r)] = tq;
+            out.recs[2 * (base + r) + 1] = hll_encode(lds.qkey[qb + __popcll(srcm) - 1]);
+          } else {
+            atomicOr(bad, 2);
+          }
+        }
+      }
+      st.nrec += __popcll(HIT);
+      st.groups += __popcll(HIT);
+
+      // run-length encode the elements of this chunk
+      if (E) {
+        const bool isE = (E >> lane) & 1ULL;
+        const uint32_t code_i = nonamb ? tax_i : kCodeAmbig;
+        const uint32_t prev_code = shfl32(code_i, lane > 0 ? lane - 1 : 0);
+        const int lo = __ffsll((long long) E) - 1;
+        const bool boundary =
+            isE && (lane == lo ? (st.run_count == 0 || code_i != st.run_code) : code_i != prev_code);
+        const uint64_t B = __ballot(boundary);
+        const uint64_t pb = B & lanemask_lt(lane);
+        const int pbl = pb ? top_lane(pb) : lane;
+        const uint32_t pb_code = shfl32(code_i, pbl);
+        const uint32_t drop = (st.run_count == 0) ? 1u : 0u;
+        if (boundary) {
+          uint32_t cc, cn;
+          bool has = true;
+          if (pb) {
+            cc = pb_code;
+            cn = (uint32_t) (lane - pbl);
+          } else {
+            cc = st.run_code;
+            cn = st.run_count + (uint32_t) (lane - lo);
+            has = st.run_count > 0;
+          }
+          if (has) {
+            const uint32_t idx = st.n_runs + __popcll(pb) - drop;
+            close_run(out, base, cap, idx, cc, cn, tab, bad);
+          }
+        }
+        st.n_runs += (uint32_t) __popcll(B) - (B ? drop : 0u);
+        const int hi = top_lane(E);
+        if (B) {
+          const int lb = top_lane(B);
+          st.run_code = uniform32(shfl32(code_i, lb));
+          st.run_count = (uint32_t) (hi - lb + 1);
         } else {
-          atomicOr(bad, 2);
+          st.run_count += (uint32_t) __popcll(E);
         }
+        st.elems += (uint64_t) __popcll(E);
       }
+      if (LK) last_taxon = uniform32(lds.qval[qb + __popcll(LK) - 1]);
     }
-    st.nrec += __popcll(HIT);
-    st.groups += __popcll(HIT);
-
-    // run-length encode the elements of this chunk
-    if (E) {
-      const bool isE = (E >> lane) & 1ULL;
-      const uint32_t code_i = nonamb ? tax_i : kCodeAmbig;
-      const uint32_t prev_code = shfl32(code_i, lane > 0 ? lane - 1 : 0);
-      const int lo = __ffsll((long long) E) - 1;
-      const bool boundary =
-          isE && (lane == lo ? (st.run_count == 0 || code_i != st.run_code) : code_i != prev_code);
-      const uint64_t B = __ballot(boundary);
-      const uint64_t pb = B & lanemask_lt(lane);
-      const int pbl = pb ? top_lane(pb) : lane;
-      const uint32_t pb_code = shfl32(code_i, pbl);
-      const uint32_t drop = (st.run_count == 0) ? 1u : 0u;
-      if (boundary) {
-        uint32_t cc, cn;
-        bool has = true;
-        if (pb) {
-          cc = pb_code;
-          cn = (uint32_t) (lane - pbl);
-        } else {
-          cc = st.run_code;
-          cn = st.run_count + (uint32_t) (lane - lo);
-          has = st.run_count > 0;
-        }
-        if (has) {
-          const uint32_t idx = st.n_runs + __popcll(pb) - drop;
-          close_run(out, base, cap, idx, cc, cn, tab, bad);
-        }
-      }
-      st.n_runs += (uint32_t) __popcll(B) - (B ? drop : 0u);
-      const int hi = top_lane(E);
-      if (B) {
-        const int lb = top_lane(B);
-        st.run_code = uniform32(shfl32(code_i, lb));
-        st.run_count = (uint32_t) (hi - lb + 1);
-      } else {
-        st.run_count += (uint32_t) __popcll(E);
-      }
-      st.elems += (uint64_t) __popcll(E);
-    }
-
-    if (A) last_amb = (int64_t) (c + top_lane(A));
-    if (NA) last_min = uniform64(shfl64(mn, top_lane(NA)));
-    if (LK) last_taxon = uniform32(shfl32(taxon, top_lane(LK)));
-    pp0 = P0;
-    pp1 = P1;
-    pp2 = P2;
-    pp3 = P3;
+    wave_sync();  // the queue is refilled by the next batch
     if (st.done) break;
   }
 }
@@ -408,7 +544,7 @@ __device__ __forceinline__ uint32_t reduce_sum(uint32_t v) {
 
 template <bool DNA, bool EXTRAS, class Table>
 __device__ void classify_fragment(const DbView &db, const ClassifyParams &P, const FragBatch &in,
-                                  const FragOut &out, uint64_t f, Table &tab, uint64_t *ring,
+                                  const FragOut &out, uint64_t f, Table &tab, WaveLds &lds,
                                   int *bad, int lane) {
   const uint64_t o0 = uniform64(in.off[2 * f]), o1 = uniform64(in.off[2 * f + 1]),
                  o2 = uniform64(in.off[2 * f + 2]);
@@ -426,7 +562,7 @@ __device__ void classify_fragment(const DbView &db, const ClassifyParams &P, con
     const char *m = in.seq + (mate ? o1 : o0);
     const uint64_t L = mate ? o2 - o1 : o1 - o0;
     for (int fr = 0; fr < nframes; fr++) {
-      scan_stream<DNA, EXTRAS>(db, P, out, base, cap, m, L, fr, st, tab, ring, bad, lane);
+      scan_stream<DNA, EXTRAS>(db, P, out, base, cap, m, L, fr, st, tab, lds, bad, lane);
       if (st.done) break;
       if (tx && fr != 5) push_marker(out, base, cap, st, kCodeFrame, tab, bad, lane);
     }
@@ -495,12 +631,6 @@ __device__ void classify_fragment(const DbView &db, const ClassifyParams &P, con
   tab.release(lane);
 }
 
-struct WaveLds {
-  uint64_t ring[128];
-  uint32_t tkey[256];
-  uint32_t tcnt[256];
-  int bad;
-};
 
 // Plain path: one wavefront per fragment, one launch per batch.
 template <bool DNA, bool EXTRAS>
@@ -510,7 +640,7 @@ __device__ __forceinline__ void plain_body(const DbView &db, const ClassifyParam
   const uint64_t f = blockIdx.x;
   if (f >= in.n) return;
   LdsTable tab{lds.tkey, lds.tcnt, P.tab_cap};
-  classify_fragment<DNA, EXTRAS>(db, P, in, out, f, tab, lds.ring, &lds.bad, threadIdx.x);
+  classify_fragment<DNA, EXTRAS>(db, P, in, out, f, tab, lds, &lds.bad, threadIdx.x);
 }
 
 // Persistent path: resident wavefronts pull work units (runs of fragments of
@@ -536,7 +666,7 @@ __device__ __forceinline__ void persistent_body(const DbView &db, const Classify
     const uint64_t hi = in.unit_start ? uniform32(in.unit_start[u + 1]) : u + 1;
     for (uint64_t j = lo; j < hi; j++) {
       const uint64_t f = uniform64(in.order ? (uint64_t) in.order[j] : j);
-      classify_fragment<DNA, EXTRAS>(db, P, in, out, f, tab, lds.ring, &lds.bad, lane);
+      classify_fragment<DNA, EXTRAS>(db, P, in, out, f, tab, lds, &lds.bad, lane);
     }
     u = uniform64((uint64_t) next) >> 6;
   }
@@ -555,7 +685,7 @@ __device__ __forceinline__ void dense_body(const DbView &db, const ClassifyParam
   DenseTable tab{scratch.counts + (uint64_t) slot * db.node_count,
                  scratch.lists + (uint64_t) slot * db.node_count, scratch.lens + slot};
   for (uint32_t i = blockIdx.x; i < n_redo; i += gridDim.x)
-    classify_fragment<DNA, true>(db, P, in, out, uniform32(redo[i]), tab, lds.ring, &lds.bad, lane);
+    classify_fragment<DNA, true>(db, P, in, out, uniform32(redo[i]), tab, lds, &lds.bad, lane);
 }
 
 // Kernel variants: _dna (nucleotide, no per-hit extras: the hot path),

@@ -12,10 +12,11 @@ extern "C" {
 // Runs the scanner over seq[start, finish) and writes one (ambiguous,
 // minimizer) pair per returned k-mer.  Returns the number of outputs, or -1
 // if cap is too small.
-long k2cpu_scan(long k, long l, uint64_t mask, uint64_t toggle, int dna, long start, long finish,
-                const char *seq, long len, uint8_t *amb, uint64_t *mins, long cap) {
+long k2cpu_scan_rc(long k, long l, uint64_t mask, uint64_t toggle, int dna, int revcom_version,
+                   long start, long finish, const char *seq, long len, uint8_t *amb,
+                   uint64_t *mins, long cap) {
   std::string s(seq, (size_t) len);
-  Scanner sc(k, l, mask, dna != 0, toggle);
+  Scanner sc(k, l, mask, dna != 0, toggle, revcom_version);
   sc.load(s, (size_t) start, finish < 0 ? SIZE_MAX : (size_t) finish);
   long n = 0;
   const uint64_t *m;
@@ -26,6 +27,11 @@ long k2cpu_scan(long k, long l, uint64_t mask, uint64_t toggle, int dna, long st
     n++;
   }
   return n;
+}
+
+long k2cpu_scan(long k, long l, uint64_t mask, uint64_t toggle, int dna, long start, long finish,
+                const char *seq, long len, uint8_t *amb, uint64_t *mins, long cap) {
+  return k2cpu_scan_rc(k, l, mask, toggle, dna, 1, start, finish, seq, len, amb, mins, cap);
 }
 
 // Six frames, NUL-separated, into out (cap bytes).  Returns bytes written.

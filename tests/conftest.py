@@ -22,3 +22,17 @@ def synth_db(tmp_path_factory):
     d = tmp_path_factory.mktemp("synth_db")
     make_synthetic_db(str(d), seed=11, levels="2,2,3", genome_len=20000)
     return str(d)
+
+
+@pytest.fixture(scope="session")
+def synth_db_rc0(synth_db, tmp_path_factory):
+    """synth_db with opts.k2d marked revcom_version 0 (the reverse complement
+    of databases built before the fix), for scanner parity on that path."""
+    import shutil
+    d = tmp_path_factory.mktemp("synth_db_rc0")
+    for f in ("hash.k2d", "taxo.k2d", "opts.k2d", "library.fna"):
+        shutil.copy(os.path.join(synth_db, f), os.path.join(str(d), f))
+    raw = bytearray(open(os.path.join(str(d), "opts.k2d"), "rb").read())
+    raw[48:52] = (0).to_bytes(4, "little")
+    open(os.path.join(str(d), "opts.k2d"), "wb").write(bytes(raw))
+    return str(d)

@@ -12,7 +12,7 @@ def load():
     lib = C.CDLL(os.path.join(HERE, "libk2g_emu.so"))
     vp = C.c_void_p
     u32, u64, i32, dbl = C.c_uint32, C.c_uint64, C.c_int32, C.c_double
-    lib.emu_classify.argtypes = [vp, u64, u32, u32, vp, u32, u32, u32, u32, u64, u64, u64, dbl, i32, u32, u32,
+    lib.emu_classify.argtypes = [vp, u64, u32, u32, vp, u32, u32, u32, u32, u64, u64, u64, u32, dbl, i32, u32, u32,
                                  vp, vp, vp, u32, u64, vp, C.c_int, C.c_uint, vp, vp, vp, vp] + [vp] * 9
     lib.emu_collectives.restype = C.c_ulonglong
     return lib
@@ -32,8 +32,9 @@ def read_db(d):
     k, l, mask, tog = np.frombuffer(o[:32], np.uint64).tolist()
     dna = o[32]
     minh = int(np.frombuffer(o[40:48], np.uint64)[0])
+    rc = int(np.frombuffer(o[48:52], np.int32)[0]) if len(o) >= 52 else 0
     return dict(cells=cells, capacity=cap, kb=kb, vb=vb, parent=parent, ext=ext, k=k, l=l, mask=mask,
-                toggle=tog, dna=dna, min_hash=minh)
+                toggle=tog, dna=dna, min_hash=minh, revcom=rc)
 
 
 def bounds(db, off, paired_flags):
@@ -88,7 +89,7 @@ def classify(lib, db, seq, off, paired_flags, *, conf=0.0, min_groups=0, quick=F
     sb = np.frombuffer(seq if len(seq) else b"\0", np.uint8)
     lib.emu_classify(db["cells"].ctypes.data, db["capacity"], db["kb"], db["vb"], db["parent"].ctypes.data,
                      len(db["parent"]), db["k"], db["l"], db["dna"], db["mask"], db["toggle"], db["min_hash"],
-                     conf, min_groups, int(quick), tab_cap, sb.ctypes.data, off.ctypes.data, pf.ctypes.data, 0,
+                     db["revcom"], conf, min_groups, int(quick), tab_cap, sb.ctypes.data, off.ctypes.data, pf.ctypes.data, 0,
                      n, order.ctypes.data, mode, grid, run_off.ctypes.data, caps.ctypes.data,
                      units.ctypes.data if units is not None else None,
                      n_units.ctypes.data if n_units is not None else None,

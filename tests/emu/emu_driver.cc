@@ -42,7 +42,7 @@ extern "C" {
 // given grid.  run_off/run_cap (n) and runs/recs buffers prepared by the caller.
 int emu_classify(const uint32_t *cells, uint64_t capacity, uint32_t kb, uint32_t vb, const uint32_t *parent,
                  uint32_t node_count, uint32_t k, uint32_t l, uint32_t dna, uint64_t spaced, uint64_t toggle,
-                 uint64_t min_hash, double conf, int32_t min_groups, uint32_t quick, uint32_t tab_cap,
+                 uint64_t min_hash, uint32_t revcom_version, double conf, int32_t min_groups, uint32_t quick, uint32_t tab_cap,
                  const char *seq, const uint64_t *off, const uint8_t *paired, uint32_t paired_all, uint64_t n,
                  const uint32_t *order, int mode, unsigned grid, const uint64_t *run_off,
                  const uint64_t *run_cap, const uint32_t *unit_start, const uint32_t *n_units,
@@ -61,7 +61,7 @@ int emu_classify(const uint32_t *cells, uint64_t capacity, uint32_t kb, uint32_t
   v.k = k;
   v.l = l;
   v.dna = dna;
-  v.revcom_version = 1;
+  v.revcom_version = revcom_version;
   const uint32_t bits = dna ? 2 : 4;
   v.lmer_mask = (1ULL << (l * bits)) - 1;
   v.spaced_mask = spaced;

@@ -43,6 +43,7 @@ inline void wait() {
 #define gridDim emu::gridDim_
 
 inline void __syncthreads() { emu::wait(); }
+inline void __builtin_amdgcn_wave_barrier() { emu::wait(); }
 
 inline unsigned long long __ballot(int pred) {
   const unsigned lane = threadIdx.x;

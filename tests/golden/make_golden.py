@@ -225,6 +225,24 @@ def scanner():
         f.write(out)
 
 
+def scanner_revcom0():
+    """DNA scans under revcom_version 0 (databases built before the reverse
+    complement fix, src/mmscanner.cc:193-215)."""
+    rng = random.Random(17)
+    cases = []
+    for k, l in ((35, 31), (31, 31), (31, 15), (8, 4), (25, 20)):
+        for t in range(30):
+            L = rng.choice([0, l - 1, l, k, k + 1, 64, 65, 130, rng.randrange(0, 300)])
+            s = "".join(rng.choice("ACGTACGTACGTNacgt") for _ in range(max(0, L))) or "-"
+            cases.append(f"{k} {l} 0 e37e28c4271b5a2d 1 0 -1 {s}")
+    inp = "\n".join(cases) + "\n"
+    out = run([os.path.join(REF, "scan_dump"), "0"], input=inp.encode()).stdout.decode()
+    with open(os.path.join(HERE, "scan_rc0_cases.txt"), "w") as f:
+        f.write(inp)
+    with open(os.path.join(HERE, "scan_rc0_expected.txt"), "w") as f:
+        f.write(out)
+
+
 def hll():
     rng = random.Random(13)
     lines = []
@@ -339,6 +357,7 @@ if __name__ == "__main__":
         raise SystemExit("build the reference first: make -C oracle ref")
     toy()
     scanner()
+    scanner_revcom0()
     hll()
     translate()
     protein_db()

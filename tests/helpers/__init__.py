@@ -128,6 +128,9 @@ class Oracle:
         lib.k2cpu_scan.argtypes = [C.c_long, C.c_long, C.c_uint64, C.c_uint64, C.c_int, C.c_long, C.c_long,
                                    C.c_char_p, C.c_long, vp, vp, C.c_long]
         lib.k2cpu_scan.restype = C.c_long
+        lib.k2cpu_scan_rc.argtypes = [C.c_long, C.c_long, C.c_uint64, C.c_uint64, C.c_int, C.c_int, C.c_long,
+                                      C.c_long, C.c_char_p, C.c_long, vp, vp, C.c_long]
+        lib.k2cpu_scan_rc.restype = C.c_long
         lib.k2cpu_translate.argtypes = [C.c_char_p, C.c_long, vp, C.c_long]
         lib.k2cpu_translate.restype = C.c_long
         lib.k2cpu_hll_script.argtypes = [C.c_char_p, vp, C.c_long]

@@ -114,3 +114,12 @@ def test_emulated_run_space_overflow(emu_lib, synth_db):
     seq, off = k2.pack_fragments(reads)
     out = _check(emu, lib, synth_db, seq, off, np.zeros(len(reads), np.uint8), g=2, cap_pct=1)
     assert (out["status"] & 2).any() and not (out["status"] & 2).all()
+
+
+def test_emulated_revcom_version0(emu_lib, synth_db_rc0):
+    import kraken2_amd as k2
+    emu, lib = emu_lib
+    _, genomes = read_fasta(os.path.join(synth_db_rc0, "library.fna"))
+    reads = edge_reads(genomes, 40, seed=71)
+    seq, off = k2.pack_fragments(reads)
+    _check(emu, lib, synth_db_rc0, seq, off, np.zeros(len(reads), np.uint8), g=0)

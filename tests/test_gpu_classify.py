@@ -171,3 +171,21 @@ def test_work_unit_sizes(env, unit_elements):
     out = batch.classify(seq, off, min_hit_groups=2, mode=1, unit_elements=unit_elements)
     ref = orc.classify(h, seq, off, min_hit_groups=2)
     _compare(k2, db, out, ref, len(reads))
+
+
+def test_classify_revcom_version0(synth_db_rc0):
+    """A database marked revcom_version 0 (old reverse complement)."""
+    import kraken2_amd as k2
+    db = k2.Db.from_dir(synth_db_rc0)
+    batch = k2.Batch(db)
+    orc = Oracle()
+    h = orc.open(synth_db_rc0)
+    _, genomes = read_fasta(os.path.join(synth_db_rc0, "library.fna"))
+    reads = edge_reads(genomes, 1500, seed=800)
+    seq, off = k2.pack_fragments(reads)
+    for mode in (0, 1):
+        out = batch.classify(seq, off, mode=mode)
+        ref = orc.classify(h, seq, off)
+        _compare(k2, db, out, ref, len(reads))
+    batch.close()
+    db.close()

@@ -53,6 +53,29 @@ def test_scanner_matches_reference_vectors(orc):
         assert got == exp, line
 
 
+def test_scanner_revcom_version0_vectors(orc):
+    """Old databases' reverse complement (revcom_version 0) against the
+    reference scanner."""
+    cases = open(os.path.join(G, "scan_rc0_cases.txt")).read().splitlines()
+    exp_lines = open(os.path.join(G, "scan_rc0_expected.txt")).read().splitlines()
+    pos = 0
+    for line in cases:
+        k, l, mask, tog, dna, a, b, s = line.split(" ")
+        s = "" if s == "-" else s
+        bs = s.encode()
+        amb = np.zeros(len(bs) + 1, np.uint8)
+        mins = np.zeros(len(bs) + 1, np.uint64)
+        n = orc.lib.k2cpu_scan_rc(int(k), int(l), int(mask, 16), int(tog, 16), int(dna), 0, int(a), int(b), bs,
+                                  len(bs), amb.ctypes.data, mins.ctypes.data, len(bs) + 1)
+        got = [(int(amb[i]), int(mins[i])) for i in range(n)]
+        assert exp_lines[pos].startswith("# ")
+        m = int(exp_lines[pos][2:])
+        exp = [(int(x.split()[0]), int(x.split()[1], 16)) for x in exp_lines[pos + 1:pos + 1 + m]]
+        pos += 1 + m
+        assert got == exp, line
+    assert pos == len(exp_lines)
+
+
 def _brute(k, l, mask, toggle, s):
     """Definition of the emitted minimizers of a scan from position 0:
     one output per k-mer end e >= k-1; ambiguous iff a non-ACGT base lies in
