@@ -154,6 +154,18 @@ int k2g_batch_redo_counts(k2g_batch *b, uint64_t *run_space, uint64_t *dense_tab
  * a verification aid for large batches. */
 int k2g_batch_digests(k2g_batch *b, uint64_t *out, uint64_t n);
 
+/* Pinned host memory for request fragments (and anything else a caller
+ * streams to the GPU): uploads from it run at full interconnect speed and
+ * asynchronously.  No reference counterpart. */
+int k2g_host_alloc(size_t bytes, void **out);
+int k2g_host_free(void *p);
+
+/* 64-bit checksum of a result set (every array in order, n and node_count
+ * included), computed on the host: two result sets of the same fragments are
+ * equal iff their checksums are (up to hash collisions).  No reference
+ * counterpart: a verification aid. */
+int k2g_results_checksum(const k2g_results *r, uint64_t *out);
+
 /* Random-gather ceiling of the resident table: n_probes independent 16-byte
  * loads at uniformly random 128-byte-aligned offsets of the table's cells
  * (the access a hash probe makes), timed on the device.  Returns the rate in

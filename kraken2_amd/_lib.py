@@ -38,6 +38,9 @@ EXPORTED_SYMBOLS = (
     "k2g_batch_redo_counts",
     "k2g_batch_digests",
     "k2g_db_gather_ceiling",
+    "k2g_host_alloc",
+    "k2g_host_free",
+    "k2g_results_checksum",
     "k2g_server_create",
     "k2g_server_submit",
     "k2g_server_wait",
@@ -166,6 +169,9 @@ def load_library(path: str = LIB_PATH):
     lib.k2g_server_release.argtypes = [vp, u64]
     lib.k2g_server_stats.argtypes = [vp, C.POINTER(u64), C.POINTER(C.c_int32), C.POINTER(u64)]
     lib.k2g_server_destroy.argtypes = [vp]
+    lib.k2g_host_alloc.argtypes = [C.c_size_t, C.POINTER(vp)]
+    lib.k2g_host_free.argtypes = [vp]
+    lib.k2g_results_checksum.argtypes = [C.POINTER(Results), C.POINTER(u64)]
     lib.k2g_db_gather_ceiling.argtypes = [vp, u64, u64, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     lib.k2g_batch_stream.restype = vp
     for name in EXPORTED_SYMBOLS:
@@ -293,6 +299,26 @@ class Batch:
         res = Results()
         _check(_lib.k2g_classify(self._h, C.byref(p), C.byref(fr), C.byref(res)))
         return _copy_results(res, hll, kmer_counts)
+
+    def classify_checksum(self, seq, off: np.ndarray, **kw) -> int:
+        """Like classify, but returns only the results' 64-bit checksum."""
+        off = np.ascontiguousarray(off, dtype=np.uint64)
+        n = (off.shape[0] - 1) // 2
+        p = ClassifyParams(kw.get("confidence", 0.0), kw.get("min_hit_groups", 0), int(kw.get("quick", False)),
+                           int(kw.get("paired", False)), kw.get("mode", 0), int(kw.get("kmer_counts", False)),
+                           int(kw.get("hll", False)), kw.get("table_slots", 0), 0, kw.get("run_cap_pct", 0), 0)
+        if isinstance(seq, np.ndarray):
+            sbuf = np.ascontiguousarray(seq, dtype=np.uint8) if len(seq) else np.zeros(1, np.uint8)
+        else:
+            sbuf = np.frombuffer(seq, dtype=np.uint8) if len(seq) else np.zeros(1, np.uint8)
+        pf = kw.get("paired_flags")
+        flags = None if pf is None else np.ascontiguousarray(pf, dtype=np.uint8)
+        fr = Fragments(sbuf.ctypes.data, off.ctypes.data, flags.ctypes.data if flags is not None else None, n)
+        res = Results()
+        _check(_lib.k2g_classify(self._h, C.byref(p), C.byref(fr), C.byref(res)))
+        h = C.c_uint64()
+        _check(_lib.k2g_results_checksum(C.byref(res), C.byref(h)))
+        return h.value
 
     def digests(self, n: int) -> np.ndarray:
         """Per-fragment 64-bit digests of the last classify call's results."""
@@ -485,15 +511,21 @@ class Server:
         self._keep[t.value] = (sbuf, off, flags, hll, kmer_counts)
         return t.value
 
-    def wait(self, ticket: int):
-        """(ClassifyOutput, {"queued_ms", "gpu_ms", "total_ms"}); releases the request."""
+    def wait(self, ticket: int, checksum_only: bool = False):
+        """(ClassifyOutput, {"queued_ms", "gpu_ms", "total_ms"}); releases the request.
+        With checksum_only the first element is the results' 64-bit checksum."""
         res = Results()
         times = (C.c_double * 3)()
         rc = _lib.k2g_server_wait(self._h, ticket, C.byref(res), times)
         try:
             _check(rc)
             _, _, _, hll, kc = self._keep[ticket]
-            out = _copy_results(res, hll, kc)
+            if checksum_only:
+                h = C.c_uint64()
+                _check(_lib.k2g_results_checksum(C.byref(res), C.byref(h)))
+                out = h.value
+            else:
+                out = _copy_results(res, hll, kc)
         finally:
             _lib.k2g_server_release(self._h, ticket)
             self._keep.pop(ticket, None)
@@ -515,3 +547,17 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def pinned_array(nbytes: int) -> np.ndarray:
+    """A uint8 array in pinned host memory (freed with the array)."""
+    lib = load_library()
+    p = C.c_void_p()
+    _check(lib.k2g_host_alloc(max(1, int(nbytes)), C.byref(p)))
+    buf = (C.c_uint8 * max(1, int(nbytes))).from_address(p.value)
+    arr = np.frombuffer(buf, dtype=np.uint8)[:int(nbytes)]
+    import weakref
+    weakref.finalize(buf, lib.k2g_host_free, C.c_void_p(p.value))
+    arr = arr.view()
+    arr.flags.writeable = True
+    return arr

@@ -538,7 +538,8 @@ extern "C" int k2g_batch_create(k2g_db *db, k2g_batch **out) {
   for (int vi = 0; vi < 3 && e == hipSuccess; vi++) {
     int per_cu = 0;
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kVariants[vi].persistent, 64, 0);
-    b->persistent_blocks[vi] = std::max(1, per_cu) * std::max(1, db->num_cus);
+    // at least one wave per cursor of the persistent kernel
+    b->persistent_blocks[vi] = std::max(8, std::max(1, per_cu) * std::max(1, db->num_cus));
   }
   if (e != hipSuccess) {
     k2g_batch_destroy(b);
@@ -625,7 +626,7 @@ extern "C" int k2g_classify_device(k2g_batch *b, const k2g_classify_params *p, c
   HIP_TRY(b->bounds.reserve(n + 1));
   HIP_TRY(b->sort_keys.reserve(n));
   HIP_TRY(b->redo_count.reserve(3));
-  HIP_TRY(b->cursor.reserve(1));
+  HIP_TRY(b->cursor.reserve(kCursorWords));
   // in a bounded region, the largest scratch any step needs is taken once
   if (b->temp.region) HIP_TRY(b->temp.reserve(temp_bytes_for(n, p->mode == 1 && !d_order)));
 
@@ -721,7 +722,7 @@ extern "C" int k2g_classify_device(k2g_batch *b, const k2g_classify_params *p, c
   if (p->mode == 0) {
     hipLaunchKernelGGL(K.plain, dim3((unsigned) n), dim3(64), 0, s, v, cp, in, out);
   } else {
-    HIP_TRY(hipMemsetAsync(b->cursor.p, 0, 8, s));
+    HIP_TRY(hipMemsetAsync(b->cursor.p, 0, kCursorWords * 8, s));
     const unsigned grid = (unsigned) std::min<uint64_t>(b->persistent_blocks[vi], n);
     hipLaunchKernelGGL(K.persistent, dim3(grid), dim3(64), 0, s, v, cp, in, out, b->cursor.p);
   }
@@ -774,7 +775,7 @@ extern "C" int k2g_classify_device(k2g_batch *b, const k2g_classify_params *p, c
       in2.order = b->space_only.p;
       in2.unit_start = nullptr;
       in2.n_units = nullptr;
-      HIP_TRY(hipMemsetAsync(b->cursor.p, 0, 8, s));
+      HIP_TRY(hipMemsetAsync(b->cursor.p, 0, kCursorWords * 8, s));
       const unsigned grid = (unsigned) std::min<uint64_t>(b->persistent_blocks[vi], n_space_only);
       hipLaunchKernelGGL(K.persistent, dim3(grid), dim3(64), 0, s, v, cp, in2, out2,
                          b->cursor.p);
@@ -1383,7 +1384,7 @@ uint64_t k2g::plan_classify_bytes(const k2g_db *db, const k2g_classify_params *p
   if (in->paired) b += region_bytes(n, 1);
   b += 6 * region_bytes(n, 4);                                    // call .. n_recs
   b += 3 * region_bytes(n + 1, 8);                                // run_off, run_cap, bounds
-  b += region_bytes(n, 4) + region_bytes(3, 4) + region_bytes(1, 8);  // sort keys, counters, cursor
+  b += region_bytes(n, 4) + region_bytes(3, 4) + region_bytes(kCursorWords, 8);  // sort keys, counters, cursors
   b += region_bytes(temp_bytes_for(n, sort), 1);
   b += region_bytes(2 * std::max<uint64_t>(space, 1), 4);         // runs
   if (p->want_hll) b += region_bytes(2 * std::max<uint64_t>(space, 1), 4);
@@ -1410,4 +1411,58 @@ uint64_t k2g::plan_classify_bytes(const k2g_db *db, const k2g_classify_params *p
   }
   b += region_bytes(n + 1, 8) * (p->want_hll ? 2 : 1);           // fetch offsets
   return b;
+}
+
+// ----------------------------------------------------------------------
+// pinned host memory and result checksums
+extern "C" int k2g_host_alloc(size_t bytes, void **out) {
+  if (!out) return fail(K2G_EUSAGE, "null argument");
+  *out = nullptr;
+  hipError_t e = hipHostMalloc(out, bytes ? bytes : 1, hipHostMallocDefault);
+  if (e != hipSuccess) return fail(K2G_EOS, std::string("pinned allocation: ") + hipGetErrorString(e));
+  return K2G_OK;
+}
+
+extern "C" int k2g_host_free(void *p) {
+  if (p) HIP_TRY(hipHostFree(p));
+  return K2G_OK;
+}
+
+namespace {
+struct Checksum {
+  uint64_t h = 0x6A09E667F3BCC909ull;
+  void word(uint64_t w) {
+    uint64_t x = h ^ (w + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2));
+    x ^= x >> 33;
+    x *= 0xff51afd7ed558ccdULL;
+    x ^= x >> 33;
+    x *= 0xc4ceb9fe1a85ec53ULL;
+    x ^= x >> 33;
+    h = x;
+  }
+  template <class T>
+  void array(const T *p, uint64_t n) {
+    word(p ? n : ~0ull);
+    if (p)
+      for (uint64_t i = 0; i < n; i++) word((uint64_t) p[i]);
+  }
+};
+}  // namespace
+
+extern "C" int k2g_results_checksum(const k2g_results *r, uint64_t *out) {
+  if (!r || !out) return fail(K2G_EUSAGE, "null argument");
+  Checksum c;
+  const uint64_t n = r->n;
+  c.word(n);
+  c.word(r->node_count);
+  c.array(r->call, n);
+  c.array(r->hit_groups, n);
+  c.array(r->total_kmers, n);
+  c.array(r->run_off, r->run_off ? n + 1 : 0);
+  c.array(r->runs, r->run_off ? 2 * r->run_off[n] : 0);
+  c.array(r->hll_off, r->hll_off ? n + 1 : 0);
+  c.array(r->hll, r->hll_off ? 2 * r->hll_off[n] : 0);
+  c.array(r->kmer_counts, r->kmer_counts ? r->node_count : 0);
+  *out = c.h;
+  return K2G_OK;
 }

@@ -80,6 +80,11 @@ __host__ __device__ inline uint64_t run_space(uint64_t e, uint32_t pct) {
   return c < e ? c : e;
 }
 
+// The persistent kernel's cursor block: kCursors cursors 128 bytes apart.
+constexpr int kCursors = 8;
+constexpr int kCursorStride = 16;
+constexpr int kCursorWords = kCursors * kCursorStride;
+
 // Per-slot scratch of the overflow path: a dense count array per taxon.
 struct DenseScratch {
   uint32_t *counts;  // slots * node_count

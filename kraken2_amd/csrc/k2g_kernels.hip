@@ -30,14 +30,12 @@ __constant__ uint8_t c_codon_code[64] = {
     6, 6, 1, 1, 1,  1,  12, 12, 12, 12, 2,  2,  7,  7,  11, 11, 11, 11, 10, 10,
     10, 10, 8, 8, 8, 8, 0, 0, 15, 15, 0, 14, 3, 3, 2, 2, 2, 2, 8, 8, 5, 5};
 
+// A/C/G/T (either case) -> 0..3, anything else 0xFF, without branches:
+// ((c >> 1) ^ (c >> 2)) & 3 maps 'a' 'c' 'g' 't' to 0 1 2 3.
 __device__ __forceinline__ uint32_t dna_code(uint8_t ch) {
-  switch (ch | 0x20) {
-    case 'a': return 0;
-    case 'c': return 1;
-    case 'g': return 2;
-    case 't': return 3;
-  }
-  return 0xFFu;
+  const uint32_t c = (uint32_t) ch | 0x20u;
+  const bool ok = c == 'a' || c == 'c' || c == 'g' || c == 't';
+  return ok ? (((c >> 1) ^ (c >> 2)) & 3u) : 0xFFu;
 }
 
 // A=0 G=1 C=2 T=3 (codon index order of the translation table)
@@ -219,10 +217,12 @@ constexpr uint64_t kOddBits = 0xAAAAAAAAAAAAAAAAULL;
 // probed together (qval), so every lane of the wave probes instead of the
 // ~1/3 of lanes whose k-mer starts a new minimizer.
 constexpr int kQueue = 192;       // queued lookups per probe batch (>= 64 + 64)
+constexpr int kRingGuard = 8;     // windows of up to 9 l-mers read without wrapping
 constexpr int kBatchChunks = 8;   // chunks per probe batch
 
 struct WaveLds {
-  uint64_t ring[128];             // window candidates by position & 127
+  uint64_t ring[kRingGuard + 128];  // window candidates: position p at kRingGuard + (p & 127),
+                                    // slots 120..127 mirrored below index kRingGuard
   uint64_t qkey[kQueue];          // minimizers to look up, in k-mer order
   uint32_t qval[kQueue];          // their table values
   uint64_t cmask[kBatchChunks][3];  // per queued chunk: non-ambiguous, LOOKUP, k-mer-end masks
@@ -275,7 +275,8 @@ __device__ __forceinline__ uint32_t probe_key(const DbView &db, uint64_t key) {
 // Scans one stream (a DNA mate, or one reading frame of a mate) and appends
 // its k-mer elements to the fragment's runs.  DNA: a nucleotide database
 // (else translated search); EXTRAS: per-hit HLL records, per-taxon counts or
-// probe statistics may be requested (their code is compiled out otherwise).
+// probe statistics may be requested (their code is compiled out otherwise);
+// W: the window's l-mer count k-l+1 when fixed at compile time (<= 9), else 0.
 //
 // The stream is processed in probe batches of up to kBatchChunks chunks of
 // 64 k-mer end positions:
@@ -286,10 +287,11 @@ __device__ __forceinline__ uint32_t probe_key(const DbView &db, uint64_t key) {
 //   C. the chunks are replayed in order: each k-mer takes the value of the
 //      latest LOOKUP at or before it (src/classify.cc:846-888) and the wave
 //      run-length encodes the elements and counts hits.
-template <bool DNA, bool EXTRAS, class Table>
+template <bool DNA, bool EXTRAS, int W, class Table>
 __device__ void scan_stream(const DbView &db, const ClassifyParams &P, const FragOut &out,
                             uint64_t base, uint64_t cap, const char *m, uint64_t L, int fr,
                             RunState &st, Table &tab, WaveLds &lds, int *bad, int lane) {
+  static_assert(W >= 0 && W <= kRingGuard + 1, "window too wide for the ring guard");
   constexpr bool dna = DNA;
   const uint64_t n = dna ? L : frame_len(L, fr);
   const uint32_t k = db.k, l = db.l;
@@ -369,7 +371,9 @@ __device__ void scan_stream(const DbView &db, const ClassifyParams &P, const Fra
         if (db.spaced_mask) cand &= db.spaced_mask;
         cand ^= db.toggle_mask;
       }
-      lds.ring[p & 127] = cand;
+      const uint32_t slot = (uint32_t) (p & 127);
+      lds.ring[kRingGuard + slot] = cand;
+      if (slot >= 128u - kRingGuard) lds.ring[slot - (128u - kRingGuard)] = cand;
       wave_sync();
 
       // k-mer ending at p: minimum over its window of l-mer candidates
@@ -377,16 +381,15 @@ __device__ void scan_stream(const DbView &db, const ClassifyParams &P, const Fra
       const bool nonamb =
           kvalid && ((k == l) ? lvalid : (lastN < (int64_t) (p + 2 - k)));
       uint64_t mn = ~0ULL;
+      if (W != 0) {  // compile-time window: reads at fixed offsets from one address
+        const uint64_t *r = &lds.ring[kRingGuard + slot - (W - 1)];
 #pragma unroll
-      for (uint32_t j = 0; j < 8; j++) {
-        if (j < w) {
-          const uint64_t v = lds.ring[(p - j) & 127];
+        for (int j = 0; j < W; j++) mn = r[j] < mn ? r[j] : mn;
+      } else {
+        for (uint32_t j = 0; j < w; j++) {
+          const uint64_t v = lds.ring[kRingGuard + ((p - j) & 127)];
           mn = v < mn ? v : mn;
         }
-      }
-      for (uint32_t j = 8; j < w; j++) {
-        const uint64_t v = lds.ring[(p - j) & 127];
-        mn = v < mn ? v : mn;
       }
       mn ^= db.toggle_mask;
 
@@ -562,7 +565,10 @@ __device__ void classify_fragment(const DbView &db, const ClassifyParams &P, con
     const char *m = in.seq + (mate ? o1 : o0);
     const uint64_t L = mate ? o2 - o1 : o1 - o0;
     for (int fr = 0; fr < nframes; fr++) {
-      scan_stream<DNA, EXTRAS>(db, P, out, base, cap, m, L, fr, st, tab, lds, bad, lane);
+      if (db.k - db.l == 4)  // the default nucleotide window (k=35, l=31)
+        scan_stream<DNA, EXTRAS, 5>(db, P, out, base, cap, m, L, fr, st, tab, lds, bad, lane);
+      else
+        scan_stream<DNA, EXTRAS, 0>(db, P, out, base, cap, m, L, fr, st, tab, lds, bad, lane);
       if (st.done) break;
       if (tx && fr != 5) push_marker(out, base, cap, st, kCodeFrame, tab, bad, lane);
     }
@@ -644,8 +650,12 @@ __device__ __forceinline__ void plain_body(const DbView &db, const ClassifyParam
 }
 
 // Persistent path: resident wavefronts pull work units (runs of fragments of
-// similar length, longest first) from a shared cursor, so long and short
-// reads share one launch and the cursor sees one atomic per unit.
+// similar length, longest first) from shared cursors, so long and short reads
+// share one launch.  The units are dealt to kCursors cursors round robin
+// (unit u to cursor u % kCursors, wave w to cursor w % kCursors) so the
+// waves' atomics spread over several cache lines; the grid has at least
+// min(kCursors, units) waves.
+
 template <bool DNA, bool EXTRAS>
 __device__ __forceinline__ void persistent_body(const DbView &db, const ClassifyParams &P,
                                                 const FragBatch &in, const FragOut &out,
@@ -654,21 +664,23 @@ __device__ __forceinline__ void persistent_body(const DbView &db, const Classify
   LdsTable tab{lds.tkey, lds.tcnt, P.tab_cap};
   const int lane = threadIdx.x;
   const uint64_t n_units = in.unit_start ? uniform32(*in.n_units) : in.n;
+  const uint32_t part = blockIdx.x % kCursors;
+  unsigned long long *cur = cursor + kCursorStride * part;
   // Every lane adds 1, so each fetch advances the cursor by exactly 64 and
-  // any lane's result / 64 is the wave's next unit; no lane-0 branch
-  // precedes the loop exit and the index is a scalar.  The next fetch is
-  // issued before the current unit is classified so its latency hides
+  // any lane's result / 64 is the wave's next unit of its share; no lane-0
+  // branch precedes the loop exit and the index is a scalar.  The next fetch
+  // is issued before the current unit is classified so its latency hides
   // behind that work.
-  uint64_t u = uniform64((uint64_t) atomicAdd(cursor, 1ULL)) >> 6;
+  uint64_t u = kCursors * (uniform64((uint64_t) atomicAdd(cur, 1ULL)) >> 6) + part;
   while (u < n_units) {
-    const unsigned long long next = atomicAdd(cursor, 1ULL);
+    const unsigned long long next = atomicAdd(cur, 1ULL);
     const uint64_t lo = in.unit_start ? uniform32(in.unit_start[u]) : u;
     const uint64_t hi = in.unit_start ? uniform32(in.unit_start[u + 1]) : u + 1;
     for (uint64_t j = lo; j < hi; j++) {
       const uint64_t f = uniform64(in.order ? (uint64_t) in.order[j] : j);
       classify_fragment<DNA, EXTRAS>(db, P, in, out, f, tab, lds, &lds.bad, lane);
     }
-    u = uniform64((uint64_t) next) >> 6;
+    u = kCursors * (uniform64((uint64_t) next) >> 6) + part;
   }
 }
 
@@ -692,11 +704,11 @@ __device__ __forceinline__ void dense_body(const DbView &db, const ClassifyParam
 // _dna_x (nucleotide with HLL records / counts / probe statistics), _prot
 // (translated search, extras available).
 #define K2G_CLASSIFY_VARIANT(SUFFIX, DNA, EXTRAS)                                              \
-  __global__ __launch_bounds__(64, 8) void k_classify_plain##SUFFIX(                          \
+  __global__ __launch_bounds__(64, 7) void k_classify_plain##SUFFIX(                          \
       DbView db, ClassifyParams P, FragBatch in, FragOut out) {                                \
     plain_body<DNA, EXTRAS>(db, P, in, out);                                                   \
   }                                                                                            \
-  __global__ __launch_bounds__(64, 8) void k_classify_persistent##SUFFIX(                     \
+  __global__ __launch_bounds__(64, 7) void k_classify_persistent##SUFFIX(                     \
       DbView db, ClassifyParams P, FragBatch in, FragOut out, unsigned long long *cursor) {    \
     persistent_body<DNA, EXTRAS>(db, P, in, out, cursor);                                      \
   }

@@ -80,11 +80,14 @@ int emu_classify(const uint32_t *cells, uint64_t capacity, uint32_t kb, uint32_t
       else k_classify_plain_dna(v, P, in, out);
     });
   } else {
-    unsigned long long cursor = 0;
+    // the library launches at least min(8, units) waves (one per cursor)
+    static unsigned long long cursor[kCursorWords];
+    for (auto &x : cursor) x = 0;
+    if (grid < 8 && grid < n) grid = (unsigned) (n < 8 ? n : 8);
     run_grid(grid, [&] {
-      if (!dna) k_classify_persistent_prot(v, P, in, out, &cursor);
-      else if (extras) k_classify_persistent_dna_x(v, P, in, out, &cursor);
-      else k_classify_persistent_dna(v, P, in, out, &cursor);
+      if (!dna) k_classify_persistent_prot(v, P, in, out, cursor);
+      else if (extras) k_classify_persistent_dna_x(v, P, in, out, cursor);
+      else k_classify_persistent_dna(v, P, in, out, cursor);
     });
   }
   return 0;

@@ -20,3 +20,43 @@ def test_library_exports_header_symbols():
     for n in names:
         assert hasattr(lib, n), f"libk2g.so does not export {n}"
     assert sorted(k2.EXPORTED_SYMBOLS) == names
+
+
+def test_results_checksum_is_host_only_and_sensitive():
+    """k2g_results_checksum runs on the host: equal result sets hash equal,
+    any changed element (a call, a run count, an HLL record) changes it."""
+    import numpy as np
+    import kraken2_amd._lib as L
+    lib = L.load_library()
+    rng = np.random.default_rng(3)
+    n = 50
+    arrs = dict(call=rng.integers(0, 9, n).astype(np.uint32), groups=rng.integers(0, 9, n).astype(np.uint32),
+                total=rng.integers(0, 200, n).astype(np.uint32))
+    run_off = np.concatenate([[0], np.cumsum(rng.integers(0, 4, n))]).astype(np.uint64)
+    runs = rng.integers(0, 9, 2 * int(run_off[-1])).astype(np.uint32)
+    hll_off = run_off.copy()
+    hll = rng.integers(0, 2**32, 2 * int(hll_off[-1]), dtype=np.uint64).astype(np.uint32)
+    kc = rng.integers(0, 9, 12).astype(np.uint64)
+
+    def ptr(a):
+        t = ctypes.c_uint64 if a.dtype == np.uint64 else ctypes.c_uint32
+        return a.ctypes.data_as(ctypes.POINTER(t))
+
+    def checksum():
+        r = L.Results()
+        r.n = n
+        r.call, r.hit_groups, r.total_kmers = (ptr(arrs[k]) for k in ("call", "groups", "total"))
+        r.run_off, r.runs = ptr(run_off), ptr(runs)
+        r.hll_off, r.hll = ptr(hll_off), ptr(hll)
+        r.kmer_counts, r.node_count = ptr(kc), 12
+        h = ctypes.c_uint64()
+        assert lib.k2g_results_checksum(ctypes.byref(r), ctypes.byref(h)) == 0
+        return h.value
+
+    base = checksum()
+    assert checksum() == base
+    for arr, i in ((arrs["call"], 7), (runs, 3), (hll, 1), (kc, 11)):
+        arr[i] ^= 1
+        assert checksum() != base
+        arr[i] ^= 1
+    assert checksum() == base

@@ -126,3 +126,25 @@ def test_server_rejects_request_over_budget(env):
         assert len(out.call) == 3
     finally:
         srv.close()
+
+
+def test_server_pinned_inputs_and_checksums(env):
+    """Requests whose reads sit in pinned host memory; results compared by
+    checksum with the same fragments classified alone."""
+    import kraken2_amd as k2
+    from kraken2_amd._lib import pinned_array
+    k2m, db, batch, genomes = env
+    reqs = _requests(k2, genomes, 16, seed=13)
+    srv = k2.Server(db, 64 << 20, slots=3)
+    try:
+        tickets, alone = [], []
+        for seq, off, p in reqs:
+            buf = pinned_array(len(seq))
+            buf[:] = np.frombuffer(seq, np.uint8)
+            tickets.append(srv.submit(buf, off, **p))
+            alone.append(batch.classify_checksum(seq, off, **p))
+        for t, ref in zip(tickets, alone):
+            h, _ = srv.wait(t, checksum_only=True)
+            assert h == ref
+    finally:
+        srv.close()
