@@ -4,6 +4,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <map>
+#include <set>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -50,6 +51,24 @@ struct Taxonomy {
   const char *name(uint64_t id) const { return names.data() + nodes[id].name_offset; }
   const char *rank(uint64_t id) const { return ranks.data() + nodes[id].rank_offset; }
 };
+
+// NCBI taxonomy as read from nodes.dmp / names.dmp, with the set of taxa to
+// keep (the root and ancestors of mapped taxa), written as taxo.k2d
+// (src/taxonomy.cc:19-185, 275-287).
+struct NcbiTaxonomy {
+  std::map<uint64_t, uint64_t> parent;
+  std::map<uint64_t, std::string> rank, name;
+  std::map<uint64_t, std::vector<uint64_t>> children;
+  std::set<uint64_t> marked;
+  bool load(const std::string &nodes_path, const std::string &names_path, std::string &err);
+  void mark(uint64_t taxid);  // the taxon and its ancestors
+  bool write_kraken(const std::string &path, std::string &err) const;
+};
+
+// Sequence ids of a FASTA header (src/build_db.cc:113-148).
+std::vector<std::string> sequence_ids(const std::string &header);
+// src/utilities.cc:14-27
+uint64_t expand_spaced_seed_mask(uint64_t mask, int bits_per_char);
 
 // Distinct-minimizer state of one taxon or clade.  The reference's
 // HyperLogLogPlusMinus (p=12, sparse p'=25) ends in a state fixed by the set

@@ -34,8 +34,8 @@ __global__ void k_classify_dense_dna(DbView, ClassifyParams, FragBatch, FragOut,
 __global__ void k_classify_dense_prot(DbView, ClassifyParams, FragBatch, FragOut, const uint32_t *,
                                       uint32_t, DenseScratch);
 
-// The kernel variants a batch launches: nucleotide without per-hit extras
-// (the hot path), nucleotide with extras, translated search.
+// The kernel variants a batch launches: nucleotide without per-hit extras or
+// quick mode (the hot path), nucleotide with them, translated search.
 struct ClassifyKernels {
   void (*plain)(DbView, ClassifyParams, FragBatch, FragOut);
   void (*persistent)(DbView, ClassifyParams, FragBatch, FragOut, unsigned long long *);
@@ -683,7 +683,7 @@ extern "C" int k2g_classify_device(k2g_batch *b, const k2g_classify_params *p, c
     HIP_TRY(b->temp.reserve(ub));
     HIP_TRY(hipcub::DeviceScan::ExclusiveSum(b->temp.p, ub, b->unit_pos.p, b->unit_scan.p, n + 1, s));
     const int vi_units = variant_index(v.dna != 0, p->want_hll || p->want_kmer_counts ||
-                                                         p->want_probe_stats);
+                                                         p->want_probe_stats || p->quick);
     hipLaunchKernelGGL(k_unit_flags, dim3(gi), dim3(256), 0, s, b->unit_scan.p, n, T,
                        (uint64_t) b->persistent_blocks[vi_units], b->unit_flag.p);
     size_t fb = 0;
@@ -716,7 +716,7 @@ extern "C" int k2g_classify_device(k2g_batch *b, const k2g_classify_params *p, c
               p->want_probe_stats ? b->lines.p : nullptr};
 
   const int vi = variant_index(v.dna != 0, p->want_hll || p->want_kmer_counts ||
-                                               p->want_probe_stats);
+                                               p->want_probe_stats || p->quick);
   const ClassifyKernels &K = kVariants[vi];
   HIP_TRY(hipEventRecord(b->ev0, s));
   if (p->mode == 0) {

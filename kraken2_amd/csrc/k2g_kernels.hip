@@ -174,19 +174,22 @@ struct RunState {
   uint64_t my_lines = 0;
 };
 
+// Writes run idx of the fragment and adds a taxon run's count to the hit
+// table (src/classify.cc:846-888).
 template <class Table>
 __device__ __forceinline__ void close_run(const FragOut &out, uint64_t base, uint64_t cap,
                                           uint32_t idx, uint32_t code, uint32_t count,
                                           Table &tab, int *bad) {
+  typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
   if (idx < cap) {
-    out.runs[2 * (base + idx)] = code;
-    out.runs[2 * (base + idx) + 1] = count;
+    u32x2 r;
+    r.x = code;
+    r.y = count;
+    *(u32x2 *) (out.runs + 2 * (base + idx)) = r;
   } else {
     atomicOr(bad, 2);
   }
-  if (code != 0 && code < kCodeAmbig) {
-    if (!tab.add(code, count)) atomicOr(bad, 1);
-  }
+  if (code != 0 && code < kCodeAmbig && !tab.add(code, count)) atomicOr(bad, 1);
 }
 
 // Appends a border marker element (mate "|:|" or frame "-:-").
@@ -448,7 +451,7 @@ __device__ void scan_stream(const DbView &db, const ClassifyParams &P, const Fra
       const uint32_t tax_i = nonamb ? tq : 0u;
       uint64_t HIT = __ballot(lookup && tq != 0);
 
-      if (P.quick) {  // src/classify.cc:880-884: stop at the first qualifying hit
+      if (EXTRAS && P.quick) {  // src/classify.cc:880-884: stop at the first qualifying hit
         const uint32_t g_le = st.groups + __popcll(HIT & lanemask_le(lane));
         const uint64_t Q = __ballot(kvalid && tax_i != 0 && (int32_t) g_le >= P.min_hit_groups);
         if (Q) {
@@ -701,8 +704,8 @@ __device__ __forceinline__ void dense_body(const DbView &db, const ClassifyParam
 }
 
 // Kernel variants: _dna (nucleotide, no per-hit extras: the hot path),
-// _dna_x (nucleotide with HLL records / counts / probe statistics), _prot
-// (translated search, extras available).
+// _dna_x (nucleotide with HLL records / counts / probe statistics / quick
+// mode), _prot (translated search, all of them available).
 #define K2G_CLASSIFY_VARIANT(SUFFIX, DNA, EXTRAS)                                              \
   __global__ __launch_bounds__(64, 7) void k_classify_plain##SUFFIX(                          \
       DbView db, ClassifyParams P, FragBatch in, FragOut out) {                                \

@@ -72,7 +72,7 @@ int emu_classify(const uint32_t *cells, uint64_t capacity, uint32_t kb, uint32_t
   FragOut out{call,    groups, total, n_runs, status,      n_recs,  run_off,
               run_cap, runs,   recs,  kmer_counts, nullptr, nullptr};
   // the variant the library would launch for this database and these outputs
-  const bool extras = recs || kmer_counts;
+  const bool extras = recs || kmer_counts || quick;
   if (mode == 0) {
     run_grid((unsigned) n, [&] {
       if (!dna) k_classify_plain_prot(v, P, in, out);

@@ -44,6 +44,7 @@ inline void wait() {
 
 inline void __syncthreads() { emu::wait(); }
 inline void __builtin_amdgcn_wave_barrier() { emu::wait(); }
+inline void __threadfence() { __atomic_thread_fence(__ATOMIC_SEQ_CST); }
 
 inline unsigned long long __ballot(int pred) {
   const unsigned lane = threadIdx.x;
