@@ -323,15 +323,15 @@ def run_serving(k2, db, tmp, pools, a, work):
 
 
 def write_sample_fasta(path1, path2, frags):
-    """frags: list of (seq1, seq2 or None, name)."""
-    with open(path1, "w") as f1:
-        f2 = open(path2, "w") if path2 else None
+    """frags: list of (seq1 bytes, seq2 bytes or None, name)."""
+    with open(path1, "wb") as f1:
+        f2 = open(path2, "wb") if path2 else None
         for s1, s2, name in frags:
             if s2 is None:
-                f1.write(f">{name}\n{s1}\n")
+                f1.write(b">%s\n%s\n" % (name.encode(), s1))
             else:
-                f1.write(f">{name}/1\n{s1}\n")
-                f2.write(f">{name}/2\n{s2}\n")
+                f1.write(b">%s/1\n%s\n" % (name.encode(), s1))
+                f2.write(b">%s/2\n%s\n" % (name.encode(), s2))
         if f2:
             f2.close()
 
@@ -602,7 +602,7 @@ def main():
         if work["kind"] == "host":
             files = [[host["reads_fa"]]]
         else:
-            m = min(n, a.cpu_sample or 400000)
+            m = min(n, a.cpu_sample or 1_000_000)
             w = min(1000, m)
             starts = np.linspace(0, n - w, max(1, m // w)).astype(np.int64)
             singles, pairs = [], []
@@ -610,8 +610,7 @@ def main():
                 sq, so2, pf = host_range(hip, d_seq, off, paired, s0, s0 + w)
                 for j in range(w):
                     m1, m2 = fragment_text(sq, so2, j)
-                    (pairs if pf[j] else singles).append((m1.decode(), m2.decode() if pf[j] else None,
-                                                          f"f{s0 + j}"))
+                    (pairs if pf[j] else singles).append((m1, m2 if pf[j] else None, f"f{s0 + j}"))
             files = []
             if singles:
                 write_sample_fasta(os.path.join(tmp, "cpu_single.fa"), None, singles)
