@@ -235,30 +235,36 @@ struct WaveLds {
   int bad;
 };
 
-// CompactHashTable::Get (src/compact_hash.h:354-374, 493-500), written for a
-// wave where every lane probes: each step loads a 16-byte group of four cells
-// and decides all four without branching; lanes leave the loop at different
-// steps.  Keys whose hash is below min_hash are not probed (src/classify.cc:
-// 819-823) and read as 0.
-__device__ __forceinline__ uint32_t probe_key(const DbView &db, uint64_t key) {
-  const uint64_t hc = fmix64(key);
-  if (hc < db.min_hash) return 0u;
-  const uint32_t vb = db.value_bits;
-  const uint32_t vmask = (1u << vb) - 1u;
-  const uint32_t ck = (uint32_t) (hc >> (64 - db.key_bits));
-  uint64_t idx = mod_capacity(hc, db.capacity, db.barrett);
-  uint64_t left = db.capacity;  // cells still to examine before a full wrap
-  for (;;) {
+// CompactHashTable::Get (src/compact_hash.h:354-374, 493-500) as a resumable
+// state for a wave where every lane probes: start() hashes the key and finds
+// its home cell; each step() loads one 16-byte group of four cells, decides
+// all four without branching, and returns true with the value once the probe
+// ends.  Keys whose hash is below min_hash are not looked up
+// (src/classify.cc:819-823) and read as 0.
+struct ProbeState {
+  uint64_t idx = 0, left = 0;
+  uint32_t ck = 0;
+  bool skip = false;  // hash below min_hash (src/classify.cc:819-823): value 0
+  __device__ __forceinline__ void start(const DbView &db, uint64_t key) {
+    const uint64_t hc = fmix64(key);
+    skip = hc < db.min_hash;
+    ck = (uint32_t) (hc >> (64 - db.key_bits));
+    idx = mod_capacity(hc, db.capacity, db.barrett);
+    left = db.capacity;
+  }
+  __device__ __forceinline__ bool step(const DbView &db, uint32_t &out) {
+    const uint32_t vb = db.value_bits;
+    const uint32_t vmask = (1u << vb) - 1u;
     const uint64_t group = idx & ~3ULL;
     const u32x4 v = *(const u32x4 *) (db.cells + group);
     const uint32_t off = (uint32_t) (idx - group);
-    uint64_t avail = 4u - off;                     // cells of this group from idx on
+    uint64_t avail = 4u - off;
     if (db.capacity - idx < avail) avail = db.capacity - idx;
     if (left < avail) avail = left;
     const uint32_t c4[4] = {v.x, v.y, v.z, v.w};
     uint32_t stop = 0, val = 0;
 #pragma unroll
-    for (int j = 3; j >= 0; j--) {  // the first stopping cell in probe order wins
+    for (int j = 3; j >= 0; j--) {
       const uint32_t c = c4[j];
       const bool in = (uint32_t) j >= off && (uint64_t) (j - off) < avail;
       const bool empty = (c & vmask) == 0u;
@@ -267,13 +273,16 @@ __device__ __forceinline__ uint32_t probe_key(const DbView &db, uint64_t key) {
       val = st ? (empty ? 0u : (c & vmask)) : val;
       stop |= st ? 1u : 0u;
     }
-    if (stop) return val;
     left -= avail;
-    if (left == 0) return 0u;
     idx += avail;
     if (idx >= db.capacity) idx = 0;
+    if (stop || left == 0 || skip) {
+      out = (stop && !skip) ? val : 0u;
+      return true;
+    }
+    return false;
   }
-}
+};
 
 // Scans one stream (a DNA mate, or one reading frame of a mate) and appends
 // its k-mer elements to the fragment's runs.  DNA: a nucleotide database
@@ -423,15 +432,44 @@ __device__ void scan_stream(const DbView &db, const ClassifyParams &P, const Fra
       wave_sync();  // the ring slots of this chunk are read by the next one
     }
 
-    // ---- B: probe the queued minimizers, every lane busy
-    for (uint32_t q0 = 0; q0 < qn; q0 += kWave) {
-      const uint32_t q = q0 + (uint32_t) lane;
-      if (q < qn) {
-        const uint64_t key = lds.qkey[q];
-        lds.qval[q] = probe_key(db, key);
-        if (EXTRAS && out.lookups && fmix64(key) >= db.min_hash) {
+    // ---- B: probe the queued minimizers.  Each lane works one key at a time
+    // and takes the next queued key as soon as its probe ends, so a step's
+    // round trip serves up to 64 probes however unequal their lengths.
+    {
+      ProbeState ps;
+      uint32_t q = (uint32_t) lane;
+      bool active = q < qn;
+      if (active) {
+        ps.start(db, lds.qkey[q]);
+        if (EXTRAS && out.lookups && !ps.skip) {
           st.my_lookups++;
-          st.my_lines += table_lines(db, key);
+          st.my_lines += table_lines(db, lds.qkey[q]);
+        }
+      }
+      uint32_t next = qn < (uint32_t) kWave ? qn : (uint32_t) kWave;
+      while (__ballot(active)) {
+        if (active) {
+          uint32_t val;
+          if (ps.step(db, val)) {
+            lds.qval[q] = val;
+            active = false;
+          }
+        }
+        const uint64_t R = __ballot(!active) & (next < qn ? ~0ULL : 0ULL);
+        if (R) {
+          if ((R >> lane) & 1ULL) {
+            q = next + (uint32_t) __popcll(R & lanemask_lt(lane));
+            if (q < qn) {
+              active = true;
+              ps.start(db, lds.qkey[q]);
+              if (EXTRAS && out.lookups && !ps.skip) {
+                st.my_lookups++;
+                st.my_lines += table_lines(db, lds.qkey[q]);
+              }
+            }
+          }
+          next += (uint32_t) __popcll(R);
+          if (next > qn) next = qn;
         }
       }
     }
