@@ -50,6 +50,7 @@ const ClassifyKernels kVariants[3] = {
 };
 __global__ void k_get_batch(DbView, const uint64_t *, uint32_t *, uint64_t);
 __global__ void k_max_value(DbView, unsigned int *);
+__global__ void k_pad_copy(uint32_t *, uint64_t, uint32_t);
 __global__ void k_build(BuildView, const char *, const SubBlock *, uint64_t);
 }  // namespace k2g
 
@@ -297,25 +298,35 @@ extern "C" int k2g_db_open(const char *hash_k2d, const char *taxo_k2d, const cha
   v.toggle_mask = ro.toggle & v.lmer_mask;
   v.min_hash = ro.min_hash;
 
-  // every stored value must be a node of the taxonomy
+  // every stored value must be a node of the taxonomy; an empty cell and at
+  // least 64 cells allow the lean probe, whose padding copies the first cells
   {
-    unsigned int *d_max = nullptr, h_max = 0;
-    e = hipMalloc((void **) &d_max, sizeof(unsigned int));
-    if (e == hipSuccess) e = hipMemset(d_max, 0, sizeof(unsigned int));
+    unsigned int *d_max = nullptr, h_max[2] = {0, 0};
+    e = hipMalloc((void **) &d_max, sizeof(h_max));
+    if (e == hipSuccess) e = hipMemset(d_max, 0, sizeof(h_max));
     if (e == hipSuccess) {
       const unsigned grid = (unsigned) std::min<uint64_t>((capacity + 255) / 256, 65536);
       hipLaunchKernelGGL(k_max_value, dim3(grid), dim3(256), 0, 0, v, d_max);
       e = hipGetLastError();
     }
-    if (e == hipSuccess) e = hipMemcpy(&h_max, d_max, sizeof(h_max), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(h_max, d_max, sizeof(h_max), hipMemcpyDeviceToHost);
     if (d_max) (void) hipFree(d_max);
+    if (e == hipSuccess && h_max[1] && capacity >= 64) {
+      if (padded > capacity) {
+        hipLaunchKernelGGL(k_pad_copy, dim3(1), dim3(64), 0, 0, db->d_cells, capacity,
+                           (uint32_t) (padded - capacity));
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+      }
+      v.lean_probe = 1;
+    }
     if (e != hipSuccess) {
       k2g_db_close(db);
       return fail(K2G_ESOFTWARE, std::string("table check failed: ") + hipGetErrorString(e));
     }
-    if (h_max >= v.node_count) {
+    if (h_max[0] >= v.node_count) {
       k2g_db_close(db);
-      return fail(K2G_EDATA, "hash table holds taxid " + std::to_string(h_max) +
+      return fail(K2G_EDATA, "hash table holds taxid " + std::to_string(h_max[0]) +
                                  " but the taxonomy has only " + std::to_string(v.node_count) +
                                  " nodes");
     }
@@ -1440,11 +1451,11 @@ struct Checksum {
     x ^= x >> 33;
     h = x;
   }
+  // n elements of p (p may be null when n is 0: an empty array has no data)
   template <class T>
   void array(const T *p, uint64_t n) {
-    word(p ? n : ~0ull);
-    if (p)
-      for (uint64_t i = 0; i < n; i++) word((uint64_t) p[i]);
+    word(n);
+    for (uint64_t i = 0; i < n; i++) word((uint64_t) p[i]);
   }
 };
 }  // namespace
@@ -1458,6 +1469,9 @@ extern "C" int k2g_results_checksum(const k2g_results *r, uint64_t *out) {
   c.array(r->call, n);
   c.array(r->hit_groups, n);
   c.array(r->total_kmers, n);
+  // which optional outputs are present is part of the result; a present
+  // output's arrays are hashed by length and content only
+  c.word((r->run_off ? 1u : 0u) | (r->hll_off ? 2u : 0u) | (r->kmer_counts ? 4u : 0u));
   c.array(r->run_off, r->run_off ? n + 1 : 0);
   c.array(r->runs, r->run_off ? 2 * r->run_off[n] : 0);
   c.array(r->hll_off, r->hll_off ? n + 1 : 0);

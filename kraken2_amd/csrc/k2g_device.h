@@ -20,7 +20,10 @@ constexpr uint64_t kNoCand = ~0ULL;  // ring sentinel: no valid l-mer
 
 // Everything a kernel needs to read the resident DB.
 struct DbView {
-  const uint32_t *cells;   // capacity cells, zero padded to a multiple of 64
+  // capacity cells, padded to a multiple of 64; with lean_probe the padding
+  // holds copies of the first cells, so a 4-cell group read across the end
+  // sees the cells a wrapping linear probe visits next
+  const uint32_t *cells;
   uint64_t capacity;
   uint64_t barrett;        // floor(2^64 / capacity) (0 when capacity == 1)
   uint32_t key_bits, value_bits;
@@ -34,6 +37,10 @@ struct DbView {
   uint64_t toggle_mask;    // already masked to the l-mer width
   uint64_t lmer_mask;
   uint64_t min_hash;       // minimum_acceptable_hash_value
+  // 1 when capacity >= 64, the padding copies the first cells and some cell
+  // is empty: every probe then ends at an empty or matching cell before it
+  // wraps all the way round, so it needs no count of cells left to visit
+  uint32_t lean_probe;
 };
 
 // MurmurHash3 64-bit finalizer (src/kv_store.h:59-67)
@@ -50,10 +57,9 @@ __device__ __forceinline__ uint64_t fmix64(uint64_t k) {
 // floor(x / d) or one less, so at most one correction step remains.
 __device__ __forceinline__ uint64_t mod_capacity(uint64_t x, uint64_t d, uint64_t m) {
   if (d == 1) return 0;
-  uint64_t q = __umul64hi(x, m);
-  uint64_t r = x - q * d;
-  while (r >= d) r -= d;
-  return r;
+  const uint64_t q = __umul64hi(x, m);
+  const uint64_t r = x - q * d;
+  return r >= d ? r - d : r;
 }
 
 // CompactHashTable::Get with linear probing (src/compact_hash.h:354-374,
@@ -137,6 +143,36 @@ __device__ __forceinline__ uint64_t uniform64(uint64_t v) {
   const uint32_t lo = (uint32_t) __builtin_amdgcn_readfirstlane((int) (uint32_t) v);
   const uint32_t hi = (uint32_t) __builtin_amdgcn_readfirstlane((int) (uint32_t) (v >> 32));
   return ((uint64_t) hi << 32) | lo;
+}
+
+// A lane's value read by every lane (lane wave-uniform): no LDS round trip.
+__device__ __forceinline__ uint32_t readlane32(uint32_t v, int lane) {
+  return (uint32_t) __builtin_amdgcn_readlane((int) v, lane);
+}
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, int lane) {
+  const uint32_t lo = (uint32_t) __builtin_amdgcn_readlane((int) (uint32_t) v, lane);
+  const uint32_t hi = (uint32_t) __builtin_amdgcn_readlane((int) (uint32_t) (v >> 32), lane);
+  return ((uint64_t) hi << 32) | lo;
+}
+
+// Lane i receives lane i-1's v and lane 0 receives `first` (a DPP wave
+// shift: one move per 32 bits, no LDS).
+__device__ __forceinline__ uint32_t wave_shr1_32(uint32_t v, uint32_t first) {
+  return (uint32_t) __builtin_amdgcn_update_dpp((int) first, (int) v, 0x138 /* wave_shr:1 */, 0xf,
+                                                0xf, false);
+}
+__device__ __forceinline__ uint64_t wave_shr1(uint64_t v, uint64_t first) {
+  const uint32_t lo = (uint32_t) __builtin_amdgcn_update_dpp(
+      (int) (uint32_t) first, (int) (uint32_t) v, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
+  const uint32_t hi = (uint32_t) __builtin_amdgcn_update_dpp(
+      (int) (uint32_t) (first >> 32), (int) (uint32_t) (v >> 32), 0x138, 0xf, 0xf, false);
+  return ((uint64_t) hi << 32) | lo;
+}
+
+// This lane's bit of a wave-uniform mask (the mask itself serves as the
+// lane predicate: no vector instructions).
+__device__ __forceinline__ bool lane_bit(uint64_t wave_mask) {
+  return __builtin_amdgcn_inverse_ballot_w64(wave_mask);
 }
 
 __device__ __forceinline__ uint64_t lanemask_lt(int lane) { return (1ULL << lane) - 1ULL; }

@@ -252,7 +252,40 @@ struct ProbeState {
     idx = mod_capacity(hc, db.capacity, db.barrett);
     left = db.capacity;
   }
+  // One 16-byte group: the cells from idx to the group's end, in probe order.
   __device__ __forceinline__ bool step(const DbView &db, uint32_t &out) {
+    return db.lean_probe ? step_lean(db, out) : step_bounded(db, out);
+  }
+  // The padding copies the first cells, so the group's cells past the end of
+  // the table are the ones the probe visits after wrapping, and an empty
+  // cell somewhere ends every probe: no bounds or remaining-cell count.
+  __device__ __forceinline__ bool step_lean(const DbView &db, uint32_t &out) {
+    const uint32_t vb = db.value_bits;
+    const uint32_t vmask = (1u << vb) - 1u;
+    const uint64_t group = idx & ~3ULL;
+    const u32x4 v = *(const u32x4 *) (db.cells + group);
+    const uint32_t off = (uint32_t) idx & 3u;
+    const uint32_t c4[4] = {v.x, v.y, v.z, v.w};
+    uint32_t val = 0;
+    bool stop = false;
+#pragma unroll
+    for (int j = 3; j >= 0; j--) {  // the lowest stopping cell wins
+      const uint32_t c = c4[j];
+      const bool st = (uint32_t) j >= off && ((c & vmask) == 0u || (c >> vb) == ck);
+      val = st ? (c & vmask) : val;  // an empty cell gives 0
+      stop |= st;
+    }
+    if (stop || skip) {
+      out = skip ? 0u : val;
+      return true;
+    }
+    idx = group + 4;
+    if (idx >= db.capacity) idx -= db.capacity;
+    return false;
+  }
+  // General tables (no empty cell, or fewer than 64 cells): stops at the
+  // table's end and after capacity cells.
+  __device__ __forceinline__ bool step_bounded(const DbView &db, uint32_t &out) {
     const uint32_t vb = db.value_bits;
     const uint32_t vmask = (1u << vb) - 1u;
     const uint64_t group = idx & ~3ULL;
@@ -331,21 +364,32 @@ __device__ void scan_stream(const DbView &db, const ClassifyParams &P, const Fra
       const uint64_t P2 = dna ? 0 : __ballot(good && (code & 4u));
       const uint64_t P3 = dna ? 0 : __ballot(good && (code & 8u));
       const uint64_t A = __ballot(amb);
-      const uint64_t mA = A & lanemask_le(lane);
-      const int64_t lastN = mA ? (int64_t) (c + top_lane(mA)) : last_amb;
+      // A clean chunk (no ambiguous base in it or in the k-1 positions before
+      // it, and past the stream's first k-1 positions) has a valid l-mer and a
+      // non-ambiguous k-mer at every position in bounds.
+      const bool clean = A == 0 && c + 1 >= k && last_amb < (int64_t) (c + 1) - (int64_t) k;
+      bool lvalid, kvalid, nonamb;
+      if (clean) {
+        lvalid = kvalid = nonamb = inb;
+      } else {
+        const uint64_t mA = A & lanemask_le(lane);
+        const int64_t lastN = mA ? (int64_t) (c + top_lane(mA)) : last_amb;
+        lvalid = inb && p + 1 >= l && lastN < (int64_t) (p + 1 - l);
+        kvalid = inb && p + 1 >= k;
+        nonamb = kvalid && ((k == l) ? lvalid : (lastN < (int64_t) (p + 2 - k)));
+      }
 
       // candidate of the l-mer ending at p
-      const bool lvalid = inb && p + 1 >= l && lastN < (int64_t) (p + 1 - l);
       uint64_t cand = kNoCand;
       if (dna) {
         // The chunk's bases as packed 2-bit streams (wave-uniform): big-endian
         // (first base in the top bits of its 32-base word) for forward l-mers,
-        // little-endian for reverse complements; each lane then cuts its
-        // l-mer out of two adjacent words.
+        // little-endian complemented bases for reverse complements; each lane
+        // then cuts its l-mer out of two adjacent words.
         const uint64_t r0l = replicate_bits((uint32_t) P0), r0h = replicate_bits((uint32_t) (P0 >> 32));
         const uint64_t r1l = replicate_bits((uint32_t) P1), r1h = replicate_bits((uint32_t) (P1 >> 32));
-        const uint64_t LEl = (r0l & kEvenBits) | (r1l & kOddBits);
-        const uint64_t LEh = (r0h & kEvenBits) | (r1h & kOddBits);
+        const uint64_t LEl = ~((r0l & kEvenBits) | (r1l & kOddBits));
+        const uint64_t LEh = ~((r0h & kEvenBits) | (r1h & kOddBits));
         const uint64_t BEl = __builtin_bitreverse64((r1l & kEvenBits) | (r0l & kOddBits));
         const uint64_t BEh = __builtin_bitreverse64((r1h & kEvenBits) | (r0h & kOddBits));
         if (lvalid) {
@@ -355,9 +399,9 @@ __device__ void scan_stream(const DbView &db, const ClassifyParams &P, const Fra
           const uint64_t fwd = ((flo >> s) | ((fhi << 1) << (63u - s))) & lmask2;
           const uint64_t top = up ? LEh : LEl, bot = up ? LEl : pLEh;
           const uint64_t t = (top << s) | ((bot >> 1) >> (63u - s));
-          const uint64_t rc = (~t) >> (64u - 2u * l);
+          const uint64_t rc = t >> (64u - 2u * l);
           uint64_t canon;
-          if (db.revcom_version == 0) {  // src/mmscanner.cc:193-215, old databases
+          if (W == 0 && db.revcom_version == 0) {  // src/mmscanner.cc:193-215, old databases
             const uint32_t d = 2u * (32u - l);
             const uint64_t rc0 = ((rc << d) | ((1ULL << d) - 1ULL)) & lmask2;
             canon = fwd < rc0 ? fwd : rc0;
@@ -389,9 +433,6 @@ __device__ void scan_stream(const DbView &db, const ClassifyParams &P, const Fra
       wave_sync();
 
       // k-mer ending at p: minimum over its window of l-mer candidates
-      const bool kvalid = inb && p + 1 >= k;
-      const bool nonamb =
-          kvalid && ((k == l) ? lvalid : (lastN < (int64_t) (p + 2 - k)));
       uint64_t mn = ~0ULL;
       if (W != 0) {  // compile-time window: reads at fixed offsets from one address
         const uint64_t *r = &lds.ring[kRingGuard + slot - (W - 1)];
@@ -405,11 +446,17 @@ __device__ void scan_stream(const DbView &db, const ClassifyParams &P, const Fra
       }
       mn ^= db.toggle_mask;
 
-      // token decision against the previous non-ambiguous minimizer
+      // token decision against the previous non-ambiguous minimizer (in a
+      // clean chunk the lanes in bounds, the next lower lane for each)
       const uint64_t NA = __ballot(nonamb);
-      const uint64_t prevm = NA & lanemask_lt(lane);
-      const uint64_t before = shfl64(mn, prevm ? top_lane(prevm) : lane);
-      const uint64_t prev_min = prevm ? before : last_min;
+      uint64_t prev_min;
+      if (clean) {
+        prev_min = wave_shr1(mn, last_min);
+      } else {
+        const uint64_t prevm = NA & lanemask_lt(lane);
+        const uint64_t before = shfl64(mn, prevm ? top_lane(prevm) : lane);
+        prev_min = prevm ? before : last_min;
+      }
       const bool lookup = nonamb && mn != prev_min;
       const uint64_t LK = __ballot(lookup);
       const uint64_t E = __ballot(kvalid);
@@ -423,7 +470,7 @@ __device__ void scan_stream(const DbView &db, const ClassifyParams &P, const Fra
       qn += (uint32_t) __popcll(LK);
       g++;
       if (A) last_amb = (int64_t) (c + top_lane(A));
-      if (NA) last_min = uniform64(shfl64(mn, top_lane(NA)));
+      if (NA) last_min = readlane64(mn, top_lane(NA));
       pp0 = P0;
       pp1 = P1;
       pp2 = P2;
@@ -457,7 +504,7 @@ __device__ void scan_stream(const DbView &db, const ClassifyParams &P, const Fra
         }
         const uint64_t R = __ballot(!active) & (next < qn ? ~0ULL : 0ULL);
         if (R) {
-          if ((R >> lane) & 1ULL) {
+          if (lane_bit(R)) {
             q = next + (uint32_t) __popcll(R & lanemask_lt(lane));
             if (q < qn) {
               active = true;
@@ -481,9 +528,9 @@ __device__ void scan_stream(const DbView &db, const ClassifyParams &P, const Fra
       const uint64_t LK = uniform64(lds.cmask[gi][1]);
       uint64_t E = uniform64(lds.cmask[gi][2]);
       const uint32_t qb = uniform32(lds.cqb[gi]);
-      const bool nonamb = (NA >> lane) & 1ULL;
-      const bool lookup = (LK >> lane) & 1ULL;
-      const bool kvalid = (E >> lane) & 1ULL;
+      const bool nonamb = lane_bit(NA);
+      const bool lookup = lane_bit(LK);
+      const bool kvalid = lane_bit(E);
       const uint64_t srcm = LK & lanemask_le(lane);
       const uint32_t tq = srcm ? lds.qval[qb + __popcll(srcm) - 1] : last_taxon;
       const uint32_t tax_i = nonamb ? tq : 0u;
@@ -501,7 +548,7 @@ __device__ void scan_stream(const DbView &db, const ClassifyParams &P, const Fra
       }
 
       // per-taxon k-mer counts and HLL records for LOOKUP hits
-      if (EXTRAS && ((HIT >> lane) & 1ULL)) {
+      if (EXTRAS && lane_bit(HIT)) {
         if (out.kmer_counts) atomicAdd(&out.kmer_counts[tq], 1ULL);
         if (out.recs) {
           const uint32_t r = st.nrec + __popcll(HIT & lanemask_lt(lane));
@@ -518,23 +565,23 @@ __device__ void scan_stream(const DbView &db, const ClassifyParams &P, const Fra
 
       // run-length encode the elements of this chunk
       if (E) {
-        const bool isE = (E >> lane) & 1ULL;
+        const bool isE = lane_bit(E);
         const uint32_t code_i = nonamb ? tax_i : kCodeAmbig;
-        const uint32_t prev_code = shfl32(code_i, lane > 0 ? lane - 1 : 0);
+        // E is a contiguous run of lanes and the code is constant between
+        // boundaries, so the run a boundary closes has the lower lane's code
+        const uint32_t prev_code = wave_shr1_32(code_i, code_i);
         const int lo = __ffsll((long long) E) - 1;
         const bool boundary =
             isE && (lane == lo ? (st.run_count == 0 || code_i != st.run_code) : code_i != prev_code);
         const uint64_t B = __ballot(boundary);
         const uint64_t pb = B & lanemask_lt(lane);
-        const int pbl = pb ? top_lane(pb) : lane;
-        const uint32_t pb_code = shfl32(code_i, pbl);
         const uint32_t drop = (st.run_count == 0) ? 1u : 0u;
         if (boundary) {
           uint32_t cc, cn;
           bool has = true;
           if (pb) {
-            cc = pb_code;
-            cn = (uint32_t) (lane - pbl);
+            cc = prev_code;
+            cn = (uint32_t) (lane - top_lane(pb));
           } else {
             cc = st.run_code;
             cn = st.run_count + (uint32_t) (lane - lo);
@@ -549,7 +596,7 @@ __device__ void scan_stream(const DbView &db, const ClassifyParams &P, const Fra
         const int hi = top_lane(E);
         if (B) {
           const int lb = top_lane(B);
-          st.run_code = uniform32(shfl32(code_i, lb));
+          st.run_code = readlane32(code_i, lb);
           st.run_count = (uint32_t) (hi - lb + 1);
         } else {
           st.run_count += (uint32_t) __popcll(E);
@@ -606,7 +653,7 @@ __device__ void classify_fragment(const DbView &db, const ClassifyParams &P, con
     const char *m = in.seq + (mate ? o1 : o0);
     const uint64_t L = mate ? o2 - o1 : o1 - o0;
     for (int fr = 0; fr < nframes; fr++) {
-      if (db.k - db.l == 4)  // the default nucleotide window (k=35, l=31)
+      if (db.k - db.l == 4 && db.revcom_version != 0)  // the default window (k=35, l=31)
         scan_stream<DNA, EXTRAS, 5>(db, P, out, base, cap, m, L, fr, st, tab, lds, bad, lane);
       else
         scan_stream<DNA, EXTRAS, 0>(db, P, out, base, cap, m, L, fr, st, tab, lds, bad, lane);
@@ -778,19 +825,32 @@ __global__ __launch_bounds__(256) void k_get_batch(DbView db, const uint64_t *ke
 }
 
 // Largest cell value (used to check that every stored taxid is a node).
+// out[0]: the largest stored value; out[1]: 1 if some cell is empty.
 __global__ __launch_bounds__(256) void k_max_value(DbView db, unsigned int *out) {
   const uint32_t vmask = (1u << db.value_bits) - 1u;
   uint32_t best = 0;
+  bool empty = false;
   for (uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x; i < db.capacity;
        i += (uint64_t) gridDim.x * blockDim.x) {
     const uint32_t v = db.cells[i] & vmask;
     best = v > best ? v : best;
+    empty |= v == 0u;
   }
   for (int d = 1; d < kWave; d <<= 1) {
     const uint32_t o = (uint32_t) __shfl_xor((unsigned int) best, d);
     best = o > best ? o : best;
   }
-  if ((threadIdx.x & (kWave - 1)) == 0) atomicMax(out, best);
+  const bool any_empty = __ballot(empty) != 0;
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    atomicMax(out, best);
+    if (any_empty) atomicOr(out + 1, 1u);
+  }
+}
+
+// Copies the first cells into the padding after the table (capacity >= 64).
+__global__ void k_pad_copy(uint32_t *cells, uint64_t capacity, uint32_t n) {
+  const uint32_t t = threadIdx.x;
+  if (t < n) cells[capacity + t] = cells[t];
 }
 
 }  // namespace k2g

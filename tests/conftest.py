@@ -36,3 +36,16 @@ def synth_db_rc0(synth_db, tmp_path_factory):
     raw[48:52] = (0).to_bytes(4, "little")
     open(os.path.join(str(d), "opts.k2d"), "wb").write(bytes(raw))
     return str(d)
+
+
+@pytest.fixture(scope="session")
+def edge_tables(synth_db, tmp_path_factory):
+    """synth_db's scanner and taxonomy with random hash tables at the probe's
+    edges: 8-bit compact keys (frequent key matches along long chains), one
+    table with a single empty cell and a capacity not a multiple of 4 (probes
+    wrap past the last cell), one with no empty cell, and one below 64 cells."""
+    from helpers import write_custom_table
+    base = tmp_path_factory.mktemp("edge_tables")
+    specs = {"one_empty_102": (102, 1), "full_128": (128, 0), "small_37": (37, 15), "sparse_1001": (1001, 400)}
+    return {name: write_custom_table(synth_db, os.path.join(str(base), name), cap, 8, empty, seed=i + 1)
+            for i, (name, (cap, empty)) in enumerate(specs.items())}

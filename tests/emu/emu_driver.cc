@@ -67,6 +67,14 @@ int emu_classify(const uint32_t *cells, uint64_t capacity, uint32_t kb, uint32_t
   v.spaced_mask = spaced;
   v.toggle_mask = toggle & v.lmer_mask;
   v.min_hash = min_hash;
+  // the library's load-time rule: an empty cell and at least 64 cells give
+  // the lean probe (the caller pads the cells with copies of the first ones)
+  {
+    const uint32_t vmask = (1u << vb) - 1u;
+    bool empty = false;
+    for (uint64_t i = 0; i < capacity && !empty; i++) empty = (cells[i] & vmask) == 0;
+    v.lean_probe = (empty && capacity >= 64) ? 1u : 0u;
+  }
   ClassifyParams P{conf, min_groups, quick, tab_cap};
   FragBatch in{seq, off, paired, paired_all, n, order, unit_start, n_units};
   FragOut out{call,    groups, total, n_runs, status,      n_recs,  run_off,

@@ -100,3 +100,15 @@ inline T atomicMax(T *p, U v) {
 #define __hip_atomic_store(p, v, order, scope) __atomic_store_n((p), (v), __ATOMIC_SEQ_CST)
 
 inline int __builtin_amdgcn_readfirstlane(int v) { return __shfl(v, 0); }
+inline int __builtin_amdgcn_readlane(int v, int lane) { return __shfl(v, lane); }
+inline bool __builtin_amdgcn_inverse_ballot_w64(unsigned long long m) { return (m >> threadIdx.x) & 1ULL; }
+// Only the wave shift right by one lane (dpp_ctrl 0x138) is emulated: lane i
+// gets lane i-1's src, lane 0 keeps `old`.
+inline int __builtin_amdgcn_update_dpp(int old, int src, int ctrl, int, int, bool) {
+  if (ctrl != 0x138) {
+    std::fprintf(stderr, "emu: unsupported dpp_ctrl %#x\n", ctrl);
+    std::abort();
+  }
+  const int up = __shfl(src, (int) threadIdx.x - 1);
+  return threadIdx.x == 0 ? old : up;
+}

@@ -45,6 +45,32 @@ def make_synthetic_db(d, seed=1, levels="2,2,3", ranks=None, genome_len=20000, s
     return d
 
 
+def write_custom_table(src, dst, capacity, key_bits, empty_cells, seed):
+    """Copies DB directory src to dst with hash.k2d replaced by a random table
+    of `capacity` cells: key_bits-bit compact keys, `empty_cells` empty cells
+    at random positions and random valid taxa elsewhere.  Long probe chains,
+    probes that wrap past the last cell and tables with no empty cell (every
+    probe of an absent key visits all cells) are then common."""
+    import shutil
+    os.makedirs(dst, exist_ok=True)
+    for f in ("taxo.k2d", "opts.k2d", "library.fna"):
+        shutil.copy(os.path.join(src, f), os.path.join(dst, f))
+    tx = open(os.path.join(src, "taxo.k2d"), "rb").read()
+    nodes = int(np.frombuffer(tx[8:16], np.uint64)[0])
+    vb = 32 - key_bits
+    assert nodes < (1 << vb)
+    rng = np.random.default_rng(seed)
+    vals = rng.integers(1, nodes, capacity, dtype=np.uint64)
+    keys = rng.integers(0, 1 << key_bits, capacity, dtype=np.uint64)
+    cells = ((keys << np.uint64(vb)) | vals).astype(np.uint32)
+    cells[rng.choice(capacity, empty_cells, replace=False)] = 0
+    hdr = np.array([capacity, capacity - empty_cells, key_bits, vb], np.uint64)
+    with open(os.path.join(dst, "hash.k2d"), "wb") as f:
+        f.write(hdr.tobytes())
+        f.write(cells.tobytes())
+    return dst
+
+
 def read_fasta(path):
     names, seqs = [], []
     with open(path) as f:

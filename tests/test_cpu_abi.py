@@ -60,3 +60,24 @@ def test_results_checksum_is_host_only_and_sensitive():
         assert checksum() != base
         arr[i] ^= 1
     assert checksum() == base
+
+    # an empty array hashes the same whether its data pointer is null or not
+    # (a server result of a request without HLL records has no record data)
+    def empty_hll(null_data):
+        r = L.Results()
+        r.n = 1
+        one = np.zeros(1, np.uint32)
+        r.call, r.hit_groups, r.total_kmers = ptr(one), ptr(one), ptr(one)
+        ro = np.array([0, 1], np.uint64)
+        rr = np.array([0, 5], np.uint32)
+        ho = np.zeros(2, np.uint64)
+        hd = np.zeros(2, np.uint32)
+        r.run_off, r.runs = ptr(ro), ptr(rr)
+        r.hll_off = ptr(ho)
+        if not null_data:
+            r.hll = ptr(hd)
+        h = ctypes.c_uint64()
+        assert lib.k2g_results_checksum(ctypes.byref(r), ctypes.byref(h)) == 0
+        return h.value
+
+    assert empty_hll(True) == empty_hll(False)

@@ -128,3 +128,18 @@ def test_emulated_revcom_version0(emu_lib, synth_db_rc0):
     reads = edge_reads(genomes, 40, seed=71)
     seq, off = k2.pack_fragments(reads)
     _check(emu, lib, synth_db_rc0, seq, off, np.zeros(len(reads), np.uint8), g=0)
+
+
+@pytest.mark.parametrize("name", ["one_empty_102", "full_128", "small_37", "sparse_1001"])
+def test_emulated_probe_edge_tables(emu_lib, edge_tables, name):
+    """Random tables with 8-bit compact keys: chains that wrap past the last
+    cell into the padding copies (lean probe), a table with no empty cell and
+    one under 64 cells (bounded probe)."""
+    import kraken2_amd as k2
+    emu, lib = emu_lib
+    d = edge_tables[name]
+    _, genomes = read_fasta(os.path.join(d, "library.fna"))
+    reads = edge_reads(genomes, 16, seed=81, min_len=40, max_len=160)
+    seq, off = k2.pack_fragments(reads)
+    out = _check(emu, lib, d, seq, off, np.zeros(len(reads), np.uint8), g=0)
+    assert (out["groups"] > 0).any()

@@ -189,3 +189,31 @@ def test_classify_revcom_version0(synth_db_rc0):
         _compare(k2, db, out, ref, len(reads))
     batch.close()
     db.close()
+
+
+@pytest.mark.parametrize("name", ["one_empty_102", "full_128", "small_37", "sparse_1001"])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_probe_edge_tables(edge_tables, name, mode):
+    """Random tables with 8-bit compact keys (frequent key matches along long
+    chains): probes that wrap past the last cell (lean probe over the padding
+    copies), a table with no empty cell and one under 64 cells (bounded
+    probe), through the library's load path; plus Get on random keys."""
+    import kraken2_amd as k2
+    d = edge_tables[name]
+    db = k2.Db.from_dir(d)
+    batch = k2.Batch(db)
+    orc = Oracle()
+    h = orc.open(d)
+    try:
+        _, genomes = read_fasta(os.path.join(d, "library.fna"))
+        reads = edge_reads(genomes, 300, seed=91 + mode, min_len=40, max_len=300)
+        seq, off = k2.pack_fragments(reads)
+        out = batch.classify(seq, off, mode=mode)
+        ref = orc.classify(h, seq, off)
+        _compare(k2, db, out, ref, len(reads))
+        assert (out.hit_groups > 0).any()
+        keys = np.random.default_rng(7).integers(0, 2**62, size=5000, dtype=np.uint64)
+        assert np.array_equal(db.get_batch(keys), orc.get_batch(h, keys))
+    finally:
+        batch.close()
+        db.close()

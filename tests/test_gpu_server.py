@@ -148,3 +148,23 @@ def test_server_pinned_inputs_and_checksums(env):
             assert h == ref
     finally:
         srv.close()
+
+
+def test_server_checksum_without_hll_records(env):
+    """A request whose fragments give no HLL record (one random pair, no
+    hits) has an empty record array on both paths: equal checksums."""
+    k2m, db, batch, genomes = env
+    import kraken2_amd as k2
+    rng = np.random.default_rng(17)
+    m1 = ["".join(rng.choice(list("ACGT"), 150)) for _ in range(1)]
+    m2 = ["".join(rng.choice(list("ACGT"), 150)) for _ in range(1)]
+    seq, off = k2.pack_fragments(m1, m2)
+    p = dict(paired=True, confidence=0.1, min_hit_groups=2, hll=True, kmer_counts=True)
+    out = batch.classify(seq, off, **p)
+    assert len(out.hll) == 0
+    srv = k2.Server(db, 64 << 20, slots=2)
+    try:
+        h, _ = srv.wait(srv.submit(seq, off, **p), checksum_only=True)
+    finally:
+        srv.close()
+    assert h == batch.classify_checksum(seq, off, **p)
