@@ -169,6 +169,17 @@ __device__ __forceinline__ uint64_t wave_shr1(uint64_t v, uint64_t first) {
   return ((uint64_t) hi << 32) | lo;
 }
 
+// A wave-uniform value kept in a vector register: vector instructions read
+// it directly, and it takes no scalar register (scalar registers spilled to
+// vector lanes cost a readlane at every use).
+template <class T>
+__device__ __forceinline__ T in_vgpr(T x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  asm volatile("" : "+v"(x));
+#endif
+  return x;
+}
+
 // This lane's bit of a wave-uniform mask (the mask itself serves as the
 // lane predicate: no vector instructions).
 __device__ __forceinline__ bool lane_bit(uint64_t wave_mask) {

@@ -177,15 +177,14 @@ struct RunState {
 // Writes run idx of the fragment and adds a taxon run's count to the hit
 // table (src/classify.cc:846-888).
 template <class Table>
-__device__ __forceinline__ void close_run(const FragOut &out, uint64_t base, uint64_t cap,
-                                          uint32_t idx, uint32_t code, uint32_t count,
-                                          Table &tab, int *bad) {
+__device__ __forceinline__ void close_run(uint32_t *runs_at, uint64_t cap, uint32_t idx,
+                                          uint32_t code, uint32_t count, Table &tab, int *bad) {
   typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
   if (idx < cap) {
     u32x2 r;
     r.x = code;
     r.y = count;
-    *(u32x2 *) (out.runs + 2 * (base + idx)) = r;
+    *(u32x2 *) (runs_at + 2 * (uint64_t) idx) = r;
   } else {
     atomicOr(bad, 2);
   }
@@ -202,7 +201,7 @@ __device__ void push_marker(const FragOut &out, uint64_t base, uint64_t cap, Run
     return;
   }
   if (st.run_count > 0) {
-    if (lane == 0) close_run(out, base, cap, st.n_runs, st.run_code, st.run_count, tab, bad);
+    if (lane == 0) close_run(out.runs + 2 * base, cap, st.n_runs, st.run_code, st.run_count, tab, bad);
     st.n_runs++;
   }
   st.run_code = code;
@@ -223,16 +222,24 @@ constexpr int kQueue = 192;       // queued lookups per probe batch (>= 64 + 64)
 constexpr int kRingGuard = 8;     // windows of up to 9 l-mers read without wrapping
 constexpr int kBatchChunks = 8;   // chunks per probe batch
 
+// KEEP_KEYS: the minimizers stay readable after their probes (HLL records
+// need them); otherwise each value overwrites its own key's low half, which
+// no lane reads again once that key's probe has started, and the wave's LDS
+// shrinks by the value array.
+template <bool KEEP_KEYS>
 struct WaveLds {
   uint64_t ring[kRingGuard + 128];  // window candidates: position p at kRingGuard + (p & 127),
                                     // slots 120..127 mirrored below index kRingGuard
   uint64_t qkey[kQueue];          // minimizers to look up, in k-mer order
-  uint32_t qval[kQueue];          // their table values
+  uint32_t qval_[KEEP_KEYS ? kQueue : 1];  // their table values (KEEP_KEYS)
   uint64_t cmask[kBatchChunks][3];  // per queued chunk: non-ambiguous, LOOKUP, k-mer-end masks
   uint32_t cqb[kBatchChunks];     // per queued chunk: queue index of its first lookup
   uint32_t tkey[256];             // LDS hit table
   uint32_t tcnt[256];
   int bad;
+  __device__ __forceinline__ uint32_t &qval(uint32_t i) {
+    return KEEP_KEYS ? qval_[i] : reinterpret_cast<uint32_t *>(qkey)[2 * i];
+  }
 };
 
 // CompactHashTable::Get (src/compact_hash.h:354-374, 493-500) as a resumable
@@ -241,29 +248,47 @@ struct WaveLds {
 // all four without branching, and returns true with the value once the probe
 // ends.  Keys whose hash is below min_hash are not looked up
 // (src/classify.cc:819-823) and read as 0.
+// The probe's table constants, copied once per stream into vector registers.
+struct ProbeTable {
+  const uint32_t *cells;
+  uint64_t capacity, barrett, min_hash;
+  // vgpr: hold them in vector registers (the lean kernel variants have the
+  // registers to spare; the others keep them scalar)
+  __device__ __forceinline__ ProbeTable(const DbView &db, bool vgpr)
+      : cells(vgpr ? in_vgpr(db.cells) : db.cells),
+        capacity(vgpr ? in_vgpr(db.capacity) : db.capacity),
+        barrett(vgpr ? in_vgpr(db.barrett) : db.barrett),
+        min_hash(vgpr ? in_vgpr(db.min_hash) : db.min_hash) {}
+};
+
 struct ProbeState {
   uint64_t idx = 0, left = 0;
   uint32_t ck = 0;
   bool skip = false;  // hash below min_hash (src/classify.cc:819-823): value 0
-  __device__ __forceinline__ void start(const DbView &db, uint64_t key) {
+  __device__ __forceinline__ void start(const DbView &db, const ProbeTable &t, uint64_t key) {
     const uint64_t hc = fmix64(key);
-    skip = hc < db.min_hash;
+    skip = hc < t.min_hash;
     ck = (uint32_t) (hc >> (64 - db.key_bits));
-    idx = mod_capacity(hc, db.capacity, db.barrett);
-    left = db.capacity;
+    if (db.capacity == 1) {
+      idx = 0;
+    } else {  // hc % capacity with one correction (see mod_capacity)
+      const uint64_t r = hc - __umul64hi(hc, t.barrett) * t.capacity;
+      idx = r >= t.capacity ? r - t.capacity : r;
+    }
+    left = t.capacity;
   }
   // One 16-byte group: the cells from idx to the group's end, in probe order.
-  __device__ __forceinline__ bool step(const DbView &db, uint32_t &out) {
-    return db.lean_probe ? step_lean(db, out) : step_bounded(db, out);
+  __device__ __forceinline__ bool step(const DbView &db, const ProbeTable &t, uint32_t &out) {
+    return db.lean_probe ? step_lean(db, t, out) : step_bounded(db, t, out);
   }
   // The padding copies the first cells, so the group's cells past the end of
   // the table are the ones the probe visits after wrapping, and an empty
   // cell somewhere ends every probe: no bounds or remaining-cell count.
-  __device__ __forceinline__ bool step_lean(const DbView &db, uint32_t &out) {
+  __device__ __forceinline__ bool step_lean(const DbView &db, const ProbeTable &t, uint32_t &out) {
     const uint32_t vb = db.value_bits;
     const uint32_t vmask = (1u << vb) - 1u;
     const uint64_t group = idx & ~3ULL;
-    const u32x4 v = *(const u32x4 *) (db.cells + group);
+    const u32x4 v = *(const u32x4 *) (t.cells + group);
     const uint32_t off = (uint32_t) idx & 3u;
     const uint32_t c4[4] = {v.x, v.y, v.z, v.w};
     uint32_t val = 0;
@@ -280,19 +305,19 @@ struct ProbeState {
       return true;
     }
     idx = group + 4;
-    if (idx >= db.capacity) idx -= db.capacity;
+    if (idx >= t.capacity) idx -= t.capacity;
     return false;
   }
   // General tables (no empty cell, or fewer than 64 cells): stops at the
   // table's end and after capacity cells.
-  __device__ __forceinline__ bool step_bounded(const DbView &db, uint32_t &out) {
+  __device__ __forceinline__ bool step_bounded(const DbView &db, const ProbeTable &t, uint32_t &out) {
     const uint32_t vb = db.value_bits;
     const uint32_t vmask = (1u << vb) - 1u;
     const uint64_t group = idx & ~3ULL;
-    const u32x4 v = *(const u32x4 *) (db.cells + group);
+    const u32x4 v = *(const u32x4 *) (t.cells + group);
     const uint32_t off = (uint32_t) (idx - group);
     uint64_t avail = 4u - off;
-    if (db.capacity - idx < avail) avail = db.capacity - idx;
+    if (t.capacity - idx < avail) avail = t.capacity - idx;
     if (left < avail) avail = left;
     const uint32_t c4[4] = {v.x, v.y, v.z, v.w};
     uint32_t stop = 0, val = 0;
@@ -308,7 +333,7 @@ struct ProbeState {
     }
     left -= avail;
     idx += avail;
-    if (idx >= db.capacity) idx = 0;
+    if (idx >= t.capacity) idx = 0;
     if (stop || left == 0 || skip) {
       out = (stop && !skip) ? val : 0u;
       return true;
@@ -335,7 +360,7 @@ struct ProbeState {
 template <bool DNA, bool EXTRAS, int W, class Table>
 __device__ void scan_stream(const DbView &db, const ClassifyParams &P, const FragOut &out,
                             uint64_t base, uint64_t cap, const char *m, uint64_t L, int fr,
-                            RunState &st, Table &tab, WaveLds &lds, int *bad, int lane) {
+                            RunState &st, Table &tab, WaveLds<EXTRAS> &lds, int *bad, int lane) {
   static_assert(W >= 0 && W <= kRingGuard + 1, "window too wide for the ring guard");
   constexpr bool dna = DNA;
   const uint64_t n = dna ? L : frame_len(L, fr);
@@ -348,6 +373,11 @@ __device__ void scan_stream(const DbView &db, const ClassifyParams &P, const Fra
   uint64_t pp0 = 0, pp1 = 0, pp2 = 0, pp3 = 0;  // previous chunk's planes (translated)
   uint64_t pBEh = 0, pLEh = 0;                   // previous chunk's upper stream words (DNA)
   const uint64_t lmask2 = (1ULL << (2 * l)) - 1ULL;
+  constexpr bool kVgprConsts = !EXTRAS;
+  const uint64_t tog = db.toggle_mask, smask = db.spaced_mask;
+  const ProbeTable ptab(db, kVgprConsts);
+  uint32_t *const runs_at = !kVgprConsts ? out.runs + 2 * base : in_vgpr(out.runs + 2 * base);
+  const uint64_t cap_v = !kVgprConsts ? cap : in_vgpr(cap);
   uint64_t c = 0;
   while (c < n) {
     // ---- A: scan chunks, queue their lookups
@@ -408,8 +438,8 @@ __device__ void scan_stream(const DbView &db, const ClassifyParams &P, const Fra
           } else {
             canon = fwd < rc ? fwd : rc;
           }
-          if (db.spaced_mask) canon &= db.spaced_mask;
-          cand = canon ^ db.toggle_mask;
+          if (db.spaced_mask) canon &= smask;
+          cand = canon ^ tog;
         }
         pBEh = BEh;
         pLEh = LEh;
@@ -424,8 +454,8 @@ __device__ void scan_stream(const DbView &db, const ClassifyParams &P, const Fra
         const uint64_t r2 = __builtin_bitreverse64(w2) >> (64 - l);
         const uint64_t r3 = __builtin_bitreverse64(w3) >> (64 - l);
         cand = (spread4(r3) << 3) | (spread4(r2) << 2) | (spread4(r1) << 1) | spread4(r0);
-        if (db.spaced_mask) cand &= db.spaced_mask;
-        cand ^= db.toggle_mask;
+        if (db.spaced_mask) cand &= smask;
+        cand ^= tog;
       }
       const uint32_t slot = (uint32_t) (p & 127);
       lds.ring[kRingGuard + slot] = cand;
@@ -444,7 +474,7 @@ __device__ void scan_stream(const DbView &db, const ClassifyParams &P, const Fra
           mn = v < mn ? v : mn;
         }
       }
-      mn ^= db.toggle_mask;
+      mn ^= tog;
 
       // token decision against the previous non-ambiguous minimizer (in a
       // clean chunk the lanes in bounds, the next lower lane for each)
@@ -487,7 +517,7 @@ __device__ void scan_stream(const DbView &db, const ClassifyParams &P, const Fra
       uint32_t q = (uint32_t) lane;
       bool active = q < qn;
       if (active) {
-        ps.start(db, lds.qkey[q]);
+        ps.start(db, ptab, lds.qkey[q]);
         if (EXTRAS && out.lookups && !ps.skip) {
           st.my_lookups++;
           st.my_lines += table_lines(db, lds.qkey[q]);
@@ -497,8 +527,8 @@ __device__ void scan_stream(const DbView &db, const ClassifyParams &P, const Fra
       while (__ballot(active)) {
         if (active) {
           uint32_t val;
-          if (ps.step(db, val)) {
-            lds.qval[q] = val;
+          if (ps.step(db, ptab, val)) {
+            lds.qval(q) = val;
             active = false;
           }
         }
@@ -508,7 +538,7 @@ __device__ void scan_stream(const DbView &db, const ClassifyParams &P, const Fra
             q = next + (uint32_t) __popcll(R & lanemask_lt(lane));
             if (q < qn) {
               active = true;
-              ps.start(db, lds.qkey[q]);
+              ps.start(db, ptab, lds.qkey[q]);
               if (EXTRAS && out.lookups && !ps.skip) {
                 st.my_lookups++;
                 st.my_lines += table_lines(db, lds.qkey[q]);
@@ -532,7 +562,7 @@ __device__ void scan_stream(const DbView &db, const ClassifyParams &P, const Fra
       const bool lookup = lane_bit(LK);
       const bool kvalid = lane_bit(E);
       const uint64_t srcm = LK & lanemask_le(lane);
-      const uint32_t tq = srcm ? lds.qval[qb + __popcll(srcm) - 1] : last_taxon;
+      const uint32_t tq = srcm ? lds.qval(qb + __popcll(srcm) - 1) : last_taxon;
       const uint32_t tax_i = nonamb ? tq : 0u;
       uint64_t HIT = __ballot(lookup && tq != 0);
 
@@ -589,7 +619,7 @@ __device__ void scan_stream(const DbView &db, const ClassifyParams &P, const Fra
           }
           if (has) {
             const uint32_t idx = st.n_runs + __popcll(pb) - drop;
-            close_run(out, base, cap, idx, cc, cn, tab, bad);
+            close_run(runs_at, cap_v, idx, cc, cn, tab, bad);
           }
         }
         st.n_runs += (uint32_t) __popcll(B) - (B ? drop : 0u);
@@ -603,7 +633,7 @@ __device__ void scan_stream(const DbView &db, const ClassifyParams &P, const Fra
         }
         st.elems += (uint64_t) __popcll(E);
       }
-      if (LK) last_taxon = uniform32(lds.qval[qb + __popcll(LK) - 1]);
+      if (LK) last_taxon = uniform32(lds.qval(qb + __popcll(LK) - 1));
     }
     wave_sync();  // the queue is refilled by the next batch
     if (st.done) break;
@@ -635,7 +665,7 @@ __device__ __forceinline__ uint32_t reduce_sum(uint32_t v) {
 
 template <bool DNA, bool EXTRAS, class Table>
 __device__ void classify_fragment(const DbView &db, const ClassifyParams &P, const FragBatch &in,
-                                  const FragOut &out, uint64_t f, Table &tab, WaveLds &lds,
+                                  const FragOut &out, uint64_t f, Table &tab, WaveLds<EXTRAS> &lds,
                                   int *bad, int lane) {
   const uint64_t o0 = uniform64(in.off[2 * f]), o1 = uniform64(in.off[2 * f + 1]),
                  o2 = uniform64(in.off[2 * f + 2]);
@@ -664,7 +694,7 @@ __device__ void classify_fragment(const DbView &db, const ClassifyParams &P, con
     if (paired && mate == 0) push_marker(out, base, cap, st, kCodeMate, tab, bad, lane);
   }
   if (st.run_count > 0) {
-    if (lane == 0) close_run(out, base, cap, st.n_runs, st.run_code, st.run_count, tab, bad);
+    if (lane == 0) close_run(out.runs + 2 * base, cap, st.n_runs, st.run_code, st.run_count, tab, bad);
     st.n_runs++;
   }
   __syncthreads();
@@ -730,7 +760,7 @@ __device__ void classify_fragment(const DbView &db, const ClassifyParams &P, con
 template <bool DNA, bool EXTRAS>
 __device__ __forceinline__ void plain_body(const DbView &db, const ClassifyParams &P,
                                            const FragBatch &in, const FragOut &out) {
-  __shared__ WaveLds lds;
+  __shared__ WaveLds<EXTRAS> lds;
   const uint64_t f = blockIdx.x;
   if (f >= in.n) return;
   LdsTable tab{lds.tkey, lds.tcnt, P.tab_cap};
@@ -748,7 +778,7 @@ template <bool DNA, bool EXTRAS>
 __device__ __forceinline__ void persistent_body(const DbView &db, const ClassifyParams &P,
                                                 const FragBatch &in, const FragOut &out,
                                                 unsigned long long *cursor) {
-  __shared__ WaveLds lds;
+  __shared__ WaveLds<EXTRAS> lds;
   LdsTable tab{lds.tkey, lds.tcnt, P.tab_cap};
   const int lane = threadIdx.x;
   const uint64_t n_units = in.unit_start ? uniform32(*in.n_units) : in.n;
@@ -779,7 +809,7 @@ __device__ __forceinline__ void dense_body(const DbView &db, const ClassifyParam
                                            const FragBatch &in, const FragOut &out,
                                            const uint32_t *redo, uint32_t n_redo,
                                            const DenseScratch &scratch) {
-  __shared__ WaveLds lds;
+  __shared__ WaveLds<true> lds;
   const int lane = threadIdx.x;
   const uint32_t slot = blockIdx.x;
   DenseTable tab{scratch.counts + (uint64_t) slot * db.node_count,
@@ -791,18 +821,20 @@ __device__ __forceinline__ void dense_body(const DbView &db, const ClassifyParam
 // Kernel variants: _dna (nucleotide, no per-hit extras: the hot path),
 // _dna_x (nucleotide with HLL records / counts / probe statistics / quick
 // mode), _prot (translated search, all of them available).
-#define K2G_CLASSIFY_VARIANT(SUFFIX, DNA, EXTRAS)                                              \
-  __global__ __launch_bounds__(64, 7) void k_classify_plain##SUFFIX(                          \
+// WAVES: resident waves per SIMD the variant is built for (its LDS allows
+// it: 32 one-wave workgroups per CU for _dna, 28 for the others).
+#define K2G_CLASSIFY_VARIANT(SUFFIX, DNA, EXTRAS, WAVES)                                       \
+  __global__ __launch_bounds__(64, WAVES) void k_classify_plain##SUFFIX(                      \
       DbView db, ClassifyParams P, FragBatch in, FragOut out) {                                \
     plain_body<DNA, EXTRAS>(db, P, in, out);                                                   \
   }                                                                                            \
-  __global__ __launch_bounds__(64, 7) void k_classify_persistent##SUFFIX(                     \
+  __global__ __launch_bounds__(64, WAVES) void k_classify_persistent##SUFFIX(                 \
       DbView db, ClassifyParams P, FragBatch in, FragOut out, unsigned long long *cursor) {    \
     persistent_body<DNA, EXTRAS>(db, P, in, out, cursor);                                      \
   }
-K2G_CLASSIFY_VARIANT(_dna, true, false)
-K2G_CLASSIFY_VARIANT(_dna_x, true, true)
-K2G_CLASSIFY_VARIANT(_prot, false, true)
+K2G_CLASSIFY_VARIANT(_dna, true, false, 8)
+K2G_CLASSIFY_VARIANT(_dna_x, true, true, 7)
+K2G_CLASSIFY_VARIANT(_prot, false, true, 7)
 #undef K2G_CLASSIFY_VARIANT
 
 __global__ __launch_bounds__(64) void k_classify_dense_dna(DbView db, ClassifyParams P, FragBatch in,
