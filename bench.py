@@ -235,17 +235,26 @@ def serve_at_rate(k2, db, reqs, rate, budget, slots, work, rng):
     results = [None] * len(reqs)
     ready = threading.Semaphore(0)
     errors = []
+    turn = iter(range(len(reqs)))
+    turn_lock = threading.Lock()
 
-    def waiter():
+    def waiter():  # one per slot: results are checksummed while other requests run
         try:
-            for i in range(len(reqs)):
-                ready.acquire()
+            while True:
+                with turn_lock:
+                    i = next(turn, None)
+                    if i is None:
+                        return
+                    ready.acquire()
+                if tickets[i] is None:  # the submitter stopped
+                    return
                 results[i] = srv.wait(tickets[i], checksum_only=True)
         except Exception as e:  # surfaced after join
             errors.append(e)
 
-    th = threading.Thread(target=waiter)
-    th.start()
+    th = [threading.Thread(target=waiter) for _ in range(max(1, slots))]
+    for t in th:
+        t.start()
     t0 = time.perf_counter()
     try:
         for i, r in enumerate(reqs):
@@ -256,7 +265,10 @@ def serve_at_rate(k2, db, reqs, rate, budget, slots, work, rng):
                                     min_hit_groups=work["min_hit_groups"], kmer_counts=True, hll=True)
             ready.release()
     finally:
-        th.join()
+        for _ in range(len(reqs) + len(th)):  # wake any waiter still waiting for a ticket
+            ready.release()
+        for t in th:
+            t.join()
     wall = time.perf_counter() - t0
     stats = srv.stats()
     srv.close()

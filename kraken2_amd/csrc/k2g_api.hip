@@ -564,8 +564,7 @@ extern "C" int k2g_batch_destroy(k2g_batch *b) {
   if (!b) return K2G_OK;
   if (b->stream) (void) hipStreamSynchronize(b->stream);
   b->for_each_device_buf([](auto &buf) { buf.release(); });
-  b->h_runs.release();
-  b->h_recs.release();
+  b->host.reset();
   if (b->ev0) (void) hipEventDestroy(b->ev0);
   if (b->ev1) (void) hipEventDestroy(b->ev1);
   if (b->stream) (void) hipStreamDestroy(b->stream);
@@ -879,18 +878,18 @@ extern "C" int k2g_batch_fetch(k2g_batch *b, k2g_results *r) {
   r->node_count = b->db->view.node_count;
   r->redone = b->redone;
   r->kernel_ms = b->kernel_ms;
-  b->h_call.resize(n);
-  b->h_groups.resize(n);
-  b->h_total.resize(n);
-  b->h_run_off.assign(n + 1, 0);
+  b->host->call.resize(n);
+  b->host->groups.resize(n);
+  b->host->total.resize(n);
+  b->host->run_off.assign(n + 1, 0);
   if (n) {
     std::vector<uint32_t> st(n);
     HIP_TRY(hipMemcpy(st.data(), b->status.p, n * 4, hipMemcpyDeviceToHost));
     for (uint64_t i = 0; i < n; i++)
       if (st[i] & 2u) return fail(K2G_ESOFTWARE, "internal error: hitlist exceeded its bound");
-    HIP_TRY(hipMemcpy(b->h_call.data(), b->call.p, n * 4, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(b->h_groups.data(), b->groups.p, n * 4, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(b->h_total.data(), b->total.p, n * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(b->host->call.data(), b->call.p, n * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(b->host->groups.data(), b->groups.p, n * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(b->host->total.data(), b->total.p, n * 4, hipMemcpyDeviceToHost));
     // compact runs: exclusive scan of counts, then a gather that writes
     // straight into pinned host memory
     auto compact = [&](const uint32_t *d_counts, const uint32_t *d_src, DevBuf<uint64_t> &off,
@@ -917,34 +916,34 @@ extern "C" int k2g_batch_fetch(k2g_batch *b, k2g_results *r) {
       HIP_TRY(hipStreamSynchronize(s));
       return K2G_OK;
     };
-    rc = compact(b->n_runs.p, b->runs.p, b->comp_off, b->h_runs, b->h_run_off);
+    rc = compact(b->n_runs.p, b->runs.p, b->comp_off, b->host->runs, b->host->run_off);
     if (rc) return rc;
     if (b->params.want_hll) {
-      rc = compact(b->n_recs.p, b->recs.p, b->comp_rec_off, b->h_recs, b->h_rec_off);
+      rc = compact(b->n_recs.p, b->recs.p, b->comp_rec_off, b->host->recs, b->host->rec_off);
       if (rc) return rc;
-      r->hll_off = b->h_rec_off.data();
-      r->hll = b->h_recs.p;
+      r->hll_off = b->host->rec_off.data();
+      r->hll = b->host->recs.p;
     }
     if (b->params.want_probe_stats) {
-      b->h_lookups.resize(n);
-      b->h_lines.resize(n);
-      HIP_TRY(hipMemcpy(b->h_lookups.data(), b->lookups.p, n * 4, hipMemcpyDeviceToHost));
-      HIP_TRY(hipMemcpy(b->h_lines.data(), b->lines.p, n * 8, hipMemcpyDeviceToHost));
-      r->lookups = b->h_lookups.data();
-      r->probe_lines = b->h_lines.data();
+      b->host->lookups.resize(n);
+      b->host->lines.resize(n);
+      HIP_TRY(hipMemcpy(b->host->lookups.data(), b->lookups.p, n * 4, hipMemcpyDeviceToHost));
+      HIP_TRY(hipMemcpy(b->host->lines.data(), b->lines.p, n * 8, hipMemcpyDeviceToHost));
+      r->lookups = b->host->lookups.data();
+      r->probe_lines = b->host->lines.data();
     }
     if (b->params.want_kmer_counts) {
-      b->h_kmer_counts.resize(b->db->view.node_count);
-      HIP_TRY(hipMemcpy(b->h_kmer_counts.data(), b->kmer_counts.p,
+      b->host->kmer_counts.resize(b->db->view.node_count);
+      HIP_TRY(hipMemcpy(b->host->kmer_counts.data(), b->kmer_counts.p,
                         b->db->view.node_count * 8, hipMemcpyDeviceToHost));
-      r->kmer_counts = b->h_kmer_counts.data();
+      r->kmer_counts = b->host->kmer_counts.data();
     }
   }
-  r->call = b->h_call.data();
-  r->hit_groups = b->h_groups.data();
-  r->total_kmers = b->h_total.data();
-  r->run_off = b->h_run_off.data();
-  r->runs = b->h_runs.p;
+  r->call = b->host->call.data();
+  r->hit_groups = b->host->groups.data();
+  r->total_kmers = b->host->total.data();
+  r->run_off = b->host->run_off.data();
+  r->runs = b->host->runs.p;
   return K2G_OK;
 }
 
@@ -1451,11 +1450,22 @@ struct Checksum {
     x ^= x >> 33;
     h = x;
   }
-  // n elements of p (p may be null when n is 0: an empty array has no data)
+  // n elements of p (p may be null when n is 0: an empty array has no data).
+  // Four independent multiply-rotate accumulators over the elements, folded
+  // into the running hash, so long arrays hash at memory speed.
   template <class T>
   void array(const T *p, uint64_t n) {
     word(n);
-    for (uint64_t i = 0; i < n; i++) word((uint64_t) p[i]);
+    uint64_t a[4] = {0x9E3779B97F4A7C15ull, 0xC2B2AE3D27D4EB4Full, 0x165667B19E3779F9ull,
+                     0x85EBCA77C2B2AE63ull};
+    uint64_t i = 0;
+    for (; i + 4 <= n; i += 4)
+      for (int j = 0; j < 4; j++) {
+        const uint64_t x = (a[j] ^ (uint64_t) p[i + j]) * 0x9FB21C651E98DF25ull;
+        a[j] = (x << 29) | (x >> 35);
+      }
+    for (int j = 0; j < 4; j++) word(a[j]);
+    for (; i < n; i++) word((uint64_t) p[i]);
   }
 };
 }  // namespace

@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <string>
+#include <memory>
 #include <vector>
 
 #include "k2g_device.h"
@@ -104,6 +105,23 @@ struct PinnedBuf {
   }
 };
 
+// Host copies of a batch's last results (hitlists and HLL records are
+// written by the compaction kernel straight into the pinned buffers).  A
+// batch owns one; the request server hands it to the finished request and
+// gives the batch another from its pool, so results are never copied.
+struct HostResults {
+  std::vector<uint32_t> call, groups, total, lookups;
+  std::vector<uint64_t> run_off, rec_off, kmer_counts, lines;
+  PinnedBuf<uint32_t> runs, recs;
+  HostResults() = default;
+  HostResults(const HostResults &) = delete;
+  HostResults &operator=(const HostResults &) = delete;
+  ~HostResults() {
+    runs.release();
+    recs.release();
+  }
+};
+
 }  // namespace k2g
 
 struct k2g_db {
@@ -139,11 +157,8 @@ struct k2g_batch {
   k2g::DevBuf<uint8_t> unit_flag;
   k2g::DevBuf<uint32_t> unit_start, n_units;
   uint32_t dense_slots = 0;
-  // host copies of the last results (hitlists and HLL records are written
-  // by the compaction kernel straight into pinned memory)
-  std::vector<uint32_t> h_call, h_groups, h_total, h_lookups;
-  std::vector<uint64_t> h_run_off, h_rec_off, h_kmer_counts, h_lines;
-  k2g::PinnedBuf<uint32_t> h_runs, h_recs;
+  // host copies of the last results
+  std::unique_ptr<k2g::HostResults> host = std::make_unique<k2g::HostResults>();
   // last launch
   uint64_t n = 0;
   k2g_classify_params params{};

@@ -18,6 +18,7 @@
 #include <cstring>
 #include <deque>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -82,11 +83,9 @@ struct Request {
   int rc = K2G_OK;
   std::string error;
   Clock::time_point t_submit, t_start, t_done;
-  // results
-  std::vector<uint32_t> call, groups, total, runs, hll;
-  std::vector<uint64_t> run_off, hll_off, kmer_counts;
-  uint64_t node_count = 0, redone = 0;
-  double kernel_ms = 0;
+  // results: the slot batch's host results, handed over without a copy
+  std::unique_ptr<k2g::HostResults> res;
+  k2g_results out{};
 };
 
 }  // namespace
@@ -107,26 +106,12 @@ struct k2g_server {
   int running = 0, max_running = 0;
   int worker_rc = K2G_OK;
   std::string worker_error;
+  // host result sets of released requests, reused by the slots (their
+  // vectors and pinned buffers keep their capacity)
+  std::vector<std::unique_ptr<k2g::HostResults>> pool;
 };
 
 namespace {
-
-void copy_results(const k2g_results &r, Request *q) {
-  const uint64_t n = r.n;
-  q->call.assign(r.call, r.call + n);
-  q->groups.assign(r.hit_groups, r.hit_groups + n);
-  q->total.assign(r.total_kmers, r.total_kmers + n);
-  q->run_off.assign(r.run_off, r.run_off + n + 1);
-  q->runs.assign(r.runs, r.runs + 2 * r.run_off[n]);
-  if (r.hll_off) {
-    q->hll_off.assign(r.hll_off, r.hll_off + n + 1);
-    q->hll.assign(r.hll, r.hll + 2 * r.hll_off[n]);
-  }
-  if (r.kmer_counts) q->kmer_counts.assign(r.kmer_counts, r.kmer_counts + r.node_count);
-  q->node_count = r.node_count;
-  q->redone = r.redone;
-  q->kernel_ms = r.kernel_ms;
-}
 
 void worker_main(k2g_server *s) {
   k2g_batch *b = nullptr;
@@ -167,10 +152,9 @@ void worker_main(k2g_server *s) {
     region.size = q->bytes;
     region.used = 0;
     b->bind_region(&region);
-    k2g_results res;
-    int r = k2g_classify(b, &q->params, &q->in, &res);
+    int r = k2g_classify(b, &q->params, &q->in, &q->out);
     if (r == K2G_OK) {
-      copy_results(res, q);
+      q->res = std::move(b->host);  // q->out points into these buffers
     } else {
       q->rc = r;
       q->error = k2g::last_error();
@@ -178,6 +162,14 @@ void worker_main(k2g_server *s) {
     b->bind_region(nullptr);
     {
       std::lock_guard<std::mutex> lk(s->mu);
+      if (!b->host) {
+        if (!s->pool.empty()) {
+          b->host = std::move(s->pool.back());
+          s->pool.pop_back();
+        } else {
+          b->host = std::make_unique<k2g::HostResults>();
+        }
+      }
       s->arena.release(off, q->bytes);
       s->in_use -= q->bytes;
       s->running--;
@@ -270,26 +262,7 @@ extern "C" int k2g_server_wait(k2g_server *s, uint64_t ticket, k2g_results *out,
     times_ms[2] = ms_between(q->t_submit, q->t_done);
   }
   if (q->rc != K2G_OK) return fail(q->rc, q->error);
-  if (out) {
-    std::memset(out, 0, sizeof(*out));
-    const uint64_t n = q->in.n;
-    out->n = n;
-    if (n) {
-      out->call = q->call.data();
-      out->hit_groups = q->groups.data();
-      out->total_kmers = q->total.data();
-      out->run_off = q->run_off.data();
-      out->runs = q->runs.data();
-      if (!q->hll_off.empty()) {
-        out->hll_off = q->hll_off.data();
-        out->hll = q->hll.data();
-      }
-      if (!q->kmer_counts.empty()) out->kmer_counts = q->kmer_counts.data();
-    }
-    out->node_count = s->db->view.node_count;
-    out->redone = q->redone;
-    out->kernel_ms = q->kernel_ms;
-  }
+  if (out) *out = q->out;  // valid until the request is released
   return K2G_OK;
 }
 
@@ -299,6 +272,9 @@ extern "C" int k2g_server_release(k2g_server *s, uint64_t ticket) {
   auto it = s->requests.find(ticket);
   if (it == s->requests.end()) return fail(K2G_EUSAGE, "unknown ticket");
   if (it->second->state != 2) return fail(K2G_EUSAGE, "request still in progress");
+  // keep a few result sets for the slots; free the rest
+  if (it->second->res && s->pool.size() < s->workers.size() + 2)
+    s->pool.push_back(std::move(it->second->res));
   delete it->second;
   s->requests.erase(it);
   return K2G_OK;
