@@ -411,9 +411,11 @@ namespace k2g {
 
 // Element count e and reserved hitlist space of every fragment; a fragment
 // that needs more than its reserved space is classified again into e.
+// too_long: set when a mate has 2^31 or more bases (the kernels count
+// positions within a read in 32 bits).
 __global__ void k_bounds(const uint64_t *off, const uint8_t *paired, uint32_t paired_all,
                          uint64_t n, uint32_t k, uint32_t dna, uint32_t pct, uint64_t *bounds,
-                         uint64_t *caps, uint32_t *keys) {
+                         uint64_t *caps, uint32_t *keys, uint32_t *too_long) {
   const uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
   if (i > n) return;
   if (i == n) {
@@ -422,8 +424,9 @@ __global__ void k_bounds(const uint64_t *off, const uint8_t *paired, uint32_t pa
     return;
   }
   const bool pr = paired ? paired[i] != 0 : paired_all != 0;
-  const uint64_t e = fragment_elements(off[2 * i + 1] - off[2 * i],
-                                       pr ? off[2 * i + 2] - off[2 * i + 1] : 0, pr, k, dna != 0);
+  const uint64_t l1 = off[2 * i + 1] - off[2 * i], l2 = pr ? off[2 * i + 2] - off[2 * i + 1] : 0;
+  if ((l1 | l2) >> 31) atomicOr(too_long, 1u);
+  const uint64_t e = fragment_elements(l1, l2, pr, k, dna != 0);
   bounds[i] = e;
   caps[i] = run_space(e, pct);
   if (keys) keys[i] = (uint32_t) (e < 0xFFFFFFFFull ? e : 0xFFFFFFFFull);
@@ -635,7 +638,7 @@ extern "C" int k2g_classify_device(k2g_batch *b, const k2g_classify_params *p, c
   HIP_TRY(b->run_cap.reserve(n + 1));
   HIP_TRY(b->bounds.reserve(n + 1));
   HIP_TRY(b->sort_keys.reserve(n));
-  HIP_TRY(b->redo_count.reserve(3));
+  HIP_TRY(b->redo_count.reserve(4));  // three redo counts, then the too-long flag
   HIP_TRY(b->cursor.reserve(kCursorWords));
   // in a bounded region, the largest scratch any step needs is taken once
   if (b->temp.region) HIP_TRY(b->temp.reserve(temp_bytes_for(n, p->mode == 1 && !d_order)));
@@ -643,17 +646,21 @@ extern "C" int k2g_classify_device(k2g_batch *b, const k2g_classify_params *p, c
   // element bounds -> reserved run space -> run offsets
   const uint32_t pct = p->run_cap_pct ? (uint32_t) p->run_cap_pct : 25u;
   const unsigned g1 = (unsigned) ((n + 1 + 255) / 256);
+  HIP_TRY(hipMemsetAsync(b->redo_count.p + 3, 0, 4, s));
   hipLaunchKernelGGL(k_bounds, dim3(g1), dim3(256), 0, s, d_off, d_paired,
                      (uint32_t) (p->paired != 0), n, v.k, v.dna, pct, b->bounds.p, b->run_cap.p,
-                     p->mode == 1 && !d_order ? b->sort_keys.p : nullptr);
+                     p->mode == 1 && !d_order ? b->sort_keys.p : nullptr, b->redo_count.p + 3);
   HIP_TRY(hipGetLastError());
   size_t tbytes = 0;
   HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tbytes, b->run_cap.p, b->run_off.p, n + 1, s));
   HIP_TRY(b->temp.reserve(tbytes));
   HIP_TRY(hipcub::DeviceScan::ExclusiveSum(b->temp.p, tbytes, b->run_cap.p, b->run_off.p, n + 1, s));
   uint64_t total_space = 0;
+  uint32_t too_long = 0;
   HIP_TRY(hipMemcpyAsync(&total_space, b->run_off.p + n, 8, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(&too_long, b->redo_count.p + 3, 4, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
+  if (too_long) return fail(K2G_EUSAGE, "a read has 2^31 or more bases");
   HIP_TRY(b->runs.reserve(2 * std::max<uint64_t>(total_space, 1)));
   if (p->want_hll) HIP_TRY(b->recs.reserve(2 * std::max<uint64_t>(total_space, 1)));
   if (p->want_kmer_counts) {

@@ -363,27 +363,30 @@ __device__ void scan_stream(const DbView &db, const ClassifyParams &P, const Fra
                             RunState &st, Table &tab, WaveLds<EXTRAS> &lds, int *bad, int lane) {
   static_assert(W >= 0 && W <= kRingGuard + 1, "window too wide for the ring guard");
   constexpr bool dna = DNA;
-  const uint64_t n = dna ? L : frame_len(L, fr);
+  // positions within a stream are 32-bit: the library rejects reads of 2^31
+  // or more bases before launching
+  const uint32_t n = (uint32_t) (dna ? L : frame_len(L, fr));
   const uint32_t k = db.k, l = db.l;
   const uint32_t w = k - l + 1;  // l-mers per k-mer window
   const uint64_t lbits = (1ULL << l) - 1ULL;
-  int64_t last_amb = -1;
+  int32_t last_amb = -1;
   uint64_t last_min = ~0ULL;
   uint32_t last_taxon = 0;
   uint64_t pp0 = 0, pp1 = 0, pp2 = 0, pp3 = 0;  // previous chunk's planes (translated)
   uint64_t pBEh = 0, pLEh = 0;                   // previous chunk's upper stream words (DNA)
   const uint64_t lmask2 = (1ULL << (2 * l)) - 1ULL;
   constexpr bool kVgprConsts = !EXTRAS;
-  const uint64_t tog = db.toggle_mask, smask = db.spaced_mask;
+  const uint64_t tog = !kVgprConsts ? db.toggle_mask : in_vgpr(db.toggle_mask);
+  const uint64_t smask = !kVgprConsts ? db.spaced_mask : in_vgpr(db.spaced_mask);
   const ProbeTable ptab(db, kVgprConsts);
   uint32_t *const runs_at = !kVgprConsts ? out.runs + 2 * base : in_vgpr(out.runs + 2 * base);
   const uint64_t cap_v = !kVgprConsts ? cap : in_vgpr(cap);
-  uint64_t c = 0;
+  uint32_t c = 0;
   while (c < n) {
     // ---- A: scan chunks, queue their lookups
     uint32_t qn = 0, g = 0;
     while (c < n && g < (uint32_t) kBatchChunks && qn + kWave <= (uint32_t) kQueue) {
-      const uint64_t p = c + lane;
+      const uint32_t p = c + (uint32_t) lane;
       const bool inb = p < n;
       uint32_t code = 0xFFu;
       if (inb) code = dna ? dna_code((uint8_t) m[p]) : frame_code(m, L, fr, p);
@@ -397,16 +400,16 @@ __device__ void scan_stream(const DbView &db, const ClassifyParams &P, const Fra
       // A clean chunk (no ambiguous base in it or in the k-1 positions before
       // it, and past the stream's first k-1 positions) has a valid l-mer and a
       // non-ambiguous k-mer at every position in bounds.
-      const bool clean = A == 0 && c + 1 >= k && last_amb < (int64_t) (c + 1) - (int64_t) k;
+      const bool clean = A == 0 && c + 1 >= k && last_amb < (int32_t) (c + 1) - (int32_t) k;
       bool lvalid, kvalid, nonamb;
       if (clean) {
         lvalid = kvalid = nonamb = inb;
       } else {
         const uint64_t mA = A & lanemask_le(lane);
-        const int64_t lastN = mA ? (int64_t) (c + top_lane(mA)) : last_amb;
-        lvalid = inb && p + 1 >= l && lastN < (int64_t) (p + 1 - l);
+        const int32_t lastN = mA ? (int32_t) (c + (uint32_t) top_lane(mA)) : last_amb;
+        lvalid = inb && p + 1 >= l && lastN < (int32_t) (p + 1) - (int32_t) l;
         kvalid = inb && p + 1 >= k;
-        nonamb = kvalid && ((k == l) ? lvalid : (lastN < (int64_t) (p + 2 - k)));
+        nonamb = kvalid && ((k == l) ? lvalid : (lastN < (int32_t) (p + 2) - (int32_t) k));
       }
 
       // candidate of the l-mer ending at p
@@ -499,7 +502,7 @@ __device__ void scan_stream(const DbView &db, const ClassifyParams &P, const Fra
       }
       qn += (uint32_t) __popcll(LK);
       g++;
-      if (A) last_amb = (int64_t) (c + top_lane(A));
+      if (A) last_amb = (int32_t) (c + (uint32_t) top_lane(A));
       if (NA) last_min = readlane64(mn, top_lane(NA));
       pp0 = P0;
       pp1 = P1;

@@ -217,3 +217,23 @@ def test_probe_edge_tables(edge_tables, name, mode):
     finally:
         batch.close()
         db.close()
+
+
+def test_read_of_2_31_bases_rejected(env):
+    """Reads of 2^31 or more bases are refused before any kernel reads the
+    sequence (positions within a read are 32-bit): the offsets alone claim
+    such a read here, over a 64-byte sequence buffer that is never read."""
+    import ctypes as C
+    from kraken2_amd.synth import Hip
+    k2, db, batch, orc, h, genomes = env
+    hip = Hip()
+    d_seq = hip.upload(np.frombuffer(b"ACGT" * 16, np.uint8))
+    d_off = hip.upload(np.array([0, 1 << 31, 1 << 31], np.uint64))
+    try:
+        lib = k2.load_library()
+        p = k2._lib.ClassifyParams(0.0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0)
+        rc = lib.k2g_classify_device(batch._h, C.byref(p), d_seq, d_off, None, 1, None)
+        assert rc == 64 and b"2^31" in lib.k2g_last_error()
+    finally:
+        hip.free(d_seq)
+        hip.free(d_off)
