@@ -224,7 +224,8 @@ extern "C" int k2g_db_open(const char *hash_k2d, const char *taxo_k2d, const cha
     delete db;
     return fail(K2G_ESOFTWARE, std::string("no HIP device: ") + hipGetErrorString(e));
   }
-  const uint64_t padded = (capacity + 63) / 64 * 64;
+  // at least 4 cells of padding: a probe's 16-byte read from its last cell
+  const uint64_t padded = (capacity + 4 + 63) / 64 * 64;
   e = hipMalloc((void **) &db->d_cells, padded * 4);
   if (e == hipSuccess) e = hipMemset(db->d_cells + capacity, 0, (padded - capacity) * 4);
   if (e == hipSuccess) e = hipMalloc((void **) &db->d_parent, parent.size() * 4);
@@ -313,7 +314,7 @@ extern "C" int k2g_db_open(const char *hash_k2d, const char *taxo_k2d, const cha
     if (d_max) (void) hipFree(d_max);
     if (e == hipSuccess && h_max[1] && capacity >= 64) {
       if (padded > capacity) {
-        hipLaunchKernelGGL(k_pad_copy, dim3(1), dim3(64), 0, 0, db->d_cells, capacity,
+        hipLaunchKernelGGL(k_pad_copy, dim3(1), dim3(128), 0, 0, db->d_cells, capacity,
                            (uint32_t) (padded - capacity));
         e = hipGetLastError();
         if (e == hipSuccess) e = hipDeviceSynchronize();

@@ -20,9 +20,9 @@ constexpr uint64_t kNoCand = ~0ULL;  // ring sentinel: no valid l-mer
 
 // Everything a kernel needs to read the resident DB.
 struct DbView {
-  // capacity cells, padded to a multiple of 64; with lean_probe the padding
-  // holds copies of the first cells, so a 4-cell group read across the end
-  // sees the cells a wrapping linear probe visits next
+  // capacity cells and at least 4 more, padded to a multiple of 64; with
+  // lean_probe the padding holds copies of the first cells, so a 4-cell read
+  // across the end sees the cells a wrapping linear probe visits next
   const uint32_t *cells;
   uint64_t capacity;
   uint64_t barrett;        // floor(2^64 / capacity) (0 when capacity == 1)

@@ -281,22 +281,23 @@ struct ProbeState {
   __device__ __forceinline__ bool step(const DbView &db, const ProbeTable &t, uint32_t &out) {
     return db.lean_probe ? step_lean(db, t, out) : step_bounded(db, t, out);
   }
-  // The padding copies the first cells, so the group's cells past the end of
-  // the table are the ones the probe visits after wrapping, and an empty
-  // cell somewhere ends every probe: no bounds or remaining-cell count.
+  // The padding copies the first cells, so cells read past the end of the
+  // table are the ones the probe visits after wrapping, and an empty cell
+  // somewhere ends every probe: no bounds or remaining-cell count.  A step
+  // reads the four cells from idx on (one 16-byte load, 4-byte aligned;
+  // the table is padded by at least four cells), all of them in probe order.
   __device__ __forceinline__ bool step_lean(const DbView &db, const ProbeTable &t, uint32_t &out) {
+    typedef uint32_t u32x4_a4 __attribute__((ext_vector_type(4), aligned(4)));
     const uint32_t vb = db.value_bits;
     const uint32_t vmask = (1u << vb) - 1u;
-    const uint64_t group = idx & ~3ULL;
-    const u32x4 v = *(const u32x4 *) (t.cells + group);
-    const uint32_t off = (uint32_t) idx & 3u;
+    const u32x4_a4 v = *(const u32x4_a4 *) (t.cells + idx);
     const uint32_t c4[4] = {v.x, v.y, v.z, v.w};
     uint32_t val = 0;
     bool stop = false;
 #pragma unroll
     for (int j = 3; j >= 0; j--) {  // the lowest stopping cell wins
       const uint32_t c = c4[j];
-      const bool st = (uint32_t) j >= off && ((c & vmask) == 0u || (c >> vb) == ck);
+      const bool st = (c & vmask) == 0u || (c >> vb) == ck;
       val = st ? (c & vmask) : val;  // an empty cell gives 0
       stop |= st;
     }
@@ -304,7 +305,7 @@ struct ProbeState {
       out = skip ? 0u : val;
       return true;
     }
-    idx = group + 4;
+    idx += 4;
     if (idx >= t.capacity) idx -= t.capacity;
     return false;
   }
@@ -885,7 +886,7 @@ __global__ __launch_bounds__(256) void k_max_value(DbView db, unsigned int *out)
 // Copies the first cells into the padding after the table (capacity >= 64).
 __global__ void k_pad_copy(uint32_t *cells, uint64_t capacity, uint32_t n) {
   const uint32_t t = threadIdx.x;
-  if (t < n) cells[capacity + t] = cells[t];
+  if (t < n) cells[capacity + t] = cells[t % capacity];
 }
 
 }  // namespace k2g

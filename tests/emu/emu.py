@@ -21,10 +21,10 @@ def load():
 def read_db(d):
     raw = np.fromfile(os.path.join(d, "hash.k2d"), dtype=np.uint8)
     cap, size, kb, vb = np.frombuffer(raw[:32].tobytes(), np.uint64).tolist()
-    cells = np.zeros((cap + 63) // 64 * 64, np.uint32)
+    cells = np.zeros((cap + 4 + 63) // 64 * 64, np.uint32)
     cells[:cap] = np.frombuffer(raw[32:].tobytes(), np.uint32)
     if cap >= 64:  # padding holds copies of the first cells, as the library loads it
-        cells[cap:] = cells[:len(cells) - cap]
+        cells[cap:] = cells[np.arange(len(cells) - cap) % cap]
     tx = open(os.path.join(d, "taxo.k2d"), "rb").read()
     nn = int(np.frombuffer(tx[8:16], np.uint64)[0])
     nodes = np.frombuffer(tx[32:32 + 56 * nn], np.uint64).reshape(nn, 7)
