@@ -72,7 +72,9 @@ typedef struct {
   int32_t min_hit_groups;   /* -g */
   int32_t quick;            /* -q */
   int32_t paired;           /* -P / -S: every fragment is a pair (unless per-fragment flags) */
-  int32_t mode;             /* 0: plain, one launch per batch; 1: persistent, longest first */
+  int32_t mode;             /* 0: plain, one launch per batch; 1: persistent, longest first;
+                             * 2: persistent when fragment lengths are uneven (the longest
+                             * above four times the mean), else plain */
   int32_t want_kmer_counts; /* -R: per-taxon LOOKUP-hit counts */
   int32_t want_hll;         /* -K: per-hit (taxon, encoded HLL hash) records */
   int32_t table_slots;      /* per-fragment LDS hit table (power of 2 in [2,256]; 0 = 256) */

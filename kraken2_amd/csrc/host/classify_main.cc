@@ -37,7 +37,7 @@ struct Options {
        single_file_pairs = false;
   double confidence_threshold = 0;
   int num_threads = 1, minimum_quality_score = 0, minimum_hit_groups = 0;
-  int gpu_mode = 1;                 // K2G_MODE: 0 plain, 1 persistent
+  int gpu_mode = 2;                 // K2G_MODE: 0 plain, 1 persistent, 2 chosen per batch
   size_t batch_bases = 256u << 20;  // bases per GPU batch (several blocks)
   std::vector<std::string> filenames;
 };
@@ -120,7 +120,10 @@ void parse(int argc, char **argv, Options &o) {
   }
   if (o.mpa_style_report) die(64, "mpa-style reports are not provided by this front-end");
   for (int i = optind; i < argc; i++) o.filenames.push_back(argv[i]);
-  if (const char *m = getenv("K2G_MODE")) o.gpu_mode = atoi(m) ? 1 : 0;
+  if (const char *m = getenv("K2G_MODE")) {
+    const int v = atoi(m);
+    o.gpu_mode = v >= 0 && v <= 2 ? v : 2;
+  }
   if (const char *b = getenv("K2G_BATCH_BASES")) o.batch_bases = strtoull(b, nullptr, 10);
 }
 

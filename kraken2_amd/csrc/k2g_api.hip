@@ -412,25 +412,38 @@ namespace k2g {
 
 // Element count e and reserved hitlist space of every fragment; a fragment
 // that needs more than its reserved space is classified again into e.
-// too_long: set when a mate has 2^31 or more bases (the kernels count
-// positions within a read in 32 bits).
+// Element bound, reserved run space and sort key per fragment, and the batch
+// statistics at stats (zeroed by the caller): [0] set when a mate has 2^31
+// or more bases (the kernels count positions within a read in 32 bits);
+// u64 at [2]: the largest element count, at [4]: their sum.
 __global__ void k_bounds(const uint64_t *off, const uint8_t *paired, uint32_t paired_all,
                          uint64_t n, uint32_t k, uint32_t dna, uint32_t pct, uint64_t *bounds,
-                         uint64_t *caps, uint32_t *keys, uint32_t *too_long) {
+                         uint64_t *caps, uint32_t *keys, uint32_t *stats) {
   const uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
-  if (i > n) return;
-  if (i == n) {
+  uint64_t e = 0;
+  if (i < n) {
+    const bool pr = paired ? paired[i] != 0 : paired_all != 0;
+    const uint64_t l1 = off[2 * i + 1] - off[2 * i], l2 = pr ? off[2 * i + 2] - off[2 * i + 1] : 0;
+    if ((l1 | l2) >> 31) atomicOr(stats, 1u);
+    e = fragment_elements(l1, l2, pr, k, dna != 0);
+    bounds[i] = e;
+    caps[i] = run_space(e, pct);
+    if (keys) keys[i] = (uint32_t) (e < 0xFFFFFFFFull ? e : 0xFFFFFFFFull);
+  } else if (i == n) {
     bounds[n] = 0;
     caps[n] = 0;
-    return;
   }
-  const bool pr = paired ? paired[i] != 0 : paired_all != 0;
-  const uint64_t l1 = off[2 * i + 1] - off[2 * i], l2 = pr ? off[2 * i + 2] - off[2 * i + 1] : 0;
-  if ((l1 | l2) >> 31) atomicOr(too_long, 1u);
-  const uint64_t e = fragment_elements(l1, l2, pr, k, dna != 0);
-  bounds[i] = e;
-  caps[i] = run_space(e, pct);
-  if (keys) keys[i] = (uint32_t) (e < 0xFFFFFFFFull ? e : 0xFFFFFFFFull);
+  // one atomic per wave for the batch's largest and total element counts
+  uint64_t mx = e, sum = e;
+  for (int d = 1; d < kWave; d <<= 1) {
+    const uint64_t om = (uint64_t) __shfl_xor((unsigned long long) mx, d);
+    sum += (uint64_t) __shfl_xor((unsigned long long) sum, d);
+    mx = om > mx ? om : mx;
+  }
+  if ((threadIdx.x & (kWave - 1)) == 0 && sum) {
+    atomicMax(reinterpret_cast<unsigned long long *>(stats + 2), (unsigned long long) mx);
+    atomicAdd(reinterpret_cast<unsigned long long *>(stats + 4), (unsigned long long) sum);
+  }
 }
 
 // Work units over the longest-first order: position j starts a unit when its
@@ -602,7 +615,8 @@ static int check_params(const k2g_classify_params *p) {
   if (!p) return fail(K2G_EUSAGE, "null params");
   if (!(p->confidence >= 0 && p->confidence <= 1))
     return fail(K2G_EUSAGE, "confidence threshold must be in [0, 1]");
-  if (p->mode != 0 && p->mode != 1) return fail(K2G_EUSAGE, "mode must be 0 (plain) or 1 (persistent)");
+  if (p->mode < 0 || p->mode > 2)
+    return fail(K2G_EUSAGE, "mode must be 0 (plain), 1 (persistent) or 2 (chosen per batch)");
   const int s = p->table_slots;
   if (s != 0 && (s < 2 || s > 256 || (s & (s - 1)) != 0))
     return fail(K2G_EUSAGE, "table_slots must be a power of two in [2, 256]");
@@ -639,29 +653,37 @@ extern "C" int k2g_classify_device(k2g_batch *b, const k2g_classify_params *p, c
   HIP_TRY(b->run_cap.reserve(n + 1));
   HIP_TRY(b->bounds.reserve(n + 1));
   HIP_TRY(b->sort_keys.reserve(n));
-  HIP_TRY(b->redo_count.reserve(4));  // three redo counts, then the too-long flag
+  HIP_TRY(b->redo_count.reserve(8));  // three redo counts, then k_bounds's statistics
   HIP_TRY(b->cursor.reserve(kCursorWords));
   // in a bounded region, the largest scratch any step needs is taken once
-  if (b->temp.region) HIP_TRY(b->temp.reserve(temp_bytes_for(n, p->mode == 1 && !d_order)));
+  const bool may_sort = p->mode != 0 && !d_order;
+  if (b->temp.region) HIP_TRY(b->temp.reserve(temp_bytes_for(n, may_sort)));
 
   // element bounds -> reserved run space -> run offsets
   const uint32_t pct = p->run_cap_pct ? (uint32_t) p->run_cap_pct : 25u;
   const unsigned g1 = (unsigned) ((n + 1 + 255) / 256);
-  HIP_TRY(hipMemsetAsync(b->redo_count.p + 3, 0, 4, s));
+  uint32_t *const stats = b->redo_count.p + 2;  // 8-byte aligned: [0] flag, [2..3] max, [4..5] sum
+  HIP_TRY(hipMemsetAsync(stats, 0, 24, s));
   hipLaunchKernelGGL(k_bounds, dim3(g1), dim3(256), 0, s, d_off, d_paired,
                      (uint32_t) (p->paired != 0), n, v.k, v.dna, pct, b->bounds.p, b->run_cap.p,
-                     p->mode == 1 && !d_order ? b->sort_keys.p : nullptr, b->redo_count.p + 3);
+                     may_sort ? b->sort_keys.p : nullptr, stats);
   HIP_TRY(hipGetLastError());
   size_t tbytes = 0;
   HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tbytes, b->run_cap.p, b->run_off.p, n + 1, s));
   HIP_TRY(b->temp.reserve(tbytes));
   HIP_TRY(hipcub::DeviceScan::ExclusiveSum(b->temp.p, tbytes, b->run_cap.p, b->run_off.p, n + 1, s));
   uint64_t total_space = 0;
-  uint32_t too_long = 0;
+  uint32_t st[6] = {0, 0, 0, 0, 0, 0};
   HIP_TRY(hipMemcpyAsync(&total_space, b->run_off.p + n, 8, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipMemcpyAsync(&too_long, b->redo_count.p + 3, 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(st, stats, 24, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
-  if (too_long) return fail(K2G_EUSAGE, "a read has 2^31 or more bases");
+  if (st[0]) return fail(K2G_EUSAGE, "a read has 2^31 or more bases");
+  // mode 2: the persistent kernel pays for its sort and work units only when
+  // lengths are uneven (the longest fragment above four times the mean);
+  // batches of similar reads run one plain launch
+  const uint64_t max_e = st[2] | ((uint64_t) st[3] << 32), sum_e = st[4] | ((uint64_t) st[5] << 32);
+  const bool persistent = p->mode == 1 || (p->mode == 2 && (d_order || max_e * n > 4 * sum_e));
+  b->ran_persistent = persistent;
   HIP_TRY(b->runs.reserve(2 * std::max<uint64_t>(total_space, 1)));
   if (p->want_hll) HIP_TRY(b->recs.reserve(2 * std::max<uint64_t>(total_space, 1)));
   if (p->want_kmer_counts) {
@@ -672,7 +694,7 @@ extern "C" int k2g_classify_device(k2g_batch *b, const k2g_classify_params *p, c
   // persistent order: longest fragments first
   const uint32_t *order = d_order;
   const uint32_t *units = nullptr;
-  if (p->mode == 1 && !order) {
+  if (persistent && !order) {
     HIP_TRY(b->order.reserve(n));
     HIP_TRY(b->sort_vals.reserve(n));
     HIP_TRY(b->sort_keys_out.reserve(n));
@@ -737,7 +759,7 @@ extern "C" int k2g_classify_device(k2g_batch *b, const k2g_classify_params *p, c
                                                p->want_probe_stats || p->quick);
   const ClassifyKernels &K = kVariants[vi];
   HIP_TRY(hipEventRecord(b->ev0, s));
-  if (p->mode == 0) {
+  if (!persistent) {
     hipLaunchKernelGGL(K.plain, dim3((unsigned) n), dim3(64), 0, s, v, cp, in, out);
   } else {
     HIP_TRY(hipMemsetAsync(b->cursor.p, 0, kCursorWords * 8, s));
@@ -1396,13 +1418,13 @@ uint64_t k2g::plan_classify_bytes(const k2g_db *db, const k2g_classify_params *p
                                          v.k, v.dna != 0);
     space += run_space(e, pct);
   }
-  const bool sort = p->mode == 1;
+  const bool sort = p->mode != 0;
   uint64_t b = 0;
   b += region_bytes(bases + 1, 1) + region_bytes(2 * n + 1, 8);  // seq, off
   if (in->paired) b += region_bytes(n, 1);
   b += 6 * region_bytes(n, 4);                                    // call .. n_recs
   b += 3 * region_bytes(n + 1, 8);                                // run_off, run_cap, bounds
-  b += region_bytes(n, 4) + region_bytes(3, 4) + region_bytes(kCursorWords, 8);  // sort keys, counters, cursors
+  b += region_bytes(n, 4) + region_bytes(8, 4) + region_bytes(kCursorWords, 8);  // sort keys, counters, cursors
   b += region_bytes(temp_bytes_for(n, sort), 1);
   b += region_bytes(2 * std::max<uint64_t>(space, 1), 4);         // runs
   if (p->want_hll) b += region_bytes(2 * std::max<uint64_t>(space, 1), 4);

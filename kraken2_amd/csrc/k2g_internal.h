@@ -163,6 +163,7 @@ struct k2g_batch {
   uint64_t n = 0;
   k2g_classify_params params{};
   uint64_t redone = 0, space_redone = 0;
+  bool ran_persistent = false;  // which kernel the last call launched
   double kernel_ms = 0;
   int persistent_blocks[3] = {0, 0, 0};  // per kernel variant: resident waves
 

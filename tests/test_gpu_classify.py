@@ -43,7 +43,7 @@ def test_get_batch_all_cells_and_misses(env, synth_db):
     assert np.array_equal(db.get_batch(keys), orc.get_batch(h, keys))
 
 
-@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("mode", [0, 1, 2])
 @pytest.mark.parametrize("paired", [False, True])
 def test_classify_edge_reads(env, mode, paired):
     k2, db, batch, orc, h, genomes = env
@@ -237,3 +237,15 @@ def test_read_of_2_31_bases_rejected(env):
     finally:
         hip.free(d_seq)
         hip.free(d_off)
+
+
+def test_mode_chosen_per_batch(env):
+    """Mode 2 picks the persistent kernel for uneven lengths and the plain one
+    for similar lengths; both batches equal the oracle."""
+    k2, db, batch, orc, h, genomes = env
+    for reads in (edge_reads(genomes, 500, seed=301, min_len=140, max_len=160),
+                  chimeric_reads(genomes, 4, seed=302, length=6000) + edge_reads(genomes, 400, seed=303)):
+        seq, off = k2.pack_fragments(reads)
+        out = batch.classify(seq, off, mode=2, min_hit_groups=2)
+        ref = orc.classify(h, seq, off, min_hit_groups=2)
+        _compare(k2, db, out, ref, len(reads))
