@@ -231,6 +231,12 @@ extern "C" int k2g_db_open(const char *hash_k2d, const char *taxo_k2d, const cha
   if (e == hipSuccess) e = hipMalloc((void **) &db->d_parent, parent.size() * 4);
   if (e == hipSuccess)
     e = hipMemcpy(db->d_parent, parent.data(), parent.size() * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    std::vector<uint32_t> span(2 * parent.size());
+    taxonomy_spans(parent.data(), parent.size(), span.data());
+    e = hipMalloc((void **) &db->d_span, span.size() * 4);
+    if (e == hipSuccess) e = hipMemcpy(db->d_span, span.data(), span.size() * 4, hipMemcpyHostToDevice);
+  }
   if (e != hipSuccess) {
     close(fd);
     k2g_db_close(db);
@@ -289,6 +295,7 @@ extern "C" int k2g_db_open(const char *hash_k2d, const char *taxo_k2d, const cha
   v.key_bits = (uint32_t) kb;
   v.value_bits = (uint32_t) vb;
   v.parent = db->d_parent;
+  v.span = db->d_span;
   v.node_count = (uint32_t) parent.size();
   v.k = (uint32_t) ro.k;
   v.l = (uint32_t) ro.l;
@@ -347,7 +354,7 @@ extern "C" int k2g_db_open(const char *hash_k2d, const char *taxo_k2d, const cha
   inf.minimum_acceptable_hash_value = ro.min_hash;
   inf.dna_db = dna ? 1 : 0;
   inf.revcom_version = ro.revcom;
-  inf.device_bytes = padded * 4 + parent.size() * 4;
+  inf.device_bytes = padded * 4 + parent.size() * 12;
   *out = db;
   return K2G_OK;
 }
@@ -356,6 +363,7 @@ extern "C" int k2g_db_close(k2g_db *db) {
   if (!db) return K2G_OK;
   if (db->d_cells) (void) hipFree(db->d_cells);
   if (db->d_parent) (void) hipFree(db->d_parent);
+  if (db->d_span) (void) hipFree(db->d_span);
   delete db;
   return K2G_OK;
 }

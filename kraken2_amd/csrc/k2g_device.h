@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 namespace k2g {
 
 constexpr int kWave = 64;
@@ -28,6 +30,10 @@ struct DbView {
   uint64_t barrett;        // floor(2^64 / capacity) (0 when capacity == 1)
   uint32_t key_bits, value_bits;
   const uint32_t *parent;  // internal parent id per node (node 0 -> 0)
+  // per node v: span[2v] = its preorder index, span[2v+1] = that plus its
+  // subtree size, so a is an ancestor-or-self of b iff b's index lies in a's
+  // span (node 0: an empty span)
+  const uint32_t *span;
   uint32_t node_count;
   // scanner parameters (opts.k2d)
   uint32_t k, l;
@@ -109,6 +115,29 @@ __device__ __forceinline__ uint32_t table_lines(const DbView &db, uint64_t key) 
       line = idx >> 5;
       lines++;
     }
+  }
+}
+
+// Preorder spans of the taxonomy (see DbView::span) from a parent array in
+// which every parent id is below its child's (host code, O(n)).
+inline void taxonomy_spans(const uint32_t *parent, size_t n, uint32_t *span) {
+  std::vector<uint32_t> size(n, 1), next(n, 0);
+  for (size_t i = n; i-- > 1;)
+    if (parent[i]) size[parent[i]] += size[i];
+  uint32_t cursor = 0;
+  if (n) span[0] = span[1] = 0;
+  for (size_t i = 1; i < n; i++) {
+    uint32_t tin;
+    if (parent[i] == 0) {
+      tin = cursor;
+      cursor += size[i];
+    } else {
+      tin = next[parent[i]];
+      next[parent[i]] += size[i];
+    }
+    next[i] = tin + 1;
+    span[2 * i] = tin;
+    span[2 * i + 1] = tin + size[i];
   }
 }
 

@@ -128,6 +128,7 @@ struct k2g_db {
   k2g::DbView view{};
   uint32_t *d_cells = nullptr;
   uint32_t *d_parent = nullptr;
+  uint32_t *d_span = nullptr;
   std::vector<uint64_t> external_ids;
   k2g_db_info info{};
   int device = 0;

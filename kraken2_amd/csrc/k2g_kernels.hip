@@ -97,6 +97,7 @@ __device__ __forceinline__ uint32_t shfl32(uint32_t v, int src) {
 // Per-fragment hit-count tables
 
 struct LdsTable {  // open addressing in LDS; taxon 0 marks an empty slot
+  static constexpr bool kSlotsPerLane = true;  // at most 256 slots: 4 per lane
   uint32_t *key, *cnt;
   uint32_t cap;
   __device__ void clear(int lane) {
@@ -143,6 +144,7 @@ __device__ __forceinline__ void store_agent(uint32_t *p, uint32_t v) {
 }
 
 struct DenseTable {  // overflow path: dense counts per taxon in global memory
+  static constexpr bool kSlotsPerLane = false;
   uint32_t *counts, *list, *len;
   __device__ void clear(int) {}
   __device__ bool add(uint32_t t, uint32_t c) {
@@ -710,16 +712,67 @@ __device__ void classify_fragment(const DbView &db, const ClassifyParams &P, con
   const uint32_t required = (uint32_t) ceil(P.confidence * (double) total);
   uint32_t best = 0, best_t = 0;
   const uint32_t ns = tab.slots();
-  for (uint32_t i = lane; i < ns; i += kWave) {
-    const uint32_t t = tab.slot_key(i);
-    if (t == 0) continue;
-    uint32_t s = 0;
-    for (uint32_t a = t; a != 0; a = db.parent[a]) s += tab.get(a);
-    if (s > best) {
-      best = s;
-      best_t = t;
-    } else if (s == best) {
-      best_t = lca(db.parent, best_t, t);
+  // LDS table with at most 64 hit taxa: each lane holds its (up to) four
+  // slots with their taxa's preorder spans, and a taxon's score (the counts
+  // of its hit ancestors-or-self) is summed over the hit taxa whose span
+  // holds it -- no walks up the parent array.  Otherwise (more taxa, or the
+  // dense table) every taxon walks its ancestors.
+  uint32_t skey[4] = {0, 0, 0, 0}, scnt[4] = {0, 0, 0, 0}, stin[4] = {0, 0, 0, 0};
+  bool by_span = false;
+  if constexpr (Table::kSlotsPerLane) {
+    uint64_t occ[4];
+    uint32_t stout[4], hits = 0;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const uint32_t i = (uint32_t) lane + (uint32_t) (kWave * r);
+      const uint32_t t = i < ns ? tab.slot_key(i) : 0u;
+      skey[r] = t;
+      scnt[r] = t ? tab.slot_cnt(i) : 0u;
+      stin[r] = stout[r] = 0;
+      if (t) {
+        stin[r] = db.span[2 * (uint64_t) t];
+        stout[r] = db.span[2 * (uint64_t) t + 1];
+      }
+      occ[r] = __ballot(t != 0);
+      hits += (uint32_t) __popcll(occ[r]);
+    }
+    if (hits <= (uint32_t) kWave) {
+      by_span = true;
+      uint32_t sc[4] = {0, 0, 0, 0};
+#pragma unroll
+      for (int r2 = 0; r2 < 4; r2++) {
+        for (uint64_t m = occ[r2]; m; m &= m - 1) {
+          const int j = __ffsll((long long) m) - 1;
+          const uint32_t at = readlane32(stin[r2], j), ao = readlane32(stout[r2], j);
+          const uint32_t ac = readlane32(scnt[r2], j);
+#pragma unroll
+          for (int r = 0; r < 4; r++) sc[r] += (at <= stin[r] && stin[r] < ao) ? ac : 0u;
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        if (!skey[r]) continue;
+        if (sc[r] > best) {
+          best = sc[r];
+          best_t = skey[r];
+        } else if (sc[r] == best) {
+          best_t = lca(db.parent, best_t, skey[r]);
+        }
+      }
+    }
+  }
+  if (!by_span) {
+    for (uint32_t i = lane; i < ns; i += kWave) {
+      const uint32_t t = tab.slot_key(i);
+      if (t == 0) continue;
+      uint32_t s = 0;
+      for (uint32_t a = t; a != 0; a = db.parent[a]) s += tab.get(a);
+      if (s > best) {
+        best = s;
+        best_t = t;
+      } else if (s == best) {
+        best_t = lca(db.parent, best_t, t);
+      }
     }
   }
   reduce_best(db.parent, best, best_t);
@@ -727,9 +780,16 @@ __device__ void classify_fragment(const DbView &db, const ClassifyParams &P, con
   uint32_t score = uniform32(call ? tab.get(call) : 0);
   while (call && score < required) {
     uint32_t part = 0;
-    for (uint32_t i = lane; i < ns; i += kWave) {
-      const uint32_t t = tab.slot_key(i);
-      if (t != 0 && is_ancestor(db.parent, call, t)) part += tab.slot_cnt(i);
+    if (by_span) {
+      const uint32_t ct = uniform32(db.span[2 * (uint64_t) call]);
+      const uint32_t co = uniform32(db.span[2 * (uint64_t) call + 1]);
+#pragma unroll
+      for (int r = 0; r < 4; r++) part += (skey[r] && ct <= stin[r] && stin[r] < co) ? scnt[r] : 0u;
+    } else {
+      for (uint32_t i = lane; i < ns; i += kWave) {
+        const uint32_t t = tab.slot_key(i);
+        if (t != 0 && is_ancestor(db.parent, call, t)) part += tab.slot_cnt(i);
+      }
     }
     score = uniform32(reduce_sum(part));
     if (score >= required) break;

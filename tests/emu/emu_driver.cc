@@ -57,6 +57,9 @@ int emu_classify(const uint32_t *cells, uint64_t capacity, uint32_t kb, uint32_t
   v.key_bits = kb;
   v.value_bits = vb;
   v.parent = parent;
+  std::vector<uint32_t> span(2 * (size_t) node_count);
+  taxonomy_spans(parent, node_count, span.data());
+  v.span = span.data();
   v.node_count = node_count;
   v.k = k;
   v.l = l;
