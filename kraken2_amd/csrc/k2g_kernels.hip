@@ -222,7 +222,8 @@ constexpr uint64_t kOddBits = 0xAAAAAAAAAAAAAAAAULL;
 // ~1/3 of lanes whose k-mer starts a new minimizer.
 constexpr int kQueue = 192;       // queued lookups per probe batch (>= 64 + 64)
 constexpr int kRingGuard = 8;     // windows of up to 9 l-mers read without wrapping
-constexpr int kBatchChunks = 8;   // chunks per probe batch
+constexpr int kBatchChunks = 8;   // chunks (and mate markers) per probe batch
+constexpr uint32_t kMateItem = 0xFFFFFFFFu;  // cqb value of a mate-marker item
 
 // KEEP_KEYS: the minimizers stay readable after their probes (HLL records
 // need them); otherwise each value overwrites its own key's low half, which
@@ -363,12 +364,16 @@ struct ProbeState {
 template <bool DNA, bool EXTRAS, int W, class Table>
 __device__ void scan_stream(const DbView &db, const ClassifyParams &P, const FragOut &out,
                             uint64_t base, uint64_t cap, const char *m, uint64_t L, int fr,
-                            RunState &st, Table &tab, WaveLds<EXTRAS> &lds, int *bad, int lane) {
+                            const char *m2, uint64_t L2, RunState &st, Table &tab,
+                            WaveLds<EXTRAS> &lds, int *bad, int lane) {
   static_assert(W >= 0 && W <= kRingGuard + 1, "window too wide for the ring guard");
   constexpr bool dna = DNA;
   // positions within a stream are 32-bit: the library rejects reads of 2^31
   // or more bases before launching
-  const uint32_t n = (uint32_t) (dna ? L : frame_len(L, fr));
+  uint32_t n = (uint32_t) (dna ? L : frame_len(L, fr));
+  // m2 (nucleotide pairs): mate 2 follows in the same probe batches, after a
+  // mate-marker item, so a pair's lookups share their probe rounds
+  bool second = m2 == nullptr;
   const uint32_t k = db.k, l = db.l;
   const uint32_t w = k - l + 1;  // l-mers per k-mer window
   const uint64_t lbits = (1ULL << l) - 1ULL;
@@ -385,10 +390,28 @@ __device__ void scan_stream(const DbView &db, const ClassifyParams &P, const Fra
   uint32_t *const runs_at = !kVgprConsts ? out.runs + 2 * base : in_vgpr(out.runs + 2 * base);
   const uint64_t cap_v = !kVgprConsts ? cap : in_vgpr(cap);
   uint32_t c = 0;
-  while (c < n) {
+  bool more = true;
+  while (more) {
     // ---- A: scan chunks, queue their lookups
     uint32_t qn = 0, g = 0;
-    while (c < n && g < (uint32_t) kBatchChunks && qn + kWave <= (uint32_t) kQueue) {
+    while (g < (uint32_t) kBatchChunks && qn + kWave <= (uint32_t) kQueue) {
+      if (c >= n) {  // this stream is done
+        if (second) {
+          more = false;
+          break;
+        }
+        // the mate marker, then mate 2 with a fresh scanner state
+        if (lane == 0) lds.cqb[g] = kMateItem;
+        g++;
+        second = true;
+        m = m2;
+        n = (uint32_t) L2;
+        c = 0;
+        last_amb = -1;
+        last_min = ~0ULL;
+        pBEh = pLEh = 0;
+        continue;
+      }
       const uint32_t p = c + (uint32_t) lane;
       const bool inb = p < n;
       uint32_t code = 0xFFu;
@@ -564,6 +587,11 @@ __device__ void scan_stream(const DbView &db, const ClassifyParams &P, const Fra
       const uint64_t LK = uniform64(lds.cmask[gi][1]);
       uint64_t E = uniform64(lds.cmask[gi][2]);
       const uint32_t qb = uniform32(lds.cqb[gi]);
+      if (qb == kMateItem) {  // mate 2 starts: its own latest-LOOKUP value
+        push_marker(out, base, cap, st, kCodeMate, tab, bad, lane);
+        last_taxon = 0;
+        continue;
+      }
       const bool nonamb = lane_bit(NA);
       const bool lookup = lane_bit(LK);
       const bool kvalid = lane_bit(E);
@@ -685,19 +713,27 @@ __device__ void classify_fragment(const DbView &db, const ClassifyParams &P, con
   __syncthreads();
 
   RunState st;
-  for (int mate = 0; mate < (paired ? 2 : 1) && !st.done; mate++) {
-    const char *m = in.seq + (mate ? o1 : o0);
-    const uint64_t L = mate ? o2 - o1 : o1 - o0;
-    for (int fr = 0; fr < nframes; fr++) {
-      if (db.k - db.l == 4 && db.revcom_version != 0)  // the default window (k=35, l=31)
-        scan_stream<DNA, EXTRAS, 5>(db, P, out, base, cap, m, L, fr, st, tab, lds, bad, lane);
-      else
-        scan_stream<DNA, EXTRAS, 0>(db, P, out, base, cap, m, L, fr, st, tab, lds, bad, lane);
+  if (!tx) {  // both mates in one scan (the mate marker between them)
+    const char *m2 = paired ? in.seq + o1 : nullptr;
+    if (db.k - db.l == 4 && db.revcom_version != 0)  // the default window (k=35, l=31)
+      scan_stream<DNA, EXTRAS, 5>(db, P, out, base, cap, in.seq + o0, o1 - o0, 0, m2, o2 - o1, st,
+                                  tab, lds, bad, lane);
+    else
+      scan_stream<DNA, EXTRAS, 0>(db, P, out, base, cap, in.seq + o0, o1 - o0, 0, m2, o2 - o1, st,
+                                  tab, lds, bad, lane);
+  } else {
+    for (int mate = 0; mate < (paired ? 2 : 1) && !st.done; mate++) {
+      const char *m = in.seq + (mate ? o1 : o0);
+      const uint64_t L = mate ? o2 - o1 : o1 - o0;
+      for (int fr = 0; fr < nframes; fr++) {
+        scan_stream<DNA, EXTRAS, 0>(db, P, out, base, cap, m, L, fr, nullptr, 0, st, tab, lds, bad,
+                                    lane);
+        if (st.done) break;
+        if (fr != 5) push_marker(out, base, cap, st, kCodeFrame, tab, bad, lane);
+      }
       if (st.done) break;
-      if (tx && fr != 5) push_marker(out, base, cap, st, kCodeFrame, tab, bad, lane);
+      if (paired && mate == 0) push_marker(out, base, cap, st, kCodeMate, tab, bad, lane);
     }
-    if (st.done) break;
-    if (paired && mate == 0) push_marker(out, base, cap, st, kCodeMate, tab, bad, lane);
   }
   if (st.run_count > 0) {
     if (lane == 0) close_run(out.runs + 2 * base, cap, st.n_runs, st.run_code, st.run_count, tab, bad);
