@@ -423,10 +423,11 @@ namespace k2g {
 // Element bound, reserved run space and sort key per fragment, and the batch
 // statistics at stats (zeroed by the caller): [0] set when a mate has 2^31
 // or more bases (the kernels count positions within a read in 32 bits);
-// u64 at [2]: the largest element count, at [4]: their sum.
+// u64 at [2]: the largest element count, at [4]: their sum (want_lengths;
+// launched with 256 threads per block).
 __global__ void k_bounds(const uint64_t *off, const uint8_t *paired, uint32_t paired_all,
                          uint64_t n, uint32_t k, uint32_t dna, uint32_t pct, uint64_t *bounds,
-                         uint64_t *caps, uint32_t *keys, uint32_t *stats) {
+                         uint64_t *caps, uint32_t *keys, uint32_t *stats, bool want_lengths) {
   const uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
   uint64_t e = 0;
   if (i < n) {
@@ -441,16 +442,30 @@ __global__ void k_bounds(const uint64_t *off, const uint8_t *paired, uint32_t pa
     bounds[n] = 0;
     caps[n] = 0;
   }
-  // one atomic per wave for the batch's largest and total element counts
+  if (!want_lengths) return;
+  // one atomic pair per block for the batch's largest and total element counts
+  __shared__ uint64_t bmax[256 / kWave], bsum[256 / kWave];
   uint64_t mx = e, sum = e;
   for (int d = 1; d < kWave; d <<= 1) {
     const uint64_t om = (uint64_t) __shfl_xor((unsigned long long) mx, d);
     sum += (uint64_t) __shfl_xor((unsigned long long) sum, d);
     mx = om > mx ? om : mx;
   }
-  if ((threadIdx.x & (kWave - 1)) == 0 && sum) {
-    atomicMax(reinterpret_cast<unsigned long long *>(stats + 2), (unsigned long long) mx);
-    atomicAdd(reinterpret_cast<unsigned long long *>(stats + 4), (unsigned long long) sum);
+  const unsigned wv = threadIdx.x / kWave;
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    bmax[wv] = mx;
+    bsum[wv] = sum;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (unsigned j = 1; j < blockDim.x / kWave; j++) {
+      mx = bmax[j] > mx ? bmax[j] : mx;
+      sum += bsum[j];
+    }
+    if (sum) {
+      atomicMax(reinterpret_cast<unsigned long long *>(stats + 2), (unsigned long long) mx);
+      atomicAdd(reinterpret_cast<unsigned long long *>(stats + 4), (unsigned long long) sum);
+    }
   }
 }
 
@@ -674,7 +689,7 @@ extern "C" int k2g_classify_device(k2g_batch *b, const k2g_classify_params *p, c
   HIP_TRY(hipMemsetAsync(stats, 0, 24, s));
   hipLaunchKernelGGL(k_bounds, dim3(g1), dim3(256), 0, s, d_off, d_paired,
                      (uint32_t) (p->paired != 0), n, v.k, v.dna, pct, b->bounds.p, b->run_cap.p,
-                     may_sort ? b->sort_keys.p : nullptr, stats);
+                     may_sort ? b->sort_keys.p : nullptr, stats, p->mode == 2);
   HIP_TRY(hipGetLastError());
   size_t tbytes = 0;
   HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tbytes, b->run_cap.p, b->run_off.p, n + 1, s));

@@ -287,20 +287,25 @@ struct ProbeState {
   // The padding copies the first cells, so cells read past the end of the
   // table are the ones the probe visits after wrapping, and an empty cell
   // somewhere ends every probe: no bounds or remaining-cell count.  A step
-  // reads the four cells from idx on (one 16-byte load, 4-byte aligned;
-  // the table is padded by at least four cells), all of them in probe order.
+  // reads four cells with one 16-byte load (4-byte aligned; the table is
+  // padded by at least four cells): those from idx on, or, when idx is among
+  // the last three cells of a 128-byte line, the line's last four (so a load
+  // never touches a second line the probe may not need).
   __device__ __forceinline__ bool step_lean(const DbView &db, const ProbeTable &t, uint32_t &out) {
     typedef uint32_t u32x4_a4 __attribute__((ext_vector_type(4), aligned(4)));
     const uint32_t vb = db.value_bits;
     const uint32_t vmask = (1u << vb) - 1u;
-    const u32x4_a4 v = *(const u32x4_a4 *) (t.cells + idx);
+    const uint64_t line_last4 = (idx | 31u) - 3u;
+    const uint64_t from = idx < line_last4 ? idx : line_last4;
+    const uint32_t off = (uint32_t) (idx - from);  // 0 except near a line's end
+    const u32x4_a4 v = *(const u32x4_a4 *) (t.cells + from);
     const uint32_t c4[4] = {v.x, v.y, v.z, v.w};
     uint32_t val = 0;
     bool stop = false;
 #pragma unroll
     for (int j = 3; j >= 0; j--) {  // the lowest stopping cell wins
       const uint32_t c = c4[j];
-      const bool st = (c & vmask) == 0u || (c >> vb) == ck;
+      const bool st = (uint32_t) j >= off && ((c & vmask) == 0u || (c >> vb) == ck);
       val = st ? (c & vmask) : val;  // an empty cell gives 0
       stop |= st;
     }
@@ -308,7 +313,7 @@ struct ProbeState {
       out = skip ? 0u : val;
       return true;
     }
-    idx += 4;
+    idx = from + 4;
     if (idx >= t.capacity) idx -= t.capacity;
     return false;
   }
