@@ -359,21 +359,29 @@ def write_sample_fasta(path1, path2, frags):
             f2.close()
 
 
-def cpu_baseline(work, d, files, threads):
+def cpu_baseline(work, d, files, threads, exe=None):
     """The reference classify (compiled from its sources into oracle/_ref) when
     present, else the oracle restatement, on all host threads.  Single reads
-    and pairs run as two invocations (one run cannot mix them); times add."""
+    and pairs run as two invocations (one run cannot mix them); times add.
+    exe: time that command line program instead (the GPU front-end)."""
     ref = os.path.join(ROOT, "oracle", "_ref", "classify")
     port = os.path.join(ROOT, "oracle", "bin", "k2cpu_classify")
-    exe, kind = (ref, "reference") if os.path.exists(ref) else (port, "port")
+    if exe:
+        kind = "gpu front-end"
+    else:
+        exe, kind = (ref, "reference") if os.path.exists(ref) else (port, "port")
     total_n, total_s, total_bp = 0, 0.0, 0.0
     for args in files:
         cmd = [exe, "-H", f"{d}/hash.k2d", "-t", f"{d}/taxo.k2d", "-o", f"{d}/opts.k2d", "-p", str(threads),
                "-T", str(work["confidence"]), "-g", str(work["min_hit_groups"]), "-O", "/dev/null", *args]
-        r = subprocess.run(cmd, capture_output=True, text=True)
+        env = dict(os.environ, K2G_TIMING="1") if kind == "gpu front-end" else None
+        r = subprocess.run(cmd, capture_output=True, text=True, env=env)
         if r.returncode != 0:
-            log(f"# cpu baseline failed: {r.stderr[-300:]}")
+            log(f"# {kind} run failed: {r.stderr[-300:]}")
             return None
+        for tl in r.stderr.splitlines():
+            if "timing:" in tl:
+                log(f"# {kind}{tl.strip().replace('timing:', ' phases:')}")
         m = re.search(r"(\d+) sequences \(([\d.]+) Mbp\) processed in ([\d.]+)s", r.stderr)
         total_n += int(m.group(1))
         total_bp += float(m.group(2))
@@ -620,7 +628,7 @@ def main():
     log(f"# gather ceiling {ceil_rate / 1e9:.2f} G lines/s ({ceil_ms:.1f} ms); classify "
         f"{probe_rate / 1e9:.2f} G lines/s")
 
-    cpu = None
+    cpu = end_to_end = None
     if not a.no_cpu:
         if work["kind"] == "host":
             files = [[host["reads_fa"]]]
@@ -644,6 +652,12 @@ def main():
             log(f"# cpu sample: {len(singles)} unpaired + {len(pairs)} pairs in {len(starts)} windows")
         cpu = cpu_baseline(work, tmp, files, os.cpu_count() or 1)
         log(f"# cpu baseline: {cpu}")
+        # the same files through the GPU front-end over the same timed interval
+        # (parse, classify, format, write); the table is loaded before it
+        gpu_cli = os.path.join(ROOT, "kraken2_amd", "bin", "k2g_classify")
+        if os.path.exists(gpu_cli):
+            end_to_end = cpu_baseline(work, tmp, files, os.cpu_count() or 1, exe=gpu_cli)
+            log(f"# gpu end to end: {end_to_end}")
 
     serving = None
     if work["kind"] == "gpu" and host.get("pools"):
@@ -706,6 +720,9 @@ def main():
         line["serving"] = serving
     if cpu:
         line["vs_cpu"] = round(value / cpu["value"], 2)
+    if end_to_end:  # PCIe, parsing and formatting included: never the headline value
+        line["gpu_end_to_end"] = end_to_end
+        line["end_to_end_vs_cpu"] = round(end_to_end["value"] / cpu["value"], 2)
     if work["kind"] == "gpu":
         line["config"].update({"table_estimate": host["estimate"], "gpu_build_s": round(host["build_s"], 2)})
     print(json.dumps(line), flush=True)

@@ -256,6 +256,55 @@ int k2g_estimate_capacity_device(const k2g_build_params *p, const char *d_seq,
                                  const uint64_t *off, uint64_t n, uint32_t n_ranges,
                                  uint64_t *estimate);
 
+/* ---- multi-database merge: the per-read core of the reference's merge tool
+ * (merge_hit_lists src/merge.cc:494-554, resolve_tree :369-421, the run
+ * folding of write_hit_list :328-367), which joins two kraken outputs of
+ * the same reads classified against two databases that share a taxonomy.
+ * Hit lists hold the values the reference parses into taxid_and_count
+ * (src/merge.cc:34-37, 423-469): external taxids and counts as 32-bit
+ * signed integers, with INT32_MAX for "A", INT32_MAX-1 for "|" and
+ * INT32_MAX-2 for "-" (src/merge.cc:176-178).  Taxids are mapped to nodes as
+ * the reference's multi-threaded path maps them (src/merge.cc:594). */
+typedef struct k2g_merger k2g_merger;
+
+#define K2G_MERGE_AMBIG 0x7FFFFFFF
+#define K2G_MERGE_MATE 0x7FFFFFFE
+#define K2G_MERGE_FRAME 0x7FFFFFFD
+
+typedef struct {
+  const int32_t *items; /* (taxid, count) pairs */
+  const uint64_t *off;  /* n + 1 pair offsets, off[0] = 0: read i is pairs [off[i], off[i+1]) */
+  uint64_t n;
+} k2g_hit_lists;
+
+typedef struct {
+  float confidence; /* merge -i */
+} k2g_merge_params;
+
+typedef struct {
+  uint64_t n;
+  /* per read: resolve_tree's value, printed as the read's taxid when either
+   * input calls the read (src/merge.cc:701-711); the external id of the
+   * call, or the node id where the confidence climb returns one (:413) */
+  const int32_t *call;
+  const uint32_t *n_tokens; /* per read: printed "taxid:count" pairs */
+  /* printed pairs of read i start at pair a.off[i] + b.off[i] (its slots
+   * hold up to the two lists' combined length); each value prints as
+   * int_to_string does (:251-326) */
+  const int32_t *tokens;
+  const uint8_t *joined; /* per pair slot: 1 = no space follows the pair */
+  double kernel_ms;
+} k2g_merge_results;
+
+/* The taxonomy (taxo.k2d, the merged taxonomy of both databases) resident
+ * on the current device. */
+int k2g_merge_open(const char *taxo_k2d, k2g_merger **out);
+/* Merges read i of a with read i of b for every i (a->n == b->n).  Results
+ * stay valid until the next call on m or k2g_merge_close. */
+int k2g_merge(k2g_merger *m, const k2g_merge_params *p, const k2g_hit_lists *a,
+              const k2g_hit_lists *b, k2g_merge_results *out);
+int k2g_merge_close(k2g_merger *m);
+
 #ifdef __cplusplus
 }
 #endif

@@ -3,7 +3,8 @@
 The hot path (minimizer scan, compact-hash probe, hit-count/LCA resolution)
 runs as HIP kernels for gfx950 inside libk2g.so; this package is the thin
 Python binding used by tests and the benchmark.  The C ABI is
-include/k2g.h; the classify command-line front-end is bin/k2g_classify.
+include/k2g.h; the command-line front-ends are bin/k2g_classify and
+bin/k2g_merge (multi-database merge of kraken outputs).
 """
 from ._lib import (  # noqa: F401
     BIN_DIR,
@@ -18,6 +19,8 @@ from ._lib import (  # noqa: F401
     ClassifyOutput,
     Db,
     K2GError,
+    Merger,
+    MergeOutput,
     Server,
     expand_spaced_seed_mask,
     format_hitlist,

@@ -56,7 +56,14 @@ EXPORTED_SYMBOLS = (
     "k2g_build_destroy",
     "k2g_estimate_capacity",
     "k2g_estimate_capacity_device",
+    "k2g_merge_open",
+    "k2g_merge",
+    "k2g_merge_close",
 )
+
+MERGE_AMBIG = 0x7FFFFFFF
+MERGE_MATE = 0x7FFFFFFE
+MERGE_FRAME = 0x7FFFFFFD
 
 
 class K2GError(RuntimeError):
@@ -134,6 +141,25 @@ class Results(C.Structure):
     ]
 
 
+class HitLists(C.Structure):
+    _fields_ = [("items", C.c_void_p), ("off", C.c_void_p), ("n", C.c_uint64)]
+
+
+class MergeParams(C.Structure):
+    _fields_ = [("confidence", C.c_float)]
+
+
+class MergeResults(C.Structure):
+    _fields_ = [
+        ("n", C.c_uint64),
+        ("call", C.POINTER(C.c_int32)),
+        ("n_tokens", C.POINTER(C.c_uint32)),
+        ("tokens", C.POINTER(C.c_int32)),
+        ("joined", C.POINTER(C.c_uint8)),
+        ("kernel_ms", C.c_double),
+    ]
+
+
 _lib = None
 
 
@@ -173,6 +199,10 @@ def load_library(path: str = LIB_PATH):
     lib.k2g_host_free.argtypes = [vp]
     lib.k2g_results_checksum.argtypes = [C.POINTER(Results), C.POINTER(u64)]
     lib.k2g_db_gather_ceiling.argtypes = [vp, u64, u64, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    lib.k2g_merge_open.argtypes = [C.c_char_p, C.POINTER(vp)]
+    lib.k2g_merge.argtypes = [vp, C.POINTER(MergeParams), C.POINTER(HitLists), C.POINTER(HitLists),
+                              C.POINTER(MergeResults)]
+    lib.k2g_merge_close.argtypes = [vp]
     lib.k2g_batch_stream.restype = vp
     for name in EXPORTED_SYMBOLS:
         if name not in ("k2g_last_error", "k2g_batch_stream"):
@@ -561,3 +591,54 @@ def pinned_array(nbytes: int) -> np.ndarray:
     arr = arr.view()
     arr.flags.writeable = True
     return arr
+
+
+@dataclass
+class MergeOutput:
+    call: np.ndarray      # int32 per read (resolve_tree's value)
+    n_tokens: np.ndarray  # uint32 per read
+    tokens: np.ndarray    # (slots, 2) int32: printed (taxid, count) pairs
+    joined: np.ndarray    # uint8 per slot: 1 = no space after the pair
+    start: np.ndarray     # uint64 per read: first slot (a.off[i] + b.off[i])
+    kernel_ms: float
+
+
+class Merger:
+    """The merged taxonomy resident on the GPU and the per-read merge of two
+    hit-list sets (k2g_merge_open / k2g_merge / k2g_merge_close)."""
+
+    def __init__(self, taxo_k2d: str):
+        self._lib = load_library()
+        self._h = C.c_void_p()
+        _check(self._lib.k2g_merge_open(taxo_k2d.encode(), C.byref(self._h)))
+
+    def merge(self, items_a, off_a, items_b, off_b, confidence: float = 0.0) -> MergeOutput:
+        ia = np.ascontiguousarray(items_a, np.int32).reshape(-1)
+        ib = np.ascontiguousarray(items_b, np.int32).reshape(-1)
+        oa = np.ascontiguousarray(off_a, np.uint64)
+        ob = np.ascontiguousarray(off_b, np.uint64)
+        a = HitLists(ia.ctypes.data, oa.ctypes.data, len(oa) - 1)
+        b = HitLists(ib.ctypes.data, ob.ctypes.data, len(ob) - 1)
+        r = MergeResults()
+        _check(self._lib.k2g_merge(self._h, C.byref(MergeParams(confidence)), C.byref(a), C.byref(b),
+                                   C.byref(r)))
+        n = int(r.n)
+        slots = int(oa[-1] + ob[-1]) if n else 0
+
+        def arr(ptr, count, dtype):
+            return np.ctypeslib.as_array(ptr, (count,)).astype(dtype) if count else np.zeros(0, dtype)
+
+        return MergeOutput(arr(r.call, n, np.int32), arr(r.n_tokens, n, np.uint32),
+                           arr(r.tokens, 2 * slots, np.int32).reshape(-1, 2), arr(r.joined, slots, np.uint8),
+                           (oa[:n] + ob[:n]).astype(np.uint64), float(r.kernel_ms))
+
+    def close(self):
+        if self._h:
+            self._lib.k2g_merge_close(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -11,7 +11,12 @@
 #include <sys/time.h>
 #include <unistd.h>
 
+#include <algorithm>
 #include <cmath>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -210,7 +215,22 @@ struct Ctx {
   Stats stats;
   k2host::TaxonCounters counters;
   std::vector<uint64_t> kmer_totals;
+  // K2G_TIMING=1: seconds waiting for parsed input, packing, on the GPU
+  // (k2g_classify), formatting and writing, and counters
+  bool timing = false;
+  double t_wait = 0, t_pack = 0, t_gpu = 0, t_format = 0, t_count = 0;
+  double t_alloc = 0, t_offsets = 0;  // parts of t_pack
+  // packed bases of one batch (pinned, kept across batches) and offsets
+  char *seq = nullptr;
+  size_t seq_cap = 0;
+  std::vector<uint64_t> off;
 };
+
+double now_s() {
+  struct timeval t;
+  gettimeofday(&t, nullptr);
+  return t.tv_sec + t.tv_usec / 1e6;
+}
 
 void append_uint(std::string &s, uint64_t v) {
   char buf[24];
@@ -279,6 +299,7 @@ void format_line(const Ctx &c, const Record &s1, const Record *s2, uint32_t call
 void run_batch(Ctx &c, std::vector<Block> &blocks, Outputs &out) {
   const Options &o = c.o;
   const bool paired = o.paired_end_processing;
+  double t = now_s();
   // fragment arrays
   size_t nfrag = 0, bases = 0;
   for (auto &b : blocks) {
@@ -293,12 +314,40 @@ void run_batch(Ctx &c, std::vector<Block> &blocks, Outputs &out) {
     for (auto &r : b.a) bases += r.seq.size();
     for (auto &r : b.b) bases += r.seq.size();
   }
-  std::string seq;
-  seq.reserve(bases);
-  std::vector<uint64_t> off(2 * nfrag + 1, 0);
-  size_t f = 0;
+  // offsets first, then each block copies its mates into the pinned buffer
+  // that is kept across batches (no page faults after the first batch, and
+  // the upload reads pinned memory)
+  double ta = now_s();
+  if (bases + 1 > c.seq_cap) {
+    if (c.seq) k2g_host_free(c.seq);
+    c.seq = nullptr;
+    c.seq_cap = std::max<size_t>(bases + 1, c.seq_cap + c.seq_cap / 2);
+    const int rc = k2g_host_alloc(c.seq_cap, (void **) &c.seq);
+    if (rc) die(rc, k2g_last_error());
+  }
+  double tb = now_s();
+  c.t_alloc += tb - ta;
+  std::vector<uint64_t> &off = c.off;
+  off.resize(2 * nfrag + 1);
+  uint64_t pos = 0;
   for (auto &b : blocks) {
-    for (size_t i = 0; i < b.frags; i++, f++) {
+    for (size_t i = 0; i < b.frags; i++) {
+      const size_t f = b.first + i;
+      const Record &r1 = paired && o.single_file_pairs ? b.a[2 * i] : b.a[i];
+      off[2 * f] = pos;
+      pos += r1.seq.size();
+      off[2 * f + 1] = pos;
+      if (paired) pos += (o.single_file_pairs ? b.a[2 * i + 1] : b.b[i]).seq.size();
+    }
+  }
+  off[2 * nfrag] = pos;
+  ta = now_s();
+  c.t_offsets += ta - tb;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(o.num_threads)
+  for (size_t bi = 0; bi < blocks.size(); bi++) {
+    Block &b = blocks[bi];
+    for (size_t i = 0; i < b.frags; i++) {
+      const size_t f = b.first + i;
       Record *r1, *r2 = nullptr;
       if (!paired) {
         r1 = &b.a[i];
@@ -313,13 +362,10 @@ void run_batch(Ctx &c, std::vector<Block> &blocks, Outputs &out) {
         mask_low_quality(*r1, o.minimum_quality_score);
         if (r2) mask_low_quality(*r2, o.minimum_quality_score);
       }
-      off[2 * f] = seq.size();
-      seq += r1->seq;
-      off[2 * f + 1] = seq.size();
-      if (r2) seq += r2->seq;
+      std::memcpy(c.seq + off[2 * f], r1->seq.data(), r1->seq.size());
+      if (r2) std::memcpy(c.seq + off[2 * f + 1], r2->seq.data(), r2->seq.size());
     }
   }
-  off[2 * nfrag] = seq.size();
 
   k2g_classify_params p{};
   p.confidence = o.confidence_threshold;
@@ -329,10 +375,14 @@ void run_batch(Ctx &c, std::vector<Block> &blocks, Outputs &out) {
   p.mode = o.gpu_mode;
   p.want_kmer_counts = !o.report_filename.empty();
   p.want_hll = !o.report_filename.empty() && o.report_kmer_data;
-  k2g_fragments fr{seq.data(), off.data(), nullptr, nfrag};
+  k2g_fragments fr{c.seq, off.data(), nullptr, nfrag};
   k2g_results res;
+  double t2 = now_s();
+  c.t_pack += t2 - t;
   int rc = k2g_classify(c.batch, &p, &fr, &res);
   if (rc) die(rc, k2g_last_error());
+  t = now_s();
+  c.t_gpu += t - t2;
 
   // format blocks in parallel, emit in order
 #pragma omp parallel for schedule(dynamic, 1) num_threads(o.num_threads)
@@ -382,6 +432,8 @@ void run_batch(Ctx &c, std::vector<Block> &blocks, Outputs &out) {
     c.stats.total_bases += b.stats.total_bases;
     c.stats.total_classified += b.stats.total_classified;
   }
+  t2 = now_s();
+  c.t_format += t2 - t;
   if (!o.report_filename.empty()) {
     // read counts per called taxon; k-mer counters per block (HLL state
     // depends on the block a hit falls in, src/classify.cc:554, 640-645)
@@ -404,40 +456,86 @@ void run_batch(Ctx &c, std::vector<Block> &blocks, Outputs &out) {
         if (res.kmer_counts[t]) c.counters[t].n_kmers += res.kmer_counts[t];
     }
   }
+  c.t_count += now_s() - t2;
 }
 
+// Reads the input in the reference's blocks (src/classify.cc:523-543) on a
+// reader thread, batches of blocks handed to the caller through a two-deep
+// queue, so parsing the next batch overlaps the GPU and the formatting of
+// this one.  Batches, blocks and output stay in input order.
 int process_files(Ctx &c, const char *f1, const char *f2, Outputs &out) {
   const Options &o = c.o;
   k2host::Reader r1, r2;
   if (!r1.open(f1)) die(66, std::string("unable to open ") + (f1 ? f1 : "stdin"));
   if (f2 && !r2.open(f2)) die(66, std::string("unable to open ") + f2);
-  bool done = false;
-  std::vector<Block> blocks;
-  while (!done) {
-    blocks.clear();
-    size_t bases = 0;
-    while (bases < o.batch_bases && blocks.size() < 4096) {
-      blocks.emplace_back();
-      Block &b = blocks.back();
-      bool ok;
-      if (!o.paired_end_processing) {
-        ok = r1.load_block(3 * 1024 * 1024, b.a) > 0;
-      } else if (!o.single_file_pairs) {
-        ok = r1.load_batch(10000, b.a) > 0;
-        if (ok) ok = r2.load_batch(10000, b.b) > 0;
-      } else {
-        ok = r1.load_batch(20000, b.a) > 0;
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<std::vector<Block>> ready;
+  std::deque<std::vector<Block>> spare;  // consumed batches, their records reused
+  bool finished = false;
+  std::thread reader([&] {
+    bool done = false;
+    while (!done) {
+      std::vector<Block> blocks;
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        if (!spare.empty()) {
+          blocks.swap(spare.front());
+          spare.pop_front();
+        }
       }
-      if (!ok) {
-        blocks.pop_back();
-        done = true;
-        break;
+      size_t bases = 0, used = 0;
+      while (bases < o.batch_bases && used < 4096) {
+        if (used == blocks.size()) blocks.emplace_back();
+        Block &b = blocks[used];
+        b.stats = Stats();
+        b.kraken.clear();
+        b.c1.clear();
+        b.c2.clear();
+        b.u1.clear();
+        b.u2.clear();
+        bool ok;
+        if (!o.paired_end_processing) {
+          ok = r1.load_block(3 * 1024 * 1024, b.a) > 0;
+        } else if (!o.single_file_pairs) {
+          ok = r1.load_batch(10000, b.a) > 0;
+          if (ok) ok = r2.load_batch(10000, b.b) > 0;
+        } else {
+          ok = r1.load_batch(20000, b.a) > 0;
+        }
+        if (!ok) {
+          done = true;
+          break;
+        }
+        used++;
+        for (auto &r : b.a) bases += r.seq.size();
+        for (auto &r : b.b) bases += r.seq.size();
       }
-      for (auto &r : b.a) bases += r.seq.size();
-      for (auto &r : b.b) bases += r.seq.size();
+      blocks.resize(used);
+      std::unique_lock<std::mutex> lk(mu);
+      cv.wait(lk, [&] { return ready.size() < 2; });
+      if (!blocks.empty()) ready.push_back(std::move(blocks));
+      if (done) finished = true;
+      cv.notify_all();
     }
-    if (!blocks.empty()) run_batch(c, blocks, out);
+  });
+  for (;;) {
+    std::vector<Block> blocks;
+    {
+      const double t = now_s();
+      std::unique_lock<std::mutex> lk(mu);
+      cv.wait(lk, [&] { return !ready.empty() || finished; });
+      c.t_wait += now_s() - t;
+      if (ready.empty()) break;
+      blocks = std::move(ready.front());
+      ready.pop_front();
+      cv.notify_all();
+    }
+    run_batch(c, blocks, out);
+    std::lock_guard<std::mutex> lk(mu);
+    if (spare.size() < 2) spare.push_back(std::move(blocks));
   }
+  reader.join();
   return 0;
 }
 
@@ -458,6 +556,7 @@ void report_stats(double seconds, const Stats &s) {  // src/classify.cc:447-477
 int main(int argc, char **argv) {
   Ctx c;
   parse(argc, argv, c.o);
+  if (const char *t = getenv("K2G_TIMING")) c.timing = atoi(t) != 0;
   std::cerr << "Loading database information...";
   std::string err;
   if (!c.tax.load(c.o.taxonomy_filename, err)) die(65, err);
@@ -499,6 +598,12 @@ int main(int argc, char **argv) {
   gettimeofday(&t2, nullptr);
   const double secs = (t2.tv_sec - t1.tv_sec) + (t2.tv_usec - t1.tv_usec) / 1e6;
   report_stats(secs, c.stats);
+  if (c.timing)
+    fprintf(stderr,
+            "  timing: input wait %.3fs, packing %.3fs (buffer %.3fs, offsets %.3fs), GPU %.3fs, "
+            "format+write %.3fs, counters %.3fs, outside the phases %.3fs\n",
+            c.t_wait, c.t_pack, c.t_alloc, c.t_offsets, c.t_gpu, c.t_format, c.t_count,
+            secs - c.t_wait - c.t_pack - c.t_gpu - c.t_format - c.t_count);
   if (!o.report_filename.empty()) {
     if (!k2host::report_kraken_style(o.report_filename, o.report_zero_counts, o.report_kmer_data,
                                      c.tax, c.counters, c.stats.total_sequences,

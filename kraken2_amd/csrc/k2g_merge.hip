@@ -1,0 +1,156 @@
+// Multi-database merge (the reference's merge tool, src/merge.cc): the
+// resident merged taxonomy and the batch entry point over k_merge
+// (k2g_merge_kernel.h).
+#include "k2g_internal.h"
+#include "k2g_merge_kernel.h"
+
+#include <climits>
+#include <cstring>
+
+using namespace k2g;
+
+namespace {
+
+template <class T>
+hipError_t upload(DevBuf<T> &d, const T *h, size_t n, hipStream_t s) {
+  hipError_t e = d.reserve(n);
+  if (e == hipSuccess && n) e = hipMemcpyAsync(d.p, h, n * sizeof(T), hipMemcpyHostToDevice, s);
+  return e;
+}
+
+}  // namespace
+
+struct k2g_merger {
+  MergeTax view{};
+  DevBuf<uint32_t> parent, span, key_node;
+  DevBuf<int32_t> ext32;
+  DevBuf<uint64_t> ext64, key;
+  DevBuf<int32_t> a_items, b_items, w_print, w_count, tokens, call;
+  DevBuf<uint64_t> a_off, b_off;
+  DevBuf<uint32_t> w_node, n_tokens;
+  DevBuf<uint8_t> joined;
+  std::vector<int32_t> h_call, h_tokens;
+  std::vector<uint32_t> h_n_tokens;
+  std::vector<uint8_t> h_joined;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  ~k2g_merger() {
+    for (auto *b : {&parent, &span, &key_node, &w_node, &n_tokens}) b->release();
+    for (auto *b : {&ext32, &a_items, &b_items, &w_print, &w_count, &tokens, &call}) b->release();
+    for (auto *b : {&ext64, &key, &a_off, &b_off}) b->release();
+    joined.release();
+    if (ev0) (void) hipEventDestroy(ev0);
+    if (ev1) (void) hipEventDestroy(ev1);
+    if (stream) (void) hipStreamDestroy(stream);
+  }
+};
+
+extern "C" int k2g_merge_open(const char *taxo_k2d, k2g_merger **out) {
+  if (!taxo_k2d || !out) return fail(K2G_EUSAGE, "null argument");
+  *out = nullptr;
+  std::vector<uint32_t> parent;
+  std::vector<uint64_t> ext;
+  std::string err;
+  int rc = read_taxonomy_nodes(taxo_k2d, parent, ext, err);
+  if (rc) return fail(rc, err);
+  const size_t n = parent.size();
+  MergeTables tb;
+  merge_tables(parent, ext, tb);
+  const std::vector<uint32_t> &span = tb.span, &key_node = tb.key_node;
+  const std::vector<int32_t> &ext32 = tb.ext32;
+  const std::vector<uint64_t> &key = tb.key;
+  auto *g = new k2g_merger();
+  hipStream_t s;
+  hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+  if (e == hipSuccess) g->stream = s;
+  if (e == hipSuccess) e = hipEventCreate(&g->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&g->ev1);
+  if (e == hipSuccess) e = upload(g->parent, parent.data(), n, s);
+  if (e == hipSuccess) e = upload(g->span, span.data(), 2 * n, s);
+  if (e == hipSuccess) e = upload(g->ext32, ext32.data(), n, s);
+  if (e == hipSuccess) e = upload(g->ext64, ext.data(), n, s);
+  if (e == hipSuccess) e = upload(g->key, key.data(), key.size(), s);
+  if (e == hipSuccess) e = upload(g->key_node, key_node.data(), key_node.size(), s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    delete g;
+    return fail(K2G_ESOFTWARE, std::string("merge taxonomy upload failed: ") + hipGetErrorString(e));
+  }
+  g->view = MergeTax{g->parent.p, g->span.p, g->ext32.p, g->ext64.p, g->key.p, g->key_node.p,
+                     (uint32_t) key.size()};
+  *out = g;
+  return K2G_OK;
+}
+
+static int check_lists(const k2g_hit_lists *h, const char *which) {
+  if (!h->off) return fail(K2G_EUSAGE, std::string("null offsets in hit lists ") + which);
+  if (h->off[0] != 0) return fail(K2G_EUSAGE, std::string("hit lists ") + which + " must start at offset 0");
+  for (uint64_t i = 0; i < h->n; i++)
+    if (h->off[i + 1] < h->off[i])
+      return fail(K2G_EUSAGE, std::string("offsets of hit lists ") + which + " must not decrease");
+  if (h->off[h->n] && !h->items) return fail(K2G_EUSAGE, std::string("null items in hit lists ") + which);
+  if (h->off[h->n] > (uint64_t) UINT32_MAX)
+    return fail(K2G_EUSAGE, std::string("hit lists ") + which + " hold 2^32 or more items");
+  return K2G_OK;
+}
+
+extern "C" int k2g_merge(k2g_merger *g, const k2g_merge_params *p, const k2g_hit_lists *a,
+                         const k2g_hit_lists *b, k2g_merge_results *out) {
+  if (!g || !p || !a || !b || !out) return fail(K2G_EUSAGE, "null argument");
+  if (a->n != b->n) return fail(K2G_EUSAGE, "the two inputs hold different numbers of reads");
+  int rc = check_lists(a, "a");
+  if (!rc) rc = check_lists(b, "b");
+  if (rc) return rc;
+  for (uint64_t i = 0; i < a->n; i++)
+    if (a->off[i + 1] - a->off[i] + b->off[i + 1] - b->off[i] > (uint64_t) UINT32_MAX)
+      return fail(K2G_EUSAGE, "a read's hit lists hold 2^32 or more items");
+  const uint64_t n = a->n, na = a->off[n], nb = b->off[n], slots = na + nb;
+  hipStream_t s = g->stream;
+  HIP_TRY(upload(g->a_items, a->items, 2 * na, s));
+  HIP_TRY(upload(g->b_items, b->items, 2 * nb, s));
+  HIP_TRY(upload(g->a_off, a->off, n + 1, s));
+  HIP_TRY(upload(g->b_off, b->off, n + 1, s));
+  HIP_TRY(g->w_print.reserve(slots));
+  HIP_TRY(g->w_count.reserve(slots));
+  HIP_TRY(g->w_node.reserve(slots));
+  HIP_TRY(g->tokens.reserve(2 * slots));
+  HIP_TRY(g->joined.reserve(slots));
+  HIP_TRY(g->n_tokens.reserve(n));
+  HIP_TRY(g->call.reserve(n));
+  HIP_TRY(hipEventRecord(g->ev0, s));
+  if (n) {
+    const uint64_t blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(k_merge, dim3((uint32_t) blocks), dim3(256), 0, s, g->view, g->a_items.p, g->a_off.p,
+                       g->b_items.p, g->b_off.p, n, p->confidence, g->w_print.p, g->w_count.p, g->w_node.p,
+                       g->tokens.p, g->joined.p, g->n_tokens.p, g->call.p);
+    HIP_TRY(hipGetLastError());
+  }
+  HIP_TRY(hipEventRecord(g->ev1, s));
+  g->h_call.resize(n);
+  g->h_n_tokens.resize(n);
+  g->h_tokens.resize(2 * slots);
+  g->h_joined.resize(slots);
+  if (n) {
+    HIP_TRY(hipMemcpyAsync(g->h_call.data(), g->call.p, n * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(g->h_n_tokens.data(), g->n_tokens.p, n * 4, hipMemcpyDeviceToHost, s));
+  }
+  if (slots) {
+    HIP_TRY(hipMemcpyAsync(g->h_tokens.data(), g->tokens.p, 2 * slots * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(g->h_joined.data(), g->joined.p, slots, hipMemcpyDeviceToHost, s));
+  }
+  HIP_TRY(hipStreamSynchronize(s));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, g->ev0, g->ev1));
+  out->n = n;
+  out->call = g->h_call.data();
+  out->n_tokens = g->h_n_tokens.data();
+  out->tokens = g->h_tokens.data();
+  out->joined = g->h_joined.data();
+  out->kernel_ms = ms;
+  return K2G_OK;
+}
+
+extern "C" int k2g_merge_close(k2g_merger *g) {
+  delete g;
+  return K2G_OK;
+}

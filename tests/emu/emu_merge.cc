@@ -1,0 +1,91 @@
+// CPU execution of the merge kernel (kraken2_amd/csrc/k2g_merge_kernel.h)
+// behind the library's merge entry points, so the k2g_merge front-end
+// (kraken2_amd/csrc/host/merge_main.cc) runs end to end without a GPU in
+// the tests.  The kernel runs one host thread per read, in read order.
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "k2g.h"
+#include "k2g_merge_kernel.h"
+
+namespace emu {
+thread_local Dim3 threadIdx_, blockIdx_;
+Dim3 blockDim_, gridDim_;
+std::barrier<> *bar = nullptr;
+unsigned long long vote_slot[64];
+uint64_t shfl_slot[64];
+std::atomic<unsigned long long> collectives{0};
+}  // namespace emu
+
+namespace {
+thread_local std::string g_error;
+}
+
+struct k2g_merger {
+  std::vector<uint32_t> parent;
+  std::vector<uint64_t> ext;
+  k2g::MergeTables tb;
+  k2g::MergeTax view{};
+  std::vector<int32_t> w_print, w_count, tokens, call;
+  std::vector<uint32_t> w_node, n_tokens;
+  std::vector<uint8_t> joined;
+};
+
+extern "C" {
+
+const char *k2g_last_error(void) { return g_error.c_str(); }
+
+int k2g_merge_open(const char *taxo_k2d, k2g_merger **out) {
+  auto *g = new k2g_merger();
+  int rc = k2g::read_taxonomy_nodes(taxo_k2d, g->parent, g->ext, g_error);
+  if (rc) {
+    delete g;
+    return rc;
+  }
+  k2g::merge_tables(g->parent, g->ext, g->tb);
+  g->view = k2g::MergeTax{g->parent.data(), g->tb.span.data(), g->tb.ext32.data(), g->ext.data(),
+                          g->tb.key.data(),   g->tb.key_node.data(), (uint32_t) g->tb.key.size()};
+  *out = g;
+  return K2G_OK;
+}
+
+int k2g_merge(k2g_merger *g, const k2g_merge_params *p, const k2g_hit_lists *a, const k2g_hit_lists *b,
+              k2g_merge_results *out) {
+  if (a->n != b->n || a->off[0] || b->off[0]) {
+    g_error = "bad hit lists";
+    return K2G_EUSAGE;
+  }
+  const uint64_t n = a->n, slots = a->off[n] + b->off[n];
+  g->w_print.assign(slots, 0);
+  g->w_count.assign(slots, 0);
+  g->w_node.assign(slots, 0);
+  g->tokens.assign(2 * slots, 0);
+  g->joined.assign(slots, 0);
+  g->n_tokens.assign(n, 0);
+  g->call.assign(n, 0);
+  emu::blockDim_.x = 1;
+  emu::gridDim_.x = (unsigned) n;
+  for (uint64_t r = 0; r < n; r++) {
+    emu::blockIdx_.x = (unsigned) r;
+    emu::threadIdx_.x = 0;
+    k2g::k_merge(g->view, a->items, a->off, b->items, b->off, n, p->confidence, g->w_print.data(),
+                 g->w_count.data(), g->w_node.data(), g->tokens.data(), g->joined.data(), g->n_tokens.data(),
+                 g->call.data());
+  }
+  out->n = n;
+  out->call = g->call.data();
+  out->n_tokens = g->n_tokens.data();
+  out->tokens = g->tokens.data();
+  out->joined = g->joined.data();
+  out->kernel_ms = 0;
+  return K2G_OK;
+}
+
+int k2g_merge_close(k2g_merger *g) {
+  delete g;
+  return K2G_OK;
+}
+
+}  // extern "C"

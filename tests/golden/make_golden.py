@@ -352,13 +352,79 @@ def protein_db():
             f.write(run([cl, *db, "-p", "2", *args]).stdout)
 
 
+# merge-classifications cases: (taxonomy dir, input 1, input 2, options)
+MERGE_CASES = {
+    "paired": ("toy", "merge/a_paired", "merge/b_paired", []),
+    "paired_conf_names": ("toy", "merge/a_paired", "merge/b_paired", ["-i", "0.3", "-n"]),
+    "edge": ("toy", "merge/a_edge", "merge/b_edge", []),
+    "edge_conf03": ("toy", "merge/a_edge", "merge/b_edge", ["-i", "0.3"]),
+    "edge_conf1_swapped": ("toy", "merge/b_edge", "merge/a_edge", ["-i", "1.0"]),
+    "edge_names_zeros": ("toy", "merge/a_edge", "merge/b_edge", ["-i", "0.05", "-n", "-z"]),
+    "edge_self": ("toy", "merge/a_edge", "merge/a_edge", ["-i", "0.7"]),
+    "quick": ("toy", "merge/a_quick", "merge/b_quick", ["-i", "0.5"]),
+    "prot_self": ("prot", "prot/single", "prot/single", ["-i", "0.2", "-n"]),
+    "prot_paired_self": ("prot", "prot/paired", "prot/paired", []),
+}
+
+
+def merge():
+    """Two databases over the two halves of the toy genomes (same k, l and
+    seed), the toy reads classified against each by the reference classify,
+    and the reference merge of those outputs under the full toy taxonomy (the
+    merged taxonomy k2 would make from both seqid2taxid maps).  Expected
+    outputs are recorded as digests."""
+    d = os.path.join(HERE, "merge")
+    os.makedirs(d, exist_ok=True)
+    files = sorted(f for f in os.listdir(TOY_IN) if f.endswith(".fa"))
+    toy_d = os.path.join(HERE, "toy")
+    for part, sel in (("a", files[0::2]), ("b", files[1::2])):
+        db = os.path.join(d, f"db_{part}.tmp")
+        os.makedirs(db, exist_ok=True)
+        lib, idmap = os.path.join(db, "library.fna"), os.path.join(db, "seqid2taxid.map")
+        with open(lib, "w") as lf, open(idmap, "w") as mf:
+            for name in sel:
+                for h, s in fasta_records(os.path.join(TOY_IN, name)):
+                    sid = h.split()[0]
+                    lf.write(f">{h}\n{s}\n")
+                    mf.write(f"{sid}\t{sid.split('|')[1]}\n")
+        build_db(db, lib, TOY_IN, idmap, 35, 31, DNA_TEMPLATE)
+        cl = [os.path.join(REF, "classify"), "-H", f"{db}/hash.k2d", "-t", f"{db}/taxo.k2d", "-o",
+              f"{db}/opts.k2d", "-p", "2"]
+        for name, args in {"paired": ["-P", "-g", "2", f"{toy_d}/reads_1.fq", f"{toy_d}/reads_2.fq"],
+                           "edge": ["-g", "0", f"{toy_d}/edge.fa"],
+                           "quick": ["-q", "-g", "2", f"{toy_d}/edge.fa"]}.items():
+            with open(os.path.join(d, f"{part}_{name}.kraken"), "wb") as f:
+                f.write(run([*cl, *args]).stdout)
+        shutil.rmtree(db)
+    expected = {}
+    for case, (tax, in1, in2, opts) in MERGE_CASES.items():
+        out, hdr, rep = (os.path.join(d, f"{case}.{x}.tmp") for x in ("out", "hdr", "rep"))
+        run([os.path.join(REF, "merge"), "-T", "2", "-t", os.path.join(HERE, tax, "taxo.k2d"), "-o", out,
+             "-c", hdr, "-r", rep, *opts, os.path.join(HERE, in1 + ".kraken"), os.path.join(HERE, in2 + ".kraken")])
+        expected[case] = {}
+        for key, path in (("output", out), ("headers", hdr), ("report", rep)):
+            data = open(path, "rb").read()
+            expected[case][key] = {"sha256": hashlib.sha256(data).hexdigest(), "lines": data.count(b"\n")}
+            os.remove(path)
+    with open(os.path.join(d, "expected.json"), "w") as f:
+        json.dump(expected, f, indent=1, sort_keys=True)
+    with open(os.path.join(d, "README"), "w") as f:
+        f.write("Inputs: the toy reads classified by the reference against two databases built from the two\n"
+                "halves of the toy genomes; expected.json: digests of the reference merge's output, classified\n"
+                "headers and report for each case of MERGE_CASES; see tests/golden/make_golden.py.\n")
+
+
 if __name__ == "__main__":
     if not os.path.exists(os.path.join(REF, "classify")):
         raise SystemExit("build the reference first: make -C oracle ref")
+    if sys.argv[1:] == ["merge"]:  # only the merge fixtures
+        merge()
+        raise SystemExit(0)
     toy()
     scanner()
     scanner_revcom0()
     hll()
     translate()
     protein_db()
+    merge()
     print("golden fixtures written")

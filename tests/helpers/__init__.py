@@ -195,3 +195,87 @@ class Oracle:
             raise RuntimeError("oracle text buffer too small")
         res["text"] = [t.decode() for t in text.raw[:used].split(b"\0")[:n]]
         return res
+
+
+def taxonomy_external_ids(taxo_k2d):
+    """External ids of a taxo.k2d's nodes (node 0 first)."""
+    raw = open(taxo_k2d, "rb").read()
+    n = int(np.frombuffer(raw, np.uint64, 1, 8)[0])
+    nodes = np.frombuffer(raw, np.uint64, 7 * n, 32).reshape(n, 7)
+    return [int(x) for x in nodes[:, 5]]
+
+
+def random_kraken_pairs(ext_ids, n, seed):
+    """Two kraken outputs of the same n reads as two databases would give
+    them: per read the same k-mer positions, ambiguous runs, mate ("|:|")
+    and frame ("-:-") borders in both, each position's taxon drawn per
+    output from a few taxa of the taxonomy (0 for no hit, sometimes an
+    unknown id), so LCA ties and confidence climbs are frequent.  Also "0:0"
+    lines of reads with no k-mer.  Returns the two texts."""
+    rng = random.Random(seed)
+    known = [e for e in ext_ids[1:] if e]
+    out_a, out_b = [], []
+
+    def rle(pos):
+        toks, prev, cnt = [], None, 0
+        for t in pos:
+            if t == prev:
+                cnt += 1
+                continue
+            if prev is not None:
+                toks.append(prev + ":" + str(cnt))
+            prev, cnt = t, 1
+        if prev is not None:
+            toks.append(prev + ":" + str(cnt))
+        return toks
+
+    for i in range(n):
+        pool = rng.sample(known, min(len(known), rng.randrange(1, 5))) + ["0"] * rng.randrange(0, 3)
+        if rng.random() < 0.05:
+            pool.append(str(10 ** 9 + rng.randrange(100)))  # not in the taxonomy
+        r = rng.random()
+        if r < 0.03:
+            lists = (["0:0"], ["0:0"])
+            lens = "0"
+        else:
+            n_seg = 2 if r < 0.35 else (6 if r < 0.45 else 1)
+            border = "|:|" if n_seg == 2 else "-:-"
+            la, lb, seg_lens = [], [], []
+            for s in range(n_seg):
+                L = rng.randrange(1, 80)
+                seg_lens.append(str(L + 30))
+                pa, pb = [], []
+                j = 0
+                while j < L:
+                    if rng.random() < 0.05:
+                        run_len = min(L - j, rng.randrange(1, 30))
+                        pa += ["A"] * run_len
+                        pb += ["A"] * run_len
+                        j += run_len
+                        continue
+                    pa.append(str(rng.choice(pool)))
+                    pb.append(str(rng.choice(pool)))
+                    j += 1
+                if s:
+                    la.append(border)
+                    lb.append(border)
+                la += rle(pa)
+                lb += rle(pb)
+            lists = (la, lb)
+            lens = "|".join(seg_lens[:2]) if n_seg == 2 else seg_lens[0]
+        for out, lst in zip((out_a, out_b), lists):
+            st = "C" if rng.random() < 0.7 else "U"
+            call = str(rng.choice(known)) if st == "C" else "0"
+            out.append(f"{st}\tread{i}\t{call}\t{lens}\t{' '.join(lst)}\n")
+    return "".join(out_a), "".join(out_b)
+
+
+# Kraken lines whose hit-list totals differ between two outputs of the
+# same reads (second list shorter or longer than the first), with count-0,
+# unknown and marker items: edge cases of the merge walk.
+RAGGED_MERGE_LINES = (
+    ["C\tr0\t1\t50\t211044:5 0:3\n", "C\tr1\t1\t50\t211044:5\n", "U\tr2\t0\t50\t0:0\n",
+     "C\tr3\t1\t50\t999999:4 11320:2 A:3\n", "C\tr4\t1\t50\t11320:7 |:| 211044:2\n"],
+    ["C\tr0\t1\t50\t211044:5\n", "C\tr1\t1\t50\t211044:2 11320:9\n", "U\tr2\t0\t50\t11320:3\n",
+     "U\tr3\t0\t50\t11320:1\n", "C\tr4\t1\t50\t11320:7 |:| 211044:2\n"],
+)

@@ -1,0 +1,106 @@
+"""k2g_merge on the GPU (the reference's merge tool, src/merge.cc).
+
+The front-end (kraken2_amd/bin/k2g_merge, merging through libk2g's
+k2g_merge entry points) against the compiled reference's outputs
+(tests/golden/merge) and against the oracle (oracle/bin/k2cpu_merge) on
+random hit lists; the entry points through ctypes on empty and ragged
+batches, checked against the oracle restatement.
+"""
+import hashlib
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+GOLD = os.path.join(HERE, "golden")
+sys.path.insert(0, GOLD)
+sys.path.insert(0, HERE)
+sys.path.insert(0, ROOT)
+from make_golden import MERGE_CASES  # noqa: E402
+from helpers import RAGGED_MERGE_LINES, random_kraken_pairs, taxonomy_external_ids  # noqa: E402
+
+import kraken2_amd as k2  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+EXE = os.path.join(ROOT, "kraken2_amd", "bin", "k2g_merge")
+ORACLE = os.path.join(ROOT, "oracle", "bin", "k2cpu_merge")
+EXPECTED = json.load(open(os.path.join(GOLD, "merge", "expected.json")))
+TOY_TAXO = os.path.join(GOLD, "toy", "taxo.k2d")
+
+
+def run_merge(exe, taxo, in1, in2, opts, d, threads=4):
+    out, hdr, rep = (os.path.join(d, x) for x in ("out.kraken", "headers.txt", "report.txt"))
+    print(f"merge: {os.path.basename(exe)} {' '.join(opts)} {os.path.basename(in1)} {os.path.basename(in2)}",
+          flush=True)
+    r = subprocess.run([exe, "-T", str(threads), "-t", taxo, "-o", out, "-c", hdr, "-r", rep, *opts, in1, in2],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    return {k: open(p, "rb").read() for k, p in (("output", out), ("headers", hdr), ("report", rep))}
+
+
+@pytest.mark.parametrize("case", sorted(MERGE_CASES))
+def test_gpu_merge_matches_reference_golden(case, tmp_path):
+    tax, in1, in2, opts = MERGE_CASES[case]
+    got = run_merge(EXE, os.path.join(GOLD, tax, "taxo.k2d"), os.path.join(GOLD, in1 + ".kraken"),
+                    os.path.join(GOLD, in2 + ".kraken"), opts, str(tmp_path))
+    for key, exp in EXPECTED[case].items():
+        assert got[key].count(b"\n") == exp["lines"], (case, key)
+        assert hashlib.sha256(got[key]).hexdigest() == exp["sha256"], (case, key)
+
+
+@pytest.mark.parametrize("seed,opts", [(11, []), (12, ["-i", "0.2", "-n", "-z"]), (13, ["-i", "0.6"])])
+def test_gpu_merge_random_vs_oracle(seed, opts, tmp_path):
+    a, b = random_kraken_pairs(taxonomy_external_ids(TOY_TAXO), 40000, seed)
+    pa, pb = tmp_path / "a.kraken", tmp_path / "b.kraken"
+    pa.write_text(a)
+    pb.write_text(b)
+    (tmp_path / "gpu").mkdir()
+    (tmp_path / "cpu").mkdir()
+    got = run_merge(EXE, TOY_TAXO, str(pa), str(pb), opts, str(tmp_path / "gpu"))
+    exp = run_merge(ORACLE, TOY_TAXO, str(pa), str(pb), opts, str(tmp_path / "cpu"))
+    for key in ("output", "headers", "report"):
+        assert got[key] == exp[key], key
+
+
+def test_gpu_merge_ragged_lines_vs_oracle(tmp_path):
+    """Hit lists whose totals differ between the inputs (the second list
+    shorter or longer than the first), count-0 and unknown taxa."""
+    lines_a, lines_b = RAGGED_MERGE_LINES
+    pa, pb = tmp_path / "a.kraken", tmp_path / "b.kraken"
+    pa.write_text("".join(lines_a))
+    pb.write_text("".join(lines_b))
+    for opts in ([], ["-i", "0.5"]):
+        d1, d2 = tmp_path / f"g{len(opts)}", tmp_path / f"c{len(opts)}"
+        d1.mkdir()
+        d2.mkdir()
+        got = run_merge(EXE, TOY_TAXO, str(pa), str(pb), opts, str(d1))
+        exp = run_merge(ORACLE, TOY_TAXO, str(pa), str(pb), opts, str(d2))
+        assert got == exp
+
+
+def test_merge_entry_points_empty_and_errors():
+    m = k2.Merger(TOY_TAXO)
+    try:
+        z = np.zeros(1, np.uint64)
+        out = m.merge(np.zeros(0, np.int32), z, np.zeros(0, np.int32), z)
+        assert len(out.call) == 0 and len(out.tokens) == 0
+        # a read with no items in the first list prints nothing
+        out = m.merge(np.zeros(0, np.int32), np.zeros(2, np.uint64), np.array([211044, 5], np.int32),
+                      np.array([0, 1], np.uint64))
+        assert out.n_tokens.tolist() == [0] and out.call.tolist() == [0]
+        with pytest.raises(k2.K2GError):  # different read counts
+            m.merge(np.zeros(0, np.int32), np.zeros(2, np.uint64), np.zeros(0, np.int32), z)
+        with pytest.raises(k2.K2GError):  # offsets not starting at 0
+            m.merge(np.array([1, 1], np.int32), np.array([1, 1], np.uint64), np.array([1, 1], np.int32),
+                    np.array([1, 1], np.uint64))
+        with pytest.raises(k2.K2GError):  # decreasing offsets
+            m.merge(np.array([1, 1], np.int32), np.array([0, 1, 0], np.uint64), np.array([1, 1], np.int32),
+                    np.array([0, 1, 1], np.uint64))
+    finally:
+        m.close()
