@@ -41,6 +41,7 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "reads/sec batched vs plain GPU classify (bit-exact) + p50/p99 request latency"
 BLOCK_BASES = 3 * 1024 * 1024       # reference block for single reads (src/classify.cc:527)
 BLOCK_PAIRS = 10000                 # reference block for pairs (src/classify.cc:31, 531)
+DNA_SEED = "1" * 17 + "01" * 7      # kraken2-build's spaced seed for l = 31 (7 spaced positions)
 
 WORKLOADS = {
     "c1": dict(kind="host", desc="c1: plain classify, 64 genomes x 200 kb, 100k x 150 bp single-end, -T 0 -g 2",
@@ -146,12 +147,14 @@ class Runner:
 
 
 def run_host_workload(a, work, k2):
-    """c1: generated on the host with k2synth, built by the deterministic CPU build."""
+    """c1: generated on the host with k2synth, built by the GPU build front-ends."""
     from helpers import K2SYNTH, make_synthetic_db, run
     from kraken2_amd.synth import Hip
     tmp = a.keep or scratch_dir("k2g_c1_")
     os.makedirs(tmp, exist_ok=True)
-    make_synthetic_db(tmp, seed=work["seed"], levels=work["levels"], genome_len=work["genome_len"])
+    make_synthetic_db(tmp, seed=work["seed"], levels=work["levels"], genome_len=work["genome_len"],
+                      tools=(os.path.join(k2.BIN_DIR, "k2g_estimate_capacity"),
+                             os.path.join(k2.BIN_DIR, "k2g_build_db")))
     reads = os.path.join(tmp, "reads.fa")
     run([K2SYNTH, "reads", "--lib", os.path.join(tmp, "library.fna"), "--out", reads, "--seed",
          str(work["seed"]), "--n", str(work["n_reads"]), "--len", str(work["read_len"]), "--sub", "0.01",
@@ -326,8 +329,9 @@ def run_serving(k2, db, tmp, pools, a, work):
         log(f"# serving mismatch: request {i} ({r['size']} fragments, {r['bases']} bases, {r['pool']}): "
             f"in the mix {all_dig[i]:#x}, alone again {[hex(x) for x in again]}, one-slot server {solo:#x}")
     batch.close()
-    # and the CPU oracle on the smaller requests (calls and hitlists)
-    small = sorted(range(len(all_reqs)), key=lambda i: all_reqs[i]["size"])[:40]
+    # and, in the CPU leg (not with --no-cpu), the oracle on the smaller
+    # requests (calls and hitlists)
+    small = [] if a.no_cpu else sorted(range(len(all_reqs)), key=lambda i: all_reqs[i]["size"])[:40]
     mism = checked = 0
     for i in small:
         r = all_reqs[i]
@@ -470,6 +474,114 @@ def copy_calibration():
     log(f"# copy calibration: 3 x {n} bytes")
 
 
+def run_merge_bench(a):
+    """Multi-database merge at scale (the reference's merge tool, src/merge.cc):
+    a synthetic library (k2synth) split into two halves by genome, one
+    database per half built on the GPU (k2g_estimate_capacity, k2g_build_db),
+    read pairs classified against each (k2g_classify -P), then the two kraken
+    outputs merged under the taxonomy of the whole library, file to file, by
+    k2g_merge and by the reference merge binary (the CPU baseline; the oracle
+    restatement where it is absent).  Outputs must be byte-identical."""
+    import kraken2_amd as k2
+    from kraken2_amd.taxonomy import write_k2_taxonomy
+    tmp = a.keep or scratch_dir("k2g_merge_")
+    os.makedirs(tmp, exist_ok=True)
+    synth = os.path.join(k2.BIN_DIR, "k2synth")
+    bins = {n: os.path.join(k2.BIN_DIR, n) for n in ("k2g_estimate_capacity", "k2g_build_db", "k2g_classify",
+                                                     "k2g_merge")}
+    threads = min(16, os.cpu_count() or 1)
+
+    def sh(cmd, **kw):
+        r = subprocess.run(cmd, capture_output=True, **kw)
+        if r.returncode != 0:
+            raise RuntimeError(f"{cmd[0]} failed ({r.returncode}): {r.stderr[-400:]!r}")
+        return r
+
+    t0 = time.time()
+    sh([synth, "db-inputs", "--out", tmp, "--seed", "5", "--levels", a.merge_levels, "--genome-len",
+        str(a.merge_genome_len), "--sub", "0.05"])
+    recs = open(os.path.join(tmp, "library.fna"), "rb").read().split(b">")[1:]
+    idmap = dict(l.split("\t") for l in open(os.path.join(tmp, "seqid2taxid.map")).read().split("\n") if l)
+    tax_dir = os.path.join(tmp, "taxonomy")
+    for part in (0, 1):
+        d = os.path.join(tmp, f"db{part}")
+        os.makedirs(d, exist_ok=True)
+        lib = b"".join(b">" + r for i, r in enumerate(recs) if i % 2 == part)
+        names = [r.split(None, 1)[0].decode() for i, r in enumerate(recs) if i % 2 == part]
+        with open(os.path.join(d, "seqid2taxid.map"), "w") as f:
+            f.write("".join(f"{n}\t{idmap[n]}\n" for n in names))
+        est = int(sh([bins["k2g_estimate_capacity"], "-k", "35", "-l", "31", "-S", DNA_SEED], input=lib).stdout)
+        sh([bins["k2g_build_db"], "-k", "35", "-l", "31", "-S", DNA_SEED, "-H", f"{d}/hash.k2d", "-t",
+            f"{d}/taxo.k2d", "-o", f"{d}/opts.k2d", "-n", tax_dir, "-m", f"{d}/seqid2taxid.map", "-c",
+            str(int((est + 8192) / 0.7)), "-p", str(threads)], input=lib)
+    taxo = os.path.join(tmp, "merged_taxo.k2d")
+    write_k2_taxonomy(os.path.join(tax_dir, "nodes.dmp"), os.path.join(tax_dir, "names.dmp"),
+                      [int(v) for v in idmap.values()], taxo)
+    r1, r2 = os.path.join(tmp, "r1.fa"), os.path.join(tmp, "r2.fa")
+    sh([synth, "reads", "--lib", os.path.join(tmp, "library.fna"), "--out", r1, "--out2", r2, "--seed", "7",
+        "--n", str(a.merge_reads), "--len", "150", "--sub", "0.01", "--random-frac", "0.1"])
+    outs = []
+    for part in (0, 1):
+        d = os.path.join(tmp, f"db{part}")
+        o = os.path.join(tmp, f"k{part}.kraken")
+        sh([bins["k2g_classify"], "-H", f"{d}/hash.k2d", "-t", f"{d}/taxo.k2d", "-o", f"{d}/opts.k2d", "-p",
+            str(threads), "-P", "-g", "2", "-O", o, r1, r2])
+        outs.append(o)
+    in_bytes = sum(os.path.getsize(o) for o in outs)
+    log(f"# merge inputs: {a.merge_reads} pairs classified against two half databases, "
+        f"{in_bytes / 1e6:.0f} MB of kraken output; setup {time.time() - t0:.1f}s")
+
+    def merge(exe, tag, env=None):
+        d = os.path.join(tmp, tag)
+        os.makedirs(d, exist_ok=True)
+        cmd = [exe, "-T", str(threads), "-t", taxo, "-o", f"{d}/out", "-c", f"{d}/hdr", "-r", f"{d}/rep",
+               "-i", str(a.merge_confidence), *outs]
+        t = time.time()
+        r = sh(cmd, env=env, text=True)
+        secs = time.time() - t
+        return secs, r.stderr, {k: open(f"{d}/{k}", "rb").read() for k in ("out", "hdr", "rep")}
+
+    ref = os.path.join(ROOT, "oracle", "_ref", "merge")
+    port = os.path.join(ROOT, "oracle", "bin", "k2cpu_merge")
+    cpu_exe, kind = (ref, "reference") if os.path.exists(ref) else (port, "port")
+    gpu_runs = []
+    for i in range(a.steps + a.warmup):
+        secs, err, got = merge(bins["k2g_merge"], "gpu", env=dict(os.environ, K2G_TIMING="1"))
+        if i >= a.warmup:
+            gpu_runs.append((secs, err))
+    secs_gpu = min(s for s, _ in gpu_runs)
+    timing = [e for s, e in gpu_runs if s == secs_gpu][0].strip().splitlines()[-1]
+    m = re.search(r"kernel ([\d.]+)s", timing)
+    kernel_s = float(m.group(1)) if m else None
+    log(f"# k2g_merge: {secs_gpu:.3f}s ({timing})")
+    secs_cpu, _, exp = merge(cpu_exe, "cpu")
+    log(f"# {kind} merge -T {threads}: {secs_cpu:.3f}s")
+    equal = got == exp
+    log(f"# byte-identical output, headers and report: {equal}")
+    n = a.merge_reads
+    items = 0
+    for o in outs:
+        with open(o, "rb") as f:
+            for line in f:
+                items += line.count(b":")
+    line = {"metric": "merged reads/s (multi-database merge of two kraken outputs, file to file)",
+            "value": round(n / secs_gpu, 1), "unit": "reads/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+            "ms_per_step": round(secs_gpu * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+            "config": {"workload": f"merge: {n} read pairs 2x150 classified (-P -g 2) against two databases built "
+                                   f"from the two halves of a {a.merge_levels} synthetic library "
+                                   f"({a.merge_genome_len} bp genomes), -i {a.merge_confidence}, -T {threads}",
+                       "input_bytes": in_bytes, "hit_list_items": items},
+            "kernel_ms": round(kernel_s * 1e3, 3) if kernel_s is not None else None,
+            "kernel_reads_per_s": round(n / kernel_s, 1) if kernel_s else None,
+            "cpu_baseline": {"value": round(n / secs_cpu, 1), "unit": "reads/s", "cores": threads, "kind": kind,
+                             "sample": f"the whole workload ({n} reads), {secs_cpu:.2f} s wall clock"},
+            "vs_cpu": round(secs_cpu / secs_gpu, 2), "bit_exact_vs_cpu": bool(equal)}
+    print(json.dumps(line), flush=True)
+    if not a.keep:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -496,6 +608,12 @@ def main():
     ap.add_argument("--serve-pool", type=int, default=300_000, help="fragments per serving pool")
     ap.add_argument("--scale", type=float, default=1.0,
                     help="scale genomes and reads down for a quick functional run (not a bench number)")
+    ap.add_argument("--merge", action="store_true",
+                    help="measure the multi-database merge (k2g_merge) instead of classify")
+    ap.add_argument("--merge-reads", type=int, default=2_000_000)
+    ap.add_argument("--merge-levels", default="8,8,8")
+    ap.add_argument("--merge-genome-len", type=int, default=100000)
+    ap.add_argument("--merge-confidence", type=float, default=0.0)
     a = ap.parse_args()
     if a.gpus != 1:
         log(f"bench.py measures one GPU: --gpus {a.gpus} is not supported (reads would shard across GPUs "
@@ -503,6 +621,9 @@ def main():
         sys.exit(2)
     if a.copy_calibration:
         copy_calibration()
+        return
+    if a.merge:
+        run_merge_bench(a)
         return
     import kraken2_amd as k2
 
@@ -595,9 +716,15 @@ def main():
         modes_agree = (np.array_equal(digests["plain"][pm], digests["batched"][pm]) if "plain" in digests
                        else True) and np.array_equal(digests["plain_one_launch"], digests["batched"])
 
+    # CPU leg, part 1: the oracle checks an even sample of fragments (the
+    # reference binary is timed further down); skipped with --no-cpu
     rng = np.random.default_rng(7)
     sample = np.sort(rng.choice(n, size=min(n, 4000), replace=False))
-    if work["kind"] == "host":
+    if a.no_cpu:
+        sample = sample[:0]
+    if not len(sample):
+        s_seq, so = b"", np.zeros(1, np.uint64)
+    elif work["kind"] == "host":
         seq_all = host["seq"]
         parts, so = [], np.zeros(2 * len(sample) + 1, np.uint64)
         pos = 0
@@ -611,7 +738,7 @@ def main():
         s_seq = b"".join(parts)
     else:
         s_seq, so, _ = host["readset"].host_fragments(sample)
-    mism, s_dig = oracle_check(k2, db, tmp, s_seq, so, paired[sample], work)
+    mism, s_dig = oracle_check(k2, db, tmp, s_seq, so, paired[sample], work) if len(sample) else (0, np.zeros(0))
     sample_digest_agree = bool(np.array_equal(s_dig, digests["batched"][sample]))
     log(f"# verify: modes agree ({plain_checked} plain / {n} one-launch fragments): {modes_agree}; oracle sample mismatches: "
         f"{mism}/{len(sample)}; sample digests equal the bench run's: {sample_digest_agree}")

@@ -24,8 +24,12 @@ def run(cmd, **kw):
 
 def make_synthetic_db(d, seed=1, levels="2,2,3", ranks=None, genome_len=20000, sub=0.05,
                       k=35, l=31, template=DNA_SEED_TEMPLATE, load_factor=0.7,
-                      block=16384, subblock=2048):
-    """Writes taxonomy + genomes with k2synth and builds hash/taxo/opts with the oracle."""
+                      block=16384, subblock=2048, tools=None):
+    """Writes taxonomy + genomes with k2synth and builds hash/taxo/opts with the
+    oracle, or with tools = (estimate_capacity, build_db) executables of the
+    same command lines (the GPU build front-ends)."""
+    est_exe, build_exe = tools or (os.path.join(ORACLE_BIN, "k2cpu_estimate_capacity"),
+                                   os.path.join(ORACLE_BIN, "k2cpu_build_db"))
     nlev = len(levels.split(","))
     if ranks is None:
         ranks = ",".join((["phylum", "class", "order", "family", "genus", "species"])[-nlev:])
@@ -33,11 +37,11 @@ def make_synthetic_db(d, seed=1, levels="2,2,3", ranks=None, genome_len=20000, s
          "--ranks", ranks, "--genome-len", str(genome_len), "--sub", str(sub)])
     lib = os.path.join(d, "library.fna")
     with open(lib, "rb") as f:
-        est = int(run([os.path.join(ORACLE_BIN, "k2cpu_estimate_capacity"), "-k", str(k), "-l", str(l),
+        est = int(run([est_exe, "-k", str(k), "-l", str(l),
                        "-S", template], stdin=f).stdout.strip())
     cap = int((est + 8192) / load_factor)
     with open(lib, "rb") as f:
-        run([os.path.join(ORACLE_BIN, "k2cpu_build_db"), "-k", str(k), "-l", str(l), "-S", template,
+        run([build_exe, "-k", str(k), "-l", str(l), "-S", template,
              "-H", os.path.join(d, "hash.k2d"), "-t", os.path.join(d, "taxo.k2d"),
              "-o", os.path.join(d, "opts.k2d"), "-n", os.path.join(d, "taxonomy"),
              "-m", os.path.join(d, "seqid2taxid.map"), "-c", str(cap), "-B", str(block),
