@@ -287,12 +287,12 @@ typedef struct {
    * input calls the read (src/merge.cc:701-711); the external id of the
    * call, or the node id where the confidence climb returns one (:413) */
   const int32_t *call;
-  const uint32_t *n_tokens; /* per read: printed "taxid:count" pairs */
-  /* printed pairs of read i start at pair a.off[i] + b.off[i] (its slots
-   * hold up to the two lists' combined length); each value prints as
-   * int_to_string does (:251-326) */
-  const int32_t *tokens;
-  const uint8_t *joined; /* per pair slot: 1 = no space follows the pair */
+  /* n + 1 offsets: read i prints pairs [token_off[i], token_off[i+1]) of
+   * tokens, as int_to_string prints each value (:251-326); a read with no
+   * pairs prints no line (:727) */
+  const uint64_t *token_off;
+  const int32_t *tokens; /* (taxid, count) pairs */
+  const uint8_t *joined; /* per pair: 1 = no space follows the pair */
   double kernel_ms;
 } k2g_merge_results;
 

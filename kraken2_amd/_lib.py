@@ -153,7 +153,7 @@ class MergeResults(C.Structure):
     _fields_ = [
         ("n", C.c_uint64),
         ("call", C.POINTER(C.c_int32)),
-        ("n_tokens", C.POINTER(C.c_uint32)),
+        ("token_off", C.POINTER(C.c_uint64)),
         ("tokens", C.POINTER(C.c_int32)),
         ("joined", C.POINTER(C.c_uint8)),
         ("kernel_ms", C.c_double),
@@ -595,11 +595,10 @@ def pinned_array(nbytes: int) -> np.ndarray:
 
 @dataclass
 class MergeOutput:
-    call: np.ndarray      # int32 per read (resolve_tree's value)
-    n_tokens: np.ndarray  # uint32 per read
-    tokens: np.ndarray    # (slots, 2) int32: printed (taxid, count) pairs
-    joined: np.ndarray    # uint8 per slot: 1 = no space after the pair
-    start: np.ndarray     # uint64 per read: first slot (a.off[i] + b.off[i])
+    call: np.ndarray       # int32 per read (resolve_tree's value)
+    token_off: np.ndarray  # uint64, n + 1: read i prints tokens[token_off[i]:token_off[i+1]]
+    tokens: np.ndarray     # (m, 2) int32: printed (taxid, count) pairs
+    joined: np.ndarray     # uint8 per pair: 1 = no space after the pair
     kernel_ms: float
 
 
@@ -623,14 +622,14 @@ class Merger:
         _check(self._lib.k2g_merge(self._h, C.byref(MergeParams(confidence)), C.byref(a), C.byref(b),
                                    C.byref(r)))
         n = int(r.n)
-        slots = int(oa[-1] + ob[-1]) if n else 0
 
         def arr(ptr, count, dtype):
             return np.ctypeslib.as_array(ptr, (count,)).astype(dtype) if count else np.zeros(0, dtype)
 
-        return MergeOutput(arr(r.call, n, np.int32), arr(r.n_tokens, n, np.uint32),
-                           arr(r.tokens, 2 * slots, np.int32).reshape(-1, 2), arr(r.joined, slots, np.uint8),
-                           (oa[:n] + ob[:n]).astype(np.uint64), float(r.kernel_ms))
+        token_off = arr(r.token_off, n + 1, np.uint64)
+        m = int(token_off[-1])
+        return MergeOutput(arr(r.call, n, np.int32), token_off, arr(r.tokens, 2 * m, np.int32).reshape(-1, 2),
+                           arr(r.joined, m, np.uint8), float(r.kernel_ms))
 
     def close(self):
         if self._h:

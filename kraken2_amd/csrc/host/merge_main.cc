@@ -18,12 +18,18 @@
 #include <getopt.h>
 #include <sys/time.h>
 
+#include <algorithm>
 #include <climits>
+#include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
 #include <iostream>
+#include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 #include <omp.h>
@@ -35,7 +41,7 @@ namespace {
 
 constexpr int32_t kAmbig = INT32_MAX, kMate = INT32_MAX - 1, kFrame = INT32_MAX - 2,
                   kValid = INT32_MAX - 3;
-constexpr size_t kBatchLines = 1u << 20;
+size_t g_batch_lines = 1u << 18;  // K2G_MERGE_BATCH_LINES
 
 void die(int code, const std::string &msg) {
   std::cerr << "merge: " << msg << "\n";
@@ -71,14 +77,30 @@ void put_int(std::string &o, int32_t v) {
     return;
   }
   char buf[16];
-  int n = 0;
+  char *e = buf + sizeof(buf), *b = e;
   uint32_t x = (uint32_t) v;
   do {
-    buf[n++] = (char) ('0' + x % 10);
+    *--b = (char) ('0' + x % 10);
     x /= 10;
   } while (x);
-  while (n) o += buf[--n];
+  o.append(b, e - b);
 }
+
+// A byte buffer that grows without initialising its new bytes.
+struct Bytes {
+  std::unique_ptr<char[]> p;
+  size_t size = 0, cap = 0;
+  char *data() { return p.get(); }
+  const char *data() const { return p.get(); }
+  void reserve(size_t n) {
+    if (n <= cap) return;
+    const size_t c = std::max(n, 2 * cap);
+    std::unique_ptr<char[]> q(new char[c]);
+    if (size) memcpy(q.get(), p.get(), size);
+    p.swap(q);
+    cap = c;
+  }
+};
 
 // Lines of a file, batch by batch, each line with its '\n' when it has one.
 class LineFile {
@@ -90,16 +112,19 @@ class LineFile {
   ~LineFile() {
     if (f_) fclose(f_);
   }
-  // Appends up to max_lines lines to buf (cleared first); starts gets each
+  // Up to max_lines lines into buf (replacing its contents); starts gets each
   // line's offset plus the end offset.
-  size_t read(size_t max_lines, std::string &buf, std::vector<size_t> &starts) {
-    buf.swap(carry_);
-    carry_.clear();
+  size_t read(size_t max_lines, Bytes &buf, std::vector<size_t> &starts) {
+    buf.size = 0;
+    buf.reserve(carry_.size + 1);
+    if (carry_.size) memcpy(buf.data(), carry_.data(), carry_.size);
+    buf.size = carry_.size;
+    carry_.size = 0;
     starts.clear();
     size_t pos = 0, lines = 0;
     for (;;) {
       while (lines < max_lines) {
-        const char *nl = (const char *) memchr(buf.data() + pos, '\n', buf.size() - pos);
+        const char *nl = (const char *) memchr(buf.data() + pos, '\n', buf.size - pos);
         if (!nl) break;
         starts.push_back(pos);
         pos = nl - buf.data() + 1;
@@ -107,21 +132,23 @@ class LineFile {
       }
       if (lines == max_lines) break;
       if (eof_) {
-        if (pos < buf.size()) {  // a last line without '\n'
+        if (pos < buf.size) {  // a last line without '\n'
           starts.push_back(pos);
-          pos = buf.size();
+          pos = buf.size;
           lines++;
         }
         break;
       }
-      const size_t old = buf.size();
-      buf.resize(old + (16u << 20));
-      const size_t got = fread(&buf[old], 1, 16u << 20, f_);
-      buf.resize(old + got);
+      constexpr size_t kChunk = 8u << 20;
+      buf.reserve(buf.size + kChunk);
+      const size_t got = fread(buf.data() + buf.size, 1, kChunk, f_);
+      buf.size += got;
       if (got == 0) eof_ = true;
     }
-    carry_.assign(buf, pos, std::string::npos);
-    buf.resize(pos);
+    carry_.reserve(buf.size - pos + 1);
+    carry_.size = buf.size - pos;
+    if (carry_.size) memcpy(carry_.data(), buf.data() + pos, carry_.size);
+    buf.size = pos;
     starts.push_back(pos);
     return lines;
   }
@@ -129,7 +156,7 @@ class LineFile {
  private:
   FILE *f_ = nullptr;
   bool eof_ = false;
-  std::string carry_;
+  Bytes carry_;
 };
 
 struct Fields {  // src/merge.cc:556-564 and the five fields of :573-579
@@ -201,6 +228,80 @@ void parse_hit_list(const char *s, size_t len, int32_t *out, uint64_t slots) {
   }
 }
 
+struct Batch {  // lines of both inputs
+  Bytes buf1, buf2;
+  std::vector<size_t> st1, st2;
+  size_t n = 0;
+};
+
+// Growable pinned host array (uploads from it run at full link speed).
+template <class T>
+struct Pinned {
+  T *p = nullptr;
+  size_t cap = 0;
+  void reserve(size_t n) {
+    if (n <= cap) return;
+    if (p) k2g_host_free(p);
+    p = nullptr;
+    cap = std::max(n + n / 4, (size_t) 1024);
+    if (k2g_host_alloc(cap * sizeof(T), (void **) &p)) die(71, k2g_last_error());
+  }
+  ~Pinned() {
+    if (p) k2g_host_free(p);
+  }
+};
+
+struct Slot {  // one batch on its way through parse, GPU merge and print
+  std::unique_ptr<Batch> batch;
+  std::vector<Fields> fa, fb;
+  Pinned<uint64_t> off_a, off_b;
+  Pinned<int32_t> items_a, items_b;
+  k2g_merge_results res{};
+};
+
+// Fields and hit lists of every line pair, on the OpenMP threads.
+void parse_batch(Slot &sl, int threads) {
+  const Batch &b = *sl.batch;
+  const size_t n = b.n;
+  sl.fa.resize(n);
+  sl.fb.resize(n);
+  sl.off_a.reserve(n + 1);
+  sl.off_b.reserve(n + 1);
+  uint64_t *oa = sl.off_a.p, *ob = sl.off_b.p;
+  oa[0] = ob[0] = 0;
+  bool bad = false;
+#pragma omp parallel for schedule(static) reduction(|| : bad) num_threads(threads)
+  for (size_t i = 0; i < n; i++) {
+    if (!get_fields(b.buf1.data() + b.st1[i], b.buf1.data() + b.st1[i + 1], sl.fa[i]) ||
+        !get_fields(b.buf2.data() + b.st2[i], b.buf2.data() + b.st2[i + 1], sl.fb[i])) {
+      bad = true;
+      continue;
+    }
+    oa[i + 1] = count_items(sl.fa[i].f[4], sl.fa[i].len[4]);
+    ob[i + 1] = count_items(sl.fb[i].f[4], sl.fb[i].len[4]);
+  }
+  if (bad) die(65, "malformed kraken output line (fewer than five fields)");
+  for (size_t i = 0; i < n; i++) {
+    oa[i + 1] += oa[i];
+    ob[i + 1] += ob[i];
+  }
+  sl.items_a.reserve(2 * oa[n] + 2);
+  sl.items_b.reserve(2 * ob[n] + 2);
+  int32_t *ia = sl.items_a.p, *ib = sl.items_b.p;
+#pragma omp parallel for schedule(static) num_threads(threads)
+  for (size_t i = 0; i < n; i++) {
+    parse_hit_list(sl.fa[i].f[4], sl.fa[i].len[4], ia + 2 * oa[i], oa[i + 1] - oa[i]);
+    parse_hit_list(sl.fb[i].f[4], sl.fb[i].len[4], ib + 2 * ob[i], ob[i + 1] - ob[i]);
+  }
+}
+
+// Hands a printed batch's line buffers back to the reader.
+void recycle(Slot &sl, std::mutex &mu, std::deque<std::unique_ptr<Batch>> &spare) {
+  std::lock_guard<std::mutex> lk(mu);
+  if (spare.size() < 2) spare.push_back(std::move(sl.batch));
+  sl.batch.reset();
+}
+
 double now_s() {
   struct timeval t;
   gettimeofday(&t, nullptr);
@@ -237,6 +338,7 @@ int main(int argc, char **argv) {
   if (!taxo) usage(argv[0], 2);
   if (threads < 1) threads = 1;
   const bool timing = getenv("K2G_TIMING") && atoi(getenv("K2G_TIMING"));
+  if (const char *v = getenv("K2G_MERGE_BATCH_LINES")) g_batch_lines = std::max(1ull, strtoull(v, nullptr, 10));
 
   k2host::Taxonomy tax;
   std::string err;
@@ -250,10 +352,12 @@ int main(int argc, char **argv) {
     auto it = ext_to_node.find(t);
     return it == ext_to_node.end() ? 0 : it->second;
   };
-  k2g_merger *mg = nullptr;
-  int rc = k2g_merge_open(taxo, &mg);
-  if (rc) die(rc, k2g_last_error());
-
+  // two merge slots: the GPU merges one batch while the host prints the other
+  k2g_merger *mg[2] = {nullptr, nullptr};
+  for (auto &m : mg) {
+    const int rc = k2g_merge_open(taxo, &m);
+    if (rc) die(rc, k2g_last_error());
+  }
   LineFile in1, in2;
   if (!in1.open(argv[optind])) die(66, std::string("unable to open ") + argv[optind]);
   if (!in2.open(argv[optind + 1])) die(66, std::string("unable to open ") + argv[optind + 1]);
@@ -262,64 +366,54 @@ int main(int argc, char **argv) {
   FILE *hdr = nullptr;
   if (headers && !(hdr = fopen(headers, "w"))) die(73, std::string("unable to open ") + headers);
 
-  std::string buf1, buf2;
-  std::vector<size_t> st1, st2;
-  std::vector<Fields> fa, fb;
-  std::vector<uint64_t> off_a, off_b;
-  std::vector<int32_t> items_a, items_b;
+  // reader thread: both files read side by side, two batches ahead
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<std::unique_ptr<Batch>> ready, spare;
+  bool finished = false, mismatch = false;
+  std::thread reader([&] {
+    for (;;) {
+      std::unique_ptr<Batch> b;
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        cv.wait(lk, [&] { return ready.size() < 2; });
+        if (!spare.empty()) {
+          b = std::move(spare.front());
+          spare.pop_front();
+        }
+      }
+      if (!b) b.reset(new Batch());
+      size_t n2 = 0;
+      std::thread second([&] { n2 = in2.read(g_batch_lines, b->buf2, b->st2); });
+      b->n = in1.read(g_batch_lines, b->buf1, b->st1);
+      second.join();
+      std::lock_guard<std::mutex> lk(mu);
+      if (n2 != b->n) {
+        mismatch = finished = true;
+      } else if (b->n == 0) {
+        finished = true;
+      } else {
+        ready.push_back(std::move(b));
+      }
+      cv.notify_all();
+      if (finished) return;
+    }
+  });
+
+  Slot slot[2];
   std::vector<std::string> text(threads), names(threads);
   std::vector<k2host::TaxonCounters> part_counters(threads);
   std::vector<uint64_t> part_unclassified(threads);
   uint64_t total_seqs = 0, total_unclassified = 0;
-  double t_read = 0, t_parse = 0, t_gpu = 0, t_format = 0, t_kernel = 0;
+  double t_wait = 0, t_parse = 0, t_gpu_wait = 0, t_format = 0, t_kernel = 0;
   const double t_start = now_s();
   omp_set_num_threads(threads);
-  for (;;) {
-    double t0 = now_s();
-    const size_t n = in1.read(kBatchLines, buf1, st1);
-    const size_t n2 = in2.read(n ? n : 1, buf2, st2);
-    if (n2 != n) die(65, "the two inputs hold different numbers of lines");
-    double t1 = now_s();
-    t_read += t1 - t0;
-    if (n == 0) break;
-    fa.resize(n);
-    fb.resize(n);
-    off_a.assign(n + 1, 0);
-    off_b.assign(n + 1, 0);
-    bool bad = false;
-#pragma omp parallel for schedule(static) reduction(|| : bad)
-    for (size_t i = 0; i < n; i++) {
-      if (!get_fields(buf1.data() + st1[i], buf1.data() + st1[i + 1], fa[i]) ||
-          !get_fields(buf2.data() + st2[i], buf2.data() + st2[i + 1], fb[i])) {
-        bad = true;
-        continue;
-      }
-      off_a[i + 1] = count_items(fa[i].f[4], fa[i].len[4]);
-      off_b[i + 1] = count_items(fb[i].f[4], fb[i].len[4]);
-    }
-    if (bad) die(65, "malformed kraken output line (fewer than five fields)");
-    for (size_t i = 0; i < n; i++) {
-      off_a[i + 1] += off_a[i];
-      off_b[i + 1] += off_b[i];
-    }
-    items_a.resize(2 * off_a[n]);
-    items_b.resize(2 * off_b[n]);
-#pragma omp parallel for schedule(static)
-    for (size_t i = 0; i < n; i++) {
-      parse_hit_list(fa[i].f[4], fa[i].len[4], items_a.data() + 2 * off_a[i], off_a[i + 1] - off_a[i]);
-      parse_hit_list(fb[i].f[4], fb[i].len[4], items_b.data() + 2 * off_b[i], off_b[i + 1] - off_b[i]);
-    }
-    t0 = now_s();
-    t_parse += t0 - t1;
-    k2g_merge_params p{conf};
-    k2g_hit_lists a{items_a.data(), off_a.data(), n}, b{items_b.data(), off_b.data(), n};
-    k2g_merge_results res;
-    rc = k2g_merge(mg, &p, &a, &b, &res);
-    if (rc) die(rc, k2g_last_error());
-    t1 = now_s();
-    t_gpu += t1 - t0;
-    t_kernel += res.kernel_ms / 1e3;
-    // print in per-thread contiguous ranges, then write in order
+
+  // prints one merged batch (src/merge.cc:701-758), in input order
+  auto print = [&](Slot &sl) {
+    const Batch &bt = *sl.batch;
+    const size_t n = bt.n;
+    const k2g_merge_results &res = sl.res;
 #pragma omp parallel num_threads(threads)
     {
       const int w = omp_get_thread_num(), nw = omp_get_num_threads();
@@ -330,7 +424,8 @@ int main(int argc, char **argv) {
       std::string taxid;
       uint64_t unc = 0;
       for (size_t i = lo; i < hi; i++) {
-        const bool classified = fa[i].f[0][0] == 'C' || fb[i].f[0][0] == 'C';
+        const Fields &fa = sl.fa[i], &fb = sl.fb[i];
+        const bool classified = fa.f[0][0] == 'C' || fb.f[0][0] == 'C';
         taxid.clear();
         if (classified)
           put_int(taxid, res.call[i]);
@@ -342,21 +437,20 @@ int main(int argc, char **argv) {
           const char *name = tax.name(node_of(taxid));
           taxid = std::string(name) + " (taxid " + taxid + ")";
         }
-        const uint32_t nt = res.n_tokens[i];
-        if (nt) {
+        const uint64_t t0 = res.token_off[i], t1 = res.token_off[i + 1];
+        if (t1 > t0) {
           o += classified ? "C\t" : "U\t";
-          o.append(fa[i].f[1], fa[i].len[1]);
+          o.append(fa.f[1], fa.len[1]);
           o += '\t';
           o += taxid;
           o += '\t';
-          o.append(fa[i].f[3], fa[i].len[3]);
+          o.append(fa.f[3], fa.len[3]);
           o += '\t';
-          const uint64_t s = off_a[i] + off_b[i];
-          for (uint32_t k = 0; k < nt; k++) {
-            put_int(o, res.tokens[2 * (s + k)]);
+          for (uint64_t k = t0; k < t1; k++) {
+            put_int(o, res.tokens[2 * k]);
             o += ':';
-            put_int(o, res.tokens[2 * (s + k) + 1]);
-            if (!res.joined[s + k]) o += ' ';
+            put_int(o, res.tokens[2 * k + 1]);
+            if (!res.joined[k]) o += ' ';
           }
           o += '\n';
         }
@@ -364,7 +458,7 @@ int main(int argc, char **argv) {
           if (report) part_counters[w][node_of(taxid)].n_reads++;
           if (hdr) {
             h += '>';
-            h.append(fa[i].f[1], fa[i].len[1]);
+            h.append(fa.f[1], fa.len[1]);
             h += '\n';
           }
         }
@@ -377,7 +471,53 @@ int main(int argc, char **argv) {
       total_unclassified += part_unclassified[w];
     }
     total_seqs += n;
-    t_format += now_s() - t1;
+  };
+
+  Slot *pending = nullptr;  // merged on the GPU, not yet printed
+  for (int j = 0;; j ^= 1) {
+    double t0 = now_s();
+    std::unique_ptr<Batch> b;
+    {
+      std::unique_lock<std::mutex> lk(mu);
+      cv.wait(lk, [&] { return !ready.empty() || finished; });
+      if (!ready.empty()) {
+        b = std::move(ready.front());
+        ready.pop_front();
+      }
+      cv.notify_all();
+    }
+    double t1 = now_s();
+    t_wait += t1 - t0;
+    if (!b) break;
+    Slot &sl = slot[j];
+    sl.batch = std::move(b);
+    parse_batch(sl, threads);
+    t0 = now_s();
+    t_parse += t0 - t1;
+    int rc = 0;
+    k2g_merge_params p{conf};
+    std::thread gpu([&] {
+      k2g_hit_lists a{sl.items_a.p, sl.off_a.p, sl.batch->n}, bb{sl.items_b.p, sl.off_b.p, sl.batch->n};
+      rc = k2g_merge(mg[j], &p, &a, &bb, &sl.res);
+    });
+    if (pending) {
+      print(*pending);
+      recycle(*pending, mu, spare);
+    }
+    t1 = now_s();
+    t_format += t1 - t0;
+    gpu.join();
+    t_gpu_wait += now_s() - t1;
+    if (rc) die(rc, k2g_last_error());
+    t_kernel += sl.res.kernel_ms / 1e3;
+    pending = &sl;
+  }
+  reader.join();
+  if (mismatch) die(65, "the two inputs hold different numbers of lines");
+  if (pending) {
+    const double t0 = now_s();
+    print(*pending);
+    t_format += now_s() - t0;
   }
   if (fclose(out) != 0) die(74, std::string("unable to write ") + out_name);
   if (hdr) {
@@ -392,11 +532,11 @@ int main(int argc, char **argv) {
                                      total_unclassified))
       die(74, std::string("unable to write report ") + report);
   }
-  k2g_merge_close(mg);
+  for (auto &m : mg) k2g_merge_close(m);
   if (timing)
     fprintf(stderr,
-            "%llu lines merged in %.3fs (read %.3fs, parse %.3fs, GPU %.3fs of which kernel %.3fs, "
-            "format+write %.3fs)\n",
-            (unsigned long long) total_seqs, now_s() - t_start, t_read, t_parse, t_gpu, t_kernel, t_format);
+            "%llu lines merged in %.3fs (input wait %.3fs, parse %.3fs, print+write %.3fs, GPU wait %.3fs; "
+            "kernels %.3fs)\n",
+            (unsigned long long) total_seqs, now_s() - t_start, t_wait, t_parse, t_format, t_gpu_wait, t_kernel);
   return 0;
 }

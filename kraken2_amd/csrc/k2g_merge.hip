@@ -4,6 +4,8 @@
 #include "k2g_internal.h"
 #include "k2g_merge_kernel.h"
 
+#include <hipcub/hipcub.hpp>
+
 #include <climits>
 #include <cstring>
 
@@ -22,23 +24,27 @@ hipError_t upload(DevBuf<T> &d, const T *h, size_t n, hipStream_t s) {
 
 struct k2g_merger {
   MergeTax view{};
-  DevBuf<uint32_t> parent, span, key_node;
+  DevBuf<uint32_t> parent, span, key_node, direct;
   DevBuf<int32_t> ext32;
   DevBuf<uint64_t> ext64, key;
-  DevBuf<int32_t> a_items, b_items, w_print, w_count, tokens, call;
-  DevBuf<uint64_t> a_off, b_off;
-  DevBuf<uint32_t> w_node, n_tokens;
-  DevBuf<uint8_t> joined;
-  std::vector<int32_t> h_call, h_tokens;
-  std::vector<uint32_t> h_n_tokens;
-  std::vector<uint8_t> h_joined;
+  DevBuf<int32_t> a_items, b_items, w_print, w_count, tokens, d_tokens, call;
+  DevBuf<uint64_t> a_off, b_off, n_tokens, tok_off;
+  DevBuf<uint32_t> w_node;
+  DevBuf<uint8_t> joined, d_joined, scan_tmp;
+  PinnedBuf<int32_t> h_call, h_tokens;
+  PinnedBuf<uint64_t> h_tok_off;
+  PinnedBuf<uint8_t> h_joined;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   ~k2g_merger() {
-    for (auto *b : {&parent, &span, &key_node, &w_node, &n_tokens}) b->release();
-    for (auto *b : {&ext32, &a_items, &b_items, &w_print, &w_count, &tokens, &call}) b->release();
-    for (auto *b : {&ext64, &key, &a_off, &b_off}) b->release();
-    joined.release();
+    for (auto *b : {&parent, &span, &key_node, &direct, &w_node}) b->release();
+    for (auto *b : {&ext32, &a_items, &b_items, &w_print, &w_count, &tokens, &d_tokens, &call}) b->release();
+    for (auto *b : {&ext64, &key, &a_off, &b_off, &n_tokens, &tok_off}) b->release();
+    for (auto *b : {&joined, &d_joined, &scan_tmp}) b->release();
+    h_call.release();
+    h_tokens.release();
+    h_tok_off.release();
+    h_joined.release();
     if (ev0) (void) hipEventDestroy(ev0);
     if (ev1) (void) hipEventDestroy(ev1);
     if (stream) (void) hipStreamDestroy(stream);
@@ -71,13 +77,14 @@ extern "C" int k2g_merge_open(const char *taxo_k2d, k2g_merger **out) {
   if (e == hipSuccess) e = upload(g->ext64, ext.data(), n, s);
   if (e == hipSuccess) e = upload(g->key, key.data(), key.size(), s);
   if (e == hipSuccess) e = upload(g->key_node, key_node.data(), key_node.size(), s);
+  if (e == hipSuccess && !tb.direct.empty()) e = upload(g->direct, tb.direct.data(), tb.direct.size(), s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) {
     delete g;
     return fail(K2G_ESOFTWARE, std::string("merge taxonomy upload failed: ") + hipGetErrorString(e));
   }
   g->view = MergeTax{g->parent.p, g->span.p, g->ext32.p, g->ext64.p, g->key.p, g->key_node.p,
-                     (uint32_t) key.size()};
+                     (uint32_t) key.size(), tb.direct.empty() ? nullptr : g->direct.p, tb.direct.size()};
   *out = g;
   return K2G_OK;
 }
@@ -115,37 +122,51 @@ extern "C" int k2g_merge(k2g_merger *g, const k2g_merge_params *p, const k2g_hit
   HIP_TRY(g->w_node.reserve(slots));
   HIP_TRY(g->tokens.reserve(2 * slots));
   HIP_TRY(g->joined.reserve(slots));
-  HIP_TRY(g->n_tokens.reserve(n));
+  HIP_TRY(g->n_tokens.reserve(n + 1));
+  HIP_TRY(g->tok_off.reserve(n + 1));
   HIP_TRY(g->call.reserve(n));
+  size_t scan_bytes = 0;
+  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, g->n_tokens.p, g->tok_off.p, n + 1, s));
+  HIP_TRY(g->scan_tmp.reserve(scan_bytes + 1));
   HIP_TRY(hipEventRecord(g->ev0, s));
+  const uint32_t blocks = (uint32_t) ((n + 255) / 256);
+  HIP_TRY(hipMemsetAsync(g->n_tokens.p + n, 0, 8, s));
   if (n) {
-    const uint64_t blocks = (n + 255) / 256;
-    hipLaunchKernelGGL(k_merge, dim3((uint32_t) blocks), dim3(256), 0, s, g->view, g->a_items.p, g->a_off.p,
-                       g->b_items.p, g->b_off.p, n, p->confidence, g->w_print.p, g->w_count.p, g->w_node.p,
-                       g->tokens.p, g->joined.p, g->n_tokens.p, g->call.p);
+    hipLaunchKernelGGL(k_merge, dim3(blocks), dim3(256), 0, s, g->view, g->a_items.p, g->a_off.p, g->b_items.p,
+                       g->b_off.p, n, p->confidence, g->w_print.p, g->w_count.p, g->w_node.p, g->tokens.p,
+                       g->joined.p, g->n_tokens.p, g->call.p);
+    HIP_TRY(hipGetLastError());
+  }
+  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(g->scan_tmp.p, scan_bytes, g->n_tokens.p, g->tok_off.p, n + 1, s));
+  uint64_t total = 0;
+  HIP_TRY(hipMemcpyAsync(&total, g->tok_off.p + n, 8, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  HIP_TRY(g->d_tokens.reserve(2 * total));
+  HIP_TRY(g->d_joined.reserve(total));
+  if (n) {
+    hipLaunchKernelGGL(k_merge_compact, dim3(blocks), dim3(256), 0, s, g->a_off.p, g->b_off.p, n, g->tok_off.p,
+                       g->tokens.p, g->joined.p, g->d_tokens.p, g->d_joined.p);
     HIP_TRY(hipGetLastError());
   }
   HIP_TRY(hipEventRecord(g->ev1, s));
-  g->h_call.resize(n);
-  g->h_n_tokens.resize(n);
-  g->h_tokens.resize(2 * slots);
-  g->h_joined.resize(slots);
-  if (n) {
-    HIP_TRY(hipMemcpyAsync(g->h_call.data(), g->call.p, n * 4, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipMemcpyAsync(g->h_n_tokens.data(), g->n_tokens.p, n * 4, hipMemcpyDeviceToHost, s));
-  }
-  if (slots) {
-    HIP_TRY(hipMemcpyAsync(g->h_tokens.data(), g->tokens.p, 2 * slots * 4, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipMemcpyAsync(g->h_joined.data(), g->joined.p, slots, hipMemcpyDeviceToHost, s));
+  HIP_TRY(g->h_call.reserve(n + 1));
+  HIP_TRY(g->h_tok_off.reserve(n + 1));
+  HIP_TRY(g->h_tokens.reserve(2 * total + 2));
+  HIP_TRY(g->h_joined.reserve(total + 1));
+  if (n) HIP_TRY(hipMemcpyAsync(g->h_call.p, g->call.p, n * 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(g->h_tok_off.p, g->tok_off.p, (n + 1) * 8, hipMemcpyDeviceToHost, s));
+  if (total) {
+    HIP_TRY(hipMemcpyAsync(g->h_tokens.p, g->d_tokens.p, 2 * total * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(g->h_joined.p, g->d_joined.p, total, hipMemcpyDeviceToHost, s));
   }
   HIP_TRY(hipStreamSynchronize(s));
   float ms = 0;
   HIP_TRY(hipEventElapsedTime(&ms, g->ev0, g->ev1));
   out->n = n;
-  out->call = g->h_call.data();
-  out->n_tokens = g->h_n_tokens.data();
-  out->tokens = g->h_tokens.data();
-  out->joined = g->h_joined.data();
+  out->call = g->h_call.p;
+  out->token_off = g->h_tok_off.p;
+  out->tokens = g->h_tokens.p;
+  out->joined = g->h_joined.p;
   out->kernel_ms = ms;
   return K2G_OK;
 }

@@ -37,12 +37,15 @@ struct MergeTax {
   const uint64_t *key;      // sorted external ids of the external-to-node map
   const uint32_t *key_node; // node of each key
   uint32_t n_keys;
+  const uint32_t *direct;   // node of every external id below n_direct (0 if none), or null
+  uint64_t n_direct;
 };
 
 // Taxonomy::GetInternalID (src/taxonomy.h:81-85) of the int taxid widened
 // as the reference widens it: 0 when unknown.
 __device__ __forceinline__ uint32_t node_of(const MergeTax &t, int32_t v) {
   const uint64_t k = (uint64_t) (int64_t) v;
+  if (t.direct) return k < t.n_direct ? t.direct[k] : 0;
   uint32_t lo = 0, hi = t.n_keys;
   while (lo < hi) {
     const uint32_t mid = (lo + hi) >> 1;
@@ -77,7 +80,7 @@ k_merge(MergeTax t, const int32_t *__restrict__ a_items, const uint64_t *__restr
         const int32_t *__restrict__ b_items, const uint64_t *__restrict__ b_off, uint64_t n,
         float conf, int32_t *__restrict__ w_print, int32_t *__restrict__ w_count,
         uint32_t *__restrict__ w_node, int32_t *__restrict__ tokens, uint8_t *__restrict__ joined,
-        uint32_t *__restrict__ n_tokens, int32_t *__restrict__ call) {
+        uint64_t *__restrict__ n_tokens, int32_t *__restrict__ call) {
   const uint64_t r = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
   const uint64_t a0 = a_off[r], a1 = a_off[r + 1], b0 = b_off[r], b1 = b_off[r + 1];
@@ -222,6 +225,23 @@ k_merge(MergeTax t, const int32_t *__restrict__ a_items, const uint64_t *__restr
   n_tokens[r] = nt;
 }
 
+// Copies read r's printed pairs from its slots to [tok_off[r], tok_off[r+1])
+// of the dense arrays (tok_off: exclusive sum of n_tokens).
+__global__ void __launch_bounds__(256)
+k_merge_compact(const uint64_t *__restrict__ a_off, const uint64_t *__restrict__ b_off, uint64_t n,
+                const uint64_t *__restrict__ tok_off, const int32_t *__restrict__ tokens,
+                const uint8_t *__restrict__ joined, int32_t *__restrict__ d_tokens, uint8_t *__restrict__ d_joined) {
+  const uint64_t r = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const uint64_t s = a_off[r] + b_off[r], d = tok_off[r], nt = tok_off[r + 1] - d;
+  const uint64_t *src = (const uint64_t *) (tokens + 2 * s);  // a (taxid, count) pair
+  uint64_t *dst = (uint64_t *) (d_tokens + 2 * d);
+  for (uint64_t k = 0; k < nt; k++) {
+    dst[k] = src[k];
+    d_joined[d + k] = joined[s + k];
+  }
+}
+
 // Parent and external ids of taxo.k2d's nodes (TaxonomyNode,
 // src/taxonomy.h:18-26); 0 or a K2G error code with err set.
 inline int read_taxonomy_nodes(const char *path, std::vector<uint32_t> &parent, std::vector<uint64_t> &ext,
@@ -262,14 +282,18 @@ inline int read_taxonomy_nodes(const char *path, std::vector<uint32_t> &parent, 
   return K2G_OK;
 }
 
-// Host tables behind MergeTax.
+// Host tables behind MergeTax.  External ids below 2^26 (NCBI taxids are)
+// also get a direct-mapped table, one load per lookup instead of a search.
 struct MergeTables {
-  std::vector<uint32_t> span, key_node;
+  std::vector<uint32_t> span, key_node, direct;
   std::vector<int32_t> ext32;
   std::vector<uint64_t> key;
 };
 
-inline void merge_tables(const std::vector<uint32_t> &parent, const std::vector<uint64_t> &ext, MergeTables &t) {
+constexpr uint64_t kMergeDirectLimit = 1ull << 26;
+
+inline void merge_tables(const std::vector<uint32_t> &parent, const std::vector<uint64_t> &ext, MergeTables &t,
+                         uint64_t direct_limit = kMergeDirectLimit) {
   const size_t n = parent.size();
   t.span.resize(2 * n);
   taxonomy_spans(parent.data(), n, t.span.data());
@@ -285,6 +309,11 @@ inline void merge_tables(const std::vector<uint32_t> &parent, const std::vector<
   for (const auto &kv : m) {
     t.key.push_back(kv.first);
     t.key_node.push_back(kv.second);
+  }
+  t.direct.clear();
+  if (t.key.back() < direct_limit) {
+    t.direct.assign(t.key.back() + 1, 0);
+    for (const auto &kv : m) t.direct[kv.first] = kv.second;
   }
 }
 

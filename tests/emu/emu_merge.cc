@@ -4,6 +4,7 @@
 // the tests.  The kernel runs one host thread per read, in read order.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -28,14 +29,25 @@ struct k2g_merger {
   std::vector<uint64_t> ext;
   k2g::MergeTables tb;
   k2g::MergeTax view{};
-  std::vector<int32_t> w_print, w_count, tokens, call;
-  std::vector<uint32_t> w_node, n_tokens;
-  std::vector<uint8_t> joined;
+  std::vector<int32_t> w_print, w_count, tokens, call, d_tokens;
+  std::vector<uint32_t> w_node;
+  std::vector<uint64_t> n_tokens, tok_off;
+  std::vector<uint8_t> joined, d_joined;
 };
 
 extern "C" {
 
 const char *k2g_last_error(void) { return g_error.c_str(); }
+
+int k2g_host_alloc(size_t bytes, void **out) {
+  *out = malloc(bytes ? bytes : 1);
+  return *out ? K2G_OK : K2G_EOS;
+}
+
+int k2g_host_free(void *p) {
+  free(p);
+  return K2G_OK;
+}
 
 int k2g_merge_open(const char *taxo_k2d, k2g_merger **out) {
   auto *g = new k2g_merger();
@@ -44,9 +56,13 @@ int k2g_merge_open(const char *taxo_k2d, k2g_merger **out) {
     delete g;
     return rc;
   }
-  k2g::merge_tables(g->parent, g->ext, g->tb);
-  g->view = k2g::MergeTax{g->parent.data(), g->tb.span.data(), g->tb.ext32.data(), g->ext.data(),
-                          g->tb.key.data(),   g->tb.key_node.data(), (uint32_t) g->tb.key.size()};
+  // K2G_EMU_MERGE_SEARCH=1: no direct-mapped table, every lookup searches
+  const char *search = getenv("K2G_EMU_MERGE_SEARCH");
+  k2g::merge_tables(g->parent, g->ext, g->tb, search && atoi(search) ? 0 : k2g::kMergeDirectLimit);
+  g->view = k2g::MergeTax{g->parent.data(),     g->tb.span.data(),     g->tb.ext32.data(),
+                          g->ext.data(),        g->tb.key.data(),      g->tb.key_node.data(),
+                          (uint32_t) g->tb.key.size(),
+                          g->tb.direct.empty() ? nullptr : g->tb.direct.data(), g->tb.direct.size()};
   *out = g;
   return K2G_OK;
 }
@@ -74,11 +90,20 @@ int k2g_merge(k2g_merger *g, const k2g_merge_params *p, const k2g_hit_lists *a, 
                  g->w_count.data(), g->w_node.data(), g->tokens.data(), g->joined.data(), g->n_tokens.data(),
                  g->call.data());
   }
+  g->tok_off.assign(n + 1, 0);
+  for (uint64_t r = 0; r < n; r++) g->tok_off[r + 1] = g->tok_off[r] + g->n_tokens[r];
+  g->d_tokens.assign(2 * g->tok_off[n], 0);
+  g->d_joined.assign(g->tok_off[n], 0);
+  for (uint64_t r = 0; r < n; r++) {
+    emu::blockIdx_.x = (unsigned) r;
+    k2g::k_merge_compact(a->off, b->off, n, g->tok_off.data(), g->tokens.data(), g->joined.data(),
+                         g->d_tokens.data(), g->d_joined.data());
+  }
   out->n = n;
   out->call = g->call.data();
-  out->n_tokens = g->n_tokens.data();
-  out->tokens = g->tokens.data();
-  out->joined = g->joined.data();
+  out->token_off = g->tok_off.data();
+  out->tokens = g->d_tokens.data();
+  out->joined = g->d_joined.data();
   out->kernel_ms = 0;
   return K2G_OK;
 }

@@ -34,12 +34,13 @@ EXPECTED = json.load(open(os.path.join(GOLD, "merge", "expected.json")))
 TOY_TAXO = os.path.join(GOLD, "toy", "taxo.k2d")
 
 
-def run_merge(exe, taxo, in1, in2, opts, d, threads=4):
+def run_merge(exe, taxo, in1, in2, opts, d, threads=4, batch=0):
     out, hdr, rep = (os.path.join(d, x) for x in ("out.kraken", "headers.txt", "report.txt"))
-    print(f"merge: {os.path.basename(exe)} {' '.join(opts)} {os.path.basename(in1)} {os.path.basename(in2)}",
-          flush=True)
+    print(f"merge: {os.path.basename(exe)} {' '.join(opts)} {os.path.basename(in1)} {os.path.basename(in2)}"
+          f" batch {batch}", flush=True)
+    env = dict(os.environ, K2G_MERGE_BATCH_LINES=str(batch)) if batch else None
     r = subprocess.run([exe, "-T", str(threads), "-t", taxo, "-o", out, "-c", hdr, "-r", rep, *opts, in1, in2],
-                       capture_output=True, text=True, timeout=300)
+                       capture_output=True, text=True, timeout=300, env=env)
     assert r.returncode == 0, r.stderr
     return {k: open(p, "rb").read() for k, p in (("output", out), ("headers", hdr), ("report", rep))}
 
@@ -54,15 +55,15 @@ def test_gpu_merge_matches_reference_golden(case, tmp_path):
         assert hashlib.sha256(got[key]).hexdigest() == exp["sha256"], (case, key)
 
 
-@pytest.mark.parametrize("seed,opts", [(11, []), (12, ["-i", "0.2", "-n", "-z"]), (13, ["-i", "0.6"])])
-def test_gpu_merge_random_vs_oracle(seed, opts, tmp_path):
+@pytest.mark.parametrize("seed,opts,batch", [(11, [], 0), (12, ["-i", "0.2", "-n", "-z"], 3000), (13, ["-i", "0.6"], 7)])
+def test_gpu_merge_random_vs_oracle(seed, opts, batch, tmp_path):
     a, b = random_kraken_pairs(taxonomy_external_ids(TOY_TAXO), 40000, seed)
     pa, pb = tmp_path / "a.kraken", tmp_path / "b.kraken"
     pa.write_text(a)
     pb.write_text(b)
     (tmp_path / "gpu").mkdir()
     (tmp_path / "cpu").mkdir()
-    got = run_merge(EXE, TOY_TAXO, str(pa), str(pb), opts, str(tmp_path / "gpu"))
+    got = run_merge(EXE, TOY_TAXO, str(pa), str(pb), opts, str(tmp_path / "gpu"), batch=batch)
     exp = run_merge(ORACLE, TOY_TAXO, str(pa), str(pb), opts, str(tmp_path / "cpu"))
     for key in ("output", "headers", "report"):
         assert got[key] == exp[key], key
@@ -93,7 +94,7 @@ def test_merge_entry_points_empty_and_errors():
         # a read with no items in the first list prints nothing
         out = m.merge(np.zeros(0, np.int32), np.zeros(2, np.uint64), np.array([211044, 5], np.int32),
                       np.array([0, 1], np.uint64))
-        assert out.n_tokens.tolist() == [0] and out.call.tolist() == [0]
+        assert out.token_off.tolist() == [0, 0] and out.call.tolist() == [0]
         with pytest.raises(k2.K2GError):  # different read counts
             m.merge(np.zeros(0, np.int32), np.zeros(2, np.uint64), np.zeros(0, np.int32), z)
         with pytest.raises(k2.K2GError):  # offsets not starting at 0

@@ -36,10 +36,10 @@ def emu_merge():
     return os.path.join(HERE, "emu", "k2g_merge_emu")
 
 
-def run_merge(exe, taxo, in1, in2, opts, d, threads=3):
+def run_merge(exe, taxo, in1, in2, opts, d, threads=3, env=None):
     out, hdr, rep = (os.path.join(d, x) for x in ("out.kraken", "headers.txt", "report.txt"))
     r = subprocess.run([exe, "-T", str(threads), "-t", taxo, "-o", out, "-c", hdr, "-r", rep, *opts, in1, in2],
-                       capture_output=True, text=True)
+                       capture_output=True, text=True, env=env)
     assert r.returncode == 0, r.stderr
     return {k: open(p, "rb").read() for k, p in (("output", out), ("headers", hdr), ("report", rep))}
 
@@ -61,25 +61,33 @@ def test_oracle_merge_matches_reference_golden(case, tmp_path):
     check_digests(run_merge(ORACLE, *golden_case(case), str(tmp_path)), case)
 
 
+@pytest.mark.parametrize("search", [False, True])
 @pytest.mark.parametrize("case", sorted(MERGE_CASES))
-def test_emulated_merge_front_end_matches_reference_golden(emu_merge, case, tmp_path):
-    check_digests(run_merge(emu_merge, *golden_case(case), str(tmp_path), threads=4), case)
+def test_emulated_merge_front_end_matches_reference_golden(emu_merge, case, search, tmp_path):
+    """search: taxids looked up by binary search over the sorted map (the
+    library's path for external ids of 2^26 and above) instead of the
+    direct-mapped table."""
+    env = dict(os.environ, K2G_EMU_MERGE_SEARCH="1" if search else "0")
+    check_digests(run_merge(emu_merge, *golden_case(case), str(tmp_path), threads=4, env=env), case)
 
 
-@pytest.mark.parametrize("seed,opts", [(1, []), (2, ["-i", "0.15"]), (3, ["-i", "0.5", "-n", "-z"]),
-                                       (4, ["-i", "1.0"])])
-def test_random_hit_lists_emulated_vs_oracle(emu_merge, seed, opts, tmp_path):
+@pytest.mark.parametrize("seed,opts,batch", [(1, [], 0), (2, ["-i", "0.15"], 7), (3, ["-i", "0.5", "-n", "-z"], 100),
+                                             (4, ["-i", "1.0"], 1)])
+def test_random_hit_lists_emulated_vs_oracle(emu_merge, seed, opts, batch, tmp_path):
+    """batch: lines per front-end batch (0 = default), so the pipeline of
+    reader, two merge slots and printing runs over many batches."""
     taxo = os.path.join(GOLD, "toy", "taxo.k2d")
     a, b = random_kraken_pairs(taxonomy_external_ids(taxo), 1500, seed)
     pa, pb = tmp_path / "a.kraken", tmp_path / "b.kraken"
     pa.write_text(a)
     pb.write_text(b)
     runs = {}
+    env = dict(os.environ, K2G_MERGE_BATCH_LINES=str(batch)) if batch else None
     for name, exe in (("oracle", ORACLE), ("emulated", emu_merge)) + ((("reference", REF),)
                                                                       if os.path.exists(REF) else ()):
         d = tmp_path / name
         d.mkdir()
-        runs[name] = run_merge(exe, taxo, str(pa), str(pb), opts, str(d))
+        runs[name] = run_merge(exe, taxo, str(pa), str(pb), opts, str(d), env=env)
     for name, got in runs.items():
         for key in ("output", "headers", "report"):
             assert got[key] == runs["oracle"][key], (name, key)
