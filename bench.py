@@ -551,7 +551,7 @@ def run_merge_bench(a):
             gpu_runs.append((secs, err))
     secs_gpu = min(s for s, _ in gpu_runs)
     timing = [e for s, e in gpu_runs if s == secs_gpu][0].strip().splitlines()[-1]
-    m = re.search(r"kernel ([\d.]+)s", timing)
+    m = re.search(r"kernels ([\d.]+)s", timing)
     kernel_s = float(m.group(1)) if m else None
     log(f"# k2g_merge: {secs_gpu:.3f}s ({timing})")
     secs_cpu, _, exp = merge(cpu_exe, "cpu")

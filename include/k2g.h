@@ -279,6 +279,7 @@ typedef struct {
 
 typedef struct {
   float confidence; /* merge -i */
+  int32_t text;     /* 1: results as printed text (text, text_off); 0: as pairs (token_off, tokens, joined) */
 } k2g_merge_params;
 
 typedef struct {
@@ -293,6 +294,10 @@ typedef struct {
   const uint64_t *token_off;
   const int32_t *tokens; /* (taxid, count) pairs */
   const uint8_t *joined; /* per pair: 1 = no space follows the pair */
+  /* text mode: read i's hit list as write_hit_list prints it (no newline) is
+   * text[text_off[i] .. text_off[i+1]); the pair arrays are then NULL */
+  const char *text;
+  const uint64_t *text_off;
   double kernel_ms;
 } k2g_merge_results;
 

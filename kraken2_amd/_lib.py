@@ -146,7 +146,7 @@ class HitLists(C.Structure):
 
 
 class MergeParams(C.Structure):
-    _fields_ = [("confidence", C.c_float)]
+    _fields_ = [("confidence", C.c_float), ("text", C.c_int32)]
 
 
 class MergeResults(C.Structure):
@@ -156,6 +156,8 @@ class MergeResults(C.Structure):
         ("token_off", C.POINTER(C.c_uint64)),
         ("tokens", C.POINTER(C.c_int32)),
         ("joined", C.POINTER(C.c_uint8)),
+        ("text", C.POINTER(C.c_char)),
+        ("text_off", C.POINTER(C.c_uint64)),
         ("kernel_ms", C.c_double),
     ]
 
@@ -600,6 +602,7 @@ class MergeOutput:
     tokens: np.ndarray     # (m, 2) int32: printed (taxid, count) pairs
     joined: np.ndarray     # uint8 per pair: 1 = no space after the pair
     kernel_ms: float
+    text: list | None = None  # with text=True: each read's hit list as printed (bytes)
 
 
 class Merger:
@@ -611,7 +614,7 @@ class Merger:
         self._h = C.c_void_p()
         _check(self._lib.k2g_merge_open(taxo_k2d.encode(), C.byref(self._h)))
 
-    def merge(self, items_a, off_a, items_b, off_b, confidence: float = 0.0) -> MergeOutput:
+    def merge(self, items_a, off_a, items_b, off_b, confidence: float = 0.0, text: bool = False) -> MergeOutput:
         ia = np.ascontiguousarray(items_a, np.int32).reshape(-1)
         ib = np.ascontiguousarray(items_b, np.int32).reshape(-1)
         oa = np.ascontiguousarray(off_a, np.uint64)
@@ -619,13 +622,18 @@ class Merger:
         a = HitLists(ia.ctypes.data, oa.ctypes.data, len(oa) - 1)
         b = HitLists(ib.ctypes.data, ob.ctypes.data, len(ob) - 1)
         r = MergeResults()
-        _check(self._lib.k2g_merge(self._h, C.byref(MergeParams(confidence)), C.byref(a), C.byref(b),
+        _check(self._lib.k2g_merge(self._h, C.byref(MergeParams(confidence, int(text))), C.byref(a), C.byref(b),
                                    C.byref(r)))
         n = int(r.n)
 
         def arr(ptr, count, dtype):
             return np.ctypeslib.as_array(ptr, (count,)).astype(dtype) if count else np.zeros(0, dtype)
 
+        if text:
+            off = arr(r.text_off, n + 1, np.uint64)
+            raw = C.string_at(r.text, int(off[-1])) if int(off[-1]) else b""
+            lines = [raw[int(off[i]):int(off[i + 1])] for i in range(n)]
+            return MergeOutput(arr(r.call, n, np.int32), None, None, None, float(r.kernel_ms), lines)
         token_off = arr(r.token_off, n + 1, np.uint64)
         m = int(token_off[-1])
         return MergeOutput(arr(r.call, n, np.int32), token_off, arr(r.tokens, 2 * m, np.int32).reshape(-1, 2),

@@ -191,22 +191,22 @@ uint64_t count_items(const char *s, size_t len) {
 // the tokens do not reach stay {0,0}); the last count loses the field's
 // last character (the line's newline).
 void parse_hit_list(const char *s, size_t len, int32_t *out, uint64_t slots) {
-  std::memset(out, 0, slots * 8);
   const char *p = s, *end = s + len;
   uint64_t i = 0;
   while (p) {
-    const char *colon = (const char *) memchr(p, ':', end - p);
-    if (!colon) break;
+    const char *colon = p;
+    while (colon < end && *colon != ':') colon++;
+    if (colon == end) break;
     const char *t = p;
     const size_t len1 = colon - t;
-    const char *c = colon + 1;
-    const char *sp = (const char *) memchr(c, ' ', end - c);
+    const char *c = colon + 1, *sp = c;
+    while (sp < end && *sp != ' ') sp++;
     size_t len2;
-    if (sp) {
+    if (sp < end) {
       len2 = sp - c;
       p = sp + 1;
     } else {
-      len2 = (size_t) (end - c) - 1;
+      len2 = (size_t) (end - c) - 1;  // drops the line's last character
       p = nullptr;
     }
     const char first = len1 ? t[0] : '\0';
@@ -226,6 +226,7 @@ void parse_hit_list(const char *s, size_t len, int32_t *out, uint64_t slots) {
     out[2 * i + 1] = count;
     i++;
   }
+  for (; i < slots; i++) out[2 * i] = out[2 * i + 1] = 0;
 }
 
 struct Batch {  // lines of both inputs
@@ -401,7 +402,8 @@ int main(int argc, char **argv) {
   });
 
   Slot slot[2];
-  std::vector<std::string> text(threads), names(threads);
+  std::vector<Bytes> text(threads);
+  std::vector<std::string> names(threads);
   std::vector<k2host::TaxonCounters> part_counters(threads);
   std::vector<uint64_t> part_unclassified(threads);
   uint64_t total_seqs = 0, total_unclassified = 0;
@@ -418,8 +420,9 @@ int main(int argc, char **argv) {
     {
       const int w = omp_get_thread_num(), nw = omp_get_num_threads();
       const size_t lo = n * w / nw, hi = n * (w + 1) / nw;
-      std::string &o = text[w], &h = names[w];
-      o.clear();
+      Bytes &o = text[w];
+      std::string &h = names[w];
+      o.size = 0;
       h.clear();
       std::string taxid;
       uint64_t unc = 0;
@@ -437,22 +440,25 @@ int main(int argc, char **argv) {
           const char *name = tax.name(node_of(taxid));
           taxid = std::string(name) + " (taxid " + taxid + ")";
         }
-        const uint64_t t0 = res.token_off[i], t1 = res.token_off[i + 1];
+        const uint64_t t0 = res.text_off[i], t1 = res.text_off[i + 1];
         if (t1 > t0) {
-          o += classified ? "C\t" : "U\t";
-          o.append(fa.f[1], fa.len[1]);
-          o += '\t';
-          o += taxid;
-          o += '\t';
-          o.append(fa.f[3], fa.len[3]);
-          o += '\t';
-          for (uint64_t k = t0; k < t1; k++) {
-            put_int(o, res.tokens[2 * k]);
-            o += ':';
-            put_int(o, res.tokens[2 * k + 1]);
-            if (!res.joined[k]) o += ' ';
-          }
-          o += '\n';
+          o.reserve(o.size + 8 + fa.len[1] + taxid.size() + fa.len[3] + (t1 - t0));
+          char *q = o.data() + o.size;
+          *q++ = classified ? 'C' : 'U';
+          *q++ = '\t';
+          memcpy(q, fa.f[1], fa.len[1]);
+          q += fa.len[1];
+          *q++ = '\t';
+          memcpy(q, taxid.data(), taxid.size());
+          q += taxid.size();
+          *q++ = '\t';
+          memcpy(q, fa.f[3], fa.len[3]);
+          q += fa.len[3];
+          *q++ = '\t';
+          memcpy(q, res.text + t0, t1 - t0);  // the hit list, printed on the GPU
+          q += t1 - t0;
+          *q++ = '\n';
+          o.size = q - o.data();
         }
         if (classified) {
           if (report) part_counters[w][node_of(taxid)].n_reads++;
@@ -466,7 +472,7 @@ int main(int argc, char **argv) {
       part_unclassified[w] = unc;
     }
     for (int w = 0; w < threads; w++) {
-      fwrite(text[w].data(), 1, text[w].size(), out);
+      fwrite(text[w].data(), 1, text[w].size, out);
       if (hdr) fwrite(names[w].data(), 1, names[w].size(), hdr);
       total_unclassified += part_unclassified[w];
     }
@@ -495,7 +501,7 @@ int main(int argc, char **argv) {
     t0 = now_s();
     t_parse += t0 - t1;
     int rc = 0;
-    k2g_merge_params p{conf};
+    k2g_merge_params p{conf, 1};
     std::thread gpu([&] {
       k2g_hit_lists a{sl.items_a.p, sl.off_a.p, sl.batch->n}, bb{sl.items_b.p, sl.off_b.p, sl.batch->n};
       rc = k2g_merge(mg[j], &p, &a, &bb, &sl.res);

@@ -28,19 +28,23 @@ struct k2g_merger {
   DevBuf<int32_t> ext32;
   DevBuf<uint64_t> ext64, key;
   DevBuf<int32_t> a_items, b_items, w_print, w_count, tokens, d_tokens, call;
-  DevBuf<uint64_t> a_off, b_off, n_tokens, tok_off;
+  DevBuf<uint64_t> a_off, b_off, n_tokens, tok_off, text_len;
   DevBuf<uint32_t> w_node;
   DevBuf<uint8_t> joined, d_joined, scan_tmp;
+  DevBuf<char> text;
   PinnedBuf<int32_t> h_call, h_tokens;
   PinnedBuf<uint64_t> h_tok_off;
   PinnedBuf<uint8_t> h_joined;
+  PinnedBuf<char> h_text;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   ~k2g_merger() {
     for (auto *b : {&parent, &span, &key_node, &direct, &w_node}) b->release();
     for (auto *b : {&ext32, &a_items, &b_items, &w_print, &w_count, &tokens, &d_tokens, &call}) b->release();
-    for (auto *b : {&ext64, &key, &a_off, &b_off, &n_tokens, &tok_off}) b->release();
+    for (auto *b : {&ext64, &key, &a_off, &b_off, &n_tokens, &tok_off, &text_len}) b->release();
     for (auto *b : {&joined, &d_joined, &scan_tmp}) b->release();
+    text.release();
+    h_text.release();
     h_call.release();
     h_tokens.release();
     h_tok_off.release();
@@ -123,50 +127,69 @@ extern "C" int k2g_merge(k2g_merger *g, const k2g_merge_params *p, const k2g_hit
   HIP_TRY(g->tokens.reserve(2 * slots));
   HIP_TRY(g->joined.reserve(slots));
   HIP_TRY(g->n_tokens.reserve(n + 1));
+  HIP_TRY(g->text_len.reserve(n + 1));
   HIP_TRY(g->tok_off.reserve(n + 1));
   HIP_TRY(g->call.reserve(n));
+  const bool text = p->text != 0;
+  uint64_t *counts = text ? g->text_len.p : g->n_tokens.p;  // what the scan lays out
   size_t scan_bytes = 0;
-  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, g->n_tokens.p, g->tok_off.p, n + 1, s));
+  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, counts, g->tok_off.p, n + 1, s));
   HIP_TRY(g->scan_tmp.reserve(scan_bytes + 1));
   HIP_TRY(hipEventRecord(g->ev0, s));
   const uint32_t blocks = (uint32_t) ((n + 255) / 256);
-  HIP_TRY(hipMemsetAsync(g->n_tokens.p + n, 0, 8, s));
+  HIP_TRY(hipMemsetAsync(counts + n, 0, 8, s));
   if (n) {
     hipLaunchKernelGGL(k_merge, dim3(blocks), dim3(256), 0, s, g->view, g->a_items.p, g->a_off.p, g->b_items.p,
                        g->b_off.p, n, p->confidence, g->w_print.p, g->w_count.p, g->w_node.p, g->tokens.p,
-                       g->joined.p, g->n_tokens.p, g->call.p);
+                       g->joined.p, g->n_tokens.p, g->text_len.p, g->call.p);
     HIP_TRY(hipGetLastError());
   }
-  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(g->scan_tmp.p, scan_bytes, g->n_tokens.p, g->tok_off.p, n + 1, s));
+  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(g->scan_tmp.p, scan_bytes, counts, g->tok_off.p, n + 1, s));
   uint64_t total = 0;
   HIP_TRY(hipMemcpyAsync(&total, g->tok_off.p + n, 8, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
-  HIP_TRY(g->d_tokens.reserve(2 * total));
-  HIP_TRY(g->d_joined.reserve(total));
-  if (n) {
-    hipLaunchKernelGGL(k_merge_compact, dim3(blocks), dim3(256), 0, s, g->a_off.p, g->b_off.p, n, g->tok_off.p,
-                       g->tokens.p, g->joined.p, g->d_tokens.p, g->d_joined.p);
-    HIP_TRY(hipGetLastError());
+  if (text) {
+    HIP_TRY(g->text.reserve(total + 1));
+    if (n) {
+      hipLaunchKernelGGL(k_merge_print, dim3(blocks), dim3(256), 0, s, g->a_off.p, g->b_off.p, n, g->n_tokens.p,
+                         g->tok_off.p, g->tokens.p, g->joined.p, g->text.p);
+      HIP_TRY(hipGetLastError());
+    }
+  } else {
+    HIP_TRY(g->d_tokens.reserve(2 * total));
+    HIP_TRY(g->d_joined.reserve(total));
+    if (n) {
+      hipLaunchKernelGGL(k_merge_compact, dim3(blocks), dim3(256), 0, s, g->a_off.p, g->b_off.p, n, g->tok_off.p,
+                         g->tokens.p, g->joined.p, g->d_tokens.p, g->d_joined.p);
+      HIP_TRY(hipGetLastError());
+    }
   }
   HIP_TRY(hipEventRecord(g->ev1, s));
   HIP_TRY(g->h_call.reserve(n + 1));
   HIP_TRY(g->h_tok_off.reserve(n + 1));
-  HIP_TRY(g->h_tokens.reserve(2 * total + 2));
-  HIP_TRY(g->h_joined.reserve(total + 1));
   if (n) HIP_TRY(hipMemcpyAsync(g->h_call.p, g->call.p, n * 4, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipMemcpyAsync(g->h_tok_off.p, g->tok_off.p, (n + 1) * 8, hipMemcpyDeviceToHost, s));
-  if (total) {
-    HIP_TRY(hipMemcpyAsync(g->h_tokens.p, g->d_tokens.p, 2 * total * 4, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipMemcpyAsync(g->h_joined.p, g->d_joined.p, total, hipMemcpyDeviceToHost, s));
+  if (text) {
+    HIP_TRY(g->h_text.reserve(total + 1));
+    if (total) HIP_TRY(hipMemcpyAsync(g->h_text.p, g->text.p, total, hipMemcpyDeviceToHost, s));
+  } else {
+    HIP_TRY(g->h_tokens.reserve(2 * total + 2));
+    HIP_TRY(g->h_joined.reserve(total + 1));
+    if (total) {
+      HIP_TRY(hipMemcpyAsync(g->h_tokens.p, g->d_tokens.p, 2 * total * 4, hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipMemcpyAsync(g->h_joined.p, g->d_joined.p, total, hipMemcpyDeviceToHost, s));
+    }
   }
   HIP_TRY(hipStreamSynchronize(s));
   float ms = 0;
   HIP_TRY(hipEventElapsedTime(&ms, g->ev0, g->ev1));
   out->n = n;
   out->call = g->h_call.p;
-  out->token_off = g->h_tok_off.p;
-  out->tokens = g->h_tokens.p;
-  out->joined = g->h_joined.p;
+  out->token_off = text ? nullptr : g->h_tok_off.p;
+  out->tokens = text ? nullptr : g->h_tokens.p;
+  out->joined = text ? nullptr : g->h_joined.p;
+  out->text = text ? g->h_text.p : nullptr;
+  out->text_off = text ? g->h_tok_off.p : nullptr;
   out->kernel_ms = ms;
   return K2G_OK;
 }

@@ -75,12 +75,37 @@ __device__ __forceinline__ int32_t wrap_add(int32_t a, int32_t b) {
   return (int32_t) ((uint32_t) a + (uint32_t) b);
 }
 
+// Characters int_to_string (src/merge.cc:251-326) prints for an int:
+// decimal digits, or one marker character ("A", "|", "-"; '?' for negative
+// values, which classify never writes and the reference prints from a stale
+// buffer).
+__device__ __forceinline__ uint32_t printed_len(int32_t v) {
+  if (v > kValid || v < 0) return 1;
+  uint32_t n = 1;
+  for (uint32_t x = (uint32_t) v; x >= 10; x /= 10) n++;
+  return n;
+}
+
+__device__ __forceinline__ char *print_int(char *o, int32_t v) {
+  if (v > kValid || v < 0) {
+    *o = v == INT32_MAX ? 'A' : v == INT32_MAX - 1 ? '|' : v == INT32_MAX - 2 ? '-' : '?';
+    return o + 1;
+  }
+  const uint32_t n = printed_len(v);
+  uint32_t x = (uint32_t) v;
+  for (uint32_t i = n; i-- > 0;) {
+    o[i] = (char) ('0' + x % 10);
+    x /= 10;
+  }
+  return o + n;
+}
+
 __global__ void __launch_bounds__(256)
 k_merge(MergeTax t, const int32_t *__restrict__ a_items, const uint64_t *__restrict__ a_off,
         const int32_t *__restrict__ b_items, const uint64_t *__restrict__ b_off, uint64_t n,
         float conf, int32_t *__restrict__ w_print, int32_t *__restrict__ w_count,
         uint32_t *__restrict__ w_node, int32_t *__restrict__ tokens, uint8_t *__restrict__ joined,
-        uint64_t *__restrict__ n_tokens, int32_t *__restrict__ call) {
+        uint64_t *__restrict__ n_tokens, uint64_t *__restrict__ text_len, int32_t *__restrict__ call) {
   const uint64_t r = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
   const uint64_t a0 = a_off[r], a1 = a_off[r + 1], b0 = b_off[r], b1 = b_off[r + 1];
@@ -192,6 +217,7 @@ k_merge(MergeTax t, const int32_t *__restrict__ a_items, const uint64_t *__restr
 
   // ---- write_hit_list (src/merge.cc:328-367): printed pairs
   uint32_t nt = 0;
+  uint64_t chars = 0;
   if (m) {
     int32_t *tk = tokens + 2 * s;
     uint8_t *jn = joined + s;
@@ -221,8 +247,29 @@ k_merge(MergeTax t, const int32_t *__restrict__ a_items, const uint64_t *__restr
     tk[2 * nt] = pt;
     tk[2 * nt + 1] = pc;
     jn[nt++] = 1;
+    for (uint32_t k = 0; k < nt; k++)
+      chars += printed_len(tk[2 * k]) + 1 + printed_len(tk[2 * k + 1]) + (jn[k] ? 0 : 1);
   }
   n_tokens[r] = nt;
+  text_len[r] = chars;
+}
+
+// Prints read r's pairs ("taxid:count", a space after each unless joined)
+// at text[text_off[r]] (text_off: exclusive sum of text_len).
+__global__ void __launch_bounds__(256)
+k_merge_print(const uint64_t *__restrict__ a_off, const uint64_t *__restrict__ b_off, uint64_t n,
+              const uint64_t *__restrict__ n_tokens, const uint64_t *__restrict__ text_off,
+              const int32_t *__restrict__ tokens, const uint8_t *__restrict__ joined, char *__restrict__ text) {
+  const uint64_t r = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const uint64_t s = a_off[r] + b_off[r], nt = n_tokens[r];
+  char *o = text + text_off[r];
+  for (uint64_t k = 0; k < nt; k++) {
+    o = print_int(o, tokens[2 * (s + k)]);
+    *o++ = ':';
+    o = print_int(o, tokens[2 * (s + k) + 1]);
+    if (!joined[s + k]) *o++ = ' ';
+  }
 }
 
 // Copies read r's printed pairs from its slots to [tok_off[r], tok_off[r+1])

@@ -31,8 +31,9 @@ struct k2g_merger {
   k2g::MergeTax view{};
   std::vector<int32_t> w_print, w_count, tokens, call, d_tokens;
   std::vector<uint32_t> w_node;
-  std::vector<uint64_t> n_tokens, tok_off;
+  std::vector<uint64_t> n_tokens, tok_off, text_len;
   std::vector<uint8_t> joined, d_joined;
+  std::vector<char> text;
 };
 
 extern "C" {
@@ -79,7 +80,8 @@ int k2g_merge(k2g_merger *g, const k2g_merge_params *p, const k2g_hit_lists *a, 
   g->w_node.assign(slots, 0);
   g->tokens.assign(2 * slots, 0);
   g->joined.assign(slots, 0);
-  g->n_tokens.assign(n, 0);
+  g->n_tokens.assign(n + 1, 0);
+  g->text_len.assign(n + 1, 0);
   g->call.assign(n, 0);
   emu::blockDim_.x = 1;
   emu::gridDim_.x = (unsigned) n;
@@ -88,22 +90,36 @@ int k2g_merge(k2g_merger *g, const k2g_merge_params *p, const k2g_hit_lists *a, 
     emu::threadIdx_.x = 0;
     k2g::k_merge(g->view, a->items, a->off, b->items, b->off, n, p->confidence, g->w_print.data(),
                  g->w_count.data(), g->w_node.data(), g->tokens.data(), g->joined.data(), g->n_tokens.data(),
-                 g->call.data());
+                 g->text_len.data(), g->call.data());
   }
+  const bool text = p->text != 0;
+  const std::vector<uint64_t> &counts = text ? g->text_len : g->n_tokens;
   g->tok_off.assign(n + 1, 0);
-  for (uint64_t r = 0; r < n; r++) g->tok_off[r + 1] = g->tok_off[r] + g->n_tokens[r];
-  g->d_tokens.assign(2 * g->tok_off[n], 0);
-  g->d_joined.assign(g->tok_off[n], 0);
-  for (uint64_t r = 0; r < n; r++) {
-    emu::blockIdx_.x = (unsigned) r;
-    k2g::k_merge_compact(a->off, b->off, n, g->tok_off.data(), g->tokens.data(), g->joined.data(),
-                         g->d_tokens.data(), g->d_joined.data());
+  for (uint64_t r = 0; r < n; r++) g->tok_off[r + 1] = g->tok_off[r] + counts[r];
+  const uint64_t total = g->tok_off[n];
+  if (text) {
+    g->text.assign(total + 1, 0);
+    for (uint64_t r = 0; r < n; r++) {
+      emu::blockIdx_.x = (unsigned) r;
+      k2g::k_merge_print(a->off, b->off, n, g->n_tokens.data(), g->tok_off.data(), g->tokens.data(),
+                         g->joined.data(), g->text.data());
+    }
+  } else {
+    g->d_tokens.assign(2 * total, 0);
+    g->d_joined.assign(total, 0);
+    for (uint64_t r = 0; r < n; r++) {
+      emu::blockIdx_.x = (unsigned) r;
+      k2g::k_merge_compact(a->off, b->off, n, g->tok_off.data(), g->tokens.data(), g->joined.data(),
+                           g->d_tokens.data(), g->d_joined.data());
+    }
   }
   out->n = n;
   out->call = g->call.data();
-  out->token_off = g->tok_off.data();
-  out->tokens = g->d_tokens.data();
-  out->joined = g->d_joined.data();
+  out->token_off = text ? nullptr : g->tok_off.data();
+  out->tokens = text ? nullptr : g->d_tokens.data();
+  out->joined = text ? nullptr : g->d_joined.data();
+  out->text = text ? g->text.data() : nullptr;
+  out->text_off = text ? g->tok_off.data() : nullptr;
   out->kernel_ms = 0;
   return K2G_OK;
 }

@@ -105,3 +105,44 @@ def test_merge_entry_points_empty_and_errors():
                     np.array([0, 1, 1], np.uint64))
     finally:
         m.close()
+
+
+def _printed(v):  # int_to_string of src/merge.cc:251-326 for the values classify writes
+    return {0x7FFFFFFF: "A", 0x7FFFFFFE: "|", 0x7FFFFFFD: "-"}.get(int(v), str(int(v)))
+
+
+def test_merge_text_mode_equals_pairs_mode():
+    """The entry point's two result forms: pairs (token_off, tokens, joined)
+    printed as write_hit_list prints them equal the text form, on random
+    lists with markers and unknown taxids, at two confidences."""
+    ext = [e for e in taxonomy_external_ids(TOY_TAXO)[1:]]
+    rng = np.random.default_rng(5)
+    n = 3000
+
+    def lists():
+        off = np.zeros(n + 1, np.uint64)
+        items = []
+        for i in range(n):
+            k = int(rng.integers(1, 12))
+            for _ in range(k):
+                r = rng.random()
+                t = (0x7FFFFFFF if r < 0.05 else 0 if r < 0.3 else 10 ** 9 if r < 0.32
+                     else int(rng.choice(ext)))
+                items.append((t, int(rng.integers(1, 9))))
+            off[i + 1] = off[i] + k
+        return np.array(items, np.int32).reshape(-1, 2), off
+
+    (ia, oa), (ib, ob) = lists(), lists()
+    m = k2.Merger(TOY_TAXO)
+    try:
+        for conf in (0.0, 0.4):
+            pairs = m.merge(ia, oa, ib, ob, confidence=conf)
+            text = m.merge(ia, oa, ib, ob, confidence=conf, text=True)
+            assert np.array_equal(pairs.call, text.call)
+            for i in range(n):
+                s, e = int(pairs.token_off[i]), int(pairs.token_off[i + 1])
+                want = "".join(_printed(pairs.tokens[k, 0]) + ":" + _printed(pairs.tokens[k, 1]) +
+                               ("" if pairs.joined[k] else " ") for k in range(s, e))
+                assert text.text[i] == want.encode(), i
+    finally:
+        m.close()
