@@ -310,6 +310,24 @@ int k2g_merge(k2g_merger *m, const k2g_merge_params *p, const k2g_hit_lists *a,
               const k2g_hit_lists *b, k2g_merge_results *out);
 int k2g_merge_close(k2g_merger *m);
 
+/* The same merge with the hit lists still as text, tokenized on the GPU as
+ * parse_hit_list does (src/merge.cc:423-469).  Read i's hit list is the
+ * fifth tab-separated field of its kraken line, text[field[i] .. field[i] +
+ * len[i]), with the line's newline when it has one; off[i+1] - off[i] must
+ * be that field's ':' count (the reference's slot count, :431-435; checked
+ * on the device). */
+typedef struct {
+  const char *text;
+  uint64_t text_bytes;
+  const uint64_t *field; /* n */
+  const uint32_t *len;   /* n */
+  const uint64_t *off;   /* n + 1, off[0] = 0 */
+  uint64_t n;
+} k2g_hit_list_text;
+
+int k2g_merge_text(k2g_merger *m, const k2g_merge_params *p, const k2g_hit_list_text *a,
+                   const k2g_hit_list_text *b, k2g_merge_results *out);
+
 #ifdef __cplusplus
 }
 #endif

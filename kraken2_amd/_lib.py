@@ -59,6 +59,7 @@ EXPORTED_SYMBOLS = (
     "k2g_merge_open",
     "k2g_merge",
     "k2g_merge_close",
+    "k2g_merge_text",
 )
 
 MERGE_AMBIG = 0x7FFFFFFF
@@ -145,6 +146,11 @@ class HitLists(C.Structure):
     _fields_ = [("items", C.c_void_p), ("off", C.c_void_p), ("n", C.c_uint64)]
 
 
+class HitListText(C.Structure):
+    _fields_ = [("text", C.c_void_p), ("text_bytes", C.c_uint64), ("field", C.c_void_p), ("len", C.c_void_p),
+                ("off", C.c_void_p), ("n", C.c_uint64)]
+
+
 class MergeParams(C.Structure):
     _fields_ = [("confidence", C.c_float), ("text", C.c_int32)]
 
@@ -205,6 +211,8 @@ def load_library(path: str = LIB_PATH):
     lib.k2g_merge.argtypes = [vp, C.POINTER(MergeParams), C.POINTER(HitLists), C.POINTER(HitLists),
                               C.POINTER(MergeResults)]
     lib.k2g_merge_close.argtypes = [vp]
+    lib.k2g_merge_text.argtypes = [vp, C.POINTER(MergeParams), C.POINTER(HitListText), C.POINTER(HitListText),
+                                   C.POINTER(MergeResults)]
     lib.k2g_batch_stream.restype = vp
     for name in EXPORTED_SYMBOLS:
         if name not in ("k2g_last_error", "k2g_batch_stream"):
@@ -638,6 +646,39 @@ class Merger:
         m = int(token_off[-1])
         return MergeOutput(arr(r.call, n, np.int32), token_off, arr(r.tokens, 2 * m, np.int32).reshape(-1, 2),
                            arr(r.joined, m, np.uint8), float(r.kernel_ms))
+
+    def merge_text(self, lines_a, lines_b, confidence: float = 0.0, item_counts=None) -> MergeOutput:
+        """Merges two lists of kraken lines (bytes, newline included) through
+        k2g_merge_text; item_counts overrides the ':' counts (for tests)."""
+        def pack(lines):
+            text = b"".join(lines)
+            field, ln, off = [], [], [0]
+            pos = 0
+            for l in lines:
+                parts = [i for i, c in enumerate(l) if c == 9]  # tab positions
+                start = parts[3] + 1 if len(parts) >= 4 else 0
+                field.append(pos + start)
+                ln.append(len(l) - start)
+                off.append(off[-1] + l[start:].count(b":"))
+                pos += len(l)
+            return (np.frombuffer(text, np.uint8) if text else np.zeros(1, np.uint8), len(text),
+                    np.array(field, np.uint64), np.array(ln, np.uint32), np.array(off, np.uint64))
+        ta, tb = pack(lines_a), pack(lines_b)
+        if item_counts is not None:
+            ta = ta[:4] + (np.concatenate([[0], np.cumsum(item_counts)]).astype(np.uint64),)
+        a = HitListText(ta[0].ctypes.data, ta[1], ta[2].ctypes.data, ta[3].ctypes.data, ta[4].ctypes.data,
+                        len(lines_a))
+        b = HitListText(tb[0].ctypes.data, tb[1], tb[2].ctypes.data, tb[3].ctypes.data, tb[4].ctypes.data,
+                        len(lines_b))
+        r = MergeResults()
+        _check(self._lib.k2g_merge_text(self._h, C.byref(MergeParams(confidence, 1)), C.byref(a), C.byref(b),
+                                        C.byref(r)))
+        n = int(r.n)
+        off = np.ctypeslib.as_array(r.text_off, (n + 1,)).astype(np.uint64) if n else np.zeros(1, np.uint64)
+        raw = C.string_at(r.text, int(off[-1])) if int(off[-1]) else b""
+        call = np.ctypeslib.as_array(r.call, (n,)).astype(np.int32) if n else np.zeros(0, np.int32)
+        return MergeOutput(call, None, None, None, float(r.kernel_ms),
+                           [raw[int(off[i]):int(off[i + 1])] for i in range(n)])
 
     def close(self):
         if self._h:

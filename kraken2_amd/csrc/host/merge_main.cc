@@ -86,18 +86,38 @@ void put_int(std::string &o, int32_t v) {
   o.append(b, e - b);
 }
 
-// A byte buffer that grows without initialising its new bytes.
+// A byte buffer that grows without initialising its new bytes; pinned
+// buffers (uploaded to the GPU) come from k2g_host_alloc.
 struct Bytes {
-  std::unique_ptr<char[]> p;
+  char *p = nullptr;
   size_t size = 0, cap = 0;
-  char *data() { return p.get(); }
-  const char *data() const { return p.get(); }
+  bool pinned = false;
+  Bytes() = default;
+  explicit Bytes(bool pin) : pinned(pin) {}
+  Bytes(const Bytes &) = delete;
+  Bytes &operator=(const Bytes &) = delete;
+  ~Bytes() { release(p); }
+  char *data() { return p; }
+  const char *data() const { return p; }
+  void release(char *q) {
+    if (!q) return;
+    if (pinned)
+      k2g_host_free(q);
+    else
+      delete[] q;
+  }
   void reserve(size_t n) {
     if (n <= cap) return;
     const size_t c = std::max(n, 2 * cap);
-    std::unique_ptr<char[]> q(new char[c]);
-    if (size) memcpy(q.get(), p.get(), size);
-    p.swap(q);
+    char *q = nullptr;
+    if (pinned) {
+      if (k2g_host_alloc(c, (void **) &q)) die(71, k2g_last_error());
+    } else {
+      q = new char[c];
+    }
+    if (size) memcpy(q, p, size);
+    release(p);
+    p = q;
     cap = c;
   }
 };
@@ -187,50 +207,8 @@ uint64_t count_items(const char *s, size_t len) {
   return n;
 }
 
-// src/merge.cc:423-469 into out[0 .. colon count): one slot per ':' (slots
-// the tokens do not reach stay {0,0}); the last count loses the field's
-// last character (the line's newline).
-void parse_hit_list(const char *s, size_t len, int32_t *out, uint64_t slots) {
-  const char *p = s, *end = s + len;
-  uint64_t i = 0;
-  while (p) {
-    const char *colon = p;
-    while (colon < end && *colon != ':') colon++;
-    if (colon == end) break;
-    const char *t = p;
-    const size_t len1 = colon - t;
-    const char *c = colon + 1, *sp = c;
-    while (sp < end && *sp != ' ') sp++;
-    size_t len2;
-    if (sp < end) {
-      len2 = sp - c;
-      p = sp + 1;
-    } else {
-      len2 = (size_t) (end - c) - 1;  // drops the line's last character
-      p = nullptr;
-    }
-    const char first = len1 ? t[0] : '\0';
-    int32_t taxid, count;
-    if (first == 'A') {
-      taxid = kAmbig;
-      count = (int32_t) (uint32_t) atou(c, len2);
-    } else if (first == '|') {
-      taxid = count = kMate;
-    } else if (first == '-') {
-      taxid = count = kFrame;
-    } else {
-      taxid = (int32_t) (uint32_t) atou(t, len1);
-      count = (int32_t) (uint32_t) atou(c, len2);
-    }
-    out[2 * i] = taxid;
-    out[2 * i + 1] = count;
-    i++;
-  }
-  for (; i < slots; i++) out[2 * i] = out[2 * i + 1] = 0;
-}
-
 struct Batch {  // lines of both inputs
-  Bytes buf1, buf2;
+  Bytes buf1{true}, buf2{true};
   std::vector<size_t> st1, st2;
   size_t n = 0;
 };
@@ -255,19 +233,21 @@ struct Pinned {
 struct Slot {  // one batch on its way through parse, GPU merge and print
   std::unique_ptr<Batch> batch;
   std::vector<Fields> fa, fb;
-  Pinned<uint64_t> off_a, off_b;
-  Pinned<int32_t> items_a, items_b;
+  Pinned<uint64_t> off_a, off_b, field_a, field_b;  // item offsets; hit-list field offsets
+  Pinned<uint32_t> len_a, len_b;                     // hit-list field lengths
   k2g_merge_results res{};
 };
 
-// Fields and hit lists of every line pair, on the OpenMP threads.
+// Fields of every line pair and the ':' count of each hit list, on the
+// OpenMP threads (the hit lists are tokenized on the GPU).
 void parse_batch(Slot &sl, int threads) {
   const Batch &b = *sl.batch;
   const size_t n = b.n;
   sl.fa.resize(n);
   sl.fb.resize(n);
-  sl.off_a.reserve(n + 1);
-  sl.off_b.reserve(n + 1);
+  for (auto *q : {&sl.off_a, &sl.off_b, &sl.field_a, &sl.field_b}) q->reserve(n + 1);
+  sl.len_a.reserve(n + 1);
+  sl.len_b.reserve(n + 1);
   uint64_t *oa = sl.off_a.p, *ob = sl.off_b.p;
   oa[0] = ob[0] = 0;
   bool bad = false;
@@ -280,19 +260,15 @@ void parse_batch(Slot &sl, int threads) {
     }
     oa[i + 1] = count_items(sl.fa[i].f[4], sl.fa[i].len[4]);
     ob[i + 1] = count_items(sl.fb[i].f[4], sl.fb[i].len[4]);
+    sl.field_a.p[i] = sl.fa[i].f[4] - b.buf1.data();
+    sl.field_b.p[i] = sl.fb[i].f[4] - b.buf2.data();
+    sl.len_a.p[i] = sl.fa[i].len[4];
+    sl.len_b.p[i] = sl.fb[i].len[4];
   }
   if (bad) die(65, "malformed kraken output line (fewer than five fields)");
   for (size_t i = 0; i < n; i++) {
     oa[i + 1] += oa[i];
     ob[i + 1] += ob[i];
-  }
-  sl.items_a.reserve(2 * oa[n] + 2);
-  sl.items_b.reserve(2 * ob[n] + 2);
-  int32_t *ia = sl.items_a.p, *ib = sl.items_b.p;
-#pragma omp parallel for schedule(static) num_threads(threads)
-  for (size_t i = 0; i < n; i++) {
-    parse_hit_list(sl.fa[i].f[4], sl.fa[i].len[4], ia + 2 * oa[i], oa[i + 1] - oa[i]);
-    parse_hit_list(sl.fb[i].f[4], sl.fb[i].len[4], ib + 2 * ob[i], ob[i + 1] - ob[i]);
   }
 }
 
@@ -503,8 +479,10 @@ int main(int argc, char **argv) {
     int rc = 0;
     k2g_merge_params p{conf, 1};
     std::thread gpu([&] {
-      k2g_hit_lists a{sl.items_a.p, sl.off_a.p, sl.batch->n}, bb{sl.items_b.p, sl.off_b.p, sl.batch->n};
-      rc = k2g_merge(mg[j], &p, &a, &bb, &sl.res);
+      const Batch &bt = *sl.batch;
+      k2g_hit_list_text a{bt.buf1.data(), bt.buf1.size, sl.field_a.p, sl.len_a.p, sl.off_a.p, bt.n};
+      k2g_hit_list_text bb{bt.buf2.data(), bt.buf2.size, sl.field_b.p, sl.len_b.p, sl.off_b.p, bt.n};
+      rc = k2g_merge_text(mg[j], &p, &a, &bb, &sl.res);
     });
     if (pending) {
       print(*pending);

@@ -31,7 +31,9 @@ struct k2g_merger {
   DevBuf<uint64_t> a_off, b_off, n_tokens, tok_off, text_len;
   DevBuf<uint32_t> w_node;
   DevBuf<uint8_t> joined, d_joined, scan_tmp;
-  DevBuf<char> text;
+  DevBuf<char> text, a_text, b_text;
+  DevBuf<uint64_t> a_field, b_field;
+  DevBuf<uint32_t> a_len, b_len, bad;
   PinnedBuf<int32_t> h_call, h_tokens;
   PinnedBuf<uint64_t> h_tok_off;
   PinnedBuf<uint8_t> h_joined;
@@ -43,7 +45,9 @@ struct k2g_merger {
     for (auto *b : {&ext32, &a_items, &b_items, &w_print, &w_count, &tokens, &d_tokens, &call}) b->release();
     for (auto *b : {&ext64, &key, &a_off, &b_off, &n_tokens, &tok_off, &text_len}) b->release();
     for (auto *b : {&joined, &d_joined, &scan_tmp}) b->release();
-    text.release();
+    for (auto *b : {&text, &a_text, &b_text}) b->release();
+    for (auto *b : {&a_field, &b_field}) b->release();
+    for (auto *b : {&a_len, &b_len, &bad}) b->release();
     h_text.release();
     h_call.release();
     h_tokens.release();
@@ -93,6 +97,9 @@ extern "C" int k2g_merge_open(const char *taxo_k2d, k2g_merger **out) {
   return K2G_OK;
 }
 
+static int run_merge(k2g_merger *g, const k2g_merge_params *p, uint64_t n, uint64_t slots,
+                     k2g_merge_results *out);
+
 static int check_lists(const k2g_hit_lists *h, const char *which) {
   if (!h->off) return fail(K2G_EUSAGE, std::string("null offsets in hit lists ") + which);
   if (h->off[0] != 0) return fail(K2G_EUSAGE, std::string("hit lists ") + which + " must start at offset 0");
@@ -115,12 +122,18 @@ extern "C" int k2g_merge(k2g_merger *g, const k2g_merge_params *p, const k2g_hit
   for (uint64_t i = 0; i < a->n; i++)
     if (a->off[i + 1] - a->off[i] + b->off[i + 1] - b->off[i] > (uint64_t) UINT32_MAX)
       return fail(K2G_EUSAGE, "a read's hit lists hold 2^32 or more items");
-  const uint64_t n = a->n, na = a->off[n], nb = b->off[n], slots = na + nb;
+  const uint64_t n = a->n, na = a->off[n], nb = b->off[n];
   hipStream_t s = g->stream;
   HIP_TRY(upload(g->a_items, a->items, 2 * na, s));
   HIP_TRY(upload(g->b_items, b->items, 2 * nb, s));
   HIP_TRY(upload(g->a_off, a->off, n + 1, s));
   HIP_TRY(upload(g->b_off, b->off, n + 1, s));
+  return run_merge(g, p, n, na + nb, out);
+}
+
+// Merges the n reads whose items and offsets are on the device.
+static int run_merge(k2g_merger *g, const k2g_merge_params *p, uint64_t n, uint64_t slots, k2g_merge_results *out) {
+  hipStream_t s = g->stream;
   HIP_TRY(g->w_print.reserve(slots));
   HIP_TRY(g->w_count.reserve(slots));
   HIP_TRY(g->w_node.reserve(slots));
@@ -192,6 +205,60 @@ extern "C" int k2g_merge(k2g_merger *g, const k2g_merge_params *p, const k2g_hit
   out->text_off = text ? g->h_tok_off.p : nullptr;
   out->kernel_ms = ms;
   return K2G_OK;
+}
+
+static int check_text(const k2g_hit_list_text *h, const char *which) {
+  const std::string w = std::string(" of hit-list text ") + which;
+  if (!h->off || (h->n && (!h->field || !h->len))) return fail(K2G_EUSAGE, "null arrays" + w);
+  if (h->off[0] != 0) return fail(K2G_EUSAGE, "item offsets" + w + " must start at 0");
+  for (uint64_t i = 0; i < h->n; i++) {
+    if (h->off[i + 1] < h->off[i]) return fail(K2G_EUSAGE, "item offsets" + w + " must not decrease");
+    if (h->field[i] > h->text_bytes || h->len[i] > h->text_bytes - h->field[i])
+      return fail(K2G_EUSAGE, "a field" + w + " lies outside the text");
+  }
+  if (h->text_bytes && !h->text) return fail(K2G_EUSAGE, "null text" + w);
+  if (h->off[h->n] > (uint64_t) UINT32_MAX) return fail(K2G_EUSAGE, "hit lists" + w + " hold 2^32 or more items");
+  return K2G_OK;
+}
+
+extern "C" int k2g_merge_text(k2g_merger *g, const k2g_merge_params *p, const k2g_hit_list_text *a,
+                              const k2g_hit_list_text *b, k2g_merge_results *out) {
+  if (!g || !p || !a || !b || !out) return fail(K2G_EUSAGE, "null argument");
+  if (a->n != b->n) return fail(K2G_EUSAGE, "the two inputs hold different numbers of reads");
+  int rc = check_text(a, "a");
+  if (!rc) rc = check_text(b, "b");
+  if (rc) return rc;
+  const uint64_t n = a->n, na = a->off[n], nb = b->off[n];
+  for (uint64_t i = 0; i < n; i++)
+    if (a->off[i + 1] - a->off[i] + b->off[i + 1] - b->off[i] > (uint64_t) UINT32_MAX)
+      return fail(K2G_EUSAGE, "a read's hit lists hold 2^32 or more items");
+  hipStream_t s = g->stream;
+  HIP_TRY(upload(g->a_text, a->text, a->text_bytes, s));
+  HIP_TRY(upload(g->b_text, b->text, b->text_bytes, s));
+  HIP_TRY(upload(g->a_field, a->field, n, s));
+  HIP_TRY(upload(g->b_field, b->field, n, s));
+  HIP_TRY(upload(g->a_len, a->len, n, s));
+  HIP_TRY(upload(g->b_len, b->len, n, s));
+  HIP_TRY(upload(g->a_off, a->off, n + 1, s));
+  HIP_TRY(upload(g->b_off, b->off, n + 1, s));
+  HIP_TRY(g->a_items.reserve(2 * na + 2));
+  HIP_TRY(g->b_items.reserve(2 * nb + 2));
+  HIP_TRY(g->bad.reserve(1));
+  HIP_TRY(hipMemsetAsync(g->bad.p, 0, 4, s));
+  if (n) {
+    const uint32_t blocks = (uint32_t) ((n + 255) / 256);
+    hipLaunchKernelGGL(k_merge_parse, dim3(blocks), dim3(256), 0, s, g->a_text.p, g->a_field.p, g->a_len.p,
+                       g->a_off.p, n, g->a_items.p, g->bad.p);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_merge_parse, dim3(blocks), dim3(256), 0, s, g->b_text.p, g->b_field.p, g->b_len.p,
+                       g->b_off.p, n, g->b_items.p, g->bad.p);
+    HIP_TRY(hipGetLastError());
+  }
+  uint32_t bad = 0;
+  HIP_TRY(hipMemcpyAsync(&bad, g->bad.p, 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (bad) return fail(K2G_EUSAGE, "item offsets do not match the ':' count of the hit-list fields");
+  return run_merge(g, p, n, na + nb, out);
 }
 
 extern "C" int k2g_merge_close(k2g_merger *g) {

@@ -254,6 +254,76 @@ k_merge(MergeTax t, const int32_t *__restrict__ a_items, const uint64_t *__restr
   text_len[r] = chars;
 }
 
+// parse_hit_list (src/merge.cc:423-469) of one hit-list field s[0..len)
+// (the fifth field of a kraken line, with the line's newline) into
+// out[0 .. slots) pairs, slots being the field's ':' count: tokens split at
+// the first ':' and the next ' ', numbers read digit by digit with no
+// checks (:217-244), the last count one character short (the newline), and
+// slots the tokens do not reach left {0,0}.
+__device__ __forceinline__ uint64_t field_atou(const char *p, uint64_t len) {
+  if (len > 20) return 0;
+  uint64_t r = 0;
+  for (uint64_t i = 0; i < len; i++) r = r * 10 + (uint64_t) (int64_t) ((int) (unsigned char) p[i] - '0');
+  return r;
+}
+
+__device__ __forceinline__ void parse_field(const char *s, uint64_t len, int32_t *out, uint64_t slots) {
+  const char *p = s, *end = s + len;
+  uint64_t i = 0;
+  bool more = true;
+  while (more && i < slots) {
+    const char *colon = p;
+    while (colon < end && *colon != ':') colon++;
+    if (colon == end) break;
+    const char *c = colon + 1, *sp = c;
+    while (sp < end && *sp != ' ') sp++;
+    const uint64_t len1 = colon - p;
+    uint64_t len2;
+    if (sp < end) {
+      len2 = sp - c;
+    } else {
+      len2 = (uint64_t) (end - c) - 1;
+      more = false;
+    }
+    const char first = len1 ? p[0] : '\0';
+    int32_t taxid, count;
+    if (first == 'A') {
+      taxid = INT32_MAX;
+      count = (int32_t) (uint32_t) field_atou(c, len2);
+    } else if (first == '|') {
+      taxid = count = INT32_MAX - 1;
+    } else if (first == '-') {
+      taxid = count = INT32_MAX - 2;
+    } else {
+      taxid = (int32_t) (uint32_t) field_atou(p, len1);
+      count = (int32_t) (uint32_t) field_atou(c, len2);
+    }
+    out[2 * i] = taxid;
+    out[2 * i + 1] = count;
+    i++;
+    p = sp + 1;
+  }
+  for (; i < slots; i++) out[2 * i] = out[2 * i + 1] = 0;
+}
+
+// One thread per read: field[r], len[r] locate read r's hit-list field in
+// text; off (n+1) gives its item slots.
+__global__ void __launch_bounds__(256)
+k_merge_parse(const char *__restrict__ text, const uint64_t *__restrict__ field, const uint32_t *__restrict__ len,
+              const uint64_t *__restrict__ off, uint64_t n, int32_t *__restrict__ items, uint32_t *__restrict__ bad) {
+  const uint64_t r = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const char *f = text + field[r];
+  const uint64_t slots = off[r + 1] - off[r];
+  uint64_t colons = 0;
+  for (uint32_t i = 0; i < len[r]; i++) colons += f[i] == ':';
+  if (colons != slots) {  // the caller's slot counts must be the fields' ':' counts
+    *bad = 1;
+    return;
+  }
+  parse_field(f, len[r], items + 2 * off[r], slots);
+}
+
 // Prints read r's pairs ("taxid:count", a space after each unless joined)
 // at text[text_off[r]] (text_off: exclusive sum of text_len).
 __global__ void __launch_bounds__(256)

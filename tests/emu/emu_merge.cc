@@ -69,11 +69,38 @@ int k2g_merge_open(const char *taxo_k2d, k2g_merger **out) {
 }
 
 int k2g_merge(k2g_merger *g, const k2g_merge_params *p, const k2g_hit_lists *a, const k2g_hit_lists *b,
+              k2g_merge_results *out);
+
+int k2g_merge_text(k2g_merger *g, const k2g_merge_params *p, const k2g_hit_list_text *a,
+                   const k2g_hit_list_text *b, k2g_merge_results *out) {
+  if (a->n != b->n || a->off[0] || b->off[0]) {
+    g_error = "bad hit lists";
+    return K2G_EUSAGE;
+  }
+  const uint64_t n = a->n;
+  std::vector<int32_t> ia(2 * a->off[n] + 2), ib(2 * b->off[n] + 2);
+  uint32_t bad = 0;
+  for (uint64_t r = 0; r < n; r++) {
+    emu::blockIdx_.x = (unsigned) r;
+    emu::threadIdx_.x = 0;
+    k2g::k_merge_parse(a->text, a->field, a->len, a->off, n, ia.data(), &bad);
+    k2g::k_merge_parse(b->text, b->field, b->len, b->off, n, ib.data(), &bad);
+  }
+  if (bad) {
+    g_error = "item offsets do not match the ':' count of the hit-list fields";
+    return K2G_EUSAGE;
+  }
+  k2g_hit_lists la{ia.data(), a->off, n}, lb{ib.data(), b->off, n};
+  return k2g_merge(g, p, &la, &lb, out);
+}
+
+int k2g_merge(k2g_merger *g, const k2g_merge_params *p, const k2g_hit_lists *a, const k2g_hit_lists *b,
               k2g_merge_results *out) {
   if (a->n != b->n || a->off[0] || b->off[0]) {
     g_error = "bad hit lists";
     return K2G_EUSAGE;
   }
+  emu::blockDim_.x = 1;
   const uint64_t n = a->n, slots = a->off[n] + b->off[n];
   g->w_print.assign(slots, 0);
   g->w_count.assign(slots, 0);

@@ -146,3 +146,23 @@ def test_merge_text_mode_equals_pairs_mode():
                 assert text.text[i] == want.encode(), i
     finally:
         m.close()
+
+
+def test_merge_text_entry_point_checks_item_counts(tmp_path):
+    """k2g_merge_text tokenizes on the GPU; its hit lists equal the oracle's
+    merged output, and slot counts that are not the fields' ':' counts are
+    rejected."""
+    lines_a = [l.encode() for l in RAGGED_MERGE_LINES[0]]
+    lines_b = [l.encode() for l in RAGGED_MERGE_LINES[1]]
+    pa, pb = tmp_path / "a.kraken", tmp_path / "b.kraken"
+    pa.write_bytes(b"".join(lines_a))
+    pb.write_bytes(b"".join(lines_b))
+    exp = run_merge(ORACLE, TOY_TAXO, str(pa), str(pb), [], str(tmp_path))["output"].decode().splitlines()
+    m = k2.Merger(TOY_TAXO)
+    try:
+        out = m.merge_text(lines_a, lines_b)
+        assert [t.decode() for t in out.text] == [l.split("\t")[4] for l in exp]
+        with pytest.raises(k2.K2GError):
+            m.merge_text(lines_a, lines_b, item_counts=[5, 1, 1, 3, 3])
+    finally:
+        m.close()
