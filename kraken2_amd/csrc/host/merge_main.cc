@@ -96,6 +96,10 @@ struct Bytes {
   explicit Bytes(bool pin) : pinned(pin) {}
   Bytes(const Bytes &) = delete;
   Bytes &operator=(const Bytes &) = delete;
+  Bytes(Bytes &&o) noexcept : p(o.p), size(o.size), cap(o.cap), pinned(o.pinned) {
+    o.p = nullptr;
+    o.size = o.cap = 0;
+  }
   ~Bytes() { release(p); }
   char *data() { return p; }
   const char *data() const { return p; }
@@ -236,6 +240,9 @@ struct Slot {  // one batch on its way through parse, GPU merge and print
   Pinned<uint64_t> off_a, off_b, field_a, field_b;  // item offsets; hit-list field offsets
   Pinned<uint32_t> len_a, len_b;                     // hit-list field lengths
   k2g_merge_results res{};
+  std::vector<Bytes> text;          // printed lines, one buffer per print thread
+  std::vector<std::string> names;   // -c lines, likewise
+  bool writing = false;             // queued for the writer thread
 };
 
 // Fields of every line pair and the ':' count of each hit list, on the
@@ -348,6 +355,7 @@ int main(int argc, char **argv) {
   std::condition_variable cv;
   std::deque<std::unique_ptr<Batch>> ready, spare;
   bool finished = false, mismatch = false;
+  size_t hint1 = 0, hint2 = 0;
   std::thread reader([&] {
     for (;;) {
       std::unique_ptr<Batch> b;
@@ -359,11 +367,17 @@ int main(int argc, char **argv) {
           spare.pop_front();
         }
       }
-      if (!b) b.reset(new Batch());
+      if (!b) {  // pinned line buffers sized from the batches read so far
+        b.reset(new Batch());
+        b->buf1.reserve(hint1 + hint1 / 4 + 1);
+        b->buf2.reserve(hint2 + hint2 / 4 + 1);
+      }
       size_t n2 = 0;
       std::thread second([&] { n2 = in2.read(g_batch_lines, b->buf2, b->st2); });
       b->n = in1.read(g_batch_lines, b->buf1, b->st1);
       second.join();
+      hint1 = std::max(hint1, b->buf1.size);
+      hint2 = std::max(hint2, b->buf2.size);
       std::lock_guard<std::mutex> lk(mu);
       if (n2 != b->n) {
         mismatch = finished = true;
@@ -378,8 +392,35 @@ int main(int argc, char **argv) {
   });
 
   Slot slot[2];
-  std::vector<Bytes> text(threads);
-  std::vector<std::string> names(threads);
+  for (auto &sl : slot) {
+    sl.text.resize(threads);
+    sl.names.resize(threads);
+  }
+  // writer thread: printed batches go to the files in order while the next
+  // batches are parsed, merged and printed
+  std::mutex wmu;
+  std::condition_variable wcv;
+  std::deque<Slot *> wq;
+  bool wdone = false;
+  std::thread writer([&] {
+    for (;;) {
+      Slot *sl;
+      {
+        std::unique_lock<std::mutex> lk(wmu);
+        wcv.wait(lk, [&] { return !wq.empty() || wdone; });
+        if (wq.empty()) return;
+        sl = wq.front();
+      }
+      for (int w = 0; w < threads; w++) {
+        fwrite(sl->text[w].data(), 1, sl->text[w].size, out);
+        if (hdr) fwrite(sl->names[w].data(), 1, sl->names[w].size(), hdr);
+      }
+      std::lock_guard<std::mutex> lk(wmu);
+      wq.pop_front();
+      sl->writing = false;
+      wcv.notify_all();
+    }
+  });
   std::vector<k2host::TaxonCounters> part_counters(threads);
   std::vector<uint64_t> part_unclassified(threads);
   uint64_t total_seqs = 0, total_unclassified = 0;
@@ -389,6 +430,10 @@ int main(int argc, char **argv) {
 
   // prints one merged batch (src/merge.cc:701-758), in input order
   auto print = [&](Slot &sl) {
+    {
+      std::unique_lock<std::mutex> lk(wmu);
+      wcv.wait(lk, [&] { return !sl.writing; });
+    }
     const Batch &bt = *sl.batch;
     const size_t n = bt.n;
     const k2g_merge_results &res = sl.res;
@@ -396,8 +441,8 @@ int main(int argc, char **argv) {
     {
       const int w = omp_get_thread_num(), nw = omp_get_num_threads();
       const size_t lo = n * w / nw, hi = n * (w + 1) / nw;
-      Bytes &o = text[w];
-      std::string &h = names[w];
+      Bytes &o = sl.text[w];
+      std::string &h = sl.names[w];
       o.size = 0;
       h.clear();
       std::string taxid;
@@ -447,12 +492,12 @@ int main(int argc, char **argv) {
       }
       part_unclassified[w] = unc;
     }
-    for (int w = 0; w < threads; w++) {
-      fwrite(text[w].data(), 1, text[w].size, out);
-      if (hdr) fwrite(names[w].data(), 1, names[w].size(), hdr);
-      total_unclassified += part_unclassified[w];
-    }
+    for (int w = 0; w < threads; w++) total_unclassified += part_unclassified[w];
     total_seqs += n;
+    std::lock_guard<std::mutex> lk(wmu);
+    sl.writing = true;
+    wq.push_back(&sl);
+    wcv.notify_all();
   };
 
   Slot *pending = nullptr;  // merged on the GPU, not yet printed
@@ -503,6 +548,12 @@ int main(int argc, char **argv) {
     print(*pending);
     t_format += now_s() - t0;
   }
+  {
+    std::lock_guard<std::mutex> lk(wmu);
+    wdone = true;
+    wcv.notify_all();
+  }
+  writer.join();
   if (fclose(out) != 0) die(74, std::string("unable to write ") + out_name);
   if (hdr) {
     fprintf(hdr, "%llu\n", (unsigned long long) total_seqs);

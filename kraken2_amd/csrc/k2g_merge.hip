@@ -29,7 +29,7 @@ struct k2g_merger {
   DevBuf<uint64_t> ext64, key;
   DevBuf<int32_t> a_items, b_items, w_print, w_count, tokens, d_tokens, call;
   DevBuf<uint64_t> a_off, b_off, n_tokens, tok_off, text_len;
-  DevBuf<uint32_t> w_node;
+  DevBuf<uint32_t> w_node, w_dcount;
   DevBuf<uint8_t> joined, d_joined, scan_tmp;
   DevBuf<char> text, a_text, b_text;
   DevBuf<uint64_t> a_field, b_field;
@@ -41,7 +41,7 @@ struct k2g_merger {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   ~k2g_merger() {
-    for (auto *b : {&parent, &span, &key_node, &direct, &w_node}) b->release();
+    for (auto *b : {&parent, &span, &key_node, &direct, &w_node, &w_dcount}) b->release();
     for (auto *b : {&ext32, &a_items, &b_items, &w_print, &w_count, &tokens, &d_tokens, &call}) b->release();
     for (auto *b : {&ext64, &key, &a_off, &b_off, &n_tokens, &tok_off, &text_len}) b->release();
     for (auto *b : {&joined, &d_joined, &scan_tmp}) b->release();
@@ -137,6 +137,7 @@ static int run_merge(k2g_merger *g, const k2g_merge_params *p, uint64_t n, uint6
   HIP_TRY(g->w_print.reserve(slots));
   HIP_TRY(g->w_count.reserve(slots));
   HIP_TRY(g->w_node.reserve(slots));
+  HIP_TRY(g->w_dcount.reserve(slots));
   HIP_TRY(g->tokens.reserve(2 * slots));
   HIP_TRY(g->joined.reserve(slots));
   HIP_TRY(g->n_tokens.reserve(n + 1));
@@ -153,8 +154,8 @@ static int run_merge(k2g_merger *g, const k2g_merge_params *p, uint64_t n, uint6
   HIP_TRY(hipMemsetAsync(counts + n, 0, 8, s));
   if (n) {
     hipLaunchKernelGGL(k_merge, dim3(blocks), dim3(256), 0, s, g->view, g->a_items.p, g->a_off.p, g->b_items.p,
-                       g->b_off.p, n, p->confidence, g->w_print.p, g->w_count.p, g->w_node.p, g->tokens.p,
-                       g->joined.p, g->n_tokens.p, g->text_len.p, g->call.p);
+                       g->b_off.p, n, p->confidence, g->w_print.p, g->w_count.p, g->w_node.p, g->w_dcount.p,
+                       g->tokens.p, g->joined.p, g->n_tokens.p, g->text_len.p, g->call.p);
     HIP_TRY(hipGetLastError());
   }
   HIP_TRY(hipcub::DeviceScan::ExclusiveSum(g->scan_tmp.p, scan_bytes, counts, g->tok_off.p, n + 1, s));

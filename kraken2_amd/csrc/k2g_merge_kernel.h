@@ -104,14 +104,18 @@ __global__ void __launch_bounds__(256)
 k_merge(MergeTax t, const int32_t *__restrict__ a_items, const uint64_t *__restrict__ a_off,
         const int32_t *__restrict__ b_items, const uint64_t *__restrict__ b_off, uint64_t n,
         float conf, int32_t *__restrict__ w_print, int32_t *__restrict__ w_count,
-        uint32_t *__restrict__ w_node, int32_t *__restrict__ tokens, uint8_t *__restrict__ joined,
+        uint32_t *__restrict__ w_node, uint32_t *__restrict__ w_dcount, int32_t *__restrict__ tokens,
+        uint8_t *__restrict__ joined,
         uint64_t *__restrict__ n_tokens, uint64_t *__restrict__ text_len, int32_t *__restrict__ call) {
   const uint64_t r = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
   const uint64_t a0 = a_off[r], a1 = a_off[r + 1], b0 = b_off[r], b1 = b_off[r + 1];
   const uint64_t s = a0 + b0;  // this read's slots
   int32_t *pr = w_print + s, *ct = w_count + s;
-  uint32_t *nd = w_node + s;
+  // distinct nodes of the merged taxa and their summed counts: the
+  // reference's hit_counts map (:505, :546-548) keyed by node
+  uint32_t *dn = w_node + s, *dc = w_dcount + s;
+  uint32_t d = 0;
 
   // ---- merge_hit_lists (src/merge.cc:494-554).  A second list shorter
   // than the first reads past its end in the reference; here a missing item
@@ -157,8 +161,22 @@ k_merge(MergeTax t, const int32_t *__restrict__ a_items, const uint64_t *__restr
       total += (uint64_t) (int64_t) fc;
       pr[m] = p;
       ct[m] = fc;
-      nd[m] = node;
       m++;
+      if (p <= kValid) {
+        uint32_t k = d;
+        if (d && dn[d - 1] == node) {
+          k = d - 1;
+        } else {
+          for (k = 0; k < d && dn[k] != node; k++) {
+          }
+          if (k == d) {
+            dn[d] = node;
+            dc[d] = 0;
+            d++;
+          }
+        }
+        dc[k] += (uint32_t) fc;
+      }
       if (adv1) {
         if (++i1 >= a1) break;
         ta = a_items[2 * i1];
@@ -180,16 +198,16 @@ k_merge(MergeTax t, const int32_t *__restrict__ a_items, const uint64_t *__restr
     }
   }
 
-  // ---- resolve_tree (src/merge.cc:369-421) over the merged items; markers
-  // carry no taxon.  Scores and sums wrap at 32 bits as the reference's do.
+  // ---- resolve_tree (src/merge.cc:369-421) over the distinct nodes;
+  // markers carry no taxon.  Scores and sums wrap at 32 bits as the
+  // reference's do.
   const uint32_t required = to_u32(ceilf(conf * (float) total));
   uint32_t max_taxon = 0, max_score = 0;
-  for (uint32_t j = 0; j < m; j++) {
-    if (pr[j] > kValid) continue;
-    const uint32_t taxon = nd[j];
+  for (uint32_t j = 0; j < d; j++) {
+    const uint32_t taxon = dn[j];
     uint32_t score = 0;
-    for (uint32_t k = 0; k < m; k++)
-      if (pr[k] <= kValid && above(t, nd[k], taxon)) score += (uint32_t) ct[k];
+    for (uint32_t k = 0; k < d; k++)
+      if (above(t, dn[k], taxon)) score += dc[k];
     if (score > max_score) {
       max_score = score;
       max_taxon = taxon;
@@ -205,8 +223,8 @@ k_merge(MergeTax t, const int32_t *__restrict__ a_items, const uint64_t *__restr
   bool node_id = false;
   while (max_taxon && max_score < required) {
     max_score = 0;
-    for (uint32_t k = 0; k < m; k++)
-      if (pr[k] <= kValid && above(t, max_taxon, nd[k])) max_score += (uint32_t) ct[k];
+    for (uint32_t k = 0; k < d; k++)
+      if (above(t, max_taxon, dn[k])) max_score += dc[k];
     if (max_score >= required) {
       node_id = true;  // the node id is returned, as the reference does (:413)
       break;

@@ -30,7 +30,7 @@ struct k2g_merger {
   k2g::MergeTables tb;
   k2g::MergeTax view{};
   std::vector<int32_t> w_print, w_count, tokens, call, d_tokens;
-  std::vector<uint32_t> w_node;
+  std::vector<uint32_t> w_node, w_dcount;
   std::vector<uint64_t> n_tokens, tok_off, text_len;
   std::vector<uint8_t> joined, d_joined;
   std::vector<char> text;
@@ -105,6 +105,7 @@ int k2g_merge(k2g_merger *g, const k2g_merge_params *p, const k2g_hit_lists *a, 
   g->w_print.assign(slots, 0);
   g->w_count.assign(slots, 0);
   g->w_node.assign(slots, 0);
+  g->w_dcount.assign(slots, 0);
   g->tokens.assign(2 * slots, 0);
   g->joined.assign(slots, 0);
   g->n_tokens.assign(n + 1, 0);
@@ -116,7 +117,8 @@ int k2g_merge(k2g_merger *g, const k2g_merge_params *p, const k2g_hit_lists *a, 
     emu::blockIdx_.x = (unsigned) r;
     emu::threadIdx_.x = 0;
     k2g::k_merge(g->view, a->items, a->off, b->items, b->off, n, p->confidence, g->w_print.data(),
-                 g->w_count.data(), g->w_node.data(), g->tokens.data(), g->joined.data(), g->n_tokens.data(),
+                 g->w_count.data(), g->w_node.data(), g->w_dcount.data(), g->tokens.data(), g->joined.data(),
+                 g->n_tokens.data(),
                  g->text_len.data(), g->call.data());
   }
   const bool text = p->text != 0;
