@@ -475,6 +475,19 @@ int process_files(Ctx &c, const char *f1, const char *f2, Outputs &out) {
   bool finished = false;
   std::thread reader([&] {
     bool done = false;
+    // single reads: records parsed a chunk at a time (side by side on the
+    // host threads for plain FASTA), then cut into the reference's blocks
+    std::vector<Record> parsed;
+    size_t next_parsed = 0;
+    auto next_record = [&](Record &r) -> bool {
+      if (next_parsed == parsed.size()) {
+        parsed.clear();
+        next_parsed = 0;
+        if (!r1.next_records(64u << 20, o.num_threads, parsed)) return false;
+      }
+      r = std::move(parsed[next_parsed++]);
+      return true;
+    };
     while (!done) {
       std::vector<Block> blocks;
       {
@@ -496,10 +509,23 @@ int process_files(Ctx &c, const char *f1, const char *f2, Outputs &out) {
         b.u2.clear();
         bool ok;
         if (!o.paired_end_processing) {
-          ok = r1.load_block(3 * 1024 * 1024, b.a) > 0;
+          // load_block's rule (src/seqreader.h:101-119): records until
+          // 3 MiB of bases
+          size_t got = 0, k = 0;
+          while (got < 3 * 1024 * 1024) {
+            if (k == b.a.size()) b.a.emplace_back();
+            if (!next_record(b.a[k])) break;
+            got += b.a[k].seq.size();
+            k++;
+          }
+          b.a.resize(k);
+          ok = k > 0;
         } else if (!o.single_file_pairs) {
+          size_t n2 = 0;
+          std::thread mate2([&] { n2 = r2.load_batch(10000, b.b); });
           ok = r1.load_batch(10000, b.a) > 0;
-          if (ok) ok = r2.load_batch(10000, b.b) > 0;
+          mate2.join();
+          if (ok) ok = n2 > 0;
         } else {
           ok = r1.load_batch(20000, b.a) > 0;
         }

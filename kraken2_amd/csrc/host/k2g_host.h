@@ -28,6 +28,14 @@ class Reader {
   // src/seqreader.h:101-119 and 146-163
   size_t load_block(size_t bases, std::vector<Record> &out);
   size_t load_batch(size_t records, std::vector<Record> &out);
+  // The next records of about `bytes` of input, appended to out in input
+  // order: the same records next() returns.  Plain FASTA (every record at a
+  // line starting with '>', no line starting with '@' or '+') is cut at
+  // record starts and parsed on `threads` threads; anything else, and all
+  // input after it, is read by next().  Returns the records appended.
+  size_t next_records(size_t bytes, int threads, std::vector<Record> &out);
+  // Parses bytes held in memory (as if they were the whole input).
+  void open_memory(const char *p, size_t n);
 
  private:
   int get();
@@ -37,6 +45,7 @@ class Reader {
   std::vector<char> buf_;
   size_t pos_ = 0, end_ = 0;
   int pending_ = 0;
+  bool serial_ = false;  // next_records: no more parallel chunks
 };
 
 // taxo.k2d for names, ranks, parents and external ids (src/taxonomy.cc:195-238).

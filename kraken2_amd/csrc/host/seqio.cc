@@ -4,8 +4,10 @@
 // comment, sequence lines are joined until a line starting with '>', '+'
 // or '@', and a '+' switches to reading as many quality characters as there
 // are bases.  A line's single trailing '\r' is dropped.
+#include <algorithm>
 #include <cctype>
 #include <cstring>
+#include <thread>
 #include <cerrno>
 #include <fcntl.h>
 #include <unistd.h>
@@ -153,6 +155,122 @@ void Record::append_text(std::string &out, const char *suffix) const {
     out += quals;
   }
   out.push_back('\n');
+}
+
+}  // namespace k2host
+
+namespace k2host {
+
+void Reader::open_memory(const char *p, size_t n) {
+  fd_ = -1;
+  own_ = false;
+  eof_ = true;
+  buf_.assign(p, p + n);
+  pos_ = 0;
+  end_ = n;
+  pending_ = 0;
+}
+
+// First record start ('>' right after '\n') at or after i in d[0..n), or n.
+static size_t record_start(const char *d, size_t n, size_t i) {
+  while (i < n) {
+    const char *q = (const char *) memchr(d + i, '>', n - i);
+    if (!q) return n;
+    const size_t k = q - d;
+    if (k > 0 && d[k - 1] == '\n') return k;
+    i = k + 1;
+  }
+  return n;
+}
+
+// True if a line in d[0..n) starts with '@' or '+' (lines that end or
+// switch a record in the record rules above).
+static bool has_marker_line(const char *d, size_t n) {
+  for (size_t i = 1; i < n; i++) {
+    const char *q = (const char *) memchr(d + i, '\n', n - i);
+    if (!q) return false;
+    i = q - d + 1;
+    if (i < n && (d[i] == '@' || d[i] == '+')) return true;
+    i--;
+  }
+  return false;
+}
+
+size_t Reader::next_records(size_t bytes, int threads, std::vector<Record> &out) {
+  const size_t before = out.size();
+  auto serial = [&] {
+    size_t got = 0;
+    Record r;
+    while (got < bytes && next(r)) {
+      got += r.header.size() + r.seq.size() + r.quals.size() + 3;
+      out.push_back(std::move(r));
+      r = Record();
+    }
+    return out.size() - before;
+  };
+  if (serial_ || threads <= 1 || pending_) {
+    serial_ = true;
+    return serial();
+  }
+  // gather at least `bytes` unread bytes at the front of buf_
+  if (pos_) {
+    memmove(buf_.data(), buf_.data() + pos_, end_ - pos_);
+    end_ -= pos_;
+    pos_ = 0;
+  }
+  while (end_ < bytes && !eof_) {
+    if (buf_.size() < bytes + (1 << 20)) buf_.resize(bytes + (1 << 20));
+    ssize_t n;
+    do {
+      n = ::read(fd_, buf_.data() + end_, buf_.size() - end_);
+    } while (n < 0 && errno == EINTR);
+    if (n <= 0)
+      eof_ = true;
+    else
+      end_ += (size_t) n;
+  }
+  const char *d = buf_.data();
+  const size_t n = end_;
+  if (n == 0) return 0;
+  // cut after the last complete record (everything, at the end of input)
+  size_t cut = n;
+  if (!eof_) {
+    cut = 0;
+    for (size_t k = n; k-- > 1;)
+      if (d[k] == '>' && d[k - 1] == '\n') {
+        cut = k;
+        break;
+      }
+  }
+  if (d[0] != '>' || cut == 0 || has_marker_line(d, cut)) {
+    serial_ = true;  // not plain FASTA here: the record rules need one reader
+    return serial();
+  }
+  // split [0, cut) at record starts and parse the parts side by side
+  const int t = (int) std::min<size_t>((size_t) threads, std::max<size_t>(1, cut >> 12));
+  std::vector<size_t> b(t + 1);
+  b[0] = 0;
+  for (int i = 1; i < t; i++) b[i] = std::max(b[i - 1], record_start(d, cut, cut * i / t));
+  b[t] = cut;
+  std::vector<std::vector<Record>> parts(t);
+  auto parse = [&](int i) {
+    if (b[i + 1] <= b[i]) return;
+    Reader r;
+    r.open_memory(d + b[i], b[i + 1] - b[i]);
+    Record rec;
+    while (r.next(rec)) {
+      parts[i].push_back(std::move(rec));
+      rec = Record();
+    }
+  };
+  std::vector<std::thread> ts;
+  for (int i = 1; i < t; i++) ts.emplace_back(parse, i);
+  parse(0);
+  for (auto &th : ts) th.join();
+  for (auto &pt : parts)
+    for (auto &rec : pt) out.push_back(std::move(rec));
+  pos_ = cut;
+  return out.size() - before;
 }
 
 }  // namespace k2host

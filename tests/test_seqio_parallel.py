@@ -1,0 +1,68 @@
+"""The front-ends' chunked record reader (Reader::next_records: plain FASTA
+cut at record starts and parsed on several threads, everything else read
+serially) returns exactly the records of the serial reader (Reader::next),
+on the reference's edge cases: multi-line and CRLF records, comments, empty
+records, FASTQ, and lines starting with '@' or '+' inside FASTA."""
+import os
+import random
+import subprocess
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TOY = os.path.join(HERE, "golden", "toy")
+
+
+@pytest.fixture(scope="module")
+def checker():
+    r = subprocess.run(["make", "-C", os.path.join(HERE, "emu"), "reader_check"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    return os.path.join(HERE, "emu", "reader_check")
+
+
+def both(checker, path, chunk, threads):
+    r = subprocess.run([checker, path, str(chunk), str(threads)], capture_output=True)
+    assert r.returncode == 0
+    serial, chunked = r.stdout.split(b"----\n")
+    return serial, chunked
+
+
+def random_fasta(rng, n, marker_at=None):
+    out = []
+    for i in range(n):
+        head = f">r{i}" + (f" comment {i}" if rng.random() < 0.3 else "")
+        seq = "".join(rng.choice("ACGTNacgt") for _ in range(rng.randrange(0, 400)))
+        lines = [seq[j:j + 70] for j in range(0, len(seq), 70)] or [""]
+        eol = "\r\n" if rng.random() < 0.1 else "\n"
+        if i == marker_at:
+            lines.insert(1, "+not a quality line")
+        out.append(head + eol + eol.join(lines) + eol)
+    return "".join(out)
+
+
+@pytest.mark.parametrize("chunk,threads", [(1 << 26, 8), (3000, 4), (200, 3), (100000, 16)])
+def test_chunked_reader_matches_serial_on_fasta(checker, chunk, threads, tmp_path):
+    rng = random.Random(chunk)
+    p = tmp_path / "r.fa"
+    p.write_text(random_fasta(rng, 3000))
+    serial, chunked = both(checker, str(p), chunk, threads)
+    assert serial.count(b"\n") == 3000
+    assert chunked == serial
+
+
+@pytest.mark.parametrize("name", ["edge.fa", "reads_1.fq"])
+def test_chunked_reader_matches_serial_on_golden_inputs(checker, name):
+    for chunk, threads in ((1 << 20, 8), (500, 4)):
+        serial, chunked = both(checker, os.path.join(TOY, name), chunk, threads)
+        assert chunked == serial
+
+
+def test_chunked_reader_falls_back_at_marker_lines(checker, tmp_path):
+    """A '+' line inside FASTA switches the record rules to quality reading;
+    the chunked reader must hand over to the serial one there."""
+    rng = random.Random(3)
+    p = tmp_path / "m.fa"
+    p.write_text(random_fasta(rng, 2000, marker_at=1500) + "@q1\nACGT\n+\nIIII\n>last\nAC\n")
+    for chunk, threads in ((4000, 4), (1 << 20, 8)):
+        serial, chunked = both(checker, str(p), chunk, threads)
+        assert chunked == serial
