@@ -10,6 +10,7 @@
 #include <thread>
 #include <cerrno>
 #include <fcntl.h>
+#include <sys/stat.h>
 #include <unistd.h>
 
 #include "k2g_host.h"
@@ -218,8 +219,44 @@ size_t Reader::next_records(size_t bytes, int threads, std::vector<Record> &out)
     end_ -= pos_;
     pos_ = 0;
   }
+  if (buf_.size() < bytes + (1 << 20)) buf_.resize(bytes + (1 << 20));
+  struct stat st;
+  if (end_ < bytes && !eof_ && fstat(fd_, &st) == 0 && S_ISREG(st.st_mode)) {
+    // a regular file: the chunk is copied in by several preads side by side,
+    // then the descriptor's offset is moved past it
+    const off_t at = lseek(fd_, 0, SEEK_CUR);
+    if (at >= 0) {
+      const size_t want = std::min<size_t>(bytes - end_, st.st_size > at ? (size_t) (st.st_size - at) : 0);
+      const int t = (int) std::max<size_t>(1, std::min<size_t>((size_t) threads, want >> 20));
+      std::vector<ssize_t> got(t, 0);
+      auto piece = [&](int i) {
+        const size_t lo = want * i / t, hi = want * (i + 1) / t;
+        size_t done = 0;
+        while (lo + done < hi) {
+          const ssize_t n = pread(fd_, buf_.data() + end_ + lo + done, hi - lo - done, at + (off_t) (lo + done));
+          if (n < 0 && errno == EINTR) continue;
+          if (n <= 0) break;
+          done += (size_t) n;
+        }
+        got[i] = (ssize_t) done;
+      };
+      std::vector<std::thread> ts;
+      for (int i = 1; i < t; i++) ts.emplace_back(piece, i);
+      piece(0);
+      for (auto &th : ts) th.join();
+      size_t total = 0;
+      bool whole = true;
+      for (int i = 0; i < t && whole; i++) {
+        const size_t lo = want * i / t, hi = want * (i + 1) / t;
+        total += (size_t) got[i];
+        whole = (size_t) got[i] == hi - lo;
+      }
+      end_ += total;
+      lseek(fd_, at + (off_t) total, SEEK_SET);
+      if (!whole) eof_ = true;  // the file ended (or shrank) inside the chunk
+    }
+  }
   while (end_ < bytes && !eof_) {
-    if (buf_.size() < bytes + (1 << 20)) buf_.resize(bytes + (1 << 20));
     ssize_t n;
     do {
       n = ::read(fd_, buf_.data() + end_, buf_.size() - end_);

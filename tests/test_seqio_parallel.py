@@ -66,3 +66,19 @@ def test_chunked_reader_falls_back_at_marker_lines(checker, tmp_path):
     for chunk, threads in ((4000, 4), (1 << 20, 8)):
         serial, chunked = both(checker, str(p), chunk, threads)
         assert chunked == serial
+
+
+def test_chunked_reader_parallel_reads_of_a_large_file(checker, tmp_path):
+    """Chunks of a few MB are copied in by several preads side by side."""
+    rng = random.Random(11)
+    recs = []
+    for i in range(30000):
+        L = rng.randrange(50, 500)
+        recs.append(f">big{i}\n" + "".join(rng.choice("ACGT") for _ in range(L)) + "\n")
+    p = tmp_path / "big.fa"
+    p.write_text("".join(recs))
+    assert p.stat().st_size > 8_000_000
+    for chunk, threads in ((4 << 20, 8), (3 << 20, 3)):
+        serial, chunked = both(checker, str(p), chunk, threads)
+        assert serial.count(b"\n") == 30000
+        assert chunked == serial
