@@ -477,15 +477,15 @@ int process_files(Ctx &c, const char *f1, const char *f2, Outputs &out) {
     bool done = false;
     // single reads: records parsed a chunk at a time (side by side on the
     // host threads for plain FASTA), then cut into the reference's blocks
-    std::vector<Record> parsed;
+    std::vector<Record> parsed, used_recs;  // records of consumed blocks, buffers reused
     size_t next_parsed = 0;
-    auto next_record = [&](Record &r) -> bool {
+    auto next_record = [&](std::vector<Record> &into) -> bool {
       if (next_parsed == parsed.size()) {
         parsed.clear();
         next_parsed = 0;
-        if (!r1.next_records(64u << 20, o.num_threads, parsed)) return false;
+        if (!r1.next_records(64u << 20, o.num_threads, parsed, &used_recs)) return false;
       }
-      r = std::move(parsed[next_parsed++]);
+      into.push_back(std::move(parsed[next_parsed++]));
       return true;
     };
     while (!done) {
@@ -511,15 +511,14 @@ int process_files(Ctx &c, const char *f1, const char *f2, Outputs &out) {
         if (!o.paired_end_processing) {
           // load_block's rule (src/seqreader.h:101-119): records until
           // 3 MiB of bases
-          size_t got = 0, k = 0;
+          for (auto &r : b.a) used_recs.push_back(std::move(r));
+          b.a.clear();
+          size_t got = 0;
           while (got < 3 * 1024 * 1024) {
-            if (k == b.a.size()) b.a.emplace_back();
-            if (!next_record(b.a[k])) break;
-            got += b.a[k].seq.size();
-            k++;
+            if (!next_record(b.a)) break;
+            got += b.a.back().seq.size();
           }
-          b.a.resize(k);
-          ok = k > 0;
+          ok = !b.a.empty();
         } else if (!o.single_file_pairs) {
           size_t n2 = 0;
           std::thread mate2([&] { n2 = r2.load_batch(10000, b.b); });

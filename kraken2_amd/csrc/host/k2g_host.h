@@ -33,7 +33,9 @@ class Reader {
   // line starting with '>', no line starting with '@' or '+') is cut at
   // record starts and parsed on `threads` threads; anything else, and all
   // input after it, is read by next().  Returns the records appended.
-  size_t next_records(size_t bytes, int threads, std::vector<Record> &out);
+  // pool (may be null): used records whose buffers the parse reuses.
+  size_t next_records(size_t bytes, int threads, std::vector<Record> &out,
+                      std::vector<Record> *pool = nullptr);
   // Parses bytes held in memory (as if they were the whole input).
   void open_memory(const char *p, size_t n);
 

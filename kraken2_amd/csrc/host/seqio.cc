@@ -197,7 +197,7 @@ static bool has_marker_line(const char *d, size_t n) {
   return false;
 }
 
-size_t Reader::next_records(size_t bytes, int threads, std::vector<Record> &out) {
+size_t Reader::next_records(size_t bytes, int threads, std::vector<Record> &out, std::vector<Record> *pool) {
   const size_t before = out.size();
   auto serial = [&] {
     size_t got = 0;
@@ -289,15 +289,30 @@ size_t Reader::next_records(size_t bytes, int threads, std::vector<Record> &out)
   b[0] = 0;
   for (int i = 1; i < t; i++) b[i] = std::max(b[i - 1], record_start(d, cut, cut * i / t));
   b[t] = cut;
-  std::vector<std::vector<Record>> parts(t);
+  std::vector<std::vector<Record>> parts(t), spare(t);
+  if (pool) {  // hand each part a share of the used records
+    const size_t share = pool->size() / t;
+    for (int i = 0; i < t; i++)
+      for (size_t k = 0; k < share; k++) {
+        spare[i].push_back(std::move(pool->back()));
+        pool->pop_back();
+      }
+  }
   auto parse = [&](int i) {
     if (b[i + 1] <= b[i]) return;
     Reader r;
     r.open_memory(d + b[i], b[i + 1] - b[i]);
-    Record rec;
-    while (r.next(rec)) {
+    for (;;) {
+      Record rec;
+      if (!spare[i].empty()) {
+        rec = std::move(spare[i].back());
+        spare[i].pop_back();
+      }
+      if (!r.next(rec)) {
+        spare[i].push_back(std::move(rec));
+        break;
+      }
       parts[i].push_back(std::move(rec));
-      rec = Record();
     }
   };
   std::vector<std::thread> ts;
@@ -306,6 +321,9 @@ size_t Reader::next_records(size_t bytes, int threads, std::vector<Record> &out)
   for (auto &th : ts) th.join();
   for (auto &pt : parts)
     for (auto &rec : pt) out.push_back(std::move(rec));
+  if (pool)
+    for (auto &sp : spare)
+      for (auto &rec : sp) pool->push_back(std::move(rec));
   pos_ = cut;
   return out.size() - before;
 }

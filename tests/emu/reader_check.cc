@@ -1,6 +1,7 @@
 // Prints every record of a file twice over: once through Reader::next and
-// once through Reader::next_records with the given chunk size and threads,
-// so a test can compare the two record streams.
+// once through Reader::next_records with the given chunk size and threads
+// (each chunk's records are handed back as the pool the next chunk parses
+// into), so a test can compare the two record streams.
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -25,9 +26,12 @@ int main(int argc, char **argv) {
   printf("----\n");
   k2host::Reader r;
   if (!r.open(argv[1])) return 66;
-  std::vector<k2host::Record> out;
-  while (r.next_records(chunk, threads, out)) {
-    for (auto &rec : out) put(rec);
+  std::vector<k2host::Record> out, pool;  // printed records go back to the pool for reuse
+  while (r.next_records(chunk, threads, out, &pool)) {
+    for (auto &rec : out) {
+      put(rec);
+      pool.push_back(std::move(rec));
+    }
     out.clear();
   }
   return 0;
