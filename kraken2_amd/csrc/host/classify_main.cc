@@ -220,6 +220,8 @@ struct Ctx {
   bool timing = false;
   double t_wait = 0, t_pack = 0, t_gpu = 0, t_format = 0, t_count = 0;
   double t_alloc = 0, t_offsets = 0;  // parts of t_pack
+  // reader thread: parsing chunks, all of its work, and waiting for queue room
+  double t_rparse = 0, t_rbusy = 0, t_rfull = 0;
   // packed bases of one batch (pinned, kept across batches) and offsets
   char *seq = nullptr;
   size_t seq_cap = 0;
@@ -483,12 +485,16 @@ int process_files(Ctx &c, const char *f1, const char *f2, Outputs &out) {
       if (next_parsed == parsed.size()) {
         parsed.clear();
         next_parsed = 0;
-        if (!r1.next_records(64u << 20, o.num_threads, parsed, &used_recs)) return false;
+        const double t0 = now_s();
+        const size_t got = r1.next_records(64u << 20, o.num_threads, parsed, &used_recs);
+        c.t_rparse += now_s() - t0;
+        if (!got) return false;
       }
       into.push_back(std::move(parsed[next_parsed++]));
       return true;
     };
     while (!done) {
+      const double t_batch = now_s();
       std::vector<Block> blocks;
       {
         std::lock_guard<std::mutex> lk(mu);
@@ -538,7 +544,10 @@ int process_files(Ctx &c, const char *f1, const char *f2, Outputs &out) {
       }
       blocks.resize(used);
       std::unique_lock<std::mutex> lk(mu);
+      const double tw = now_s();
+      c.t_rbusy += tw - t_batch;
       cv.wait(lk, [&] { return ready.size() < 2; });
+      c.t_rfull += now_s() - tw;
       if (!blocks.empty()) ready.push_back(std::move(blocks));
       if (done) finished = true;
       cv.notify_all();
@@ -626,9 +635,10 @@ int main(int argc, char **argv) {
   if (c.timing)
     fprintf(stderr,
             "  timing: input wait %.3fs, packing %.3fs (buffer %.3fs, offsets %.3fs), GPU %.3fs, "
-            "format+write %.3fs, counters %.3fs, outside the phases %.3fs\n",
+            "format+write %.3fs, counters %.3fs, outside the phases %.3fs; reader thread: busy %.3fs "
+            "(chunk reading and parsing %.3fs), waiting for queue room %.3fs\n",
             c.t_wait, c.t_pack, c.t_alloc, c.t_offsets, c.t_gpu, c.t_format, c.t_count,
-            secs - c.t_wait - c.t_pack - c.t_gpu - c.t_format - c.t_count);
+            secs - c.t_wait - c.t_pack - c.t_gpu - c.t_format - c.t_count, c.t_rbusy, c.t_rparse, c.t_rfull);
   if (!o.report_filename.empty()) {
     if (!k2host::report_kraken_style(o.report_filename, o.report_zero_counts, o.report_kmer_data,
                                      c.tax, c.counters, c.stats.total_sequences,

@@ -279,7 +279,7 @@ size_t Reader::next_records(size_t bytes, int threads, std::vector<Record> &out,
         break;
       }
   }
-  if (d[0] != '>' || cut == 0 || has_marker_line(d, cut)) {
+  if (d[0] != '>' || cut == 0) {
     serial_ = true;  // not plain FASTA here: the record rules need one reader
     return serial();
   }
@@ -298,8 +298,15 @@ size_t Reader::next_records(size_t bytes, int threads, std::vector<Record> &out,
         pool->pop_back();
       }
   }
+  std::vector<char> marker(t, 0);
   auto parse = [&](int i) {
     if (b[i + 1] <= b[i]) return;
+    // every line start of the chunk lies in exactly one part (parts begin at
+    // record starts)
+    if (has_marker_line(d + b[i], b[i + 1] - b[i])) {
+      marker[i] = 1;
+      return;
+    }
     Reader r;
     r.open_memory(d + b[i], b[i + 1] - b[i]);
     for (;;) {
@@ -319,6 +326,17 @@ size_t Reader::next_records(size_t bytes, int threads, std::vector<Record> &out,
   for (int i = 1; i < t; i++) ts.emplace_back(parse, i);
   parse(0);
   for (auto &th : ts) th.join();
+  if (std::find(marker.begin(), marker.end(), 1) != marker.end()) {
+    // a line starts with '@' or '+': the record rules need one reader from
+    // this chunk on (the parsed parts are dropped, their buffers kept)
+    if (pool)
+      for (int i = 0; i < t; i++) {
+        for (auto &rec : parts[i]) pool->push_back(std::move(rec));
+        for (auto &rec : spare[i]) pool->push_back(std::move(rec));
+      }
+    serial_ = true;
+    return serial();
+  }
   for (auto &pt : parts)
     for (auto &rec : pt) out.push_back(std::move(rec));
   if (pool)
