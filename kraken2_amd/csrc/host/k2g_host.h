@@ -36,12 +36,15 @@ class Reader {
   // pool (may be null): used records whose buffers the parse reuses.
   size_t next_records(size_t bytes, int threads, std::vector<Record> &out,
                       std::vector<Record> *pool = nullptr);
-  // Parses bytes held in memory (as if they were the whole input).
+  // Parses bytes held in memory (as if they were the whole input); they must
+  // stay valid while the reader is used.
   void open_memory(const char *p, size_t n);
 
  private:
   int get();
   void line_into(std::string &dst, bool append, int *last);
+  const char *base() const { return ext_ ? ext_ : buf_.data(); }
+  const char *ext_ = nullptr;  // open_memory: the bytes, read in place
   int fd_ = -1;
   bool own_ = false, eof_ = false;
   std::vector<char> buf_;

@@ -47,7 +47,7 @@ int Reader::get() {
     pos_ = 0;
     end_ = (size_t) n;
   }
-  return (unsigned char) buf_[pos_++];
+  return (unsigned char) base()[pos_++];
 }
 
 // Rest of the current line into dst.
@@ -57,12 +57,12 @@ void Reader::line_into(std::string &dst, bool append, int *last) {
   for (;;) {
     // fast path: copy up to the next newline inside the buffer
     if (pos_ < end_) {
-      const char *b = buf_.data() + pos_;
-      const char *e = buf_.data() + end_;
+      const char *b = base() + pos_;
+      const char *e = base() + end_;
       const char *nl = (const char *) memchr(b, '\n', e - b);
       if (nl) {
         dst.append(b, nl - b);
-        pos_ = nl - buf_.data() + 1;
+        pos_ = nl - base() + 1;
         c = '\n';
         break;
       }
@@ -166,7 +166,8 @@ void Reader::open_memory(const char *p, size_t n) {
   fd_ = -1;
   own_ = false;
   eof_ = true;
-  buf_.assign(p, p + n);
+  buf_.clear();
+  ext_ = p;  // read in place
   pos_ = 0;
   end_ = n;
   pending_ = 0;
