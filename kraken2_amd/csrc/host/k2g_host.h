@@ -51,11 +51,6 @@ class Reader {
   size_t pos_ = 0, end_ = 0;
   int pending_ = 0;
   bool serial_ = false;  // next_records: no more parallel chunks
-  size_t parse_chunk(const char *d, size_t cut, int threads, std::vector<Record> &out,
-                     std::vector<Record> *pool);
-  const char *map_ = nullptr;  // next_records on a regular file: the file mapped
-  size_t map_size_ = 0, map_pos_ = 0;
-  bool map_tried_ = false;
 };
 
 // taxo.k2d for names, ranks, parents and external ids (src/taxonomy.cc:195-238).

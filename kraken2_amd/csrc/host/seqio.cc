@@ -6,12 +6,10 @@
 // are bases.  A line's single trailing '\r' is dropped.
 #include <algorithm>
 #include <cctype>
-#include <cstdint>
 #include <cstring>
 #include <thread>
 #include <cerrno>
 #include <fcntl.h>
-#include <sys/mman.h>
 #include <sys/stat.h>
 #include <unistd.h>
 
@@ -32,7 +30,6 @@ bool Reader::open(const char *path) {
 }
 
 Reader::~Reader() {
-  if (map_) munmap((void *) map_, map_size_);
   if (own_ && fd_ > 2) ::close(fd_);
 }
 
@@ -201,73 +198,6 @@ static bool has_marker_line(const char *d, size_t n) {
   return false;
 }
 
-// Parses the complete plain-FASTA records of d[0..cut) on up to `threads`
-// threads, appending them to out in order; SIZE_MAX (nothing appended) if a
-// line starts with '@' or '+'.
-size_t Reader::parse_chunk(const char *d, size_t cut, int threads, std::vector<Record> &out,
-                           std::vector<Record> *pool) {
-  const size_t before = out.size();
-  // split [0, cut) at record starts and parse the parts side by side
-  const int t = (int) std::min<size_t>((size_t) threads, std::max<size_t>(1, cut >> 12));
-  std::vector<size_t> b(t + 1);
-  b[0] = 0;
-  for (int i = 1; i < t; i++) b[i] = std::max(b[i - 1], record_start(d, cut, cut * i / t));
-  b[t] = cut;
-  std::vector<std::vector<Record>> parts(t), spare(t);
-  if (pool) {  // hand each part a share of the used records
-    const size_t share = pool->size() / t;
-    for (int i = 0; i < t; i++)
-      for (size_t k = 0; k < share; k++) {
-        spare[i].push_back(std::move(pool->back()));
-        pool->pop_back();
-      }
-  }
-  std::vector<char> marker(t, 0);
-  auto parse = [&](int i) {
-    if (b[i + 1] <= b[i]) return;
-    // every line start of the chunk lies in exactly one part (parts begin at
-    // record starts)
-    if (has_marker_line(d + b[i], b[i + 1] - b[i])) {
-      marker[i] = 1;
-      return;
-    }
-    Reader r;
-    r.open_memory(d + b[i], b[i + 1] - b[i]);
-    for (;;) {
-      Record rec;
-      if (!spare[i].empty()) {
-        rec = std::move(spare[i].back());
-        spare[i].pop_back();
-      }
-      if (!r.next(rec)) {
-        spare[i].push_back(std::move(rec));
-        break;
-      }
-      parts[i].push_back(std::move(rec));
-    }
-  };
-  std::vector<std::thread> ts;
-  for (int i = 1; i < t; i++) ts.emplace_back(parse, i);
-  parse(0);
-  for (auto &th : ts) th.join();
-  if (std::find(marker.begin(), marker.end(), 1) != marker.end()) {
-    // a line starts with '@' or '+': the record rules need one reader from
-    // this chunk on (the parsed parts are dropped, their buffers kept)
-    if (pool)
-      for (int i = 0; i < t; i++) {
-        for (auto &rec : parts[i]) pool->push_back(std::move(rec));
-        for (auto &rec : spare[i]) pool->push_back(std::move(rec));
-      }
-    return SIZE_MAX;
-  }
-  for (auto &pt : parts)
-    for (auto &rec : pt) out.push_back(std::move(rec));
-  if (pool)
-    for (auto &sp : spare)
-      for (auto &rec : sp) pool->push_back(std::move(rec));
-  return out.size() - before;
-}
-
 size_t Reader::next_records(size_t bytes, int threads, std::vector<Record> &out, std::vector<Record> *pool) {
   const size_t before = out.size();
   auto serial = [&] {
@@ -283,46 +213,6 @@ size_t Reader::next_records(size_t bytes, int threads, std::vector<Record> &out,
   if (serial_ || threads <= 1 || pending_) {
     serial_ = true;
     return serial();
-  }
-  // a regular file read from its start is mapped: chunks are parsed where
-  // they lie, one copy fewer than reading them into the buffer
-  if (!map_ && !map_tried_) {
-    map_tried_ = true;
-    struct stat st;
-    if (pos_ == end_ && !eof_ && fstat(fd_, &st) == 0 && S_ISREG(st.st_mode) && st.st_size > 0 &&
-        lseek(fd_, 0, SEEK_CUR) == 0) {
-      void *m = mmap(nullptr, (size_t) st.st_size, PROT_READ, MAP_PRIVATE, fd_, 0);
-      if (m != MAP_FAILED) {
-        map_ = (const char *) m;
-        map_size_ = (size_t) st.st_size;
-        map_pos_ = 0;
-      }
-    }
-  }
-  if (map_) {
-    const char *d = map_ + map_pos_;
-    const size_t n = std::min(bytes, map_size_ - map_pos_);
-    if (n == 0) return 0;
-    const bool last = map_pos_ + n == map_size_;
-    size_t cut = n;
-    if (!last) {
-      cut = 0;
-      for (size_t k = n; k-- > 1;)
-        if (d[k] == '>' && d[k - 1] == '\n') {
-          cut = k;
-          break;
-        }
-    }
-    const size_t got = d[0] == '>' && cut ? parse_chunk(d, cut, threads, out, pool) : 0;
-    if (got == SIZE_MAX || !(d[0] == '>' && cut)) {
-      // not plain FASTA from here: the serial reader continues at this chunk
-      serial_ = true;
-      lseek(fd_, (off_t) map_pos_, SEEK_SET);
-      pos_ = end_ = 0;
-      return serial();
-    }
-    map_pos_ += cut;
-    return out.size() - before;
   }
   // gather at least `bytes` unread bytes at the front of buf_
   if (pos_) {
@@ -394,11 +284,65 @@ size_t Reader::next_records(size_t bytes, int threads, std::vector<Record> &out,
     serial_ = true;  // not plain FASTA here: the record rules need one reader
     return serial();
   }
-  const size_t got = parse_chunk(d, cut, threads, out, pool);
-  if (got == SIZE_MAX) {
+  // split [0, cut) at record starts and parse the parts side by side
+  const int t = (int) std::min<size_t>((size_t) threads, std::max<size_t>(1, cut >> 12));
+  std::vector<size_t> b(t + 1);
+  b[0] = 0;
+  for (int i = 1; i < t; i++) b[i] = std::max(b[i - 1], record_start(d, cut, cut * i / t));
+  b[t] = cut;
+  std::vector<std::vector<Record>> parts(t), spare(t);
+  if (pool) {  // hand each part a share of the used records
+    const size_t share = pool->size() / t;
+    for (int i = 0; i < t; i++)
+      for (size_t k = 0; k < share; k++) {
+        spare[i].push_back(std::move(pool->back()));
+        pool->pop_back();
+      }
+  }
+  std::vector<char> marker(t, 0);
+  auto parse = [&](int i) {
+    if (b[i + 1] <= b[i]) return;
+    // every line start of the chunk lies in exactly one part (parts begin at
+    // record starts)
+    if (has_marker_line(d + b[i], b[i + 1] - b[i])) {
+      marker[i] = 1;
+      return;
+    }
+    Reader r;
+    r.open_memory(d + b[i], b[i + 1] - b[i]);
+    for (;;) {
+      Record rec;
+      if (!spare[i].empty()) {
+        rec = std::move(spare[i].back());
+        spare[i].pop_back();
+      }
+      if (!r.next(rec)) {
+        spare[i].push_back(std::move(rec));
+        break;
+      }
+      parts[i].push_back(std::move(rec));
+    }
+  };
+  std::vector<std::thread> ts;
+  for (int i = 1; i < t; i++) ts.emplace_back(parse, i);
+  parse(0);
+  for (auto &th : ts) th.join();
+  if (std::find(marker.begin(), marker.end(), 1) != marker.end()) {
+    // a line starts with '@' or '+': the record rules need one reader from
+    // this chunk on (the parsed parts are dropped, their buffers kept)
+    if (pool)
+      for (int i = 0; i < t; i++) {
+        for (auto &rec : parts[i]) pool->push_back(std::move(rec));
+        for (auto &rec : spare[i]) pool->push_back(std::move(rec));
+      }
     serial_ = true;
     return serial();
   }
+  for (auto &pt : parts)
+    for (auto &rec : pt) out.push_back(std::move(rec));
+  if (pool)
+    for (auto &sp : spare)
+      for (auto &rec : sp) pool->push_back(std::move(rec));
   pos_ = cut;
   return out.size() - before;
 }
