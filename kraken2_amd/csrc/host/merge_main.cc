@@ -522,12 +522,14 @@ int main(int argc, char **argv) {
     t0 = now_s();
     t_parse += t0 - t1;
     int rc = 0;
+    std::string gpu_error;  // the library's last error is per thread
     k2g_merge_params p{conf, 1};
     std::thread gpu([&] {
       const Batch &bt = *sl.batch;
       k2g_hit_list_text a{bt.buf1.data(), bt.buf1.size, sl.field_a.p, sl.len_a.p, sl.off_a.p, bt.n};
       k2g_hit_list_text bb{bt.buf2.data(), bt.buf2.size, sl.field_b.p, sl.len_b.p, sl.off_b.p, bt.n};
       rc = k2g_merge_text(mg[j], &p, &a, &bb, &sl.res);
+      if (rc) gpu_error = k2g_last_error();
     });
     if (pending) {
       print(*pending);
@@ -537,7 +539,7 @@ int main(int argc, char **argv) {
     t_format += t1 - t0;
     gpu.join();
     t_gpu_wait += now_s() - t1;
-    if (rc) die(rc, k2g_last_error());
+    if (rc) die(rc, gpu_error);
     t_kernel += sl.res.kernel_ms / 1e3;
     pending = &sl;
   }
@@ -554,10 +556,10 @@ int main(int argc, char **argv) {
     wcv.notify_all();
   }
   writer.join();
-  if (fclose(out) != 0) die(74, std::string("unable to write ") + out_name);
+  if (ferror(out) || fclose(out) != 0) die(74, std::string("unable to write ") + out_name);
   if (hdr) {
     fprintf(hdr, "%llu\n", (unsigned long long) total_seqs);
-    if (fclose(hdr) != 0) die(74, std::string("unable to write ") + headers);
+    if (ferror(hdr) || fclose(hdr) != 0) die(74, std::string("unable to write ") + headers);
   }
   if (report) {
     k2host::TaxonCounters counters;
