@@ -808,7 +808,7 @@ def main():
         "vs_baseline": None,
         "dtype": "u64/u32 integer",
         "data": "synthetic (seeded generators; table built on the GPU)" if work["kind"] == "gpu"
-                else "synthetic (seeded generator; table built by the deterministic CPU build)",
+                else "synthetic (seeded generator; table built by the GPU build front-ends, k2g_estimate_capacity and k2g_build_db)",
         "config": {"workload": work["desc"] + ("" if a.scale == 1.0 else f" (scaled x{a.scale})"), "fragments": n,
                    "bases": total_bases, "table_cells": int(db.info.capacity), "parallelism": "single GPU"},
         "mbp_per_s": round(total_bases / batched_s / 1e6, 1),
