@@ -315,13 +315,14 @@ int k2g_merge_close(k2g_merger *m);
  * fifth tab-separated field of its kraken line, text[field[i] .. field[i] +
  * len[i]), with the line's newline when it has one; off[i+1] - off[i] must
  * be that field's ':' count (the reference's slot count, :431-435; checked
- * on the device). */
+ * on the device).  off may be NULL: the counts are then taken on the
+ * device. */
 typedef struct {
   const char *text;
   uint64_t text_bytes;
   const uint64_t *field; /* n */
   const uint32_t *len;   /* n */
-  const uint64_t *off;   /* n + 1, off[0] = 0 */
+  const uint64_t *off;   /* n + 1, off[0] = 0; or NULL (counted on the device) */
   uint64_t n;
 } k2g_hit_list_text;
 

@@ -205,12 +205,6 @@ bool get_fields(const char *p, const char *end, Fields &fl) {
   return k == 5;
 }
 
-uint64_t count_items(const char *s, size_t len) {
-  uint64_t n = 0;
-  for (size_t i = 0; i < len; i++) n += s[i] == ':';
-  return n;
-}
-
 struct Batch {  // lines of both inputs
   Bytes buf1{true}, buf2{true};
   std::vector<size_t> st1, st2;
@@ -237,7 +231,7 @@ struct Pinned {
 struct Slot {  // one batch on its way through parse, GPU merge and print
   std::unique_ptr<Batch> batch;
   std::vector<Fields> fa, fb;
-  Pinned<uint64_t> off_a, off_b, field_a, field_b;  // item offsets; hit-list field offsets
+  Pinned<uint64_t> field_a, field_b;  // hit-list field offsets
   Pinned<uint32_t> len_a, len_b;                     // hit-list field lengths
   k2g_merge_results res{};
   std::vector<Bytes> text;          // printed lines, one buffer per print thread
@@ -245,18 +239,16 @@ struct Slot {  // one batch on its way through parse, GPU merge and print
   bool writing = false;             // queued for the writer thread
 };
 
-// Fields of every line pair and the ':' count of each hit list, on the
-// OpenMP threads (the hit lists are tokenized on the GPU).
+// Fields of every line pair, on the OpenMP threads (the hit lists are
+// counted and tokenized on the GPU).
 void parse_batch(Slot &sl, int threads) {
   const Batch &b = *sl.batch;
   const size_t n = b.n;
   sl.fa.resize(n);
   sl.fb.resize(n);
-  for (auto *q : {&sl.off_a, &sl.off_b, &sl.field_a, &sl.field_b}) q->reserve(n + 1);
+  for (auto *q : {&sl.field_a, &sl.field_b}) q->reserve(n + 1);
   sl.len_a.reserve(n + 1);
   sl.len_b.reserve(n + 1);
-  uint64_t *oa = sl.off_a.p, *ob = sl.off_b.p;
-  oa[0] = ob[0] = 0;
   bool bad = false;
 #pragma omp parallel for schedule(static) reduction(|| : bad) num_threads(threads)
   for (size_t i = 0; i < n; i++) {
@@ -265,18 +257,12 @@ void parse_batch(Slot &sl, int threads) {
       bad = true;
       continue;
     }
-    oa[i + 1] = count_items(sl.fa[i].f[4], sl.fa[i].len[4]);
-    ob[i + 1] = count_items(sl.fb[i].f[4], sl.fb[i].len[4]);
     sl.field_a.p[i] = sl.fa[i].f[4] - b.buf1.data();
     sl.field_b.p[i] = sl.fb[i].f[4] - b.buf2.data();
     sl.len_a.p[i] = sl.fa[i].len[4];
     sl.len_b.p[i] = sl.fb[i].len[4];
   }
   if (bad) die(65, "malformed kraken output line (fewer than five fields)");
-  for (size_t i = 0; i < n; i++) {
-    oa[i + 1] += oa[i];
-    ob[i + 1] += ob[i];
-  }
 }
 
 // Hands a printed batch's line buffers back to the reader.
@@ -526,8 +512,8 @@ int main(int argc, char **argv) {
     k2g_merge_params p{conf, 1};
     std::thread gpu([&] {
       const Batch &bt = *sl.batch;
-      k2g_hit_list_text a{bt.buf1.data(), bt.buf1.size, sl.field_a.p, sl.len_a.p, sl.off_a.p, bt.n};
-      k2g_hit_list_text bb{bt.buf2.data(), bt.buf2.size, sl.field_b.p, sl.len_b.p, sl.off_b.p, bt.n};
+      k2g_hit_list_text a{bt.buf1.data(), bt.buf1.size, sl.field_a.p, sl.len_a.p, nullptr, bt.n};
+      k2g_hit_list_text bb{bt.buf2.data(), bt.buf2.size, sl.field_b.p, sl.len_b.p, nullptr, bt.n};
       rc = k2g_merge_text(mg[j], &p, &a, &bb, &sl.res);
       if (rc) gpu_error = k2g_last_error();
     });

@@ -210,15 +210,16 @@ static int run_merge(k2g_merger *g, const k2g_merge_params *p, uint64_t n, uint6
 
 static int check_text(const k2g_hit_list_text *h, const char *which) {
   const std::string w = std::string(" of hit-list text ") + which;
-  if (!h->off || (h->n && (!h->field || !h->len))) return fail(K2G_EUSAGE, "null arrays" + w);
-  if (h->off[0] != 0) return fail(K2G_EUSAGE, "item offsets" + w + " must start at 0");
+  if (h->n && (!h->field || !h->len)) return fail(K2G_EUSAGE, "null arrays" + w);
+  if (h->off && h->off[0] != 0) return fail(K2G_EUSAGE, "item offsets" + w + " must start at 0");
   for (uint64_t i = 0; i < h->n; i++) {
-    if (h->off[i + 1] < h->off[i]) return fail(K2G_EUSAGE, "item offsets" + w + " must not decrease");
+    if (h->off && h->off[i + 1] < h->off[i]) return fail(K2G_EUSAGE, "item offsets" + w + " must not decrease");
     if (h->field[i] > h->text_bytes || h->len[i] > h->text_bytes - h->field[i])
       return fail(K2G_EUSAGE, "a field" + w + " lies outside the text");
   }
   if (h->text_bytes && !h->text) return fail(K2G_EUSAGE, "null text" + w);
-  if (h->off[h->n] > (uint64_t) UINT32_MAX) return fail(K2G_EUSAGE, "hit lists" + w + " hold 2^32 or more items");
+  if (h->off && h->off[h->n] > (uint64_t) UINT32_MAX)
+    return fail(K2G_EUSAGE, "hit lists" + w + " hold 2^32 or more items");
   return K2G_OK;
 }
 
@@ -229,10 +230,11 @@ extern "C" int k2g_merge_text(k2g_merger *g, const k2g_merge_params *p, const k2
   int rc = check_text(a, "a");
   if (!rc) rc = check_text(b, "b");
   if (rc) return rc;
-  const uint64_t n = a->n, na = a->off[n], nb = b->off[n];
-  for (uint64_t i = 0; i < n; i++)
-    if (a->off[i + 1] - a->off[i] + b->off[i + 1] - b->off[i] > (uint64_t) UINT32_MAX)
-      return fail(K2G_EUSAGE, "a read's hit lists hold 2^32 or more items");
+  const uint64_t n = a->n;
+  if (a->off && b->off)
+    for (uint64_t i = 0; i < n; i++)
+      if (a->off[i + 1] - a->off[i] + b->off[i + 1] - b->off[i] > (uint64_t) UINT32_MAX)
+        return fail(K2G_EUSAGE, "a read's hit lists hold 2^32 or more items");
   hipStream_t s = g->stream;
   HIP_TRY(upload(g->a_text, a->text, a->text_bytes, s));
   HIP_TRY(upload(g->b_text, b->text, b->text_bytes, s));
@@ -240,14 +242,38 @@ extern "C" int k2g_merge_text(k2g_merger *g, const k2g_merge_params *p, const k2
   HIP_TRY(upload(g->b_field, b->field, n, s));
   HIP_TRY(upload(g->a_len, a->len, n, s));
   HIP_TRY(upload(g->b_len, b->len, n, s));
-  HIP_TRY(upload(g->a_off, a->off, n + 1, s));
-  HIP_TRY(upload(g->b_off, b->off, n + 1, s));
+  // item offsets: the caller's, or the fields' ':' counts scanned on the device
+  const uint32_t blocks = (uint32_t) ((n + 255) / 256);
+  uint64_t tot[2] = {0, 0};
+  for (int side = 0; side < 2; side++) {
+    const k2g_hit_list_text *h = side ? b : a;
+    DevBuf<uint64_t> &off = side ? g->b_off : g->a_off;
+    if (h->off) {
+      HIP_TRY(upload(off, h->off, n + 1, s));
+      tot[side] = h->off[n];
+      continue;
+    }
+    HIP_TRY(off.reserve(n + 1));
+    HIP_TRY(g->n_tokens.reserve(n + 1));  // scratch here: the counts
+    HIP_TRY(hipMemsetAsync(g->n_tokens.p + n, 0, 8, s));
+    if (n) {
+      hipLaunchKernelGGL(k_merge_count, dim3(blocks), dim3(256), 0, s, side ? g->b_text.p : g->a_text.p,
+                         side ? g->b_field.p : g->a_field.p, side ? g->b_len.p : g->a_len.p, n, g->n_tokens.p);
+      HIP_TRY(hipGetLastError());
+    }
+    size_t scan_bytes = 0;
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, g->n_tokens.p, off.p, n + 1, s));
+    HIP_TRY(g->scan_tmp.reserve(scan_bytes + 1));
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(g->scan_tmp.p, scan_bytes, g->n_tokens.p, off.p, n + 1, s));
+    HIP_TRY(hipMemcpyAsync(&tot[side], off.p + n, 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+  }
+  const uint64_t na = tot[0], nb = tot[1];
   HIP_TRY(g->a_items.reserve(2 * na + 2));
   HIP_TRY(g->b_items.reserve(2 * nb + 2));
   HIP_TRY(g->bad.reserve(1));
   HIP_TRY(hipMemsetAsync(g->bad.p, 0, 4, s));
   if (n) {
-    const uint32_t blocks = (uint32_t) ((n + 255) / 256);
     hipLaunchKernelGGL(k_merge_parse, dim3(blocks), dim3(256), 0, s, g->a_text.p, g->a_field.p, g->a_len.p,
                        g->a_off.p, n, g->a_items.p, g->bad.p);
     HIP_TRY(hipGetLastError());

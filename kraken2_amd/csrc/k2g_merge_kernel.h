@@ -324,6 +324,19 @@ __device__ __forceinline__ void parse_field(const char *s, uint64_t len, int32_t
   for (; i < slots; i++) out[2 * i] = out[2 * i + 1] = 0;
 }
 
+// ':' count of each read's hit-list field (its item slots), one thread per
+// read; counts[n] is left for the caller (the scan's total).
+__global__ void __launch_bounds__(256)
+k_merge_count(const char *__restrict__ text, const uint64_t *__restrict__ field, const uint32_t *__restrict__ len,
+              uint64_t n, uint64_t *__restrict__ counts) {
+  const uint64_t r = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const char *f = text + field[r];
+  uint64_t colons = 0;
+  for (uint32_t i = 0; i < len[r]; i++) colons += f[i] == ':';
+  counts[r] = colons;
+}
+
 // One thread per read: field[r], len[r] locate read r's hit-list field in
 // text; off (n+1) gives its item slots.
 __global__ void __launch_bounds__(256)

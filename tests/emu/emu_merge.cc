@@ -73,24 +73,41 @@ int k2g_merge(k2g_merger *g, const k2g_merge_params *p, const k2g_hit_lists *a, 
 
 int k2g_merge_text(k2g_merger *g, const k2g_merge_params *p, const k2g_hit_list_text *a,
                    const k2g_hit_list_text *b, k2g_merge_results *out) {
-  if (a->n != b->n || a->off[0] || b->off[0]) {
+  if (a->n != b->n || (a->off && a->off[0]) || (b->off && b->off[0])) {
     g_error = "bad hit lists";
     return K2G_EUSAGE;
   }
   const uint64_t n = a->n;
-  std::vector<int32_t> ia(2 * a->off[n] + 2), ib(2 * b->off[n] + 2);
+  // offsets from the caller, or the fields' ':' counts (k_merge_count)
+  std::vector<uint64_t> oa(n + 1, 0), ob(n + 1, 0);
+  for (int side = 0; side < 2; side++) {
+    const k2g_hit_list_text *h = side ? b : a;
+    std::vector<uint64_t> &o = side ? ob : oa;
+    if (h->off) {
+      o.assign(h->off, h->off + n + 1);
+      continue;
+    }
+    std::vector<uint64_t> cnt(n + 1, 0);
+    for (uint64_t r = 0; r < n; r++) {
+      emu::blockIdx_.x = (unsigned) r;
+      emu::threadIdx_.x = 0;
+      k2g::k_merge_count(h->text, h->field, h->len, n, cnt.data());
+    }
+    for (uint64_t r = 0; r < n; r++) o[r + 1] = o[r] + cnt[r];
+  }
+  std::vector<int32_t> ia(2 * oa[n] + 2), ib(2 * ob[n] + 2);
   uint32_t bad = 0;
   for (uint64_t r = 0; r < n; r++) {
     emu::blockIdx_.x = (unsigned) r;
     emu::threadIdx_.x = 0;
-    k2g::k_merge_parse(a->text, a->field, a->len, a->off, n, ia.data(), &bad);
-    k2g::k_merge_parse(b->text, b->field, b->len, b->off, n, ib.data(), &bad);
+    k2g::k_merge_parse(a->text, a->field, a->len, oa.data(), n, ia.data(), &bad);
+    k2g::k_merge_parse(b->text, b->field, b->len, ob.data(), n, ib.data(), &bad);
   }
   if (bad) {
     g_error = "item offsets do not match the ':' count of the hit-list fields";
     return K2G_EUSAGE;
   }
-  k2g_hit_lists la{ia.data(), a->off, n}, lb{ib.data(), b->off, n};
+  k2g_hit_lists la{ia.data(), oa.data(), n}, lb{ib.data(), ob.data(), n};
   return k2g_merge(g, p, &la, &lb, out);
 }
 
