@@ -283,3 +283,57 @@ RAGGED_MERGE_LINES = (
     ["C\tr0\t1\t50\t211044:5\n", "C\tr1\t1\t50\t211044:2 11320:9\n", "U\tr2\t0\t50\t11320:3\n",
      "U\tr3\t0\t50\t11320:1\n", "C\tr4\t1\t50\t11320:7 |:| 211044:2\n"],
 )
+
+
+def large_scratch_dir(prefix, need_gb=40):
+    """A temporary directory for multi-GB databases: /dev/shm when it has
+    need_gb free, else the default temporary directory."""
+    import shutil
+    import tempfile
+    try:
+        if shutil.disk_usage("/dev/shm").free >= (need_gb << 30):
+            return tempfile.mkdtemp(prefix=prefix, dir="/dev/shm")
+    except OSError:
+        pass
+    return tempfile.mkdtemp(prefix=prefix)
+
+
+def fragment_lists(seq, off, idx):
+    """(mates 1, mates 2) of fragments idx of a packed (seq, 2n+1 offsets) set."""
+    m1, m2 = [], []
+    for i in idx:
+        a, b, c = (int(x) for x in off[2 * i:2 * i + 3])
+        m1.append(seq[a:b])
+        m2.append(seq[b:c])
+    return m1, m2
+
+
+def compare_with_oracle(k2, db, out, orc, h, seq, off, paired_flags, *, confidence=0.0, min_hit_groups=0,
+                        lines=False):
+    """Compares library results `out` of fragments (seq, off, paired_flags)
+    with the oracle, classifying single and paired fragments as two oracle
+    runs (one reference run cannot mix them).  Checks call, hit groups,
+    total k-mers and the hitlist text of every fragment, and with lines=True
+    the per-fragment lookups and distinct probe lines (out must carry probe
+    statistics).  Returns the number of fragments compared."""
+    n = len(paired_flags)
+    checked = 0
+    for pflag in (0, 1):
+        sel = [i for i in range(n) if int(paired_flags[i]) == pflag]
+        if not sel:
+            continue
+        m1, m2 = fragment_lists(seq, off, sel)
+        s2, o2 = k2.pack_fragments(m1, m2 if pflag else None)
+        ref = orc.classify(h, s2, o2, paired=bool(pflag), confidence=confidence,
+                           min_hit_groups=min_hit_groups, lines=lines)
+        for j, i in enumerate(sel):
+            runs = out.runs[out.run_off[i]:out.run_off[i + 1]]
+            got = (int(out.call[i]), int(out.hit_groups[i]), int(out.total_kmers[i]),
+                   k2.format_hitlist(runs, db.external_ids))
+            exp = (int(ref["call"][j]), int(ref["groups"][j]), int(ref["total"][j]), ref["text"][j])
+            assert got == exp, f"fragment {i} (paired {pflag}): {got[:3]} != {exp[:3]}"
+            if lines:
+                assert (int(out.lookups[i]), int(out.probe_lines[i])) == \
+                       (int(ref["lookups"][j]), int(ref["lines"][j])), f"probe statistics of fragment {i}"
+            checked += 1
+    return checked
