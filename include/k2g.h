@@ -241,6 +241,39 @@ int k2g_build_insert_device(k2g_builder *b, const char *d_seq, const uint64_t *o
                             const uint32_t *taxid, uint64_t n);
 int k2g_build_get_info(k2g_builder *b, uint64_t *capacity, uint64_t *size, uint64_t *key_bits,
                        uint64_t *value_bits);
+
+/* Verification of a built table against the sequences it was built from.
+ * No reference counterpart: a check that src/build_db.cc never makes.  The
+ * build windows are scanned again on the device, and every (minimizer, taxid)
+ * pair the build inserts (src/build_db.h:186-326) is collected, one hash
+ * partition at a time.  Each distinct minimizer is then looked up in the
+ * builder's table.  A minimizer whose probe meets a cell holding another
+ * minimizer's identical compacted key shares that cell
+ * (src/compact_hash.h:432-447), so the cell holds the LCA over both.  The
+ * check therefore groups minimizers by the cell their lookup ends at and
+ * requires every cell to hold exactly the LCA of the taxa of the minimizers
+ * ending there.  Arguments are those of k2g_build_insert_device for the
+ * same sequences. */
+typedef struct {
+  uint64_t occurrences;  /* (minimizer, taxid) pairs inserted */
+  uint64_t keys;         /* distinct minimizers */
+  uint64_t cells;        /* distinct cells their lookups end at */
+  uint64_t missing;      /* minimizers whose lookup finds nothing: must be 0 */
+  uint64_t not_covering; /* minimizers whose value is not an ancestor-or-self of each of their
+                          * taxa: must be 0 */
+  uint64_t shared_cells; /* cells reached by more than one minimizer (compacted-key collisions) */
+  uint64_t shared_keys;  /* minimizers in those cells */
+  uint64_t differ;       /* minimizers whose value is not the LCA of their own taxa */
+  uint64_t unexplained;  /* cells whose value is not the LCA of the taxa of every minimizer
+                          * ending there: must be 0, which puts every differing minimizer in
+                          * a shared cell */
+  uint64_t orphan_cells; /* filled cells no minimizer reaches: must be 0 */
+  uint32_t partitions;   /* hash partitions the pairs were collected in */
+  uint32_t pad;
+} k2g_build_check;
+
+int k2g_build_verify(k2g_builder *b, const char *d_seq, const uint64_t *off, const uint32_t *taxid,
+                     uint64_t n, k2g_build_check *out);
 int k2g_build_download(k2g_builder *b, uint32_t *cells, uint64_t capacity);
 /* hash.k2d in the reference's WriteTable layout (src/compact_hash.h:343-352). */
 int k2g_build_write(k2g_builder *b, const char *hash_k2d);

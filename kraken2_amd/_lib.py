@@ -51,6 +51,7 @@ EXPORTED_SYMBOLS = (
     "k2g_build_insert",
     "k2g_build_insert_device",
     "k2g_build_get_info",
+    "k2g_build_verify",
     "k2g_build_download",
     "k2g_build_write",
     "k2g_build_destroy",
@@ -469,6 +470,12 @@ def expand_spaced_seed_mask(template: str, bits_per_char: int) -> int:
     return out & 0xFFFFFFFFFFFFFFFF
 
 
+class BuildCheck(C.Structure):
+    _fields_ = [(name, C.c_uint64) for name in
+                ("occurrences", "keys", "cells", "missing", "not_covering", "shared_cells", "shared_keys",
+                 "differ", "unexplained", "orphan_cells")] + [("partitions", C.c_uint32), ("pad", C.c_uint32)]
+
+
 class Builder:
     """GPU table construction (atomicCAS + LCA) into a resident cell array."""
 
@@ -497,6 +504,17 @@ class Builder:
         buf = np.frombuffer(seq, np.uint8) if len(seq) else np.zeros(1, np.uint8)
         _check(_lib.k2g_build_insert(self._h, buf.ctypes.data, off.ctypes.data, taxid.ctypes.data,
                                      taxid.shape[0]))
+
+    def verify_device(self, d_seq, off: np.ndarray, taxid: np.ndarray) -> dict:
+        """k2g_build_verify over device-resident sequences (those inserted)."""
+        off = np.ascontiguousarray(off, dtype=np.uint64)
+        taxid = np.ascontiguousarray(taxid, dtype=np.uint32)
+        _lib.k2g_build_verify.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64,
+                                          C.POINTER(BuildCheck)]
+        chk = BuildCheck()
+        _check(_lib.k2g_build_verify(self._h, d_seq, off.ctypes.data, taxid.ctypes.data, taxid.shape[0],
+                                     C.byref(chk)))
+        return {name: int(getattr(chk, name)) for name, _ in BuildCheck._fields_ if name != "pad"}
 
     def info(self):
         v = [C.c_uint64() for _ in range(4)]

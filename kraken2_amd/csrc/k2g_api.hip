@@ -52,6 +52,10 @@ __global__ void k_get_batch(DbView, const uint64_t *, uint32_t *, uint64_t);
 __global__ void k_max_value(DbView, unsigned int *);
 __global__ void k_pad_copy(uint32_t *, uint64_t, uint32_t);
 __global__ void k_build(BuildView, const char *, const SubBlock *, uint64_t);
+__global__ void k_verify_keys(DbView, const uint64_t *, const uint32_t *, uint64_t, uint64_t *,
+                              uint32_t *, unsigned long long *, unsigned long long *);
+__global__ void k_verify_cells(DbView, const uint64_t *, const uint32_t *, uint64_t,
+                               unsigned long long *);
 }  // namespace k2g
 
 using namespace k2g;
@@ -1226,6 +1230,153 @@ extern "C" int k2g_build_insert(k2g_builder *b, const char *seq, const uint64_t 
   std::vector<uint64_t> rel(n + 1);
   for (uint64_t i = 0; i <= n; i++) rel[i] = off[i] - base;
   return k2g_build_insert_device(b, b->seq.p, rel.data(), taxid, n);
+}
+
+// Table verification: the build's (minimizer, taxid) pairs collected in
+// hash partitions (every minimizer sharing a cell with another has the same
+// compacted key, so a cell's minimizers all fall in one partition), sorted
+// by minimizer, looked up, then sorted by the cell they reach.
+extern "C" int k2g_build_verify(k2g_builder *b, const char *d_seq, const uint64_t *off,
+                                const uint32_t *taxid, uint64_t n, k2g_build_check *out) {
+  if (!b || !out || (n && (!d_seq || !off || !taxid))) return fail(K2G_EUSAGE, "null argument");
+  std::memset(out, 0, sizeof(*out));
+  int rc = check_window_lengths(b->params);
+  if (rc) return rc;
+  uint64_t bases = 0;
+  for (uint64_t i = 0; i < n; i++) {
+    if (off[i + 1] < off[i]) return fail(K2G_EUSAGE, "sequence offsets must not decrease");
+    if (taxid[i] >= b->node_count) return fail(K2G_EDATA, "taxid is not a node of the taxonomy");
+    if (taxid[i]) bases += off[i + 1] - off[i];
+  }
+  std::vector<SubBlock> items;
+  build_windows(b->params, off, taxid, n, 0, items);
+  hipStream_t s = b->stream;
+  HIP_TRY(hipStreamSynchronize(s));
+  // pairs per window <= its k-mers; minimizer windows of w l-mers make about
+  // 2/(w+1) of the k-mers new minimizers: room for 1.5 times that, cut into
+  // partitions of at most 2^30 pairs that use at most a third of free HBM
+  const uint64_t w = b->params.k - b->params.l + 1;
+  const uint64_t expect = bases * 2 / (w + 1) * 3 / 2 + (1u << 20);
+  size_t free_b = 0, total_b = 0;
+  HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+  uint32_t parts = 1;
+  auto per_part = [&](uint32_t p) { return expect / p + (1u << 20); };
+  while (parts < (1u << 16) &&
+         (per_part(parts) > (3ull << 29) || per_part(parts) * 40 > free_b / 3))
+    parts *= 2;
+  uint32_t part_bits = (uint32_t) __builtin_ctz(parts);
+  uint64_t cap = per_part(parts);
+  DevBuf<SubBlock> d_items;
+  DevBuf<uint64_t> ka, kb;
+  DevBuf<uint32_t> ta, tb;
+  DevBuf<unsigned long long> cnt;
+  DevBuf<unsigned char> temp;
+  auto release = [&] {
+    d_items.release();
+    ka.release();
+    kb.release();
+    ta.release();
+    tb.release();
+    cnt.release();
+    temp.release();
+  };
+  auto run = [&]() -> int {
+    HIP_TRY(d_items.reserve(std::max<size_t>(items.size(), 1)));
+    if (!items.empty())
+      HIP_TRY(hipMemcpyAsync(d_items.p, items.data(), items.size() * sizeof(SubBlock),
+                             hipMemcpyHostToDevice, s));
+    HIP_TRY(ka.reserve(cap));
+    HIP_TRY(kb.reserve(cap));
+    HIP_TRY(ta.reserve(cap));
+    HIP_TRY(tb.reserve(cap));
+    HIP_TRY(cnt.reserve(kVSlots + 2));
+    HIP_TRY(hipMemsetAsync(cnt.p, 0, (kVSlots + 2) * 8, s));
+    size_t tb1 = 0;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tb1, ka.p, kb.p, ta.p, tb.p, (int) cap, 0, 64, s));
+    HIP_TRY(temp.reserve(tb1));
+    DbView dv{};
+    dv.cells = b->d_cells;
+    dv.capacity = b->capacity;
+    dv.barrett = b->view.barrett;
+    dv.key_bits = b->view.key_bits;
+    dv.value_bits = b->view.value_bits;
+    dv.parent = b->d_parent;
+    dv.node_count = (uint32_t) b->node_count;
+    const int key_bits_sorted = b->params.dna_db ? (int) (2 * b->params.l) : (int) (4 * b->params.l);
+    int cell_bits = 1;
+    while (cell_bits < 64 && (1ull << cell_bits) <= b->capacity) cell_bits++;
+    unsigned long long *const n_emit = cnt.p + kVSlots, *const n_found = cnt.p + kVSlots + 1;
+    const uint64_t grid_b = std::min<uint64_t>(std::max<size_t>(items.size(), 1),
+                                               (uint64_t) std::max(1, b->num_cus) * 32);
+    for (uint32_t part = 0; part < parts; part++) {
+      HIP_TRY(hipMemsetAsync(n_emit, 0, 16, s));
+      BuildView v = b->view;
+      v.cells = nullptr;
+      v.sample = nullptr;
+      v.emit_key = ka.p;
+      v.emit_tax = ta.p;
+      v.n_emit = n_emit;
+      v.emit_cap = cap;
+      v.part_bits = part_bits;
+      v.part = part;
+      if (!items.empty()) {
+        hipLaunchKernelGGL(k_build, dim3((unsigned) grid_b), dim3(64), 0, s, v, d_seq, d_items.p,
+                           (uint64_t) items.size());
+        HIP_TRY(hipGetLastError());
+      }
+      unsigned long long m = 0;
+      HIP_TRY(hipMemcpyAsync(&m, n_emit, 8, hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipStreamSynchronize(s));
+      if (m > cap) return 1;  // partition overflow: more partitions
+      if (m == 0) continue;
+      size_t tbytes = tb1;
+      HIP_TRY(hipcub::DeviceRadixSort::SortPairs(temp.p, tbytes, ka.p, kb.p, ta.p, tb.p, (int) m, 0,
+                                                 key_bits_sorted, s));
+      const unsigned g = (unsigned) ((m + 255) / 256);
+      hipLaunchKernelGGL(k_verify_keys, dim3(g), dim3(256), 0, s, dv, kb.p, tb.p, (uint64_t) m, ka.p,
+                         ta.p, n_found, cnt.p);
+      HIP_TRY(hipGetLastError());
+      unsigned long long u = 0;
+      HIP_TRY(hipMemcpyAsync(&u, n_found, 8, hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipStreamSynchronize(s));
+      if (u == 0) continue;
+      tbytes = tb1;
+      HIP_TRY(hipcub::DeviceRadixSort::SortPairs(temp.p, tbytes, ka.p, kb.p, ta.p, tb.p, (int) u, 0,
+                                                 cell_bits, s));
+      hipLaunchKernelGGL(k_verify_cells, dim3((unsigned) ((u + 255) / 256)), dim3(256), 0, s, dv, kb.p,
+                         tb.p, (uint64_t) u, cnt.p);
+      HIP_TRY(hipGetLastError());
+    }
+    unsigned long long c[kVSlots];
+    HIP_TRY(hipMemcpyAsync(c, cnt.p, sizeof(c), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    out->occurrences = c[kVOcc];
+    out->keys = c[kVKeys];
+    out->cells = c[kVCells];
+    out->missing = c[kVMissing];
+    out->not_covering = c[kVNotCovering];
+    out->shared_cells = c[kVSharedCells];
+    out->shared_keys = c[kVSharedKeys];
+    out->differ = c[kVDiffer];
+    out->unexplained = c[kVUnexplained];
+    out->partitions = parts;
+    return 0;
+  };
+  for (;;) {  // a partition that overflows its room: twice as many partitions
+    rc = run();
+    if (rc != 1 || parts >= (1u << 16)) break;
+    parts *= 2;
+    part_bits++;
+    cap = per_part(parts);
+  }
+  release();
+  if (rc == 1) return fail(K2G_ESOFTWARE, "verification partition overflow");
+  if (rc) return rc;
+  uint64_t size = 0;
+  rc = k2g_build_get_info(b, nullptr, &size, nullptr, nullptr);
+  if (rc) return rc;
+  out->orphan_cells = size > out->cells ? size - out->cells : 0;
+  return K2G_OK;
 }
 
 extern "C" int k2g_build_get_info(k2g_builder *b, uint64_t *capacity, uint64_t *size,

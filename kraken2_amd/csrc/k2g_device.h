@@ -96,6 +96,28 @@ __device__ __forceinline__ uint32_t table_get(const DbView &db, uint64_t key) {
   }
 }
 
+// table_get that also reports the cell the probe ends at: *where = the
+// index of the cell holding key's compacted key, or capacity when the probe
+// ends at an empty cell or visits every cell (then the value is 0).
+__device__ __forceinline__ uint32_t table_find(const DbView &db, uint64_t key, uint64_t *where) {
+  const uint64_t hc = fmix64(key);
+  const uint32_t ck = (uint32_t) (hc >> (64 - db.key_bits));
+  const uint32_t vb = db.value_bits;
+  const uint32_t vmask = (1u << vb) - 1u;
+  uint64_t idx = mod_capacity(hc, db.capacity, db.barrett);
+  for (uint64_t left = db.capacity; left; left--) {
+    const uint32_t c = db.cells[idx];
+    if ((c & vmask) == 0) break;
+    if ((c >> vb) == ck) {
+      *where = idx;
+      return c & vmask;
+    }
+    if (++idx == db.capacity) idx = 0;
+  }
+  *where = db.capacity;
+  return 0;
+}
+
 // Distinct 128-byte lines of the cell array that table_get(key) reads, from
 // the home cell through the terminating cell (measurement only).
 __device__ __forceinline__ uint32_t table_lines(const DbView &db, uint64_t key) {

@@ -115,6 +115,29 @@ struct BuildView {
   unsigned long long *n_sample;
   uint64_t sample_cap;
   uint32_t sample_ranges;
+  // verification mode (cells == null, sample == null): every (minimizer,
+  // taxid) pair the build would insert whose fmix64 has its top part_bits
+  // bits equal to part is appended to emit_key / emit_tax (at most
+  // emit_cap; *n_emit counts them all)
+  uint64_t *emit_key;
+  uint32_t *emit_tax;
+  unsigned long long *n_emit;
+  uint64_t emit_cap;
+  uint32_t part_bits, part;
+};
+
+// Counters of the table verification (k_verify_keys, k_verify_cells).
+enum VerifySlot {
+  kVOcc,          // (minimizer, taxid) pairs inserted
+  kVKeys,         // distinct minimizers
+  kVCells,        // distinct cells their lookups end at
+  kVMissing,      // minimizers whose lookup finds no cell
+  kVNotCovering,  // minimizers whose value is not an ancestor-or-self of each of their taxa
+  kVSharedCells,  // cells reached by more than one minimizer (same compacted key)
+  kVSharedKeys,   // minimizers in such cells
+  kVDiffer,       // minimizers whose value is not the LCA of their own taxa
+  kVUnexplained,  // cells whose value is not the LCA of the taxa of every minimizer ending there
+  kVSlots
 };
 
 // One scanner window of the deterministic build: sequence bytes

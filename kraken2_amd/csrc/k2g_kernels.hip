@@ -1115,7 +1115,25 @@ __device__ void build_window(const BuildView &b, const char *m, uint64_t n, uint
     const uint64_t prev_min = prevm ? before : last_min;
     // consecutive repeats insert the same (minimizer, taxid) pair again,
     // which leaves the cell unchanged; insert each run once
-    if (nonamb && mn != prev_min) {
+    const bool ins = nonamb && mn != prev_min;
+    if (b.emit_key) {
+      // verification: the pairs this window inserts, one hash partition at a
+      // time, appended with one atomic per wave
+      const uint64_t hm = fmix64(mn);
+      const bool e = ins && !(b.min_clear && hm < b.min_clear) &&
+                     (b.part_bits == 0 || (uint32_t) (hm >> (64 - b.part_bits)) == b.part);
+      const uint64_t EM = __ballot(e);
+      if (EM) {
+        const int fl = __ffsll((long long) EM) - 1;
+        unsigned long long at0 = 0;
+        if (lane == fl) at0 = atomicAdd(b.n_emit, (unsigned long long) __popcll(EM));
+        const uint64_t at = readlane64((uint64_t) at0, fl) + (uint64_t) __popcll(EM & lanemask_lt(lane));
+        if (e && at < b.emit_cap) {
+          b.emit_key[at] = mn;
+          b.emit_tax[at] = taxid;
+        }
+      }
+    } else if (ins) {
       if (b.sample) {
         if ((fmix64(mn) & 1023u) < b.sample_ranges) {
           const unsigned long long at = atomicAdd(b.n_sample, 1ULL);
@@ -1131,6 +1149,94 @@ __device__ void build_window(const BuildView &b, const char *m, uint64_t n, uint
     pp1 = P1;
     pp2 = P2;
     pp3 = P3;
+  }
+}
+
+// ---- verification of a built table against the pairs its build inserted
+// (sorted by minimizer, then the minimizers sorted by the cell they reach)
+
+// Sum of a per-lane count over the wave, added to *dst by one lane.
+__device__ __forceinline__ void wave_add(unsigned long long *dst, uint64_t v, int lane) {
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) v += shfl64(v, lane ^ d);
+  if (lane == 0 && v) atomicAdd(dst, (unsigned long long) v);
+}
+
+// One thread per sorted pair; the first pair of each minimizer's run takes
+// the LCA of the run's taxa (the value a build inserting only this
+// minimizer leaves, src/build_db.h:186-194), looks the minimizer up and
+// checks that the value found is an ancestor-or-self of every one of its
+// taxa.  Found minimizers are appended as (cell, own LCA) for the per-cell
+// pass.
+__global__ __launch_bounds__(256) void k_verify_keys(DbView db, const uint64_t *key, const uint32_t *tax,
+                                                      uint64_t m, uint64_t *cell_out, uint32_t *lca_out,
+                                                      unsigned long long *n_out, unsigned long long *cnt) {
+  const uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & (kWave - 1);
+  const bool head = i < m && (i == 0 || key[i] != key[i - 1]);
+  uint64_t occ = 0, where = db.capacity;
+  uint32_t own = 0, val = 0;
+  bool cover = true;
+  if (head) {
+    const uint64_t k = key[i];
+    own = tax[i];
+    uint64_t j = i + 1;
+    for (; j < m && key[j] == k; j++) own = lca(db.parent, own, tax[j]);
+    occ = j - i;
+    val = table_find(db, k, &where);
+    if (val)
+      for (uint64_t q = i; q < j; q++) cover = cover && is_ancestor(db.parent, val, tax[q]);
+  }
+  const bool found = head && val != 0;
+  const uint64_t F = __ballot(found);
+  if (F) {
+    const int fl = __ffsll((long long) F) - 1;
+    unsigned long long at0 = 0;
+    if (lane == fl) at0 = atomicAdd(n_out, (unsigned long long) __popcll(F));
+    const uint64_t at = readlane64((uint64_t) at0, fl) + (uint64_t) __popcll(F & lanemask_lt(lane));
+    if (found) {
+      cell_out[at] = where;
+      lca_out[at] = own;
+    }
+  }
+  wave_add(&cnt[kVOcc], occ, lane);
+  const uint64_t H = __ballot(head);
+  const uint64_t NC = __ballot(found && !cover);
+  const uint64_t D = __ballot(found && val != own);
+  if (lane == 0) {
+    if (H) atomicAdd(&cnt[kVKeys], (unsigned long long) __popcll(H));
+    if (H & ~F) atomicAdd(&cnt[kVMissing], (unsigned long long) __popcll(H & ~F));
+    if (NC) atomicAdd(&cnt[kVNotCovering], (unsigned long long) __popcll(NC));
+    if (D) atomicAdd(&cnt[kVDiffer], (unsigned long long) __popcll(D));
+  }
+}
+
+// One thread per (cell, own LCA) pair sorted by cell: the first of each
+// cell's run checks that the cell holds the LCA of every minimizer ending
+// there (a lone minimizer: its own LCA).
+__global__ __launch_bounds__(256) void k_verify_cells(DbView db, const uint64_t *cell, const uint32_t *own,
+                                                       uint64_t m, unsigned long long *cnt) {
+  const uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & (kWave - 1);
+  const bool head = i < m && (i == 0 || cell[i] != cell[i - 1]);
+  uint64_t len = 0;
+  bool bad = false;
+  if (head) {
+    const uint64_t c = cell[i];
+    uint32_t a = own[i];
+    uint64_t j = i + 1;
+    for (; j < m && cell[j] == c; j++) a = lca(db.parent, a, own[j]);
+    len = j - i;
+    const uint32_t vmask = (1u << db.value_bits) - 1u;
+    bad = (db.cells[c] & vmask) != a;
+  }
+  const bool shared = head && len > 1;
+  wave_add(&cnt[kVSharedKeys], shared ? len : 0, lane);
+  const uint64_t H = __ballot(head), S = __ballot(shared), B = __ballot(bad);
+  if (lane == 0) {
+    if (H) atomicAdd(&cnt[kVCells], (unsigned long long) __popcll(H));
+    if (S) atomicAdd(&cnt[kVSharedCells], (unsigned long long) __popcll(S));
+    if (B) atomicAdd(&cnt[kVUnexplained], (unsigned long long) __popcll(B));
   }
 }
 

@@ -86,7 +86,7 @@ class Workload:
 
     def __init__(self, out_dir: str, *, seed: int, n_genomes=4000, per_group=5, genome_median=2.5e6,
                  genome_sigma=0.3, genome_min=5e5, genome_max=6e6, divergent_frac=0.08, snp_rate=0.01,
-                 k=35, l=31, load_factor=0.7, block=16384, subblock=2048, log=print):
+                 k=35, l=31, load_factor=0.7, block=16384, subblock=2048, verify=False, log=print):
         self.hip = Hip()
         self.synth = _synth()
         self.dir = out_dir
@@ -150,6 +150,12 @@ class Workload:
         self.build_s = time.time() - t1
         log(f"# GPU build: {info['size']} cells filled of {info['capacity']} "
             f"({info['size'] / info['capacity']:.3f}) in {self.build_s:.1f}s")
+        self.build_check = None
+        if verify:  # every inserted minimizer looked up, compacted-key collisions counted
+            t1 = time.time()
+            self.build_check = bld.verify_device(self.d_genomes, self.genome_off, self.genome_taxid)
+            self.build_check["seconds"] = round(time.time() - t1, 2)
+            log(f"# build verification: {self.build_check}")
         t1 = time.time()
         bld.write(os.path.join(out_dir, "hash.k2d"))
         bld.close()

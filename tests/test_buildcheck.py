@@ -1,0 +1,58 @@
+"""The host-side table check (tests/helpers/buildcheck.py), which checks
+k2g_build_verify on the GPU, run on tables the oracle's deterministic build
+writes (byte-identical to the reference build_db on the golden inputs):
+every inserted minimizer is found, each cell holds the LCA of the
+minimizers ending there, no filled cell is left over.  A planted change to
+one cell, or one minimizer left out of the enumeration, fails it."""
+import os
+
+import numpy as np
+
+from helpers import DNA_SEED_TEMPLATE, Oracle, read_fasta, taxonomy_external_ids
+from helpers import buildcheck as bc
+
+
+def _inputs(d):
+    import kraken2_amd as k2
+    opts = np.fromfile(os.path.join(d, "opts.k2d"), np.uint64, 4)
+    k, l, mask, toggle = (int(x) for x in opts)
+    assert mask == k2.expand_spaced_seed_mask(DNA_SEED_TEMPLATE, 2)
+    raw = open(os.path.join(d, "taxo.k2d"), "rb").read()
+    nn = int(np.frombuffer(raw, np.uint64, 1, 8)[0])
+    parent = np.frombuffer(raw, np.uint64, 7 * nn, 32).reshape(nn, 7)[:, 0].astype(np.int64)
+    e2i = {e: i for i, e in enumerate(taxonomy_external_ids(os.path.join(d, "taxo.k2d")))}
+    names, genomes = read_fasta(os.path.join(d, "library.fna"))
+    idmap = dict(x.split("\t") for x in open(os.path.join(d, "seqid2taxid.map")).read().split("\n") if x)
+    taxa = [e2i[int(idmap[nm])] for nm in names]
+    keys, tax = bc.inserted_pairs(Oracle(), genomes, taxa, k, l, mask, toggle, 16384, 2048)
+    uk, own = bc.key_lcas(keys, tax, parent)
+    return parent, uk, own
+
+
+def test_oracle_built_table_passes(synth_db):
+    parent, uk, own = _inputs(synth_db)
+    cells, kb, vb, size = bc.read_table(os.path.join(synth_db, "hash.k2d"))
+    r = bc.check_table(cells, kb, vb, size, parent, uk, own)
+    assert r["keys"] > 10000
+    for f in ("missing", "not_covering", "unexplained", "orphan_cells"):
+        assert r[f] == 0, (f, r[f])
+    assert r["differ"] <= r["shared_keys"]
+    assert (r["values"][~r["shared_key_mask"]].astype(np.int64) == own[~r["shared_key_mask"]]).all()
+
+
+def test_planted_mistakes_fail(synth_db):
+    parent, uk, own = _inputs(synth_db)
+    cells, kb, vb, size = bc.read_table(os.path.join(synth_db, "hash.k2d"))
+    cells = cells.copy()
+    vmask = (1 << vb) - 1
+    # a cell holding the wrong taxon (its parent instead)
+    val, where = bc.probe(cells, kb, vb, uk[:1000])
+    j = int(np.flatnonzero((val != 0) & (parent[val] != 0))[0])
+    c = int(cells[where[j]])
+    cells[where[j]] = np.uint32((c & ~vmask) | int(parent[c & vmask]))
+    r = bc.check_table(cells, kb, vb, size, parent, uk, own)
+    assert r["unexplained"] == 1
+    # a minimizer missing from the enumeration leaves its cell an orphan
+    cells, kb, vb, size = bc.read_table(os.path.join(synth_db, "hash.k2d"))
+    r = bc.check_table(cells, kb, vb, size, parent, uk[1:], own[1:])
+    assert r["orphan_cells"] + r["unexplained"] >= 1

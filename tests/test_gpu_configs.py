@@ -93,23 +93,93 @@ def test_probe_statistics_edge_tables(edge_tables, name):
 def c1(tmp_path_factory):
     """Config 1: seed 1, root -> 4 phyla -> 16 genera -> 64 species, 200 kb
     genomes (species = genus root + 5% substitutions), k=35 l=31, load 0.7,
-    built by the GPU build front-ends; 100k x 150 bp reads."""
+    built by the GPU build front-ends; 100k x 150 bp reads.  The same inputs
+    also built by the oracle's deterministic build (the reference's)."""
     import kraken2_amd as k2
     d = str(tmp_path_factory.mktemp("c1"))
     _log("config 1: building the database with k2g_estimate_capacity / k2g_build_db")
     make_synthetic_db(d, seed=1, levels="4,4,4", genome_len=200000,
                       tools=(os.path.join(k2.BIN_DIR, "k2g_estimate_capacity"),
                              os.path.join(k2.BIN_DIR, "k2g_build_db")))
+    d_orc = str(tmp_path_factory.mktemp("c1_oracle_build"))
+    _log("config 1: the same inputs built by the oracle")
+    make_synthetic_db(d_orc, seed=1, levels="4,4,4", genome_len=200000)
     reads = os.path.join(d, "reads.fa")
     run([K2SYNTH, "reads", "--lib", os.path.join(d, "library.fna"), "--out", reads, "--seed", "1", "--n",
          "100000", "--len", "150", "--sub", "0.01", "--random-frac", "0.1"])
     _, seqs = read_fasta(reads)
-    return d, seqs
+    return d, seqs, d_orc
+
+
+def test_c1_table_builds(c1):
+    """Every minimizer the build inserts is found in the GPU-built and the
+    oracle-built config-1 tables; each cell holds the LCA of the taxa of the
+    minimizers whose lookup ends there (so a value differs from a
+    minimizer's own LCA only where it shares its cell, a compacted-key
+    collision); minimizers alone in their cell in both tables have equal
+    values.  k2g_build_verify on a GPU build reports the counts this host
+    check computes from that build's cells."""
+    import kraken2_amd as k2
+    from kraken2_amd.synth import Hip
+    from helpers import DNA_SEED_TEMPLATE, taxonomy_external_ids
+    from helpers import buildcheck as bc
+    d, _, d_orc = c1
+    opts = np.fromfile(os.path.join(d, "opts.k2d"), np.uint64, 4)
+    k, l, mask, toggle = (int(x) for x in opts)
+    assert mask == k2.expand_spaced_seed_mask(DNA_SEED_TEMPLATE, 2)
+    assert open(os.path.join(d, "taxo.k2d"), "rb").read() == open(os.path.join(d_orc, "taxo.k2d"), "rb").read()
+    raw = open(os.path.join(d, "taxo.k2d"), "rb").read()
+    nn = int(np.frombuffer(raw, np.uint64, 1, 8)[0])
+    parent = np.frombuffer(raw, np.uint64, 7 * nn, 32).reshape(nn, 7)[:, 0].astype(np.int64)
+    ext = taxonomy_external_ids(os.path.join(d, "taxo.k2d"))
+    e2i = {e: i for i, e in enumerate(ext)}
+    names, genomes = read_fasta(os.path.join(d, "library.fna"))
+    idmap = dict(line.split("\t") for line in open(os.path.join(d, "seqid2taxid.map")).read().split("\n") if line)
+    taxa = [e2i[int(idmap[nm])] for nm in names]
+    orc = Oracle()
+    _log("config 1: inserted pairs from the oracle scanner")
+    keys, tax = bc.inserted_pairs(orc, genomes, taxa, k, l, mask, toggle, 16384, 2048)
+    uk, own = bc.key_lcas(keys, tax, parent)
+    res = {}
+    for name, dd in (("gpu", d), ("oracle", d_orc)):
+        cells, kb, vb, size = bc.read_table(os.path.join(dd, "hash.k2d"))
+        r = bc.check_table(cells, kb, vb, size, parent, uk, own)
+        _log(f"config 1 {name} table: " + str({x: r[x] for x in r if x not in ("values", "shared_key_mask")}))
+        for must_be_zero in ("missing", "not_covering", "unexplained", "orphan_cells"):
+            assert r[must_be_zero] == 0, (name, must_be_zero, r[must_be_zero])
+        assert r["differ"] <= r["shared_keys"]
+        res[name] = r
+    alone = ~res["gpu"]["shared_key_mask"] & ~res["oracle"]["shared_key_mask"]
+    assert np.array_equal(res["gpu"]["values"][alone], res["oracle"]["values"][alone])
+    assert np.array_equal(res["gpu"]["values"][alone].astype(np.int64), own[alone])
+
+    # k2g_build_verify on an in-process GPU build of the same inputs
+    cap = int(np.fromfile(os.path.join(d, "hash.k2d"), np.uint64, 1)[0])
+    seq, goff = k2.pack_fragments(genomes)
+    gstart = goff[0::2]
+    hip = Hip()
+    d_seq = hip.upload(np.frombuffer(seq, np.uint8))
+    bld = k2.Builder(os.path.join(d, "taxo.k2d"), cap, k=k, l=l, spaced_seed_mask=mask, toggle_mask=toggle,
+                     block_size=16384, subblock_size=2048)
+    try:
+        tx = np.array(taxa, np.uint32)
+        bld.insert(seq, gstart, tx)
+        chk = bld.verify_device(d_seq, gstart, tx)
+        info = bld.info()
+        host = bc.check_table(bld.cells(), info["key_bits"], info["value_bits"], info["size"], parent, uk, own)
+    finally:
+        bld.close()
+        hip.free(d_seq)
+    _log(f"config 1 k2g_build_verify: {chk}")
+    assert chk["occurrences"] == len(keys)
+    for f in ("keys", "cells", "missing", "not_covering", "shared_cells", "shared_keys", "differ",
+              "unexplained", "orphan_cells"):
+        assert chk[f] == host[f], (f, chk[f], host[f])
 
 
 def test_c1_every_read_matches_oracle(c1):
     import kraken2_amd as k2
-    d, seqs = c1
+    d, seqs, _ = c1
     assert len(seqs) == 100000
     db = k2.Db.from_dir(d)
     assert db.info.capacity > 5_000_000
@@ -147,7 +217,7 @@ def c2_table():
     tmp = large_scratch_dir("k2g_test_c2_")
     try:
         _log(f"config 2 table: 4000 genomes into {tmp}")
-        wl = Workload(tmp, seed=2, n_genomes=4000, log=_log)
+        wl = Workload(tmp, seed=2, n_genomes=4000, verify=True, log=_log)
         yield wl, tmp
     finally:
         wl_ = locals().get("wl")
@@ -235,6 +305,20 @@ def _full_size_check(k2, wl, tmp, n, *, f_single, f_paired, paired_len, off_frac
             rs.free()
         batch.close()
         db.close()
+
+
+def test_c2_table_verified(c2_table):
+    """The config-2 table built on the GPU: every inserted minimizer is
+    found, its value covers each of its taxa, and every cell holds the LCA
+    of the taxa of the minimizers ending there; minimizers whose value is not
+    their own LCA all share a cell with another minimizer of the same
+    compacted key (src/compact_hash.h:432-447)."""
+    wl, _ = c2_table
+    chk = wl.build_check
+    for f in ("missing", "not_covering", "unexplained", "orphan_cells"):
+        assert chk[f] == 0, (f, chk)
+    assert chk["keys"] > 1.5e9 and chk["cells"] > 1.5e9
+    assert chk["differ"] <= chk["shared_keys"] and chk["shared_cells"] * 2 <= chk["shared_keys"]
 
 
 def test_c3_full_size(c2_table):
