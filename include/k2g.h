@@ -156,6 +156,38 @@ int k2g_batch_redo_counts(k2g_batch *b, uint64_t *run_space, uint64_t *dense_tab
  * a verification aid for large batches. */
 int k2g_batch_digests(k2g_batch *b, uint64_t *out, uint64_t n);
 
+/* ---- classify without device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same launch sequence and results as k2g_classify_device with
+ * run_cap_pct = 100, but every scratch buffer (element bounds, status words,
+ * the longest-first sort and its work units, redo lists, cursors, hipCUB
+ * temporary storage, the dense hit tables of the overflow path) is carved
+ * from a 128 MiB statically allocated device array per GPU, held until the
+ * process ends; at most 524288 fragments per call (larger calls fail with
+ * K2G_EUSAGE).  Results go to the caller's device arrays: fragment i's runs
+ * are (code, count) pairs runs[2*run_off[i] ..), n_runs[i] of them, inside
+ * space of its exact element count (run_off[n] = the batch's element
+ * total, which must not exceed run_capacity; K2G_EUSAGE otherwise); HLL
+ * records likewise at the same offsets.  The function returns when the
+ * last kernels are enqueued on hip_stream (NULL = the null stream); the
+ * results are ready when that stream is.  Every call on a GPU uses that
+ * GPU's one array: calls that overlap on one GPU (on other host threads or
+ * other streams before the previous call's stream finished) race. */
+typedef struct {
+  uint32_t *call, *hit_groups, *total_kmers, *n_runs; /* n entries each */
+  uint64_t *run_off;                                  /* n + 1 */
+  uint32_t *runs;                                     /* 2 * run_capacity words */
+  uint64_t run_capacity;
+  uint32_t *hll;         /* want_hll: 2 * run_capacity words, (taxon, encoded hash) pairs */
+  uint32_t *n_hll;       /* want_hll: n */
+  uint64_t *kmer_counts; /* want_kmer_counts: node_count */
+} k2g_device_results;
+
+int k2g_classify_device_noalloc(k2g_db *db, const k2g_classify_params *p, const char *d_seq,
+                                const uint64_t *d_off, const uint8_t *d_paired, uint64_t n,
+                                const k2g_device_results *out, void *hip_stream);
+
 /* Pinned host memory for request fragments (and anything else a caller
  * streams to the GPU): uploads from it run at full interconnect speed and
  * asynchronously.  No reference counterpart. */

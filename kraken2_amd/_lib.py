@@ -37,6 +37,7 @@ EXPORTED_SYMBOLS = (
     "k2g_batch_fetch_counts",
     "k2g_batch_redo_counts",
     "k2g_batch_digests",
+    "k2g_classify_device_noalloc",
     "k2g_db_gather_ceiling",
     "k2g_host_alloc",
     "k2g_host_free",

@@ -590,17 +590,27 @@ extern "C" int k2g_batch_create(k2g_db *db, k2g_batch **out) {
   hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&b->ev0);
   if (e == hipSuccess) e = hipEventCreate(&b->ev1);
-  for (int vi = 0; vi < 3 && e == hipSuccess; vi++) {
-    int per_cu = 0;
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kVariants[vi].persistent, 64, 0);
-    // at least one wave per cursor of the persistent kernel
-    b->persistent_blocks[vi] = std::max(8, std::max(1, per_cu) * std::max(1, db->num_cus));
-  }
   if (e != hipSuccess) {
     k2g_batch_destroy(b);
     return fail(K2G_ESOFTWARE, std::string("batch setup: ") + hipGetErrorString(e));
   }
+  const int rc = k2g::persistent_grid(b, db->num_cus);
+  if (rc) {
+    k2g_batch_destroy(b);
+    return rc;
+  }
   *out = b;
+  return K2G_OK;
+}
+
+// Resident one-wave workgroups of each persistent kernel variant on the
+// current device (at least one per cursor).
+int k2g::persistent_grid(k2g_batch *b, int num_cus) {
+  for (int vi = 0; vi < 3; vi++) {
+    int per_cu = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kVariants[vi].persistent, 64, 0));
+    b->persistent_blocks[vi] = std::max(8, std::max(1, per_cu) * std::max(1, num_cus));
+  }
   return K2G_OK;
 }
 
@@ -655,6 +665,12 @@ static int check_params(const k2g_classify_params *p) {
 extern "C" int k2g_classify_device(k2g_batch *b, const k2g_classify_params *p, const char *d_seq,
                                    const uint64_t *d_off, const uint8_t *d_paired, uint64_t n,
                                    const uint32_t *d_order) {
+  return k2g::classify_launch(b, p, d_seq, d_off, d_paired, n, d_order);
+}
+
+int k2g::classify_launch(k2g_batch *b, const k2g_classify_params *p, const char *d_seq,
+                         const uint64_t *d_off, const uint8_t *d_paired, uint64_t n,
+                         const uint32_t *d_order) {
   if (!b) return fail(K2G_EUSAGE, "null batch");
   int rc = check_params(p);
   if (rc) return rc;
@@ -711,6 +727,9 @@ extern "C" int k2g_classify_device(k2g_batch *b, const k2g_classify_params *p, c
   const uint64_t max_e = st[2] | ((uint64_t) st[3] << 32), sum_e = st[4] | ((uint64_t) st[5] << 32);
   const bool persistent = p->mode == 1 || (p->mode == 2 && (d_order || max_e * n > 4 * sum_e));
   b->ran_persistent = persistent;
+  if (b->fixed_outputs && 2 * total_space > b->runs.cap)
+    return fail(K2G_EUSAGE, "run capacity " + std::to_string(b->runs.cap / 2) +
+                                " is below the batch's element count " + std::to_string(total_space));
   HIP_TRY(b->runs.reserve(2 * std::max<uint64_t>(total_space, 1)));
   if (p->want_hll) HIP_TRY(b->recs.reserve(2 * std::max<uint64_t>(total_space, 1)));
   if (p->want_kmer_counts) {
@@ -785,7 +804,7 @@ extern "C" int k2g_classify_device(k2g_batch *b, const k2g_classify_params *p, c
   const int vi = variant_index(v.dna != 0, p->want_hll || p->want_kmer_counts ||
                                                p->want_probe_stats || p->quick);
   const ClassifyKernels &K = kVariants[vi];
-  HIP_TRY(hipEventRecord(b->ev0, s));
+  if (b->ev0) HIP_TRY(hipEventRecord(b->ev0, s));
   if (!persistent) {
     hipLaunchKernelGGL(K.plain, dim3((unsigned) n), dim3(64), 0, s, v, cp, in, out);
   } else {
@@ -794,7 +813,7 @@ extern "C" int k2g_classify_device(k2g_batch *b, const k2g_classify_params *p, c
     hipLaunchKernelGGL(K.persistent, dim3(grid), dim3(64), 0, s, v, cp, in, out, b->cursor.p);
   }
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipEventRecord(b->ev1, s));
+  if (b->ev1) HIP_TRY(hipEventRecord(b->ev1, s));
 
   // fragments to classify again: out of hitlist space, or hit table overflowed
   HIP_TRY(b->redo.reserve(n));
@@ -852,7 +871,18 @@ extern "C" int k2g_classify_device(k2g_batch *b, const k2g_classify_params *p, c
   }
   if (n_redo) {
     // hit table overflowed: classify again with a dense per-taxon table
-    const uint32_t slots = std::min<uint32_t>(n_redo, 256);
+    uint32_t slots = std::min<uint32_t>(n_redo, 256);
+    if (b->fixed_outputs && b->seq.region) {
+      // a fixed region: as many slots as its remaining room holds
+      const Region &r = *b->seq.region;
+      const uint64_t per_slot = 2 * region_bytes(v.node_count, 4) + kRegionAlign;
+      const uint64_t room = r.size > r.used + region_bytes(slots, 4) ? r.size - r.used - region_bytes(slots, 4)
+                                                                      : 0;
+      slots = (uint32_t) std::min<uint64_t>(slots, room / per_slot);
+      if (slots == 0)
+        return fail(K2G_EUSAGE, "no room left in the fixed scratch for one dense hit table of " +
+                                    std::to_string(v.node_count) + " taxa");
+    }
     if (b->dense_slots < slots) {
       HIP_TRY(b->dense_counts.reserve((uint64_t) slots * v.node_count));
       HIP_TRY(b->dense_lists.reserve((uint64_t) slots * v.node_count));

@@ -81,6 +81,13 @@ struct DevBuf {
     release();
     region = r;
   }
+  // Uses the caller's n elements at q (never freed: r must be non-null, the
+  // region the rest of the batch is carved from).
+  void adopt(T *q, size_t n, Region *r) {
+    region = r;
+    p = q;
+    cap = q ? n : 0;
+  }
 };
 
 // Growable pinned host buffer, also addressed by kernels (zero-copy writes).
@@ -167,6 +174,9 @@ struct k2g_batch {
   bool ran_persistent = false;  // which kernel the last call launched
   double kernel_ms = 0;
   int persistent_blocks[3] = {0, 0, 0};  // per kernel variant: resident waves
+  // outputs (call .. n_recs, run_off, runs, recs, kmer_counts) are the
+  // caller's arrays, adopted with their sizes: never grown or freed
+  bool fixed_outputs = false;
 
   template <class F>
   void for_each_device_buf(F f) {
@@ -187,6 +197,14 @@ struct k2g_batch {
 };
 
 namespace k2g {
+// The launch sequence of k2g_classify_device (results left in the batch's
+// device buffers, or in adopted caller arrays when b->fixed_outputs).
+int classify_launch(k2g_batch *b, const k2g_classify_params *p, const char *d_seq,
+                    const uint64_t *d_off, const uint8_t *d_paired, uint64_t n,
+                    const uint32_t *d_order);
+// Resident waves of each persistent kernel variant on the current device,
+// into b->persistent_blocks.
+int persistent_grid(k2g_batch *b, int num_cus);
 // Device bytes a k2g_classify call (host fragments, results fetched) takes
 // from a bound Region, as an upper bound; requires run_cap_pct == 100.
 uint64_t plan_classify_bytes(const k2g_db *db, const k2g_classify_params *p,
