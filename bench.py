@@ -119,6 +119,36 @@ def reference_blocks(off, paired):
     return blocks
 
 
+FRONTEND_BATCH_BASES = 256 << 20   # the classify front-end's GPU batch (csrc/host/classify_main.cc)
+FRONTEND_BATCH_BLOCKS = 4096
+
+
+def frontend_batches(blocks, off, paired):
+    """The classify front-end's GPU batches over the reference blocks: whole
+    blocks until 256 MiB of bases or 4096 blocks, one launch each; single
+    reads and pairs (separate inputs of the front-end) never share a batch."""
+    out, s = [], 0
+    while s < len(blocks):
+        e, bases = s, 0
+        kind = paired[blocks[s][0]]
+        while e < len(blocks) and bases < FRONTEND_BATCH_BASES and e - s < FRONTEND_BATCH_BLOCKS and \
+                paired[blocks[e][0]] == kind:
+            bases += int(off[2 * blocks[e][1]]) - int(off[2 * blocks[e][0]])
+            e += 1
+        out.append((blocks[s][0], blocks[e - 1][1]))
+        s = e
+    return out
+
+
+def cpu_model():
+    """The host CPU's model name and logical CPU count (from /proc/cpuinfo)."""
+    try:
+        names = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name")]
+        return f"{names[0]} ({len(names)} logical CPUs on the host)" if names else None
+    except OSError:
+        return None
+
+
 class Runner:
     """Drives k2g_classify_device over ranges of device-resident fragments on one batch."""
 
@@ -179,7 +209,7 @@ def run_host_workload(a, work, k2):
 def run_gpu_workload(a, work, k2):
     from kraken2_amd.synth import Workload
     tmp = a.keep or scratch_dir(f"k2g_{a.workload}_")
-    wl = Workload(tmp, seed=2, n_genomes=max(5, int(4000 * a.scale)), log=log)
+    wl = Workload(tmp, seed=2, n_genomes=max(5, int(4000 * a.scale)), verify=not a.no_verify, log=log)
     t0 = time.time()
     rs = wl.make_reads(max(100, int(work["n_reads"] * a.scale)), f_single=work["f_single"], f_paired=work["f_paired"],
                        paired_len=work["paired_len"], off_frac=work["off_frac"], seed=work["seed"],
@@ -189,7 +219,8 @@ def run_gpu_workload(a, work, k2):
     wl.free_genomes()
     return tmp, wl.hip, rs.d_seq, rs.d_off, rs.d_paired, rs.off, rs.paired, dict(readset=rs, build_s=wl.build_s,
                                                                                capacity=wl.capacity,
-                                                                               estimate=wl.estimate, pools=pools)
+                                                                               estimate=wl.estimate, pools=pools,
+                                                                               build_check=wl.build_check)
 
 
 def make_pools(wl, a):
@@ -391,6 +422,7 @@ def cpu_baseline(work, d, files, threads, exe=None):
         total_bp += float(m.group(2))
         total_s += float(m.group(3))
     return {"value": round(total_n / total_s, 1), "unit": "reads/s", "cores": threads, "kind": kind,
+            "cpu_model": cpu_model(), "binary": os.path.relpath(exe, ROOT),
             "sample": f"{total_n} fragments ({total_bp:.1f} Mbp) drawn evenly from the workload; the binary's own "
                       f"timed interval (after DB load, through parse, classify and output), {total_s:.2f} s"}
 
@@ -449,15 +481,25 @@ def oracle_check(k2, db, tmp, seq, off, pflags, work):
 
 def profiled_traffic(workload):
     """HBM bytes per launch of the persistent kernel measured by counters for
-    this workload (profiles/traffic.txt), or None."""
+    this workload (profiles/traffic.txt), or None.  Raw FETCH_SIZE is scaled
+    by the random-line calibration recorded there (k_gather_ceiling's raw
+    FETCH_SIZE per random 128-byte line it loads, against 128 bytes);
+    WRITE_SIZE is bytes already."""
     path = os.path.join(ROOT, "profiles", "traffic.txt")
     if not os.path.exists(path):
         return None
+    kern = calib = None
     for line in open(path):
         f = line.split()
-        if len(f) == 4 and not line.startswith("#") and f[0] == workload:
-            return int(f[2]) + int(f[3])
-    return None
+        if len(f) != 4 or line.startswith("#"):
+            continue
+        if f[0] == workload:
+            kern = (int(f[2]) * 1024, int(f[3]) * 1024)
+        elif f[0] == "gather":
+            calib = 128.0 / (int(f[2]) * 1024 / int(f[3]))
+    if kern is None or calib is None:
+        return None
+    return int(kern[0] * calib + kern[1])
 
 
 def copy_calibration():
@@ -644,15 +686,17 @@ def main():
 
     chunks = split_chunks(run_space(off, paired, k), int(a.chunk_space))
     blocks = reference_blocks(off, paired)
+    fbatches = frontend_batches(blocks, off, paired)
     runner = Runner(k2, db, d_seq, d_off, d_paired, work)
 
     passes = {
-        "plain": (blocks, 0),             # one plain launch per reference input block
-        "plain_one_launch": (chunks, 0),  # the plain kernel over the whole batch at once
+        "plain": (fbatches, 0),           # one plain launch per front-end batch (256 MiB of bases)
+        "plain_blocks": (blocks, 0),      # one plain launch per reference input block (3 MiB / 10 000 pairs)
+        "plain_one_launch": (chunks, 0),  # the plain kernel over the whole read set at once
         "batched": (chunks, 1),           # persistent, longest first
     }
     if a.skip_plain:
-        del passes["plain"]
+        del passes["plain"], passes["plain_blocks"]
 
     def run_pass(name):
         ranges, mode = passes[name]
@@ -663,9 +707,9 @@ def main():
     # over min(K, plain_steps) steps after a warmup over its first blocks.
     timing = {}
     for name in passes:
-        steps = a.steps if name != "plain" else max(1, min(a.steps, a.plain_steps))
+        steps = a.steps if name != "plain_blocks" else max(1, min(a.steps, a.plain_steps))
         for _ in range(a.warmup):
-            if name == "plain":
+            if name == "plain_blocks":
                 for s0, e0 in blocks[:16]:
                     runner.run(s0, e0, 0)
             else:
@@ -680,6 +724,7 @@ def main():
         log(f"# {name}: {wall * 1e3:.1f} ms/step ({len(passes[name][0])} launches), kernel {np.mean(kms):.1f} ms")
     batched_s, batched_kms, _ = timing["batched"]
     plain_s, plain_kms, plain_steps = timing.get("plain", (float("nan"), float("nan"), 0))
+    pblk_s, pblk_kms, pblk_steps = timing.get("plain_blocks", (float("nan"), float("nan"), 0))
 
     # ---- verification and measurement passes (outside the timed region)
     digests = {}
@@ -690,7 +735,7 @@ def main():
         if a.no_verify and name != "batched":
             continue
         ranges, mode = passes[name]
-        if name == "plain" and not a.full_verify and len(ranges) > 256:
+        if name == "plain_blocks" and not a.full_verify and len(ranges) > 256:
             ranges = ranges[::max(1, len(ranges) // 256)]  # an even spread of the blocks
         dig = np.zeros(n, np.uint64)
         mask = np.zeros(n, bool)
@@ -713,10 +758,12 @@ def main():
     if a.no_verify:
         modes_agree, plain_checked = True, 0
     else:
-        pm = checked.get("plain", np.zeros(n, bool))
+        pm = checked.get("plain_blocks", np.zeros(n, bool))
         plain_checked = int(pm.sum())
-        modes_agree = (np.array_equal(digests["plain"][pm], digests["batched"][pm]) if "plain" in digests
-                       else True) and np.array_equal(digests["plain_one_launch"], digests["batched"])
+        modes_agree = (np.array_equal(digests["plain_blocks"][pm], digests["batched"][pm])
+                       if "plain_blocks" in digests else True) and \
+            np.array_equal(digests["plain_one_launch"], digests["batched"]) and \
+            (np.array_equal(digests["plain"], digests["batched"]) if "plain" in digests else True)
 
     # CPU leg, part 1: the oracle checks an even sample of fragments (the
     # reference binary is timed further down); skipped with --no-cpu
@@ -742,7 +789,8 @@ def main():
         s_seq, so, _ = host["readset"].host_fragments(sample)
     mism, s_dig = oracle_check(k2, db, tmp, s_seq, so, paired[sample], work) if len(sample) else (0, np.zeros(0))
     sample_digest_agree = bool(np.array_equal(s_dig, digests["batched"][sample]))
-    log(f"# verify: modes agree ({plain_checked} plain / {n} one-launch fragments): {modes_agree}; oracle sample mismatches: "
+    log(f"# verify: modes agree ({plain_checked} per-block / {n} front-end batch and one-launch fragments): "
+        f"{modes_agree}; oracle sample mismatches: "
         f"{mism}/{len(sample)}; sample digests equal the bench run's: {sample_digest_agree}")
 
     # roofline of the persistent classify kernel: algorithmic bytes per launch
@@ -816,14 +864,21 @@ def main():
         "persistent_launches_per_step": len(chunks),
         "plain": None if a.skip_plain else {
             "reads_per_s": round(n / plain_s, 1), "ms_per_step": round(plain_s * 1e3, 3),
-            "kernel_ms_per_step": round(plain_kms, 3), "launches_per_step": len(blocks),
+            "kernel_ms_per_step": round(plain_kms, 3), "launches_per_step": len(fbatches),
             "timed_steps": plain_steps,
-            "batches": "reference input blocks: 3 MiB of unpaired bases or 10 000 pairs"},
+            "batches": "the classify front-end's GPU batches: whole reference blocks up to 256 MiB of bases "
+                       "or 4096 blocks, one plain launch each"},
+        "plain_blocks": None if a.skip_plain else {
+            "reads_per_s": round(n / pblk_s, 1), "ms_per_step": round(pblk_s * 1e3, 3),
+            "kernel_ms_per_step": round(pblk_kms, 3), "launches_per_step": len(blocks),
+            "timed_steps": pblk_steps,
+            "batches": "reference input blocks: 3 MiB of unpaired bases or 10 000 pairs, one plain launch each"},
         "plain_one_launch": {"reads_per_s": round(n / timing["plain_one_launch"][0], 1),
                              "ms_per_step": round(timing["plain_one_launch"][0] * 1e3, 3),
                              "kernel_ms_per_step": round(timing["plain_one_launch"][1], 3),
                              "launches_per_step": len(chunks)},
         "batched_vs_plain": None if a.skip_plain else round(plain_s / batched_s, 3),
+        "batched_vs_plain_blocks": None if a.skip_plain else round(pblk_s / batched_s, 3),
         "batched_vs_plain_one_launch": round(timing["plain_one_launch"][0] / batched_s, 3),
         "bit_exact": bool(modes_agree and mism == 0 and sample_digest_agree),
         "verify": {"modes_equal": bool(modes_agree), "plain_fragments_compared": plain_checked,
@@ -854,6 +909,8 @@ def main():
         line["end_to_end_vs_cpu"] = round(end_to_end["value"] / cpu["value"], 2)
     if work["kind"] == "gpu":
         line["config"].update({"table_estimate": host["estimate"], "gpu_build_s": round(host["build_s"], 2)})
+        if host.get("build_check"):
+            line["build_verify"] = host["build_check"]
     print(json.dumps(line), flush=True)
     if not serving:
         runner.close()
