@@ -183,6 +183,116 @@ __global__ void k_reads(ReadParams rp, const uint8_t *kind, const uint64_t *off,
   }
 }
 
+// ---- proteomes and back-translated reads (translated search)
+//
+// Taxon t belongs to group t / per_group.  The group owns n_proteins
+// ancestral proteins whose residues are i.i.d. from fixed background
+// amino-acid frequencies; taxon t's copy keeps each residue with
+// probability `identity` and otherwise draws it again from the background
+// (two siblings then agree at about identity^2 of the positions).  Every
+// protein is stored followed by '*', as the build appends one.
+
+// Background frequencies (percent x 100) of the 20 amino acids, in the
+// order of kAminoAcids; they sum to 10000.
+constexpr char kAminoAcids[21] = "ARNDCQEGHILKMFPSTWYV";
+struct AaTable {
+  uint16_t cum[20];  // cumulative frequencies
+  char aa[20];
+};
+
+__device__ inline char background_aa(const AaTable &t, uint64_t x) {
+  const uint32_t u = (uint32_t) (x % 10000);
+  int i = 0;
+  while (i < 19 && u >= t.cum[i]) i++;
+  return t.aa[i];
+}
+
+struct ProteomeParams {
+  uint64_t seed;
+  uint32_t per_group, n_proteins;
+  double identity;
+  AaTable bg;
+};
+
+struct ProtChunk {
+  uint64_t out;   // output offset
+  uint32_t taxon, protein, pos, len, prot_len;  // residues [pos, pos+len) of a protein of prot_len
+};
+
+__global__ void k_proteomes(ProteomeParams pp, const ProtChunk *chunks, uint64_t n, char *out) {
+  const uint64_t c = blockIdx.x;
+  if (c >= n) return;
+  const ProtChunk ch = chunks[c];
+  const uint64_t anc = (uint64_t) (ch.taxon / pp.per_group) * pp.n_proteins + ch.protein;
+  const uint64_t own = (uint64_t) ch.taxon * pp.n_proteins + ch.protein;
+  for (uint32_t i = threadIdx.x; i < ch.len; i += blockDim.x) {
+    const uint32_t q = ch.pos + i;
+    char a;
+    if (q == ch.prot_len) {
+      a = '*';
+    } else if (unit(h3(pp.seed ^ 0x5151, own, q)) < pp.identity) {
+      a = background_aa(pp.bg, h3(pp.seed ^ 0x6161, anc, q));
+    } else {
+      a = background_aa(pp.bg, h3(pp.seed ^ 0x7171, own, q));
+    }
+    out[ch.out + i] = a;
+  }
+}
+
+// Codons of each amino acid (kAminoAcids order, standard genetic code),
+// bases packed two bits each (A=0 C=1 G=2 T=3, first base high).
+struct BackTable {
+  uint8_t n[20];
+  uint8_t codon[20][6];
+};
+
+struct ProtReadParams {
+  uint64_t seed;
+  uint32_t read_len, n_taxa, n_proteins;
+  double sub_rate;
+  BackTable back;
+};
+
+// One thread per read: a random protein of a random taxon, a window of
+// read_len/3 residues back-translated with uniformly random synonymous
+// codons, 1% substitutions, either strand.  Proteins shorter than the
+// window are passed over for another (up to 64 tries, then the read is
+// random sequence).
+__global__ void k_protein_reads(ProtReadParams rp, const char *prot, const uint64_t *poff, uint64_t n,
+                                char *seq) {
+  const uint64_t r = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  Rng rng{h3(rp.seed, 0xBEE5, r)};
+  const uint32_t codons = rp.read_len / 3;
+  char *o = seq + r * rp.read_len;
+  const char *P = nullptr;
+  for (int tries = 0; tries < 64 && !P; tries++) {
+    const uint64_t s = (rng.next() % rp.n_taxa) * rp.n_proteins + rng.next() % rp.n_proteins;
+    const uint64_t L = poff[s + 1] - poff[s] - 1;  // without the '*'
+    if (L >= codons) P = prot + poff[s] + rng.next() % (L - codons + 1);
+  }
+  const bool rev = (rng.next() & 1) != 0;
+  for (uint32_t i = 0; i < rp.read_len; i++) o[i] = "ACGT"[rng.next() & 3];
+  if (P) {
+    for (uint32_t j = 0; j < codons; j++) {
+      int a = 0;
+      while (a < 19 && kAminoAcids[a] != P[j]) a++;
+      const uint8_t c = rp.back.codon[a][rng.next() % rp.back.n[a]];
+      for (int b = 0; b < 3; b++) o[3 * j + b] = "ACGT"[(c >> (4 - 2 * b)) & 3];
+    }
+  }
+  for (uint32_t i = 0; i < rp.read_len; i++)
+    if (rng.u() < rp.sub_rate) o[i] = other_base(o[i], rng.next());
+  if (rev) {
+    for (uint32_t i = 0, j = rp.read_len - 1; i < j; i++, j--) {
+      const char t = o[i];
+      o[i] = comp(o[j]);
+      o[j] = comp(t);
+    }
+    if (rp.read_len & 1) o[rp.read_len / 2] = comp(o[rp.read_len / 2]);
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -272,6 +382,88 @@ int k2s_reads(uint64_t seed, uint64_t n, const uint8_t *kind, const uint64_t *of
   (void) hipFree(dk);
   (void) hipFree(doff);
   (void) hipFree(dg);
+  return e == hipSuccess ? 0 : 70;
+}
+
+// Ancestral protein lengths per (group, protein), lognormal, clipped (host,
+// deterministic).
+int k2s_protein_lengths(uint64_t seed, uint64_t n, double median, double sigma, double lo, double hi,
+                        uint32_t *lens) {
+  std::mt19937_64 eng(seed ^ 0x9807);
+  std::normal_distribution<double> nd(0.0, 1.0);
+  for (uint64_t i = 0; i < n; i++) {
+    double v = std::exp(std::log(median) + sigma * nd(eng));
+    lens[i] = (uint32_t) std::min(std::max(v, lo), hi);
+  }
+  return 0;
+}
+
+// Proteomes into d_out: sequence s = t * n_proteins + p (taxon t, protein
+// p) occupies [off[s], off[s+1]) = the protein then '*' (host offsets, whose
+// lengths are group lengths + 1).
+int k2s_proteomes(uint64_t seed, uint32_t n_taxa, uint32_t per_group, uint32_t n_proteins, double identity,
+                  const uint64_t *off, char *d_out) {
+  std::vector<ProtChunk> chunks;
+  for (uint32_t t = 0; t < n_taxa; t++)
+    for (uint32_t p = 0; p < n_proteins; p++) {
+      const uint64_t s = (uint64_t) t * n_proteins + p;
+      const uint32_t L = (uint32_t) (off[s + 1] - off[s]);  // including '*'
+      for (uint32_t q = 0; q < L; q += 4096)
+        chunks.push_back(ProtChunk{off[s] + q, t, p, q, std::min<uint32_t>(4096, L - q), L - 1});
+    }
+  if (chunks.empty()) return 0;
+  ProteomeParams pp{seed, per_group ? per_group : 1, n_proteins, identity, {}};
+  static const uint16_t freq[20] = {825, 553, 406, 545, 137, 393, 675, 707, 227, 596,
+                                    966, 584, 242, 386, 470, 656, 534, 108, 292, 698};
+  uint32_t acc = 0;
+  for (int i = 0; i < 20; i++) {
+    acc += freq[i];
+    pp.bg.cum[i] = (uint16_t) acc;
+    pp.bg.aa[i] = kAminoAcids[i];
+  }
+  pp.bg.cum[19] = 10000;
+  const size_t step = 1u << 22;  // chunks per launch (grid.x bound)
+  ProtChunk *d = nullptr;
+  if (hipMalloc((void **) &d, std::min(chunks.size(), step) * sizeof(ProtChunk)) != hipSuccess) return 71;
+  hipError_t e = hipSuccess;
+  for (size_t c0 = 0; c0 < chunks.size() && e == hipSuccess; c0 += step) {
+    const size_t m = std::min(step, chunks.size() - c0);
+    e = hipMemcpy(d, chunks.data() + c0, m * sizeof(ProtChunk), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_proteomes, dim3((unsigned) m), dim3(256), 0, 0, pp, d, (uint64_t) m, d_out);
+      e = hipDeviceSynchronize();
+    }
+  }
+  (void) hipFree(d);
+  return e == hipSuccess ? 0 : 70;
+}
+
+// n reads of read_len bases back-translated from the proteomes (d_prot,
+// sequence offsets poff on the host as for k2s_proteomes), at d_seq + r *
+// read_len.
+int k2s_protein_reads(uint64_t seed, uint64_t n, uint32_t read_len, uint32_t n_taxa, uint32_t n_proteins,
+                      const char *d_prot, const uint64_t *poff, double sub_rate, char *d_seq) {
+  // standard genetic code, codons in TCAG order (first base slowest)
+  static const char code[65] = "FFLLSSSSYY**CC*WLLLLPPPPHHQQRRRRIIIMTTTTNNKKSSRRVVVVAAAADDEEGGGG";
+  static const int tcag_to_acgt[4] = {3, 1, 0, 2};
+  ProtReadParams rp{seed, read_len, n_taxa, n_proteins, sub_rate, {}};
+  for (int i = 0; i < 64; i++) {
+    const char aa = code[i];
+    const char *hit = std::strchr(kAminoAcids, aa);
+    if (aa == '*' || !hit) continue;
+    const int a = (int) (hit - kAminoAcids);
+    const int b0 = tcag_to_acgt[i >> 4], b1 = tcag_to_acgt[(i >> 2) & 3], b2 = tcag_to_acgt[i & 3];
+    rp.back.codon[a][rp.back.n[a]++] = (uint8_t) ((b0 << 4) | (b1 << 2) | b2);
+  }
+  const uint64_t n_seq = (uint64_t) n_taxa * n_proteins;
+  uint64_t *dp = nullptr;
+  if (hipMalloc((void **) &dp, (n_seq + 1) * 8) != hipSuccess) return 71;
+  if (hipMemcpy(dp, poff, (n_seq + 1) * 8, hipMemcpyHostToDevice) != hipSuccess) return 71;
+  if (n)
+    hipLaunchKernelGGL(k_protein_reads, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, 0, rp, d_prot, dp, n,
+                       d_seq);
+  hipError_t e = hipDeviceSynchronize();
+  (void) hipFree(dp);
   return e == hipSuccess ? 0 : 70;
 }
 

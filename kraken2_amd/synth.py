@@ -235,3 +235,128 @@ class ReadSet:
         for p in (self.d_seq, self.d_off, self.d_paired):
             self.hip.free(p)
         self.d_seq = self.d_off = self.d_paired = None
+
+
+PROTEIN_TEMPLATE = "1" * 12
+
+
+def _synth_protein(lib):
+    u64, u32, dbl, vp = C.c_uint64, C.c_uint32, C.c_double, C.c_void_p
+    lib.k2s_protein_lengths.argtypes = [u64, u64, dbl, dbl, dbl, dbl, vp]
+    lib.k2s_proteomes.argtypes = [u64, u32, u32, u32, dbl, vp, vp]
+    lib.k2s_protein_reads.argtypes = [u64, u64, u32, u32, u32, vp, vp, dbl, vp]
+    return lib
+
+
+class ProteinWorkload:
+    """Translated search at the real shape: synthetic proteomes (taxa in
+    sibling groups of `per_group`; each group's ancestral proteins have
+    lognormal lengths and background amino-acid frequencies; each taxon keeps
+    a residue with probability `identity`, so siblings share about
+    identity^2 of their residues), a protein table (k=15, l=12) built from
+    them on the GPU, and back-translated nucleotide reads generated on the
+    GPU.  Everything is a function of the seed."""
+
+    def __init__(self, out_dir: str, *, seed: int, n_taxa=2000, per_group=5, n_proteins=3000,
+                 length_median=300.0, length_sigma=0.5, length_min=30.0, length_max=5000.0,
+                 identity=0.8 ** 0.5, k=15, l=12, load_factor=0.7, block=16384, subblock=2048,
+                 verify=False, log=print):
+        self.hip = Hip()
+        self.synth = _synth_protein(_synth())
+        self.dir = out_dir
+        self.seed = seed
+        os.makedirs(out_dir, exist_ok=True)
+        t0 = time.time()
+        parent, rank, name, levels = make_taxonomy()
+        genera = levels[-2]
+        n_groups = (n_taxa + per_group - 1) // per_group
+        step = max(1, len(genera) // n_groups)
+        kids = {}
+        for sp in levels[-1]:
+            kids.setdefault(parent[sp], []).append(sp)
+        taxa_ext = []
+        for g in genera[::step][:n_groups]:
+            taxa_ext.extend(kids[g][:per_group])
+        taxa_ext = taxa_ext[:n_taxa]
+        with open(os.path.join(out_dir, "taxo.k2d"), "wb") as f:
+            f.write(build_k2_taxonomy(parent, rank, name, taxa_ext))
+        _, ext = read_k2_taxonomy(os.path.join(out_dir, "taxo.k2d"))
+        e2i = {e: i for i, e in enumerate(ext)}
+        self.n_taxa, self.n_proteins = n_taxa, n_proteins
+        # ancestral lengths per (group, protein); a taxon's proteins take its group's
+        glens = np.zeros(n_groups * n_proteins, np.uint32)
+        self.synth.k2s_protein_lengths(seed, glens.shape[0], length_median, length_sigma, length_min, length_max,
+                                       glens.ctypes.data)
+        lens = glens.reshape(n_groups, n_proteins)[np.arange(n_taxa) // per_group].reshape(-1).astype(np.uint64)
+        self.seq_off = np.zeros(n_taxa * n_proteins + 1, np.uint64)
+        self.seq_off[1:] = np.cumsum(lens + 1)  # each protein followed by '*'
+        self.residues = int(self.seq_off[-1])
+        self.seq_taxid = np.repeat(np.array([e2i[e] for e in taxa_ext], np.uint32), n_proteins)
+        self.d_prot = self.hip.malloc(self.residues)
+        rc = self.synth.k2s_proteomes(seed, n_taxa, per_group, n_proteins, identity, self.seq_off.ctypes.data,
+                                      self.d_prot)
+        if rc:
+            raise RuntimeError(f"proteome generation failed ({rc})")
+        log(f"# proteomes: {n_taxa} taxa x {n_proteins} proteins = {self.residues / 1e9:.2f} G residues "
+            f"in {time.time() - t0:.1f}s")
+        lib = _lib.load_library()
+        mask = _lib.expand_spaced_seed_mask(PROTEIN_TEMPLATE, 4)
+        toggle = 0xE37E28C4271B5A2D
+        bp = _lib.BuildParams(k, l, mask, toggle, 0, block, subblock, 0, 1)
+        lib.k2g_estimate_capacity_device.argtypes = [C.POINTER(_lib.BuildParams), C.c_void_p, C.c_void_p,
+                                                     C.c_uint64, C.c_uint32, C.POINTER(C.c_uint64)]
+        est = C.c_uint64()
+        t1 = time.time()
+        n_seq = n_taxa * n_proteins
+        _lib._check(lib.k2g_estimate_capacity_device(C.byref(bp), self.d_prot, self.seq_off.ctypes.data, n_seq, 4,
+                                                     C.byref(est)))
+        self.estimate = est.value
+        self.capacity = int((est.value + 8192) / load_factor)
+        log(f"# capacity estimate {est.value} -> {self.capacity} cells in {time.time() - t1:.1f}s")
+        t1 = time.time()
+        bld = _lib.Builder(os.path.join(out_dir, "taxo.k2d"), self.capacity, k=k, l=l, spaced_seed_mask=mask,
+                           toggle_mask=toggle, block_size=block, subblock_size=subblock, dna=False)
+        lib.k2g_build_insert_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]
+        _lib._check(lib.k2g_build_insert_device(bld._h, self.d_prot, self.seq_off.ctypes.data,
+                                                self.seq_taxid.ctypes.data, n_seq))
+        info = bld.info()
+        self.build_s = time.time() - t1
+        log(f"# GPU protein build: {info['size']} cells filled of {info['capacity']} "
+            f"({info['size'] / info['capacity']:.3f}) in {self.build_s:.1f}s")
+        self.build_check = None
+        if verify:
+            t1 = time.time()
+            self.build_check = bld.verify_device(self.d_prot, self.seq_off, self.seq_taxid)
+            self.build_check["seconds"] = round(time.time() - t1, 2)
+            log(f"# build verification: {self.build_check}")
+        t1 = time.time()
+        bld.write(os.path.join(out_dir, "hash.k2d"))
+        bld.close()
+        opts = np.zeros(8, np.uint64)
+        opts[0], opts[1], opts[2], opts[3] = k, l, mask, toggle
+        raw = bytearray(opts.tobytes())
+        raw[32] = 0                                   # protein database
+        raw[48:52] = (1).to_bytes(4, "little")        # revcom_version
+        with open(os.path.join(out_dir, "opts.k2d"), "wb") as f:
+            f.write(bytes(raw))
+        log(f"# table written in {time.time() - t1:.1f}s")
+
+    def free_proteomes(self):
+        self.hip.free(self.d_prot)
+        self.d_prot = None
+
+    def make_reads(self, n, *, read_len=150, sub=0.01, seed=None):
+        """n single-end nucleotide reads back-translated from random windows
+        of the proteomes (uniformly random synonymous codons, `sub`
+        substitutions, either strand)."""
+        seed = self.seed if seed is None else seed
+        off = np.zeros(2 * n + 1, np.uint64)
+        ends = np.arange(1, n + 1, dtype=np.uint64) * np.uint64(read_len)
+        off[1:2 * n:2] = ends
+        off[2:2 * n + 1:2] = ends
+        d_seq = self.hip.malloc(n * read_len)
+        rc = self.synth.k2s_protein_reads(seed, n, read_len, self.n_taxa, self.n_proteins, self.d_prot,
+                                          self.seq_off.ctypes.data, sub, d_seq)
+        if rc:
+            raise RuntimeError(f"read generation failed ({rc})")
+        return ReadSet(self.hip, d_seq, off, np.zeros(n, np.uint8))

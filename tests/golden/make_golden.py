@@ -296,6 +296,26 @@ def translate():
 
 
 CODONS = {}
+AMINO = "ACDEFGHIKLMNPQRSTVWY"
+
+
+def protein_library(taxa, rng):
+    """Genus-level ancestral proteins (8 per genus of three taxa) and species
+    copies at 80% identity: (FASTA text, seqid2taxid.map text, proteins).
+    The golden protein database was built by the compiled reference from
+    exactly this library: rng = random.Random(23), taxa as k2synth
+    db-inputs --seed 21 --levels 2,2,3 lists them."""
+    anc = [["".join(rng.choice(AMINO) for _ in range(rng.randrange(150, 400))) for _ in range(8)]
+           for _ in range(4)]
+    fa, mp, prots = [], [], []
+    for ti, tax in enumerate(taxa):
+        for pi, p in enumerate(anc[ti // 3]):
+            q = "".join(c if rng.random() < 0.8 else rng.choice(AMINO) for c in p)
+            name = f"p{ti}_{pi}"
+            fa.append(f">{name}\n{q}\n")
+            mp.append(f"{name}\t{tax}\n")
+            prots.append(q)
+    return "".join(fa), "".join(mp), prots
 
 
 def protein_db():
@@ -307,7 +327,6 @@ def protein_db():
          "--levels", "2,2,3", "--ranks", "phylum,genus,species", "--genome-len", "10", "--sub", "0"])
     taxa = [line.split("\t")[1].strip() for line in open(os.path.join(tmp, "seqid2taxid.map"))]
     rng = random.Random(23)
-    aa = "ACDEFGHIKLMNPQRSTVWY"
     code = "KKNNRRSSTTTTIMIIEEDDGGGGAAAAVVVVQQHHRRRRPPPPLLLL**YY*WCCSSSSLLFF"
     bases = "AGCT"
     back = {}
@@ -316,17 +335,10 @@ def protein_db():
     # genus-level ancestral proteins, species = 80% identity copies
     lib = os.path.join(tmp, "library.faa")
     idmap = os.path.join(tmp, "seqid2taxid.map")
-    prots = []
+    fa, mp, prots = protein_library(taxa, rng)
     with open(lib, "w") as lf, open(idmap, "w") as mf:
-        anc = [["".join(rng.choice(aa) for _ in range(rng.randrange(150, 400))) for _ in range(8)]
-               for _ in range(4)]
-        for ti, tax in enumerate(taxa):
-            for pi, p in enumerate(anc[ti // 3]):
-                q = "".join(c if rng.random() < 0.8 else rng.choice(aa) for c in p)
-                name = f"p{ti}_{pi}"
-                lf.write(f">{name}\n{q}\n")
-                mf.write(f"{name}\t{tax}\n")
-                prots.append(q)
+        lf.write(fa)
+        mf.write(mp)
     est, cap = build_db(d, lib, os.path.join(tmp, "taxonomy"), idmap, 15, 12, "1" * 12, protein=True)
     shutil.copytree(os.path.join(tmp, "taxonomy"), os.path.join(d, "taxonomy"), dirs_exist_ok=True)
     shutil.rmtree(tmp)

@@ -36,11 +36,14 @@ def read_table(path):
     return cells, kb, vb, size
 
 
-def inserted_pairs(orc, genomes, taxa, k, l, mask, toggle, block, subblock):
-    """(minimizers, taxa) of every insertion the build makes, in window order."""
+def inserted_pairs(orc, genomes, taxa, k, l, mask, toggle, block, subblock, dna=True):
+    """(minimizers, taxa) of every insertion the build makes, in window order.
+    Protein sequences get the '*' the build appends (src/build_db.h:113-114)."""
     keys, tax = [], []
     for g, t in zip(genomes, taxa):
         raw = g.encode() if isinstance(g, str) else g
+        if not dna and not raw.endswith(b"*"):
+            raw += b"*"
         L = len(raw)
         cap = 2 * subblock + 2 * k + 16
         amb = np.zeros(cap, np.uint8)
@@ -49,7 +52,7 @@ def inserted_pairs(orc, genomes, taxa, k, l, mask, toggle, block, subblock):
             bfin = min(j + block + k - 1, L)
             for i in range(j, bfin, subblock):
                 fin = min(i + subblock + k - 1, bfin)
-                n = orc.lib.k2cpu_scan(k, l, mask, toggle, 1, i, fin, raw, L, amb.ctypes.data,
+                n = orc.lib.k2cpu_scan(k, l, mask, toggle, int(dna), i, fin, raw, L, amb.ctypes.data,
                                        mins.ctypes.data, cap)
                 assert n >= 0
                 na = mins[:n][amb[:n] == 0]

@@ -98,3 +98,71 @@ def test_build_db_cli_matches_reference_build(tmp_path):
                        capture_output=True)
     assert r.returncode == 0, r.stderr.decode()
     assert open(out).read() == open(os.path.join(TOY, "paired.kraken")).read()
+
+
+def test_protein_build_db_cli_matches_reference_build(tmp_path):
+    """k2g_estimate_capacity -X / k2g_build_db -X on the library the
+    compiled reference built the golden protein database from (k=15, l=12):
+    the same estimate and header, byte-identical taxo.k2d, field-identical
+    opts.k2d; every minimizer the build inserts is found in both tables with
+    each cell the LCA of the minimizers ending there and equal values where
+    neither table merges compacted keys; and the GPU-built table classifies
+    the golden reads (translated search, -R -K report) byte for byte as the
+    reference did."""
+    import importlib.util
+    import random
+    import kraken2_amd as k2
+    from helpers import K2SYNTH, taxonomy_external_ids
+    from helpers import buildcheck as bc
+    spec = importlib.util.spec_from_file_location("make_golden", os.path.join(HERE, "golden", "make_golden.py"))
+    mg = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mg)
+    d = str(tmp_path)
+    inp = os.path.join(d, "inputs")
+    r = subprocess.run([K2SYNTH, "db-inputs", "--out", inp, "--seed", "21", "--levels", "2,2,3", "--ranks",
+                        "phylum,genus,species", "--genome-len", "10", "--sub", "0"], capture_output=True)
+    assert r.returncode == 0, r.stderr.decode()
+    taxa = [line.split("\t")[1].strip() for line in open(os.path.join(inp, "seqid2taxid.map"))]
+    fa, mp, prots = mg.protein_library(taxa, random.Random(23))
+    open(os.path.join(d, "seqid2taxid.map"), "w").write(mp)
+    golden = os.path.join(HERE, "golden", "prot")
+    r = subprocess.run([_bin("k2g_estimate_capacity"), "-k", "15", "-l", "12", "-S", "1" * 12, "-X"],
+                       input=fa.encode(), capture_output=True)
+    assert r.returncode == 0, r.stderr.decode()
+    cap = int((int(r.stdout) + 8192) / 0.7)
+    h_ref = struct.unpack_from("<4Q", open(os.path.join(golden, "hash.k2d"), "rb").read(32))
+    assert cap == h_ref[0]
+    cmd = [_bin("k2g_build_db"), "-k", "15", "-l", "12", "-S", "1" * 12, "-X", "-H", f"{d}/hash.k2d",
+           "-t", f"{d}/taxo.k2d", "-o", f"{d}/opts.k2d", "-n", os.path.join(golden, "taxonomy"), "-m",
+           os.path.join(d, "seqid2taxid.map"), "-c", str(cap), "-p", "1", "-B", "16384", "-b", "2048", "-r", "0"]
+    r = subprocess.run(cmd, input=fa.encode(), capture_output=True)
+    assert r.returncode == 0, r.stderr.decode()
+    assert open(f"{d}/taxo.k2d", "rb").read() == open(os.path.join(golden, "taxo.k2d"), "rb").read()
+    mine, ref = open(f"{d}/opts.k2d", "rb").read(), open(os.path.join(golden, "opts.k2d"), "rb").read()
+    assert mine[:33] == ref[:33] and mine[40:60] == ref[40:60]
+    assert struct.unpack_from("<4Q", open(f"{d}/hash.k2d", "rb").read(32)) == h_ref
+
+    raw = open(f"{d}/taxo.k2d", "rb").read()
+    nn = int(np.frombuffer(raw, np.uint64, 1, 8)[0])
+    parent = np.frombuffer(raw, np.uint64, 7 * nn, 32).reshape(nn, 7)[:, 0].astype(np.int64)
+    e2i = {e: i for i, e in enumerate(taxonomy_external_ids(f"{d}/taxo.k2d"))}
+    keys, tax = bc.inserted_pairs(Oracle(), prots, [e2i[int(t)] for t in taxa for _ in range(8)], 15, 12,
+                                  (1 << 48) - 1, 0xE37E28C4271B5A2D, 16384, 2048, dna=False)
+    uk, own = bc.key_lcas(keys, tax, parent)
+    res = {}
+    for name, dd in (("gpu", d), ("reference", golden)):
+        cells, kb, vb, size = bc.read_table(os.path.join(dd, "hash.k2d"))
+        res[name] = bc.check_table(cells, kb, vb, size, parent, uk, own)
+        for f in ("missing", "not_covering", "unexplained", "orphan_cells"):
+            assert res[name][f] == 0, (name, f, res[name][f])
+    alone = ~res["gpu"]["shared_key_mask"] & ~res["reference"]["shared_key_mask"]
+    assert alone.sum() > 0.99 * len(uk)
+    assert np.array_equal(res["gpu"]["values"][alone], res["reference"]["values"][alone])
+
+    out, rep = os.path.join(d, "single.kraken"), os.path.join(d, "single.kreport")
+    r = subprocess.run([_bin("k2g_classify"), "-H", f"{d}/hash.k2d", "-t", f"{d}/taxo.k2d", "-o", f"{d}/opts.k2d",
+                        "-g", "0", "-R", rep, "-K", "-O", out, os.path.join(golden, "reads_1.fa")],
+                       capture_output=True)
+    assert r.returncode == 0, r.stderr.decode()
+    assert open(out).read() == open(os.path.join(golden, "single.kraken")).read()
+    assert open(rep).read() == open(os.path.join(golden, "single.kreport")).read()

@@ -269,7 +269,7 @@ def _stratified_sample(kind, per_kind, seed):
     return np.sort(np.concatenate(idx))
 
 
-def _full_size_check(k2, wl, tmp, n, *, f_single, f_paired, paired_len, off_frac, seed, confidence, per_kind):
+def _full_size_check(k2, wl, tmp, n, *, seed, confidence, per_kind, **read_kw):
     db = k2.Db.from_dir(tmp)
     batch = k2.Batch(db)
     rs = None
@@ -277,8 +277,7 @@ def _full_size_check(k2, wl, tmp, n, *, f_single, f_paired, paired_len, off_frac
     h = None
     try:
         _log(f"generating {n} reads (seed {seed}) on the GPU")
-        rs = wl.make_reads(n, f_single=f_single, f_paired=f_paired, paired_len=paired_len, off_frac=off_frac,
-                           seed=seed, group_pairs=True)
+        rs = wl.make_reads(n, seed=seed, **read_kw)
         _log(f"{rs.n} fragments, {rs.bases / 1e9:.2f} Gbp; plain kernel")
         d_plain = _device_digests(k2, batch, rs, 0, confidence, 2)
         _log("persistent kernel")
@@ -328,7 +327,7 @@ def test_c3_full_size(c2_table):
     import kraken2_amd as k2
     wl, tmp = c2_table
     kinds = _full_size_check(k2, wl, tmp, 10_000_000, f_single=0.4, f_paired=0.3, paired_len=250, off_frac=0.0,
-                             seed=3, confidence=0.1, per_kind=1500)
+                             seed=3, confidence=0.1, per_kind=1500, group_pairs=True)
     assert set(np.unique(kinds).tolist()) == {0, 1, 2}  # singles, pairs and long reads all sampled
 
 
@@ -338,4 +337,42 @@ def test_c2_full_size(c2_table):
     import kraken2_amd as k2
     wl, tmp = c2_table
     _full_size_check(k2, wl, tmp, 50_000_000, f_single=0.0, f_paired=1.0, paired_len=150, off_frac=0.2, seed=2,
-                     confidence=0.1, per_kind=4000)
+                     confidence=0.1, per_kind=4000, group_pairs=True)
+
+
+# ---------------------------------------------------------------- config 4
+
+@pytest.fixture(scope="module")
+def c4_table():
+    """The config-4 protein table: 2000 taxa x 3000 proteins (lognormal
+    lengths, median 300 aa, background amino-acid frequencies, siblings
+    sharing ~80% of residues), k=15 l=12, built and verified on the GPU."""
+    from kraken2_amd.synth import ProteinWorkload
+    tmp = large_scratch_dir("k2g_test_c4_", need_gb=20)
+    wl = None
+    try:
+        _log(f"config 4 table: 2000 x 3000 proteins into {tmp}")
+        wl = ProteinWorkload(tmp, seed=4, verify=True, log=_log)
+        yield wl, tmp
+    finally:
+        if wl is not None and wl.d_prot is not None:
+            wl.free_proteomes()
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+def test_c4_table_verified(c4_table):
+    wl, _ = c4_table
+    chk = wl.build_check
+    for f in ("missing", "not_covering", "unexplained", "orphan_cells"):
+        assert chk[f] == 0, (f, chk)
+    assert wl.residues > 1.8e9 and chk["keys"] > 5e8
+    assert chk["differ"] <= chk["shared_keys"]
+
+
+def test_c4_full_size(c4_table):
+    """Config 4: 5M nucleotide reads of 150 bp back-translated from the
+    proteomes (random synonymous codons, 1% substitutions, either strand),
+    six-frame translated search (src/aa_translate.cc:41-100), -g 2."""
+    import kraken2_amd as k2
+    wl, tmp = c4_table
+    _full_size_check(k2, wl, tmp, 5_000_000, seed=4, confidence=0.0, per_kind=4000)
