@@ -52,6 +52,10 @@ WORKLOADS = {
                "(4000 genomes, ~2.5e9 cells), -T 0.1 -g 2",
                seed=3, n_reads=10_000_000, f_single=0.4, f_paired=0.3, paired_len=250, off_frac=0.0,
                confidence=0.1, min_hit_groups=2),
+    "c4": dict(kind="protein", desc="c4: translated classify, 5M x 150 bp reads back-translated (random synonymous "
+               "codons, 1% substitutions, either strand) from 2000 synthetic proteomes x 3000 proteins (lognormal "
+               "median 300 aa, siblings ~80% identical), protein table k=15 l=12, six-frame search, -T 0 -g 2",
+               seed=4, n_reads=5_000_000, confidence=0.0, min_hit_groups=2),
     "c2": dict(kind="gpu", desc="c2: plain classify at real shape, 50M pairs 2x150 (20% off-target GC 0.41) "
                "against 4000 genomes / ~2.5e9 cells, -T 0.1 -g 2 -P",
                seed=2, n_reads=50_000_000, f_single=0.0, f_paired=1.0, paired_len=150, off_frac=0.2,
@@ -221,6 +225,20 @@ def run_gpu_workload(a, work, k2):
                                                                                capacity=wl.capacity,
                                                                                estimate=wl.estimate, pools=pools,
                                                                                build_check=wl.build_check)
+
+
+def run_protein_workload(a, work, k2):
+    """c4: proteomes, protein table and back-translated reads, all on the GPU."""
+    from kraken2_amd.synth import ProteinWorkload
+    tmp = a.keep or scratch_dir("k2g_c4_")
+    wl = ProteinWorkload(tmp, seed=work["seed"], n_taxa=max(5, int(2000 * a.scale)), verify=not a.no_verify,
+                         log=log)
+    t0 = time.time()
+    rs = wl.make_reads(max(100, int(work["n_reads"] * a.scale)), seed=work["seed"])
+    log(f"# reads: {rs.n} fragments, {rs.bases / 1e9:.2f} Gbp in {time.time() - t0:.1f}s")
+    wl.free_proteomes()
+    return tmp, wl.hip, rs.d_seq, rs.d_off, rs.d_paired, rs.off, rs.paired, dict(
+        readset=rs, build_s=wl.build_s, capacity=wl.capacity, estimate=wl.estimate, build_check=wl.build_check)
 
 
 def make_pools(wl, a):
@@ -675,6 +693,8 @@ def main():
     t_start = time.time()
     if work["kind"] == "host":
         tmp, hip, d_seq, d_off, d_paired, off, paired, host = run_host_workload(a, work, k2)
+    elif work["kind"] == "protein":
+        tmp, hip, d_seq, d_off, d_paired, off, paired, host = run_protein_workload(a, work, k2)
     else:
         tmp, hip, d_seq, d_off, d_paired, off, paired, host = run_gpu_workload(a, work, k2)
     n = len(paired)
@@ -855,7 +875,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u64/u32 integer",
-        "data": "synthetic (seeded generators; table built on the GPU)" if work["kind"] == "gpu"
+        "data": "synthetic (seeded generators; table built on the GPU)" if work["kind"] != "host"
                 else "synthetic (seeded generator; table built by the GPU build front-ends, k2g_estimate_capacity and k2g_build_db)",
         "config": {"workload": work["desc"] + ("" if a.scale == 1.0 else f" (scaled x{a.scale})"), "fragments": n,
                    "bases": total_bases, "table_cells": int(db.info.capacity), "parallelism": "single GPU"},
@@ -907,7 +927,7 @@ def main():
     if end_to_end:  # PCIe, parsing and formatting included: never the headline value
         line["gpu_end_to_end"] = end_to_end
         line["end_to_end_vs_cpu"] = round(end_to_end["value"] / cpu["value"], 2)
-    if work["kind"] == "gpu":
+    if work["kind"] in ("gpu", "protein"):
         line["config"].update({"table_estimate": host["estimate"], "gpu_build_s": round(host["build_s"], 2)})
         if host.get("build_check"):
             line["build_verify"] = host["build_check"]
