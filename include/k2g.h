@@ -188,6 +188,27 @@ int k2g_classify_device_noalloc(k2g_db *db, const k2g_classify_params *p, const 
                                 const uint64_t *d_off, const uint8_t *d_paired, uint64_t n,
                                 const k2g_device_results *out, void *hip_stream);
 
+/* ---- report: ReportKrakenStyle (src/reports.cc:241-259) of one classify
+ * call's results, written to path.  The fragments (their offsets and paired
+ * flags; seq is not read) are cut into the reference's input blocks
+ * (src/classify.cc:523-543: single reads until 3 MiB of bases, 10 000 pairs;
+ * a change between single and paired fragments starts a new block, as a new
+ * input does).  Each classified fragment counts one read for its call.  With
+ * kmer_data (-K) the per-taxon k-mer counts and distinct-minimizer
+ * estimates are folded block by block from the results' HLL records, the
+ * way the reference merges its per-block sketches (src/classify.cc:554,
+ * 640-645; src/hyperloglogplus.cc:471-509); the results must then carry
+ * them (want_hll).  For a served request this is the report the reference's
+ * daemon writes for the same reads run as one classify call. */
+typedef struct {
+  int32_t report_zeros; /* -z */
+  int32_t kmer_data;    /* -K */
+  int32_t paired;       /* every fragment is a pair (when in->paired is NULL) */
+} k2g_report_params;
+
+int k2g_report_write(k2g_db *db, const k2g_fragments *in, const k2g_results *r,
+                     const k2g_report_params *p, const char *path);
+
 /* Pinned host memory for request fragments (and anything else a caller
  * streams to the GPU): uploads from it run at full interconnect speed and
  * asynchronously.  No reference counterpart. */

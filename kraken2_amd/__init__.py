@@ -26,4 +26,5 @@ from ._lib import (  # noqa: F401
     format_hitlist,
     load_library,
     pack_fragments,
+    write_report,
 )

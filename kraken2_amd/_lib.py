@@ -42,6 +42,7 @@ EXPORTED_SYMBOLS = (
     "k2g_host_alloc",
     "k2g_host_free",
     "k2g_results_checksum",
+    "k2g_report_write",
     "k2g_server_create",
     "k2g_server_submit",
     "k2g_server_wait",
@@ -424,6 +425,46 @@ def _copy_results(res: Results, hll: bool, kmer_counts: bool) -> ClassifyOutput:
         redone=int(res.redone),
         kernel_ms=float(res.kernel_ms),
     )
+
+
+class ReportParams(C.Structure):
+    _fields_ = [("report_zeros", C.c_int32), ("kmer_data", C.c_int32), ("paired", C.c_int32)]
+
+
+def write_report(db: "Db", off: np.ndarray, out: ClassifyOutput, path: str, *, paired=False, paired_flags=None,
+                 zeros=False, kmer_data=False):
+    """kreport of one classify call's results (k2g_report_write): the
+    fragments' offsets and paired flags cut them into the reference's input
+    blocks; kmer_data needs results with HLL records."""
+    lib = load_library()
+    lib.k2g_report_write.argtypes = [C.c_void_p, C.POINTER(Fragments), C.POINTER(Results),
+                                     C.POINTER(ReportParams), C.c_char_p]
+    off = np.ascontiguousarray(off, dtype=np.uint64)
+    n = (off.shape[0] - 1) // 2
+    flags = None if paired_flags is None else np.ascontiguousarray(paired_flags, dtype=np.uint8)
+    fr = Fragments(None, off.ctypes.data, flags.ctypes.data if flags is not None else None, n)
+    keep = []
+
+    def ptr(a, ct):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a)
+        keep.append(a)
+        return a.ctypes.data_as(C.POINTER(ct))
+
+    r = Results()
+    r.n = n
+    r.call = ptr(out.call, C.c_uint32)
+    r.hit_groups = ptr(out.hit_groups, C.c_uint32)
+    r.total_kmers = ptr(out.total_kmers, C.c_uint32)
+    r.run_off = ptr(out.run_off, C.c_uint64)
+    r.runs = ptr(out.runs, C.c_uint32)
+    r.hll_off = ptr(out.hll_off, C.c_uint64)
+    r.hll = ptr(out.hll, C.c_uint32)
+    r.kmer_counts = ptr(out.kmer_counts, C.c_uint64)
+    r.node_count = int(db.info.node_count)
+    _check(lib.k2g_report_write(db._h, C.byref(fr), C.byref(r),
+                                C.byref(ReportParams(int(zeros), int(kmer_data), int(paired))), path.encode()))
 
 
 def format_hitlist(runs: np.ndarray, external_ids: np.ndarray) -> str:

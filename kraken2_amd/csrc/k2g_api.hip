@@ -345,6 +345,13 @@ extern "C" int k2g_db_open(const char *hash_k2d, const char *taxo_k2d, const cha
   }
 
   db->external_ids = std::move(ext);
+  {
+    std::string err;
+    if (!db->tax.load(taxo_k2d, err)) {
+      k2g_db_close(db);
+      return fail(K2G_EDATA, err);
+    }
+  }
   k2g_db_info &inf = db->info;
   inf.capacity = capacity;
   inf.size = hh[1];
@@ -1655,6 +1662,52 @@ uint64_t k2g::plan_classify_bytes(const k2g_db *db, const k2g_classify_params *p
   }
   b += region_bytes(n + 1, 8) * (p->want_hll ? 2 : 1);           // fetch offsets
   return b;
+}
+
+// ----------------------------------------------------------------------
+// kreport of one call's results (src/reports.cc:241-259), counters folded in
+// the reference's input blocks (src/classify.cc:523-543, 554, 640-645)
+extern "C" int k2g_report_write(k2g_db *db, const k2g_fragments *in, const k2g_results *r,
+                                const k2g_report_params *p, const char *path) {
+  if (!db || !in || !r || !p || !path) return fail(K2G_EUSAGE, "null argument");
+  const uint64_t n = in->n;
+  if (r->n != n) return fail(K2G_EUSAGE, "results and fragments differ in count");
+  if (n && (!in->off || !r->call)) return fail(K2G_EUSAGE, "null fragment offsets or calls");
+  if (p->kmer_data && n && (!r->hll_off || !r->hll))
+    return fail(K2G_EUSAGE, "a k-mer data report needs results with HLL records (want_hll)");
+  if (r->node_count != db->tax.nodes.size()) return fail(K2G_EUSAGE, "results of another database");
+  auto paired = [&](uint64_t i) { return in->paired ? in->paired[i] != 0 : p->paired != 0; };
+  k2host::TaxonCounters counters;
+  uint64_t classified = 0;
+  std::vector<std::pair<uint32_t, uint32_t>> recs;
+  for (uint64_t i = 0; i < n;) {
+    const bool pr = paired(i);
+    const uint64_t start = i;
+    if (pr) {
+      while (i < n && paired(i) && i - start < 10000) i++;
+    } else {
+      uint64_t bases = 0;
+      while (i < n && !paired(i) && bases < 3 * 1024 * 1024) {
+        bases += in->off[2 * i + 1] - in->off[2 * i];
+        i++;
+      }
+    }
+    recs.clear();
+    for (uint64_t f = start; f < i; f++) {
+      if (r->call[f] >= r->node_count) return fail(K2G_EUSAGE, "call is not a node of the taxonomy");
+      if (r->call[f]) {
+        counters[r->call[f]].n_reads++;
+        classified++;
+      }
+      if (p->kmer_data)
+        for (uint64_t j = r->hll_off[f]; j < r->hll_off[f + 1]; j++) recs.emplace_back(r->hll[2 * j], r->hll[2 * j + 1]);
+    }
+    if (p->kmer_data) k2host::fold_block_records(recs, counters);
+  }
+  if (!k2host::report_kraken_style(path, p->report_zeros != 0, p->kmer_data != 0, db->tax, counters, n,
+                                   n - classified))
+    return fail(K2G_EIO, std::string("unable to write report ") + path);
+  return K2G_OK;
 }
 
 // ----------------------------------------------------------------------

@@ -11,6 +11,7 @@
 #include <memory>
 #include <vector>
 
+#include "host/k2g_host.h"
 #include "k2g_device.h"
 #include "k2g_kernels.h"
 
@@ -137,6 +138,7 @@ struct k2g_db {
   uint32_t *d_parent = nullptr;
   uint32_t *d_span = nullptr;
   std::vector<uint64_t> external_ids;
+  k2host::Taxonomy tax;  // names and ranks, for reports
   k2g_db_info info{};
   int device = 0;
   int num_cus = 0;

@@ -105,6 +105,39 @@ void *k2cpu_db_open(const char *hash, const char *taxo, const char *opts) {
 
 void k2cpu_db_close(void *h) { delete (K2cpuDb *) h; }
 
+// The oracle's whole classify driver (input blocks, per-block counters and
+// HLL state, kraken output, kreport: src/classify.cc:304-352, 479-708,
+// src/reports.cc:241-259) over input files, with the handle's resident
+// index instead of loading one: the reference classify command line
+// "-p threads [-P] -T confidence -g min_hit_groups [-R report [-K] [-z]]
+// -O kraken_out file1 [file2]".  Returns the driver's exit status.
+int k2cpu_run_files(void *h, int threads, int paired, double confidence, int min_hit_groups,
+                    const char *report, int kmer_data, int zeros, const char *kraken_out,
+                    const char *file1, const char *file2) {
+  auto *d = (K2cpuDb *) h;
+  ClassifyOptions o;
+  o.num_threads = threads > 0 ? threads : 1;
+  o.paired_end_processing = paired != 0;
+  o.confidence_threshold = confidence;
+  o.minimum_hit_groups = min_hit_groups;
+  o.use_translated_search = !d->idx.opts.dna_db;
+  if (report) o.report_filename = report;
+  o.report_kmer_data = kmer_data != 0;
+  o.report_zero_counts = zeros != 0;
+  if (kraken_out) o.kraken_output_filename = kraken_out;
+  o.filenames.push_back(file1);
+  if (file2) o.filenames.push_back(file2);
+  ClassifyStats stats;
+  TaxonCounters counters;
+  double secs = 0;
+  int rc = run_classify(o, d->idx, stats, counters, &secs);
+  if (rc) return rc;
+  if (report)
+    report_kraken_style(o.report_filename, o.report_zero_counts, o.report_kmer_data, d->idx.tax, counters,
+                        stats.total_sequences, stats.total_sequences - stats.total_classified);
+  return 0;
+}
+
 void k2cpu_get_batch(void *h, const uint64_t *keys, uint32_t *out, long n) {
   ((K2cpuDb *) h)->idx.table.get_batch(keys, out, (size_t) n);
 }

@@ -165,7 +165,20 @@ class Oracle:
         lib.k2cpu_translate.restype = C.c_long
         lib.k2cpu_hll_script.argtypes = [C.c_char_p, vp, C.c_long]
         lib.k2cpu_hll_script.restype = C.c_long
+        lib.k2cpu_run_files.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_int, C.c_char_p, C.c_int, C.c_int,
+                                        C.c_char_p, C.c_char_p, C.c_char_p]
+        lib.k2cpu_run_files.restype = C.c_int
         self.lib = lib
+
+    def run_files(self, h, file1, file2=None, *, kraken_out, report=None, kmer_data=False, zeros=False,
+                  confidence=0.0, min_hit_groups=0, threads=8):
+        """The oracle's classify driver over FASTA/FASTQ files with the handle's
+        index (blocks, counters, kraken output and kreport)."""
+        enc = lambda x: x.encode() if x else None  # noqa: E731
+        rc = self.lib.k2cpu_run_files(h, threads, int(file2 is not None), confidence, min_hit_groups, enc(report),
+                                      int(kmer_data), int(zeros), enc(kraken_out), enc(file1), enc(file2))
+        if rc:
+            raise RuntimeError(f"oracle classify driver failed ({rc})")
 
     def open(self, d):
         h = self.lib.k2cpu_db_open(os.path.join(d, "hash.k2d").encode(), os.path.join(d, "taxo.k2d").encode(),

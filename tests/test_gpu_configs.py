@@ -376,3 +376,90 @@ def test_c4_full_size(c4_table):
     import kraken2_amd as k2
     wl, tmp = c4_table
     _full_size_check(k2, wl, tmp, 5_000_000, seed=4, confidence=0.0, per_kind=4000)
+
+
+# ---------------------------------------------------------------- config 5
+
+def test_c5_requests_reports_match_oracle(c2_table, tmp_path):
+    """Config-5-shaped serving against the resident config-2 table: requests
+    of log-uniform size (1 to 20 000 fragments) sliced from the config-2 and
+    config-3 read mixes, -T 0.1 -g 2 -R -K, submitted together to a server
+    with four slots.  Every request's results equal the same fragments
+    classified alone; its kraken output lines (call and hitlist) and its
+    -R -K kreport (read counts, k-mer counts, distinct-minimizer estimates
+    folded in the reference's input blocks) equal the oracle's classify
+    driver run over the request's reads as one classify call
+    (src/classify.cc:373-430 runs each request that way)."""
+    import kraken2_amd as k2
+    wl, tmp = c2_table
+    db = k2.Db.from_dir(tmp)
+    batch = k2.Batch(db)
+    orc = Oracle()
+    h = None
+    pools = {}
+    try:
+        for name, kw in (("pairs", dict(f_single=0.0, f_paired=1.0, paired_len=150, off_frac=0.2, seed=52)),
+                         ("uneven", dict(f_single=0.4, f_paired=0.3, paired_len=250, seed=53, group_pairs=True))):
+            rs = wl.make_reads(40000, **kw)
+            seq = rs.hip.download(rs.d_seq, rs.bases)
+            pools[name] = (seq, rs.off.copy(), rs.paired.copy())
+            rs.free()
+        rng = np.random.default_rng(55)
+        reqs = []
+        for i in range(24):
+            name = ("pairs", "uneven")[i % 2]
+            seq, off, pf = pools[name]
+            n_pool = len(pf)
+            size = int(np.exp(rng.uniform(0, np.log(20000))))
+            want_pair = bool(i % 4 == 1) if name == "uneven" else True
+            idx = np.flatnonzero(pf == int(want_pair))
+            s0 = int(idx[0]) + int(rng.integers(0, max(1, len(idx) - size)))
+            e0 = min(s0 + size, int(idx[-1]) + 1, n_pool)
+            a0, a2 = int(off[2 * s0]), int(off[2 * e0])
+            reqs.append((seq[a0:a2], (off[2 * s0:2 * e0 + 1] - off[2 * s0]).astype(np.uint64), want_pair))
+        kw = dict(confidence=0.1, min_hit_groups=2, kmer_counts=True, hll=True)
+        srv = k2.Server(db, 8 << 30, slots=4)
+        try:
+            tickets = [srv.submit(s, o, paired=p, **kw) for s, o, p in reqs]
+            outs = [srv.wait(t)[0] for t in tickets]
+            st = srv.stats()
+        finally:
+            srv.close()
+        assert st["max_concurrent"] > 1
+        _log(f"config 5: {len(reqs)} requests served ({sum(len(o) // 2 for _, o, _ in reqs)} fragments); "
+             "the oracle on each")
+        h = orc.open(tmp)
+        for i, ((s, o, p), out) in enumerate(zip(reqs, outs)):
+            n = (len(o) - 1) // 2
+            alone = batch.classify(s, o, paired=p, mode=2, **kw)
+            for f in ("call", "hit_groups", "total_kmers", "run_off", "runs", "hll_off", "hll", "kmer_counts"):
+                assert np.array_equal(getattr(out, f), getattr(alone, f)), (i, f)
+            # the request's reads as input files, run through the oracle's driver
+            d = tmp_path / f"r{i}"
+            d.mkdir()
+            m1, m2 = [], []
+            for j in range(n):
+                a0, a1, a2 = (int(x) for x in o[2 * j:2 * j + 3])
+                m1.append(b">r%d%s\n%s\n" % (j, b"/1" if p else b"", s[a0:a1]))
+                if p:
+                    m2.append(b">r%d/2\n%s\n" % (j, s[a1:a2]))
+            (d / "1.fa").write_bytes(b"".join(m1))
+            if p:
+                (d / "2.fa").write_bytes(b"".join(m2))
+            orc.run_files(h, str(d / "1.fa"), str(d / "2.fa") if p else None, kraken_out=str(d / "orc.kraken"),
+                          report=str(d / "orc.kreport"), kmer_data=True, confidence=0.1, min_hit_groups=2,
+                          threads=16)
+            lines = (d / "orc.kraken").read_text().splitlines()
+            assert len(lines) == n
+            for j, line in enumerate(lines):
+                f = line.split("\t")
+                runs = out.runs[out.run_off[j]:out.run_off[j + 1]]
+                assert (f[0], int(f[2]), f[4]) == ("C" if out.call[j] else "U", int(db.external_ids[out.call[j]]),
+                                                   k2.format_hitlist(runs, db.external_ids)), (i, j)
+            k2.write_report(db, o, out, str(d / "gpu.kreport"), paired=p, kmer_data=True)
+            assert (d / "gpu.kreport").read_text() == (d / "orc.kreport").read_text(), f"request {i} kreport"
+    finally:
+        if h:
+            orc.lib.k2cpu_db_close(h)
+        batch.close()
+        db.close()
