@@ -250,7 +250,7 @@ def make_pools(wl, a):
     for name, key, seed in (("pairs 2x150", "c2", 52), ("uneven mix", "c3", 53)):
         w = WORKLOADS[key]
         rs = wl.make_reads(m, f_single=w["f_single"], f_paired=w["f_paired"], paired_len=w["paired_len"],
-                           off_frac=w["off_frac"], seed=seed)
+                           off_frac=w["off_frac"], seed=seed, group_pairs=True)
         from kraken2_amd._lib import pinned_array
         seq = pinned_array(rs.bases)  # requests upload from pinned memory
         seq[:] = np.frombuffer(rs.hip.download(rs.d_seq, rs.bases), np.uint8)
@@ -263,14 +263,18 @@ def make_pools(wl, a):
 
 def serve_requests(pools, n_req, max_reads, rng):
     """Request sizes log-uniform in [1, max_reads] fragments, each a random
-    slice of one of the pools."""
+    slice of one of the pools, within its single reads or within its pairs
+    (a request is one classify call, which reads either)."""
     names = sorted(pools)
     reqs = []
     for _ in range(n_req):
         name = names[int(rng.integers(len(names)))]
         pool = pools[name]
-        size = min(pool["n"], int(np.exp(rng.uniform(0.0, np.log(max_reads)))))
-        s0 = int(rng.integers(0, pool["n"] - size + 1))
+        pf = pool["paired"]
+        n_single = int((pf == 0).sum())  # the pools hold their single reads first
+        lo, hi = (0, n_single) if rng.uniform() < n_single / len(pf) else (n_single, len(pf))
+        size = min(hi - lo, int(np.exp(rng.uniform(0.0, np.log(max_reads)))))
+        s0 = lo + int(rng.integers(0, hi - lo - size + 1))
         off = pool["off"][2 * s0:2 * (s0 + size) + 1]
         a0, a2 = int(off[0]), int(off[-1])
         reqs.append(dict(seq=pool["seq"][a0:a2], off=(off - off[0]).astype(np.uint64),
@@ -285,6 +289,7 @@ def serve_at_rate(k2, db, reqs, rate, budget, slots, work, rng):
     arrivals = np.cumsum(rng.exponential(1.0 / rate, len(reqs))) if rate else np.zeros(len(reqs))
     tickets = [None] * len(reqs)
     results = [None] * len(reqs)
+    submitted = [0.0] * len(reqs)
     ready = threading.Semaphore(0)
     errors = []
     turn = iter(range(len(reqs)))
@@ -304,7 +309,7 @@ def serve_at_rate(k2, db, reqs, rate, budget, slots, work, rng):
         except Exception as e:  # surfaced after join
             errors.append(e)
 
-    th = [threading.Thread(target=waiter) for _ in range(max(1, slots))]
+    th = [threading.Thread(target=waiter) for _ in range(max(8, slots))]
     for t in th:
         t.start()
     t0 = time.perf_counter()
@@ -313,6 +318,7 @@ def serve_at_rate(k2, db, reqs, rate, budget, slots, work, rng):
             delay = t0 + arrivals[i] - time.perf_counter()
             if delay > 0:
                 time.sleep(delay)
+            submitted[i] = time.perf_counter() - t0
             tickets[i] = srv.submit(r["seq"], r["off"], paired_flags=r["paired"], confidence=work["confidence"],
                                     min_hit_groups=work["min_hit_groups"], kmer_counts=True, hll=True)
             ready.release()
@@ -321,81 +327,129 @@ def serve_at_rate(k2, db, reqs, rate, budget, slots, work, rng):
             ready.release()
         for t in th:
             t.join()
-    wall = time.perf_counter() - t0
     stats = srv.stats()
     srv.close()
     if errors:
         raise errors[0]
+    # the server's own clock: from the first arrival to the last request done
+    # (the waiter threads' checksums run beside it, outside this span)
+    wall = max(submitted[i] + results[i][1]["total_ms"] / 1e3 for i in range(len(reqs)))
     return results, wall, stats
 
 
 def run_serving(k2, db, tmp, pools, a, work):
     """Config 5: the resident table serves requests of log-uniform size with
-    -R -K (per-taxon counts and HLL records) under a device-memory budget."""
+    -R -K (per-taxon counts and HLL records) under a device-memory budget.
+    One request set is served closed-loop by one slot and by all slots (the
+    saturation rates), then with Poisson arrivals at 0.5x, 1x and 2x the
+    one-slot rate.  Every request must equal itself classified alone, and a
+    spread of requests is checked against the oracle's classify driver:
+    output lines and the -R -K kreport."""
     rng = np.random.default_rng(5)
     max_reads = max(10, int(200_000 * min(1.0, a.scale * 20)))
     budget = int(a.serve_budget_gb * (1 << 30))
-    # service time of one request at a time (one slot, closed loop)
-    probe = serve_requests(pools, 30, max_reads, rng)
-    res, wall, _ = serve_at_rate(k2, db, probe, 0, budget, 1, work, rng)
-    seq_ms = float(np.mean([t["gpu_ms"] for _, t in res]))
-    mu = 1000.0 / seq_ms
-    log(f"# serving: one request at a time takes {seq_ms:.1f} ms on average ({mu:.1f} req/s)")
-    rates_out = []
-    all_reqs, all_dig = [], []
-    for mult in (0.5, 1.0, 2.0):
-        reqs = serve_requests(pools, a.serve_requests, max_reads, rng)
-        rate = mult * mu
-        res, wall, st = serve_at_rate(k2, db, reqs, rate, budget, a.serve_slots, work, rng)
+    reqs = serve_requests(pools, a.serve_requests, max_reads, rng)
+    reads = sum(r["size"] for r in reqs)
+    bases = sum(r["bases"] for r in reqs)
+    legs, digests = [], []
+
+    def leg(name, rate, slots):
+        res, wall, st = serve_at_rate(k2, db, reqs, rate, budget, slots, work, rng)
         lat = np.array([t["total_ms"] for _, t in res])
-        qd = np.array([t["queued_ms"] for _, t in res])
-        reads = sum(r["size"] for r in reqs)
-        bases = sum(r["bases"] for r in reqs)
-        row = {"offered_req_per_s": round(rate, 2), "completed_req_per_s": round(len(reqs) / wall, 2),
-               "requests": len(reqs), "reads_per_s": round(reads / wall, 1), "mbp_per_s": round(bases / wall / 1e6, 1),
+        row = {"leg": name, "slots": slots, "offered_req_per_s": round(rate, 2) if rate else None,
+               "completed_req_per_s": round(len(reqs) / wall, 2), "reads_per_s": round(reads / wall, 1),
+               "mbp_per_s": round(bases / wall / 1e6, 1),
                "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
-               "queued_p50_ms": round(float(np.percentile(qd, 50)), 3),
+               "service_ms_mean": round(float(np.mean([t["gpu_ms"] for _, t in res])), 3),
                "max_concurrent": st["max_concurrent"], "peak_bytes": st["peak_bytes"]}
-        rates_out.append(row)
-        log(f"# serving at {rate:.1f} req/s: {row}")
-        all_reqs += reqs
-        all_dig += [d for d, _ in res]
-    # every request alone (one batch, one request at a time) must give the same results
+        log(f"# serving {name}: {row}")
+        legs.append(row)
+        digests.append([d for d, _ in res])
+        return row
+
+    one = leg("closed loop, one slot", 0, 1)
+    leg(f"closed loop, {a.serve_slots} slots", 0, a.serve_slots)
+    mu = one["completed_req_per_s"]
+    for mult in (0.5, 1.0, 2.0):
+        leg(f"Poisson {mult}x the one-slot rate", mult * mu, a.serve_slots)
+    # every request alone (one batch, one request at a time) gives the same results
     batch = k2.Batch(db)
     kw = dict(confidence=work["confidence"], min_hit_groups=work["min_hit_groups"], kmer_counts=True, hll=True)
-    mismatched = [i for i, (r, d) in enumerate(zip(all_reqs, all_dig))
-                  if batch.classify_checksum(r["seq"], r["off"], paired_flags=r["paired"], mode=1, **kw) != d]
-    alone_equal = not mismatched
-    for i in mismatched[:4]:  # describe a few: which side changes when run again
-        r = all_reqs[i]
-        again = [batch.classify_checksum(r["seq"], r["off"], paired_flags=r["paired"], mode=m, run_cap_pct=pct, **kw)
-                 for m, pct in ((1, 0), (1, 100), (0, 100))]
+    alone = [batch.classify_checksum(r["seq"], r["off"], paired_flags=r["paired"], mode=1, **kw) for r in reqs]
+    batch.close()
+    mismatched = sorted({i for dig in digests for i, d in enumerate(dig) if d != alone[i]})
+    # the oracle's classify driver on a spread of request sizes
+    n_check = 0 if a.no_cpu else min(len(reqs), a.serve_oracle)
+    by_size = sorted(range(len(reqs)), key=lambda i: reqs[i]["size"])
+    pick = [by_size[int(j)] for j in np.linspace(0, len(reqs) - 1, n_check)] if n_check else []
+    checked = mism = rep_mism = 0
+    if pick:
+        from helpers import Oracle
+        orc = Oracle()
+        h = orc.open(tmp)
         srv = k2.Server(db, budget, 1)
         try:
-            solo = srv.wait(srv.submit(r["seq"], r["off"], paired_flags=r["paired"], **kw), checksum_only=True)[0]
+            for i in pick:
+                r = reqs[i]
+                out, _ = srv.wait(srv.submit(r["seq"], r["off"], paired_flags=r["paired"], **kw))
+                m, rm = oracle_request_check(k2, db, orc, h, tmp, r, out, work)
+                mism += m
+                rep_mism += rm
+                checked += r["size"]
         finally:
             srv.close()
-        log(f"# serving mismatch: request {i} ({r['size']} fragments, {r['bases']} bases, {r['pool']}): "
-            f"in the mix {all_dig[i]:#x}, alone again {[hex(x) for x in again]}, one-slot server {solo:#x}")
-    batch.close()
-    # and, in the CPU leg (not with --no-cpu), the oracle on the smaller
-    # requests (calls and hitlists)
-    small = [] if a.no_cpu else sorted(range(len(all_reqs)), key=lambda i: all_reqs[i]["size"])[:40]
-    mism = checked = 0
-    for i in small:
-        r = all_reqs[i]
-        m, _ = oracle_check(k2, db, tmp, r["seq"].tobytes(), r["off"], r["paired"], work)
-        mism += m
-        checked += r["size"]
-    log(f"# serving verify: all {len(all_reqs)} requests equal alone: {alone_equal}; oracle mismatches "
-        f"{mism} over {checked} fragments of {len(small)} requests")
-    return {"workload": "c5: requests of log-uniform size in [1, %d] fragments sliced from the pairs 2x150 and "
-                        "uneven-length mixes, Poisson arrivals, -R -K (counts + HLL records), against the resident "
-                        "table" % max_reads,
-            "budget_bytes": budget, "slots": a.serve_slots, "sequential_service_ms": round(seq_ms, 3),
-            "rates": rates_out, "bit_exact_vs_alone": bool(alone_equal), "mismatched_requests": len(mismatched),
-            "oracle_requests": len(small),
-            "oracle_fragments": checked, "oracle_mismatches": int(mism)}
+            orc.lib.k2cpu_db_close(h)
+    log(f"# serving verify: {len(mismatched)} requests differ from themselves alone; oracle: {mism} fragment "
+        f"and {rep_mism} kreport mismatches over {len(pick)} requests ({checked} fragments)")
+    sat = legs[1]["completed_req_per_s"] / mu
+    two = legs[-1]["completed_req_per_s"] / mu
+    return {"workload": "c5: %d requests of log-uniform size in [1, %d] fragments sliced from the pairs 2x150 "
+                        "and uneven-length mixes (single reads or pairs), -R -K (counts + HLL records), against "
+                        "the resident table" % (len(reqs), max_reads),
+            "requests": len(reqs), "reads": reads, "bases": bases,
+            "budget_bytes": budget, "slots": a.serve_slots, "legs": legs,
+            "saturation_vs_one_slot": round(sat, 3), "completed_at_2x_vs_one_slot": round(two, 3),
+            "bit_exact_vs_alone": not mismatched, "mismatched_requests": len(mismatched),
+            "oracle_requests": len(pick), "oracle_fragments": checked, "oracle_mismatches": int(mism),
+            "oracle_kreport_mismatches": int(rep_mism)}
+
+
+def oracle_request_check(k2, db, orc, h, tmp, r, out, work):
+    """One served request against the oracle's classify driver run over the
+    request's reads as input files (one classify call, src/classify.cc:
+    373-430): (fragments whose output line differs, 1 if the -R -K kreport
+    differs)."""
+    d = tempfile.mkdtemp(prefix="req_", dir=tmp)
+    try:
+        paired = bool(r["paired"][0]) if r["size"] else False
+        seq, off = r["seq"], r["off"]
+        m1, m2 = [], []
+        for j in range(r["size"]):
+            a0, a1, a2 = (int(x) for x in off[2 * j:2 * j + 3])
+            m1.append(b">r%d%s\n%s\n" % (j, b"/1" if paired else b"", seq[a0:a1].tobytes()))
+            if paired:
+                m2.append(b">r%d/2\n%s\n" % (j, seq[a1:a2].tobytes()))
+        f1, f2 = os.path.join(d, "1.fa"), os.path.join(d, "2.fa")
+        open(f1, "wb").write(b"".join(m1))
+        if paired:
+            open(f2, "wb").write(b"".join(m2))
+        orc.run_files(h, f1, f2 if paired else None, kraken_out=os.path.join(d, "o.kraken"),
+                      report=os.path.join(d, "o.kreport"), kmer_data=True, confidence=work["confidence"],
+                      min_hit_groups=work["min_hit_groups"], threads=os.cpu_count() or 1)
+        mism = 0
+        lines = open(os.path.join(d, "o.kraken")).read().splitlines()
+        if len(lines) != r["size"]:
+            return r["size"], 1
+        for j, line in enumerate(lines):
+            f = line.split("\t")
+            runs = out.runs[out.run_off[j]:out.run_off[j + 1]]
+            mism += (f[0], int(f[2]), f[4]) != ("C" if out.call[j] else "U", int(db.external_ids[out.call[j]]),
+                                                 k2.format_hitlist(runs, db.external_ids))
+        k2.write_report(db, off, out, os.path.join(d, "g.kreport"), paired=paired, kmer_data=True)
+        rep = open(os.path.join(d, "g.kreport")).read() != open(os.path.join(d, "o.kreport")).read()
+        return mism, int(rep)
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
 
 
 def write_sample_fasta(path1, path2, frags):
@@ -664,7 +718,8 @@ def main():
                     help="only run a 1 GiB device copy (calibrates HBM byte counters) and exit")
     ap.add_argument("--full-verify", action="store_true", help="digest every plain block, not an even spread")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
-    ap.add_argument("--serve-requests", type=int, default=300, help="requests per offered rate")
+    ap.add_argument("--serve-requests", type=int, default=300, help="requests in the serving set")
+    ap.add_argument("--serve-oracle", type=int, default=48, help="served requests checked against the oracle")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-budget-gb", type=float, default=32.0)
     ap.add_argument("--serve-pool", type=int, default=300_000, help="fragments per serving pool")
@@ -918,9 +973,9 @@ def main():
         "cpu_baseline": cpu,
     }
     if serving:
-        mid = serving["rates"][1]
+        mid = serving["legs"][3]  # Poisson arrivals at the one-slot rate
         line["latency_ms"] = {"p50": mid["p50_ms"], "p99": mid["p99_ms"],
-                              "offered_req_per_s": mid["offered_req_per_s"]}
+                              "offered_req_per_s": mid["offered_req_per_s"], "slots": mid["slots"]}
         line["serving"] = serving
     if cpu:
         line["vs_cpu"] = round(value / cpu["value"], 2)

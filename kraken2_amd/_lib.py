@@ -595,7 +595,7 @@ class Server:
         self._keep = {}
 
     def submit(self, seq: bytes, off: np.ndarray, *, paired=False, paired_flags=None, confidence=0.0,
-               min_hit_groups=0, quick=False, mode=1, kmer_counts=False, hll=False, table_slots=0) -> int:
+               min_hit_groups=0, quick=False, mode=2, kmer_counts=False, hll=False, table_slots=0) -> int:
         off = np.ascontiguousarray(off, dtype=np.uint64)
         n = (off.shape[0] - 1) // 2
         p = ClassifyParams(confidence, min_hit_groups, int(quick), int(paired), mode,

@@ -30,16 +30,16 @@ K2G_DECLARE_VARIANT(_dna_x)
 K2G_DECLARE_VARIANT(_prot)
 #undef K2G_DECLARE_VARIANT
 __global__ void k_classify_dense_dna(DbView, ClassifyParams, FragBatch, FragOut, const uint32_t *,
-                                     uint32_t, DenseScratch);
+                                     const uint32_t *, DenseScratch);
 __global__ void k_classify_dense_prot(DbView, ClassifyParams, FragBatch, FragOut, const uint32_t *,
-                                      uint32_t, DenseScratch);
+                                      const uint32_t *, DenseScratch);
 
 // The kernel variants a batch launches: nucleotide without per-hit extras or
 // quick mode (the hot path), nucleotide with them, translated search.
 struct ClassifyKernels {
   void (*plain)(DbView, ClassifyParams, FragBatch, FragOut);
   void (*persistent)(DbView, ClassifyParams, FragBatch, FragOut, unsigned long long *);
-  void (*dense)(DbView, ClassifyParams, FragBatch, FragOut, const uint32_t *, uint32_t,
+  void (*dense)(DbView, ClassifyParams, FragBatch, FragOut, const uint32_t *, const uint32_t *,
                 DenseScratch);
 };
 inline int variant_index(bool dna, bool extras) { return dna ? (extras ? 1 : 0) : 2; }
@@ -582,6 +582,21 @@ __global__ __launch_bounds__(64) void k_compact(const uint64_t *src_off, const u
   }
 }
 
+// v[0 .. n) = x (a fill that, unlike a memset node, is safe to record in a
+// graph and replay).
+__global__ void k_fill_u32(uint32_t *v, uint32_t n, uint32_t x) {
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) v[i] = x;
+}
+
+// *out |= every status word (one atomic per wave).
+__global__ __launch_bounds__(256) void k_status_or(const uint32_t *status, uint64_t n, unsigned int *out) {
+  uint32_t acc = 0;
+  for (uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t) gridDim.x * blockDim.x)
+    acc |= status[i];
+  for (int d = 1; d < kWave; d <<= 1) acc |= (uint32_t) __shfl_xor((unsigned int) acc, d);
+  if ((threadIdx.x & (kWave - 1)) == 0 && acc) atomicOr(out, acc);
+}
+
 __global__ void k_widen(const uint32_t *in, uint64_t *out, uint64_t n) {
   const uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = in[i];
@@ -722,16 +737,25 @@ int k2g::classify_launch(k2g_batch *b, const k2g_classify_params *p, const char 
   HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tbytes, b->run_cap.p, b->run_off.p, n + 1, s));
   HIP_TRY(b->temp.reserve(tbytes));
   HIP_TRY(hipcub::DeviceScan::ExclusiveSum(b->temp.p, tbytes, b->run_cap.p, b->run_off.p, n + 1, s));
-  uint64_t total_space = 0;
-  uint32_t st[6] = {0, 0, 0, 0, 0, 0};
-  HIP_TRY(hipMemcpyAsync(&total_space, b->run_off.p + n, 8, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipMemcpyAsync(st, stats, 24, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
-  if (st[0]) return fail(K2G_EUSAGE, "a read has 2^31 or more bases");
+  uint64_t total_space = 0, max_e = 0, sum_e = 0;
+  if (b->plan.valid) {  // the caller's host-side statistics: nothing to read back
+    b->plan.valid = false;
+    if (b->plan.too_long) return fail(K2G_EUSAGE, "a read has 2^31 or more bases");
+    total_space = b->plan.space;
+    max_e = b->plan.max_e;
+    sum_e = b->plan.sum_e;
+  } else {
+    uint32_t st[6] = {0, 0, 0, 0, 0, 0};
+    HIP_TRY(hipMemcpyAsync(&total_space, b->run_off.p + n, 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(st, stats, 24, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (st[0]) return fail(K2G_EUSAGE, "a read has 2^31 or more bases");
+    max_e = st[2] | ((uint64_t) st[3] << 32);
+    sum_e = st[4] | ((uint64_t) st[5] << 32);
+  }
   // mode 2: the persistent kernel pays for its sort and work units only when
   // lengths are uneven (the longest fragment above four times the mean);
   // batches of similar reads run one plain launch
-  const uint64_t max_e = st[2] | ((uint64_t) st[3] << 32), sum_e = st[4] | ((uint64_t) st[5] << 32);
   const bool persistent = p->mode == 1 || (p->mode == 2 && (d_order || max_e * n > 4 * sum_e));
   b->ran_persistent = persistent;
   if (b->fixed_outputs && 2 * total_space > b->runs.cap)
@@ -830,9 +854,15 @@ int k2g::classify_launch(k2g_batch *b, const k2g_classify_params *p, const char 
   hipLaunchKernelGGL(k_collect_redo, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, s,
                      b->status.p, n, b->redo.p, b->space.p, b->space_only.p, b->redo_count.p);
   HIP_TRY(hipGetLastError());
+  // with exact hitlist space no fragment can need more room, and the dense
+  // redo takes its fragment count from the device: no host round trip
+  const bool on_device = pct >= 100;
   uint32_t counts[3] = {0, 0, 0};
-  HIP_TRY(hipMemcpyAsync(counts, b->redo_count.p, 12, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
+  if (!on_device) {
+    HIP_TRY(hipMemcpyAsync(counts, b->redo_count.p, 12, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+  }
+  b->redo_on_device = on_device;
   const uint32_t n_redo = counts[0], n_space = counts[1], n_space_only = counts[2];
   FragOut out2 = out;
   out2.kmer_counts = nullptr;  // the first pass counted every fragment
@@ -876,9 +906,9 @@ int k2g::classify_launch(k2g_batch *b, const k2g_classify_params *p, const char 
     }
     b->space_redone = n_space;
   }
-  if (n_redo) {
+  if (n_redo || on_device) {
     // hit table overflowed: classify again with a dense per-taxon table
-    uint32_t slots = std::min<uint32_t>(n_redo, 256);
+    uint32_t slots = on_device ? dense_slots_for(n, v.node_count) : std::min<uint32_t>(n_redo, 256);
     if (b->fixed_outputs && b->seq.region) {
       // a fixed region: as many slots as its remaining room holds
       const Region &r = *b->seq.region;
@@ -890,22 +920,33 @@ int k2g::classify_launch(k2g_batch *b, const k2g_classify_params *p, const char 
         return fail(K2G_EUSAGE, "no room left in the fixed scratch for one dense hit table of " +
                                     std::to_string(v.node_count) + " taxa");
     }
-    if (b->dense_slots < slots) {
+    if (b->dense_slots < slots) {  // each slot's table is cleared by the kernel that uses it
       HIP_TRY(b->dense_counts.reserve((uint64_t) slots * v.node_count));
       HIP_TRY(b->dense_lists.reserve((uint64_t) slots * v.node_count));
       HIP_TRY(b->dense_lens.reserve(slots));
-      HIP_TRY(hipMemsetAsync(b->dense_counts.p, 0, (uint64_t) slots * v.node_count * 4, s));
-      HIP_TRY(hipMemsetAsync(b->dense_lens.p, 0, slots * 4, s));
       b->dense_slots = slots;
     }
     DenseScratch ds{b->dense_counts.p, b->dense_lists.p, b->dense_lens.p, slots};
     hipLaunchKernelGGL(K.dense, dim3(slots), dim3(64), 0, s, v, cp, in, out2, b->redo.p,
-                       n_redo, ds);
+                       b->redo_count.p, ds);
     HIP_TRY(hipGetLastError());
     b->redone = n_redo;
   }
   return K2G_OK;
 }
+
+namespace k2g {
+// The dense-redo count of a call whose redo was sized on the device.
+static int settle_redo_count(k2g_batch *b) {
+  if (!b->redo_on_device || !b->n) return K2G_OK;
+  uint32_t c = 0;
+  HIP_TRY(hipMemcpyAsync(&c, b->redo_count.p, 4, hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  b->redone = c;
+  b->redo_on_device = false;
+  return K2G_OK;
+}
+}  // namespace k2g
 
 extern "C" int k2g_batch_fetch_counts(k2g_batch *b, uint32_t *call, uint32_t *hit_groups,
                                       uint32_t *total_kmers, uint32_t *n_runs, uint32_t *lookups,
@@ -931,6 +972,8 @@ extern "C" int k2g_batch_fetch_counts(k2g_batch *b, uint32_t *call, uint32_t *hi
 
 extern "C" int k2g_batch_redo_counts(k2g_batch *b, uint64_t *run_space, uint64_t *dense_table) {
   if (!b) return fail(K2G_EUSAGE, "null batch");
+  const int rc = k2g::settle_redo_count(b);
+  if (rc) return rc;
   if (run_space) *run_space = b->space_redone;
   if (dense_table) *dense_table = b->redone;
   return K2G_OK;
@@ -963,81 +1006,109 @@ extern "C" int k2g_batch_sync(k2g_batch *b, double *kernel_ms) {
 
 extern "C" int k2g_batch_fetch(k2g_batch *b, k2g_results *r) {
   if (!b || !r) return fail(K2G_EUSAGE, "null argument");
-  int rc = k2g_batch_sync(b, nullptr);
-  if (rc) return rc;
   const uint64_t n = b->n;
   hipStream_t s = b->stream;
   std::memset(r, 0, sizeof(*r));
   r->n = n;
   r->node_count = b->db->view.node_count;
-  r->redone = b->redone;
-  r->kernel_ms = b->kernel_ms;
-  b->host->call.resize(n);
-  b->host->groups.resize(n);
-  b->host->total.resize(n);
-  b->host->run_off.assign(n + 1, 0);
+  HostResults &h = *b->host;
+  HIP_TRY(h.call.reserve(std::max<uint64_t>(n, 1)));
+  HIP_TRY(h.groups.reserve(std::max<uint64_t>(n, 1)));
+  HIP_TRY(h.total.reserve(std::max<uint64_t>(n, 1)));
+  HIP_TRY(h.run_off.reserve(n + 1));
+  HIP_TRY(h.words.reserve(8));
+  h.run_off.p[0] = 0;
+  const bool hll = n && b->params.want_hll, stats = n && b->params.want_probe_stats;
+  if (hll) HIP_TRY(h.rec_off.reserve(n + 1));
   if (n) {
-    std::vector<uint32_t> st(n);
-    HIP_TRY(hipMemcpy(st.data(), b->status.p, n * 4, hipMemcpyDeviceToHost));
-    for (uint64_t i = 0; i < n; i++)
-      if (st[i] & 2u) return fail(K2G_ESOFTWARE, "internal error: hitlist exceeded its bound");
-    HIP_TRY(hipMemcpy(b->host->call.data(), b->call.p, n * 4, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(b->host->groups.data(), b->groups.p, n * 4, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(b->host->total.data(), b->total.p, n * 4, hipMemcpyDeviceToHost));
-    // compact runs: exclusive scan of counts, then a gather that writes
-    // straight into pinned host memory
-    auto compact = [&](const uint32_t *d_counts, const uint32_t *d_src, DevBuf<uint64_t> &off,
-                       PinnedBuf<uint32_t> &dst, std::vector<uint64_t> &h_off) -> int {
+    // phase 1, no host round trip: OR of every status word, the results'
+    // scalars into pinned host memory, and the scans that lay out the
+    // hitlists and HLL records densely; then one wait for the totals
+    HIP_TRY(b->fetch_word.reserve(1));
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, b->fetch_word.p, 1u, 0u);
+    hipLaunchKernelGGL(k_status_or, dim3((unsigned) std::min<uint64_t>((n + 255) / 256, 2048)), dim3(256), 0, s,
+                       b->status.p, n, b->fetch_word.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(h.words.p, b->fetch_word.p, 4, hipMemcpyDeviceToHost, s));
+    if (b->redo_on_device) HIP_TRY(hipMemcpyAsync(h.words.p + 1, b->redo_count.p, 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(h.call.p, b->call.p, n * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(h.groups.p, b->groups.p, n * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(h.total.p, b->total.p, n * 4, hipMemcpyDeviceToHost, s));
+    auto scan = [&](const uint32_t *d_counts, DevBuf<uint64_t> &off, DevBuf<uint64_t> &widened,
+                    PinnedBuf<uint64_t> &h_off) -> int {
       HIP_TRY(off.reserve(n + 1));
-      HIP_TRY(b->bounds.reserve(n + 1));
-      hipLaunchKernelGGL(k_widen, dim3((unsigned) ((n + 1 + 255) / 256)), dim3(256), 0, s,
-                         d_counts, b->bounds.p, n);
+      HIP_TRY(widened.reserve(n + 1));
+      hipLaunchKernelGGL(k_widen, dim3((unsigned) ((n + 1 + 255) / 256)), dim3(256), 0, s, d_counts,
+                         widened.p, n);
       size_t tb = 0;
-      HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, b->bounds.p, off.p, n + 1, s));
+      HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, widened.p, off.p, n + 1, s));
       HIP_TRY(b->temp.reserve(tb));
-      HIP_TRY(hipcub::DeviceScan::ExclusiveSum(b->temp.p, tb, b->bounds.p, off.p, n + 1, s));
-      h_off.resize(n + 1);
-      HIP_TRY(hipMemcpyAsync(h_off.data(), off.p, (n + 1) * 8, hipMemcpyDeviceToHost, s));
-      HIP_TRY(hipStreamSynchronize(s));
-      const uint64_t tot = h_off[n];
+      HIP_TRY(hipcub::DeviceScan::ExclusiveSum(b->temp.p, tb, widened.p, off.p, n + 1, s));
+      HIP_TRY(hipMemcpyAsync(h_off.p, off.p, (n + 1) * 8, hipMemcpyDeviceToHost, s));
+      return K2G_OK;
+    };
+    int rc = scan(b->n_runs.p, b->comp_off, b->bounds, h.run_off);
+    if (rc) return rc;
+    if (hll) {
+      rc = scan(b->n_recs.p, b->comp_rec_off, b->bounds, h.rec_off);  // stream order: bounds is free again
+      if (rc) return rc;
+    }
+    if (stats) {
+      HIP_TRY(h.lookups.reserve(n));
+      HIP_TRY(h.lines.reserve(n));
+      HIP_TRY(hipMemcpyAsync(h.lookups.p, b->lookups.p, n * 4, hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipMemcpyAsync(h.lines.p, b->lines.p, n * 8, hipMemcpyDeviceToHost, s));
+    }
+    if (b->params.want_kmer_counts) {
+      HIP_TRY(h.kmer_counts.reserve(b->db->view.node_count));
+      HIP_TRY(hipMemcpyAsync(h.kmer_counts.p, b->kmer_counts.p, b->db->view.node_count * 8,
+                             hipMemcpyDeviceToHost, s));
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    if (h.words.p[0] & 2u) return fail(K2G_ESOFTWARE, "internal error: hitlist exceeded its bound");
+    if (b->redo_on_device) {
+      b->redone = h.words.p[1];
+      b->redo_on_device = false;
+    }
+    // phase 2: each fragment's runs (and records) written straight into
+    // pinned, mapped host memory at their dense offsets; one more wait
+    auto compact = [&](const uint32_t *d_counts, const uint32_t *d_src, const DevBuf<uint64_t> &off,
+                       uint64_t tot, PinnedBuf<uint32_t> &dst) -> int {
       HIP_TRY(dst.reserve(2 * std::max<uint64_t>(tot, 1)));
       uint32_t *d_dst = nullptr;
       HIP_TRY(hipHostGetDevicePointer((void **) &d_dst, dst.p, 0));
       const unsigned grid = (unsigned) std::min<uint64_t>(n, 65536);
-      hipLaunchKernelGGL(k_compact, dim3(grid), dim3(64), 0, s, b->run_off.p, d_counts, off.p,
-                         d_src, d_dst, n);
+      hipLaunchKernelGGL(k_compact, dim3(grid), dim3(64), 0, s, b->run_off.p, d_counts, off.p, d_src, d_dst, n);
       HIP_TRY(hipGetLastError());
-      HIP_TRY(hipStreamSynchronize(s));
       return K2G_OK;
     };
-    rc = compact(b->n_runs.p, b->runs.p, b->comp_off, b->host->runs, b->host->run_off);
+    rc = compact(b->n_runs.p, b->runs.p, b->comp_off, h.run_off.p[n], h.runs);
     if (rc) return rc;
-    if (b->params.want_hll) {
-      rc = compact(b->n_recs.p, b->recs.p, b->comp_rec_off, b->host->recs, b->host->rec_off);
+    if (hll) {
+      rc = compact(b->n_recs.p, b->recs.p, b->comp_rec_off, h.rec_off.p[n], h.recs);
       if (rc) return rc;
-      r->hll_off = b->host->rec_off.data();
-      r->hll = b->host->recs.p;
     }
-    if (b->params.want_probe_stats) {
-      b->host->lookups.resize(n);
-      b->host->lines.resize(n);
-      HIP_TRY(hipMemcpy(b->host->lookups.data(), b->lookups.p, n * 4, hipMemcpyDeviceToHost));
-      HIP_TRY(hipMemcpy(b->host->lines.data(), b->lines.p, n * 8, hipMemcpyDeviceToHost));
-      r->lookups = b->host->lookups.data();
-      r->probe_lines = b->host->lines.data();
-    }
-    if (b->params.want_kmer_counts) {
-      b->host->kmer_counts.resize(b->db->view.node_count);
-      HIP_TRY(hipMemcpy(b->host->kmer_counts.data(), b->kmer_counts.p,
-                        b->db->view.node_count * 8, hipMemcpyDeviceToHost));
-      r->kmer_counts = b->host->kmer_counts.data();
-    }
+    HIP_TRY(hipStreamSynchronize(s));
   }
-  r->call = b->host->call.data();
-  r->hit_groups = b->host->groups.data();
-  r->total_kmers = b->host->total.data();
-  r->run_off = b->host->run_off.data();
-  r->runs = b->host->runs.p;
+  float ms = 0;
+  if (n && b->ev0) HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+  b->kernel_ms = ms;
+  r->kernel_ms = ms;
+  r->redone = b->redone;
+  r->call = h.call.p;
+  r->hit_groups = h.groups.p;
+  r->total_kmers = h.total.p;
+  r->run_off = h.run_off.p;
+  r->runs = h.runs.p;
+  if (hll) {
+    r->hll_off = h.rec_off.p;
+    r->hll = h.recs.p;
+  }
+  if (stats) {
+    r->lookups = h.lookups.p;
+    r->probe_lines = h.lines.p;
+  }
+  if (n && b->params.want_kmer_counts) r->kmer_counts = h.kmer_counts.p;
   return K2G_OK;
 }
 
@@ -1615,19 +1686,30 @@ extern "C" int k2g_db_gather_ceiling(k2g_db *db, uint64_t n_probes, uint64_t see
 // device memory plan of one k2g_classify call in a bound region (mirrors
 // the reservations of k2g_classify, k2g_classify_device and k2g_batch_fetch)
 uint64_t k2g::plan_classify_bytes(const k2g_db *db, const k2g_classify_params *p,
-                                  const k2g_fragments *in) {
+                                  const k2g_fragments *in, k2g_batch::HostPlan *stats) {
   const uint64_t n = in->n;
   if (n == 0) return 0;
   const DbView &v = db->view;
   const uint64_t bases = in->off[2 * n];
   const uint32_t pct = p->run_cap_pct ? (uint32_t) p->run_cap_pct : 25u;
-  uint64_t space = 0;
+  uint64_t space = 0, max_e = 0, sum_e = 0;
+  bool too_long = false;
   for (uint64_t i = 0; i < n; i++) {
     const bool pr = in->paired ? in->paired[i] != 0 : p->paired != 0;
-    const uint64_t e = fragment_elements(in->off[2 * i + 1] - in->off[2 * i],
-                                         pr ? in->off[2 * i + 2] - in->off[2 * i + 1] : 0, pr,
-                                         v.k, v.dna != 0);
+    const uint64_t l1 = in->off[2 * i + 1] - in->off[2 * i];
+    const uint64_t l2 = pr ? in->off[2 * i + 2] - in->off[2 * i + 1] : 0;
+    too_long |= ((l1 | l2) >> 31) != 0;
+    const uint64_t e = fragment_elements(l1, l2, pr, v.k, v.dna != 0);
     space += run_space(e, pct);
+    max_e = std::max(max_e, e);
+    sum_e += e;
+  }
+  if (stats) {
+    stats->valid = true;
+    stats->too_long = too_long;
+    stats->space = space;
+    stats->max_e = max_e;
+    stats->sum_e = sum_e;
   }
   const bool sort = p->mode != 0;
   uint64_t b = 0;
@@ -1647,7 +1729,8 @@ uint64_t k2g::plan_classify_bytes(const k2g_db *db, const k2g_classify_params *p
   }
   if (p->want_probe_stats) b += region_bytes(n, 4) + region_bytes(n, 8);
   b += 3 * region_bytes(n, 4);                                    // redo lists
-  const uint64_t slots = std::min<uint64_t>(n, 256);              // dense-table redo, worst case
+  const uint64_t slots = pct >= 100 ? dense_slots_for(n, v.node_count)  // dense-table redo, worst case
+                                    : std::min<uint64_t>(n, 256);
   b += 2 * region_bytes(slots * v.node_count, 4) + region_bytes(slots, 4);
   if (pct < 100) {                                                // exact-space redo, worst case
     b += 2 * region_bytes(n + 1, 8);
@@ -1661,6 +1744,7 @@ uint64_t k2g::plan_classify_bytes(const k2g_db *db, const k2g_classify_params *p
     b += region_bytes(2 * (space + exact), 4) * (p->want_hll ? 2 : 1);
   }
   b += region_bytes(n + 1, 8) * (p->want_hll ? 2 : 1);           // fetch offsets
+  b += region_bytes(1, 4);                                        // fetch status word
   return b;
 }
 

@@ -118,15 +118,15 @@ struct PinnedBuf {
 // batch owns one; the request server hands it to the finished request and
 // gives the batch another from its pool, so results are never copied.
 struct HostResults {
-  std::vector<uint32_t> call, groups, total, lookups;
-  std::vector<uint64_t> run_off, rec_off, kmer_counts, lines;
-  PinnedBuf<uint32_t> runs, recs;
+  // all pinned, so the device copies into them asynchronously
+  PinnedBuf<uint32_t> call, groups, total, lookups, runs, recs, words;
+  PinnedBuf<uint64_t> run_off, rec_off, kmer_counts, lines;
   HostResults() = default;
   HostResults(const HostResults &) = delete;
   HostResults &operator=(const HostResults &) = delete;
   ~HostResults() {
-    runs.release();
-    recs.release();
+    for (auto *b : {&call, &groups, &total, &lookups, &runs, &recs, &words}) b->release();
+    for (auto *b : {&run_off, &rec_off, &kmer_counts, &lines}) b->release();
   }
 };
 
@@ -166,6 +166,7 @@ struct k2g_batch {
   k2g::DevBuf<uint64_t> unit_pos, unit_scan;
   k2g::DevBuf<uint8_t> unit_flag;
   k2g::DevBuf<uint32_t> unit_start, n_units;
+  k2g::DevBuf<unsigned int> fetch_word;  // OR of the status words at fetch
   uint32_t dense_slots = 0;
   // host copies of the last results
   std::unique_ptr<k2g::HostResults> host = std::make_unique<k2g::HostResults>();
@@ -179,6 +180,15 @@ struct k2g_batch {
   // outputs (call .. n_recs, run_off, runs, recs, kmer_counts) are the
   // caller's arrays, adopted with their sizes: never grown or freed
   bool fixed_outputs = false;
+  // the next call's batch statistics, when the caller computed them on the
+  // host (the server's admission plan): the launch then reads nothing back
+  struct HostPlan {
+    bool valid = false, too_long = false;
+    uint64_t space = 0, max_e = 0, sum_e = 0;
+  } plan;
+  // the dense-table redo of the last call was sized on the device: its
+  // count is read when results are fetched
+  bool redo_on_device = false;
 
   template <class F>
   void for_each_device_buf(F f) {
@@ -189,7 +199,7 @@ struct k2g_batch {
     f(runs), f(recs), f(kmer_counts), f(cursor), f(lines), f(lookups);
     f(redo_count), f(space), f(space_only), f(temp), f(sort_keys), f(sort_keys_out), f(sort_vals);
     f(dense_counts), f(dense_lists), f(dense_lens);
-    f(unit_pos), f(unit_scan), f(unit_flag), f(unit_start), f(n_units);
+    f(unit_pos), f(unit_scan), f(unit_flag), f(unit_start), f(n_units), f(fetch_word);
   }
   // Allocates every device buffer from r from now on (null: hipMalloc).
   void bind_region(k2g::Region *r) {
@@ -210,5 +220,11 @@ int persistent_grid(k2g_batch *b, int num_cus);
 // Device bytes a k2g_classify call (host fragments, results fetched) takes
 // from a bound Region, as an upper bound; requires run_cap_pct == 100.
 uint64_t plan_classify_bytes(const k2g_db *db, const k2g_classify_params *p,
-                             const k2g_fragments *in);
+                             const k2g_fragments *in, k2g_batch::HostPlan *stats = nullptr);
+// Dense hit tables the overflow path of a call over n fragments takes when
+// it is launched without a host round trip (at most 64 MiB of them).
+inline uint32_t dense_slots_for(uint64_t n, uint64_t node_count) {
+  const uint64_t fit = (64ull << 20) / (8 * (node_count ? node_count : 1));
+  return (uint32_t) std::max<uint64_t>(1, std::min<uint64_t>(std::min<uint64_t>(n, 256), fit));
+}
 }  // namespace k2g

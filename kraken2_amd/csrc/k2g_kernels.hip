@@ -616,9 +616,20 @@ __device__ void scan_stream(const DbView &db, const ClassifyParams &P, const Fra
         }
       }
 
-      // per-taxon k-mer counts and HLL records for LOOKUP hits
+      // per-taxon k-mer counts for LOOKUP hits: one atomic per distinct
+      // taxon among the chunk's hits (neighbouring k-mers mostly share one)
+      if (EXTRAS && out.kmer_counts && HIT) {
+        uint64_t left = HIT;
+        while (left) {
+          const int f = __ffsll((long long) left) - 1;
+          const uint32_t t = readlane32(tq, f);
+          const uint64_t same = __ballot(lane_bit(left) && tq == t);
+          if (lane == f) atomicAdd(&out.kmer_counts[t], (unsigned long long) __popcll(same));
+          left &= ~same;
+        }
+      }
+      // HLL records for LOOKUP hits
       if (EXTRAS && lane_bit(HIT)) {
-        if (out.kmer_counts) atomicAdd(&out.kmer_counts[tq], 1ULL);
         if (out.recs) {
           const uint32_t r = st.nrec + __popcll(HIT & lanemask_lt(lane));
           if (r < cap) {
@@ -909,16 +920,25 @@ __device__ __forceinline__ void persistent_body(const DbView &db, const Classify
 
 // Overflow path: fragments whose distinct hit taxa exceeded the LDS table are
 // classified again with a dense per-taxon count array (one per block).
+// The number of listed fragments is read from device memory (*n_redo), so
+// the launch needs no host round trip; a grid of any size covers them.
 template <bool DNA>
 __device__ __forceinline__ void dense_body(const DbView &db, const ClassifyParams &P,
                                            const FragBatch &in, const FragOut &out,
-                                           const uint32_t *redo, uint32_t n_redo,
+                                           const uint32_t *redo, const uint32_t *n_redo_p,
                                            const DenseScratch &scratch) {
   __shared__ WaveLds<true> lds;
   const int lane = threadIdx.x;
   const uint32_t slot = blockIdx.x;
+  const uint32_t n_redo = uniform32(*n_redo_p);
+  if (slot >= n_redo) return;
   DenseTable tab{scratch.counts + (uint64_t) slot * db.node_count,
                  scratch.lists + (uint64_t) slot * db.node_count, scratch.lens + slot};
+  // the slot's table starts empty (its memory may hold anything); after each
+  // fragment release() clears the entries that fragment used
+  for (uint32_t t = lane; t < db.node_count; t += kWave) store_agent(&tab.counts[t], 0u);
+  if (lane == 0) store_agent(tab.len, 0u);
+  __syncthreads();
   for (uint32_t i = blockIdx.x; i < n_redo; i += gridDim.x)
     classify_fragment<DNA, true>(db, P, in, out, uniform32(redo[i]), tab, lds, &lds.bad, lane);
 }
@@ -944,12 +964,12 @@ K2G_CLASSIFY_VARIANT(_prot, false, true, 7)
 
 __global__ __launch_bounds__(64) void k_classify_dense_dna(DbView db, ClassifyParams P, FragBatch in,
                                                             FragOut out, const uint32_t *redo,
-                                                            uint32_t n_redo, DenseScratch scratch) {
+                                                            const uint32_t *n_redo, DenseScratch scratch) {
   dense_body<true>(db, P, in, out, redo, n_redo, scratch);
 }
 __global__ __launch_bounds__(64) void k_classify_dense_prot(DbView db, ClassifyParams P,
                                                              FragBatch in, FragOut out,
-                                                             const uint32_t *redo, uint32_t n_redo,
+                                                             const uint32_t *redo, const uint32_t *n_redo,
                                                              DenseScratch scratch) {
   dense_body<false>(db, P, in, out, redo, n_redo, scratch);
 }

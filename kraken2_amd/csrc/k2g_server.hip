@@ -79,6 +79,7 @@ struct Request {
   k2g_classify_params params{};
   k2g_fragments in{};
   uint64_t bytes = 0;
+  k2g_batch::HostPlan plan;  // the batch statistics the admission plan computed
   int state = 0;  // 0 queued, 1 on the GPU, 2 done
   int rc = K2G_OK;
   std::string error;
@@ -152,7 +153,9 @@ void worker_main(k2g_server *s) {
     region.size = q->bytes;
     region.used = 0;
     b->bind_region(&region);
+    b->plan = q->plan;  // the launch reads no statistics back
     int r = k2g_classify(b, &q->params, &q->in, &q->out);
+    b->plan.valid = false;
     if (r == K2G_OK) {
       q->res = std::move(b->host);  // q->out points into these buffers
     } else {
@@ -220,7 +223,7 @@ extern "C" int k2g_server_submit(k2g_server *s, const k2g_classify_params *p,
   q->params = *p;
   q->params.run_cap_pct = 100;  // exact hitlist space: the plan is an upper bound
   q->in = *in;
-  q->bytes = k2g::plan_classify_bytes(s->db, &q->params, in);
+  q->bytes = k2g::plan_classify_bytes(s->db, &q->params, in, &q->plan);
   q->bytes = (q->bytes + k2g::kRegionAlign - 1) / k2g::kRegionAlign * k2g::kRegionAlign;
   if (q->bytes > s->budget) {
     const uint64_t need = q->bytes;
