@@ -180,6 +180,74 @@ class Runner:
         self.batch.close()
 
 
+def run_graph_pipeline(k2, db, runner, range_sets, off, batched_digests, a, work):
+    """The persistent pass as graph replays (k2g_pipeline): autotuned on an
+    even sample of the read set, then, for each set of fragment ranges, W
+    warmup and K timed steps of one replay per range, beside the direct
+    persistent path (k2g_classify_device) over the same ranges.  The
+    replays' digests are compared with the batched pass's."""
+    n = (len(off) - 1) // 2
+    kw = dict(confidence=work["confidence"], min_hit_groups=work["min_hit_groups"])
+    stride = max(1, n // max(1, a.tune_sample))
+    t0 = time.perf_counter()
+    best, rep = k2.tune_pipeline(db, runner.d_seq, runner.d_off, runner.d_paired, n, stride=stride, **kw)
+    tune_s = time.perf_counter() - t0
+    log(f"# graph pipeline: tuned on {rep['sample_fragments']} fragments in {tune_s:.1f}s "
+        f"({rep['evaluated']} configurations, {rep['mismatched']} mismatched): {best}, "
+        f"{rep['default_ms']:.2f} -> {rep['best_ms']:.2f} ms")
+
+    def dev(s, e):
+        return (runner.d_seq, C.c_void_p(runner.d_off.value + 16 * s),
+                C.c_void_p(runner.d_paired.value + s) if runner.d_paired is not None else None, e - s)
+
+    def timed(name, step):
+        for _ in range(a.warmup):
+            step()
+        t = time.perf_counter()
+        dms = []
+        for _ in range(a.steps):
+            dms.append(step())
+            log(f"# {name}: step {len(dms)}/{a.steps} device {dms[-1]:.1f} ms")
+        wall = (time.perf_counter() - t) / a.steps
+        log(f"# {name}: {wall * 1e3:.1f} ms/step, device {np.mean(dms):.1f} ms")
+        return wall, float(np.mean(dms))
+
+    out = {"tuning": best,
+           "tune": {"sample_fragments": rep["sample_fragments"], "stride": stride,
+                    "configurations": rep["evaluated"], "mismatched": rep["mismatched"],
+                    "default_ms": round(rep["default_ms"], 3), "best_ms": round(rep["best_ms"], 3),
+                    "seconds": round(tune_s, 1)},
+           "chain": "element bounds, reserved hitlist layout, length-bucket work units, persistent classify, "
+                    "out-of-space redo into an overflow pool, dense redo: one HIP graph replay per range, no "
+                    "host round trip"}
+    all_equal = True
+    for set_name, ranges in range_sets.items():
+        cap_n = max(e - s for s, e in ranges)
+        cap_b = max(int(off[2 * e] - off[2 * s]) for s, e in ranges)
+        pl = k2.Pipeline(db, cap_n, cap_b, host_io=False, tuning=best, **kw)
+        _, grid = pl.params()
+        g_wall, g_ms = timed(f"graph over {set_name}", lambda: sum(pl.run_device(*dev(s, e)) for s, e in ranges))
+        d_wall, d_ms = timed(f"direct over {set_name}", lambda: sum(runner.run(s, e, 1) for s, e in ranges))
+        equal = None
+        if batched_digests is not None:
+            dig = np.zeros(n, np.uint64)
+            for s, e in ranges:
+                pl.run_device(*dev(s, e))
+                dig[s:e] = pl.digests(e - s)
+            equal = bool(np.array_equal(dig, batched_digests))
+            all_equal = all_equal and equal
+            log(f"# graph over {set_name}: digests equal the batched pass's: {equal}")
+        pl.close()
+        out[set_name] = {"replays_per_step": len(ranges), "grid": grid,
+                         "graph": {"reads_per_s": round(n / g_wall, 1), "ms_per_step": round(g_wall * 1e3, 3),
+                                   "device_ms_per_step": round(g_ms, 3)},
+                         "direct": {"reads_per_s": round(n / d_wall, 1), "ms_per_step": round(d_wall * 1e3, 3),
+                                    "kernel_ms_per_step": round(d_ms, 3)},
+                         "graph_vs_direct": round(d_wall / g_wall, 3), "digests_equal_batched": equal}
+    out["digests_equal_batched"] = all_equal if batched_digests is not None else None
+    return out
+
+
 def run_host_workload(a, work, k2):
     """c1: generated on the host with k2synth, built by the GPU build front-ends."""
     from helpers import K2SYNTH, make_synthetic_db, run
@@ -372,16 +440,17 @@ def run_serving(k2, db, tmp, pools, a, work):
     mu = one["completed_req_per_s"]
     for mult in (0.5, 1.0, 2.0):
         leg(f"Poisson {mult}x the one-slot rate", mult * mu, a.serve_slots)
-    # every request alone (one batch, one request at a time) gives the same results
-    batch = k2.Batch(db)
+    # every request equals itself served alone: the one-slot closed loop runs
+    # each request by itself, the other legs run them side by side
+    alone = digests[0]
+    mismatched = sorted({i for dig in digests[1:] for i, d in enumerate(dig) if d != alone[i]})
     kw = dict(confidence=work["confidence"], min_hit_groups=work["min_hit_groups"], kmer_counts=True, hll=True)
-    alone = [batch.classify_checksum(r["seq"], r["off"], paired_flags=r["paired"], mode=1, **kw) for r in reqs]
-    batch.close()
-    mismatched = sorted({i for dig in digests for i, d in enumerate(dig) if d != alone[i]})
-    # the oracle's classify driver on a spread of request sizes
+    # the oracle's classify driver on a spread of request sizes (up to
+    # serve_oracle_max fragments, so the CPU check stays short)
     n_check = 0 if a.no_cpu else min(len(reqs), a.serve_oracle)
-    by_size = sorted(range(len(reqs)), key=lambda i: reqs[i]["size"])
-    pick = [by_size[int(j)] for j in np.linspace(0, len(reqs) - 1, n_check)] if n_check else []
+    by_size = sorted((i for i in range(len(reqs)) if reqs[i]["size"] <= a.serve_oracle_max),
+                     key=lambda i: reqs[i]["size"])
+    pick = sorted({by_size[int(j)] for j in np.linspace(0, len(by_size) - 1, n_check)}) if n_check and by_size else []
     checked = mism = rep_mism = 0
     if pick:
         from helpers import Oracle
@@ -396,6 +465,8 @@ def run_serving(k2, db, tmp, pools, a, work):
                 mism += m
                 rep_mism += rm
                 checked += r["size"]
+                log(f"# serving oracle check: request of {r['size']} fragments: {m} line and {rm} kreport "
+                    "mismatches")
         finally:
             srv.close()
             orc.lib.k2cpu_db_close(h)
@@ -718,8 +789,12 @@ def main():
                     help="only run a 1 GiB device copy (calibrates HBM byte counters) and exit")
     ap.add_argument("--full-verify", action="store_true", help="digest every plain block, not an even spread")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--no-pipeline", action="store_true", help="skip the graph-pipeline leg")
+    ap.add_argument("--tune-sample", type=int, default=500_000, help="fragments the autotuner times on")
     ap.add_argument("--serve-requests", type=int, default=300, help="requests in the serving set")
-    ap.add_argument("--serve-oracle", type=int, default=48, help="served requests checked against the oracle")
+    ap.add_argument("--serve-oracle", type=int, default=24, help="served requests checked against the oracle")
+    ap.add_argument("--serve-oracle-max", type=int, default=50000,
+                    help="largest request (fragments) checked against the oracle")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-budget-gb", type=float, default=32.0)
     ap.add_argument("--serve-pool", type=int, default=300_000, help="fragments per serving pool")
@@ -840,6 +915,9 @@ def main():
             np.array_equal(digests["plain_one_launch"], digests["batched"]) and \
             (np.array_equal(digests["plain"], digests["batched"]) if "plain" in digests else True)
 
+    graph = None if a.no_pipeline else run_graph_pipeline(
+        k2, db, runner, {"chunks": chunks, "frontend_batches": fbatches}, off, digests.get("batched"), a, work)
+
     # CPU leg, part 1: the oracle checks an even sample of fragments (the
     # reference binary is timed further down); skipped with --no-cpu
     rng = np.random.default_rng(7)
@@ -955,7 +1033,9 @@ def main():
         "batched_vs_plain": None if a.skip_plain else round(plain_s / batched_s, 3),
         "batched_vs_plain_blocks": None if a.skip_plain else round(pblk_s / batched_s, 3),
         "batched_vs_plain_one_launch": round(timing["plain_one_launch"][0] / batched_s, 3),
-        "bit_exact": bool(modes_agree and mism == 0 and sample_digest_agree),
+        "batched_graph": graph,
+        "bit_exact": bool(modes_agree and mism == 0 and sample_digest_agree and
+                          (graph is None or graph["digests_equal_batched"] is not False)),
         "verify": {"modes_equal": bool(modes_agree), "plain_fragments_compared": plain_checked,
                    "one_launch_fragments_compared": n, "oracle_sample": int(len(sample)),
                    "oracle_mismatches": int(mism), "sample_digests_match_bench_run": sample_digest_agree},

@@ -209,6 +209,75 @@ typedef struct {
 int k2g_report_write(k2g_db *db, const k2g_fragments *in, const k2g_results *r,
                      const k2g_report_params *p, const char *path);
 
+/* ---- graph-replayed batch pipeline.  No reference counterpart: the
+ * reference classifies block by block on CPU threads (ProcessFiles,
+ * src/classify.cc:479-708).  The per-batch launch chain of k2g_classify --
+ * upload (host input), element bounds and hitlist layout, longest-first work
+ * units, the persistent classify kernel, the dense-table redo and the
+ * writeback of results into pinned host memory (host output) -- recorded
+ * once as a HIP graph for a batch capacity and replayed for every batch
+ * that fits, with no host round trip between its steps.  Work units come
+ * from length buckets: fragments are binned by their element counts
+ * (2^bucket_shift bins per doubling), the bins laid out longest first and
+ * each cut into units of about unit_elements elements.  Results equal
+ * k2g_classify's on the same fragments. */
+typedef struct k2g_pipeline k2g_pipeline;
+
+typedef struct {
+  uint64_t max_fragments; /* batch capacity */
+  uint64_t max_bases;
+  int32_t host_io;        /* 1: batches from host memory, results into pinned host memory
+                           * (k2g_pipeline_run, exact hitlist space); 0: device-resident
+                           * batches, results left on the device (k2g_pipeline_run_device;
+                           * run_cap_pct of the classify parameters reserved, 0 = 25 as in
+                           * k2g_classify_device; fragments that outgrow their reservation
+                           * take exact space from an overflow pool of an eighth of the
+                           * reserved space, at least 2^20 elements, within the same replay) */
+  int32_t unit_elements;  /* elements per work unit (reads per wavefront claim); 0 = 4096 */
+  int32_t bucket_shift;   /* 2^bucket_shift length buckets per doubling, in [0, 2] */
+  int32_t probe_depth;    /* chunks of 64 k-mers whose lookups one probe round takes, [1, 8]; 0 = 8 */
+  int32_t waves_per_cu;   /* resident classify waves per compute unit; 0 = as many as fit */
+} k2g_pipeline_params;
+
+int k2g_pipeline_create(k2g_db *db, const k2g_classify_params *p, const k2g_pipeline_params *pp,
+                        k2g_pipeline **out);
+/* Host fragments -> host results (valid until the next run).  The bases,
+ * offsets and flags are copied into the pipeline's pinned staging first
+ * (nothing is copied when they already lie there). */
+int k2g_pipeline_run(k2g_pipeline *pl, const k2g_fragments *in, k2g_results *out);
+/* Device-resident fragments; results stay in the pipeline's device buffers.
+ * ms (may be NULL) receives the graph's device time. */
+int k2g_pipeline_run_device(k2g_pipeline *pl, const char *d_seq, const uint64_t *d_off,
+                            const uint8_t *d_paired, uint64_t n, double *ms);
+/* Per-fragment result digests of the last run (as k2g_batch_digests). */
+int k2g_pipeline_digests(k2g_pipeline *pl, uint64_t *out, uint64_t n);
+/* The pipeline's parameters and its classify grid (workgroups). */
+int k2g_pipeline_get_params(const k2g_pipeline *pl, k2g_pipeline_params *pp, int32_t *grid);
+/* Fragments of the last batch classified again: out of reserved hitlist
+ * space (into the overflow pool), and with the dense hit table. */
+int k2g_pipeline_redo_counts(const k2g_pipeline *pl, uint64_t *space, uint64_t *dense);
+int k2g_pipeline_destroy(k2g_pipeline *pl);
+
+/* Autotuner: chooses unit_elements, bucket_shift, probe_depth and
+ * waves_per_cu for a workload by timing graph replays on a sample of it
+ * (every stride-th fragment of the device-resident set), by coordinate
+ * descent from *best (two sweeps).  Every configuration must give the same
+ * per-fragment results; report->mismatched counts any that did not (0 on a
+ * correct build).  *best receives the fastest configuration.  This
+ * generalizes the reference's one tuning knob, the prefetch distance of
+ * GetBatch (K2_PREFETCH_DIST, src/compact_hash.h:381-384). */
+typedef struct {
+  uint64_t sample_fragments, sample_bases;
+  uint32_t evaluated;  /* configurations timed */
+  uint32_t mismatched; /* configurations whose results differed from the first */
+  double default_ms;   /* the starting configuration's graph time on the sample */
+  double best_ms;      /* the chosen one's */
+} k2g_tune_report;
+
+int k2g_pipeline_tune(k2g_db *db, const k2g_classify_params *p, const char *d_seq,
+                      const uint64_t *d_off, const uint8_t *d_paired, uint64_t n, uint64_t stride,
+                      k2g_pipeline_params *best, k2g_tune_report *report);
+
 /* Pinned host memory for request fragments (and anything else a caller
  * streams to the GPU): uploads from it run at full interconnect speed and
  * asynchronously.  No reference counterpart. */

@@ -21,10 +21,12 @@ from ._lib import (  # noqa: F401
     K2GError,
     Merger,
     MergeOutput,
+    Pipeline,
     Server,
     expand_spaced_seed_mask,
     format_hitlist,
     load_library,
     pack_fragments,
+    tune_pipeline,
     write_report,
 )

@@ -59,6 +59,14 @@ EXPORTED_SYMBOLS = (
     "k2g_build_destroy",
     "k2g_estimate_capacity",
     "k2g_estimate_capacity_device",
+    "k2g_pipeline_create",
+    "k2g_pipeline_run",
+    "k2g_pipeline_run_device",
+    "k2g_pipeline_digests",
+    "k2g_pipeline_get_params",
+    "k2g_pipeline_redo_counts",
+    "k2g_pipeline_destroy",
+    "k2g_pipeline_tune",
     "k2g_merge_open",
     "k2g_merge",
     "k2g_merge_close",
@@ -210,6 +218,15 @@ def load_library(path: str = LIB_PATH):
     lib.k2g_host_free.argtypes = [vp]
     lib.k2g_results_checksum.argtypes = [C.POINTER(Results), C.POINTER(u64)]
     lib.k2g_db_gather_ceiling.argtypes = [vp, u64, u64, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    lib.k2g_pipeline_create.argtypes = [vp, C.POINTER(ClassifyParams), C.POINTER(PipelineParams), C.POINTER(vp)]
+    lib.k2g_pipeline_run.argtypes = [vp, C.POINTER(Fragments), C.POINTER(Results)]
+    lib.k2g_pipeline_run_device.argtypes = [vp, vp, vp, vp, u64, C.POINTER(C.c_double)]
+    lib.k2g_pipeline_digests.argtypes = [vp, vp, u64]
+    lib.k2g_pipeline_get_params.argtypes = [vp, C.POINTER(PipelineParams), C.POINTER(C.c_int32)]
+    lib.k2g_pipeline_redo_counts.argtypes = [vp, C.POINTER(u64), C.POINTER(u64)]
+    lib.k2g_pipeline_destroy.argtypes = [vp]
+    lib.k2g_pipeline_tune.argtypes = [vp, C.POINTER(ClassifyParams), vp, vp, vp, u64, u64, C.POINTER(PipelineParams),
+                                      C.POINTER(TuneReport)]
     lib.k2g_merge_open.argtypes = [C.c_char_p, C.POINTER(vp)]
     lib.k2g_merge.argtypes = [vp, C.POINTER(MergeParams), C.POINTER(HitLists), C.POINTER(HitLists),
                               C.POINTER(MergeResults)]
@@ -647,6 +664,116 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+class PipelineParams(C.Structure):
+    _fields_ = [("max_fragments", C.c_uint64), ("max_bases", C.c_uint64), ("host_io", C.c_int32),
+                ("unit_elements", C.c_int32), ("bucket_shift", C.c_int32), ("probe_depth", C.c_int32),
+                ("waves_per_cu", C.c_int32)]
+
+    def tuning(self) -> dict:
+        return {k: int(getattr(self, k)) for k in ("unit_elements", "bucket_shift", "probe_depth", "waves_per_cu")}
+
+
+class TuneReport(C.Structure):
+    _fields_ = [("sample_fragments", C.c_uint64), ("sample_bases", C.c_uint64), ("evaluated", C.c_uint32),
+                ("mismatched", C.c_uint32), ("default_ms", C.c_double), ("best_ms", C.c_double)]
+
+
+def _classify_params(kw) -> ClassifyParams:
+    return ClassifyParams(kw.get("confidence", 0.0), kw.get("min_hit_groups", 0), int(kw.get("quick", False)),
+                          int(kw.get("paired", False)), 0, int(kw.get("kmer_counts", False)),
+                          int(kw.get("hll", False)), kw.get("table_slots", 0), 0, kw.get("run_cap_pct", 0), 0)
+
+
+class Pipeline:
+    """A batch pipeline recorded as one HIP graph (k2g_pipeline_*): batches up
+    to max_fragments / max_bases replay it.  host_io=True takes host
+    fragments and returns host results (run); host_io=False classifies
+    device-resident fragments and leaves the results on the device
+    (run_device, digests).  Tuning keys: unit_elements, bucket_shift,
+    probe_depth, waves_per_cu."""
+
+    def __init__(self, db: Db, max_fragments: int, max_bases: int, *, host_io=True, tuning=None, **kw):
+        self.db = db
+        self._kw = kw
+        t = tuning or {}
+        pp = PipelineParams(int(max_fragments), int(max_bases), int(host_io), int(t.get("unit_elements", 0)),
+                            int(t.get("bucket_shift", 0)), int(t.get("probe_depth", 0)),
+                            int(t.get("waves_per_cu", 0)))
+        h = C.c_void_p()
+        _check(load_library().k2g_pipeline_create(db._h, C.byref(_classify_params(kw)), C.byref(pp), C.byref(h)))
+        self._h = h
+
+    def _fragments(self, seq, off, paired_flags):
+        off = np.ascontiguousarray(off, dtype=np.uint64)
+        n = (off.shape[0] - 1) // 2
+        if isinstance(seq, np.ndarray):
+            sbuf = np.ascontiguousarray(seq, dtype=np.uint8) if len(seq) else np.zeros(1, np.uint8)
+        else:
+            sbuf = np.frombuffer(seq, dtype=np.uint8) if len(seq) else np.zeros(1, np.uint8)
+        flags = None if paired_flags is None else np.ascontiguousarray(paired_flags, dtype=np.uint8)
+        fr = Fragments(sbuf.ctypes.data, off.ctypes.data, flags.ctypes.data if flags is not None else None, n)
+        return fr, (sbuf, off, flags)
+
+    def run(self, seq, off, *, paired_flags=None, checksum_only=False):
+        """ClassifyOutput of one host batch (or its 64-bit checksum)."""
+        fr, keep = self._fragments(seq, off, paired_flags)
+        res = Results()
+        _check(_lib.k2g_pipeline_run(self._h, C.byref(fr), C.byref(res)))
+        if checksum_only:
+            h = C.c_uint64()
+            _check(_lib.k2g_results_checksum(C.byref(res), C.byref(h)))
+            return h.value
+        return _copy_results(res, bool(self._kw.get("hll")), bool(self._kw.get("kmer_counts")))
+
+    def run_device(self, d_seq, d_off, d_paired, n: int) -> float:
+        """Classifies n device-resident fragments (device addresses); returns
+        the graph's device time in ms."""
+        ms = C.c_double()
+        _check(_lib.k2g_pipeline_run_device(self._h, d_seq, d_off, d_paired, int(n), C.byref(ms)))
+        return ms.value
+
+    def digests(self, n: int) -> np.ndarray:
+        """Per-fragment digests of the last batch (as Batch.digests)."""
+        out = np.zeros(max(1, n), dtype=np.uint64)
+        _check(_lib.k2g_pipeline_digests(self._h, out.ctypes.data, int(n)))
+        return out[:n]
+
+    def params(self) -> tuple[dict, int]:
+        """(tuning parameters in effect, classify grid in workgroups)."""
+        pp, grid = PipelineParams(), C.c_int32()
+        _check(_lib.k2g_pipeline_get_params(self._h, C.byref(pp), C.byref(grid)))
+        return pp.tuning(), grid.value
+
+    def redo_counts(self) -> tuple[int, int]:
+        """(fragments of the last batch redone into the overflow pool, redone with the dense table)."""
+        a, b = C.c_uint64(), C.c_uint64()
+        _check(_lib.k2g_pipeline_redo_counts(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def close(self):
+        if self._h:
+            _lib.k2g_pipeline_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def tune_pipeline(db: Db, d_seq, d_off, d_paired, n: int, stride: int = 1, start=None, **kw) -> tuple[dict, dict]:
+    """Autotunes a pipeline on every stride-th fragment of a device-resident
+    set: (best tuning parameters, report)."""
+    t = start or {}
+    best = PipelineParams(0, 0, 0, int(t.get("unit_elements", 0)), int(t.get("bucket_shift", 0)),
+                          int(t.get("probe_depth", 0)), int(t.get("waves_per_cu", 0)))
+    rep = TuneReport()
+    _check(load_library().k2g_pipeline_tune(db._h, C.byref(_classify_params(kw)), d_seq, d_off, d_paired, int(n),
+                                            int(stride), C.byref(best), C.byref(rep)))
+    return best.tuning(), {k: getattr(rep, k) for k, _ in TuneReport._fields_}
 
 
 def pinned_array(nbytes: int) -> np.ndarray:

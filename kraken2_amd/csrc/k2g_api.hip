@@ -819,6 +819,7 @@ int k2g::classify_launch(k2g_batch *b, const k2g_classify_params *p, const char 
   cp.min_hit_groups = p->min_hit_groups;
   cp.quick = p->quick ? 1u : 0u;
   cp.tab_cap = p->table_slots ? (uint32_t) p->table_slots : 256u;
+  cp.batch_chunks = 8;
   FragBatch in{d_seq, d_off,  d_paired, (uint32_t) (p->paired != 0), n, order, units,
                units ? b->n_units.p : nullptr};
   if (p->want_probe_stats) {

@@ -10,6 +10,19 @@ struct ClassifyParams {
   int32_t min_hit_groups;  // -g
   uint32_t quick;          // -q
   uint32_t tab_cap;        // LDS hit-table slots per fragment (power of 2, <= 256)
+  uint32_t batch_chunks;   // probe depth: chunks whose lookups are queued before one probe
+                           // round, in [1, 8] (8 unless tuned)
+};
+
+// One batch of a graph-replayed pipeline (k2g_pipeline): the graph's kernels
+// read the batch's arrays and sizes from this device-resident descriptor, so
+// one instantiated graph serves every batch that fits its capacity.
+struct PipeDesc {
+  const char *seq;
+  const uint64_t *off;
+  const uint8_t *paired;  // per-fragment flags, or null: paired_all
+  uint64_t n, bases;
+  uint32_t paired_all, pad;
 };
 
 // Fragment i: mate 1 = seq[off[2i], off[2i+1]), mate 2 = seq[off[2i+1], off[2i+2]).

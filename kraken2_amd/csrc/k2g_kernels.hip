@@ -399,7 +399,9 @@ __device__ void scan_stream(const DbView &db, const ClassifyParams &P, const Fra
   while (more) {
     // ---- A: scan chunks, queue their lookups
     uint32_t qn = 0, g = 0;
-    while (g < (uint32_t) kBatchChunks && qn + kWave <= (uint32_t) kQueue) {
+    // wave-uniform, clamped to [1, kBatchChunks] (0 would queue nothing)
+    const uint32_t depth = P.batch_chunks - 1u < (uint32_t) kBatchChunks ? P.batch_chunks : (uint32_t) kBatchChunks;
+    while (g < depth && qn + kWave <= (uint32_t) kQueue) {
       if (c >= n) {  // this stream is done
         if (second) {
           more = false;
@@ -948,6 +950,17 @@ __device__ __forceinline__ void dense_body(const DbView &db, const ClassifyParam
 // mode), _prot (translated search, all of them available).
 // WAVES: resident waves per SIMD the variant is built for (its LDS allows
 // it: 32 one-wave workgroups per CU for _dna, 28 for the others).
+// The batch's arrays and sizes of a graph-replayed pipeline, read from its
+// descriptor into the kernel's batch view.
+__device__ __forceinline__ FragBatch with_desc(FragBatch in, const PipeDesc *d) {
+  in.seq = d->seq;
+  in.off = d->off;
+  in.paired = d->paired;
+  in.paired_all = d->paired_all;
+  in.n = d->n;
+  return in;
+}
+
 #define K2G_CLASSIFY_VARIANT(SUFFIX, DNA, EXTRAS, WAVES)                                       \
   __global__ __launch_bounds__(64, WAVES) void k_classify_plain##SUFFIX(                      \
       DbView db, ClassifyParams P, FragBatch in, FragOut out) {                                \
@@ -956,12 +969,27 @@ __device__ __forceinline__ void dense_body(const DbView &db, const ClassifyParam
   __global__ __launch_bounds__(64, WAVES) void k_classify_persistent##SUFFIX(                 \
       DbView db, ClassifyParams P, FragBatch in, FragOut out, unsigned long long *cursor) {    \
     persistent_body<DNA, EXTRAS>(db, P, in, out, cursor);                                      \
+  }                                                                                            \
+  __global__ __launch_bounds__(64, WAVES) void k_classify_pipe##SUFFIX(                       \
+      DbView db, ClassifyParams P, const PipeDesc *d, FragBatch in, FragOut out,               \
+      unsigned long long *cursor) {                                                            \
+    persistent_body<DNA, EXTRAS>(db, P, with_desc(in, d), out, cursor);                        \
   }
 K2G_CLASSIFY_VARIANT(_dna, true, false, 8)
 K2G_CLASSIFY_VARIANT(_dna_x, true, true, 7)
 K2G_CLASSIFY_VARIANT(_prot, false, true, 7)
 #undef K2G_CLASSIFY_VARIANT
 
+__global__ __launch_bounds__(64) void k_classify_dense_pipe_dna(DbView db, ClassifyParams P, const PipeDesc *d,
+                                                                 FragBatch in, FragOut out, const uint32_t *redo,
+                                                                 const uint32_t *n_redo, DenseScratch scratch) {
+  dense_body<true>(db, P, with_desc(in, d), out, redo, n_redo, scratch);
+}
+__global__ __launch_bounds__(64) void k_classify_dense_pipe_prot(DbView db, ClassifyParams P, const PipeDesc *d,
+                                                                  FragBatch in, FragOut out, const uint32_t *redo,
+                                                                  const uint32_t *n_redo, DenseScratch scratch) {
+  dense_body<false>(db, P, with_desc(in, d), out, redo, n_redo, scratch);
+}
 __global__ __launch_bounds__(64) void k_classify_dense_dna(DbView db, ClassifyParams P, FragBatch in,
                                                             FragOut out, const uint32_t *redo,
                                                             const uint32_t *n_redo, DenseScratch scratch) {

@@ -78,7 +78,7 @@ int emu_classify(const uint32_t *cells, uint64_t capacity, uint32_t kb, uint32_t
     for (uint64_t i = 0; i < capacity && !empty; i++) empty = (cells[i] & vmask) == 0;
     v.lean_probe = (empty && capacity >= 64) ? 1u : 0u;
   }
-  ClassifyParams P{conf, min_groups, quick, tab_cap};
+  ClassifyParams P{conf, min_groups, quick, tab_cap, 8};
   FragBatch in{seq, off, paired, paired_all, n, order, unit_start, n_units};
   FragOut out{call,    groups, total, n_runs, status,      n_recs,  run_off,
               run_cap, runs,   recs,  kmer_counts, nullptr, nullptr};
