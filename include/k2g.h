@@ -76,7 +76,15 @@ typedef struct {
                              * 2: persistent when fragment lengths are uneven (the longest
                              * above four times the mean), else plain */
   int32_t want_kmer_counts; /* -R: per-taxon LOOKUP-hit counts */
-  int32_t want_hll;         /* -K: per-hit (taxon, encoded HLL hash) records */
+  int32_t want_hll;         /* -K: 1 = per-hit (taxon, encoded HLL hash) records (hll_off, hll);
+                             * 2 = the records reduced on the device to one distinct set per
+                             * (input block, taxon) with the block sketch's register switch
+                             * decided (sketch_*); blocks are the reference's input blocks over
+                             * the call's fragments (src/classify.cc:523-543): runs of pairs
+                             * 10 000 at a time, runs of single reads until 3 MiB of bases.
+                             * 2 needs host fragments (k2g_classify, the server), fewer than
+                             * 2^31 records per call, and block and taxon ids that fit 32 bits
+                             * together */
   int32_t table_slots;      /* per-fragment LDS hit table (power of 2 in [2,256]; 0 = 256) */
   int32_t want_probe_stats; /* per-fragment probes and distinct 128-byte table lines touched */
   int32_t run_cap_pct;      /* hitlist space reserved per fragment, in percent of its element
@@ -117,6 +125,19 @@ typedef struct {
   uint64_t node_count;
   uint64_t redone;             /* fragments that took the dense-table path */
   double kernel_ms;            /* device time of the classify kernels */
+  /* want_hll == 2: one group per (input block, taxon) with HLL records, ordered
+   * by block, then taxon.  Group g's distinct encoded hashes are
+   * sketch_vals[sketch_off[g] .. sketch_off[g+1]), ascending; sketch_dense[g]
+   * says whether the block's sketch of the taxon switched to registers
+   * (src/hyperloglogplus.cc:471-509: the sparse list already held 1024
+   * distinct values when another insert arrived). */
+  uint64_t n_sketches;
+  const uint32_t *sketch_block;
+  const uint32_t *sketch_taxon;
+  const uint32_t *sketch_dense;
+  const uint64_t *sketch_n;    /* records (LOOKUP hits) the group observed */
+  const uint64_t *sketch_off;  /* n_sketches + 1 */
+  const uint32_t *sketch_vals;
 } k2g_results;
 
 int k2g_batch_create(k2g_db *db, k2g_batch **out);
@@ -198,8 +219,10 @@ int k2g_classify_device_noalloc(k2g_db *db, const k2g_classify_params *p, const 
  * estimates are folded block by block from the results' HLL records, the
  * way the reference merges its per-block sketches (src/classify.cc:554,
  * 640-645; src/hyperloglogplus.cc:471-509); the results must then carry
- * them (want_hll).  For a served request this is the report the reference's
- * daemon writes for the same reads run as one classify call. */
+ * them: per-hit records (want_hll 1) or the device's per-(block, taxon)
+ * sets (want_hll 2, same blocks).  For a served request this is the report
+ * the reference's daemon writes for the same reads run as one classify
+ * call. */
 typedef struct {
   int32_t report_zeros; /* -z */
   int32_t kmer_data;    /* -K */

@@ -150,6 +150,13 @@ class Results(C.Structure):
         ("node_count", C.c_uint64),
         ("redone", C.c_uint64),
         ("kernel_ms", C.c_double),
+        ("n_sketches", C.c_uint64),
+        ("sketch_block", C.POINTER(C.c_uint32)),
+        ("sketch_taxon", C.POINTER(C.c_uint32)),
+        ("sketch_dense", C.POINTER(C.c_uint32)),
+        ("sketch_n", C.POINTER(C.c_uint64)),
+        ("sketch_off", C.POINTER(C.c_uint64)),
+        ("sketch_vals", C.POINTER(C.c_uint32)),
     ]
 
 
@@ -260,6 +267,9 @@ class ClassifyOutput:
     probe_lines: np.ndarray | None
     redone: int
     kernel_ms: float
+    # hll=2: per-(input block, taxon) distinct sets reduced on the device:
+    # arrays block, taxon, dense, n, off (groups + 1), vals
+    sets: dict | None = None
 
 
 class Db:
@@ -428,6 +438,13 @@ def _copy_results(res: Results, hll: bool, kmer_counts: bool) -> ClassifyOutput:
         kc = _arr(res.kmer_counts, res.node_count, np.uint64)
     lk = _arr(res.lookups, n, np.uint32) if res.lookups else None
     ln = _arr(res.probe_lines, n, np.uint64) if res.probe_lines else None
+    sets = None
+    if res.sketch_off:
+        g = int(res.n_sketches)
+        soff = _arr(res.sketch_off, g + 1, np.uint64)
+        sets = {"block": _arr(res.sketch_block, g, np.uint32), "taxon": _arr(res.sketch_taxon, g, np.uint32),
+                "dense": _arr(res.sketch_dense, g, np.uint32), "n": _arr(res.sketch_n, g, np.uint64),
+                "off": soff, "vals": _arr(res.sketch_vals, int(soff[-1]), np.uint32)}
     return ClassifyOutput(
         call=_arr(res.call, n, np.uint32),
         hit_groups=_arr(res.hit_groups, n, np.uint32),
@@ -441,6 +458,7 @@ def _copy_results(res: Results, hll: bool, kmer_counts: bool) -> ClassifyOutput:
         probe_lines=ln,
         redone=int(res.redone),
         kernel_ms=float(res.kernel_ms),
+        sets=sets,
     )
 
 
@@ -452,7 +470,8 @@ def write_report(db: "Db", off: np.ndarray, out: ClassifyOutput, path: str, *, p
                  zeros=False, kmer_data=False):
     """kreport of one classify call's results (k2g_report_write): the
     fragments' offsets and paired flags cut them into the reference's input
-    blocks; kmer_data needs results with HLL records."""
+    blocks; kmer_data needs results with HLL records (hll=1) or the
+    device's per-(block, taxon) sets (hll=2)."""
     lib = load_library()
     lib.k2g_report_write.argtypes = [C.c_void_p, C.POINTER(Fragments), C.POINTER(Results),
                                      C.POINTER(ReportParams), C.c_char_p]
@@ -479,6 +498,14 @@ def write_report(db: "Db", off: np.ndarray, out: ClassifyOutput, path: str, *, p
     r.hll_off = ptr(out.hll_off, C.c_uint64)
     r.hll = ptr(out.hll, C.c_uint32)
     r.kmer_counts = ptr(out.kmer_counts, C.c_uint64)
+    if out.sets is not None:
+        r.n_sketches = len(out.sets["block"])
+        r.sketch_block = ptr(out.sets["block"], C.c_uint32)
+        r.sketch_taxon = ptr(out.sets["taxon"], C.c_uint32)
+        r.sketch_dense = ptr(out.sets["dense"], C.c_uint32)
+        r.sketch_n = ptr(out.sets["n"], C.c_uint64)
+        r.sketch_off = ptr(out.sets["off"], C.c_uint64)
+        r.sketch_vals = ptr(out.sets["vals"] if len(out.sets["vals"]) else np.zeros(1, np.uint32), C.c_uint32)
     r.node_count = int(db.info.node_count)
     _check(lib.k2g_report_write(db._h, C.byref(fr), C.byref(r),
                                 C.byref(ReportParams(int(zeros), int(kmer_data), int(paired))), path.encode()))

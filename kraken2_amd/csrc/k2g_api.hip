@@ -681,6 +681,7 @@ static int check_params(const k2g_classify_params *p) {
     return fail(K2G_EUSAGE, "table_slots must be a power of two in [2, 256]");
   if (p->run_cap_pct < 0 || p->run_cap_pct > 100)
     return fail(K2G_EUSAGE, "run_cap_pct must be in [0, 100]");
+  if (p->want_hll < 0 || p->want_hll > 2) return fail(K2G_EUSAGE, "want_hll must be 0, 1 or 2");
   return K2G_OK;
 }
 
@@ -694,8 +695,13 @@ int k2g::classify_launch(k2g_batch *b, const k2g_classify_params *p, const char 
                          const uint64_t *d_off, const uint8_t *d_paired, uint64_t n,
                          const uint32_t *d_order) {
   if (!b) return fail(K2G_EUSAGE, "null batch");
+  const bool sketch_blocks = b->sk_pending;  // this call's input blocks were uploaded
+  b->sk_pending = false;
   int rc = check_params(p);
   if (rc) return rc;
+  if (p->want_hll == 2 && n && !sketch_blocks)
+    return fail(K2G_EUSAGE, "want_hll 2 needs host fragments (k2g_classify or the server): the input blocks "
+                            "come from their offsets");
   k2g_db *db = b->db;
   const DbView &v = db->view;
   b->n = n;
@@ -1020,6 +1026,7 @@ extern "C" int k2g_batch_fetch(k2g_batch *b, k2g_results *r) {
   HIP_TRY(h.words.reserve(8));
   h.run_off.p[0] = 0;
   const bool hll = n && b->params.want_hll, stats = n && b->params.want_probe_stats;
+  const bool sketches = hll && b->params.want_hll == 2;
   if (hll) HIP_TRY(h.rec_off.reserve(n + 1));
   if (n) {
     // phase 1, no host round trip: OR of every status word, the results'
@@ -1085,8 +1092,12 @@ extern "C" int k2g_batch_fetch(k2g_batch *b, k2g_results *r) {
     };
     rc = compact(b->n_runs.p, b->runs.p, b->comp_off, h.run_off.p[n], h.runs);
     if (rc) return rc;
-    if (hll) {
+    if (hll && !sketches) {
       rc = compact(b->n_recs.p, b->recs.p, b->comp_rec_off, h.rec_off.p[n], h.recs);
+      if (rc) return rc;
+    }
+    if (sketches) {  // the records reduced to per-(block, taxon) sets on the device
+      rc = sketch_fetch(b, h, h.rec_off.p[n]);
       if (rc) return rc;
     }
     HIP_TRY(hipStreamSynchronize(s));
@@ -1101,9 +1112,18 @@ extern "C" int k2g_batch_fetch(k2g_batch *b, k2g_results *r) {
   r->total_kmers = h.total.p;
   r->run_off = h.run_off.p;
   r->runs = h.runs.p;
-  if (hll) {
+  if (hll && !sketches) {
     r->hll_off = h.rec_off.p;
     r->hll = h.recs.p;
+  }
+  if (sketches) {
+    r->n_sketches = h.sk_words.p[0];
+    r->sketch_block = h.sk_block.p;
+    r->sketch_taxon = h.sk_taxon.p;
+    r->sketch_dense = h.sk_dense.p;
+    r->sketch_n = h.sk_n.p;
+    r->sketch_off = h.sk_off.p;
+    r->sketch_vals = h.sk_vals.p;
   }
   if (stats) {
     r->lookups = h.lookups.p;
@@ -1132,6 +1152,13 @@ extern "C" int k2g_classify(k2g_batch *b, const k2g_classify_params *p, const k2
     if (in->paired) {
       HIP_TRY(b->paired.reserve(n));
       HIP_TRY(hipMemcpyAsync(b->paired.p, in->paired, n, hipMemcpyHostToDevice, s));
+    }
+    if (p->want_hll == 2) {  // the per-(block, taxon) HLL sets need the input blocks
+      std::vector<uint64_t> starts;
+      k2g::input_blocks(in, p->paired != 0, starts);
+      rc = k2g::sketch_upload_blocks(b, starts);
+      if (rc) return rc;
+      b->sk_pending = true;
     }
   }
   rc = k2g_classify_device(b, p, b->seq.p, b->off.p, in->paired ? b->paired.p : nullptr, n,
@@ -1746,6 +1773,11 @@ uint64_t k2g::plan_classify_bytes(const k2g_db *db, const k2g_classify_params *p
   }
   b += region_bytes(n + 1, 8) * (p->want_hll ? 2 : 1);           // fetch offsets
   b += region_bytes(1, 4);                                        // fetch status word
+  if (p->want_hll == 2) {                                         // per-(block, taxon) HLL sets
+    std::vector<uint64_t> starts;
+    input_blocks(in, p->paired != 0, starts);
+    b += sketch_plan_bytes(db, sum_e, starts.size() - 1);
+  }
   return b;
 }
 
@@ -1758,36 +1790,45 @@ extern "C" int k2g_report_write(k2g_db *db, const k2g_fragments *in, const k2g_r
   const uint64_t n = in->n;
   if (r->n != n) return fail(K2G_EUSAGE, "results and fragments differ in count");
   if (n && (!in->off || !r->call)) return fail(K2G_EUSAGE, "null fragment offsets or calls");
-  if (p->kmer_data && n && (!r->hll_off || !r->hll))
-    return fail(K2G_EUSAGE, "a k-mer data report needs results with HLL records (want_hll)");
+  const bool sets = r->sketch_off != nullptr;
+  if (p->kmer_data && n && !(r->hll_off && r->hll) && !(sets && r->sketch_vals))
+    return fail(K2G_EUSAGE, "a k-mer data report needs results with HLL records or sets (want_hll)");
   if (r->node_count != db->tax.nodes.size()) return fail(K2G_EUSAGE, "results of another database");
-  auto paired = [&](uint64_t i) { return in->paired ? in->paired[i] != 0 : p->paired != 0; };
+  std::vector<uint64_t> starts;
+  k2g::input_blocks(in, p->paired != 0, starts);
+  const uint64_t n_blocks = starts.size() - 1;
   k2host::TaxonCounters counters;
   uint64_t classified = 0;
-  std::vector<std::pair<uint32_t, uint32_t>> recs;
-  for (uint64_t i = 0; i < n;) {
-    const bool pr = paired(i);
-    const uint64_t start = i;
-    if (pr) {
-      while (i < n && paired(i) && i - start < 10000) i++;
-    } else {
-      uint64_t bases = 0;
-      while (i < n && !paired(i) && bases < 3 * 1024 * 1024) {
-        bases += in->off[2 * i + 1] - in->off[2 * i];
-        i++;
-      }
+  for (uint64_t f = 0; f < n; f++) {
+    if (r->call[f] >= r->node_count) return fail(K2G_EUSAGE, "call is not a node of the taxonomy");
+    if (r->call[f]) {
+      counters[r->call[f]].n_reads++;
+      classified++;
     }
-    recs.clear();
-    for (uint64_t f = start; f < i; f++) {
-      if (r->call[f] >= r->node_count) return fail(K2G_EUSAGE, "call is not a node of the taxonomy");
-      if (r->call[f]) {
-        counters[r->call[f]].n_reads++;
-        classified++;
-      }
-      if (p->kmer_data)
+  }
+  if (p->kmer_data && sets) {
+    // the device's per-(block, taxon) sets: folded like the block sketches
+    // they stand for (a block's sketch merges into its taxon's total)
+    for (uint64_t g = 0; g < r->n_sketches; g++) {
+      if (r->sketch_block[g] >= n_blocks || r->sketch_taxon[g] >= r->node_count ||
+          r->sketch_off[g + 1] < r->sketch_off[g])
+        return fail(K2G_EUSAGE, "HLL sets of another call");
+      k2host::Distinct d;
+      d.n_observed = r->sketch_n[g];
+      d.dense = r->sketch_dense[g] != 0;
+      d.encoded.assign(r->sketch_vals + r->sketch_off[g], r->sketch_vals + r->sketch_off[g + 1]);
+      k2host::TaxonCounter &tc = counters[r->sketch_taxon[g]];
+      tc.n_kmers += d.n_observed;
+      tc.distinct.merge(d);
+    }
+  } else if (p->kmer_data) {
+    std::vector<std::pair<uint32_t, uint32_t>> recs;
+    for (uint64_t blk = 0; blk < n_blocks; blk++) {
+      recs.clear();
+      for (uint64_t f = starts[blk]; f < starts[blk + 1]; f++)
         for (uint64_t j = r->hll_off[f]; j < r->hll_off[f + 1]; j++) recs.emplace_back(r->hll[2 * j], r->hll[2 * j + 1]);
+      k2host::fold_block_records(recs, counters);
     }
-    if (p->kmer_data) k2host::fold_block_records(recs, counters);
   }
   if (!k2host::report_kraken_style(path, p->report_zeros != 0, p->kmer_data != 0, db->tax, counters, n,
                                    n - classified))
@@ -1859,6 +1900,16 @@ extern "C" int k2g_results_checksum(const k2g_results *r, uint64_t *out) {
   c.array(r->hll_off, r->hll_off ? n + 1 : 0);
   c.array(r->hll, r->hll_off ? 2 * r->hll_off[n] : 0);
   c.array(r->kmer_counts, r->kmer_counts ? r->node_count : 0);
+  if (r->sketch_off) {  // per-(block, taxon) HLL sets
+    const uint64_t g = r->n_sketches;
+    c.word(8u);
+    c.array(r->sketch_block, g);
+    c.array(r->sketch_taxon, g);
+    c.array(r->sketch_dense, g);
+    c.array(r->sketch_n, g);
+    c.array(r->sketch_off, g + 1);
+    c.array(r->sketch_vals, r->sketch_off[g]);
+  }
   *out = c.h;
   return K2G_OK;
 }

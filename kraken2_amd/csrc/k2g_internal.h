@@ -121,12 +121,17 @@ struct HostResults {
   // all pinned, so the device copies into them asynchronously
   PinnedBuf<uint32_t> call, groups, total, lookups, runs, recs, words;
   PinnedBuf<uint64_t> run_off, rec_off, kmer_counts, lines;
+  // per-(block, taxon) HLL sets (want_hll == 2); sk_words: groups, values
+  PinnedBuf<uint32_t> sk_block, sk_taxon, sk_dense, sk_vals;
+  PinnedBuf<uint64_t> sk_n, sk_off, sk_words;
   HostResults() = default;
   HostResults(const HostResults &) = delete;
   HostResults &operator=(const HostResults &) = delete;
   ~HostResults() {
-    for (auto *b : {&call, &groups, &total, &lookups, &runs, &recs, &words}) b->release();
-    for (auto *b : {&run_off, &rec_off, &kmer_counts, &lines}) b->release();
+    for (auto *b : {&call, &groups, &total, &lookups, &runs, &recs, &words, &sk_block, &sk_taxon, &sk_dense,
+                    &sk_vals})
+      b->release();
+    for (auto *b : {&run_off, &rec_off, &kmer_counts, &lines, &sk_n, &sk_off, &sk_words}) b->release();
   }
 };
 
@@ -167,6 +172,12 @@ struct k2g_batch {
   k2g::DevBuf<uint8_t> unit_flag;
   k2g::DevBuf<uint32_t> unit_start, n_units;
   k2g::DevBuf<unsigned int> fetch_word;  // OR of the status words at fetch
+  // per-(block, taxon) HLL sets (want_hll == 2): the call's input-block
+  // starts, sort keys and record indices, groups
+  k2g::DevBuf<uint64_t> sk_starts, sk_keys, sk_wide, sk_start, sk_off;
+  k2g::DevBuf<uint32_t> sk_idx, sk_idx_alt, sk_gkey, sk_n, sk_dense, sk_count;
+  uint64_t sk_blocks = 0;    // input blocks of the last call with want_hll == 2
+  bool sk_pending = false;   // k2g_classify uploaded the next call's block starts
   uint32_t dense_slots = 0;
   // host copies of the last results
   std::unique_ptr<k2g::HostResults> host = std::make_unique<k2g::HostResults>();
@@ -200,6 +211,8 @@ struct k2g_batch {
     f(redo_count), f(space), f(space_only), f(temp), f(sort_keys), f(sort_keys_out), f(sort_vals);
     f(dense_counts), f(dense_lists), f(dense_lens);
     f(unit_pos), f(unit_scan), f(unit_flag), f(unit_start), f(n_units), f(fetch_word);
+    f(sk_starts), f(sk_keys), f(sk_wide), f(sk_start), f(sk_off);
+    f(sk_idx), f(sk_idx_alt), f(sk_gkey), f(sk_n), f(sk_dense), f(sk_count);
   }
   // Allocates every device buffer from r from now on (null: hipMalloc).
   void bind_region(k2g::Region *r) {
@@ -214,6 +227,18 @@ namespace k2g {
 int classify_launch(k2g_batch *b, const k2g_classify_params *p, const char *d_seq,
                     const uint64_t *d_off, const uint8_t *d_paired, uint64_t n,
                     const uint32_t *d_order);
+// The reference's input blocks over a call's fragments (src/classify.cc:
+// 523-543): the first fragment of each block, then n.
+void input_blocks(const k2g_fragments *in, bool paired_all, std::vector<uint64_t> &starts);
+// Per-(block, taxon) HLL sets (want_hll == 2): whether a call's block and
+// taxon ids fit the sort key; the device bytes the reduction takes over a
+// call of `elements` k-mer elements; the upload of a call's block starts
+// (sets b->sk_blocks); the reduction of the last call's records into h
+// (enqueued on the batch's stream; the caller waits).
+int sketch_check(const k2g_db *db, uint64_t n_blocks);
+uint64_t sketch_plan_bytes(const k2g_db *db, uint64_t elements, uint64_t n_blocks);
+int sketch_upload_blocks(k2g_batch *b, const std::vector<uint64_t> &starts);
+int sketch_fetch(k2g_batch *b, HostResults &h, uint64_t records);
 // Resident waves of each persistent kernel variant on the current device,
 // into b->persistent_blocks.
 int persistent_grid(k2g_batch *b, int num_cus);

@@ -648,6 +648,8 @@ int check_pipeline_params(const k2g_classify_params *p, const k2g_pipeline_param
     return fail(K2G_EUSAGE, "table_slots must be a power of two in [2, 256]");
   if (p->run_cap_pct < 0 || p->run_cap_pct > 100) return fail(K2G_EUSAGE, "run_cap_pct must be in [0, 100]");
   if (p->want_probe_stats) return fail(K2G_EUSAGE, "probe statistics need k2g_classify_device");
+  if (p->want_hll < 0 || p->want_hll > 1)
+    return fail(K2G_EUSAGE, "a pipeline returns per-hit HLL records (want_hll 0 or 1)");
   if (pp->max_fragments == 0 || pp->max_fragments >= 0xFFFFFFFFull)
     return fail(K2G_EUSAGE, "max_fragments must be in [1, 2^32 - 2]");
   if (pp->bucket_shift < 0 || pp->bucket_shift > (int) kMaxShift)
