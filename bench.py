@@ -350,7 +350,7 @@ def serve_requests(pools, n_req, max_reads, rng):
     return reqs
 
 
-def serve_at_rate(k2, db, reqs, rate, budget, slots, work, rng):
+def serve_at_rate(k2, db, reqs, rate, budget, slots, work, rng, hll):
     """Poisson arrivals at `rate` requests/s into a fresh server; returns
     per-request (results checksum, times) and the server's statistics."""
     srv = k2.Server(db, budget, slots)
@@ -388,7 +388,7 @@ def serve_at_rate(k2, db, reqs, rate, budget, slots, work, rng):
                 time.sleep(delay)
             submitted[i] = time.perf_counter() - t0
             tickets[i] = srv.submit(r["seq"], r["off"], paired_flags=r["paired"], confidence=work["confidence"],
-                                    min_hit_groups=work["min_hit_groups"], kmer_counts=True, hll=True)
+                                    min_hit_groups=work["min_hit_groups"], kmer_counts=True, hll=hll)
             ready.release()
     finally:
         for _ in range(len(reqs) + len(th)):  # wake any waiter still waiting for a ticket
@@ -422,7 +422,7 @@ def run_serving(k2, db, tmp, pools, a, work):
     legs, digests = [], []
 
     def leg(name, rate, slots):
-        res, wall, st = serve_at_rate(k2, db, reqs, rate, budget, slots, work, rng)
+        res, wall, st = serve_at_rate(k2, db, reqs, rate, budget, slots, work, rng, a.serve_hll)
         lat = np.array([t["total_ms"] for _, t in res])
         row = {"leg": name, "slots": slots, "offered_req_per_s": round(rate, 2) if rate else None,
                "completed_req_per_s": round(len(reqs) / wall, 2), "reads_per_s": round(reads / wall, 1),
@@ -444,7 +444,8 @@ def run_serving(k2, db, tmp, pools, a, work):
     # each request by itself, the other legs run them side by side
     alone = digests[0]
     mismatched = sorted({i for dig in digests[1:] for i, d in enumerate(dig) if d != alone[i]})
-    kw = dict(confidence=work["confidence"], min_hit_groups=work["min_hit_groups"], kmer_counts=True, hll=True)
+    kw = dict(confidence=work["confidence"], min_hit_groups=work["min_hit_groups"], kmer_counts=True,
+              hll=a.serve_hll)
     # the oracle's classify driver on a spread of request sizes (up to
     # serve_oracle_max fragments, so the CPU check stays short)
     n_check = 0 if a.no_cpu else min(len(reqs), a.serve_oracle)
@@ -474,9 +475,11 @@ def run_serving(k2, db, tmp, pools, a, work):
         f"and {rep_mism} kreport mismatches over {len(pick)} requests ({checked} fragments)")
     sat = legs[1]["completed_req_per_s"] / mu
     two = legs[-1]["completed_req_per_s"] / mu
+    hll_form = ("per-(input block, taxon) distinct sets reduced on the GPU" if a.serve_hll == 2
+                else "per-hit HLL records")
     return {"workload": "c5: %d requests of log-uniform size in [1, %d] fragments sliced from the pairs 2x150 "
-                        "and uneven-length mixes (single reads or pairs), -R -K (counts + HLL records), against "
-                        "the resident table" % (len(reqs), max_reads),
+                        "and uneven-length mixes (single reads or pairs), -R -K (counts + %s), against "
+                        "the resident table" % (len(reqs), max_reads, hll_form),
             "requests": len(reqs), "reads": reads, "bases": bases,
             "budget_bytes": budget, "slots": a.serve_slots, "legs": legs,
             "saturation_vs_one_slot": round(sat, 3), "completed_at_2x_vs_one_slot": round(two, 3),
@@ -796,6 +799,8 @@ def main():
     ap.add_argument("--serve-oracle-max", type=int, default=50000,
                     help="largest request (fragments) checked against the oracle")
     ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-hll", type=int, default=2, choices=(1, 2),
+                    help="-K data of served requests: 1 per-hit records, 2 per-(block, taxon) sets from the GPU")
     ap.add_argument("--serve-budget-gb", type=float, default=32.0)
     ap.add_argument("--serve-pool", type=int, default=300_000, help="fragments per serving pool")
     ap.add_argument("--scale", type=float, default=1.0,

@@ -170,6 +170,13 @@ int k2g_batch_fetch_counts(k2g_batch *b, uint32_t *call, uint32_t *hit_groups,
  * into exact hitlist space (run_cap_pct overflow) and with the dense
  * per-taxon table (more distinct hit taxa than table_slots). */
 int k2g_batch_redo_counts(k2g_batch *b, uint64_t *run_space, uint64_t *dense_table);
+/* Long fragments of the last call that were split over waves, and their
+ * segments (both 0 when the call did not split).  Persistent mode splits the
+ * fragments of more than 4096 k-mers into segments of about 2048 when the
+ * batch's longest fragment outweighs an even share of the batch per
+ * resident wave (nucleotide databases; not with quick mode or probe
+ * statistics).  No reference counterpart: results are the same either way. */
+int k2g_batch_split_counts(k2g_batch *b, uint64_t *fragments, uint64_t *segments);
 /* One 64-bit digest per fragment of the last classify call (call, hit
  * groups, total k-mers and every hitlist run, in order), copied to
  * out[0..n).  Comparing digests checks two runs over the same fragments for

@@ -36,6 +36,7 @@ EXPORTED_SYMBOLS = (
     "k2g_batch_stream",
     "k2g_batch_fetch_counts",
     "k2g_batch_redo_counts",
+    "k2g_batch_split_counts",
     "k2g_batch_digests",
     "k2g_classify_device_noalloc",
     "k2g_db_gather_ceiling",
@@ -215,6 +216,7 @@ def load_library(path: str = LIB_PATH):
     lib.k2g_batch_fetch_counts.argtypes = [vp] + [vp] * 6 + [u64]
     lib.k2g_batch_redo_counts.argtypes = [vp, C.POINTER(u64), C.POINTER(u64)]
     lib.k2g_batch_digests.argtypes = [vp, vp, u64]
+    lib.k2g_batch_split_counts.argtypes = [vp, C.POINTER(u64), C.POINTER(u64)]
     lib.k2g_server_create.argtypes = [vp, C.POINTER(ServerParams), C.POINTER(vp)]
     lib.k2g_server_submit.argtypes = [vp, C.POINTER(ClassifyParams), C.POINTER(Fragments), C.POINTER(u64)]
     lib.k2g_server_wait.argtypes = [vp, u64, C.POINTER(Results), C.POINTER(C.c_double)]
@@ -410,6 +412,12 @@ class Batch:
         if probe_stats:
             res["lookups"], res["probe_lines"] = lk[:n], ln[:n]
         return res
+
+    def split_counts(self) -> tuple[int, int]:
+        """(long fragments of the last call split over waves, their segments)."""
+        a, b = C.c_uint64(), C.c_uint64()
+        _check(_lib.k2g_batch_split_counts(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def redo_counts(self) -> tuple[int, int]:
         """(fragments redone into exact hitlist space, fragments redone with the dense table)."""

@@ -439,6 +439,9 @@ namespace k2g {
 __global__ void k_bounds(const uint64_t *off, const uint8_t *paired, uint32_t paired_all,
                          uint64_t n, uint32_t k, uint32_t dna, uint32_t pct, uint64_t *bounds,
                          uint64_t *caps, uint32_t *keys, uint32_t *stats, bool want_lengths) {
+  // stats: [0] a read of 2^31 or more bases; [2..3] largest, [4..5] total
+  // element count; [6..7] elements and [8..9] number of the fragments with
+  // more than 2 * kSegElements (split candidates)
   const uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
   uint64_t e = 0;
   if (i < n) {
@@ -454,24 +457,35 @@ __global__ void k_bounds(const uint64_t *off, const uint8_t *paired, uint32_t pa
     caps[n] = 0;
   }
   if (!want_lengths) return;
-  // one atomic pair per block for the batch's largest and total element counts
-  __shared__ uint64_t bmax[256 / kWave], bsum[256 / kWave];
-  uint64_t mx = e, sum = e;
+  // one atomic each per block for the batch's statistics
+  __shared__ uint64_t bmax[256 / kWave], bsum[256 / kWave], bls[256 / kWave], blc[256 / kWave];
+  const bool lng = e > 2ull * kSegElements;
+  uint64_t mx = e, sum = e, ls = lng ? e : 0, lc = lng ? 1 : 0;
   for (int d = 1; d < kWave; d <<= 1) {
     const uint64_t om = (uint64_t) __shfl_xor((unsigned long long) mx, d);
     sum += (uint64_t) __shfl_xor((unsigned long long) sum, d);
+    ls += (uint64_t) __shfl_xor((unsigned long long) ls, d);
+    lc += (uint64_t) __shfl_xor((unsigned long long) lc, d);
     mx = om > mx ? om : mx;
   }
   const unsigned wv = threadIdx.x / kWave;
   if ((threadIdx.x & (kWave - 1)) == 0) {
     bmax[wv] = mx;
     bsum[wv] = sum;
+    bls[wv] = ls;
+    blc[wv] = lc;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     for (unsigned j = 1; j < blockDim.x / kWave; j++) {
       mx = bmax[j] > mx ? bmax[j] : mx;
       sum += bsum[j];
+      ls += bls[j];
+      lc += blc[j];
+    }
+    if (lc) {
+      atomicAdd(reinterpret_cast<unsigned long long *>(stats + 6), (unsigned long long) ls);
+      atomicAdd(reinterpret_cast<unsigned long long *>(stats + 8), (unsigned long long) lc);
     }
     if (sum) {
       atomicMax(reinterpret_cast<unsigned long long *>(stats + 2), (unsigned long long) mx);
@@ -709,6 +723,7 @@ int k2g::classify_launch(k2g_batch *b, const k2g_classify_params *p, const char 
   b->redone = 0;
   b->space_redone = 0;
   b->kernel_ms = 0;
+  b->split_last = false;
   if (n == 0) return K2G_OK;
   if (n >= (1ULL << 32)) return fail(K2G_EUSAGE, "batch too large (fragments must fit 32 bits)");
   if (!d_seq || !d_off) return fail(K2G_EUSAGE, "null fragment arrays");
@@ -724,7 +739,7 @@ int k2g::classify_launch(k2g_batch *b, const k2g_classify_params *p, const char 
   HIP_TRY(b->run_cap.reserve(n + 1));
   HIP_TRY(b->bounds.reserve(n + 1));
   HIP_TRY(b->sort_keys.reserve(n));
-  HIP_TRY(b->redo_count.reserve(8));  // three redo counts, then k_bounds's statistics
+  HIP_TRY(b->redo_count.reserve(12));  // three redo counts, then k_bounds's statistics
   HIP_TRY(b->cursor.reserve(kCursorWords));
   // in a bounded region, the largest scratch any step needs is taken once
   const bool may_sort = p->mode != 0 && !d_order;
@@ -733,31 +748,35 @@ int k2g::classify_launch(k2g_batch *b, const k2g_classify_params *p, const char 
   // element bounds -> reserved run space -> run offsets
   const uint32_t pct = p->run_cap_pct ? (uint32_t) p->run_cap_pct : 25u;
   const unsigned g1 = (unsigned) ((n + 1 + 255) / 256);
-  uint32_t *const stats = b->redo_count.p + 2;  // 8-byte aligned: [0] flag, [2..3] max, [4..5] sum
-  HIP_TRY(hipMemsetAsync(stats, 0, 24, s));
+  uint32_t *const stats = b->redo_count.p + 2;  // 8-byte aligned (see k_bounds)
+  HIP_TRY(hipMemsetAsync(stats, 0, 40, s));
   hipLaunchKernelGGL(k_bounds, dim3(g1), dim3(256), 0, s, d_off, d_paired,
                      (uint32_t) (p->paired != 0), n, v.k, v.dna, pct, b->bounds.p, b->run_cap.p,
-                     may_sort ? b->sort_keys.p : nullptr, stats, p->mode == 2);
+                     may_sort ? b->sort_keys.p : nullptr, stats, p->mode != 0);
   HIP_TRY(hipGetLastError());
   size_t tbytes = 0;
   HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tbytes, b->run_cap.p, b->run_off.p, n + 1, s));
   HIP_TRY(b->temp.reserve(tbytes));
   HIP_TRY(hipcub::DeviceScan::ExclusiveSum(b->temp.p, tbytes, b->run_cap.p, b->run_off.p, n + 1, s));
-  uint64_t total_space = 0, max_e = 0, sum_e = 0;
+  uint64_t total_space = 0, max_e = 0, sum_e = 0, split_sum = 0, split_cnt = 0;
   if (b->plan.valid) {  // the caller's host-side statistics: nothing to read back
     b->plan.valid = false;
     if (b->plan.too_long) return fail(K2G_EUSAGE, "a read has 2^31 or more bases");
     total_space = b->plan.space;
     max_e = b->plan.max_e;
     sum_e = b->plan.sum_e;
+    split_sum = b->plan.split_sum;
+    split_cnt = b->plan.split_cnt;
   } else {
-    uint32_t st[6] = {0, 0, 0, 0, 0, 0};
+    uint32_t st[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     HIP_TRY(hipMemcpyAsync(&total_space, b->run_off.p + n, 8, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipMemcpyAsync(st, stats, 24, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(st, stats, 40, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     if (st[0]) return fail(K2G_EUSAGE, "a read has 2^31 or more bases");
     max_e = st[2] | ((uint64_t) st[3] << 32);
     sum_e = st[4] | ((uint64_t) st[5] << 32);
+    split_sum = st[6] | ((uint64_t) st[7] << 32);
+    split_cnt = st[8] | ((uint64_t) st[9] << 32);
   }
   // mode 2: the persistent kernel pays for its sort and work units only when
   // lengths are uneven (the longest fragment above four times the mean);
@@ -843,6 +862,14 @@ int k2g::classify_launch(k2g_batch *b, const k2g_classify_params *p, const char 
                                                p->want_probe_stats || p->quick);
   const ClassifyKernels &K = kVariants[vi];
   if (b->ev0) HIP_TRY(hipEventRecord(b->ev0, s));
+  // a batch whose longest fragment outweighs a wave's share splits its long
+  // fragments over waves (segments before, stitch after the persistent kernel)
+  const bool split = persistent && !d_order && !b->fixed_outputs &&
+                     split_wanted(p, v.dna != 0, persistent, max_e, sum_e, (uint64_t) b->persistent_blocks[vi]);
+  if (split) {
+    rc = split_launch(b, p, cp, in, out, split_sum, split_cnt, 1);
+    if (rc) return rc;
+  }
   if (!persistent) {
     hipLaunchKernelGGL(K.plain, dim3((unsigned) n), dim3(64), 0, s, v, cp, in, out);
   } else {
@@ -851,6 +878,10 @@ int k2g::classify_launch(k2g_batch *b, const k2g_classify_params *p, const char 
     hipLaunchKernelGGL(K.persistent, dim3(grid), dim3(64), 0, s, v, cp, in, out, b->cursor.p);
   }
   HIP_TRY(hipGetLastError());
+  if (split) {
+    rc = split_launch(b, p, cp, in, out, split_sum, split_cnt, 2);
+    if (rc) return rc;
+  }
   if (b->ev1) HIP_TRY(hipEventRecord(b->ev1, s));
 
   // fragments to classify again: out of hitlist space, or hit table overflowed
@@ -905,6 +936,7 @@ int k2g::classify_launch(k2g_batch *b, const k2g_classify_params *p, const char 
       in2.order = b->space_only.p;
       in2.unit_start = nullptr;
       in2.n_units = nullptr;
+      in2.split = nullptr;  // classified whole
       HIP_TRY(hipMemsetAsync(b->cursor.p, 0, kCursorWords * 8, s));
       const unsigned grid = (unsigned) std::min<uint64_t>(b->persistent_blocks[vi], n_space_only);
       hipLaunchKernelGGL(K.persistent, dim3(grid), dim3(64), 0, s, v, cp, in2, out2,
@@ -983,6 +1015,18 @@ extern "C" int k2g_batch_redo_counts(k2g_batch *b, uint64_t *run_space, uint64_t
   if (rc) return rc;
   if (run_space) *run_space = b->space_redone;
   if (dense_table) *dense_table = b->redone;
+  return K2G_OK;
+}
+
+extern "C" int k2g_batch_split_counts(k2g_batch *b, uint64_t *fragments, uint64_t *segments) {
+  if (!b) return fail(K2G_EUSAGE, "null batch");
+  uint32_t c[2] = {0, 0};
+  if (b->split_last && b->sp_count.p) {
+    HIP_TRY(hipMemcpyAsync(c, b->sp_count.p, 8, hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+  }
+  if (fragments) *fragments = c[1];
+  if (segments) *segments = c[0];
   return K2G_OK;
 }
 
@@ -1720,7 +1764,7 @@ uint64_t k2g::plan_classify_bytes(const k2g_db *db, const k2g_classify_params *p
   const DbView &v = db->view;
   const uint64_t bases = in->off[2 * n];
   const uint32_t pct = p->run_cap_pct ? (uint32_t) p->run_cap_pct : 25u;
-  uint64_t space = 0, max_e = 0, sum_e = 0;
+  uint64_t space = 0, max_e = 0, sum_e = 0, split_sum = 0, split_cnt = 0;
   bool too_long = false;
   for (uint64_t i = 0; i < n; i++) {
     const bool pr = in->paired ? in->paired[i] != 0 : p->paired != 0;
@@ -1731,6 +1775,10 @@ uint64_t k2g::plan_classify_bytes(const k2g_db *db, const k2g_classify_params *p
     space += run_space(e, pct);
     max_e = std::max(max_e, e);
     sum_e += e;
+    if (e > 2ull * kSegElements) {
+      split_sum += e;
+      split_cnt++;
+    }
   }
   if (stats) {
     stats->valid = true;
@@ -1738,6 +1786,8 @@ uint64_t k2g::plan_classify_bytes(const k2g_db *db, const k2g_classify_params *p
     stats->space = space;
     stats->max_e = max_e;
     stats->sum_e = sum_e;
+    stats->split_sum = split_sum;
+    stats->split_cnt = split_cnt;
   }
   const bool sort = p->mode != 0;
   uint64_t b = 0;
@@ -1756,6 +1806,8 @@ uint64_t k2g::plan_classify_bytes(const k2g_db *db, const k2g_classify_params *p
     b += region_bytes(n + 1, 4) + region_bytes(1, 4);             // unit starts, unit count
   }
   if (p->want_probe_stats) b += region_bytes(n, 4) + region_bytes(n, 8);
+  if (sort && v.dna && !p->quick && !p->want_probe_stats && max_e > 2ull * kSegElements)
+    b += split_plan_bytes(n, split_sum, split_cnt, p->want_hll != 0);  // long fragments split over waves
   b += 3 * region_bytes(n, 4);                                    // redo lists
   const uint64_t slots = pct >= 100 ? dense_slots_for(n, v.node_count)  // dense-table redo, worst case
                                     : std::min<uint64_t>(n, 256);

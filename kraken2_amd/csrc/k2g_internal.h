@@ -176,6 +176,14 @@ struct k2g_batch {
   // starts, sort keys and record indices, groups
   k2g::DevBuf<uint64_t> sk_starts, sk_keys, sk_wide, sk_start, sk_off;
   k2g::DevBuf<uint32_t> sk_idx, sk_idx_alt, sk_gkey, sk_n, sk_dense, sk_count;
+  // long fragments split over waves (persistent mode): per-fragment segment
+  // counts and starts, the split list, the segments and their scratch
+  k2g::DevBuf<uint32_t> sp_cnt, sp_list, sp_count, sp_frag, sp_mate, sp_a, sp_b, sp_runs_n, sp_nrec, sp_groups,
+      sp_status, sp_runs, sp_recs;
+  k2g::DevBuf<uint64_t> sp_wide, sp_start, sp_cap, sp_off;
+  uint64_t sp_segs = 0;       // segment capacity of the last split call
+  bool sp_active = false;     // the call being launched splits
+  bool split_last = false;    // the last call split fragments
   uint64_t sk_blocks = 0;    // input blocks of the last call with want_hll == 2
   bool sk_pending = false;   // k2g_classify uploaded the next call's block starts
   uint32_t dense_slots = 0;
@@ -196,6 +204,7 @@ struct k2g_batch {
   struct HostPlan {
     bool valid = false, too_long = false;
     uint64_t space = 0, max_e = 0, sum_e = 0;
+    uint64_t split_sum = 0, split_cnt = 0;  // elements and number of the fragments a split would cut
   } plan;
   // the dense-table redo of the last call was sized on the device: its
   // count is read when results are fetched
@@ -213,11 +222,15 @@ struct k2g_batch {
     f(unit_pos), f(unit_scan), f(unit_flag), f(unit_start), f(n_units), f(fetch_word);
     f(sk_starts), f(sk_keys), f(sk_wide), f(sk_start), f(sk_off);
     f(sk_idx), f(sk_idx_alt), f(sk_gkey), f(sk_n), f(sk_dense), f(sk_count);
+    f(sp_cnt), f(sp_list), f(sp_count), f(sp_frag), f(sp_mate), f(sp_a), f(sp_b), f(sp_runs_n), f(sp_nrec),
+        f(sp_groups), f(sp_status), f(sp_runs), f(sp_recs);
+    f(sp_wide), f(sp_start), f(sp_cap), f(sp_off);
   }
   // Allocates every device buffer from r from now on (null: hipMalloc).
   void bind_region(k2g::Region *r) {
     for_each_device_buf([r](auto &buf) { buf.bind(r); });
     dense_slots = 0;
+    split_last = false;
   }
 };
 
@@ -227,6 +240,19 @@ namespace k2g {
 int classify_launch(k2g_batch *b, const k2g_classify_params *p, const char *d_seq,
                     const uint64_t *d_off, const uint8_t *d_paired, uint64_t n,
                     const uint32_t *d_order);
+// Long nucleotide fragments split over waves: segments of about this many
+// k-mers, for fragments of more than twice as many, in batches whose longest
+// fragment outweighs an even share of the batch per resident wave.
+constexpr uint32_t kSegElements = 2048;
+bool split_wanted(const k2g_classify_params *p, bool dna, bool persistent, uint64_t max_e, uint64_t sum_e,
+                  uint64_t waves);
+// Device bytes a split takes (upper bound, for Region plans), given the
+// elements and number of the fragments it cuts.
+uint64_t split_plan_bytes(uint64_t n, uint64_t split_sum, uint64_t split_cnt, bool hll);
+// phase 1 (before the persistent kernel): plan and launch the segments, and
+// set in.split; phase 2 (after it): the stitch.  Enqueued on b->stream.
+int split_launch(k2g_batch *b, const k2g_classify_params *p, const ClassifyParams &cp, FragBatch &in,
+                 const FragOut &out, uint64_t split_sum, uint64_t split_cnt, int phase);
 // The reference's input blocks over a call's fragments (src/classify.cc:
 // 523-543): the first fragment of each block, then n.
 void input_blocks(const k2g_fragments *in, bool paired_all, std::vector<uint64_t> &starts);

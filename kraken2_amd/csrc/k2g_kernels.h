@@ -38,6 +38,30 @@ struct FragBatch {
   // fragment per unit.
   const uint32_t *unit_start;
   const uint32_t *n_units;
+  // Per fragment, nonzero when the fragment is classified by segments
+  // (SegBatch) instead: the persistent kernel skips it.  Null: none is.
+  const uint32_t *split;
+};
+
+// Long nucleotide fragments split over waves.  Segment s classifies the
+// k-mers ending at positions [a, b) of one mate of fragment frag: it scans
+// that mate from k bases before a (the k-mer ending at a - 1, free of
+// ambiguous bases by the choice of a, supplies the previous minimizer and
+// the taxon a repeat of it takes) and writes its runs and HLL records to its
+// own space in the scratch.  A stitch pass then joins a fragment's segments
+// in order (runs that meet at a seam with one code merge; the mate marker
+// sits between the mates), rebuilds the hit table from the joined runs and
+// resolves the call.  The first segment of a mate starts at a = k - 1; a
+// mate shorter than k has one empty segment.
+struct SegBatch {
+  uint32_t *frag, *mate, *a, *b;              // per segment
+  uint64_t *off, *cap;                        // per segment: its run / record space in the scratch
+  uint32_t *n_runs, *nrec, *groups, *status;  // per segment: what it produced
+  uint32_t *runs, *recs;                      // the scratch
+  const uint32_t *count;                      // [0] segments, [1] split fragments
+  const uint32_t *split_list;                 // the split fragments
+  const uint32_t *seg_cnt;                    // per fragment: its segments (0: not split)
+  const uint64_t *seg_start;                  // per fragment: its first segment
 };
 
 struct FragOut {

@@ -136,6 +136,12 @@ struct LdsTable {  // open addressing in LDS; taxon 0 marks an empty slot
 
 // Loads that bypass a possibly stale vector L1 line: the dense table is
 // updated with L2 atomics by every lane of the block.
+// A segment of a split fragment counts no hits: the stitch pass rebuilds the
+// fragment's table from its joined runs.
+struct NullTable {
+  __device__ __forceinline__ bool add(uint32_t, uint32_t) { return true; }
+};
+
 __device__ __forceinline__ uint32_t load_agent(const uint32_t *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -356,6 +362,9 @@ struct ProbeState {
 // (else translated search); EXTRAS: per-hit HLL records, per-taxon counts or
 // probe statistics may be requested (their code is compiled out otherwise);
 // W: the window's l-mer count k-l+1 when fixed at compile time (<= 9), else 0.
+// first_emit: k-mers ending before this position of the stream are scanned
+// (they set the previous minimizer and its taxon) but are not elements of
+// the fragment: no run, hit, record or count (0 for a whole stream).
 //
 // The stream is processed in probe batches of up to kBatchChunks chunks of
 // 64 k-mer end positions:
@@ -370,7 +379,7 @@ template <bool DNA, bool EXTRAS, int W, class Table>
 __device__ void scan_stream(const DbView &db, const ClassifyParams &P, const FragOut &out,
                             uint64_t base, uint64_t cap, const char *m, uint64_t L, int fr,
                             const char *m2, uint64_t L2, RunState &st, Table &tab,
-                            WaveLds<EXTRAS> &lds, int *bad, int lane) {
+                            WaveLds<EXTRAS> &lds, int *bad, int lane, uint32_t first_emit = 0) {
   static_assert(W >= 0 && W <= kRingGuard + 1, "window too wide for the ring guard");
   constexpr bool dna = DNA;
   // positions within a stream are 32-bit: the library rejects reads of 2^31
@@ -525,7 +534,7 @@ __device__ void scan_stream(const DbView &db, const ClassifyParams &P, const Fra
       }
       const bool lookup = nonamb && mn != prev_min;
       const uint64_t LK = __ballot(lookup);
-      const uint64_t E = __ballot(kvalid);
+      const uint64_t E = __ballot(kvalid && p >= first_emit);
       if (lookup) lds.qkey[qn + __popcll(LK & lanemask_lt(lane))] = mn;
       if (lane == 0) {
         lds.cmask[g][0] = NA;
@@ -605,7 +614,7 @@ __device__ void scan_stream(const DbView &db, const ClassifyParams &P, const Fra
       const uint64_t srcm = LK & lanemask_le(lane);
       const uint32_t tq = srcm ? lds.qval(qb + __popcll(srcm) - 1) : last_taxon;
       const uint32_t tax_i = nonamb ? tq : 0u;
-      uint64_t HIT = __ballot(lookup && tq != 0);
+      uint64_t HIT = __ballot(lookup && kvalid && tq != 0);
 
       if (EXTRAS && P.quick) {  // src/classify.cc:880-884: stop at the first qualifying hit
         const uint32_t g_le = st.groups + __popcll(HIT & lanemask_le(lane));
@@ -715,6 +724,10 @@ __device__ __forceinline__ uint32_t reduce_sum(uint32_t v) {
   return v;
 }
 
+template <bool EXTRAS, class Table>
+__device__ void resolve_fragment(const DbView &db, const ClassifyParams &P, const FragOut &out, uint64_t f,
+                                 const RunState &st, Table &tab, bool paired, bool tx, int *bad, int lane);
+
 template <bool DNA, bool EXTRAS, class Table>
 __device__ void classify_fragment(const DbView &db, const ClassifyParams &P, const FragBatch &in,
                                   const FragOut &out, uint64_t f, Table &tab, WaveLds<EXTRAS> &lds,
@@ -758,8 +771,15 @@ __device__ void classify_fragment(const DbView &db, const ClassifyParams &P, con
     st.n_runs++;
   }
   __syncthreads();
+  resolve_fragment<EXTRAS>(db, P, out, f, st, tab, paired, tx, bad, lane);
+}
 
-  // ResolveTree (src/classify.cc:710-762)
+// ResolveTree (src/classify.cc:710-762) over a fragment's hit table, the
+// confidence and minimum-hit-group rules (src/classify.cc:890-905), and the
+// fragment's per-fragment outputs.
+template <bool EXTRAS, class Table>
+__device__ void resolve_fragment(const DbView &db, const ClassifyParams &P, const FragOut &out, uint64_t f,
+                                 const RunState &st, Table &tab, bool paired, bool tx, int *bad, int lane) {
   uint64_t total = st.elems;
   if (paired) total--;
   if (tx) total -= paired ? 4 : 2;
@@ -914,6 +934,7 @@ __device__ __forceinline__ void persistent_body(const DbView &db, const Classify
     const uint64_t hi = in.unit_start ? uniform32(in.unit_start[u + 1]) : u + 1;
     for (uint64_t j = lo; j < hi; j++) {
       const uint64_t f = uniform64(in.order ? (uint64_t) in.order[j] : j);
+      if (in.split && uniform32(in.split[f]) != 0) continue;  // classified by segments
       classify_fragment<DNA, EXTRAS>(db, P, in, out, f, tab, lds, &lds.bad, lane);
     }
     u = kCursors * (uniform64((uint64_t) next) >> 6) + part;
@@ -943,6 +964,235 @@ __device__ __forceinline__ void dense_body(const DbView &db, const ClassifyParam
   __syncthreads();
   for (uint32_t i = blockIdx.x; i < n_redo; i += gridDim.x)
     classify_fragment<DNA, true>(db, P, in, out, uniform32(redo[i]), tab, lds, &lds.bad, lane);
+}
+
+// ---- long fragments split over waves: the segment layout (see SegBatch)
+__device__ __host__ inline uint32_t mate_kmers(uint64_t L, uint32_t k) { return L >= k ? (uint32_t) (L - k + 1) : 0u; }
+
+__device__ __host__ inline uint32_t mate_segments(uint32_t e, uint32_t seg) { return e ? (e + seg - 1) / seg : 1u; }
+
+// Step 1: per fragment, its segment count (0: not split); the split
+// fragments listed; cnt[n] = 0 for the scan.
+__global__ __launch_bounds__(256) void k_seg_count(const uint64_t *off, const uint8_t *paired, uint32_t paired_all,
+                                                   uint64_t n, uint32_t k, uint32_t seg, const uint64_t *elements,
+                                                   uint32_t *cnt, uint32_t *list, uint32_t *count) {
+  const uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > n) return;
+  if (i == n) {
+    cnt[n] = 0;
+    return;
+  }
+  uint32_t c = 0;
+  if (elements[i] > 2ull * seg) {
+    const bool pr = paired ? paired[i] != 0 : paired_all != 0;
+    c = mate_segments(mate_kmers(off[2 * i + 1] - off[2 * i], k), seg);
+    if (pr) c += mate_segments(mate_kmers(off[2 * i + 2] - off[2 * i + 1], k), seg);
+    list[atomicAdd(&count[1], 1u)] = (uint32_t) i;
+  }
+  cnt[i] = c;
+}
+
+// Step 3: one thread per split fragment lays out its segments: per mate, cut
+// points from the last to the first, each moved forward past ambiguous bases
+// (never beyond the next cut), and each segment's reserved run space.
+__global__ __launch_bounds__(64) void k_seg_plan(const char *seq, const uint64_t *off, const uint8_t *paired,
+                                                 uint32_t paired_all, uint32_t k, uint32_t seg, uint32_t pct,
+                                                 const uint32_t *list, const uint64_t *seg_start, uint32_t *count,
+                                                 uint64_t seg_total_at, SegBatch sg) {
+  const uint32_t nf = count[1];
+  const uint64_t t = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (t == 0) count[0] = (uint32_t) seg_start[seg_total_at];
+  for (uint64_t i = t; i < nf; i += (uint64_t) gridDim.x * blockDim.x) {
+    const uint32_t f = list[i];
+    const bool pr = paired ? paired[f] != 0 : paired_all != 0;
+    uint64_t s = seg_start[f];
+    for (uint32_t mate = 0; mate < (pr ? 2u : 1u); mate++) {
+      const char *m = seq + off[2 * f + mate];
+      const uint32_t L = (uint32_t) (off[2 * f + mate + 1] - off[2 * f + mate]);
+      const uint32_t e = mate_kmers(L, k), segs = mate_segments(e, seg);
+      uint32_t next = e ? L : 0u;  // k-mers end at positions [k - 1, L)
+      for (uint32_t j = segs; j-- > 0;) {
+        uint32_t a = e ? (k - 1) + (uint32_t) (((uint64_t) j * e) / segs) : 0u;
+        if (j > 0) {
+          // the k bases [a - k, a) must be free of ambiguous bases: past the
+          // last ambiguous one among them, and check again (a >= k here)
+          while (a < next) {
+            uint32_t amb = 0xFFFFFFFFu;
+            for (uint32_t r = a; r > a - k; r--)
+              if (dna_code((uint8_t) m[r - 1]) == 0xFFu) {
+                amb = r - 1;
+                break;
+              }
+            if (amb == 0xFFFFFFFFu) break;
+            a = amb + k + 1;
+          }
+          if (a > next) a = next;
+        }
+        sg.frag[s + j] = f;
+        sg.mate[s + j] = mate;
+        sg.a[s + j] = a;
+        sg.b[s + j] = next;
+        const uint64_t el = next > a ? next - a : 0u;
+        sg.cap[s + j] = run_space(el, pct);
+        next = a;
+      }
+      s += segs;
+    }
+  }
+}
+
+// Segments of split fragments (nucleotide): one wave per segment, each a
+// fresh scan of its stretch of one mate (see SegBatch).
+template <bool EXTRAS>
+__device__ __forceinline__ void segment_body(const DbView &db, const ClassifyParams &P, const FragBatch &in,
+                                             const FragOut &out, const SegBatch &sg) {
+  __shared__ WaveLds<EXTRAS> lds;
+  const int lane = threadIdx.x;
+  const uint32_t k = db.k;
+  const uint32_t ns = uniform32(sg.count[0]);
+  FragOut so = out;  // the segment's runs and records go to the scratch
+  so.runs = sg.runs;
+  so.recs = out.recs ? sg.recs : nullptr;
+  so.lookups = nullptr;
+  so.lines = nullptr;
+  for (uint32_t s = blockIdx.x; s < ns; s += gridDim.x) {
+    const uint32_t a = uniform32(sg.a[s]), b = uniform32(sg.b[s]);
+    uint32_t nr = 0, nrec = 0, groups = 0, status = 0;
+    if (b > a) {
+      const uint64_t f = uniform32(sg.frag[s]);
+      const uint32_t mate = uniform32(sg.mate[s]);
+      const char *m = in.seq + uniform64(in.off[2 * f + mate]);
+      const bool first = a + 1 == k;  // the mate's first k-mer: no scan before it
+      const uint32_t s0 = first ? 0u : a - k, first_emit = first ? 0u : k;
+      const uint64_t base = uniform64(sg.off[s]), cap = uniform64(sg.cap[s]);
+      if (lane == 0) lds.bad = 0;
+      __syncthreads();
+      RunState st;
+      NullTable nt;
+      if (db.k - db.l == 4 && db.revcom_version != 0)
+        scan_stream<true, EXTRAS, 5>(db, P, so, base, cap, m + s0, b - s0, 0, nullptr, 0, st, nt, lds, &lds.bad,
+                                     lane, first_emit);
+      else
+        scan_stream<true, EXTRAS, 0>(db, P, so, base, cap, m + s0, b - s0, 0, nullptr, 0, st, nt, lds, &lds.bad,
+                                     lane, first_emit);
+      if (st.run_count > 0) {
+        if (lane == 0) close_run(so.runs + 2 * base, cap, st.n_runs, st.run_code, st.run_count, nt, &lds.bad);
+        st.n_runs++;
+      }
+      __syncthreads();
+      nr = st.n_runs;
+      nrec = st.nrec;
+      groups = st.groups;
+      status = (uint32_t) uniform32((uint32_t) lds.bad);
+    }
+    if (lane == 0) {
+      sg.n_runs[s] = nr;
+      sg.nrec[s] = nrec;
+      sg.groups[s] = groups;
+      sg.status[s] = status;
+    }
+    __syncthreads();  // the next segment reuses the LDS
+  }
+}
+
+// Joins each split fragment's segments in order into the fragment's runs
+// and records, rebuilds its hit table from the joined runs, and resolves it.
+template <bool EXTRAS>
+__device__ __forceinline__ void stitch_body(const DbView &db, const ClassifyParams &P, const FragBatch &in,
+                                            const FragOut &out, const SegBatch &sg) {
+  __shared__ WaveLds<EXTRAS> lds;
+  LdsTable tab{lds.tkey, lds.tcnt, P.tab_cap};
+  const int lane = threadIdx.x;
+  const uint32_t nf = uniform32(sg.count[1]);
+  for (uint32_t i = blockIdx.x; i < nf; i += gridDim.x) {
+    const uint64_t f = uniform32(sg.split_list[i]);
+    const bool paired = uniform32(in.paired ? in.paired[f] : in.paired_all) != 0;
+    const uint64_t base = uniform64(out.run_off[f]), cap = uniform64(out.run_cap[f]);
+    uint32_t *const runs_at = out.runs + 2 * base;
+    tab.clear(lane);
+    if (lane == 0) lds.bad = 0;
+    __syncthreads();
+    RunState st;
+    uint32_t status = 0;
+    const uint64_t s_lo = uniform64(sg.seg_start[f]), s_hi = s_lo + uniform32(sg.seg_cnt[f]);
+    for (uint64_t s = s_lo; s < s_hi; s++) {
+      const uint32_t mate = uniform32(sg.mate[s]);
+      if (mate == 1 && (s == s_lo || uniform32(sg.mate[s - 1]) == 0))
+        push_marker(out, base, cap, st, kCodeMate, tab, &lds.bad, lane);
+      const uint32_t a = uniform32(sg.a[s]), b = uniform32(sg.b[s]);
+      if (b <= a) continue;
+      const uint32_t seg_status = uniform32(sg.status[s]);
+      status |= seg_status;
+      // a segment that outgrew its space is incomplete: the fragment is
+      // classified again whole (status bit 1), so nothing of it is joined
+      if (seg_status & 2u) continue;
+      st.elems += b - a;
+      st.groups += uniform32(sg.groups[s]);
+      const uint64_t so = uniform64(sg.off[s]);
+      // HLL records, in order
+      const uint32_t nrec = uniform32(sg.nrec[s]);
+      if (out.recs) {
+        for (uint32_t j = (uint32_t) lane; j < nrec; j += kWave) {
+          const uint64_t r = st.nrec + j;
+          if (r < cap) {
+            out.recs[2 * (base + r)] = sg.recs[2 * (so + j)];
+            out.recs[2 * (base + r) + 1] = sg.recs[2 * (so + j) + 1];
+          } else {
+            atomicOr(&lds.bad, 2);
+          }
+        }
+      }
+      st.nrec += nrec;
+      // runs: the first may continue the open run; the last stays open
+      const uint32_t r = uniform32(sg.n_runs[s]);
+      if (r == 0) continue;
+      const uint32_t *R = sg.runs + 2 * so;
+      const uint32_t c0 = uniform32(R[0]);
+      uint32_t n0 = uniform32(R[1]);
+      if (st.run_count > 0 && st.run_code == c0) {
+        n0 += st.run_count;
+      } else if (st.run_count > 0) {
+        if (lane == 0) close_run(runs_at, cap, st.n_runs, st.run_code, st.run_count, tab, &lds.bad);
+        st.n_runs++;
+      }
+      if (r == 1) {
+        st.run_code = c0;
+        st.run_count = n0;
+        continue;
+      }
+      if (lane == 0) close_run(runs_at, cap, st.n_runs, c0, n0, tab, &lds.bad);
+      for (uint32_t j = 1u + (uint32_t) lane; j + 1 < r; j += kWave)
+        close_run(runs_at, cap, st.n_runs + j, R[2 * j], R[2 * j + 1], tab, &lds.bad);
+      st.n_runs += r - 1;
+      st.run_code = uniform32(R[2 * (r - 1)]);
+      st.run_count = uniform32(R[2 * (r - 1) + 1]);
+    }
+    if (st.run_count > 0) {
+      if (lane == 0) close_run(runs_at, cap, st.n_runs, st.run_code, st.run_count, tab, &lds.bad);
+      st.n_runs++;
+    }
+    if (lane == 0 && status) atomicOr(&lds.bad, (int) status);
+    __syncthreads();
+    resolve_fragment<EXTRAS>(db, P, out, f, st, tab, paired, false, &lds.bad, lane);
+    __syncthreads();  // the next fragment reuses the LDS
+  }
+}
+
+__global__ __launch_bounds__(64) void k_classify_segments_dna(DbView db, ClassifyParams P, FragBatch in,
+                                                              FragOut out, SegBatch sg) {
+  segment_body<false>(db, P, in, out, sg);
+}
+__global__ __launch_bounds__(64) void k_classify_segments_dna_x(DbView db, ClassifyParams P, FragBatch in,
+                                                                FragOut out, SegBatch sg) {
+  segment_body<true>(db, P, in, out, sg);
+}
+__global__ __launch_bounds__(64) void k_stitch_dna(DbView db, ClassifyParams P, FragBatch in, FragOut out,
+                                                   SegBatch sg) {
+  stitch_body<false>(db, P, in, out, sg);
+}
+__global__ __launch_bounds__(64) void k_stitch_dna_x(DbView db, ClassifyParams P, FragBatch in, FragOut out,
+                                                     SegBatch sg) {
+  stitch_body<true>(db, P, in, out, sg);
 }
 
 // Kernel variants: _dna (nucleotide, no per-hit extras: the hot path),

@@ -14,6 +14,8 @@ def load():
     u32, u64, i32, dbl = C.c_uint32, C.c_uint64, C.c_int32, C.c_double
     lib.emu_classify.argtypes = [vp, u64, u32, u32, vp, u32, u32, u32, u32, u64, u64, u64, u32, dbl, i32, u32, u32,
                                  vp, vp, vp, u32, u64, vp, C.c_int, C.c_uint, vp, vp, vp, vp] + [vp] * 9
+    lib.emu_classify_split.argtypes = [vp, u64, u32, u32, vp, u32, u32, u32, u64, u64, u64, u32, dbl, i32, u32,
+                                       vp, vp, vp, u64, vp, C.c_uint, vp, vp, vp, u32] + [vp] * 9
     lib.emu_collectives.restype = C.c_ulonglong
     return lib
 
@@ -104,3 +106,37 @@ def classify(lib, db, seq, off, paired_flags, *, conf=0.0, min_groups=0, quick=F
     outs["recs"] = recs
     outs["kmer_counts"] = kc
     return outs
+
+
+def classify_split(lib, db, seq, off, paired_flags, *, conf=0.0, min_groups=0, tab_cap=256, grid=4, hll=False,
+                   kmer_counts=False, seg_pct=100):
+    """The split path (nucleotide, persistent, exact fragment space): long
+    fragments classified by segments and stitched.  Returns (outputs, segments)."""
+    n = (len(off) - 1) // 2
+    off = np.ascontiguousarray(off, np.uint64)
+    pf = np.ascontiguousarray(paired_flags, np.uint8)
+    b = bounds(db, off, pf)
+    caps = b.copy()
+    run_off = np.zeros(n + 1, np.uint64)
+    run_off[1:] = np.cumsum(caps)
+    tot = int(run_off[-1])
+    order = np.argsort(-b.astype(np.int64), kind="stable").astype(np.uint32)
+    outs = {k: np.zeros(n, np.uint32) for k in ("call", "groups", "total", "n_runs", "status", "n_recs")}
+    runs = np.zeros(2 * max(tot, 1), np.uint32)
+    recs = np.zeros(2 * max(tot, 1), np.uint32) if hll else None
+    kc = np.zeros(len(db["parent"]), np.uint64) if kmer_counts else None
+    sb = np.frombuffer(seq if len(seq) else b"\0", np.uint8)
+    segs = lib.emu_classify_split(db["cells"].ctypes.data, db["capacity"], db["kb"], db["vb"],
+                                  db["parent"].ctypes.data, len(db["parent"]), db["k"], db["l"], db["mask"],
+                                  db["toggle"], db["min_hash"], db["revcom"], conf, min_groups, tab_cap,
+                                  sb.ctypes.data, off.ctypes.data, pf.ctypes.data, n, order.ctypes.data, grid,
+                                  run_off.ctypes.data, caps.ctypes.data, b.ctypes.data, seg_pct,
+                                  *[outs[k].ctypes.data for k in ("call", "groups", "total", "n_runs", "status",
+                                                                  "n_recs")],
+                                  runs.ctypes.data, recs.ctypes.data if recs is not None else None,
+                                  kc.ctypes.data if kc is not None else None)
+    outs["run_off"] = run_off
+    outs["runs"] = runs
+    outs["recs"] = recs
+    outs["kmer_counts"] = kc
+    return outs, segs

@@ -143,3 +143,60 @@ def test_emulated_probe_edge_tables(emu_lib, edge_tables, name):
     seq, off = k2.pack_fragments(reads)
     out = _check(emu, lib, d, seq, off, np.zeros(len(reads), np.uint8), g=0)
     assert (out["groups"] > 0).any()
+
+
+@pytest.mark.parametrize("extras", [False, True])
+def test_emulated_split_long_fragments(emu_lib, synth_db, extras):
+    """Long fragments split over waves (segments of 2048 k-mers, cut points
+    moved past ambiguous bases, mates split separately), stitched: every
+    fragment's call, counters, hitlist, HLL records and the per-taxon k-mer
+    counts equal the unsplit scan's, and calls and hitlists the oracle's."""
+    emu, lib = emu_lib
+    import kraken2_amd as k2
+    names, genomes = read_fasta(os.path.join(synth_db, "library.fna"))
+    rng = random.Random(31)
+    m1 = chimeric_reads(genomes, 10, seed=32, seg=700, length=9000)
+    m2 = chimeric_reads(genomes, 10, seed=33, seg=300, length=6000)
+    # ambiguous stretches near and at the would-be cut points, lowercase, short mates
+    for r in range(len(m1)):
+        s = list(m1[r])
+        for c in (2048 + 34 - 3, 4096 + 20, 6000):
+            for j in range(c, min(len(s), c + rng.choice([1, 5, 40]))):
+                s[j] = rng.choice("NRYn")
+        m1[r] = "".join(s).lower() if r % 3 == 0 else "".join(s)
+    m2[4] = m2[4][:20]
+    m2[5] = "N" * 5000
+    m1 += edge_reads(genomes, 6, seed=34)
+    m2 += edge_reads(genomes, 6, seed=35)
+    flags = np.array([i % 2 for i in range(len(m1))], np.uint8)
+    m2 = [b if f else "" for b, f in zip(m2, flags)]
+    seq, off = k2.pack_fragments(m1, m2)
+    db = emu.read_db(synth_db)
+    whole = emu.classify(lib, db, seq, off, flags, mode=1, hll=extras, kmer_counts=extras)
+    split, segs = emu.classify_split(lib, db, seq, off, flags, hll=extras, kmer_counts=extras)
+    assert segs > 20
+    n = len(flags)
+    for i in range(n):
+        for f in ("call", "groups", "total", "n_runs", "status", "n_recs"):
+            assert int(split[f][i]) == int(whole[f][i]), (i, f)
+        ro = int(whole["run_off"][i])
+        nr = int(whole["n_runs"][i])
+        assert np.array_equal(split["runs"][2 * ro:2 * (ro + nr)], whole["runs"][2 * ro:2 * (ro + nr)]), i
+        if extras:
+            nrec = int(whole["n_recs"][i])
+            assert np.array_equal(split["recs"][2 * ro:2 * (ro + nrec)], whole["recs"][2 * ro:2 * (ro + nrec)]), i
+    if extras:
+        assert np.array_equal(split["kmer_counts"], whole["kmer_counts"])
+    orc = Oracle()
+    h = orc.open(synth_db)
+    for pf in (0, 1):
+        idx = np.flatnonzero(flags == pf)
+        parts1 = [m1[i] for i in idx]
+        parts2 = [m2[i] for i in idx] if pf else None
+        s2, o2 = k2.pack_fragments(parts1, parts2)
+        ref = orc.classify(h, s2, o2, paired=bool(pf))
+        for j, i in enumerate(idx):
+            ro = int(split["run_off"][i])
+            runs = split["runs"][2 * ro:2 * (ro + int(split["n_runs"][i]))].reshape(-1, 2)
+            assert (int(split["call"][i]), k2.format_hitlist(runs, db["ext"])) == \
+                (int(ref["call"][j]), ref["text"][j]), i
