@@ -7,8 +7,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <string>
+#include <map>
 #include <memory>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "host/k2g_host.h"
@@ -91,25 +93,97 @@ struct DevBuf {
   }
 };
 
-// Growable pinned host buffer, also addressed by kernels (zero-copy writes).
+// First-fit allocator of byte ranges of [0, size) (not thread-safe).
+class ByteArena {
+ public:
+  explicit ByteArena(uint64_t size = 0) {
+    if (size) free_[0] = size;
+  }
+  bool alloc(uint64_t bytes, uint64_t *off) {
+    if (bytes == 0) {
+      *off = 0;
+      return true;
+    }
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second < bytes) continue;
+      *off = it->first;
+      const uint64_t rest = it->second - bytes;
+      const uint64_t at = it->first + bytes;
+      free_.erase(it);
+      if (rest) free_[at] = rest;
+      return true;
+    }
+    return false;
+  }
+  void release(uint64_t off, uint64_t bytes) {
+    if (bytes == 0) return;
+    auto it = free_.emplace(off, bytes).first;
+    auto next = std::next(it);
+    if (next != free_.end() && it->first + it->second == next->first) {
+      it->second += next->second;
+      free_.erase(next);
+    }
+    if (it != free_.begin()) {
+      auto prev = std::prev(it);
+      if (prev->first + prev->second == it->first) {
+        prev->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+
+ private:
+  std::map<uint64_t, uint64_t> free_;
+};
+
+// One pinned, device-mapped host allocation carved into buffers (thread-safe).
+// Freeing pinned memory waits for the whole device, so a request loop that
+// grows and drops result buffers takes them from here instead.
+struct HostArena {
+  char *base = nullptr;
+  uint64_t size = 0;
+  std::mutex mu;
+  ByteArena ranges;
+  void *take(uint64_t bytes) {
+    bytes = (bytes + 255) / 256 * 256;
+    std::lock_guard<std::mutex> lk(mu);
+    uint64_t off = 0;
+    return ranges.alloc(bytes, &off) ? base + off : nullptr;
+  }
+  void give(void *p, uint64_t bytes) {
+    bytes = (bytes + 255) / 256 * 256;
+    std::lock_guard<std::mutex> lk(mu);
+    ranges.release((uint64_t) ((char *) p - base), bytes);
+  }
+};
+
+// Growable pinned host buffer, also addressed by kernels (zero-copy writes);
+// from a HostArena when one is set (hipHostMalloc when it is full).
 template <class T>
 struct PinnedBuf {
   T *p = nullptr;
   size_t cap = 0;
+  HostArena *arena = nullptr;
+  bool in_arena = false;
   hipError_t reserve(size_t n) {
     if (n <= cap) return hipSuccess;
-    if (p) (void) hipHostFree(p);
-    p = nullptr;
-    cap = 0;
+    release();
     const size_t want = std::max<size_t>(n + n / 2, 1024);
+    if (arena && (p = (T *) arena->take(want * sizeof(T))) != nullptr) {
+      in_arena = true;
+      cap = want;
+      return hipSuccess;
+    }
     hipError_t e = hipHostMalloc((void **) &p, want * sizeof(T), hipHostMallocMapped);
     if (e == hipSuccess) cap = want;
     return e;
   }
   void release() {
-    if (p) (void) hipHostFree(p);
+    if (p && in_arena) arena->give(p, cap * sizeof(T));
+    else if (p) (void) hipHostFree(p);
     p = nullptr;
     cap = 0;
+    in_arena = false;
   }
 };
 
@@ -124,7 +198,23 @@ struct HostResults {
   // per-(block, taxon) HLL sets (want_hll == 2); sk_words: groups, values
   PinnedBuf<uint32_t> sk_block, sk_taxon, sk_dense, sk_vals;
   PinnedBuf<uint64_t> sk_n, sk_off, sk_words;
+  uint64_t served = 0;  // largest element bound of a call these buffers held (request server)
   HostResults() = default;
+  explicit HostResults(HostArena *a) {
+    for (auto *x : {&call, &groups, &total, &lookups, &runs, &recs, &words, &sk_block, &sk_taxon, &sk_dense,
+                    &sk_vals})
+      x->arena = a;
+    for (auto *x : {&run_off, &rec_off, &kmer_counts, &lines, &sk_n, &sk_off, &sk_words}) x->arena = a;
+  }
+  // pinned bytes held
+  uint64_t bytes() const {
+    uint64_t b = 0;
+    for (auto *x : {&call, &groups, &total, &lookups, &runs, &recs, &words, &sk_block, &sk_taxon, &sk_dense,
+                    &sk_vals})
+      b += x->cap * 4;
+    for (auto *x : {&run_off, &rec_off, &kmer_counts, &lines, &sk_n, &sk_off, &sk_words}) b += x->cap * 8;
+    return b;
+  }
   HostResults(const HostResults &) = delete;
   HostResults &operator=(const HostResults &) = delete;
   ~HostResults() {

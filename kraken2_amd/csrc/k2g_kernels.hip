@@ -992,17 +992,19 @@ __global__ __launch_bounds__(256) void k_seg_count(const uint64_t *off, const ui
   cnt[i] = c;
 }
 
-// Step 3: one thread per split fragment lays out its segments: per mate, cut
-// points from the last to the first, each moved forward past ambiguous bases
-// (never beyond the next cut), and each segment's reserved run space.
+// Step 3: one wave per split fragment lays out its segments.  Per mate, the
+// lanes move the evenly spaced cut points forward past ambiguous bases (the k
+// bases before a cut must hold none), then lane 0 walks them from the last to
+// the first, keeping each at or below the next, and writes each segment's
+// range and reserved run space.
 __global__ __launch_bounds__(64) void k_seg_plan(const char *seq, const uint64_t *off, const uint8_t *paired,
                                                  uint32_t paired_all, uint32_t k, uint32_t seg, uint32_t pct,
                                                  const uint32_t *list, const uint64_t *seg_start, uint32_t *count,
                                                  uint64_t seg_total_at, SegBatch sg) {
   const uint32_t nf = count[1];
-  const uint64_t t = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
-  if (t == 0) count[0] = (uint32_t) seg_start[seg_total_at];
-  for (uint64_t i = t; i < nf; i += (uint64_t) gridDim.x * blockDim.x) {
+  const uint32_t lane = threadIdx.x;
+  if (blockIdx.x == 0 && lane == 0) count[0] = (uint32_t) seg_start[seg_total_at];
+  for (uint32_t i = blockIdx.x; i < nf; i += gridDim.x) {
     const uint32_t f = list[i];
     const bool pr = paired ? paired[f] != 0 : paired_all != 0;
     uint64_t s = seg_start[f];
@@ -1010,13 +1012,10 @@ __global__ __launch_bounds__(64) void k_seg_plan(const char *seq, const uint64_t
       const char *m = seq + off[2 * f + mate];
       const uint32_t L = (uint32_t) (off[2 * f + mate + 1] - off[2 * f + mate]);
       const uint32_t e = mate_kmers(L, k), segs = mate_segments(e, seg);
-      uint32_t next = e ? L : 0u;  // k-mers end at positions [k - 1, L)
-      for (uint32_t j = segs; j-- > 0;) {
+      for (uint32_t j = lane; j < segs; j += blockDim.x) {
         uint32_t a = e ? (k - 1) + (uint32_t) (((uint64_t) j * e) / segs) : 0u;
-        if (j > 0) {
-          // the k bases [a - k, a) must be free of ambiguous bases: past the
-          // last ambiguous one among them, and check again (a >= k here)
-          while (a < next) {
+        if (j > 0) {  // a >= k: past the last ambiguous base among [a - k, a), until none
+          while (a < L) {
             uint32_t amb = 0xFFFFFFFFu;
             for (uint32_t r = a; r > a - k; r--)
               if (dna_code((uint8_t) m[r - 1]) == 0xFFu) {
@@ -1026,16 +1025,25 @@ __global__ __launch_bounds__(64) void k_seg_plan(const char *seq, const uint64_t
             if (amb == 0xFFFFFFFFu) break;
             a = amb + k + 1;
           }
-          if (a > next) a = next;
+          if (a > L) a = L;
         }
-        sg.frag[s + j] = f;
-        sg.mate[s + j] = mate;
         sg.a[s + j] = a;
-        sg.b[s + j] = next;
-        const uint64_t el = next > a ? next - a : 0u;
-        sg.cap[s + j] = run_space(el, pct);
-        next = a;
       }
+      __syncthreads();
+      if (lane == 0) {
+        uint32_t next = e ? L : 0u;  // k-mers end at positions [k - 1, L)
+        for (uint32_t j = segs; j-- > 0;) {
+          uint32_t a = sg.a[s + j];
+          if (a > next) a = next;
+          sg.frag[s + j] = f;
+          sg.mate[s + j] = mate;
+          sg.a[s + j] = a;
+          sg.b[s + j] = next;
+          sg.cap[s + j] = run_space(next > a ? next - a : 0u, pct);
+          next = a;
+        }
+      }
+      __syncthreads();
       s += segs;
     }
   }

@@ -31,48 +31,11 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
-// First-fit allocator of byte ranges of [0, size).
-class Arena {
- public:
-  explicit Arena(uint64_t size) {
-    if (size) free_[0] = size;
-  }
-  bool alloc(uint64_t bytes, uint64_t *off) {
-    if (bytes == 0) {
-      *off = 0;
-      return true;
-    }
-    for (auto it = free_.begin(); it != free_.end(); ++it) {
-      if (it->second < bytes) continue;
-      *off = it->first;
-      const uint64_t rest = it->second - bytes;
-      const uint64_t at = it->first + bytes;
-      free_.erase(it);
-      if (rest) free_[at] = rest;
-      return true;
-    }
-    return false;
-  }
-  void release(uint64_t off, uint64_t bytes) {
-    if (bytes == 0) return;
-    auto it = free_.emplace(off, bytes).first;
-    auto next = std::next(it);
-    if (next != free_.end() && it->first + it->second == next->first) {
-      it->second += next->second;
-      free_.erase(next);
-    }
-    if (it != free_.begin()) {
-      auto prev = std::prev(it);
-      if (prev->first + prev->second == it->first) {
-        prev->second += it->second;
-        free_.erase(it);
-      }
-    }
-  }
+using Arena = k2g::ByteArena;
 
- private:
-  std::map<uint64_t, uint64_t> free_;
-};
+// Host result sets the server keeps for reuse (at most this many, holding at
+// most half its result memory).
+constexpr size_t kPoolSets = 64;
 
 struct Request {
   uint64_t ticket = 0;
@@ -110,6 +73,9 @@ struct k2g_server {
   // host result sets of released requests, reused by the slots (their
   // vectors and pinned buffers keep their capacity)
   std::vector<std::unique_ptr<k2g::HostResults>> pool;
+  // pinned host memory the result sets are carved from (k2g_server_destroy
+  // returns every set to it before freeing it)
+  k2g::HostArena host;
 };
 
 namespace {
@@ -149,6 +115,25 @@ void worker_main(k2g_server *s) {
       s->running++;
       s->max_running = std::max(s->max_running, s->running);
     }
+    {
+      // the host result set that fits this request best, the slot's own or a
+      // pooled one: the smallest that held a call of at least this request's
+      // element bound, else the largest (pinned buffers grow by reallocation,
+      // and freeing pinned memory waits for the whole device)
+      std::lock_guard<std::mutex> lk(s->mu);
+      const uint64_t need = q->plan.space;
+      auto better = [need](uint64_t c, uint64_t than) {
+        return (c >= need) != (than >= need) ? c >= need : (c >= need ? c < than : c > than);
+      };
+      size_t pick = s->pool.size();
+      uint64_t got = b->host->served;
+      for (size_t i = 0; i < s->pool.size(); i++)
+        if (better(s->pool[i]->served, got)) {
+          pick = i;
+          got = s->pool[i]->served;
+        }
+      if (pick < s->pool.size()) std::swap(b->host, s->pool[pick]);
+    }
     region.base = s->d_base + off;
     region.size = q->bytes;
     region.used = 0;
@@ -157,6 +142,7 @@ void worker_main(k2g_server *s) {
     int r = k2g_classify(b, &q->params, &q->in, &q->out);
     b->plan.valid = false;
     if (r == K2G_OK) {
+      b->host->served = std::max(b->host->served, q->plan.space);
       q->res = std::move(b->host);  // q->out points into these buffers
     } else {
       q->rc = r;
@@ -170,7 +156,7 @@ void worker_main(k2g_server *s) {
           b->host = std::move(s->pool.back());
           s->pool.pop_back();
         } else {
-          b->host = std::make_unique<k2g::HostResults>();
+          b->host = std::make_unique<k2g::HostResults>(&s->host);
         }
       }
       s->arena.release(off, q->bytes);
@@ -208,6 +194,15 @@ extern "C" int k2g_server_create(k2g_db *db, const k2g_server_params *p, k2g_ser
     return fail(K2G_EOS, std::string("server budget allocation: ") + hipGetErrorString(e));
   }
   s->arena = Arena(s->budget);
+  // result sets: pinned host memory of half the device budget, within [256 MiB, 32 GiB]
+  s->host.size = std::min<uint64_t>(std::max<uint64_t>(s->budget / 2, 256ull << 20), 32ull << 30);
+  e = hipHostMalloc((void **) &s->host.base, s->host.size, hipHostMallocMapped);
+  if (e != hipSuccess) {
+    (void) hipFree(s->d_base);
+    delete s;
+    return fail(K2G_EOS, std::string("server result memory: ") + hipGetErrorString(e));
+  }
+  s->host.ranges = k2g::ByteArena(s->host.size);
   for (int i = 0; i < p->slots; i++) s->workers.emplace_back(worker_main, s);
   *out = s;
   return K2G_OK;
@@ -271,15 +266,25 @@ extern "C" int k2g_server_wait(k2g_server *s, uint64_t ticket, k2g_results *out,
 
 extern "C" int k2g_server_release(k2g_server *s, uint64_t ticket) {
   if (!s) return fail(K2G_EUSAGE, "null server");
-  std::lock_guard<std::mutex> lk(s->mu);
-  auto it = s->requests.find(ticket);
-  if (it == s->requests.end()) return fail(K2G_EUSAGE, "unknown ticket");
-  if (it->second->state != 2) return fail(K2G_EUSAGE, "request still in progress");
-  // keep a few result sets for the slots; free the rest
-  if (it->second->res && s->pool.size() < s->workers.size() + 2)
-    s->pool.push_back(std::move(it->second->res));
-  delete it->second;
-  s->requests.erase(it);
+  std::unique_ptr<k2g::HostResults> drop;  // freed after the lock is released
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->requests.find(ticket);
+    if (it == s->requests.end()) return fail(K2G_EUSAGE, "unknown ticket");
+    if (it->second->state != 2) return fail(K2G_EUSAGE, "request still in progress");
+    // result sets are kept for reuse up to kPoolSets sets and half the result
+    // memory; beyond that they are freed
+    if (it->second->res) {
+      uint64_t held = 0;
+      for (const auto &x : s->pool) held += x->bytes();
+      if (s->pool.size() < kPoolSets && held + it->second->res->bytes() <= s->host.size / 2)
+        s->pool.push_back(std::move(it->second->res));
+      else
+        drop = std::move(it->second->res);
+    }
+    delete it->second;
+    s->requests.erase(it);
+  }
   return K2G_OK;
 }
 
@@ -302,7 +307,9 @@ extern "C" int k2g_server_destroy(k2g_server *s) {
   s->cv_work.notify_all();
   for (auto &t : s->workers) t.join();
   for (auto &kv : s->requests) delete kv.second;
+  s->pool.clear();  // every result set is back in the arena before it is freed
   if (s->d_base) (void) hipFree(s->d_base);
+  if (s->host.base) (void) hipHostFree(s->host.base);
   delete s;
   return K2G_OK;
 }

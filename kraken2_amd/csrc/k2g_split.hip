@@ -115,7 +115,7 @@ int split_launch(k2g_batch *b, const k2g_classify_params *p, const ClassifyParam
                 b->sp_cap.p,     b->sp_runs_n.p, b->sp_nrec.p,   b->sp_groups.p, b->sp_status.p,
                 b->sp_runs.p,    p->want_hll ? b->sp_recs.p : nullptr,
                 b->sp_count.p,   b->sp_list.p,   b->sp_cnt.p,    b->sp_start.p};
-    const unsigned gp = (unsigned) std::min<uint64_t>((n + 63) / 64, 4096);
+    const unsigned gp = (unsigned) std::min<uint64_t>(split_cnt, 4096);
     hipLaunchKernelGGL(k_seg_plan, dim3(gp), dim3(64), 0, s, in.seq, in.off, in.paired, in.paired_all, v.k,
                        (uint32_t) kSegElements, 100u, (const uint32_t *) b->sp_list.p,
                        (const uint64_t *) b->sp_start.p, b->sp_count.p, n, sg);

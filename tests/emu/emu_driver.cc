@@ -166,7 +166,7 @@ int emu_classify_split(const uint32_t *cells, uint64_t capacity, uint32_t kb, ui
   SegBatch sg{frag.data(), mate.data(), a.data(), b.data(), soff.data(), cap.data(), nr.data(), nrec.data(),
               grp.data(),  st.data(),   nullptr,  nullptr,  count.data(), list.data(), cnt.data(), start.data()};
   // 3-4. cut points and run spaces
-  run_grid((unsigned) std::max<uint64_t>(1, (count[1] + 63) / 64), [&] {
+  run_grid((unsigned) std::max<uint32_t>(1, count[1]), [&] {
     k_seg_plan(seq, off, paired, 0u, k, kSegElementsEmu, seg_pct, list.data(), start.data(), count.data(), n, sg);
   });
   for (uint64_t s = 0; s < segs; s++) soff[s + 1] = soff[s] + cap[s];
