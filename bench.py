@@ -419,17 +419,21 @@ def run_serving(k2, db, tmp, pools, a, work):
     reads = sum(r["size"] for r in reqs)
     bases = sum(r["bases"] for r in reqs)
     legs, digests = [], []
-    # one server per slot count, each warmed by one untimed pass over the
-    # request set (a resident service has its result memory laid out)
+    # one server per slot count at a time, warmed by one untimed pass over
+    # the request set (a resident service has its result memory laid out)
     servers = {}
-    for slots in (1, a.serve_slots):
+
+    def server(slots):
         if slots not in servers:
+            for old in list(servers):
+                servers.pop(old).close()
             servers[slots] = k2.Server(db, budget, slots)
             serve_at_rate(servers[slots], reqs, 0, work, rng, a.serve_hll, slots)
             log(f"# serving: {slots}-slot server warmed")
+        return servers[slots]
 
     def leg(name, rate, slots):
-        res, wall, st = serve_at_rate(servers[slots], reqs, rate, work, rng, a.serve_hll, slots)
+        res, wall, st = serve_at_rate(server(slots), reqs, rate, work, rng, a.serve_hll, slots)
         lat = np.array([t["total_ms"] for _, t in res])
         row = {"leg": name, "slots": slots, "offered_req_per_s": round(rate, 2) if rate else None,
                "completed_req_per_s": round(len(reqs) / wall, 2), "reads_per_s": round(reads / wall, 1),

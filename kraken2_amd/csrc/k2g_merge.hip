@@ -266,8 +266,11 @@ extern "C" int k2g_merge_text(k2g_merger *g, const k2g_merge_params *p, const k2
     HIP_TRY(g->scan_tmp.reserve(scan_bytes + 1));
     HIP_TRY(hipcub::DeviceScan::ExclusiveSum(g->scan_tmp.p, scan_bytes, g->n_tokens.p, off.p, n + 1, s));
     HIP_TRY(hipMemcpyAsync(&tot[side], off.p + n, 8, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
   }
+  if (!a->off || !b->off) HIP_TRY(hipStreamSynchronize(s));  // one wait for the device totals
+  if (tot[0] > (uint64_t) UINT32_MAX || tot[1] > (uint64_t) UINT32_MAX)
+    return fail(K2G_EUSAGE, std::string("hit lists ") + (tot[0] > (uint64_t) UINT32_MAX ? "a" : "b") +
+                                " hold 2^32 or more items");
   const uint64_t na = tot[0], nb = tot[1];
   HIP_TRY(g->a_items.reserve(2 * na + 2));
   HIP_TRY(g->b_items.reserve(2 * nb + 2));
@@ -275,15 +278,21 @@ extern "C" int k2g_merge_text(k2g_merger *g, const k2g_merge_params *p, const k2
   HIP_TRY(hipMemsetAsync(g->bad.p, 0, 4, s));
   if (n) {
     hipLaunchKernelGGL(k_merge_parse, dim3(blocks), dim3(256), 0, s, g->a_text.p, g->a_field.p, g->a_len.p,
-                       g->a_off.p, n, g->a_items.p, g->bad.p);
+                       g->a_off.p, n, g->a_items.p, g->bad.p, a->off != nullptr);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_merge_parse, dim3(blocks), dim3(256), 0, s, g->b_text.p, g->b_field.p, g->b_len.p,
-                       g->b_off.p, n, g->b_items.p, g->bad.p);
+                       g->b_off.p, n, g->b_items.p, g->bad.p, b->off != nullptr);
     HIP_TRY(hipGetLastError());
+    if (!a->off || !b->off) {  // the per-read limit the host checks when both offsets are given
+      hipLaunchKernelGGL(k_merge_limits, dim3(blocks), dim3(256), 0, s, (const uint64_t *) g->a_off.p,
+                         (const uint64_t *) g->b_off.p, n, g->bad.p);
+      HIP_TRY(hipGetLastError());
+    }
   }
   uint32_t bad = 0;
   HIP_TRY(hipMemcpyAsync(&bad, g->bad.p, 4, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
+  if (bad & 2u) return fail(K2G_EUSAGE, "a read's hit lists hold 2^32 or more items");
   if (bad) return fail(K2G_EUSAGE, "item offsets do not match the ':' count of the hit-list fields");
   return run_merge(g, p, n, na + nb, out);
 }

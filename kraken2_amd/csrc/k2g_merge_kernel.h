@@ -338,21 +338,36 @@ k_merge_count(const char *__restrict__ text, const uint64_t *__restrict__ field,
 }
 
 // One thread per read: field[r], len[r] locate read r's hit-list field in
-// text; off (n+1) gives its item slots.
+// text; off (n+1) gives its item slots.  check: the offsets are the caller's,
+// so each read's slot count must be its field's ':' count (offsets counted on
+// the device are those counts already).
 __global__ void __launch_bounds__(256)
 k_merge_parse(const char *__restrict__ text, const uint64_t *__restrict__ field, const uint32_t *__restrict__ len,
-              const uint64_t *__restrict__ off, uint64_t n, int32_t *__restrict__ items, uint32_t *__restrict__ bad) {
+              const uint64_t *__restrict__ off, uint64_t n, int32_t *__restrict__ items, uint32_t *__restrict__ bad,
+              bool check) {
   const uint64_t r = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
   const char *f = text + field[r];
   const uint64_t slots = off[r + 1] - off[r];
-  uint64_t colons = 0;
-  for (uint32_t i = 0; i < len[r]; i++) colons += f[i] == ':';
-  if (colons != slots) {  // the caller's slot counts must be the fields' ':' counts
-    *bad = 1;
-    return;
+  if (check) {
+    uint64_t colons = 0;
+    for (uint32_t i = 0; i < len[r]; i++) colons += f[i] == ':';
+    if (colons != slots) {
+      atomicOr(bad, 1u);
+      return;
+    }
   }
   parse_field(f, len[r], items + 2 * off[r], slots);
+}
+
+// Sets bad bit 1 when a read's two hit lists hold 2^32 or more items together
+// (k_merge counts a read's merged items in 32 bits).
+__global__ void __launch_bounds__(256)
+k_merge_limits(const uint64_t *__restrict__ a_off, const uint64_t *__restrict__ b_off, uint64_t n,
+               uint32_t *__restrict__ bad) {
+  const uint64_t r = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  if (a_off[r + 1] - a_off[r] + b_off[r + 1] - b_off[r] > 0xFFFFFFFFull) atomicOr(bad, 2u);
 }
 
 // Prints read r's pairs ("taxid:count", a space after each unless joined)

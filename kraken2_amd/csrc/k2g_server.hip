@@ -194,14 +194,18 @@ extern "C" int k2g_server_create(k2g_db *db, const k2g_server_params *p, k2g_ser
     return fail(K2G_EOS, std::string("server budget allocation: ") + hipGetErrorString(e));
   }
   s->arena = Arena(s->budget);
-  // result sets: pinned host memory of half the device budget, within [256 MiB, 32 GiB]
-  s->host.size = std::min<uint64_t>(std::max<uint64_t>(s->budget / 2, 256ull << 20), 32ull << 30);
-  e = hipHostMalloc((void **) &s->host.base, s->host.size, hipHostMallocMapped);
-  if (e != hipSuccess) {
-    (void) hipFree(s->d_base);
-    delete s;
-    return fail(K2G_EOS, std::string("server result memory: ") + hipGetErrorString(e));
+  // result sets: pinned host memory of a quarter of the device budget,
+  // within [256 MiB, 16 GiB], halved while the host cannot pin that much
+  // (without any, result buffers are pinned one by one)
+  for (uint64_t want = std::min<uint64_t>(std::max<uint64_t>(s->budget / 4, 256ull << 20), 16ull << 30);
+       want >= (256ull << 20); want /= 2) {
+    if (hipHostMalloc((void **) &s->host.base, want, hipHostMallocMapped) == hipSuccess) {
+      s->host.size = want;
+      break;
+    }
+    s->host.base = nullptr;
   }
+  (void) hipGetLastError();  // a refused size is not an error of the next call
   s->host.ranges = k2g::ByteArena(s->host.size);
   for (int i = 0; i < p->slots; i++) s->workers.emplace_back(worker_main, s);
   *out = s;

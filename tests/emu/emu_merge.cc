@@ -100,8 +100,13 @@ int k2g_merge_text(k2g_merger *g, const k2g_merge_params *p, const k2g_hit_list_
   for (uint64_t r = 0; r < n; r++) {
     emu::blockIdx_.x = (unsigned) r;
     emu::threadIdx_.x = 0;
-    k2g::k_merge_parse(a->text, a->field, a->len, oa.data(), n, ia.data(), &bad);
-    k2g::k_merge_parse(b->text, b->field, b->len, ob.data(), n, ib.data(), &bad);
+    k2g::k_merge_parse(a->text, a->field, a->len, oa.data(), n, ia.data(), &bad, a->off != nullptr);
+    k2g::k_merge_parse(b->text, b->field, b->len, ob.data(), n, ib.data(), &bad, b->off != nullptr);
+    if (!a->off || !b->off) k2g::k_merge_limits(oa.data(), ob.data(), n, &bad);
+  }
+  if (bad & 2u) {
+    g_error = "a read's hit lists hold 2^32 or more items";
+    return K2G_EUSAGE;
   }
   if (bad) {
     g_error = "item offsets do not match the ':' count of the hit-list fields";
