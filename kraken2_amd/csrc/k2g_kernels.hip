@@ -229,7 +229,8 @@ constexpr uint64_t kOddBits = 0xAAAAAAAAAAAAAAAAULL;
 constexpr int kQueue = 192;       // queued lookups per probe batch (>= 64 + 64)
 constexpr int kRingGuard = 8;     // windows of up to 9 l-mers read without wrapping
 constexpr int kBatchChunks = 8;   // chunks (and mate markers) per probe batch
-constexpr uint32_t kMateItem = 0xFFFFFFFFu;  // cqb value of a mate-marker item
+constexpr uint32_t kMateItem = 0xFFFFFFFFu;   // cqb value of a mate-marker item
+constexpr uint32_t kFrameItem = 0xFFFFFFFEu;  // cqb value of a frame-marker item
 
 // KEEP_KEYS: the minimizers stay readable after their probes (HLL records
 // need them); otherwise each value overwrites its own key's low half, which
@@ -357,8 +358,11 @@ struct ProbeState {
   }
 };
 
-// Scans one stream (a DNA mate, or one reading frame of a mate) and appends
-// its k-mer elements to the fragment's runs.  DNA: a nucleotide database
+// Scans a fragment's streams in order -- its mates (nucleotide), or the six
+// reading frames of each mate (translated), with a marker element between
+// streams (src/classify.cc:797-838) -- and appends their k-mer elements to
+// the fragment's runs.  Every stream starts a fresh scanner; the streams
+// share probe batches, so a short read's frames fill one batch's lanes.  DNA: a nucleotide database
 // (else translated search); EXTRAS: per-hit HLL records, per-taxon counts or
 // probe statistics may be requested (their code is compiled out otherwise);
 // W: the window's l-mer count k-l+1 when fixed at compile time (<= 9), else 0.
@@ -377,17 +381,21 @@ struct ProbeState {
 //      run-length encodes the elements and counts hits.
 template <bool DNA, bool EXTRAS, int W, class Table>
 __device__ void scan_stream(const DbView &db, const ClassifyParams &P, const FragOut &out,
-                            uint64_t base, uint64_t cap, const char *m, uint64_t L, int fr,
-                            const char *m2, uint64_t L2, RunState &st, Table &tab,
-                            WaveLds<EXTRAS> &lds, int *bad, int lane, uint32_t first_emit = 0) {
+                            uint64_t base, uint64_t cap, const char *m1, uint64_t L1, const char *m2,
+                            uint64_t L2, RunState &st, Table &tab, WaveLds<EXTRAS> &lds, int *bad, int lane,
+                            uint32_t first_emit = 0) {
   static_assert(W >= 0 && W <= kRingGuard + 1, "window too wide for the ring guard");
   constexpr bool dna = DNA;
+  constexpr uint32_t F = dna ? 1u : 6u;  // streams per mate
+  // streams: (mate, frame) = (si / F, si % F); m2 null: one mate
+  const uint32_t n_streams = (m2 ? 2u : 1u) * F;
+  uint32_t si = 0;
+  const char *m = m1;
+  uint64_t L = L1;
+  int fr = 0;
   // positions within a stream are 32-bit: the library rejects reads of 2^31
   // or more bases before launching
   uint32_t n = (uint32_t) (dna ? L : frame_len(L, fr));
-  // m2 (nucleotide pairs): mate 2 follows in the same probe batches, after a
-  // mate-marker item, so a pair's lookups share their probe rounds
-  bool second = m2 == nullptr;
   const uint32_t k = db.k, l = db.l;
   const uint32_t w = k - l + 1;  // l-mers per k-mer window
   const uint64_t lbits = (1ULL << l) - 1ULL;
@@ -412,20 +420,23 @@ __device__ void scan_stream(const DbView &db, const ClassifyParams &P, const Fra
     const uint32_t depth = P.batch_chunks - 1u < (uint32_t) kBatchChunks ? P.batch_chunks : (uint32_t) kBatchChunks;
     while (g < depth && qn + kWave <= (uint32_t) kQueue) {
       if (c >= n) {  // this stream is done
-        if (second) {
+        if (si + 1 == n_streams) {
           more = false;
           break;
         }
-        // the mate marker, then mate 2 with a fresh scanner state
-        if (lane == 0) lds.cqb[g] = kMateItem;
+        // the marker, then the next stream with a fresh scanner state
+        si++;
+        if (lane == 0) lds.cqb[g] = si % F == 0 ? kMateItem : kFrameItem;
         g++;
-        second = true;
-        m = m2;
-        n = (uint32_t) L2;
+        m = si >= F ? m2 : m1;
+        L = si >= F ? L2 : L1;
+        fr = (int) (si % F);
+        n = (uint32_t) (dna ? L : frame_len(L, fr));
         c = 0;
         last_amb = -1;
         last_min = ~0ULL;
         pBEh = pLEh = 0;
+        pp0 = pp1 = pp2 = pp3 = 0;
         continue;
       }
       const uint32_t p = c + (uint32_t) lane;
@@ -603,8 +614,8 @@ __device__ void scan_stream(const DbView &db, const ClassifyParams &P, const Fra
       const uint64_t LK = uniform64(lds.cmask[gi][1]);
       uint64_t E = uniform64(lds.cmask[gi][2]);
       const uint32_t qb = uniform32(lds.cqb[gi]);
-      if (qb == kMateItem) {  // mate 2 starts: its own latest-LOOKUP value
-        push_marker(out, base, cap, st, kCodeMate, tab, bad, lane);
+      if (qb == kMateItem || qb == kFrameItem) {  // a new stream: its own latest-LOOKUP value
+        push_marker(out, base, cap, st, qb == kMateItem ? kCodeMate : kCodeFrame, tab, bad, lane);
         last_taxon = 0;
         continue;
       }
@@ -736,7 +747,6 @@ __device__ void classify_fragment(const DbView &db, const ClassifyParams &P, con
                  o2 = uniform64(in.off[2 * f + 2]);
   const bool paired = uniform32(in.paired ? in.paired[f] : in.paired_all) != 0;
   constexpr bool tx = !DNA;
-  const int nframes = tx ? 6 : 1;
   const uint64_t base = uniform64(out.run_off[f]);
   const uint64_t cap = uniform64(out.run_cap[f]);
   tab.clear(lane);
@@ -744,28 +754,15 @@ __device__ void classify_fragment(const DbView &db, const ClassifyParams &P, con
   __syncthreads();
 
   RunState st;
-  if (!tx) {  // both mates in one scan (the mate marker between them)
-    const char *m2 = paired ? in.seq + o1 : nullptr;
-    if (db.k - db.l == 4 && db.revcom_version != 0)  // the default window (k=35, l=31)
-      scan_stream<DNA, EXTRAS, 5>(db, P, out, base, cap, in.seq + o0, o1 - o0, 0, m2, o2 - o1, st,
-                                  tab, lds, bad, lane);
-    else
-      scan_stream<DNA, EXTRAS, 0>(db, P, out, base, cap, in.seq + o0, o1 - o0, 0, m2, o2 - o1, st,
-                                  tab, lds, bad, lane);
-  } else {
-    for (int mate = 0; mate < (paired ? 2 : 1) && !st.done; mate++) {
-      const char *m = in.seq + (mate ? o1 : o0);
-      const uint64_t L = mate ? o2 - o1 : o1 - o0;
-      for (int fr = 0; fr < nframes; fr++) {
-        scan_stream<DNA, EXTRAS, 0>(db, P, out, base, cap, m, L, fr, nullptr, 0, st, tab, lds, bad,
-                                    lane);
-        if (st.done) break;
-        if (fr != 5) push_marker(out, base, cap, st, kCodeFrame, tab, bad, lane);
-      }
-      if (st.done) break;
-      if (paired && mate == 0) push_marker(out, base, cap, st, kCodeMate, tab, bad, lane);
-    }
-  }
+  // both mates (and, translated, all six frames of each) in one scan, the
+  // markers between them
+  const char *m2 = paired ? in.seq + o1 : nullptr;
+  if (!tx && db.k - db.l == 4 && db.revcom_version != 0)  // the default window (k=35, l=31)
+    scan_stream<DNA, EXTRAS, 5>(db, P, out, base, cap, in.seq + o0, o1 - o0, m2, o2 - o1, st, tab, lds, bad,
+                                lane);
+  else
+    scan_stream<DNA, EXTRAS, 0>(db, P, out, base, cap, in.seq + o0, o1 - o0, m2, o2 - o1, st, tab, lds, bad,
+                                lane);
   if (st.run_count > 0) {
     if (lane == 0) close_run(out.runs + 2 * base, cap, st.n_runs, st.run_code, st.run_count, tab, bad);
     st.n_runs++;
@@ -1078,10 +1075,10 @@ __device__ __forceinline__ void segment_body(const DbView &db, const ClassifyPar
       RunState st;
       NullTable nt;
       if (db.k - db.l == 4 && db.revcom_version != 0)
-        scan_stream<true, EXTRAS, 5>(db, P, so, base, cap, m + s0, b - s0, 0, nullptr, 0, st, nt, lds, &lds.bad,
+        scan_stream<true, EXTRAS, 5>(db, P, so, base, cap, m + s0, b - s0, nullptr, 0, st, nt, lds, &lds.bad,
                                      lane, first_emit);
       else
-        scan_stream<true, EXTRAS, 0>(db, P, so, base, cap, m + s0, b - s0, 0, nullptr, 0, st, nt, lds, &lds.bad,
+        scan_stream<true, EXTRAS, 0>(db, P, so, base, cap, m + s0, b - s0, nullptr, 0, st, nt, lds, &lds.bad,
                                      lane, first_emit);
       if (st.run_count > 0) {
         if (lane == 0) close_run(so.runs + 2 * base, cap, st.n_runs, st.run_code, st.run_count, nt, &lds.bad);
