@@ -180,7 +180,7 @@ class Runner:
         self.batch.close()
 
 
-def run_graph_pipeline(k2, db, runner, range_sets, off, batched_digests, a, work):
+def run_graph_pipeline(k2, db, runner, range_sets, off, paired, hip, batched_digests, a, work):
     """The persistent pass as graph replays (k2g_pipeline): autotuned on an
     even sample of the read set, then, for each set of fragment ranges, W
     warmup and K timed steps of one replay per range, beside the direct
@@ -245,7 +245,65 @@ def run_graph_pipeline(k2, db, runner, range_sets, off, batched_digests, a, work
                                     "kernel_ms_per_step": round(d_ms, 3)},
                          "graph_vs_direct": round(d_wall / g_wall, 3), "digests_equal_batched": equal}
     out["digests_equal_batched"] = all_equal if batched_digests is not None else None
+    out["host_batches"] = run_host_pipeline(k2, db, runner, range_sets["frontend_batches"][:a.host_batches], off,
+                                            paired, hip, best, a, work)
     return out
+
+
+def run_host_pipeline(k2, db, runner, batches, off, paired, hip, tuning, a, work):
+    """Upload and writeback included: front-end batches written into the host
+    pipeline's pinned staging (as the classify front-end's reader writes its
+    batches), each run by one graph replay that reads the batch over the bus
+    and writes calls, counters and hitlists into pinned host memory.  The
+    timed region is the run of each batch (k2g_pipeline_run); filling the
+    staging is the reader's work and is not in it.  Every batch's results are
+    checked against k2g_classify on the same fragments."""
+    kw = dict(confidence=work["confidence"], min_hit_groups=work["min_hit_groups"])
+    cap_n = max(e - s for s, e in batches)
+    cap_b = max(int(off[2 * e] - off[2 * s]) for s, e in batches)
+    pl = k2.Pipeline(db, cap_n, cap_b, host_io=True, tuning=tuning, **kw)
+    lib = k2.load_library()
+    st_seq, st_off, st_pf = pl.staging(cap_n, cap_b)
+    host = [host_range(hip, runner.d_seq, off, paired, s, e) for s, e in batches]
+    def run(j):
+        sq, so, pf = host[j]
+        n = len(pf)
+        st_seq[:len(sq)] = np.frombuffer(sq, np.uint8)
+        st_off[:2 * n + 1] = so
+        st_pf[:n] = pf
+        fr = k2._lib.Fragments(st_seq.ctypes.data, st_off.ctypes.data, st_pf.ctypes.data, n)
+        res = k2._lib.Results()
+        t = time.perf_counter()
+        rc = lib.k2g_pipeline_run(pl._h, C.byref(fr), C.byref(res))
+        dt = time.perf_counter() - t
+        if rc:
+            raise k2.K2GError(rc, lib.k2g_last_error().decode())
+        return dt, res
+
+    for j in range(min(len(host), a.warmup)):
+        run(j)
+    total = 0.0
+    for step in range(a.steps):
+        total += sum(run(j)[0] for j in range(len(host)))
+        log(f"# host pipeline: step {step + 1}/{a.steps}")
+    wall = total / a.steps
+    equal = True
+    for j, (sq, so, pf) in enumerate(host):
+        _, res = run(j)
+        h = C.c_uint64()
+        lib.k2g_results_checksum(C.byref(res), C.byref(h))
+        equal = equal and h.value == runner.batch.classify_checksum(sq, so, paired_flags=pf, mode=1, **kw)
+    pl.close()
+    n = sum(e - s for s, e in batches)
+    bases = sum(len(x[0]) for x in host)
+    log(f"# host pipeline: {len(batches)} front-end batches, {n} fragments, {bases / 1e9:.2f} Gbp in {wall * 1e3:.1f} ms "
+        f"per pass ({n / wall / 1e6:.2f} M reads/s, {bases / wall / 1e9:.1f} GB/s up); results equal k2g_classify's: "
+        f"{equal}")
+    return {"batches": len(batches), "fragments": n, "bases": bases, "ms_per_pass": round(wall * 1e3, 3),
+            "reads_per_s": round(n / wall, 1), "upload_gb_per_s": round(bases / wall / 1e9, 2),
+            "results_equal_classify": equal,
+            "timed": "k2g_pipeline_run per batch: upload over the bus, the classify chain, calls, counters and "
+                     "hitlists written into pinned host memory; the batches lie in the pipeline's pinned staging"}
 
 
 def run_host_workload(a, work, k2):
@@ -820,6 +878,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
     ap.add_argument("--no-pipeline", action="store_true", help="skip the graph-pipeline leg")
     ap.add_argument("--tune-sample", type=int, default=500_000, help="fragments the autotuner times on")
+    ap.add_argument("--host-batches", type=int, default=8, help="front-end batches of the host-pipeline leg")
     ap.add_argument("--serve-requests", type=int, default=300, help="requests in the serving set")
     ap.add_argument("--serve-oracle", type=int, default=24, help="served requests checked against the oracle")
     ap.add_argument("--serve-oracle-max", type=int, default=50000,
@@ -947,7 +1006,8 @@ def main():
             (np.array_equal(digests["plain"], digests["batched"]) if "plain" in digests else True)
 
     graph = None if a.no_pipeline else run_graph_pipeline(
-        k2, db, runner, {"chunks": chunks, "frontend_batches": fbatches}, off, digests.get("batched"), a, work)
+        k2, db, runner, {"chunks": chunks, "frontend_batches": fbatches}, off, paired, hip, digests.get("batched"), a,
+        work)
 
     # CPU leg, part 1: the oracle checks an even sample of fragments (the
     # reference binary is timed further down); skipped with --no-cpu

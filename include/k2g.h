@@ -275,6 +275,11 @@ int k2g_pipeline_create(k2g_db *db, const k2g_classify_params *p, const k2g_pipe
  * offsets and flags are copied into the pipeline's pinned staging first
  * (nothing is copied when they already lie there). */
 int k2g_pipeline_run(k2g_pipeline *pl, const k2g_fragments *in, k2g_results *out);
+/* The host pipeline's pinned staging (max_bases bytes, 2 * max_fragments + 1
+ * offsets, max_fragments pair flags): a reader that writes a batch straight
+ * into it and passes these pointers to k2g_pipeline_run skips the copy.
+ * K2G_EUSAGE for a device pipeline. */
+int k2g_pipeline_staging(k2g_pipeline *pl, char **seq, uint64_t **off, uint8_t **paired);
 /* Device-resident fragments; results stay in the pipeline's device buffers.
  * ms (may be NULL) receives the graph's device time. */
 int k2g_pipeline_run_device(k2g_pipeline *pl, const char *d_seq, const uint64_t *d_off,

@@ -63,6 +63,7 @@ EXPORTED_SYMBOLS = (
     "k2g_pipeline_create",
     "k2g_pipeline_run",
     "k2g_pipeline_run_device",
+    "k2g_pipeline_staging",
     "k2g_pipeline_digests",
     "k2g_pipeline_get_params",
     "k2g_pipeline_redo_counts",
@@ -230,6 +231,7 @@ def load_library(path: str = LIB_PATH):
     lib.k2g_pipeline_create.argtypes = [vp, C.POINTER(ClassifyParams), C.POINTER(PipelineParams), C.POINTER(vp)]
     lib.k2g_pipeline_run.argtypes = [vp, C.POINTER(Fragments), C.POINTER(Results)]
     lib.k2g_pipeline_run_device.argtypes = [vp, vp, vp, vp, u64, C.POINTER(C.c_double)]
+    lib.k2g_pipeline_staging.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)]
     lib.k2g_pipeline_digests.argtypes = [vp, vp, u64]
     lib.k2g_pipeline_get_params.argtypes = [vp, C.POINTER(PipelineParams), C.POINTER(C.c_int32)]
     lib.k2g_pipeline_redo_counts.argtypes = [vp, C.POINTER(u64), C.POINTER(u64)]
@@ -761,6 +763,16 @@ class Pipeline:
             _check(_lib.k2g_results_checksum(C.byref(res), C.byref(h)))
             return h.value
         return _copy_results(res, bool(self._kw.get("hll")), bool(self._kw.get("kmer_counts")))
+
+    def staging(self, max_fragments: int, max_bases: int):
+        """(bases, offsets, pair flags) numpy views of the host pipeline's
+        pinned staging: a batch written there is run without a copy."""
+        sq, so, sp = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        _check(_lib.k2g_pipeline_staging(self._h, C.byref(sq), C.byref(so), C.byref(sp)))
+        seq = np.ctypeslib.as_array((C.c_uint8 * max(1, max_bases)).from_address(sq.value))
+        off = np.ctypeslib.as_array((C.c_uint64 * (2 * max_fragments + 1)).from_address(so.value))
+        pf = np.ctypeslib.as_array((C.c_uint8 * max(1, max_fragments)).from_address(sp.value))
+        return seq, off, pf
 
     def run_device(self, d_seq, d_off, d_paired, n: int) -> float:
         """Classifies n device-resident fragments (device addresses); returns

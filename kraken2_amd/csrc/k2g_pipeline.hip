@@ -891,6 +891,15 @@ extern "C" int k2g_pipeline_run(k2g_pipeline *pl, const k2g_fragments *in, k2g_r
   return K2G_OK;
 }
 
+extern "C" int k2g_pipeline_staging(k2g_pipeline *pl, char **seq, uint64_t **off, uint8_t **paired) {
+  if (!pl) return fail(K2G_EUSAGE, "null pipeline");
+  if (!pl->host_io) return fail(K2G_EUSAGE, "a device pipeline has no host staging");
+  if (seq) *seq = pl->h_seq;
+  if (off) *off = pl->h_off;
+  if (paired) *paired = pl->h_paired;
+  return K2G_OK;
+}
+
 extern "C" int k2g_pipeline_run_device(k2g_pipeline *pl, const char *d_seq, const uint64_t *d_off,
                                        const uint8_t *d_paired, uint64_t n, double *ms) {
   if (!pl) return fail(K2G_EUSAGE, "null pipeline");

@@ -281,3 +281,32 @@ def test_translated_pipeline_matches_classify():
     finally:
         batch.close()
         db.close()
+
+
+def test_host_pipeline_runs_from_its_staging(env):
+    """A batch written straight into the pipeline's pinned staging runs
+    without a copy and gives k2g_classify's results."""
+    k2, db, batch, genomes, path = env
+    m1, m2, flags = _mixed_set(genomes, 2000, seed=111)
+    seq, off = _pack(k2, m1, m2)
+    pl = k2.Pipeline(db, 2000, len(seq), host_io=True, min_hit_groups=2, hll=True, kmer_counts=True)
+    try:
+        st_seq, st_off, st_pf = pl.staging(2000, len(seq))
+        for lo, hi in [(0, 2000), (500, 900)]:
+            sseq, soff = _pack(k2, m1[lo:hi], m2[lo:hi])
+            n = hi - lo
+            st_seq[:len(sseq)] = np.frombuffer(sseq, np.uint8)
+            st_off[:2 * n + 1] = soff
+            st_pf[:n] = flags[lo:hi]
+            got = pl.run(st_seq[:max(1, len(sseq))], st_off[:2 * n + 1], paired_flags=st_pf[:n], checksum_only=True)
+            want = batch.classify_checksum(sseq, soff, paired_flags=flags[lo:hi], mode=1, min_hit_groups=2,
+                                           hll=True, kmer_counts=True)
+            assert got == want, (lo, hi)
+    finally:
+        pl.close()
+    dev = k2.Pipeline(db, 10, 1000, host_io=False)
+    try:
+        with pytest.raises(k2.K2GError, match="staging"):
+            dev.staging(10, 1000)
+    finally:
+        dev.close()
