@@ -968,6 +968,7 @@ def main():
 
     # ---- verification and measurement passes (outside the timed region)
     digests = {}
+    stats_pass_differs = [0]
     lookups = lines = n_runs_total = 0
     space_redone = dense_redone = 0
     checked = {}
@@ -981,10 +982,15 @@ def main():
         mask = np.zeros(n, bool)
         for s, e in ranges:
             mask[s:e] = True
-            stats = name == "batched"
-            runner.run(s, e, mode, stats=stats)
+            runner.run(s, e, mode)
             dig[s:e] = runner.batch.digests(e - s)
-            if stats:
+            if name == "batched":
+                # probe statistics come from the kernels with per-fragment
+                # counters (the wave scan for every read): their results must
+                # equal the timed pass's
+                runner.run(s, e, mode, stats=True)
+                if not np.array_equal(runner.batch.digests(e - s), dig[s:e]):
+                    stats_pass_differs[0] += 1
                 c = runner.batch.fetch_counts(e - s, probe_stats=True)
                 lookups += int(c["lookups"].sum())
                 lines += int(c["probe_lines"].sum())
@@ -1000,8 +1006,9 @@ def main():
     else:
         pm = checked.get("plain_blocks", np.zeros(n, bool))
         plain_checked = int(pm.sum())
-        modes_agree = (np.array_equal(digests["plain_blocks"][pm], digests["batched"][pm])
-                       if "plain_blocks" in digests else True) and \
+        modes_agree = stats_pass_differs[0] == 0 and \
+            (np.array_equal(digests["plain_blocks"][pm], digests["batched"][pm])
+             if "plain_blocks" in digests else True) and \
             np.array_equal(digests["plain_one_launch"], digests["batched"]) and \
             (np.array_equal(digests["plain"], digests["batched"]) if "plain" in digests else True)
 

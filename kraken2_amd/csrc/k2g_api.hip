@@ -839,7 +839,7 @@ int k2g::classify_launch(k2g_batch *b, const k2g_classify_params *p, const char 
     units = b->unit_start.p;
   }
 
-  ClassifyParams cp;
+  ClassifyParams cp{};
   cp.confidence = p->confidence;
   cp.min_hit_groups = p->min_hit_groups;
   cp.quick = p->quick ? 1u : 0u;

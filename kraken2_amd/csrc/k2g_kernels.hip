@@ -287,6 +287,13 @@ struct ProbeState {
     }
     left = t.capacity;
   }
+  // start() from a key hashed ahead (prehash_key).
+  __device__ __forceinline__ void start_packed(const ProbeTable &t, uint64_t e) {
+    idx = (uint32_t) e;
+    ck = (uint32_t) (e >> 32) & 0x7FFFFFFFu;
+    skip = (e >> 63) != 0;
+    left = t.capacity;
+  }
   // One 16-byte group: the cells from idx to the group's end, in probe order.
   __device__ __forceinline__ bool step(const DbView &db, const ProbeTable &t, uint32_t &out) {
     return db.lean_probe ? step_lean(db, t, out) : step_bounded(db, t, out);
@@ -357,6 +364,156 @@ struct ProbeState {
     return false;
   }
 };
+
+// A queued key hashed ahead of its probe (tables of fewer than 2^32 cells):
+// home cell in the low 32 bits, compacted key in bits 32-62 (key_bits <= 31),
+// bit 63 set when the hash is below min_hash (src/classify.cc:819-823).
+__device__ __forceinline__ uint64_t prehash_key(const DbView &db, const ProbeTable &t, uint64_t key) {
+  const uint64_t hc = fmix64(key);
+  const uint64_t skip = hc < t.min_hash ? 1ULL : 0ULL;
+  const uint64_t ck = hc >> (64 - db.key_bits);
+  uint64_t idx = 0;
+  if (db.capacity != 1) {  // hc % capacity with one correction (see mod_capacity)
+    const uint64_t r = hc - __umul64hi(hc, t.barrett) * t.capacity;
+    idx = r >= t.capacity ? r - t.capacity : r;
+  }
+  return idx | (ck << 32) | (skip << 63);
+}
+
+// The last probes of a batch, at most kTailProbes left and no key waiting
+// (lean tables only: an empty cell ends every probe): probe j of the na
+// left takes the G = 64 / 2^ceil(log2 na) lanes j*G .. j*G+G-1, lane i of
+// them reading the 4 cells idx + 4i .. idx + 4i + 3 of its probe sequence,
+// so a round covers 4G consecutive cells.  The lowest lane of the group with
+// an empty or matching cell gives the value (the first such cell in probe
+// order); otherwise the group moves on by 4G cells.  A linear probe at load
+// 0.7 averages 6 cells but the longest of a batch of a hundred runs to about
+// 50, which one lane would walk 4 cells per round.
+constexpr uint32_t kTailProbes = 16;
+__device__ __forceinline__ void probe_tail(const DbView &db, const ProbeTable &t, uint64_t A, const ProbeState &ps,
+                                        uint32_t q, uint32_t *qval, uint32_t qstride, int lane) {
+  typedef uint32_t u32x4_a4 __attribute__((ext_vector_type(4), aligned(4)));
+  const uint32_t na = (uint32_t) __popcll(A);
+  const uint32_t lg = na <= 1 ? 0u : (na <= 2 ? 1u : (na <= 4 ? 2u : (na <= 8 ? 3u : 4u)));
+  const uint32_t G = 64u >> lg;
+  const uint32_t j = (uint32_t) lane >> (6 - lg), i = (uint32_t) lane & (G - 1u);
+  uint64_t m = A;
+  for (uint32_t c = 0; c < j && m; c++) m &= m - 1;  // the j-th probe left
+  const bool has = j < na;
+  const int src = m ? __ffsll((long long) m) - 1 : 0;
+  uint64_t idx = shfl64(ps.idx, src);
+  const uint32_t ck = shfl32(ps.ck, src);
+  const uint32_t qq = shfl32(q, src);
+  const uint64_t gmask = G == 64u ? ~0ULL : (((1ULL << G) - 1ULL) << (j * G));
+  const uint32_t vb = db.value_bits;
+  const uint32_t vmask = (1u << vb) - 1u;
+  bool done = !has;
+  uint32_t result = 0;
+  while (__ballot(!done)) {
+    uint32_t pos = 4u, val = 0;  // first stopping cell of this lane's four (4: none)
+    if (!done) {
+      uint64_t g = idx + 4u * i;
+      while (g >= t.capacity) g -= t.capacity;  // capacity >= 64, 4i < 256
+      // cells g .. g+3; past the end the padding holds copies of the first cells
+      const u32x4_a4 v = *(const u32x4_a4 *) (t.cells + g);
+      const uint32_t c4[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int c = 3; c >= 0; c--) {
+        const bool st = (c4[c] & vmask) == 0u || (c4[c] >> vb) == ck;
+        pos = st ? (uint32_t) c : pos;
+        val = st ? (c4[c] & vmask) : val;
+      }
+    }
+    const uint64_t S = __ballot(pos < 4u) & gmask;
+    const uint32_t got = shfl32(val, S ? __ffsll((long long) S) - 1 : lane);
+    if (!done) {
+      if (S) {
+        done = true;
+        result = got;
+      } else {
+        idx += 4u * G;
+        while (idx >= t.capacity) idx -= t.capacity;
+      }
+    }
+  }
+  if (has && i == 0) qval[qq * qstride] = result;
+}
+
+// Probes queue entries [0, qn): each lane works one key at a time and takes
+// the next queued key as soon as its probe ends (values land at
+// qval[q * qstride]).  PACKED: the entries were hashed ahead (prehash_key);
+// LEAN: db.lean_probe, and the last probes are split over the idle lanes
+// (probe_tail).  on_start(q, ps) runs as a lane starts entry q.
+struct NoStartHook {
+  __device__ __forceinline__ void operator()(uint32_t, const ProbeState &) const {}
+};
+template <bool PACKED, bool LEAN, class OnStart = NoStartHook>
+__device__ __forceinline__ void probe_loop(const DbView &db, const ProbeTable &ptab, const uint64_t *qkey,
+                                           uint32_t *qval, uint32_t qstride, uint32_t qn, int lane,
+                                           OnStart on_start = OnStart()) {
+  ProbeState ps;
+  uint32_t q = (uint32_t) lane;
+  bool active = q < qn;
+  if (active) {
+    if (PACKED)
+      ps.start_packed(ptab, qkey[q]);
+    else
+      ps.start(db, ptab, qkey[q]);
+    on_start(q, ps);
+  }
+  uint32_t next = qn < (uint32_t) kWave ? qn : (uint32_t) kWave;
+  for (;;) {
+    const uint64_t A = __ballot(active);
+    if (!A) break;
+    if (LEAN && next >= qn && (uint32_t) __popcll(A) <= kTailProbes) {
+      probe_tail(db, ptab, A, ps, q, qval, qstride, lane);
+      break;
+    }
+    if (active) {
+      uint32_t val;
+      if (LEAN ? ps.step_lean(db, ptab, val) : ps.step_bounded(db, ptab, val)) {
+        qval[q * qstride] = val;
+        active = false;
+      }
+    }
+    const uint64_t R = __ballot(!active) & (next < qn ? ~0ULL : 0ULL);
+    if (R) {
+      if (lane_bit(R)) {
+        q = next + (uint32_t) __popcll(R & lanemask_lt(lane));
+        if (q < qn) {
+          active = true;
+          if (PACKED)
+            ps.start_packed(ptab, qkey[q]);
+          else
+            ps.start(db, ptab, qkey[q]);
+          on_start(q, ps);
+        }
+      }
+      next += (uint32_t) __popcll(R);
+      if (next > qn) next = qn;
+    }
+  }
+}
+
+// probe_loop for the table at hand: below 2^32 cells the keys are hashed
+// first, densely (each entry turned into its packed home cell and compacted
+// key), so a lane that takes a new key only reads it.
+__device__ __forceinline__ void probe_queue(const DbView &db, const ProbeTable &ptab, uint64_t *qkey,
+                                            uint32_t *qval, uint32_t qstride, uint32_t qn, int lane) {
+  if (db.capacity <= 0xFFFFFFFFull) {
+    for (uint32_t e = (uint32_t) lane; e < qn; e += kWave) qkey[e] = prehash_key(db, ptab, qkey[e]);
+    wave_sync();
+    if (db.lean_probe)
+      probe_loop<true, true>(db, ptab, qkey, qval, qstride, qn, lane);
+    else
+      probe_loop<true, false>(db, ptab, qkey, qval, qstride, qn, lane);
+  } else {
+    if (db.lean_probe)
+      probe_loop<false, true>(db, ptab, qkey, qval, qstride, qn, lane);
+    else
+      probe_loop<false, false>(db, ptab, qkey, qval, qstride, qn, lane);
+  }
+}
 
 // Scans a fragment's streams in order -- its mates (nucleotide), or the six
 // reading frames of each mate (translated), with a marker element between
@@ -568,43 +725,19 @@ __device__ void scan_stream(const DbView &db, const ClassifyParams &P, const Fra
     // ---- B: probe the queued minimizers.  Each lane works one key at a time
     // and takes the next queued key as soon as its probe ends, so a step's
     // round trip serves up to 64 probes however unequal their lengths.
-    {
-      ProbeState ps;
-      uint32_t q = (uint32_t) lane;
-      bool active = q < qn;
-      if (active) {
-        ps.start(db, ptab, lds.qkey[q]);
-        if (EXTRAS && out.lookups && !ps.skip) {
+    if constexpr (!EXTRAS) {
+      probe_queue(db, ptab, lds.qkey, reinterpret_cast<uint32_t *>(lds.qkey), 2, qn, lane);
+    } else {  // the keys stay readable (HLL records); probe statistics per lane
+      auto stats = [&](uint32_t q, const ProbeState &ps) {
+        if (out.lookups && !ps.skip) {
           st.my_lookups++;
           st.my_lines += table_lines(db, lds.qkey[q]);
         }
-      }
-      uint32_t next = qn < (uint32_t) kWave ? qn : (uint32_t) kWave;
-      while (__ballot(active)) {
-        if (active) {
-          uint32_t val;
-          if (ps.step(db, ptab, val)) {
-            lds.qval(q) = val;
-            active = false;
-          }
-        }
-        const uint64_t R = __ballot(!active) & (next < qn ? ~0ULL : 0ULL);
-        if (R) {
-          if (lane_bit(R)) {
-            q = next + (uint32_t) __popcll(R & lanemask_lt(lane));
-            if (q < qn) {
-              active = true;
-              ps.start(db, ptab, lds.qkey[q]);
-              if (EXTRAS && out.lookups && !ps.skip) {
-                st.my_lookups++;
-                st.my_lines += table_lines(db, lds.qkey[q]);
-              }
-            }
-          }
-          next += (uint32_t) __popcll(R);
-          if (next > qn) next = qn;
-        }
-      }
+      };
+      if (db.lean_probe)
+        probe_loop<false, true>(db, ptab, lds.qkey, &lds.qval(0), 1, qn, lane, stats);
+      else
+        probe_loop<false, false>(db, ptab, lds.qkey, &lds.qval(0), 1, qn, lane, stats);
     }
     wave_sync();
 

@@ -547,7 +547,7 @@ int capture(k2g_pipeline *p) {
   const PipeKernels &K = kPipeVariants[vi];
   const uint32_t shift = (uint32_t) p->pp.bucket_shift;
   const uint32_t T = p->pp.unit_elements > 0 ? (uint32_t) p->pp.unit_elements : 4096u;
-  ClassifyParams cp;
+  ClassifyParams cp{};
   cp.confidence = p->params.confidence;
   cp.min_hit_groups = p->params.min_hit_groups;
   cp.quick = p->params.quick ? 1u : 0u;
