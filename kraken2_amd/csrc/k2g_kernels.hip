@@ -390,6 +390,7 @@ __device__ __forceinline__ uint64_t prehash_key(const DbView &db, const ProbeTab
 // 0.7 averages 6 cells but the longest of a batch of a hundred runs to about
 // 50, which one lane would walk 4 cells per round.
 constexpr uint32_t kTailProbes = 16;
+static_assert(kTailProbes <= 16, "probe_tail's groups are at least 4 lanes: at most 16 probes");
 __device__ __forceinline__ void probe_tail(const DbView &db, const ProbeTable &t, uint64_t A, const ProbeState &ps,
                                         uint32_t q, uint32_t *qval, uint32_t qstride, int lane) {
   typedef uint32_t u32x4_a4 __attribute__((ext_vector_type(4), aligned(4)));
