@@ -13,6 +13,7 @@
 #include <unistd.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <condition_variable>
 #include <deque>
@@ -24,9 +25,6 @@
 #include <iostream>
 #include <string>
 #include <vector>
-#ifdef _OPENMP
-#include <omp.h>
-#endif
 
 #include "k2g.h"
 #include "k2g_host.h"
@@ -181,13 +179,6 @@ void open_outputs(const Options &o, Outputs &out) {
     out.kraken = o.kraken_output_filename == "-" ? nullptr : open_out(o.kraken_output_filename);
 }
 
-std::string trim_pair_info(const std::string &id) {  // src/classify.cc:764-771
-  const size_t n = id.size();
-  if (n <= 2) return id;
-  if (id[n - 2] == '/' && (id[n - 1] == '1' || id[n - 1] == '2')) return id.substr(0, n - 2);
-  return id;
-}
-
 void mask_low_quality(Record &r, int q) {  // src/classify.cc:1061-1071
   if (!r.fastq) return;
   if (r.seq.size() != r.quals.size()) {
@@ -199,34 +190,107 @@ void mask_low_quality(Record &r, int q) {  // src/classify.cc:1061-1071
     if ((r.quals[i] - '!') < q) r.seq[i] = 'x';
 }
 
+// Runs fn(i) for every i in [0, n) on up to `threads` threads, items handed
+// out one at a time.  The workers are plain threads that end with the loop:
+// OpenMP's workers spin between parallel regions and take the cores that the
+// reader thread's parse needs at the same time.
+template <class F>
+void parallel_for(size_t n, int threads, F fn) {
+  const size_t t = std::min<size_t>(n, (size_t) std::max(1, threads));
+  if (t <= 1) {
+    for (size_t i = 0; i < n; i++) fn(i);
+    return;
+  }
+  std::atomic<size_t> next{0};
+  auto work = [&] {
+    for (size_t i; (i = next.fetch_add(1, std::memory_order_relaxed)) < n;) fn(i);
+  };
+  std::vector<std::thread> ts;
+  for (size_t k = 1; k < t; k++) ts.emplace_back(work);
+  work();
+  for (auto &th : ts) th.join();
+}
+
+// Output text written through a raw pointer into space reserved ahead: no
+// per-character bounds checks, and no zero fill once the buffer has grown.
+struct Text {
+  std::vector<char> v;
+  size_t n = 0;
+  char *reserve(size_t k) {
+    if (n + k > v.size()) v.resize(std::max(2 * v.size(), n + k));
+    return v.data() + n;
+  }
+  void commit(const char *end) { n = end - v.data(); }
+};
+
+char *put_uint(char *p, uint64_t v) {
+  char buf[20];
+  int n = 0;
+  do {
+    buf[n++] = (char) ('0' + v % 10);
+    v /= 10;
+  } while (v);
+  while (n) *p++ = buf[--n];
+  return p;
+}
+
+char *put_str(char *p, const char *s, size_t n) {
+  memcpy(p, s, n);
+  return p + n;
+}
+
 struct Block {
   std::vector<Record> a, b;
   size_t frags = 0;   // fragments in this block
   size_t first = 0;   // index of its first fragment in the GPU batch
-  std::string kraken, c1, c2, u1, u2;
+};
+
+// At most kPieceFrags fragments of one block: the unit of the parallel
+// packing and formatting, with its own output text and statistics (written
+// out in input order).
+constexpr size_t kPieceFrags = 512;
+struct Piece {
+  size_t block = 0, lo = 0, hi = 0;
+  Text kraken;
+  std::string c1, c2, u1, u2;
   Stats stats;
+};
+
+// One GPU batch in flight: its blocks, its packed bases (pinned, kept across
+// batches), its library batch and the results of its last call.  Two slots
+// alternate, so one batch's GPU call overlaps the formatting of the batch
+// before it and the packing of the batch after it.
+struct Slot {
+  std::vector<Block> blocks;
+  std::vector<Piece> pieces;  // the first npieces are this batch's
+  size_t npieces = 0;
+  k2g_batch *batch = nullptr;
+  char *seq = nullptr;
+  size_t seq_cap = 0;
+  std::vector<uint64_t> off;
+  size_t nfrag = 0;
+  k2g_results res{};
+  int rc = 0;
+  double gpu_s = 0;
 };
 
 struct Ctx {
   Options o;
   k2host::Taxonomy tax;
   k2g_db *db = nullptr;
-  k2g_batch *batch = nullptr;
+  Slot slots[2];
   bool translated = false;
   Stats stats;
   k2host::TaxonCounters counters;
   std::vector<uint64_t> kmer_totals;
-  // K2G_TIMING=1: seconds waiting for parsed input, packing, on the GPU
-  // (k2g_classify), formatting and writing, and counters
+  // K2G_TIMING=1: seconds waiting for parsed input, packing, in GPU calls
+  // (k2g_classify, on the GPU thread), formatting and writing, and counters
   bool timing = false;
   double t_wait = 0, t_pack = 0, t_gpu = 0, t_format = 0, t_count = 0;
   double t_alloc = 0, t_offsets = 0;  // parts of t_pack
   // reader thread: parsing chunks, all of its work, and waiting for queue room
   double t_rparse = 0, t_rbusy = 0, t_rfull = 0;
-  // packed bases of one batch (pinned, kept across batches) and offsets
-  char *seq = nullptr;
-  size_t seq_cap = 0;
-  std::vector<uint64_t> off;
+  double t_gpu_wait = 0;  // main thread waiting for a GPU call to finish
 };
 
 double now_s() {
@@ -235,77 +299,96 @@ double now_s() {
   return t.tv_sec + t.tv_usec / 1e6;
 }
 
-void append_uint(std::string &s, uint64_t v) {
-  char buf[24];
-  int n = 0;
-  do {
-    buf[n++] = (char) ('0' + v % 10);
-    v /= 10;
-  } while (v);
-  while (n) s.push_back(buf[--n]);
-}
-
 // One kraken output line (src/classify.cc:908-945, 950-991).
 void format_line(const Ctx &c, const Record &s1, const Record *s2, uint32_t call,
-                 const uint32_t *runs, uint64_t nruns, std::string &out) {
+                 const uint32_t *runs, uint64_t nruns, Text &out) {
   const auto &tax = c.tax;
-  out += call ? "C\t" : "U\t";
-  if (s2)
-    out += trim_pair_info(s1.header);
-  else
-    out += s1.header;
-  out.push_back('\t');
+  const std::string &id = s1.header;
+  size_t id_len = id.size();
+  // a pair's id without its /1 or /2 (src/classify.cc:764-771)
+  if (s2 && id_len > 2 && id[id_len - 2] == '/' && (id[id_len - 1] == '1' || id[id_len - 1] == '2'))
+    id_len -= 2;
   const uint64_t ext = tax.nodes[call].external_id;
+  const char *name = nullptr;
+  size_t name_len = 0;
   if (c.o.print_scientific_name) {
-    out += call ? tax.name(call) : "unclassified";
-    out += " (taxid ";
-    append_uint(out, ext);
-    out += ")";
+    name = call ? tax.name(call) : "unclassified";
+    name_len = strlen(name);
+  }
+  // every field's longest text: 20 digits per number, 32 bytes per run
+  char *p = out.reserve(id_len + name_len + 128 + 32 * nruns);
+  p = put_str(p, call ? "C\t" : "U\t", 2);
+  p = put_str(p, id.data(), id_len);
+  *p++ = '\t';
+  if (name) {
+    p = put_str(p, name, name_len);
+    p = put_str(p, " (taxid ", 8);
+    p = put_uint(p, ext);
+    *p++ = ')';
   } else {
-    append_uint(out, ext);
+    p = put_uint(p, ext);
   }
-  out.push_back('\t');
-  append_uint(out, s1.seq.size());
+  *p++ = '\t';
+  p = put_uint(p, s1.seq.size());
   if (s2) {
-    out.push_back('|');
-    append_uint(out, s2->seq.size());
+    *p++ = '|';
+    p = put_uint(p, s2->seq.size());
   }
-  out.push_back('\t');
+  *p++ = '\t';
   if (c.o.quick_mode) {
-    append_uint(out, ext);
-    out += ":Q";
+    p = put_uint(p, ext);
+    p = put_str(p, ":Q", 2);
   } else if (nruns == 0) {
-    out += "0:0";
+    p = put_str(p, "0:0", 3);
   } else {
     for (uint64_t i = 0; i < nruns; i++) {
       const uint32_t code = runs[2 * i], cnt = runs[2 * i + 1];
       const bool last = i + 1 == nruns;
       if (code == K2G_CODE_MATE) {
-        out += last ? "|:|" : "|:| ";
+        p = put_str(p, "|:| ", last ? 3 : 4);
       } else if (code == K2G_CODE_FRAME) {
-        out += last ? "-:-" : "-:- ";
+        p = put_str(p, "-:- ", last ? 3 : 4);
       } else if (code == K2G_CODE_AMBIG) {
-        out += "A:";
-        append_uint(out, cnt);
-        out.push_back(' ');
+        p = put_str(p, "A:", 2);
+        p = put_uint(p, cnt);
+        *p++ = ' ';
       } else {
-        append_uint(out, tax.nodes[code].external_id);
-        out.push_back(':');
-        append_uint(out, cnt);
-        if (!last) out.push_back(' ');
+        p = put_uint(p, tax.nodes[code].external_id);
+        *p++ = ':';
+        p = put_uint(p, cnt);
+        if (!last) *p++ = ' ';
       }
     }
   }
-  out.push_back('\n');
+  *p++ = '\n';
+  out.commit(p);
 }
 
-void run_batch(Ctx &c, std::vector<Block> &blocks, Outputs &out) {
+// The record pair of fragment i of a block.
+void fragment_records(const Options &o, Block &b, size_t i, Record *&r1, Record *&r2) {
+  r2 = nullptr;
+  if (!o.paired_end_processing) {
+    r1 = &b.a[i];
+  } else if (o.single_file_pairs) {
+    r1 = &b.a[2 * i];
+    r2 = &b.a[2 * i + 1];
+  } else {
+    r1 = &b.a[i];
+    r2 = &b.b[i];
+  }
+}
+
+// Offsets of the slot's fragments, then each block's mates copied into the
+// slot's pinned buffer (kept across batches: no page faults after the first
+// batch, and the upload reads pinned memory).
+void pack_batch(Ctx &c, Slot &sl) {
   const Options &o = c.o;
   const bool paired = o.paired_end_processing;
   double t = now_s();
-  // fragment arrays
   size_t nfrag = 0, bases = 0;
-  for (auto &b : blocks) {
+  sl.npieces = 0;
+  for (size_t bi = 0; bi < sl.blocks.size(); bi++) {
+    Block &b = sl.blocks[bi];
     b.first = nfrag;
     if (!paired)
       b.frags = b.a.size();
@@ -316,136 +399,142 @@ void run_batch(Ctx &c, std::vector<Block> &blocks, Outputs &out) {
     nfrag += b.frags;
     for (auto &r : b.a) bases += r.seq.size();
     for (auto &r : b.b) bases += r.seq.size();
+    for (size_t lo = 0; lo < b.frags; lo += kPieceFrags) {
+      if (sl.npieces == sl.pieces.size()) sl.pieces.emplace_back();
+      Piece &pc = sl.pieces[sl.npieces++];
+      pc.block = bi;
+      pc.lo = lo;
+      pc.hi = std::min(lo + kPieceFrags, b.frags);
+    }
   }
-  // offsets first, then each block copies its mates into the pinned buffer
-  // that is kept across batches (no page faults after the first batch, and
-  // the upload reads pinned memory)
   double ta = now_s();
-  if (bases + 1 > c.seq_cap) {
-    if (c.seq) k2g_host_free(c.seq);
-    c.seq = nullptr;
-    c.seq_cap = std::max<size_t>(bases + 1, c.seq_cap + c.seq_cap / 2);
-    const int rc = k2g_host_alloc(c.seq_cap, (void **) &c.seq);
+  if (bases + 1 > sl.seq_cap) {
+    if (sl.seq) k2g_host_free(sl.seq);
+    sl.seq = nullptr;
+    sl.seq_cap = std::max<size_t>(bases + 1, sl.seq_cap + sl.seq_cap / 2);
+    const int rc = k2g_host_alloc(sl.seq_cap, (void **) &sl.seq);
     if (rc) die(rc, k2g_last_error());
   }
   double tb = now_s();
   c.t_alloc += tb - ta;
-  std::vector<uint64_t> &off = c.off;
+  std::vector<uint64_t> &off = sl.off;
   off.resize(2 * nfrag + 1);
   uint64_t pos = 0;
-  for (auto &b : blocks) {
+  for (auto &b : sl.blocks) {
     for (size_t i = 0; i < b.frags; i++) {
       const size_t f = b.first + i;
-      const Record &r1 = paired && o.single_file_pairs ? b.a[2 * i] : b.a[i];
+      Record *r1, *r2;
+      fragment_records(o, b, i, r1, r2);
       off[2 * f] = pos;
-      pos += r1.seq.size();
+      pos += r1->seq.size();
       off[2 * f + 1] = pos;
-      if (paired) pos += (o.single_file_pairs ? b.a[2 * i + 1] : b.b[i]).seq.size();
+      if (r2) pos += r2->seq.size();
     }
   }
   off[2 * nfrag] = pos;
   ta = now_s();
   c.t_offsets += ta - tb;
-#pragma omp parallel for schedule(dynamic, 1) num_threads(o.num_threads)
-  for (size_t bi = 0; bi < blocks.size(); bi++) {
-    Block &b = blocks[bi];
-    for (size_t i = 0; i < b.frags; i++) {
+  parallel_for(sl.npieces, o.num_threads, [&](size_t pi) {
+    const Piece &pc = sl.pieces[pi];
+    Block &b = sl.blocks[pc.block];
+    for (size_t i = pc.lo; i < pc.hi; i++) {
       const size_t f = b.first + i;
-      Record *r1, *r2 = nullptr;
-      if (!paired) {
-        r1 = &b.a[i];
-      } else if (o.single_file_pairs) {
-        r1 = &b.a[2 * i];
-        r2 = &b.a[2 * i + 1];
-      } else {
-        r1 = &b.a[i];
-        r2 = &b.b[i];
-      }
+      Record *r1, *r2;
+      fragment_records(o, b, i, r1, r2);
       if (o.minimum_quality_score > 0) {
         mask_low_quality(*r1, o.minimum_quality_score);
         if (r2) mask_low_quality(*r2, o.minimum_quality_score);
       }
-      std::memcpy(c.seq + off[2 * f], r1->seq.data(), r1->seq.size());
-      if (r2) std::memcpy(c.seq + off[2 * f + 1], r2->seq.data(), r2->seq.size());
+      std::memcpy(sl.seq + off[2 * f], r1->seq.data(), r1->seq.size());
+      if (r2) std::memcpy(sl.seq + off[2 * f + 1], r2->seq.data(), r2->seq.size());
     }
-  }
+  });
+  sl.nfrag = nfrag;
+  c.t_pack += now_s() - t;
+}
 
+// The slot's GPU call (k2g_classify: upload, classify, results into host
+// memory); runs on the GPU thread, so its errors are kept for the main thread.
+void gpu_batch(const Ctx &c, Slot &sl) {
+  const Options &o = c.o;
   k2g_classify_params p{};
   p.confidence = o.confidence_threshold;
   p.min_hit_groups = o.minimum_hit_groups;
   p.quick = o.quick_mode;
-  p.paired = paired;
+  p.paired = o.paired_end_processing;
   p.mode = o.gpu_mode;
   p.want_kmer_counts = !o.report_filename.empty();
   p.want_hll = !o.report_filename.empty() && o.report_kmer_data;
-  k2g_fragments fr{c.seq, off.data(), nullptr, nfrag};
-  k2g_results res;
-  double t2 = now_s();
-  c.t_pack += t2 - t;
-  int rc = k2g_classify(c.batch, &p, &fr, &res);
-  if (rc) die(rc, k2g_last_error());
-  t = now_s();
-  c.t_gpu += t - t2;
+  k2g_fragments fr{sl.seq, sl.off.data(), nullptr, sl.nfrag};
+  const double t = now_s();
+  sl.rc = k2g_classify(sl.batch, &p, &fr, &sl.res);
+  sl.gpu_s = now_s() - t;
+}
 
-  // format blocks in parallel, emit in order
-#pragma omp parallel for schedule(dynamic, 1) num_threads(o.num_threads)
-  for (size_t bi = 0; bi < blocks.size(); bi++) {
-    Block &b = blocks[bi];
-    b.kraken.clear();
-    for (size_t i = 0; i < b.frags; i++) {
+// Output lines of the slot's blocks (formatted in parallel, written in
+// order), statistics and counters.
+void finish_batch(Ctx &c, Slot &sl, Outputs &out, const std::string &gpu_error) {
+  const Options &o = c.o;
+  if (sl.rc) die(sl.rc, gpu_error);
+  c.t_gpu += sl.gpu_s;
+  const k2g_results &res = sl.res;
+  double t = now_s();
+  parallel_for(sl.npieces, o.num_threads, [&](size_t pi) {
+    Piece &pc = sl.pieces[pi];
+    Block &b = sl.blocks[pc.block];
+    pc.kraken.n = 0;
+    pc.c1.clear();
+    pc.c2.clear();
+    pc.u1.clear();
+    pc.u2.clear();
+    pc.stats = Stats();
+    for (size_t i = pc.lo; i < pc.hi; i++) {
       const size_t g = b.first + i;
-      Record *r1, *r2 = nullptr;
-      if (!paired) {
-        r1 = &b.a[i];
-      } else if (o.single_file_pairs) {
-        r1 = &b.a[2 * i];
-        r2 = &b.a[2 * i + 1];
-      } else {
-        r1 = &b.a[i];
-        r2 = &b.b[i];
-      }
+      Record *r1, *r2;
+      fragment_records(o, b, i, r1, r2);
       const uint32_t call = res.call[g];
-      b.stats.total_sequences++;
-      if (call) b.stats.total_classified++;
-      b.stats.total_bases += r1->seq.size() + (r2 ? r2->seq.size() : 0);
+      pc.stats.total_sequences++;
+      if (call) pc.stats.total_classified++;
+      pc.stats.total_bases += r1->seq.size() + (r2 ? r2->seq.size() : 0);
       if (out.kraken)
         format_line(c, *r1, r2, call, res.runs + 2 * res.run_off[g], res.run_off[g + 1] - res.run_off[g],
-                    b.kraken);
+                    pc.kraken);
       if (call) {
         if (out.c1) {
           char suffix[48];
           snprintf(suffix, sizeof(suffix), " kraken:taxid|%llu",
                    (unsigned long long) c.tax.nodes[call].external_id);
-          r1->append_text(b.c1, suffix);
-          if (r2) r2->append_text(b.c2, suffix);
+          r1->append_text(pc.c1, suffix);
+          if (r2) r2->append_text(pc.c2, suffix);
         }
       } else if (out.u1) {
-        r1->append_text(b.u1, nullptr);
-        if (r2) r2->append_text(b.u2, nullptr);
+        r1->append_text(pc.u1, nullptr);
+        if (r2) r2->append_text(pc.u2, nullptr);
       }
     }
+  });
+  for (size_t pi = 0; pi < sl.npieces; pi++) {
+    const Piece &pc = sl.pieces[pi];
+    if (out.kraken) fwrite(pc.kraken.v.data(), 1, pc.kraken.n, out.kraken);
+    if (out.c1) fwrite(pc.c1.data(), 1, pc.c1.size(), out.c1);
+    if (out.c2) fwrite(pc.c2.data(), 1, pc.c2.size(), out.c2);
+    if (out.u1) fwrite(pc.u1.data(), 1, pc.u1.size(), out.u1);
+    if (out.u2) fwrite(pc.u2.data(), 1, pc.u2.size(), out.u2);
+    c.stats.total_sequences += pc.stats.total_sequences;
+    c.stats.total_bases += pc.stats.total_bases;
+    c.stats.total_classified += pc.stats.total_classified;
   }
-  for (auto &b : blocks) {
-    if (out.kraken) fwrite(b.kraken.data(), 1, b.kraken.size(), out.kraken);
-    if (out.c1) fwrite(b.c1.data(), 1, b.c1.size(), out.c1);
-    if (out.c2) fwrite(b.c2.data(), 1, b.c2.size(), out.c2);
-    if (out.u1) fwrite(b.u1.data(), 1, b.u1.size(), out.u1);
-    if (out.u2) fwrite(b.u2.data(), 1, b.u2.size(), out.u2);
-    c.stats.total_sequences += b.stats.total_sequences;
-    c.stats.total_bases += b.stats.total_bases;
-    c.stats.total_classified += b.stats.total_classified;
-  }
-  t2 = now_s();
+  double t2 = now_s();
   c.t_format += t2 - t;
   if (!o.report_filename.empty()) {
     // read counts per called taxon; k-mer counters per block (HLL state
     // depends on the block a hit falls in, src/classify.cc:554, 640-645)
-    for (auto &b : blocks)
+    for (auto &b : sl.blocks)
       for (size_t i = 0; i < b.frags; i++)
         if (res.call[b.first + i]) c.counters[res.call[b.first + i]].n_reads++;
     if (o.report_kmer_data) {
       std::vector<std::pair<uint32_t, uint32_t>> recs;
-      for (auto &b : blocks) {
+      for (auto &b : sl.blocks) {
         recs.clear();
         for (size_t i = 0; i < b.frags; i++) {
           const size_t g = b.first + i;
@@ -478,22 +567,40 @@ int process_files(Ctx &c, const char *f1, const char *f2, Outputs &out) {
   bool finished = false;
   std::thread reader([&] {
     bool done = false;
-    // single reads: records parsed a chunk at a time (side by side on the
-    // host threads for plain FASTA), then cut into the reference's blocks
-    std::vector<Record> parsed, used_recs;  // records of consumed blocks, buffers reused
-    size_t next_parsed = 0;
-    auto next_record = [&](std::vector<Record> &into) -> bool {
-      if (next_parsed == parsed.size()) {
-        parsed.clear();
-        next_parsed = 0;
+    // Each input file's records, parsed a chunk at a time (side by side on
+    // the host threads for plain FASTA), then cut into the reference's
+    // blocks; `used` holds records of consumed blocks, whose buffers the
+    // next chunk's parse reuses.  Chunks start at 4 MiB, so the first block
+    // is ready early, and double up to 64 MiB.
+    struct Stream {
+      k2host::Reader *r;
+      std::vector<Record> parsed, used;
+      size_t next = 0;
+      size_t chunk = 4u << 20;
+    };
+    Stream s1{&r1}, s2{&r2};
+    auto take = [&](Stream &s, std::vector<Record> &into) -> bool {
+      if (s.next == s.parsed.size()) {
+        s.parsed.clear();
+        s.next = 0;
         const double t0 = now_s();
-        const size_t got = r1.next_records(64u << 20, o.num_threads, parsed, &used_recs);
+        const size_t got = s.r->next_records(s.chunk, o.num_threads, s.parsed, &s.used);
+        s.chunk = std::min<size_t>(2 * s.chunk, 64u << 20);
         c.t_rparse += now_s() - t0;
         if (!got) return false;
       }
-      into.push_back(std::move(parsed[next_parsed++]));
+      into.push_back(std::move(s.parsed[s.next++]));
       return true;
     };
+    auto take_n = [&](Stream &s, size_t n, std::vector<Record> &into) {
+      for (auto &r : into) s.used.push_back(std::move(r));
+      into.clear();
+      while (into.size() < n && take(s, into)) {
+      }
+    };
+    // The first batches are small, so the GPU starts early; each later one
+    // doubles up to batch_bases.
+    size_t target = std::min<size_t>(o.batch_bases, 16u << 20);
     while (!done) {
       const double t_batch = now_s();
       std::vector<Block> blocks;
@@ -505,35 +612,29 @@ int process_files(Ctx &c, const char *f1, const char *f2, Outputs &out) {
         }
       }
       size_t bases = 0, used = 0;
-      while (bases < o.batch_bases && used < 4096) {
+      while (bases < target && used < 4096) {
         if (used == blocks.size()) blocks.emplace_back();
         Block &b = blocks[used];
-        b.stats = Stats();
-        b.kraken.clear();
-        b.c1.clear();
-        b.c2.clear();
-        b.u1.clear();
-        b.u2.clear();
         bool ok;
         if (!o.paired_end_processing) {
           // load_block's rule (src/seqreader.h:101-119): records until
           // 3 MiB of bases
-          for (auto &r : b.a) used_recs.push_back(std::move(r));
+          for (auto &r : b.a) s1.used.push_back(std::move(r));
           b.a.clear();
           size_t got = 0;
           while (got < 3 * 1024 * 1024) {
-            if (!next_record(b.a)) break;
+            if (!take(s1, b.a)) break;
             got += b.a.back().seq.size();
           }
           ok = !b.a.empty();
         } else if (!o.single_file_pairs) {
-          size_t n2 = 0;
-          std::thread mate2([&] { n2 = r2.load_batch(10000, b.b); });
-          ok = r1.load_batch(10000, b.a) > 0;
-          mate2.join();
-          if (ok) ok = n2 > 0;
+          // load_batch's rule (src/seqreader.h:146-163) on each file
+          take_n(s1, 10000, b.a);
+          take_n(s2, 10000, b.b);
+          ok = !b.a.empty() && !b.b.empty();
         } else {
-          ok = r1.load_batch(20000, b.a) > 0;
+          take_n(s1, 20000, b.a);
+          ok = !b.a.empty();
         }
         if (!ok) {
           done = true;
@@ -543,6 +644,7 @@ int process_files(Ctx &c, const char *f1, const char *f2, Outputs &out) {
         for (auto &r : b.a) bases += r.seq.size();
         for (auto &r : b.b) bases += r.seq.size();
       }
+      target = std::min(o.batch_bases, 2 * target);
       blocks.resize(used);
       std::unique_lock<std::mutex> lk(mu);
       const double tw = now_s();
@@ -554,6 +656,23 @@ int process_files(Ctx &c, const char *f1, const char *f2, Outputs &out) {
       cv.notify_all();
     }
   });
+  // Batch i is packed while batch i-1 is on the GPU; then batch i goes to
+  // the GPU thread while batch i-1 is formatted and written, in input order.
+  std::thread gpu;
+  std::string gpu_error;
+  int cur = 0;
+  bool prev = false;  // slots[cur ^ 1] holds a batch whose GPU call was started
+  auto join_gpu = [&] {
+    if (!gpu.joinable()) return;
+    const double t = now_s();
+    gpu.join();
+    c.t_gpu_wait += now_s() - t;
+  };
+  auto recycle = [&](Slot &sl) {
+    std::lock_guard<std::mutex> lk(mu);
+    if (spare.size() < 2) spare.push_back(std::move(sl.blocks));
+    sl.blocks.clear();
+  };
   for (;;) {
     std::vector<Block> blocks;
     {
@@ -566,9 +685,25 @@ int process_files(Ctx &c, const char *f1, const char *f2, Outputs &out) {
       ready.pop_front();
       cv.notify_all();
     }
-    run_batch(c, blocks, out);
-    std::lock_guard<std::mutex> lk(mu);
-    if (spare.size() < 2) spare.push_back(std::move(blocks));
+    Slot &sl = c.slots[cur];
+    sl.blocks = std::move(blocks);
+    pack_batch(c, sl);
+    join_gpu();
+    gpu = std::thread([&c, &sl, &gpu_error] {
+      gpu_batch(c, sl);
+      if (sl.rc) gpu_error = k2g_last_error();
+    });
+    if (prev) {
+      finish_batch(c, c.slots[cur ^ 1], out, gpu_error);
+      recycle(c.slots[cur ^ 1]);
+    }
+    prev = true;
+    cur ^= 1;
+  }
+  join_gpu();
+  if (prev) {
+    finish_batch(c, c.slots[cur ^ 1], out, gpu_error);
+    recycle(c.slots[cur ^ 1]);
   }
   reader.join();
   return 0;
@@ -611,8 +746,10 @@ int main(int argc, char **argv) {
   k2g_db_info info;
   k2g_db_get_info(c.db, &info);
   c.translated = !info.dna_db;
-  rc = k2g_batch_create(c.db, &c.batch);
-  if (rc) die(rc, k2g_last_error());
+  for (Slot &sl : c.slots) {
+    rc = k2g_batch_create(c.db, &sl.batch);
+    if (rc) die(rc, k2g_last_error());
+  }
   std::cerr << " done." << std::endl;
 
   Outputs out;
@@ -642,11 +779,12 @@ int main(int argc, char **argv) {
   report_stats(secs, c.stats);
   if (c.timing)
     fprintf(stderr,
-            "  timing: input wait %.3fs, packing %.3fs (buffer %.3fs, offsets %.3fs), GPU %.3fs, "
-            "format+write %.3fs, counters %.3fs, outside the phases %.3fs; reader thread: busy %.3fs "
-            "(chunk reading and parsing %.3fs), waiting for queue room %.3fs\n",
-            c.t_wait, c.t_pack, c.t_alloc, c.t_offsets, c.t_gpu, c.t_format, c.t_count,
-            secs - c.t_wait - c.t_pack - c.t_gpu - c.t_format - c.t_count, c.t_rbusy, c.t_rparse, c.t_rfull);
+            "  timing: input wait %.3fs, packing %.3fs (buffer %.3fs, offsets %.3fs), waiting for the GPU %.3fs "
+            "(GPU calls %.3fs, overlapped), format+write %.3fs, counters %.3fs, outside the phases %.3fs; "
+            "reader thread: busy %.3fs (chunk reading and parsing %.3fs), waiting for queue room %.3fs\n",
+            c.t_wait, c.t_pack, c.t_alloc, c.t_offsets, c.t_gpu_wait, c.t_gpu, c.t_format, c.t_count,
+            secs - c.t_wait - c.t_pack - c.t_gpu_wait - c.t_format - c.t_count, c.t_rbusy, c.t_rparse,
+            c.t_rfull);
   if (!o.report_filename.empty()) {
     if (!k2host::report_kraken_style(o.report_filename, o.report_zero_counts, o.report_kmer_data,
                                      c.tax, c.counters, c.stats.total_sequences,
@@ -656,7 +794,10 @@ int main(int argc, char **argv) {
   for (FILE *f : {out.c1, out.c2, out.u1, out.u2})
     if (f) fclose(f);
   if (out.kraken && out.kraken != stdout) fclose(out.kraken);
-  k2g_batch_destroy(c.batch);
+  for (Slot &sl : c.slots) {
+    k2g_batch_destroy(sl.batch);
+    if (sl.seq) k2g_host_free(sl.seq);
+  }
   k2g_db_close(c.db);
   return 0;
 }
