@@ -23,13 +23,12 @@
 #include <cstdlib>
 #include <cstring>
 #include <iostream>
+#include <memory>
 #include <string>
 #include <vector>
 
 #include "k2g.h"
 #include "k2g_host.h"
-
-using k2host::Record;
 
 namespace {
 
@@ -43,6 +42,9 @@ struct Options {
   int num_threads = 1, minimum_quality_score = 0, minimum_hit_groups = 0;
   int gpu_mode = 2;                 // K2G_MODE: 0 plain, 1 persistent, 2 chosen per batch
   size_t batch_bases = 256u << 20;  // bases per GPU batch (several blocks)
+  // K2G_PARSE_THREADS / K2G_FORMAT_THREADS: threads of the reader's parse and
+  // of the formatting (0: -p); the two run at the same time
+  int parse_threads = 0, format_threads = 0;
   std::vector<std::string> filenames;
 };
 
@@ -129,6 +131,8 @@ void parse(int argc, char **argv, Options &o) {
     o.gpu_mode = v >= 0 && v <= 2 ? v : 2;
   }
   if (const char *b = getenv("K2G_BATCH_BASES")) o.batch_bases = strtoull(b, nullptr, 10);
+  if (const char *t = getenv("K2G_PARSE_THREADS")) o.parse_threads = std::max(0, atoi(t));
+  if (const char *t = getenv("K2G_FORMAT_THREADS")) o.format_threads = std::max(0, atoi(t));
 }
 
 struct Outputs {
@@ -179,36 +183,12 @@ void open_outputs(const Options &o, Outputs &out) {
     out.kraken = o.kraken_output_filename == "-" ? nullptr : open_out(o.kraken_output_filename);
 }
 
-void mask_low_quality(Record &r, int q) {  // src/classify.cc:1061-1071
+// Bases whose quality is below q become 'x' (src/classify.cc:1061-1071); the
+// reader keeps as many qualities as bases.
+void mask_low_quality(const k2host::FlatRec &r, const char *quals, char *bases, int q) {
   if (!r.fastq) return;
-  if (r.seq.size() != r.quals.size()) {
-    std::cerr << "classify: " << r.header << ": Sequence length (" << r.seq.size()
-              << ") != Quality string length (" << r.quals.size() << ")\n";
-    exit(65);
-  }
-  for (size_t i = 0; i < r.seq.size(); i++)
-    if ((r.quals[i] - '!') < q) r.seq[i] = 'x';
-}
-
-// Runs fn(i) for every i in [0, n) on up to `threads` threads, items handed
-// out one at a time.  The workers are plain threads that end with the loop:
-// OpenMP's workers spin between parallel regions and take the cores that the
-// reader thread's parse needs at the same time.
-template <class F>
-void parallel_for(size_t n, int threads, F fn) {
-  const size_t t = std::min<size_t>(n, (size_t) std::max(1, threads));
-  if (t <= 1) {
-    for (size_t i = 0; i < n; i++) fn(i);
-    return;
-  }
-  std::atomic<size_t> next{0};
-  auto work = [&] {
-    for (size_t i; (i = next.fetch_add(1, std::memory_order_relaxed)) < n;) fn(i);
-  };
-  std::vector<std::thread> ts;
-  for (size_t k = 1; k < t; k++) ts.emplace_back(work);
-  work();
-  for (auto &th : ts) th.join();
+  for (uint64_t i = 0; i < r.seq_len; i++)
+    if ((quals[i] - '!') < q) bases[i] = 'x';
 }
 
 // Output text written through a raw pointer into space reserved ahead: no
@@ -239,15 +219,18 @@ char *put_str(char *p, const char *s, size_t n) {
   return p + n;
 }
 
-struct Block {
-  std::vector<Record> a, b;
-  size_t frags = 0;   // fragments in this block
-  size_t first = 0;   // index of its first fragment in the GPU batch
+// A block of the reference's (src/classify.cc:523-543): the unit of the
+// output order and of the per-block HLL state.  Its records are
+// recs[rec_first, rec_first + nrec) of the batch, its fragments
+// [frag_first, frag_first + frags) (a pair's mates are two records side by
+// side; an odd last record of interleaved pairs has no fragment).
+struct BlockSpan {
+  size_t rec_first = 0, nrec = 0, frag_first = 0, frags = 0;
 };
 
 // At most kPieceFrags fragments of one block: the unit of the parallel
-// packing and formatting, with its own output text and statistics (written
-// out in input order).
+// formatting, with its own output text and statistics (written out in
+// input order).
 constexpr size_t kPieceFrags = 512;
 struct Piece {
   size_t block = 0, lo = 0, hi = 0;
@@ -256,40 +239,61 @@ struct Piece {
   Stats stats;
 };
 
-// One GPU batch in flight: its blocks, its packed bases (pinned, kept across
-// batches), its library batch and the results of its last call.  Two slots
-// alternate, so one batch's GPU call overlaps the formatting of the batch
-// before it and the packing of the batch after it.
+// The GPU side of a batch: a library batch and the results of its last
+// call.  Three slots rotate: two batches' GPU calls overlap each other and
+// the formatting of the batch before them.
 struct Slot {
-  std::vector<Block> blocks;
-  std::vector<Piece> pieces;  // the first npieces are this batch's
-  size_t npieces = 0;
   k2g_batch *batch = nullptr;
-  char *seq = nullptr;
-  size_t seq_cap = 0;
-  std::vector<uint64_t> off;
-  size_t nfrag = 0;
   k2g_results res{};
   int rc = 0;
   double gpu_s = 0;
+  std::string error;
+};
+
+// One batch of input: records in fragment order with their bases back to
+// back in pinned memory (the GPU input as it stands, mate 1 then mate 2 of
+// a pair) and their ids, comments and qualities in `text`, cut into blocks.
+struct Batch {
+  std::vector<k2host::FlatRec> recs;
+  std::vector<char> text;
+  k2host::SeqSink seq;
+  std::vector<BlockSpan> blocks;
+  size_t nfrag = 0, bases = 0;  // fragments and bases of the blocks
+  std::vector<uint64_t> off;    // 2 nfrag + 1 fragment offsets into seq
+  std::vector<Piece> pieces;    // the first npieces are this batch's
+  size_t npieces = 0;
+  Slot *slot = nullptr;         // where its GPU call ran
+  void clear() {
+    recs.clear();
+    text.clear();
+    seq.n = 0;
+    blocks.clear();
+    nfrag = bases = 0;
+  }
+  ~Batch() {
+    if (seq.p) k2g_host_free(seq.p);
+  }
 };
 
 struct Ctx {
   Options o;
   k2host::Taxonomy tax;
   k2g_db *db = nullptr;
-  Slot slots[2];
+  Slot slots[3];
+  k2host::WorkPool *pool = nullptr;  // the main thread's workers (formatting)
   bool translated = false;
   Stats stats;
   k2host::TaxonCounters counters;
   std::vector<uint64_t> kmer_totals;
-  // K2G_TIMING=1: seconds waiting for parsed input, packing, in GPU calls
-  // (k2g_classify, on the GPU thread), formatting and writing, and counters
+  // K2G_TIMING=1: seconds waiting for parsed input, preparing batches
+  // (offsets, masking), waiting for and in GPU calls (k2g_classify, on the
+  // GPU thread), formatting and writing, and counters
   bool timing = false;
   double t_wait = 0, t_pack = 0, t_gpu = 0, t_format = 0, t_count = 0;
-  double t_alloc = 0, t_offsets = 0;  // parts of t_pack
-  // reader thread: parsing chunks, all of its work, and waiting for queue room
+  // reader thread: parsing chunks, all of its work, and waiting for room
   double t_rparse = 0, t_rbusy = 0, t_rfull = 0;
+  k2host::ReadTiming read;  // parts of t_rparse
+  double t_ralloc = 0, t_rcopy = 0;  // pinned allocation; copies (carry, pairs)
   double t_gpu_wait = 0;  // main thread waiting for a GPU call to finish
 };
 
@@ -300,13 +304,12 @@ double now_s() {
 }
 
 // One kraken output line (src/classify.cc:908-945, 950-991).
-void format_line(const Ctx &c, const Record &s1, const Record *s2, uint32_t call,
-                 const uint32_t *runs, uint64_t nruns, Text &out) {
+// id: mate 1's id; len2: mate 2's bases (paired only).
+void format_line(const Ctx &c, const char *id, size_t id_len, bool paired, uint64_t len1, uint64_t len2,
+                 uint32_t call, const uint32_t *runs, uint64_t nruns, Text &out) {
   const auto &tax = c.tax;
-  const std::string &id = s1.header;
-  size_t id_len = id.size();
   // a pair's id without its /1 or /2 (src/classify.cc:764-771)
-  if (s2 && id_len > 2 && id[id_len - 2] == '/' && (id[id_len - 1] == '1' || id[id_len - 1] == '2'))
+  if (paired && id_len > 2 && id[id_len - 2] == '/' && (id[id_len - 1] == '1' || id[id_len - 1] == '2'))
     id_len -= 2;
   const uint64_t ext = tax.nodes[call].external_id;
   const char *name = nullptr;
@@ -318,7 +321,7 @@ void format_line(const Ctx &c, const Record &s1, const Record *s2, uint32_t call
   // every field's longest text: 20 digits per number, 32 bytes per run
   char *p = out.reserve(id_len + name_len + 128 + 32 * nruns);
   p = put_str(p, call ? "C\t" : "U\t", 2);
-  p = put_str(p, id.data(), id_len);
+  p = put_str(p, id, id_len);
   *p++ = '\t';
   if (name) {
     p = put_str(p, name, name_len);
@@ -329,10 +332,10 @@ void format_line(const Ctx &c, const Record &s1, const Record *s2, uint32_t call
     p = put_uint(p, ext);
   }
   *p++ = '\t';
-  p = put_uint(p, s1.seq.size());
-  if (s2) {
+  p = put_uint(p, len1);
+  if (paired) {
     *p++ = '|';
-    p = put_uint(p, s2->seq.size());
+    p = put_uint(p, len2);
   }
   *p++ = '\t';
   if (c.o.quick_mode) {
@@ -364,98 +367,75 @@ void format_line(const Ctx &c, const Record &s1, const Record *s2, uint32_t call
   out.commit(p);
 }
 
-// The record pair of fragment i of a block.
-void fragment_records(const Options &o, Block &b, size_t i, Record *&r1, Record *&r2) {
-  r2 = nullptr;
-  if (!o.paired_end_processing) {
-    r1 = &b.a[i];
-  } else if (o.single_file_pairs) {
-    r1 = &b.a[2 * i];
-    r2 = &b.a[2 * i + 1];
-  } else {
-    r1 = &b.a[i];
-    r2 = &b.b[i];
-  }
+// Record index of mate 1 of fragment i of a block (mate 2 follows it).
+size_t mate1(const Options &o, const BlockSpan &b, size_t i) {
+  return b.rec_first + (o.paired_end_processing ? 2 * i : i);
 }
 
-// Offsets of the slot's fragments, then each block's mates copied into the
-// slot's pinned buffer (kept across batches: no page faults after the first
-// batch, and the upload reads pinned memory).
-void pack_batch(Ctx &c, Slot &sl) {
+// One record as FASTA/FASTQ text (src/seqreader.cc:20-49).
+void append_record(std::string &out, const Batch &B, const k2host::FlatRec &r, const char *suffix) {
+  const char *t = B.text.data() + r.text_at;
+  out.push_back(r.fastq ? '@' : '>');
+  out.append(t, r.id_len);
+  if (suffix) out += suffix;
+  if (r.cmt_len) {
+    out.push_back(' ');
+    out.append(t + r.id_len, r.cmt_len);
+  }
+  out.push_back('\n');
+  out.append(B.seq.p + r.seq_at, r.seq_len);
+  if (r.fastq) {
+    out += "\n+\n";
+    out.append(t + r.id_len + r.cmt_len, r.seq_len);
+  }
+  out.push_back('\n');
+}
+
+// Fragment offsets of the batch (its bases already lie in fragment order),
+// pieces for the formatting, and low-quality masking (-Q).
+void prepare_batch(Ctx &c, Batch &B) {
   const Options &o = c.o;
   const bool paired = o.paired_end_processing;
-  double t = now_s();
-  size_t nfrag = 0, bases = 0;
-  sl.npieces = 0;
-  for (size_t bi = 0; bi < sl.blocks.size(); bi++) {
-    Block &b = sl.blocks[bi];
-    b.first = nfrag;
-    if (!paired)
-      b.frags = b.a.size();
-    else if (o.single_file_pairs)
-      b.frags = b.a.size() / 2;
-    else
-      b.frags = std::min(b.a.size(), b.b.size());
-    nfrag += b.frags;
-    for (auto &r : b.a) bases += r.seq.size();
-    for (auto &r : b.b) bases += r.seq.size();
+  const double t = now_s();
+  B.off.resize(2 * B.nfrag + 1);
+  B.npieces = 0;
+  uint64_t end = 0;
+  for (size_t bi = 0; bi < B.blocks.size(); bi++) {
+    const BlockSpan &b = B.blocks[bi];
+    for (size_t i = 0; i < b.frags; i++) {
+      const size_t g = b.frag_first + i;
+      const k2host::FlatRec &r1 = B.recs[mate1(o, b, i)];
+      B.off[2 * g] = r1.seq_at;
+      B.off[2 * g + 1] = r1.seq_at + r1.seq_len;
+      end = B.off[2 * g + 1];
+      if (paired) end += B.recs[mate1(o, b, i) + 1].seq_len;
+    }
     for (size_t lo = 0; lo < b.frags; lo += kPieceFrags) {
-      if (sl.npieces == sl.pieces.size()) sl.pieces.emplace_back();
-      Piece &pc = sl.pieces[sl.npieces++];
+      if (B.npieces == B.pieces.size()) B.pieces.emplace_back();
+      Piece &pc = B.pieces[B.npieces++];
       pc.block = bi;
       pc.lo = lo;
       pc.hi = std::min(lo + kPieceFrags, b.frags);
     }
   }
-  double ta = now_s();
-  if (bases + 1 > sl.seq_cap) {
-    if (sl.seq) k2g_host_free(sl.seq);
-    sl.seq = nullptr;
-    sl.seq_cap = std::max<size_t>(bases + 1, sl.seq_cap + sl.seq_cap / 2);
-    const int rc = k2g_host_alloc(sl.seq_cap, (void **) &sl.seq);
-    if (rc) die(rc, k2g_last_error());
-  }
-  double tb = now_s();
-  c.t_alloc += tb - ta;
-  std::vector<uint64_t> &off = sl.off;
-  off.resize(2 * nfrag + 1);
-  uint64_t pos = 0;
-  for (auto &b : sl.blocks) {
-    for (size_t i = 0; i < b.frags; i++) {
-      const size_t f = b.first + i;
-      Record *r1, *r2;
-      fragment_records(o, b, i, r1, r2);
-      off[2 * f] = pos;
-      pos += r1->seq.size();
-      off[2 * f + 1] = pos;
-      if (r2) pos += r2->seq.size();
-    }
-  }
-  off[2 * nfrag] = pos;
-  ta = now_s();
-  c.t_offsets += ta - tb;
-  parallel_for(sl.npieces, o.num_threads, [&](size_t pi) {
-    const Piece &pc = sl.pieces[pi];
-    Block &b = sl.blocks[pc.block];
-    for (size_t i = pc.lo; i < pc.hi; i++) {
-      const size_t f = b.first + i;
-      Record *r1, *r2;
-      fragment_records(o, b, i, r1, r2);
-      if (o.minimum_quality_score > 0) {
-        mask_low_quality(*r1, o.minimum_quality_score);
-        if (r2) mask_low_quality(*r2, o.minimum_quality_score);
-      }
-      std::memcpy(sl.seq + off[2 * f], r1->seq.data(), r1->seq.size());
-      if (r2) std::memcpy(sl.seq + off[2 * f + 1], r2->seq.data(), r2->seq.size());
-    }
-  });
-  sl.nfrag = nfrag;
+  B.off[2 * B.nfrag] = end;
+  if (o.minimum_quality_score > 0)
+    c.pool->run(B.npieces, [&](size_t pi) {
+      const Piece &pc = B.pieces[pi];
+      const BlockSpan &b = B.blocks[pc.block];
+      for (size_t i = pc.lo; i < pc.hi; i++)
+        for (size_t m = 0; m < (paired ? 2u : 1u); m++) {
+          const k2host::FlatRec &r = B.recs[mate1(o, b, i) + m];
+          mask_low_quality(r, B.text.data() + r.text_at + r.id_len + r.cmt_len, B.seq.p + r.seq_at,
+                           o.minimum_quality_score);
+        }
+    });
   c.t_pack += now_s() - t;
 }
 
-// The slot's GPU call (k2g_classify: upload, classify, results into host
-// memory); runs on the GPU thread, so its errors are kept for the main thread.
-void gpu_batch(const Ctx &c, Slot &sl) {
+// The batch's GPU call (k2g_classify: upload, classify, results into host
+// memory); runs on the GPU thread, so its error text is kept in the slot.
+void gpu_batch(const Ctx &c, Batch &B, Slot &sl) {
   const Options &o = c.o;
   k2g_classify_params p{};
   p.confidence = o.confidence_threshold;
@@ -465,23 +445,27 @@ void gpu_batch(const Ctx &c, Slot &sl) {
   p.mode = o.gpu_mode;
   p.want_kmer_counts = !o.report_filename.empty();
   p.want_hll = !o.report_filename.empty() && o.report_kmer_data;
-  k2g_fragments fr{sl.seq, sl.off.data(), nullptr, sl.nfrag};
+  k2g_fragments fr{B.seq.p, B.off.data(), nullptr, B.nfrag};
   const double t = now_s();
+  B.slot = &sl;
   sl.rc = k2g_classify(sl.batch, &p, &fr, &sl.res);
+  if (sl.rc) sl.error = k2g_last_error();
   sl.gpu_s = now_s() - t;
 }
 
-// Output lines of the slot's blocks (formatted in parallel, written in
+// Output lines of the batch's blocks (formatted in parallel, written in
 // order), statistics and counters.
-void finish_batch(Ctx &c, Slot &sl, Outputs &out, const std::string &gpu_error) {
+void finish_batch(Ctx &c, Batch &B, Outputs &out) {
   const Options &o = c.o;
-  if (sl.rc) die(sl.rc, gpu_error);
+  const bool paired = o.paired_end_processing;
+  Slot &sl = *B.slot;
+  if (sl.rc) die(sl.rc, sl.error);
   c.t_gpu += sl.gpu_s;
   const k2g_results &res = sl.res;
   double t = now_s();
-  parallel_for(sl.npieces, o.num_threads, [&](size_t pi) {
-    Piece &pc = sl.pieces[pi];
-    Block &b = sl.blocks[pc.block];
+  c.pool->run(B.npieces, [&](size_t pi) {
+    Piece &pc = B.pieces[pi];
+    const BlockSpan &b = B.blocks[pc.block];
     pc.kraken.n = 0;
     pc.c1.clear();
     pc.c2.clear();
@@ -489,32 +473,32 @@ void finish_batch(Ctx &c, Slot &sl, Outputs &out, const std::string &gpu_error) 
     pc.u2.clear();
     pc.stats = Stats();
     for (size_t i = pc.lo; i < pc.hi; i++) {
-      const size_t g = b.first + i;
-      Record *r1, *r2;
-      fragment_records(o, b, i, r1, r2);
+      const size_t g = b.frag_first + i;
+      const k2host::FlatRec &r1 = B.recs[mate1(o, b, i)];
+      const k2host::FlatRec *r2 = paired ? &B.recs[mate1(o, b, i) + 1] : nullptr;
       const uint32_t call = res.call[g];
       pc.stats.total_sequences++;
       if (call) pc.stats.total_classified++;
-      pc.stats.total_bases += r1->seq.size() + (r2 ? r2->seq.size() : 0);
+      pc.stats.total_bases += r1.seq_len + (r2 ? r2->seq_len : 0);
       if (out.kraken)
-        format_line(c, *r1, r2, call, res.runs + 2 * res.run_off[g], res.run_off[g + 1] - res.run_off[g],
-                    pc.kraken);
+        format_line(c, B.text.data() + r1.text_at, r1.id_len, paired, r1.seq_len, r2 ? r2->seq_len : 0, call,
+                    res.runs + 2 * res.run_off[g], res.run_off[g + 1] - res.run_off[g], pc.kraken);
       if (call) {
         if (out.c1) {
           char suffix[48];
           snprintf(suffix, sizeof(suffix), " kraken:taxid|%llu",
                    (unsigned long long) c.tax.nodes[call].external_id);
-          r1->append_text(pc.c1, suffix);
-          if (r2) r2->append_text(pc.c2, suffix);
+          append_record(pc.c1, B, r1, suffix);
+          if (r2) append_record(pc.c2, B, *r2, suffix);
         }
       } else if (out.u1) {
-        r1->append_text(pc.u1, nullptr);
-        if (r2) r2->append_text(pc.u2, nullptr);
+        append_record(pc.u1, B, r1, nullptr);
+        if (r2) append_record(pc.u2, B, *r2, nullptr);
       }
     }
   });
-  for (size_t pi = 0; pi < sl.npieces; pi++) {
-    const Piece &pc = sl.pieces[pi];
+  for (size_t pi = 0; pi < B.npieces; pi++) {
+    const Piece &pc = B.pieces[pi];
     if (out.kraken) fwrite(pc.kraken.v.data(), 1, pc.kraken.n, out.kraken);
     if (out.c1) fwrite(pc.c1.data(), 1, pc.c1.size(), out.c1);
     if (out.c2) fwrite(pc.c2.data(), 1, pc.c2.size(), out.c2);
@@ -529,18 +513,15 @@ void finish_batch(Ctx &c, Slot &sl, Outputs &out, const std::string &gpu_error) 
   if (!o.report_filename.empty()) {
     // read counts per called taxon; k-mer counters per block (HLL state
     // depends on the block a hit falls in, src/classify.cc:554, 640-645)
-    for (auto &b : sl.blocks)
-      for (size_t i = 0; i < b.frags; i++)
-        if (res.call[b.first + i]) c.counters[res.call[b.first + i]].n_reads++;
+    for (size_t g = 0; g < B.nfrag; g++)
+      if (res.call[g]) c.counters[res.call[g]].n_reads++;
     if (o.report_kmer_data) {
       std::vector<std::pair<uint32_t, uint32_t>> recs;
-      for (auto &b : sl.blocks) {
+      for (const BlockSpan &b : B.blocks) {
         recs.clear();
-        for (size_t i = 0; i < b.frags; i++) {
-          const size_t g = b.first + i;
+        for (size_t g = b.frag_first; g < b.frag_first + b.frags; g++)
           for (uint64_t j = res.hll_off[g]; j < res.hll_off[g + 1]; j++)
             recs.emplace_back(res.hll[2 * j], res.hll[2 * j + 1]);
-        }
         k2host::fold_block_records(recs, c.counters);
       }
     } else {
@@ -551,161 +532,324 @@ void finish_batch(Ctx &c, Slot &sl, Outputs &out, const std::string &gpu_error) 
   c.t_count += now_s() - t2;
 }
 
+// Pinned memory for a batch's bases (grown by copying what it holds).
+void grow_pinned(k2host::SeqSink &s, size_t need) {
+  const size_t cap = std::max(need, s.cap + s.cap / 2);
+  char *p = nullptr;
+  const int rc = k2g_host_alloc(cap, (void **) &p);
+  if (rc) die(rc, k2g_last_error());
+  if (s.n) memcpy(p, s.p, s.n);
+  if (s.p) k2g_host_free(s.p);
+  s.p = p;
+  s.cap = cap;
+}
+
+// Records [r0, end) of `from` become the first records of `to` (empty).
+void carry_records(Batch &from, size_t r0, Batch &to) {
+  if (r0 == from.recs.size()) return;
+  const uint64_t s0 = from.recs[r0].seq_at, t0 = from.recs[r0].text_at;
+  const size_t sn = from.seq.n - s0;
+  if (sn) memcpy(to.seq.room(sn), from.seq.p + s0, sn);
+  to.seq.n = sn;
+  to.text.assign(from.text.begin() + t0, from.text.end());
+  for (size_t r = r0; r < from.recs.size(); r++) {
+    k2host::FlatRec f = from.recs[r];
+    f.seq_at -= s0;
+    f.text_at -= t0;
+    to.recs.push_back(f);
+  }
+  from.recs.resize(r0);
+  from.text.resize(t0);
+  from.seq.n = s0;
+}
+
+// Records of one input file parsed a chunk at a time into its own flat
+// arrays, handed out in order (two-file pairs, whose mates are copied next
+// to each other into the batch).
+struct Stage {
+  k2host::Reader *r = nullptr;
+  std::vector<k2host::FlatRec> recs;
+  std::vector<char> text;
+  k2host::ByteBuf bases;
+  k2host::SeqSink seq;
+  size_t next = 0;
+  size_t chunk = 4u << 20;
+  bool eof = false;
+  Stage() {
+    seq.grow = [this](k2host::SeqSink &s, size_t need) {
+      bases.resize(std::max(need, 2 * bases.size()), s.n);
+      s.p = bases.data();
+      s.cap = bases.size();
+    };
+  }
+};
+
+// Drops a stage's consumed records, moving the rest (and their text and
+// bases) to the front.
+void compact_stage(Stage &s) {
+  if (s.next == 0) return;
+  if (s.next < s.recs.size()) {
+    const uint64_t s0 = s.recs[s.next].seq_at, t0 = s.recs[s.next].text_at;
+    memmove(s.seq.p, s.seq.p + s0, s.seq.n - s0);
+    s.seq.n -= s0;
+    s.text.erase(s.text.begin(), s.text.begin() + t0);
+    s.recs.erase(s.recs.begin(), s.recs.begin() + s.next);
+    for (auto &r : s.recs) {
+      r.seq_at -= s0;
+      r.text_at -= t0;
+    }
+  } else {
+    s.recs.clear();
+    s.text.clear();
+    s.seq.n = 0;
+  }
+  s.next = 0;
+}
+
 // Reads the input in the reference's blocks (src/classify.cc:523-543) on a
-// reader thread, batches of blocks handed to the caller through a two-deep
-// queue, so parsing the next batch overlaps the GPU and the formatting of
-// this one.  Batches, blocks and output stay in input order.
+// reader thread into batches of whole blocks (a two-deep queue), so parsing
+// the next batch overlaps the GPU call and the formatting of this one.
+// Single reads and interleaved pairs are parsed straight into the batch's
+// pinned memory; two-file pairs are parsed per file and their mates copied
+// in side by side.  Batches, blocks and output stay in input order.
 int process_files(Ctx &c, const char *f1, const char *f2, Outputs &out) {
   const Options &o = c.o;
+  const bool paired = o.paired_end_processing, two_files = paired && !o.single_file_pairs;
   k2host::Reader r1, r2;
   if (!r1.open(f1)) die(66, std::string("unable to open ") + (f1 ? f1 : "stdin"));
   if (f2 && !r2.open(f2)) die(66, std::string("unable to open ") + f2);
+  const int parse_threads = o.parse_threads ? o.parse_threads : o.num_threads;
+  k2host::WorkPool parse_pool(parse_threads);
+  r1.set_pool(&parse_pool);
+  r2.set_pool(&parse_pool);
+  r1.reserve_chunk(64u << 20);
+  if (f2) r2.reserve_chunk(64u << 20);
   std::mutex mu;
   std::condition_variable cv;
-  std::deque<std::vector<Block>> ready;
-  std::deque<std::vector<Block>> spare;  // consumed batches, their records reused
+  std::deque<Batch *> ready, spare;  // filled batches; consumed ones, reused
+  std::vector<std::unique_ptr<Batch>> owned;
+  constexpr size_t kMaxBatches = 6;  // one filling, two ready, three on the main thread
   bool finished = false;
+  const size_t kBlockBases = 3 * 1024 * 1024, kPairBlock = 10000;
   std::thread reader([&] {
-    bool done = false;
-    // Each input file's records, parsed a chunk at a time (side by side on
-    // the host threads for plain FASTA), then cut into the reference's
-    // blocks; `used` holds records of consumed blocks, whose buffers the
-    // next chunk's parse reuses.  Chunks start at 4 MiB, so the first block
-    // is ready early, and double up to 64 MiB.
-    struct Stream {
-      k2host::Reader *r;
-      std::vector<Record> parsed, used;
-      size_t next = 0;
-      size_t chunk = 4u << 20;
+    double t_batch = now_s();
+    auto get_batch = [&]() -> Batch * {
+      std::unique_lock<std::mutex> lk(mu);
+      if (spare.empty() && owned.size() < kMaxBatches) {
+        owned.emplace_back(new Batch());
+        owned.back()->seq.grow = [&c](k2host::SeqSink &sk, size_t need) {
+          const double t0 = now_s();
+          grow_pinned(sk, need);
+          c.t_ralloc += now_s() - t0;
+        };
+        return owned.back().get();
+      }
+      const double tw = now_s();
+      cv.wait(lk, [&] { return !spare.empty(); });
+      c.t_rfull += now_s() - tw;
+      Batch *B = spare.front();
+      spare.pop_front();
+      B->clear();
+      return B;
     };
-    Stream s1{&r1}, s2{&r2};
-    auto take = [&](Stream &s, std::vector<Record> &into) -> bool {
-      if (s.next == s.parsed.size()) {
-        s.parsed.clear();
-        s.next = 0;
-        const double t0 = now_s();
-        const size_t got = s.r->next_records(s.chunk, o.num_threads, s.parsed, &s.used);
-        s.chunk = std::min<size_t>(2 * s.chunk, 64u << 20);
-        c.t_rparse += now_s() - t0;
-        if (!got) return false;
-      }
-      into.push_back(std::move(s.parsed[s.next++]));
-      return true;
-    };
-    auto take_n = [&](Stream &s, size_t n, std::vector<Record> &into) {
-      for (auto &r : into) s.used.push_back(std::move(r));
-      into.clear();
-      while (into.size() < n && take(s, into)) {
-      }
-    };
-    // The first batches are small, so the GPU starts early; each later one
-    // doubles up to batch_bases.
-    size_t target = std::min<size_t>(o.batch_bases, 16u << 20);
-    while (!done) {
-      const double t_batch = now_s();
-      std::vector<Block> blocks;
-      {
-        std::lock_guard<std::mutex> lk(mu);
-        if (!spare.empty()) {
-          blocks.swap(spare.front());
-          spare.pop_front();
-        }
-      }
-      size_t bases = 0, used = 0;
-      while (bases < target && used < 4096) {
-        if (used == blocks.size()) blocks.emplace_back();
-        Block &b = blocks[used];
-        bool ok;
-        if (!o.paired_end_processing) {
-          // load_block's rule (src/seqreader.h:101-119): records until
-          // 3 MiB of bases
-          for (auto &r : b.a) s1.used.push_back(std::move(r));
-          b.a.clear();
-          size_t got = 0;
-          while (got < 3 * 1024 * 1024) {
-            if (!take(s1, b.a)) break;
-            got += b.a.back().seq.size();
-          }
-          ok = !b.a.empty();
-        } else if (!o.single_file_pairs) {
-          // load_batch's rule (src/seqreader.h:146-163) on each file
-          take_n(s1, 10000, b.a);
-          take_n(s2, 10000, b.b);
-          ok = !b.a.empty() && !b.b.empty();
-        } else {
-          take_n(s1, 20000, b.a);
-          ok = !b.a.empty();
-        }
-        if (!ok) {
-          done = true;
-          break;
-        }
-        used++;
-        for (auto &r : b.a) bases += r.seq.size();
-        for (auto &r : b.b) bases += r.seq.size();
-      }
-      target = std::min(o.batch_bases, 2 * target);
-      blocks.resize(used);
+    auto ship = [&](Batch *B, bool last) {
       std::unique_lock<std::mutex> lk(mu);
       const double tw = now_s();
       c.t_rbusy += tw - t_batch;
       cv.wait(lk, [&] { return ready.size() < 2; });
       c.t_rfull += now_s() - tw;
-      if (!blocks.empty()) ready.push_back(std::move(blocks));
-      if (done) finished = true;
+      t_batch = now_s();
+      if (B && !B->blocks.empty()) ready.push_back(B);
+      else if (B) spare.push_back(B);
+      if (last) finished = true;
       cv.notify_all();
+    };
+    auto parse_into = [&](k2host::Reader &r, size_t chunk, std::vector<k2host::FlatRec> &recs,
+                          std::vector<char> &text, k2host::SeqSink &seq) {
+      const double t0 = now_s();
+      const size_t got = r.next_flat(chunk, parse_threads, recs, text, seq);
+      c.t_rparse += now_s() - t0;
+      return got;
+    };
+    // The first batches are small, so the GPU starts early; each later one
+    // doubles up to batch_bases.  Parse chunks grow from 4 MiB to 64 MiB.
+    size_t target = std::min<size_t>(o.batch_bases, 16u << 20);
+    size_t chunk = 4u << 20;
+    const size_t kMaxChunk = 64u << 20;
+    Batch *B = get_batch();
+    B->seq.room(target + 2 * chunk);
+    auto close_block = [&](size_t rec_first, size_t nrec, size_t frags, uint64_t bases) {
+      B->blocks.push_back({rec_first, nrec, B->nfrag, frags});
+      B->nfrag += frags;
+      B->bases += bases;
+    };
+    auto full = [&] { return B->bases >= target || B->blocks.size() >= 4096; };
+    auto next_batch = [&](size_t carry_from) {
+      Batch *N = get_batch();
+      target = std::min(o.batch_bases, 2 * target);
+      N->seq.room(target + 2 * chunk);
+      const double t0 = now_s();
+      carry_records(*B, carry_from, *N);
+      c.t_rcopy += now_s() - t0;
+      ship(B, false);
+      B = N;
+    };
+    if (!two_files) {
+      // single reads: records until 3 MiB of bases (load_block,
+      // src/seqreader.h:101-119); interleaved pairs: 20 000 records
+      // (load_batch, src/seqreader.h:146-163), one fragment per two
+      size_t open = 0, scan = 0;  // first record of the open block; records scanned
+      uint64_t open_bases = 0;
+      bool eof = false;
+      for (;;) {
+        bool shipped = false;
+        while (scan < B->recs.size()) {
+          open_bases += B->recs[scan].seq_len;
+          scan++;
+          const bool done = paired ? scan - open == 2 * kPairBlock : open_bases >= kBlockBases;
+          if (!done) continue;
+          close_block(open, scan - open, paired ? (scan - open) / 2 : scan - open, open_bases);
+          open = scan;
+          open_bases = 0;
+          if (full()) {
+            next_batch(open);
+            open = scan = 0;
+            shipped = true;
+            break;
+          }
+        }
+        if (shipped) continue;
+        if (eof) break;
+        // about the bases the batch still needs (and a block's worth), so
+        // little is parsed past the batch's last block and carried over
+        const size_t need = target > B->seq.n ? target - B->seq.n : 0;
+        const size_t want = std::min(chunk, std::max<size_t>(1u << 20, need + need / 16 + kBlockBases));
+        if (!parse_into(r1, want, B->recs, B->text, B->seq)) {
+          eof = true;
+          if (open < B->recs.size()) {
+            close_block(open, B->recs.size() - open, paired ? (B->recs.size() - open) / 2 : B->recs.size() - open,
+                        open_bases);
+            open = scan = B->recs.size();
+          }
+          break;
+        }
+        chunk = std::min(2 * chunk, kMaxChunk);
+      }
+    } else {
+      // two files: 10 000 records from each (load_batch on both,
+      // src/classify.cc:528-532) make min(n1, n2) pairs; the mates are
+      // copied side by side into the batch, pieces of a block in parallel
+      Stage s1, s2;
+      s1.r = &r1;
+      s2.r = &r2;
+      // at least k records ahead of s.next, or all that are left
+      auto ensure = [&](Stage &s, size_t k) {
+        while (!s.eof && s.recs.size() - s.next < k) {
+          const double t0 = now_s();
+          compact_stage(s);
+          c.t_rcopy += now_s() - t0;
+          if (!parse_into(*s.r, s.chunk, s.recs, s.text, s.seq))
+            s.eof = true;
+          else
+            s.chunk = std::min(2 * s.chunk, kMaxChunk);
+        }
+        return std::min(k, s.recs.size() - s.next);
+      };
+      std::vector<uint64_t> tat, sat;  // text and bases of each pair, then their offsets
+      for (;;) {
+        const size_t n1 = ensure(s1, kPairBlock);
+        const size_t n2 = n1 ? ensure(s2, kPairBlock) : 0;
+        if (!n1 || !n2) break;
+        const double t0 = now_s();
+        const size_t n = std::min(n1, n2), first = B->recs.size();
+        auto text_len = [](const k2host::FlatRec &r) {
+          return (uint64_t) r.id_len + r.cmt_len + (r.fastq ? r.seq_len : 0);
+        };
+        tat.resize(n + 1);
+        sat.resize(n + 1);
+        tat[0] = B->text.size();
+        sat[0] = B->seq.n;
+        for (size_t k = 0; k < n; k++) {
+          const k2host::FlatRec &a = s1.recs[s1.next + k], &b = s2.recs[s2.next + k];
+          tat[k + 1] = tat[k] + text_len(a) + text_len(b);
+          sat[k + 1] = sat[k] + a.seq_len + b.seq_len;
+        }
+        B->recs.resize(first + 2 * n);
+        B->text.resize(tat[n]);
+        B->seq.room(sat[n] - sat[0]);
+        const size_t pieces = (n + kPieceFrags - 1) / kPieceFrags;
+        parse_pool.run(pieces, [&](size_t pi) {
+          for (size_t k = pi * kPieceFrags; k < std::min(n, (pi + 1) * kPieceFrags); k++) {
+            uint64_t ta = tat[k], sa = sat[k];
+            for (const Stage *st : {&s1, &s2}) {
+              const k2host::FlatRec &r = st->recs[st->next + k];
+              k2host::FlatRec &f = B->recs[first + 2 * k + (st == &s2)];
+              f = r;
+              f.text_at = ta;
+              f.seq_at = sa;
+              const uint64_t tl = text_len(r);
+              memcpy(B->text.data() + ta, st->text.data() + r.text_at, tl);
+              if (r.seq_len) memcpy(B->seq.p + sa, st->seq.p + r.seq_at, r.seq_len);
+              ta += tl;
+              sa += r.seq_len;
+            }
+          }
+        });
+        B->seq.n = sat[n];
+        s1.next += n1;  // records past the shorter file's end are dropped
+        s2.next += n2;
+        c.t_rcopy += now_s() - t0;
+        close_block(first, 2 * n, n, sat[n] - sat[0]);
+        if (full()) next_batch(B->recs.size());
+      }
     }
+    ship(B, true);
   });
-  // Batch i is packed while batch i-1 is on the GPU; then batch i goes to
-  // the GPU thread while batch i-1 is formatted and written, in input order.
-  std::thread gpu;
-  std::string gpu_error;
-  int cur = 0;
-  bool prev = false;  // slots[cur ^ 1] holds a batch whose GPU call was started
-  auto join_gpu = [&] {
-    if (!gpu.joinable()) return;
+  // Batch i goes to the GPU (its own thread and slot) while batch i-1 may
+  // still be there, so one upload overlaps the other's kernels; then batch
+  // i-2 is formatted and written.  Batches finish in input order.
+  constexpr int kSlots = 3;
+  std::thread gpu[kSlots];
+  Batch *inflight[kSlots] = {};
+  auto finish = [&](int k) {
+    if (!inflight[k]) return;
     const double t = now_s();
-    gpu.join();
+    gpu[k].join();
     c.t_gpu_wait += now_s() - t;
-  };
-  auto recycle = [&](Slot &sl) {
+    finish_batch(c, *inflight[k], out);
     std::lock_guard<std::mutex> lk(mu);
-    if (spare.size() < 2) spare.push_back(std::move(sl.blocks));
-    sl.blocks.clear();
+    spare.push_back(inflight[k]);
+    inflight[k] = nullptr;
+    cv.notify_all();
   };
-  for (;;) {
-    std::vector<Block> blocks;
+  size_t i = 0;
+  for (;; i++) {
+    Batch *B;
     {
       const double t = now_s();
       std::unique_lock<std::mutex> lk(mu);
       cv.wait(lk, [&] { return !ready.empty() || finished; });
       c.t_wait += now_s() - t;
       if (ready.empty()) break;
-      blocks = std::move(ready.front());
+      B = ready.front();
       ready.pop_front();
       cv.notify_all();
     }
-    Slot &sl = c.slots[cur];
-    sl.blocks = std::move(blocks);
-    pack_batch(c, sl);
-    join_gpu();
-    gpu = std::thread([&c, &sl, &gpu_error] {
-      gpu_batch(c, sl);
-      if (sl.rc) gpu_error = k2g_last_error();
-    });
-    if (prev) {
-      finish_batch(c, c.slots[cur ^ 1], out, gpu_error);
-      recycle(c.slots[cur ^ 1]);
-    }
-    prev = true;
-    cur ^= 1;
+    prepare_batch(c, *B);
+    const int k = (int) (i % kSlots);  // batch i-3's slot, finished already
+    inflight[k] = B;
+    Slot &sl = c.slots[k];
+    gpu[k] = std::thread([&c, B, &sl] { gpu_batch(c, *B, sl); });
+    finish((int) ((i + 1) % kSlots));  // batch i-2
   }
-  join_gpu();
-  if (prev) {
-    finish_batch(c, c.slots[cur ^ 1], out, gpu_error);
-    recycle(c.slots[cur ^ 1]);
-  }
+  finish((int) ((i + 1) % kSlots));  // batches i-2 and i-1
+  finish((int) ((i + 2) % kSlots));
   reader.join();
+  for (const k2host::Reader *r : {&r1, &r2}) {
+    c.read.fill += r->timing.fill;
+    c.read.count += r->timing.count;
+    c.read.write += r->timing.write;
+  }
   return 0;
 }
 
@@ -724,10 +868,9 @@ void report_stats(double seconds, const Stats &s) {  // src/classify.cc:447-477
 }  // namespace
 
 int main(int argc, char **argv) {
-  // records' sequence buffers are allocated by the parser threads and
-  // recycled by the main thread: keep freed memory in the heap instead of
-  // returning it to the system (and faulting it in again for the next
-  // chunk), measured 1.4x faster parsing
+  // input chunks, parse arrays and output text grow and shrink batch after
+  // batch: keep freed memory in the heap instead of returning it to the
+  // system (and faulting it in again for the next batch)
   mallopt(M_TRIM_THRESHOLD, 1 << 30);
   mallopt(M_MMAP_THRESHOLD, 1 << 30);
   mallopt(M_TOP_PAD, 256 << 20);
@@ -752,6 +895,8 @@ int main(int argc, char **argv) {
   }
   std::cerr << " done." << std::endl;
 
+  k2host::WorkPool pool(c.o.format_threads ? c.o.format_threads : c.o.num_threads);
+  c.pool = &pool;
   Outputs out;
   open_outputs(c.o, out);
   struct timeval t1, t2;
@@ -779,12 +924,13 @@ int main(int argc, char **argv) {
   report_stats(secs, c.stats);
   if (c.timing)
     fprintf(stderr,
-            "  timing: input wait %.3fs, packing %.3fs (buffer %.3fs, offsets %.3fs), waiting for the GPU %.3fs "
-            "(GPU calls %.3fs, overlapped), format+write %.3fs, counters %.3fs, outside the phases %.3fs; "
-            "reader thread: busy %.3fs (chunk reading and parsing %.3fs), waiting for queue room %.3fs\n",
-            c.t_wait, c.t_pack, c.t_alloc, c.t_offsets, c.t_gpu_wait, c.t_gpu, c.t_format, c.t_count,
+            "  timing: input wait %.3fs, preparing batches %.3fs, waiting for the GPU %.3fs (GPU calls %.3fs, "
+            "overlapped), format+write %.3fs, counters %.3fs, outside the phases %.3fs; reader thread: busy %.3fs "
+            "(chunk reading and parsing %.3fs: reading %.3fs, counting %.3fs, writing %.3fs; pinned allocation "
+            "%.3fs; copies %.3fs), waiting for room %.3fs\n",
+            c.t_wait, c.t_pack, c.t_gpu_wait, c.t_gpu, c.t_format, c.t_count,
             secs - c.t_wait - c.t_pack - c.t_gpu_wait - c.t_format - c.t_count, c.t_rbusy, c.t_rparse,
-            c.t_rfull);
+            c.read.fill, c.read.count, c.read.write, c.t_ralloc, c.t_rcopy, c.t_rfull);
   if (!o.report_filename.empty()) {
     if (!k2host::report_kraken_style(o.report_filename, o.report_zero_counts, o.report_kmer_data,
                                      c.tax, c.counters, c.stats.total_sequences,
@@ -794,10 +940,7 @@ int main(int argc, char **argv) {
   for (FILE *f : {out.c1, out.c2, out.u1, out.u2})
     if (f) fclose(f);
   if (out.kraken && out.kraken != stdout) fclose(out.kraken);
-  for (Slot &sl : c.slots) {
-    k2g_batch_destroy(sl.batch);
-    if (sl.seq) k2g_host_free(sl.seq);
-  }
+  for (Slot &sl : c.slots) k2g_batch_destroy(sl.batch);
   k2g_db_close(c.db);
   return 0;
 }

@@ -2,10 +2,17 @@
 // taxonomy names/ranks, distinct-minimizer estimates and the kreport.
 #pragma once
 #include <cstddef>
+#include <atomic>
+#include <condition_variable>
 #include <cstdint>
+#include <cstring>
+#include <functional>
 #include <map>
+#include <memory>
+#include <mutex>
 #include <set>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -16,6 +23,79 @@ struct Record {
   bool fastq = false;
   std::string header, comment, seq, quals;
   void append_text(std::string &out, const char *header_suffix) const;  // src/seqreader.cc:20-49
+};
+
+// A record parsed into flat arrays: its id, comment and (FASTQ) qualities one
+// after another in a text buffer, its bases in a sequence sink.
+struct FlatRec {
+  uint64_t text_at = 0;  // id, then comment, then seq_len qualities if fastq
+  uint64_t seq_at = 0;   // bases in the sink
+  uint64_t seq_len = 0;
+  uint32_t id_len = 0, cmt_len = 0;
+  bool fastq = false;
+};
+
+// Where a flat parse writes bases: p[0, n) written, p[n, cap) free.
+// grow(sink, need) makes cap >= need and keeps p[0, n) (p may move).
+struct SeqSink {
+  char *p = nullptr;
+  size_t n = 0, cap = 0;
+  std::function<void(SeqSink &, size_t)> grow;
+  char *room(size_t k) {
+    if (n + k > cap) grow(*this, n + k);
+    return p + n;
+  }
+};
+
+// Threads that wait (blocked, not spinning) for work: run(n, fn) calls
+// fn(0..n-1) on the workers and the calling thread, items handed out one at
+// a time, and returns when all are done.  One caller at a time.
+class WorkPool {
+ public:
+  explicit WorkPool(int threads);
+  ~WorkPool();
+  void run(size_t n, const std::function<void(size_t)> &fn);
+  int size() const { return (int) workers_.size() + 1; }
+
+ private:
+  void work();
+  std::vector<std::thread> workers_;
+  std::mutex mu_;
+  std::condition_variable cv_, done_cv_;
+  const std::function<void(size_t)> *fn_ = nullptr;
+  size_t n_ = 0, acked_ = 0;
+  std::atomic<size_t> next_{0};
+  uint64_t gen_ = 0;
+  bool stop_ = false;
+};
+
+// Seconds a reader spent filling its buffer and in the two passes of the
+// flat parse.
+struct ReadTiming {
+  double fill = 0, count = 0, write = 0;
+};
+
+// A byte buffer that is not zero-filled when it grows: only the first
+// `keep` bytes are copied over.
+class ByteBuf {
+ public:
+  char *data() { return p_.get(); }
+  const char *data() const { return p_.get(); }
+  size_t size() const { return n_; }
+  void resize(size_t n, size_t keep) {
+    if (n > cap_) {
+      std::unique_ptr<char[]> q(new char[n]);
+      if (keep) memcpy(q.get(), p_.get(), std::min(keep, n_));
+      p_ = std::move(q);
+      cap_ = n;
+    }
+    n_ = n;
+  }
+  void clear() { n_ = 0; }
+
+ private:
+  std::unique_ptr<char[]> p_;
+  size_t n_ = 0, cap_ = 0;
 };
 
 // Buffered reader (src/kseq.h record rules: '>'/'@' headers, multi-line
@@ -36,6 +116,17 @@ class Reader {
   // pool (may be null): used records whose buffers the parse reuses.
   size_t next_records(size_t bytes, int threads, std::vector<Record> &out,
                       std::vector<Record> *pool = nullptr);
+  // next_records into flat arrays: the records appended to recs, their ids,
+  // comments and qualities to text, their bases to seq back to back (a
+  // plain-FASTA chunk is counted, then written, by parts side by side).
+  size_t next_flat(size_t bytes, int threads, std::vector<FlatRec> &recs, std::vector<char> &text,
+                   SeqSink &seq);
+  // Runs the chunk reads and parse parts on this pool (default: threads
+  // started per chunk).
+  void set_pool(WorkPool *pool) { pool_ = pool; }
+  // Room for chunks of up to `bytes` from the start (no regrowth later).
+  void reserve_chunk(size_t bytes) { buf_.resize(std::max(buf_.size(), bytes + (1 << 20)), end_); }
+  ReadTiming timing;
   // Parses bytes held in memory (as if they were the whole input); they must
   // stay valid while the reader is used.
   void open_memory(const char *p, size_t n);
@@ -43,14 +134,20 @@ class Reader {
  private:
   int get();
   void line_into(std::string &dst, bool append, int *last);
+  // Reads at least `bytes` unread bytes to the front of the buffer: 0 at
+  // the end of input, 1 with cut = the end of the last whole record (plain
+  // FASTA from the start), 2 when only next() can read on (serial_ set).
+  int fill_chunk(size_t bytes, int threads, size_t &cut);
   const char *base() const { return ext_ ? ext_ : buf_.data(); }
   const char *ext_ = nullptr;  // open_memory: the bytes, read in place
   int fd_ = -1;
   bool own_ = false, eof_ = false;
-  std::vector<char> buf_;
+  ByteBuf buf_;
   size_t pos_ = 0, end_ = 0;
   int pending_ = 0;
   bool serial_ = false;  // next_records: no more parallel chunks
+  WorkPool *pool_ = nullptr;
+  void run(int t, const std::function<void(size_t)> &fn);
 };
 
 // taxo.k2d for names, ranks, parents and external ids (src/taxonomy.cc:195-238).

@@ -6,6 +6,7 @@
 // are bases.  A line's single trailing '\r' is dropped.
 #include <algorithm>
 #include <cctype>
+#include <chrono>
 #include <cstring>
 #include <thread>
 #include <cerrno>
@@ -25,7 +26,7 @@ bool Reader::open(const char *path) {
     fd_ = 0;
     own_ = false;
   }
-  buf_.resize(1 << 20);
+  buf_.resize(1 << 20, 0);
   return fd_ >= 0;
 }
 
@@ -162,6 +163,71 @@ void Record::append_text(std::string &out, const char *suffix) const {
 
 namespace k2host {
 
+WorkPool::WorkPool(int threads) {
+  for (int i = 1; i < threads; i++) workers_.emplace_back([this] { work(); });
+}
+
+WorkPool::~WorkPool() {
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    stop_ = true;
+  }
+  cv_.notify_all();
+  for (auto &t : workers_) t.join();
+}
+
+void WorkPool::work() {
+  uint64_t seen = 0;
+  for (;;) {
+    const std::function<void(size_t)> *fn;
+    size_t n;
+    {
+      std::unique_lock<std::mutex> lk(mu_);
+      cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+      if (stop_) return;
+      seen = gen_;
+      fn = fn_;
+      n = n_;
+    }
+    for (size_t i; (i = next_.fetch_add(1, std::memory_order_relaxed)) < n;) (*fn)(i);
+    std::lock_guard<std::mutex> lk(mu_);
+    if (++acked_ == workers_.size()) done_cv_.notify_all();
+  }
+}
+
+void WorkPool::run(size_t n, const std::function<void(size_t)> &fn) {
+  if (n == 0) return;
+  if (workers_.empty() || n == 1) {
+    for (size_t i = 0; i < n; i++) fn(i);
+    return;
+  }
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    fn_ = &fn;
+    n_ = n;
+    next_.store(0, std::memory_order_relaxed);
+    acked_ = 0;
+    gen_++;
+  }
+  cv_.notify_all();
+  for (size_t i; (i = next_.fetch_add(1, std::memory_order_relaxed)) < n;) fn(i);
+  // every worker takes part in every run (if only to find no items left), so
+  // none can still hold this run's fn when the next run starts
+  std::unique_lock<std::mutex> lk(mu_);
+  done_cv_.wait(lk, [&] { return acked_ == workers_.size(); });
+}
+
+void Reader::run(int t, const std::function<void(size_t)> &fn) {
+  if (pool_) {
+    pool_->run((size_t) t, fn);
+    return;
+  }
+  std::vector<std::thread> ts;
+  for (int i = 1; i < t; i++) ts.emplace_back(fn, (size_t) i);
+  fn(0);
+  for (auto &th : ts) th.join();
+}
+
 void Reader::open_memory(const char *p, size_t n) {
   fd_ = -1;
   own_ = false;
@@ -171,6 +237,10 @@ void Reader::open_memory(const char *p, size_t n) {
   pos_ = 0;
   end_ = n;
   pending_ = 0;
+}
+
+static double now_seconds() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
 // First record start ('>' right after '\n') at or after i in d[0..n), or n.
@@ -198,29 +268,14 @@ static bool has_marker_line(const char *d, size_t n) {
   return false;
 }
 
-size_t Reader::next_records(size_t bytes, int threads, std::vector<Record> &out, std::vector<Record> *pool) {
-  const size_t before = out.size();
-  auto serial = [&] {
-    size_t got = 0;
-    Record r;
-    while (got < bytes && next(r)) {
-      got += r.header.size() + r.seq.size() + r.quals.size() + 3;
-      out.push_back(std::move(r));
-      r = Record();
-    }
-    return out.size() - before;
-  };
-  if (serial_ || threads <= 1 || pending_) {
-    serial_ = true;
-    return serial();
-  }
+int Reader::fill_chunk(size_t bytes, int threads, size_t &cut) {
   // gather at least `bytes` unread bytes at the front of buf_
   if (pos_) {
     memmove(buf_.data(), buf_.data() + pos_, end_ - pos_);
     end_ -= pos_;
     pos_ = 0;
   }
-  if (buf_.size() < bytes + (1 << 20)) buf_.resize(bytes + (1 << 20));
+  if (buf_.size() < bytes + (1 << 20)) buf_.resize(bytes + (1 << 20), end_);
   struct stat st;
   if (end_ < bytes && !eof_ && fstat(fd_, &st) == 0 && S_ISREG(st.st_mode)) {
     // a regular file: the chunk is copied in by several preads side by side,
@@ -230,7 +285,7 @@ size_t Reader::next_records(size_t bytes, int threads, std::vector<Record> &out,
       const size_t want = std::min<size_t>(bytes - end_, st.st_size > at ? (size_t) (st.st_size - at) : 0);
       const int t = (int) std::max<size_t>(1, std::min<size_t>((size_t) threads, want >> 20));
       std::vector<ssize_t> got(t, 0);
-      auto piece = [&](int i) {
+      auto piece = [&](size_t i) {
         const size_t lo = want * i / t, hi = want * (i + 1) / t;
         size_t done = 0;
         while (lo + done < hi) {
@@ -241,10 +296,7 @@ size_t Reader::next_records(size_t bytes, int threads, std::vector<Record> &out,
         }
         got[i] = (ssize_t) done;
       };
-      std::vector<std::thread> ts;
-      for (int i = 1; i < t; i++) ts.emplace_back(piece, i);
-      piece(0);
-      for (auto &th : ts) th.join();
+      run(t, piece);
       size_t total = 0;
       bool whole = true;
       for (int i = 0; i < t && whole; i++) {
@@ -271,7 +323,7 @@ size_t Reader::next_records(size_t bytes, int threads, std::vector<Record> &out,
   const size_t n = end_;
   if (n == 0) return 0;
   // cut after the last complete record (everything, at the end of input)
-  size_t cut = n;
+  cut = n;
   if (!eof_) {
     cut = 0;
     for (size_t k = n; k-- > 1;)
@@ -282,14 +334,44 @@ size_t Reader::next_records(size_t bytes, int threads, std::vector<Record> &out,
   }
   if (d[0] != '>' || cut == 0) {
     serial_ = true;  // not plain FASTA here: the record rules need one reader
-    return serial();
+    return 2;
   }
-  // split [0, cut) at record starts and parse the parts side by side
-  const int t = (int) std::min<size_t>((size_t) threads, std::max<size_t>(1, cut >> 12));
+  return 1;
+}
+
+// Part boundaries of [0, cut): t parts starting at record starts.
+static std::vector<size_t> split_parts(const char *d, size_t cut, int t) {
   std::vector<size_t> b(t + 1);
   b[0] = 0;
   for (int i = 1; i < t; i++) b[i] = std::max(b[i - 1], record_start(d, cut, cut * i / t));
   b[t] = cut;
+  return b;
+}
+
+size_t Reader::next_records(size_t bytes, int threads, std::vector<Record> &out, std::vector<Record> *pool) {
+  const size_t before = out.size();
+  auto serial = [&] {
+    size_t got = 0;
+    Record r;
+    while (got < bytes && next(r)) {
+      got += r.header.size() + r.seq.size() + r.quals.size() + 3;
+      out.push_back(std::move(r));
+      r = Record();
+    }
+    return out.size() - before;
+  };
+  if (serial_ || threads <= 1 || pending_) {
+    serial_ = true;
+    return serial();
+  }
+  size_t cut = 0;
+  const int got = fill_chunk(bytes, threads, cut);
+  if (got == 0) return 0;
+  if (got == 2) return serial();
+  const char *d = buf_.data();
+  // split [0, cut) at record starts and parse the parts side by side
+  const int t = (int) std::min<size_t>((size_t) threads, std::max<size_t>(1, cut >> 12));
+  const std::vector<size_t> b = split_parts(d, cut, t);
   std::vector<std::vector<Record>> parts(t), spare(t);
   if (pool) {  // hand each part a share of the used records
     const size_t share = pool->size() / t;
@@ -300,7 +382,7 @@ size_t Reader::next_records(size_t bytes, int threads, std::vector<Record> &out,
       }
   }
   std::vector<char> marker(t, 0);
-  auto parse = [&](int i) {
+  auto parse = [&](size_t i) {
     if (b[i + 1] <= b[i]) return;
     // every line start of the chunk lies in exactly one part (parts begin at
     // record starts)
@@ -323,10 +405,7 @@ size_t Reader::next_records(size_t bytes, int threads, std::vector<Record> &out,
       parts[i].push_back(std::move(rec));
     }
   };
-  std::vector<std::thread> ts;
-  for (int i = 1; i < t; i++) ts.emplace_back(parse, i);
-  parse(0);
-  for (auto &th : ts) th.join();
+  run(t, parse);
   if (std::find(marker.begin(), marker.end(), 1) != marker.end()) {
     // a line starts with '@' or '+': the record rules need one reader from
     // this chunk on (the parsed parts are dropped, their buffers kept)
@@ -345,6 +424,156 @@ size_t Reader::next_records(size_t bytes, int threads, std::vector<Record> &out,
       for (auto &rec : sp) pool->push_back(std::move(rec));
   pos_ = cut;
   return out.size() - before;
+}
+
+// Records, text and bases of a plain-FASTA part d[0, n): n starts at a
+// record start, and with WRITE false only counts them (and stops at a line
+// starting with '@' or '+', setting marker).  The record rules of next()
+// over whole lines: the id runs to the first whitespace, the rest of that
+// line is the comment, sequence lines are joined up to the next line
+// starting with '>', empty lines are skipped, and a line's last '\r' is
+// dropped when the text it ends is longer than one character.
+struct FlatCount {
+  size_t recs = 0, text = 0, seq = 0;
+  bool marker = false;
+};
+
+template <bool WRITE>
+static void scan_fasta(const char *d, size_t n, FlatCount &c, FlatRec *recs, char *text, uint64_t text_at,
+                       char *seq, uint64_t seq_at) {
+  size_t i = 0;
+  while (i < n) {
+    i++;  // the '>' of the record
+    const size_t id0 = i;
+    while (i < n && !isspace((unsigned char) d[i])) i++;
+    const size_t id_len = i - id0;
+    if (id_len == 0 && i == n) break;  // a lone '>' at the end of input: no record
+    size_t c0 = i, cmt_len = 0;
+    if (i < n && d[i] != '\n') {
+      c0 = i + 1;
+      const char *nl = (const char *) memchr(d + c0, '\n', n - c0);
+      const size_t le = nl ? (size_t) (nl - d) : n;
+      cmt_len = le - c0;
+      if (cmt_len > 1 && d[le - 1] == '\r') cmt_len--;
+      i = nl ? le + 1 : n;
+    } else if (i < n) {
+      i++;
+    }
+    char *sp = WRITE ? seq + c.seq : nullptr;
+    size_t slen = 0;
+    while (i < n && d[i] != '>') {
+      if (d[i] == '\n') {
+        i++;
+        continue;
+      }
+      if (!WRITE && (d[i] == '@' || d[i] == '+')) {
+        c.marker = true;
+        return;
+      }
+      const char *nl = (const char *) memchr(d + i, '\n', n - i);
+      const size_t le = nl ? (size_t) (nl - d) : n;
+      // the dropped '\r' is not written: the byte after a part's last base
+      // belongs to the next part
+      size_t keep = le - i;
+      if (slen + keep > 1 && d[le - 1] == '\r') keep--;
+      if (WRITE) memcpy(sp + slen, d + i, keep);
+      slen += keep;
+      i = nl ? le + 1 : n;
+    }
+    if (WRITE) {
+      FlatRec &r = recs[c.recs];
+      r.text_at = text_at + c.text;
+      r.id_len = (uint32_t) id_len;
+      r.cmt_len = (uint32_t) cmt_len;
+      r.seq_at = seq_at + c.seq;
+      r.seq_len = slen;
+      r.fastq = false;
+      memcpy(text + c.text, d + id0, id_len);
+      memcpy(text + c.text + id_len, d + c0, cmt_len);
+    }
+    c.recs++;
+    c.text += id_len + cmt_len;
+    c.seq += slen;
+  }
+}
+
+static void append_flat(const Record &r, std::vector<FlatRec> &recs, std::vector<char> &text, SeqSink &seq) {
+  FlatRec f;
+  f.text_at = text.size();
+  f.id_len = (uint32_t) r.header.size();
+  f.cmt_len = (uint32_t) r.comment.size();
+  f.fastq = r.fastq;
+  text.insert(text.end(), r.header.begin(), r.header.end());
+  text.insert(text.end(), r.comment.begin(), r.comment.end());
+  if (r.fastq) text.insert(text.end(), r.quals.begin(), r.quals.end());  // as many as bases
+  f.seq_at = seq.n;
+  f.seq_len = r.seq.size();
+  if (f.seq_len) memcpy(seq.room(f.seq_len), r.seq.data(), f.seq_len);
+  seq.n += f.seq_len;
+  recs.push_back(f);
+}
+
+size_t Reader::next_flat(size_t bytes, int threads, std::vector<FlatRec> &recs, std::vector<char> &text,
+                         SeqSink &seq) {
+  const size_t before = recs.size();
+  auto serial = [&] {
+    size_t got = 0;
+    Record r;
+    while (got < bytes && next(r)) {
+      got += r.header.size() + r.seq.size() + r.quals.size() + 3;
+      append_flat(r, recs, text, seq);
+    }
+    return recs.size() - before;
+  };
+  if (serial_ || threads <= 1 || pending_) {
+    serial_ = true;
+    return serial();
+  }
+  size_t cut = 0;
+  const double t0 = now_seconds();
+  const int got = fill_chunk(bytes, threads, cut);
+  timing.fill += now_seconds() - t0;
+  if (got == 0) return 0;
+  if (got == 2) return serial();
+  const char *d = buf_.data();
+  const int t = (int) std::min<size_t>((size_t) threads, std::max<size_t>(1, cut >> 12));
+  const std::vector<size_t> b = split_parts(d, cut, t);
+  std::vector<FlatCount> cnt(t);
+  double t1 = now_seconds();
+  run(t, [&](size_t i) { scan_fasta<false>(d + b[i], b[i + 1] - b[i], cnt[i], nullptr, nullptr, 0, nullptr, 0); });
+  double t2 = now_seconds();
+  timing.count += t2 - t1;
+  for (int i = 0; i < t; i++)
+    if (cnt[i].marker) {
+      // a line starts with '@' or '+': the record rules need one reader from
+      // this chunk on
+      serial_ = true;
+      return serial();
+    }
+  std::vector<FlatCount> at(t);  // where each part writes
+  FlatCount total;
+  total.recs = before;
+  total.text = text.size();
+  total.seq = seq.n;
+  for (int i = 0; i < t; i++) {
+    at[i] = total;
+    total.recs += cnt[i].recs;
+    total.text += cnt[i].text;
+    total.seq += cnt[i].seq;
+  }
+  recs.resize(total.recs);
+  text.resize(total.text);
+  seq.room(total.seq - seq.n);
+  t1 = now_seconds();
+  run(t, [&](size_t i) {
+    FlatCount c;
+    scan_fasta<true>(d + b[i], b[i + 1] - b[i], c, recs.data() + at[i].recs, text.data() + at[i].text,
+                     at[i].text, seq.p + at[i].seq, at[i].seq);
+  });
+  timing.write += now_seconds() - t1;
+  seq.n = total.seq;
+  pos_ = cut;
+  return recs.size() - before;
 }
 
 }  // namespace k2host

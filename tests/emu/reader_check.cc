@@ -1,7 +1,9 @@
-// Prints every record of a file twice over: once through Reader::next and
-// once through Reader::next_records with the given chunk size and threads
-// (each chunk's records are handed back as the pool the next chunk parses
-// into), so a test can compare the two record streams.
+// Prints every record of a file three times over: through Reader::next,
+// through Reader::next_records with the given chunk size and threads (each
+// chunk's records are handed back as the pool the next chunk parses into),
+// and through Reader::next_flat on a WorkPool (the flat arrays kept over two
+// chunks, then cleared, so records are appended at nonzero offsets too), so
+// a test can compare the three record streams.
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -33,6 +35,34 @@ int main(int argc, char **argv) {
       pool.push_back(std::move(rec));
     }
     out.clear();
+  }
+  printf("----\n");
+  k2host::Reader rf;
+  if (!rf.open(argv[1])) return 66;
+  k2host::WorkPool workers(threads);  // the flat parse runs its parts on a pool
+  rf.set_pool(&workers);
+  std::vector<k2host::FlatRec> recs;
+  std::vector<char> text, bases;
+  k2host::SeqSink sink;
+  sink.grow = [&](k2host::SeqSink &s, size_t need) {
+    bases.resize(std::max(need, 2 * bases.size()));
+    s.p = bases.data();
+    s.cap = bases.size();
+  };
+  size_t shown = 0;
+  for (int call = 0; rf.next_flat(chunk, threads, recs, text, sink); call++) {
+    for (; shown < recs.size(); shown++) {
+      const k2host::FlatRec &f = recs[shown];
+      const char *t = text.data() + f.text_at;
+      printf("%d|%.*s|%.*s|%.*s|%.*s\n", f.fastq ? 1 : 0, (int) f.id_len, t, (int) f.cmt_len, t + f.id_len,
+             (int) f.seq_len, sink.p + f.seq_at, f.fastq ? (int) f.seq_len : 0, t + f.id_len + f.cmt_len);
+    }
+    if (call % 2) {
+      recs.clear();
+      text.clear();
+      sink.n = 0;
+      shown = 0;
+    }
   }
   return 0;
 }

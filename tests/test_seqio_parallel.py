@@ -1,6 +1,7 @@
-"""The front-ends' chunked record reader (Reader::next_records: plain FASTA
-cut at record starts and parsed on several threads, everything else read
-serially) returns exactly the records of the serial reader (Reader::next),
+"""The front-ends' chunked record readers (Reader::next_records and
+Reader::next_flat: plain FASTA cut at record starts and parsed on several
+threads, everything else read serially) return exactly the records of the
+serial reader (Reader::next),
 on the reference's edge cases: multi-line and CRLF records, comments, empty
 records, FASTQ, and lines starting with '@' or '+' inside FASTA."""
 import os
@@ -21,9 +22,12 @@ def checker():
 
 
 def both(checker, path, chunk, threads):
+    """(serial records, chunked records); the flat parse's records must equal
+    the serial ones too."""
     r = subprocess.run([checker, path, str(chunk), str(threads)], capture_output=True)
     assert r.returncode == 0
-    serial, chunked = r.stdout.split(b"----\n")
+    serial, chunked, flat = r.stdout.split(b"----\n")
+    assert flat == serial
     return serial, chunked
 
 
@@ -82,3 +86,26 @@ def test_chunked_reader_parallel_reads_of_a_large_file(checker, tmp_path):
         serial, chunked = both(checker, str(p), chunk, threads)
         assert serial.count(b"\n") == 30000
         assert chunked == serial
+
+
+@pytest.mark.parametrize("case", [b">a\nAC\n>", b">a\nAC\n>b", b">\nACGT\n>x y\tz\r\nA\r\n\r\nC\r\n",
+                                  b">a b\n\r\n>b\n\rA\n", b">a\x00b c\nA\x00C\n>c\n", b">a\nAC\r",
+                                  b">a\r\nAC\r\n>b\r\n\r\n>c"])
+def test_readers_agree_on_record_edges(checker, case, tmp_path):
+    """A lone '>' or an id at the end of input, empty ids, tabs and NULs in
+    headers, lone '\\r' lines and a last line without '\\n'."""
+    p = tmp_path / "e.fa"
+    p.write_bytes(case)
+    for chunk, threads in ((3, 2), (1 << 20, 4)):
+        serial, chunked = both(checker, str(p), chunk, threads)
+        assert chunked == serial
+
+
+def test_work_pool_runs_each_item_once():
+    """WorkPool::run back to back: every item once, by its own run's function."""
+    r = subprocess.run(["make", "-C", os.path.join(HERE, "emu"), "pool_check"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    for threads in ("2", "8", "16"):
+        r = subprocess.run([os.path.join(HERE, "emu", "pool_check"), threads, "5000"], capture_output=True,
+                           text=True, timeout=120)
+        assert r.returncode == 0, r.stdout
