@@ -96,7 +96,9 @@ typedef struct {
 } k2g_classify_params;
 
 /* Fragment i: mate 1 = seq[off[2i], off[2i+1]), mate 2 = seq[off[2i+1],
- * off[2i+2]) (empty for single reads).  paired may be NULL (then
+ * off[2i+2]).  A single read's mate 2 range is not read: it may be empty or
+ * hold other bytes (the text between records of a FASTA file read in place,
+ * which the classify front-end passes as is).  paired may be NULL (then
  * params.paired applies to all) or give a 0/1 flag per fragment. */
 typedef struct {
   const char *seq;

@@ -41,7 +41,11 @@ struct Options {
   double confidence_threshold = 0;
   int num_threads = 1, minimum_quality_score = 0, minimum_hit_groups = 0;
   int gpu_mode = 2;                 // K2G_MODE: 0 plain, 1 persistent, 2 chosen per batch
-  size_t batch_bases = 256u << 20;  // bases per GPU batch (several blocks)
+  // bases per GPU batch (several blocks): small enough that the pinned
+  // memory of the batches in flight is touched few times over, large
+  // enough that a GPU call's fixed costs stay small
+  size_t batch_bases = 64u << 20;
+  int gpu_calls = 2;  // K2G_GPU_CALLS: GPU calls in flight at once
   // K2G_PARSE_THREADS / K2G_FORMAT_THREADS: threads of the reader's parse and
   // of the formatting (0: -p); the two run at the same time
   int parse_threads = 0, format_threads = 0;
@@ -132,6 +136,7 @@ void parse(int argc, char **argv, Options &o) {
   }
   if (const char *b = getenv("K2G_BATCH_BASES")) o.batch_bases = strtoull(b, nullptr, 10);
   if (const char *t = getenv("K2G_PARSE_THREADS")) o.parse_threads = std::max(0, atoi(t));
+  if (const char *t = getenv("K2G_GPU_CALLS")) o.gpu_calls = std::min(8, std::max(1, atoi(t)));
   if (const char *t = getenv("K2G_FORMAT_THREADS")) o.format_threads = std::max(0, atoi(t));
 }
 
@@ -279,7 +284,7 @@ struct Ctx {
   Options o;
   k2host::Taxonomy tax;
   k2g_db *db = nullptr;
-  Slot slots[3];
+  std::vector<Slot> slots;  // gpu_calls + 1
   k2host::WorkPool *pool = nullptr;  // the main thread's workers (formatting)
   bool translated = false;
   Stats stats;
@@ -544,7 +549,8 @@ void grow_pinned(k2host::SeqSink &s, size_t need) {
   s.cap = cap;
 }
 
-// Records [r0, end) of `from` become the first records of `to` (empty).
+// Records [r0, end) of `from` become the first records of `to` (empty),
+// with the bytes of `from` from the first one's bases on.
 void carry_records(Batch &from, size_t r0, Batch &to) {
   if (r0 == from.recs.size()) return;
   const uint64_t s0 = from.recs[r0].seq_at, t0 = from.recs[r0].text_at;
@@ -628,7 +634,7 @@ int process_files(Ctx &c, const char *f1, const char *f2, Outputs &out) {
   std::condition_variable cv;
   std::deque<Batch *> ready, spare;  // filled batches; consumed ones, reused
   std::vector<std::unique_ptr<Batch>> owned;
-  constexpr size_t kMaxBatches = 6;  // one filling, two ready, three on the main thread
+  const size_t kMaxBatches = c.slots.size() + 3;  // one filling, two ready, the slots'
   bool finished = false;
   const size_t kBlockBases = 3 * 1024 * 1024, kPairBlock = 10000;
   std::thread reader([&] {
@@ -664,10 +670,15 @@ int process_files(Ctx &c, const char *f1, const char *f2, Outputs &out) {
       if (last) finished = true;
       cv.notify_all();
     };
+    // Single reads keep the file's layout (the header text between them is
+    // the single reads' unread mate 2 range); pairs are copied mate next to
+    // mate.
+    const bool in_place = !paired;
     auto parse_into = [&](k2host::Reader &r, size_t chunk, std::vector<k2host::FlatRec> &recs,
                           std::vector<char> &text, k2host::SeqSink &seq) {
       const double t0 = now_s();
-      const size_t got = r.next_flat(chunk, parse_threads, recs, text, seq);
+      const size_t got = in_place ? r.next_inplace(chunk, parse_threads, recs, text, seq)
+                                  : r.next_flat(chunk, parse_threads, recs, text, seq);
       c.t_rparse += now_s() - t0;
       return got;
     };
@@ -676,8 +687,11 @@ int process_files(Ctx &c, const char *f1, const char *f2, Outputs &out) {
     size_t target = std::min<size_t>(o.batch_bases, 16u << 20);
     size_t chunk = 4u << 20;
     const size_t kMaxChunk = 64u << 20;
+    // pinned memory costs most on first touch (its pages are mapped then):
+    // each batch reserves what it will hold, the chunk sizes keep to that
+    auto reserve = [&](Batch *X) { X->seq.room(target + target / 8 + (8u << 20)); };
     Batch *B = get_batch();
-    B->seq.room(target + 2 * chunk);
+    reserve(B);
     auto close_block = [&](size_t rec_first, size_t nrec, size_t frags, uint64_t bases) {
       B->blocks.push_back({rec_first, nrec, B->nfrag, frags});
       B->nfrag += frags;
@@ -687,7 +701,7 @@ int process_files(Ctx &c, const char *f1, const char *f2, Outputs &out) {
     auto next_batch = [&](size_t carry_from) {
       Batch *N = get_batch();
       target = std::min(o.batch_bases, 2 * target);
-      N->seq.room(target + 2 * chunk);
+      reserve(N);
       const double t0 = now_s();
       carry_records(*B, carry_from, *N);
       c.t_rcopy += now_s() - t0;
@@ -805,12 +819,13 @@ int process_files(Ctx &c, const char *f1, const char *f2, Outputs &out) {
     }
     ship(B, true);
   });
-  // Batch i goes to the GPU (its own thread and slot) while batch i-1 may
-  // still be there, so one upload overlaps the other's kernels; then batch
-  // i-2 is formatted and written.  Batches finish in input order.
-  constexpr int kSlots = 3;
-  std::thread gpu[kSlots];
-  Batch *inflight[kSlots] = {};
+  // Batch i goes to the GPU (its own thread and slot) while the batches
+  // before it may still be there (gpu_calls in flight, so uploads overlap
+  // other batches' kernels); then the oldest one is formatted and written.
+  // Batches finish in input order.
+  const int kSlots = (int) c.slots.size();
+  std::vector<std::thread> gpu(kSlots);
+  std::vector<Batch *> inflight(kSlots, nullptr);
   auto finish = [&](int k) {
     if (!inflight[k]) return;
     const double t = now_s();
@@ -836,14 +851,13 @@ int process_files(Ctx &c, const char *f1, const char *f2, Outputs &out) {
       cv.notify_all();
     }
     prepare_batch(c, *B);
-    const int k = (int) (i % kSlots);  // batch i-3's slot, finished already
+    const int k = (int) (i % kSlots);  // batch i - kSlots's slot, finished already
     inflight[k] = B;
     Slot &sl = c.slots[k];
     gpu[k] = std::thread([&c, B, &sl] { gpu_batch(c, *B, sl); });
-    finish((int) ((i + 1) % kSlots));  // batch i-2
+    finish((int) ((i + 1) % kSlots));  // batch i - kSlots + 1
   }
-  finish((int) ((i + 1) % kSlots));  // batches i-2 and i-1
-  finish((int) ((i + 2) % kSlots));
+  for (int j = 1; j < kSlots; j++) finish((int) ((i + j) % kSlots));  // the rest, oldest first
   reader.join();
   for (const k2host::Reader *r : {&r1, &r2}) {
     c.read.fill += r->timing.fill;
@@ -889,6 +903,7 @@ int main(int argc, char **argv) {
   k2g_db_info info;
   k2g_db_get_info(c.db, &info);
   c.translated = !info.dna_db;
+  c.slots.resize(c.o.gpu_calls + 1);
   for (Slot &sl : c.slots) {
     rc = k2g_batch_create(c.db, &sl.batch);
     if (rc) die(rc, k2g_last_error());

@@ -75,6 +75,13 @@ struct ReadTiming {
   double fill = 0, count = 0, write = 0;
 };
 
+// A record whose bases span several lines, for next_inplace to join: its
+// index and the end of its text in the buffer.
+struct InplaceJoin {
+  size_t rec;
+  uint64_t end;
+};
+
 // A byte buffer that is not zero-filled when it grows: only the first
 // `keep` bytes are copied over.
 class ByteBuf {
@@ -121,6 +128,14 @@ class Reader {
   // plain-FASTA chunk is counted, then written, by parts side by side).
   size_t next_flat(size_t bytes, int threads, std::vector<FlatRec> &recs, std::vector<char> &text,
                    SeqSink &seq);
+  // next_flat for plain FASTA without a counting pass: the records are
+  // indexed where they lie in the chunk (a record's later sequence lines
+  // moved up against its first), then the chunk is copied to raw as it
+  // stands: record r's bases are raw.p[seq_at, seq_at + seq_len), with the
+  // header text between records left in between.  Other input goes to
+  // next_flat's serial path (bases back to back).
+  size_t next_inplace(size_t bytes, int threads, std::vector<FlatRec> &recs, std::vector<char> &text,
+                      SeqSink &raw);
   // Runs the chunk reads and parse parts on this pool (default: threads
   // started per chunk).
   void set_pool(WorkPool *pool) { pool_ = pool; }
@@ -138,6 +153,13 @@ class Reader {
   // the end of input, 1 with cut = the end of the last whole record (plain
   // FASTA from the start), 2 when only next() can read on (serial_ set).
   int fill_chunk(size_t bytes, int threads, size_t &cut);
+  // Up to `bytes` more input at dst (several preads side by side for a
+  // regular file); sets eof_ when the input ends.
+  size_t read_into(char *dst, size_t bytes, int threads);
+  // next_inplace's per-part records, text and multi-line records
+  std::vector<std::vector<FlatRec>> inplace_recs_;
+  std::vector<std::vector<char>> inplace_text_;
+  std::vector<std::vector<InplaceJoin>> inplace_joins_;
   const char *base() const { return ext_ ? ext_ : buf_.data(); }
   const char *ext_ = nullptr;  // open_memory: the bytes, read in place
   int fd_ = -1;

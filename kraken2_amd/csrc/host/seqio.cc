@@ -268,6 +268,52 @@ static bool has_marker_line(const char *d, size_t n) {
   return false;
 }
 
+size_t Reader::read_into(char *dst, size_t bytes, int threads) {
+  size_t have = 0;
+  struct stat st;
+  if (!eof_ && fstat(fd_, &st) == 0 && S_ISREG(st.st_mode)) {
+    // a regular file: copied in by several preads side by side, then the
+    // descriptor's offset is moved past what was read
+    const off_t at = lseek(fd_, 0, SEEK_CUR);
+    if (at >= 0) {
+      const size_t want = std::min<size_t>(bytes, st.st_size > at ? (size_t) (st.st_size - at) : 0);
+      const int t = (int) std::max<size_t>(1, std::min<size_t>((size_t) threads, want >> 20));
+      std::vector<ssize_t> got(t, 0);
+      auto piece = [&](size_t i) {
+        const size_t lo = want * i / t, hi = want * (i + 1) / t;
+        size_t done = 0;
+        while (lo + done < hi) {
+          const ssize_t n = pread(fd_, dst + lo + done, hi - lo - done, at + (off_t) (lo + done));
+          if (n < 0 && errno == EINTR) continue;
+          if (n <= 0) break;
+          done += (size_t) n;
+        }
+        got[i] = (ssize_t) done;
+      };
+      run(t, piece);
+      bool whole = true;
+      for (int i = 0; i < t && whole; i++) {
+        const size_t lo = want * i / t, hi = want * (i + 1) / t;
+        have += (size_t) got[i];
+        whole = (size_t) got[i] == hi - lo;
+      }
+      lseek(fd_, at + (off_t) have, SEEK_SET);
+      if (!whole) eof_ = true;  // the file ended (or shrank) inside the chunk
+    }
+  }
+  while (have < bytes && !eof_) {
+    ssize_t n;
+    do {
+      n = ::read(fd_, dst + have, bytes - have);
+    } while (n < 0 && errno == EINTR);
+    if (n <= 0)
+      eof_ = true;
+    else
+      have += (size_t) n;
+  }
+  return have;
+}
+
 int Reader::fill_chunk(size_t bytes, int threads, size_t &cut) {
   // gather at least `bytes` unread bytes at the front of buf_
   if (pos_) {
@@ -276,49 +322,7 @@ int Reader::fill_chunk(size_t bytes, int threads, size_t &cut) {
     pos_ = 0;
   }
   if (buf_.size() < bytes + (1 << 20)) buf_.resize(bytes + (1 << 20), end_);
-  struct stat st;
-  if (end_ < bytes && !eof_ && fstat(fd_, &st) == 0 && S_ISREG(st.st_mode)) {
-    // a regular file: the chunk is copied in by several preads side by side,
-    // then the descriptor's offset is moved past it
-    const off_t at = lseek(fd_, 0, SEEK_CUR);
-    if (at >= 0) {
-      const size_t want = std::min<size_t>(bytes - end_, st.st_size > at ? (size_t) (st.st_size - at) : 0);
-      const int t = (int) std::max<size_t>(1, std::min<size_t>((size_t) threads, want >> 20));
-      std::vector<ssize_t> got(t, 0);
-      auto piece = [&](size_t i) {
-        const size_t lo = want * i / t, hi = want * (i + 1) / t;
-        size_t done = 0;
-        while (lo + done < hi) {
-          const ssize_t n = pread(fd_, buf_.data() + end_ + lo + done, hi - lo - done, at + (off_t) (lo + done));
-          if (n < 0 && errno == EINTR) continue;
-          if (n <= 0) break;
-          done += (size_t) n;
-        }
-        got[i] = (ssize_t) done;
-      };
-      run(t, piece);
-      size_t total = 0;
-      bool whole = true;
-      for (int i = 0; i < t && whole; i++) {
-        const size_t lo = want * i / t, hi = want * (i + 1) / t;
-        total += (size_t) got[i];
-        whole = (size_t) got[i] == hi - lo;
-      }
-      end_ += total;
-      lseek(fd_, at + (off_t) total, SEEK_SET);
-      if (!whole) eof_ = true;  // the file ended (or shrank) inside the chunk
-    }
-  }
-  while (end_ < bytes && !eof_) {
-    ssize_t n;
-    do {
-      n = ::read(fd_, buf_.data() + end_, buf_.size() - end_);
-    } while (n < 0 && errno == EINTR);
-    if (n <= 0)
-      eof_ = true;
-    else
-      end_ += (size_t) n;
-  }
+  if (end_ < bytes && !eof_) end_ += read_into(buf_.data() + end_, bytes - end_, threads);
   const char *d = buf_.data();
   const size_t n = end_;
   if (n == 0) return 0;
@@ -511,6 +515,166 @@ static void append_flat(const Record &r, std::vector<FlatRec> &recs, std::vector
   if (f.seq_len) memcpy(seq.room(f.seq_len), r.seq.data(), f.seq_len);
   seq.n += f.seq_len;
   recs.push_back(f);
+}
+
+// Records of a plain-FASTA part d[0, n) indexed where they lie (the record
+// rules of scan_fasta): seq_at = at + the offset of the record's first
+// sequence line, seq_len its bases.  Records whose bases span several lines
+// are listed in `joins` (record, end of its text) for join_lines; nothing
+// is written to d.  Stops at a line starting with '@' or '+' (marker).
+static void index_fasta(const char *d, size_t n, uint64_t at, std::vector<FlatRec> &recs, std::vector<char> &text,
+                        std::vector<InplaceJoin> &joins, bool &marker) {
+  size_t i = 0;
+  while (i < n) {
+    i++;  // the '>' of the record
+    const size_t id0 = i;
+    while (i < n && !isspace((unsigned char) d[i])) i++;
+    const size_t id_len = i - id0;
+    if (id_len == 0 && i == n) break;  // a lone '>' at the end of input: no record
+    size_t c0 = i, cmt_len = 0;
+    if (i < n && d[i] != '\n') {
+      c0 = i + 1;
+      const char *nl = (const char *) memchr(d + c0, '\n', n - c0);
+      const size_t le = nl ? (size_t) (nl - d) : n;
+      cmt_len = le - c0;
+      if (cmt_len > 1 && d[le - 1] == '\r') cmt_len--;
+      i = nl ? le + 1 : n;
+    } else if (i < n) {
+      i++;
+    }
+    FlatRec r;
+    r.text_at = text.size();
+    r.id_len = (uint32_t) id_len;
+    r.cmt_len = (uint32_t) cmt_len;
+    text.insert(text.end(), d + id0, d + id0 + id_len);
+    text.insert(text.end(), d + c0, d + c0 + cmt_len);
+    uint64_t slen = 0;
+    size_t lines = 0, first = i;
+    while (i < n && d[i] != '>') {
+      if (d[i] == '\n') {
+        i++;
+        continue;
+      }
+      if (d[i] == '@' || d[i] == '+') {
+        marker = true;
+        return;
+      }
+      const char *nl = (const char *) memchr(d + i, '\n', n - i);
+      const size_t le = nl ? (size_t) (nl - d) : n;
+      size_t keep = le - i;
+      if (slen + keep > 1 && d[le - 1] == '\r') keep--;
+      if (lines++ == 0) first = i;
+      slen += keep;
+      i = nl ? le + 1 : n;
+    }
+    r.seq_at = at + first;
+    r.seq_len = slen;
+    if (lines > 1) joins.push_back({recs.size(), at + i});
+    recs.push_back(r);
+  }
+}
+
+// Moves the sequence lines of a record that spans several lines up against
+// its first line: raw[from, end) is the record's text from its first
+// sequence line on (the rules of index_fasta again).
+static void join_lines(char *raw, uint64_t from, uint64_t end) {
+  uint64_t i = from, w = from, slen = 0;
+  while (i < end) {
+    if (raw[i] == '\n') {
+      i++;
+      continue;
+    }
+    const char *nl = (const char *) memchr(raw + i, '\n', end - i);
+    const uint64_t le = nl ? (uint64_t) (nl - raw) : end;
+    uint64_t keep = le - i;
+    if (slen + keep > 1 && raw[le - 1] == '\r') keep--;
+    if (w != i) memmove(raw + w, raw + i, keep);
+    w += keep;
+    slen += keep;
+    i = nl ? le + 1 : end;
+  }
+}
+
+size_t Reader::next_inplace(size_t bytes, int threads, std::vector<FlatRec> &recs, std::vector<char> &text,
+                            SeqSink &raw) {
+  const size_t before = recs.size();
+  auto serial = [&] {
+    size_t got = 0;
+    Record r;
+    while (got < bytes && next(r)) {
+      got += r.header.size() + r.seq.size() + r.quals.size() + 3;
+      append_flat(r, recs, text, raw);
+    }
+    return recs.size() - before;
+  };
+  if (serial_ || threads <= 1 || pending_) {
+    serial_ = true;
+    return serial();
+  }
+  size_t cut = 0;
+  const double t0 = now_seconds();
+  const int got = fill_chunk(bytes, threads, cut);
+  timing.fill += now_seconds() - t0;
+  if (got == 0) return 0;
+  if (got == 2) return serial();
+  char *d = buf_.data();
+  const uint64_t at = raw.n;  // where d[0] lands in raw
+  // parts of about 512 KiB: each is indexed and then copied while it is
+  // still in the core's cache
+  const int t = (int) std::max<size_t>(1, std::min<size_t>(cut >> 12, std::max<size_t>(threads, cut >> 19)));
+  const std::vector<size_t> b = split_parts(d, cut, t);
+  char *dst = raw.room(cut);
+  // per-part arrays, kept across calls (their memory is reused)
+  auto &prec = inplace_recs_;
+  auto &ptext = inplace_text_;
+  auto &pjoin = inplace_joins_;
+  if ((int) prec.size() < t) {
+    prec.resize(t);
+    ptext.resize(t);
+    pjoin.resize(t);
+  }
+  for (int i = 0; i < t; i++) {
+    prec[i].clear();
+    ptext[i].clear();
+    pjoin[i].clear();
+  }
+  std::vector<char> marker(t, 0);
+  double t1 = now_seconds();
+  run(t, [&](size_t i) {
+    bool m = false;
+    index_fasta(d + b[i], b[i + 1] - b[i], at + b[i], prec[i], ptext[i], pjoin[i], m);
+    marker[i] = m;
+    if (!m) memcpy(dst + b[i], d + b[i], b[i + 1] - b[i]);
+  });
+  timing.count += now_seconds() - t1;
+  if (std::find(marker.begin(), marker.end(), 1) != marker.end()) {
+    serial_ = true;  // raw.n is unchanged: next() reads the chunk again
+    return serial();
+  }
+  // multi-line records joined in raw, the parts' arrays one after another
+  t1 = now_seconds();
+  std::vector<size_t> rat(t + 1), tat(t + 1);
+  rat[0] = before;
+  tat[0] = text.size();
+  for (int i = 0; i < t; i++) {
+    rat[i + 1] = rat[i] + prec[i].size();
+    tat[i + 1] = tat[i] + ptext[i].size();
+  }
+  recs.resize(rat[t]);
+  text.resize(tat[t]);
+  run(t, [&](size_t i) {
+    for (const InplaceJoin &j : pjoin[i]) join_lines(raw.p, prec[i][j.rec].seq_at, j.end);
+    for (size_t k = 0; k < prec[i].size(); k++) {
+      FlatRec r = prec[i][k];
+      r.text_at += tat[i];
+      recs[rat[i] + k] = r;
+    }
+    if (!ptext[i].empty()) memcpy(text.data() + tat[i], ptext[i].data(), ptext[i].size());
+  });
+  raw.n += cut;
+  timing.write += now_seconds() - t1;
+  pos_ = cut;
+  return recs.size() - before;
 }
 
 size_t Reader::next_flat(size_t bytes, int threads, std::vector<FlatRec> &recs, std::vector<char> &text,

@@ -1,9 +1,9 @@
-// Prints every record of a file three times over: through Reader::next,
+// Prints every record of a file four times over: through Reader::next,
 // through Reader::next_records with the given chunk size and threads (each
 // chunk's records are handed back as the pool the next chunk parses into),
 // and through Reader::next_flat on a WorkPool (the flat arrays kept over two
-// chunks, then cleared, so records are appended at nonzero offsets too), so
-// a test can compare the three record streams.
+// chunks, then cleared, so records are appended at nonzero offsets too) and
+// through Reader::next_inplace, so a test can compare the record streams.
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -61,6 +61,36 @@ int main(int argc, char **argv) {
       recs.clear();
       text.clear();
       sink.n = 0;
+      shown = 0;
+    }
+  }
+  printf("----\n");
+  // next_inplace: records indexed where they lie, the chunks copied as they
+  // stand (cleared every other call)
+  k2host::Reader ri;
+  if (!ri.open(argv[1])) return 66;
+  ri.set_pool(&workers);
+  recs.clear();
+  text.clear();
+  std::vector<char> raw;
+  k2host::SeqSink rs;
+  rs.grow = [&](k2host::SeqSink &s, size_t need) {
+    raw.resize(std::max(need, 2 * raw.size()));
+    s.p = raw.data();
+    s.cap = raw.size();
+  };
+  shown = 0;
+  for (int call = 0; ri.next_inplace(chunk, threads, recs, text, rs); call++) {
+    for (; shown < recs.size(); shown++) {
+      const k2host::FlatRec &f = recs[shown];
+      const char *t = text.data() + f.text_at;
+      printf("%d|%.*s|%.*s|%.*s|%.*s\n", f.fastq ? 1 : 0, (int) f.id_len, t, (int) f.cmt_len, t + f.id_len,
+             (int) f.seq_len, rs.p + f.seq_at, f.fastq ? (int) f.seq_len : 0, t + f.id_len + f.cmt_len);
+    }
+    if (call % 2) {
+      recs.clear();
+      text.clear();
+      rs.n = 0;
       shown = 0;
     }
   }

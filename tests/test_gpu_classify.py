@@ -249,3 +249,53 @@ def test_mode_chosen_per_batch(env):
         out = batch.classify(seq, off, mode=2, min_hit_groups=2)
         ref = orc.classify(h, seq, off, min_hit_groups=2)
         _compare(k2, db, out, ref, len(reads))
+
+
+def _gapped_singles(reads, genomes, rng):
+    """Single reads laid out as a FASTA file read in place: the bytes after
+    each read (mate 2's range, which a single read does not read) hold a
+    newline, the next header and a stretch of genome that would hit the
+    table if it were scanned."""
+    parts, off, pos = [b">r0 first\n"], [], 10
+    for i, s in enumerate(reads):
+        s = s.encode()
+        off += [pos, pos + len(s)]
+        parts.append(s)
+        pos += len(s)
+        g = genomes[rng.randrange(len(genomes))]
+        p = rng.randrange(len(g) - 200)
+        gap = b"\n>r%d comment\n" % (i + 1) + g[p:p + rng.randint(0, 200)].encode() + b"\n"
+        parts.append(gap)
+        pos += len(gap)
+    off.append(pos)
+    return b"".join(parts), np.array(off, dtype=np.uint64)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_single_reads_ignore_the_mate2_range(env, mode):
+    """Bytes between single reads change nothing: calls, hitlists, k-mer
+    counts and HLL records (per hit, and per block and taxon) equal those of
+    the same reads packed back to back, short and long reads alike."""
+    k2, db, batch, orc, h, genomes = env
+    rng = random.Random(31 + mode)
+    reads = edge_reads(genomes, 1500, seed=500 + mode, min_len=30, max_len=400)
+    for _ in range(12):  # long reads take the segment path
+        g = genomes[rng.randrange(len(genomes))]
+        L = rng.randint(5000, min(len(g), 40000))
+        p = rng.randrange(len(g) - L + 1)
+        reads.insert(rng.randrange(len(reads)), g[p:p + L])
+    seq0, off0 = k2.pack_fragments(reads)
+    seq1, off1 = _gapped_singles(reads, genomes, rng)
+    n = len(reads)
+    for hll in (1, 2):
+        a = batch.classify(seq0, off0, mode=mode, min_hit_groups=2, confidence=0.1, kmer_counts=True, hll=hll)
+        b = batch.classify(seq1, off1, mode=mode, min_hit_groups=2, confidence=0.1, kmer_counts=True, hll=hll)
+        for f in ("call", "hit_groups", "total_kmers", "run_off", "runs", "kmer_counts"):
+            assert np.array_equal(getattr(a, f), getattr(b, f)), (f, hll)
+        if hll == 1:
+            assert np.array_equal(a.hll_off, b.hll_off) and np.array_equal(a.hll, b.hll)
+        else:
+            for key in a.sets:
+                assert np.array_equal(a.sets[key], b.sets[key]), key
+    ref = orc.classify(h, seq0, off0, confidence=0.1, min_hit_groups=2)
+    _compare(k2, db, b, ref, n)

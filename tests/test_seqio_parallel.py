@@ -1,5 +1,6 @@
-"""The front-ends' chunked record readers (Reader::next_records and
-Reader::next_flat: plain FASTA cut at record starts and parsed on several
+"""The front-ends' chunked record readers (Reader::next_records,
+Reader::next_flat and Reader::next_inplace: plain FASTA cut at record
+starts and parsed on several
 threads, everything else read serially) return exactly the records of the
 serial reader (Reader::next),
 on the reference's edge cases: multi-line and CRLF records, comments, empty
@@ -22,12 +23,13 @@ def checker():
 
 
 def both(checker, path, chunk, threads):
-    """(serial records, chunked records); the flat parse's records must equal
-    the serial ones too."""
+    """(serial records, chunked records); the flat and in-place parses'
+    records must equal the serial ones too."""
     r = subprocess.run([checker, path, str(chunk), str(threads)], capture_output=True)
     assert r.returncode == 0
-    serial, chunked, flat = r.stdout.split(b"----\n")
+    serial, chunked, flat, inplace = r.stdout.split(b"----\n")
     assert flat == serial
+    assert inplace == serial
     return serial, chunked
 
 
