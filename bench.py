@@ -21,6 +21,7 @@ Prints one JSON line.
 """
 import argparse
 import ctypes as C
+import hashlib
 import json
 import os
 import re
@@ -657,6 +658,31 @@ def cpu_baseline(work, d, files, threads, exe=None):
                       f"timed interval (after DB load, through parse, classify and output), {total_s:.2f} s"}
 
 
+def outputs_equal(work, d, files, threads, gpu_cli):
+    """The GPU front-end's kraken output on the sample files, byte for byte
+    against the CPU baseline binary's (the reference's compiled classify when
+    present, else the oracle restatement), outside the timed runs."""
+    ref = os.path.join(ROOT, "oracle", "_ref", "classify")
+    cpu_exe = ref if os.path.exists(ref) else os.path.join(ROOT, "oracle", "bin", "k2cpu_classify")
+    for k, args in enumerate(files):
+        digests = []
+        for exe in (cpu_exe, gpu_cli):
+            path = os.path.join(d, f"eq_{k}.kraken")
+            cmd = [exe, "-H", f"{d}/hash.k2d", "-t", f"{d}/taxo.k2d", "-o", f"{d}/opts.k2d", "-p", str(threads),
+                   "-T", str(work["confidence"]), "-g", str(work["min_hit_groups"]), "-O", path, *args]
+            if subprocess.run(cmd, capture_output=True).returncode != 0:
+                return False
+            h = hashlib.sha256()
+            with open(path, "rb") as f:
+                for blk in iter(lambda: f.read(1 << 24), b""):
+                    h.update(blk)
+            digests.append(h.hexdigest())
+            os.remove(path)
+        if digests[0] != digests[1]:
+            return False
+    return True
+
+
 def host_range(hip, d_seq, off, paired, s, e):
     """(seq bytes, rebased 2m+1 offsets, paired flags) of fragments [s, e)."""
     a0, a2 = int(off[2 * s]), int(off[2 * e])
@@ -1086,6 +1112,11 @@ def main():
         if os.path.exists(gpu_cli):
             end_to_end = cpu_baseline(work, tmp, files, os.cpu_count() or 1, exe=gpu_cli)
             log(f"# gpu end to end: {end_to_end}")
+            if end_to_end and cpu:
+                end_to_end["output_equal_" + cpu["kind"]] = outputs_equal(work, tmp, files, os.cpu_count() or 1,
+                                                                          gpu_cli)
+                log(f"# gpu end to end: kraken output equal to the {cpu['kind']} binary's: "
+                    f"{end_to_end['output_equal_' + cpu['kind']]}")
 
     serving = None
     if work["kind"] == "gpu" and host.get("pools"):
