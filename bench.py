@@ -124,13 +124,13 @@ def reference_blocks(off, paired):
     return blocks
 
 
-FRONTEND_BATCH_BASES = 256 << 20   # the classify front-end's GPU batch (csrc/host/classify_main.cc)
+FRONTEND_BATCH_BASES = 64 << 20    # the classify front-end's GPU batch (csrc/host/classify_main.cc)
 FRONTEND_BATCH_BLOCKS = 4096
 
 
 def frontend_batches(blocks, off, paired):
     """The classify front-end's GPU batches over the reference blocks: whole
-    blocks until 256 MiB of bases or 4096 blocks, one launch each; single
+    blocks until 64 MiB of bases or 4096 blocks, one launch each; single
     reads and pairs (separate inputs of the front-end) never share a batch."""
     out, s = [], 0
     while s < len(blocks):
@@ -956,7 +956,7 @@ def main():
     runner = Runner(k2, db, d_seq, d_off, d_paired, work)
 
     passes = {
-        "plain": (fbatches, 0),           # one plain launch per front-end batch (256 MiB of bases)
+        "plain": (fbatches, 0),           # one plain launch per front-end batch (64 MiB of bases)
         "plain_blocks": (blocks, 0),      # one plain launch per reference input block (3 MiB / 10 000 pairs)
         "plain_one_launch": (chunks, 0),  # the plain kernel over the whole read set at once
         "batched": (chunks, 1),           # persistent, longest first
@@ -1148,7 +1148,7 @@ def main():
             "reads_per_s": round(n / plain_s, 1), "ms_per_step": round(plain_s * 1e3, 3),
             "kernel_ms_per_step": round(plain_kms, 3), "launches_per_step": len(fbatches),
             "timed_steps": plain_steps,
-            "batches": "the classify front-end's GPU batches: whole reference blocks up to 256 MiB of bases "
+            "batches": "the classify front-end's GPU batches: whole reference blocks up to 64 MiB of bases "
                        "or 4096 blocks, one plain launch each"},
         "plain_blocks": None if a.skip_plain else {
             "reads_per_s": round(n / pblk_s, 1), "ms_per_step": round(pblk_s * 1e3, 3),
