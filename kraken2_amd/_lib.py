@@ -881,9 +881,12 @@ class Merger:
         return MergeOutput(arr(r.call, n, np.int32), token_off, arr(r.tokens, 2 * m, np.int32).reshape(-1, 2),
                            arr(r.joined, m, np.uint8), float(r.kernel_ms))
 
-    def merge_text(self, lines_a, lines_b, confidence: float = 0.0, item_counts=None) -> MergeOutput:
+    def merge_text(self, lines_a, lines_b, confidence: float = 0.0, item_counts=None,
+                   device_counts=(False, False)) -> MergeOutput:
         """Merges two lists of kraken lines (bytes, newline included) through
-        k2g_merge_text; item_counts overrides the ':' counts (for tests)."""
+        k2g_merge_text; item_counts overrides side a's ':' counts (for tests);
+        device_counts[s] passes no item offsets for side s, so the library
+        counts the items on the GPU (as bin/k2g_merge does)."""
         def pack(lines):
             text = b"".join(lines)
             field, ln, off = [], [], [0]
@@ -900,10 +903,10 @@ class Merger:
         ta, tb = pack(lines_a), pack(lines_b)
         if item_counts is not None:
             ta = ta[:4] + (np.concatenate([[0], np.cumsum(item_counts)]).astype(np.uint64),)
-        a = HitListText(ta[0].ctypes.data, ta[1], ta[2].ctypes.data, ta[3].ctypes.data, ta[4].ctypes.data,
-                        len(lines_a))
-        b = HitListText(tb[0].ctypes.data, tb[1], tb[2].ctypes.data, tb[3].ctypes.data, tb[4].ctypes.data,
-                        len(lines_b))
+        a = HitListText(ta[0].ctypes.data, ta[1], ta[2].ctypes.data, ta[3].ctypes.data,
+                        None if device_counts[0] else ta[4].ctypes.data, len(lines_a))
+        b = HitListText(tb[0].ctypes.data, tb[1], tb[2].ctypes.data, tb[3].ctypes.data,
+                        None if device_counts[1] else tb[4].ctypes.data, len(lines_b))
         r = MergeResults()
         _check(self._lib.k2g_merge_text(self._h, C.byref(MergeParams(confidence, 1)), C.byref(a), C.byref(b),
                                         C.byref(r)))

@@ -164,5 +164,13 @@ def test_merge_text_entry_point_checks_item_counts(tmp_path):
         assert [t.decode() for t in out.text] == [l.split("\t")[4] for l in exp]
         with pytest.raises(k2.K2GError):
             m.merge_text(lines_a, lines_b, item_counts=[5, 1, 1, 3, 3])
+        # item offsets counted on the GPU for one side or both, and empty input
+        want = [l.split("\t")[4] for l in exp]
+        for sides in ((True, True), (True, False), (False, True)):
+            out = m.merge_text(lines_a, lines_b, device_counts=sides)
+            assert [t.decode() for t in out.text] == want, sides
+        for sides in ((True, True), (False, True)):
+            out = m.merge_text([], [], device_counts=sides)
+            assert len(out.text) == 0 and len(out.call) == 0
     finally:
         m.close()
