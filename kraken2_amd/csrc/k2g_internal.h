@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <map>
 #include <memory>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -144,11 +145,19 @@ struct HostArena {
   uint64_t size = 0;
   std::mutex mu;
   ByteArena ranges;
+  // frees idle memory of the arena's owner back into it (false: none left);
+  // called without the arena's lock held
+  std::function<bool()> reclaim;
   void *take(uint64_t bytes) {
     bytes = (bytes + 255) / 256 * 256;
-    std::lock_guard<std::mutex> lk(mu);
-    uint64_t off = 0;
-    return ranges.alloc(bytes, &off) ? base + off : nullptr;
+    for (;;) {
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        uint64_t off = 0;
+        if (ranges.alloc(bytes, &off)) return base + off;
+      }
+      if (!reclaim || !reclaim()) return nullptr;
+    }
   }
   void give(void *p, uint64_t bytes) {
     bytes = (bytes + 255) / 256 * 256;

@@ -207,6 +207,22 @@ extern "C" int k2g_server_create(k2g_db *db, const k2g_server_params *p, k2g_ser
   }
   (void) hipGetLastError();  // a refused size is not an error of the next call
   s->host.ranges = k2g::ByteArena(s->host.size);
+  // a slot whose results do not fit takes back the largest pooled set's
+  // memory before pinning more (freeing pinned memory waits for the whole
+  // device, which stalls every slot)
+  s->host.reclaim = [s] {
+    std::unique_ptr<k2g::HostResults> victim;
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      if (s->pool.empty()) return false;
+      size_t big = 0;
+      for (size_t i = 1; i < s->pool.size(); i++)
+        if (s->pool[i]->bytes() > s->pool[big]->bytes()) big = i;
+      victim = std::move(s->pool[big]);
+      s->pool.erase(s->pool.begin() + big);
+    }
+    return true;  // victim's buffers go back to the arena here
+  };
   for (int i = 0; i < p->slots; i++) s->workers.emplace_back(worker_main, s);
   *out = s;
   return K2G_OK;
