@@ -293,6 +293,11 @@ int k2g_pipeline_get_params(const k2g_pipeline *pl, k2g_pipeline_params *pp, int
 /* Fragments of the last batch classified again: out of reserved hitlist
  * space (into the overflow pool), and with the dense hit table. */
 int k2g_pipeline_redo_counts(const k2g_pipeline *pl, uint64_t *space, uint64_t *dense);
+/* Fragments of the last batch split over waves, and their segments (long
+ * nucleotide fragments, when the batch's longest one outweighs a wave's
+ * share).  No counterpart in the reference, which scans each read on one
+ * thread; for tests and measurement. */
+int k2g_pipeline_split_counts(const k2g_pipeline *pl, uint64_t *fragments, uint64_t *segments);
 int k2g_pipeline_destroy(k2g_pipeline *pl);
 
 /* Autotuner: chooses unit_elements, bucket_shift, probe_depth and

@@ -67,6 +67,7 @@ EXPORTED_SYMBOLS = (
     "k2g_pipeline_digests",
     "k2g_pipeline_get_params",
     "k2g_pipeline_redo_counts",
+    "k2g_pipeline_split_counts",
     "k2g_pipeline_destroy",
     "k2g_pipeline_tune",
     "k2g_merge_open",
@@ -235,6 +236,7 @@ def load_library(path: str = LIB_PATH):
     lib.k2g_pipeline_digests.argtypes = [vp, vp, u64]
     lib.k2g_pipeline_get_params.argtypes = [vp, C.POINTER(PipelineParams), C.POINTER(C.c_int32)]
     lib.k2g_pipeline_redo_counts.argtypes = [vp, C.POINTER(u64), C.POINTER(u64)]
+    lib.k2g_pipeline_split_counts.argtypes = [vp, C.POINTER(u64), C.POINTER(u64)]
     lib.k2g_pipeline_destroy.argtypes = [vp]
     lib.k2g_pipeline_tune.argtypes = [vp, C.POINTER(ClassifyParams), vp, vp, vp, u64, u64, C.POINTER(PipelineParams),
                                       C.POINTER(TuneReport)]
@@ -798,6 +800,12 @@ class Pipeline:
         a, b = C.c_uint64(), C.c_uint64()
         _check(_lib.k2g_pipeline_redo_counts(self._h, C.byref(a), C.byref(b)))
         return a.value, b.value
+
+    def split_counts(self) -> tuple[int, int]:
+        """(fragments of the last batch split over waves, their segments)."""
+        f, g = C.c_uint64(), C.c_uint64()
+        _check(_lib.k2g_pipeline_split_counts(self._h, C.byref(f), C.byref(g)))
+        return f.value, g.value
 
     def close(self):
         if self._h:

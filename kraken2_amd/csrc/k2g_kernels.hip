@@ -1123,15 +1123,42 @@ __global__ __launch_bounds__(256) void k_seg_count(const uint64_t *off, const ui
   cnt[i] = c;
 }
 
+// Step 1 inside a graph-replayed pipeline: the batch from its descriptor,
+// each fragment's elements from the bounds pass, and the split only when the
+// batch's longest fragment outweighs a wave's even share of the batch
+// (stats[0] largest, [1] total elements, of `waves`) and the split
+// fragments' elements fit the scratch (stats[2] <= scratch).
+__global__ __launch_bounds__(256) void k_seg_count_pipe(const PipeDesc *d, uint32_t k, uint32_t seg,
+                                                        const uint64_t *elements, const unsigned long long *stats,
+                                                        uint64_t waves, uint64_t scratch, uint32_t *cnt,
+                                                        uint32_t *list, uint32_t *count) {
+  const uint64_t n = d->n;
+  const uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > n) return;
+  if (i == n) {
+    cnt[n] = 0;
+    return;
+  }
+  const bool want = stats[0] > 2ull * seg && stats[0] * waves > stats[1] && stats[2] <= scratch;
+  uint32_t c = 0;
+  if (want && elements[i] > 2ull * seg) {
+    const bool pr = d->paired ? d->paired[i] != 0 : d->paired_all != 0;
+    c = mate_segments(mate_kmers(d->off[2 * i + 1] - d->off[2 * i], k), seg);
+    if (pr) c += mate_segments(mate_kmers(d->off[2 * i + 2] - d->off[2 * i + 1], k), seg);
+    list[atomicAdd(&count[1], 1u)] = (uint32_t) i;
+  }
+  cnt[i] = c;
+}
+
 // Step 3: one wave per split fragment lays out its segments.  Per mate, the
 // lanes move the evenly spaced cut points forward past ambiguous bases (the k
 // bases before a cut must hold none), then lane 0 walks them from the last to
 // the first, keeping each at or below the next, and writes each segment's
 // range and reserved run space.
-__global__ __launch_bounds__(64) void k_seg_plan(const char *seq, const uint64_t *off, const uint8_t *paired,
-                                                 uint32_t paired_all, uint32_t k, uint32_t seg, uint32_t pct,
-                                                 const uint32_t *list, const uint64_t *seg_start, uint32_t *count,
-                                                 uint64_t seg_total_at, SegBatch sg) {
+__device__ __forceinline__ void seg_plan_body(const char *seq, const uint64_t *off, const uint8_t *paired,
+                                              uint32_t paired_all, uint32_t k, uint32_t seg, uint32_t pct,
+                                              const uint32_t *list, const uint64_t *seg_start, uint32_t *count,
+                                              uint64_t seg_total_at, const SegBatch &sg) {
   const uint32_t nf = count[1];
   const uint32_t lane = threadIdx.x;
   if (blockIdx.x == 0 && lane == 0) count[0] = (uint32_t) seg_start[seg_total_at];
@@ -1178,6 +1205,13 @@ __global__ __launch_bounds__(64) void k_seg_plan(const char *seq, const uint64_t
       s += segs;
     }
   }
+}
+
+__global__ __launch_bounds__(64) void k_seg_plan(const char *seq, const uint64_t *off, const uint8_t *paired,
+                                                 uint32_t paired_all, uint32_t k, uint32_t seg, uint32_t pct,
+                                                 const uint32_t *list, const uint64_t *seg_start, uint32_t *count,
+                                                 uint64_t seg_total_at, SegBatch sg) {
+  seg_plan_body(seq, off, paired, paired_all, k, seg, pct, list, seg_start, count, seg_total_at, sg);
 }
 
 // Segments of split fragments (nucleotide): one wave per segment, each a
@@ -1368,6 +1402,31 @@ K2G_CLASSIFY_VARIANT(_dna, true, false, 8)
 K2G_CLASSIFY_VARIANT(_dna_x, true, true, 7)
 K2G_CLASSIFY_VARIANT(_prot, false, true, 7)
 #undef K2G_CLASSIFY_VARIANT
+
+// The long-fragment split inside a graph-replayed pipeline: the same kernels
+// with the batch read from its descriptor.
+__global__ __launch_bounds__(64) void k_seg_plan_pipe(const PipeDesc *d, uint32_t k, uint32_t seg, uint32_t pct,
+                                                      const uint32_t *list, const uint64_t *seg_start,
+                                                      uint32_t *count, SegBatch sg) {
+  seg_plan_body(d->seq, d->off, d->paired, d->paired_all, k, seg, pct, list, seg_start, count, d->n, sg);
+}
+__global__ __launch_bounds__(64) void k_classify_segments_pipe_dna(DbView db, ClassifyParams P, const PipeDesc *d,
+                                                                   FragBatch in, FragOut out, SegBatch sg) {
+  segment_body<false>(db, P, with_desc(in, d), out, sg);
+}
+__global__ __launch_bounds__(64) void k_classify_segments_pipe_dna_x(DbView db, ClassifyParams P,
+                                                                     const PipeDesc *d, FragBatch in, FragOut out,
+                                                                     SegBatch sg) {
+  segment_body<true>(db, P, with_desc(in, d), out, sg);
+}
+__global__ __launch_bounds__(64) void k_stitch_pipe_dna(DbView db, ClassifyParams P, const PipeDesc *d, FragBatch in,
+                                                        FragOut out, SegBatch sg) {
+  stitch_body<false>(db, P, with_desc(in, d), out, sg);
+}
+__global__ __launch_bounds__(64) void k_stitch_pipe_dna_x(DbView db, ClassifyParams P, const PipeDesc *d,
+                                                          FragBatch in, FragOut out, SegBatch sg) {
+  stitch_body<true>(db, P, with_desc(in, d), out, sg);
+}
 
 __global__ __launch_bounds__(64) void k_classify_dense_pipe_dna(DbView db, ClassifyParams P, const PipeDesc *d,
                                                                  FragBatch in, FragOut out, const uint32_t *redo,

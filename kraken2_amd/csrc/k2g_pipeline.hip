@@ -18,7 +18,11 @@
 //   * the dense-table redo reads its fragment count from device memory;
 //   * with reserved (not exact) hitlist space, fragments that outgrow their
 //     reservation take exact space from an overflow pool by an atomic bump
-//     and are classified again in the same graph.
+//     and are classified again in the same graph;
+//   * long nucleotide fragments are split over waves (the direct path's
+//     segments and stitch) when the device finds that the batch's longest
+//     fragment outweighs a wave's share, so a small batch does not wait on
+//     its longest read.
 // Reference: the per-block loop of ProcessFiles (src/classify.cc:479-708)
 // that the chain replaces, one ClassifySequence per read
 // (src/classify.cc:773-948).
@@ -44,6 +48,16 @@ __global__ void k_classify_dense_pipe_dna(DbView, ClassifyParams, const PipeDesc
                                           const uint32_t *, const uint32_t *, DenseScratch);
 __global__ void k_classify_dense_pipe_prot(DbView, ClassifyParams, const PipeDesc *, FragBatch, FragOut,
                                            const uint32_t *, const uint32_t *, DenseScratch);
+__global__ void k_seg_count_pipe(const PipeDesc *, uint32_t, uint32_t, const uint64_t *, const unsigned long long *,
+                                 uint64_t, uint64_t, uint32_t *, uint32_t *, uint32_t *);
+__global__ void k_seg_plan_pipe(const PipeDesc *, uint32_t, uint32_t, uint32_t, const uint32_t *, const uint64_t *,
+                                uint32_t *, SegBatch);
+__global__ void k_classify_segments_pipe_dna(DbView, ClassifyParams, const PipeDesc *, FragBatch, FragOut,
+                                             SegBatch);
+__global__ void k_classify_segments_pipe_dna_x(DbView, ClassifyParams, const PipeDesc *, FragBatch, FragOut,
+                                               SegBatch);
+__global__ void k_stitch_pipe_dna(DbView, ClassifyParams, const PipeDesc *, FragBatch, FragOut, SegBatch);
+__global__ void k_stitch_pipe_dna_x(DbView, ClassifyParams, const PipeDesc *, FragBatch, FragOut, SegBatch);
 
 namespace {
 
@@ -98,6 +112,17 @@ struct PipeBufs {
                         //     bit 2: overflow pool exhausted; [1] OR of the status words
   unsigned long long *kmer_counts;
   uint32_t node_count;
+  // long nucleotide fragments split over waves (null when the pipeline
+  // does not split): per fragment its elements and segment count, the split
+  // list, segment starts; per segment its layout and outputs; the scratch
+  // of the segments' runs and records
+  uint64_t *elements;
+  unsigned long long *estats;  // [0] largest, [1] total elements, [2] elements of splittable fragments
+  uint32_t *sp_cnt, *sp_list, *sp_count;
+  uint64_t *sp_wide, *sp_start, *sp_tiles;
+  PipeDesc *desc_seg;  // n = the batch's segments (the scan of their run spaces)
+  uint32_t *sp_frag, *sp_mate, *sp_a, *sp_b, *sp_runs_n, *sp_nrec, *sp_groups, *sp_status, *sp_runs, *sp_recs;
+  uint64_t *sp_cap, *sp_off;
   // results
   uint32_t *call, *groups, *total, *n_runs, *status, *n_recs, *runs, *recs;
   uint64_t *comp_off, *comp_rec_off;
@@ -144,6 +169,8 @@ __global__ __launch_bounds__(256) void k_pipe_reset(PipeBufs b) {
   if (t < 4) b.redo_count[t] = 0;
   if (t < 2) b.flags[t] = 0;
   if (t == 0 && b.pool_used) *b.pool_used = 0;
+  if (b.estats && t < 3) b.estats[t] = 0;
+  if (b.sp_count && t < 2) b.sp_count[t] = 0;
   if (b.kmer_counts)
     for (uint32_t i = t; i < b.node_count; i += step) b.kmer_counts[i] = 0;
 }
@@ -158,17 +185,33 @@ __global__ __launch_bounds__(256) void k_pipe_bounds(PipeBufs b, uint32_t k, uin
   const PipeDesc &d = *b.desc;
   const uint64_t n = d.n;
   const uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t e = 0;
   if (i < n) {
     const bool pr = d.paired ? d.paired[i] != 0 : d.paired_all != 0;
     const uint64_t l1 = d.off[2 * i + 1] - d.off[2 * i], l2 = pr ? d.off[2 * i + 2] - d.off[2 * i + 1] : 0;
     if ((l1 | l2) >> 31) atomicOr(&b.flags[0], 1u);
-    const uint64_t e = fragment_elements(l1, l2, pr, k, dna != 0);
+    e = fragment_elements(l1, l2, pr, k, dna != 0);
     b.run_cap[i] = run_space(e, pct);
     const uint32_t bk = length_bucket(e, shift);
     b.bucket_of[i] = (uint16_t) bk;
     atomicAdd(&hist[bk], 1u);
+    if (b.elements) b.elements[i] = e;
   } else if (i == n) {
     b.run_cap[n] = 0;
+  }
+  if (b.estats) {  // largest, total and splittable elements: a wave's reduction, one atomic each
+    uint64_t mx = e, sum = e, split = e > 2ull * kSegElements ? e : 0;
+    for (int dlt = 1; dlt < kWave; dlt <<= 1) {
+      const uint64_t o = (uint64_t) __shfl_xor((unsigned long long) mx, dlt);
+      mx = o > mx ? o : mx;
+      sum += (uint64_t) __shfl_xor((unsigned long long) sum, dlt);
+      split += (uint64_t) __shfl_xor((unsigned long long) split, dlt);
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0 && sum) {
+      atomicMax(&b.estats[0], (unsigned long long) mx);
+      atomicAdd(&b.estats[1], (unsigned long long) sum);
+      if (split) atomicAdd(&b.estats[2], (unsigned long long) split);
+    }
   }
   __syncthreads();
   for (uint32_t j = threadIdx.x; j < kMaxBuckets; j += blockDim.x)
@@ -370,6 +413,17 @@ __global__ void k_pipe_desc2(PipeBufs b) {
   }
 }
 
+// The segments' count as the descriptor of the scan of their run spaces
+// (whose last entry is zeroed for it).
+__global__ void k_pipe_seg_desc(PipeBufs b) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const uint32_t segs = b.sp_count[0];
+  b.sp_cap[segs] = 0;
+  PipeDesc d = *b.desc;
+  d.n = segs;
+  *b.desc_seg = d;
+}
+
 // counts (u32, n from the descriptor) widened into v[0 .. n] (v[n] = 0).
 __global__ __launch_bounds__(256) void k_pipe_widen(const PipeDesc *d, const uint32_t *counts, uint64_t *v) {
   const uint64_t n = d->n;
@@ -385,6 +439,8 @@ __global__ __launch_bounds__(256) void k_pipe_out(PipeBufs b, bool hll) {
     b.h_words[1] = b.flags[1];
     b.h_words[2] = b.redo_count[0];
     b.h_words[3] = b.redo_count[1];
+    b.h_words[4] = b.sp_count ? b.sp_count[1] : 0u;  // fragments split over waves
+    b.h_words[5] = b.sp_count ? b.sp_count[0] : 0u;  // and their segments
   }
   if (b.flags[0] & 2u) return;  // over capacity
   const uint64_t n = b.desc->n;
@@ -410,6 +466,8 @@ __global__ void k_pipe_flags(PipeBufs b) {
     b.h_words[1] = b.flags[1];
     b.h_words[2] = b.redo_count[0];
     b.h_words[3] = b.redo_count[1];
+    b.h_words[4] = b.sp_count ? b.sp_count[1] : 0u;  // fragments split over waves
+    b.h_words[5] = b.sp_count ? b.sp_count[0] : 0u;  // and their segments
   }
 }
 
@@ -457,11 +515,14 @@ struct PipeKernels {
   void (*classify)(DbView, ClassifyParams, const PipeDesc *, FragBatch, FragOut, unsigned long long *);
   void (*dense)(DbView, ClassifyParams, const PipeDesc *, FragBatch, FragOut, const uint32_t *, const uint32_t *,
                 DenseScratch);
+  // the long-fragment split (null: the variant does not split)
+  void (*segments)(DbView, ClassifyParams, const PipeDesc *, FragBatch, FragOut, SegBatch);
+  void (*stitch)(DbView, ClassifyParams, const PipeDesc *, FragBatch, FragOut, SegBatch);
 };
 static const PipeKernels kPipeVariants[3] = {
-    {k_classify_pipe_dna, k_classify_dense_pipe_dna},
-    {k_classify_pipe_dna_x, k_classify_dense_pipe_dna},
-    {k_classify_pipe_prot, k_classify_dense_pipe_prot},
+    {k_classify_pipe_dna, k_classify_dense_pipe_dna, k_classify_segments_pipe_dna, k_stitch_pipe_dna},
+    {k_classify_pipe_dna_x, k_classify_dense_pipe_dna, k_classify_segments_pipe_dna_x, k_stitch_pipe_dna_x},
+    {k_classify_pipe_prot, k_classify_dense_pipe_prot, nullptr, nullptr},
 };
 
 }  // namespace k2g
@@ -478,6 +539,7 @@ struct k2g_pipeline {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   uint64_t max_elements = 0, max_space = 0, pool = 0;
+  uint64_t split_scratch = 0, split_segs = 0;  // 0: no split
   uint32_t pct = 100;
   uint32_t dense_slots = 0;
   int grid = 0;  // persistent workgroups
@@ -491,7 +553,7 @@ struct k2g_pipeline {
   uint32_t *h_call = nullptr, *h_groups = nullptr, *h_total = nullptr, *h_runs = nullptr, *h_recs = nullptr,
            *h_words = nullptr;
   uint64_t *h_run_off = nullptr, *h_rec_off = nullptr, *h_kmer = nullptr;
-  uint64_t last_n = 0, last_dense = 0, last_space = 0;
+  uint64_t last_n = 0, last_dense = 0, last_space = 0, last_split = 0, last_segments = 0;
   float last_ms = 0;
 };
 
@@ -570,8 +632,44 @@ int capture(k2g_pipeline *p) {
   hipLaunchKernelGGL(k_pipe_buckets, dim3(1), dim3(64), 0, s, b, T, shift, p->max_space);
   hipLaunchKernelGGL(k_pipe_scatter, dim3(g256), dim3(256), 0, s, b);
   hipLaunchKernelGGL(k_pipe_units, dim3(g256), dim3(256), 0, s, b);
+  // long fragments split over waves (when the device finds it pays): their
+  // segment counts and starts, each mate's cut points and the segments' run
+  // spaces, then the segments; the classify kernel skips those fragments and
+  // the stitch joins them
+  const bool split = b.sp_cnt != nullptr;
+  const SegBatch sg{b.sp_frag,   b.sp_mate,  b.sp_a,      b.sp_b,      b.sp_off,    b.sp_cap,
+                    b.sp_runs_n, b.sp_nrec,  b.sp_groups, b.sp_status, b.sp_runs,
+                    p->params.want_hll ? b.sp_recs : nullptr,
+                    b.sp_count,  b.sp_list,  b.sp_cnt,    b.sp_start};
+  if (split) {
+    const unsigned seg_tiles = (unsigned) ((p->split_segs + 1 + kTile - 1) / kTile);
+    hipLaunchKernelGGL(k_seg_count_pipe, dim3(g256), dim3(256), 0, s, (const PipeDesc *) b.desc, v.k,
+                       (uint32_t) kSegElements, (const uint64_t *) b.elements, (const unsigned long long *) b.estats,
+                       (uint64_t) p->grid, p->split_scratch, b.sp_cnt, b.sp_list, b.sp_count);
+    hipLaunchKernelGGL(k_pipe_widen, dim3(g256), dim3(256), 0, s, (const PipeDesc *) b.desc,
+                       (const uint32_t *) b.sp_cnt, b.sp_wide);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(256), 0, s, (const PipeDesc *) b.desc, b.sp_wide, b.tiles);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kTopThreads), 0, s, (const PipeDesc *) b.desc, b.tiles);
+    hipLaunchKernelGGL(k_scan_apply, dim3(tiles), dim3(256), 0, s, (const PipeDesc *) b.desc, b.sp_wide, b.tiles,
+                       b.sp_start);
+    hipLaunchKernelGGL(k_seg_plan_pipe, dim3((unsigned) std::min<uint64_t>(nmax, 4096)), dim3(64), 0, s,
+                       (const PipeDesc *) b.desc, v.k, (uint32_t) kSegElements, 100u, (const uint32_t *) b.sp_list,
+                       (const uint64_t *) b.sp_start, b.sp_count, sg);
+    hipLaunchKernelGGL(k_pipe_seg_desc, dim3(1), dim3(64), 0, s, b);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(seg_tiles), dim3(256), 0, s, (const PipeDesc *) b.desc_seg, b.sp_cap,
+                       b.sp_tiles);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kTopThreads), 0, s, (const PipeDesc *) b.desc_seg, b.sp_tiles);
+    hipLaunchKernelGGL(k_scan_apply, dim3(seg_tiles), dim3(256), 0, s, (const PipeDesc *) b.desc_seg, b.sp_cap,
+                       b.sp_tiles, b.sp_off);
+    hipLaunchKernelGGL(K.segments, dim3((unsigned) p->grid), dim3(64), 0, s, v, cp, (const PipeDesc *) b.desc, in,
+                       out, sg);
+    in.split = b.sp_cnt;
+  }
   hipLaunchKernelGGL(K.classify, dim3((unsigned) p->grid), dim3(64), 0, s, v, cp, (const PipeDesc *) b.desc, in,
                      out, b.cursor);
+  if (split)
+    hipLaunchKernelGGL(K.stitch, dim3((unsigned) std::min<uint64_t>(nmax, 16384)), dim3(64), 0, s, v, cp,
+                       (const PipeDesc *) b.desc, in, out, sg);
   hipLaunchKernelGGL(k_pipe_collect, dim3(g256), dim3(256), 0, s, b);
   FragOut out2 = out;
   out2.kmer_counts = nullptr;  // the first pass counted every fragment
@@ -751,6 +849,35 @@ extern "C" int k2g_pipeline_create(k2g_db *db, const k2g_classify_params *p, con
   K2G_D(b.dense_lens, pl->dense_slots);
   K2G_D(b.flags, 2);
   if (kc) K2G_D(b.kmer_counts, v.node_count);
+  if (kPipeVariants[variant_of(db, *p)].segments && !p->quick) {
+    // the split's scratch holds the runs of the split fragments (at most 2^28
+    // elements; a batch whose split fragments need more is not split)
+    pl->split_scratch = std::max<uint64_t>(std::min<uint64_t>(pl->max_elements, 1ull << 28), 1);
+    pl->split_segs = std::min<uint64_t>(2 * pl->split_scratch / kSegElements + 2,
+                                        pl->split_scratch / kSegElements + 2 * nmax + 2);
+    const uint64_t sn = pl->split_segs;
+    K2G_D(b.elements, nmax);
+    K2G_D(b.estats, 3);
+    K2G_D(b.sp_cnt, nmax + 1);
+    K2G_D(b.sp_list, nmax);
+    K2G_D(b.sp_count, 2);
+    K2G_D(b.sp_wide, nmax + 1);
+    K2G_D(b.sp_start, nmax + 1);
+    K2G_D(b.sp_tiles, (sn + 1) / kTile + 2);
+    K2G_D(b.desc_seg, 1);
+    K2G_D(b.sp_frag, sn);
+    K2G_D(b.sp_mate, sn);
+    K2G_D(b.sp_a, sn);
+    K2G_D(b.sp_b, sn);
+    K2G_D(b.sp_runs_n, sn);
+    K2G_D(b.sp_nrec, sn);
+    K2G_D(b.sp_groups, sn);
+    K2G_D(b.sp_status, sn);
+    K2G_D(b.sp_cap, sn + 1);
+    K2G_D(b.sp_off, sn + 1);
+    K2G_D(b.sp_runs, 2 * pl->split_scratch);
+    if (hll) K2G_D(b.sp_recs, 2 * pl->split_scratch);
+  }
   K2G_D(b.call, nmax);
   K2G_D(b.groups, nmax);
   K2G_D(b.total, nmax);
@@ -760,7 +887,7 @@ extern "C" int k2g_pipeline_create(k2g_db *db, const k2g_classify_params *p, con
   K2G_D(b.runs, 2 * (pl->max_space + pl->pool));
   if (hll) K2G_D(b.recs, 2 * (pl->max_space + pl->pool));
   K2G_H(pl->h_desc, 1);
-  K2G_H(pl->h_words, 4);
+  K2G_H(pl->h_words, 6);
   if (pl->host_io) {
     K2G_D(b.seq, bmax + 16);
     K2G_D(b.off, 2 * nmax + 1);
@@ -837,6 +964,8 @@ int launch(k2g_pipeline *pl) {
                                 std::to_string(pl->pool) + " elements) holds; raise run_cap_pct (100 is exact)");
   pl->last_dense = pl->h_words[2];
   pl->last_space = pl->h_words[3];
+  pl->last_split = pl->h_words[4];
+  pl->last_segments = pl->h_words[5];
   return K2G_OK;
 }
 }  // namespace
@@ -950,6 +1079,13 @@ extern "C" int k2g_pipeline_redo_counts(const k2g_pipeline *pl, uint64_t *space,
   if (!pl) return fail(K2G_EUSAGE, "null pipeline");
   if (space) *space = pl->last_space;
   if (dense) *dense = pl->last_dense;
+  return K2G_OK;
+}
+
+extern "C" int k2g_pipeline_split_counts(const k2g_pipeline *pl, uint64_t *fragments, uint64_t *segments) {
+  if (!pl) return fail(K2G_EUSAGE, "null pipeline");
+  if (fragments) *fragments = pl->last_split;
+  if (segments) *segments = pl->last_segments;
   return K2G_OK;
 }
 

@@ -310,3 +310,60 @@ def test_host_pipeline_runs_from_its_staging(env):
             dev.staging(10, 1000)
     finally:
         dev.close()
+
+
+@pytest.mark.parametrize("host_io,extras", [(False, False), (True, True)])
+def test_pipeline_splits_long_reads(env, host_io, extras):
+    """Batches whose longest reads outweigh a wave's share are split over
+    waves inside the graph (segments, then the stitch): results equal
+    k2g_classify's, replay after replay, and a batch of short reads alone is
+    not split."""
+    k2, db, batch, genomes, path = env
+    kw = dict(confidence=0.1, min_hit_groups=2, hll=extras, kmer_counts=extras)
+    rng = random.Random(91)
+    m1 = edge_reads(genomes, 1500, seed=92, min_len=50, max_len=300)
+    for _ in range(12):
+        g = genomes[rng.randrange(len(genomes))]
+        L = rng.randint(12000, min(len(g), 40000))
+        p = rng.randrange(len(g) - L + 1)
+        m1.insert(rng.randrange(len(m1)), g[p:p + L])
+    seq, off = _pack(k2, m1, None)
+    n = len(m1)
+    want = batch.classify(seq, off, mode=1, **kw)
+    pl = k2.Pipeline(db, n, len(seq), host_io=host_io, **kw)
+    try:
+        if host_io:
+            for rep in range(2):
+                _log(f"host pipeline with long reads, replay {rep}")
+                got = pl.run(seq, off)
+                for f in ("call", "hit_groups", "total_kmers", "run_off", "runs"):
+                    assert np.array_equal(getattr(got, f), getattr(want, f)), f
+                if extras:
+                    assert np.array_equal(got.hll_off, want.hll_off) and np.array_equal(got.hll, want.hll)
+                    assert np.array_equal(got.kmer_counts, want.kmer_counts)
+                frags, segs = pl.split_counts()
+                assert frags >= 12 and segs > frags, (frags, segs)
+            short = [r for r in m1 if len(r) < 1000]
+            s2, o2 = _pack(k2, short, None)
+            _log("host pipeline, short reads only")
+            got = pl.run(s2, o2)
+            ref = batch.classify(s2, o2, mode=1, **kw)
+            assert np.array_equal(got.call, ref.call) and np.array_equal(got.runs, ref.runs)
+            assert pl.split_counts() == (0, 0)
+        else:
+            from kraken2_amd.synth import Hip
+            hip = Hip()
+            d_seq, d_off = hip.upload(np.frombuffer(seq, np.uint8)), hip.upload(off)
+            try:
+                for rep in range(2):
+                    _log(f"device pipeline with long reads, replay {rep}")
+                    pl.run_device(d_seq, d_off, None, n)
+                    batch.classify(seq, off, mode=1, **kw)
+                    assert np.array_equal(pl.digests(n), batch.digests(n))
+                    frags, segs = pl.split_counts()
+                    assert frags >= 12 and segs > frags, (frags, segs)
+            finally:
+                hip.free(d_seq)
+                hip.free(d_off)
+    finally:
+        pl.close()
