@@ -630,6 +630,11 @@ int process_files(Ctx &c, const char *f1, const char *f2, Outputs &out) {
   r2.set_pool(&parse_pool);
   r1.reserve_chunk(64u << 20);
   if (f2) r2.reserve_chunk(64u << 20);
+  // FASTQ qualities are kept only for -Q masking and the -C/-U records
+  const bool quals = o.minimum_quality_score > 0 || !o.classified_output_filename.empty() ||
+                     !o.unclassified_output_filename.empty();
+  r1.keep_qualities(quals);
+  r2.keep_qualities(quals);
   std::mutex mu;
   std::condition_variable cv;
   std::deque<Batch *> ready, spare;  // filled batches; consumed ones, reused

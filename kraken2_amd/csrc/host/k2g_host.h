@@ -25,8 +25,9 @@ struct Record {
   void append_text(std::string &out, const char *header_suffix) const;  // src/seqreader.cc:20-49
 };
 
-// A record parsed into flat arrays: its id, comment and (FASTQ) qualities one
-// after another in a text buffer, its bases in a sequence sink.
+// A record parsed into flat arrays: its id, comment and (FASTQ, unless the
+// reader drops them) qualities one after another in a text buffer, its
+// bases in a sequence sink.
 struct FlatRec {
   uint64_t text_at = 0;  // id, then comment, then seq_len qualities if fastq
   uint64_t seq_at = 0;   // bases in the sink
@@ -128,7 +129,8 @@ class Reader {
   // plain-FASTA chunk is counted, then written, by parts side by side).
   size_t next_flat(size_t bytes, int threads, std::vector<FlatRec> &recs, std::vector<char> &text,
                    SeqSink &seq);
-  // next_flat for plain FASTA without a counting pass: the records are
+  // next_flat for plain FASTA without a counting pass (FASTQ as next_flat
+  // reads it): the records are
   // indexed where they lie in the chunk (a record's later sequence lines
   // moved up against its first), then the chunk is copied to raw as it
   // stands: record r's bases are raw.p[seq_at, seq_at + seq_len), with the
@@ -139,6 +141,9 @@ class Reader {
   // Runs the chunk reads and parse parts on this pool (default: threads
   // started per chunk).
   void set_pool(WorkPool *pool) { pool_ = pool; }
+  // next_flat and next_inplace: keep FASTQ qualities in the text (default);
+  // without them a record's text is its id and comment only
+  void keep_qualities(bool keep) { keep_quals_ = keep; }
   // Room for chunks of up to `bytes` from the start (no regrowth later).
   void reserve_chunk(size_t bytes) { buf_.resize(std::max(buf_.size(), bytes + (1 << 20)), end_); }
   ReadTiming timing;
@@ -169,6 +174,9 @@ class Reader {
   int pending_ = 0;
   bool serial_ = false;  // next_records: no more parallel chunks
   WorkPool *pool_ = nullptr;
+  bool keep_quals_ = true;
+  bool flat_fastq(size_t cut, int threads, std::vector<FlatRec> &recs, std::vector<char> &text, SeqSink &seq,
+                  size_t before);
   void run(int t, const std::function<void(size_t)> &fn);
 };
 

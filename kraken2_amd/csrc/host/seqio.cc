@@ -314,6 +314,61 @@ size_t Reader::read_into(char *dst, size_t bytes, int threads) {
   return have;
 }
 
+// The lines of a 4-line FASTQ record: header, one sequence line, '+' line,
+// one quality line (line starts, ends before '\n', and the record's end).
+struct Fastq4 {
+  size_t h1, s0, s1, q0, end;
+  uint64_t slen;
+};
+
+// True if d[p] starts a 4-line FASTQ record whose lines end inside d[0, n)
+// (the quality line may end at n): '@', then a sequence line that next()
+// reads as the whole sequence (not empty, not starting with '>', '+' or
+// '@'), a line starting with '+', and a quality line of as many characters
+// as bases (a line's last '\r' dropped when the text is longer than one
+// character).  next() reads such a record the same way; anything else is
+// left to it.
+static bool fastq4(const char *d, size_t n, size_t p, Fastq4 &r) {
+  if (p >= n || d[p] != '@') return false;
+  auto line_end = [&](size_t i) -> size_t {
+    const char *nl = i < n ? (const char *) memchr(d + i, '\n', n - i) : nullptr;
+    return nl ? (size_t) (nl - d) : n;
+  };
+  r.h1 = line_end(p);
+  r.s0 = r.h1 + 1;
+  if (r.s0 >= n) return false;
+  const char c = d[r.s0];
+  if (c == '\n' || c == '>' || c == '+' || c == '@') return false;
+  r.s1 = line_end(r.s0);
+  const size_t p0 = r.s1 + 1;
+  if (p0 >= n || d[p0] != '+') return false;
+  const size_t p1 = line_end(p0);
+  if (p1 >= n) return false;
+  r.q0 = p1 + 1;
+  const size_t q1 = line_end(r.q0);
+  uint64_t slen = r.s1 - r.s0, qlen = q1 - r.q0;
+  if (slen > 1 && d[r.s1 - 1] == '\r') slen--;
+  if (qlen > 1 && d[q1 - 1] == '\r') qlen--;
+  if (qlen != slen) return false;
+  r.slen = slen;
+  r.end = q1 < n ? q1 + 1 : n;
+  return true;
+}
+
+// First 4-line FASTQ record start at or after i in d[0, n), or n.
+static size_t fastq_record_start(const char *d, size_t n, size_t i) {
+  Fastq4 r;
+  if (i == 0) return fastq4(d, n, 0, r) ? 0 : n;
+  while (i < n) {
+    const char *nl = (const char *) memchr(d + i - 1, '\n', n - (i - 1));
+    if (!nl) return n;
+    const size_t k = (size_t) (nl - d) + 1;
+    if (k < n && d[k] == '@' && fastq4(d, n, k, r)) return k;
+    i = k + 1;
+  }
+  return n;
+}
+
 int Reader::fill_chunk(size_t bytes, int threads, size_t &cut) {
   // gather at least `bytes` unread bytes at the front of buf_
   if (pos_) {
@@ -326,6 +381,23 @@ int Reader::fill_chunk(size_t bytes, int threads, size_t &cut) {
   const char *d = buf_.data();
   const size_t n = end_;
   if (n == 0) return 0;
+  if (d[0] == '@') {
+    // FASTQ: cut after the last whole 4-line record (everything, at the end
+    // of input; the parts' parse checks that the records tile the chunk)
+    cut = eof_ ? n : 0;
+    Fastq4 r;
+    for (size_t k = n; !eof_ && k-- > 1;)
+      if (d[k] == '@' && d[k - 1] == '\n' && fastq4(d, n, k, r) && r.end < n) {
+        cut = r.end;
+        break;
+      }
+    if (!eof_ && cut == 0 && fastq4(d, n, 0, r) && r.end < n) cut = r.end;
+    if (cut == 0) {
+      serial_ = true;
+      return 2;
+    }
+    return 3;
+  }
   // cut after the last complete record (everything, at the end of input)
   cut = n;
   if (!eof_) {
@@ -344,12 +416,50 @@ int Reader::fill_chunk(size_t bytes, int threads, size_t &cut) {
 }
 
 // Part boundaries of [0, cut): t parts starting at record starts.
-static std::vector<size_t> split_parts(const char *d, size_t cut, int t) {
+static std::vector<size_t> split_parts(const char *d, size_t cut, int t, bool fastq = false) {
   std::vector<size_t> b(t + 1);
   b[0] = 0;
-  for (int i = 1; i < t; i++) b[i] = std::max(b[i - 1], record_start(d, cut, cut * i / t));
+  for (int i = 1; i < t; i++)
+    b[i] = std::max(b[i - 1], fastq ? fastq_record_start(d, cut, cut * i / t) : record_start(d, cut, cut * i / t));
   b[t] = cut;
   return b;
+}
+
+// Records of a FASTQ part d[0, n) (starting at a record start), indexed
+// where they lie: every record must be a 4-line record (fastq4) and the
+// records must end exactly at n, else `unusual` is set and the chunk is left
+// to next().  Text per record: id, comment, qualities.
+static void index_fastq(const char *d, size_t n, uint64_t at, std::vector<FlatRec> &recs, std::vector<char> &text,
+                        bool quals, bool &unusual) {
+  size_t i = 0;
+  Fastq4 q;
+  while (i < n) {
+    if (!fastq4(d, n, i, q) || q.end > n) {
+      unusual = true;
+      return;
+    }
+    size_t j = i + 1;
+    while (j < q.h1 && !isspace((unsigned char) d[j])) j++;
+    const size_t id_len = j - (i + 1);
+    size_t c0 = j, cmt_len = 0;
+    if (j < q.h1) {  // the whitespace after the id, then the comment
+      c0 = j + 1;
+      cmt_len = q.h1 - c0;
+      if (cmt_len > 1 && d[q.h1 - 1] == '\r') cmt_len--;
+    }
+    FlatRec r;
+    r.text_at = text.size();
+    r.id_len = (uint32_t) id_len;
+    r.cmt_len = (uint32_t) cmt_len;
+    r.fastq = true;
+    r.seq_at = at + q.s0;
+    r.seq_len = q.slen;
+    text.insert(text.end(), d + i + 1, d + i + 1 + id_len);
+    text.insert(text.end(), d + c0, d + c0 + cmt_len);
+    if (quals) text.insert(text.end(), d + q.q0, d + q.q0 + q.slen);
+    recs.push_back(r);
+    i = q.end;
+  }
 }
 
 size_t Reader::next_records(size_t bytes, int threads, std::vector<Record> &out, std::vector<Record> *pool) {
@@ -371,7 +481,10 @@ size_t Reader::next_records(size_t bytes, int threads, std::vector<Record> &out,
   size_t cut = 0;
   const int got = fill_chunk(bytes, threads, cut);
   if (got == 0) return 0;
-  if (got == 2) return serial();
+  if (got != 1) {  // FASTQ is read by next() here
+    serial_ = true;
+    return serial();
+  }
   const char *d = buf_.data();
   // split [0, cut) at record starts and parse the parts side by side
   const int t = (int) std::min<size_t>((size_t) threads, std::max<size_t>(1, cut >> 12));
@@ -501,7 +614,8 @@ static void scan_fasta(const char *d, size_t n, FlatCount &c, FlatRec *recs, cha
   }
 }
 
-static void append_flat(const Record &r, std::vector<FlatRec> &recs, std::vector<char> &text, SeqSink &seq) {
+static void append_flat(const Record &r, std::vector<FlatRec> &recs, std::vector<char> &text, SeqSink &seq,
+                        bool quals = true) {
   FlatRec f;
   f.text_at = text.size();
   f.id_len = (uint32_t) r.header.size();
@@ -509,7 +623,7 @@ static void append_flat(const Record &r, std::vector<FlatRec> &recs, std::vector
   f.fastq = r.fastq;
   text.insert(text.end(), r.header.begin(), r.header.end());
   text.insert(text.end(), r.comment.begin(), r.comment.end());
-  if (r.fastq) text.insert(text.end(), r.quals.begin(), r.quals.end());  // as many as bases
+  if (r.fastq && quals) text.insert(text.end(), r.quals.begin(), r.quals.end());  // as many as bases
   f.seq_at = seq.n;
   f.seq_len = r.seq.size();
   if (f.seq_len) memcpy(seq.room(f.seq_len), r.seq.data(), f.seq_len);
@@ -603,7 +717,7 @@ size_t Reader::next_inplace(size_t bytes, int threads, std::vector<FlatRec> &rec
     Record r;
     while (got < bytes && next(r)) {
       got += r.header.size() + r.seq.size() + r.quals.size() + 3;
-      append_flat(r, recs, text, raw);
+      append_flat(r, recs, text, raw, keep_quals_);
     }
     return recs.size() - before;
   };
@@ -617,6 +731,9 @@ size_t Reader::next_inplace(size_t bytes, int threads, std::vector<FlatRec> &rec
   timing.fill += now_seconds() - t0;
   if (got == 0) return 0;
   if (got == 2) return serial();
+  // FASTQ: the bases copied back to back (its quality lines are as long as
+  // the bases, so the file's layout would double the bytes moved)
+  if (got == 3) return flat_fastq(cut, threads, recs, text, raw, before) ? recs.size() - before : serial();
   char *d = buf_.data();
   const uint64_t at = raw.n;  // where d[0] lands in raw
   // parts of about 512 KiB: each is indexed and then copied while it is
@@ -677,6 +794,66 @@ size_t Reader::next_inplace(size_t bytes, int threads, std::vector<FlatRec> &rec
   return recs.size() - before;
 }
 
+bool Reader::flat_fastq(size_t cut, int threads, std::vector<FlatRec> &recs, std::vector<char> &text, SeqSink &seq,
+                        size_t before) {
+  // the parts indexed, then each record's bases copied back to back
+  const char *d = buf_.data();
+  const int t = (int) std::max<size_t>(1, std::min<size_t>(cut >> 12, std::max<size_t>(threads, cut >> 19)));
+  const std::vector<size_t> b = split_parts(d, cut, t, true);
+  auto &prec = inplace_recs_;
+  auto &ptext = inplace_text_;
+  if ((int) prec.size() < t) {
+    prec.resize(t);
+    ptext.resize(t);
+    inplace_joins_.resize(t);
+  }
+  std::vector<char> unusual(t, 0);
+  double t1 = now_seconds();
+  run(t, [&](size_t i) {
+    prec[i].clear();
+    ptext[i].clear();
+    bool m = false;
+    index_fastq(d + b[i], b[i + 1] - b[i], b[i], prec[i], ptext[i], keep_quals_, m);
+    unusual[i] = m;
+  });
+  timing.count += now_seconds() - t1;
+  if (std::find(unusual.begin(), unusual.end(), 1) != unusual.end()) {
+    serial_ = true;  // nothing was written: next() reads the chunk again
+    return false;
+  }
+  t1 = now_seconds();
+  std::vector<uint64_t> rat(t + 1), tat(t + 1), sat(t + 1);
+  rat[0] = before;
+  tat[0] = text.size();
+  sat[0] = seq.n;
+  for (int i = 0; i < t; i++) {
+    uint64_t bases = 0;
+    for (const FlatRec &r : prec[i]) bases += r.seq_len;
+    rat[i + 1] = rat[i] + prec[i].size();
+    tat[i + 1] = tat[i] + ptext[i].size();
+    sat[i + 1] = sat[i] + bases;
+  }
+  recs.resize(rat[t]);
+  text.resize(tat[t]);
+  seq.room(sat[t] - seq.n);
+  run(t, [&](size_t i) {
+    uint64_t pos = sat[i];
+    for (size_t k = 0; k < prec[i].size(); k++) {
+      FlatRec r = prec[i][k];
+      memcpy(seq.p + pos, d + r.seq_at, r.seq_len);
+      r.seq_at = pos;
+      r.text_at += tat[i];
+      pos += r.seq_len;
+      recs[rat[i] + k] = r;
+    }
+    if (!ptext[i].empty()) memcpy(text.data() + tat[i], ptext[i].data(), ptext[i].size());
+  });
+  seq.n = sat[t];
+  timing.write += now_seconds() - t1;
+  pos_ = cut;
+  return true;
+}
+
 size_t Reader::next_flat(size_t bytes, int threads, std::vector<FlatRec> &recs, std::vector<char> &text,
                          SeqSink &seq) {
   const size_t before = recs.size();
@@ -685,7 +862,7 @@ size_t Reader::next_flat(size_t bytes, int threads, std::vector<FlatRec> &recs, 
     Record r;
     while (got < bytes && next(r)) {
       got += r.header.size() + r.seq.size() + r.quals.size() + 3;
-      append_flat(r, recs, text, seq);
+      append_flat(r, recs, text, seq, keep_quals_);
     }
     return recs.size() - before;
   };
@@ -700,6 +877,7 @@ size_t Reader::next_flat(size_t bytes, int threads, std::vector<FlatRec> &recs, 
   if (got == 0) return 0;
   if (got == 2) return serial();
   const char *d = buf_.data();
+  if (got == 3) return flat_fastq(cut, threads, recs, text, seq, before) ? recs.size() - before : serial();
   const int t = (int) std::min<size_t>((size_t) threads, std::max<size_t>(1, cut >> 12));
   const std::vector<size_t> b = split_parts(d, cut, t);
   std::vector<FlatCount> cnt(t);

@@ -111,3 +111,53 @@ def test_work_pool_runs_each_item_once():
         r = subprocess.run([os.path.join(HERE, "emu", "pool_check"), threads, "5000"], capture_output=True,
                            text=True, timeout=120)
         assert r.returncode == 0, r.stdout
+
+
+def random_fastq(rng, n, eol="\n", weird_at=None):
+    """4-line FASTQ whose quality lines often start with '@' or '+' (what a
+    parallel cut must not mistake for a record start)."""
+    out = []
+    for i in range(n):
+        seq = "".join(rng.choice("ACGTN") for _ in range(rng.randrange(1, 300)))
+        qual = "".join(rng.choice("@+!#5?IJ") for _ in range(len(seq)))
+        head = f"@q{i}" + (f" note {i}" if rng.random() < 0.3 else "")
+        plus = "+" + (f"q{i}" if rng.random() < 0.2 else "")
+        e = eol if rng.random() < 0.9 else "\r\n"
+        rec = head + e + seq + e + plus + e + qual + e
+        if i == weird_at:  # a two-line sequence: next() alone reads it
+            rec = head + e + seq + e + seq + e + plus + e + qual + qual + e
+        out.append(rec)
+    return "".join(out)
+
+
+@pytest.mark.parametrize("chunk,threads", [(1 << 26, 8), (3000, 4), (700, 3), (100000, 16)])
+def test_chunked_readers_match_serial_on_fastq(checker, chunk, threads, tmp_path):
+    rng = random.Random(chunk + 7)
+    p = tmp_path / "r.fq"
+    p.write_text(random_fastq(rng, 3000))
+    serial, chunked = both(checker, str(p), chunk, threads)
+    assert serial.count(b"\n") == 3000
+    assert chunked == serial
+
+
+def test_fastq_readers_hand_over_at_a_multiline_record(checker, tmp_path):
+    rng = random.Random(9)
+    p = tmp_path / "m.fq"
+    p.write_text(random_fastq(rng, 2000, weird_at=1500) + ">fa\nACGT\n@z\nAC\n+\n@@\n")
+    for chunk, threads in ((4000, 4), (1 << 20, 8)):
+        serial, chunked = both(checker, str(p), chunk, threads)
+        assert chunked == serial
+
+
+@pytest.mark.parametrize("case", [b"@a\nACGT\n+\nIIII", b"@a\nACGT\n+\nIII\n@b\nAC\n+\nII\n",
+                                  b"@a x\r\nAC\r\n+\r\n@@\r\n@b\nA\n+\n+\n", b"@a\nACGT\n+\nIIIII\n@b\nA\n+\nI\n",
+                                  b"@a\n\n+\n\n@b\nA\n+\nI\n", b"@a\nA\n+\n", b"@\nAC\n+\nII\n@"])
+def test_fastq_record_edges(checker, case, tmp_path):
+    """Quality line without '\\n' at the end, a short or long quality line
+    (next() ends the input there), CRLF, an empty sequence, a record cut
+    after its '+' line, empty ids and a lone '@' at the end."""
+    p = tmp_path / "e.fq"
+    p.write_bytes(case)
+    for chunk, threads in ((3, 2), (1 << 20, 4)):
+        serial, chunked = both(checker, str(p), chunk, threads)
+        assert chunked == serial
