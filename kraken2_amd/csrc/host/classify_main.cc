@@ -780,7 +780,13 @@ int process_files(Ctx &c, const char *f1, const char *f2, Outputs &out) {
         const size_t n2 = n1 ? ensure(s2, kPairBlock) : 0;
         if (!n1 || !n2) break;
         const double t0 = now_s();
-        const size_t n = std::min(n1, n2), first = B->recs.size();
+        // every whole block both files hold now (10 000 pairs each) goes in
+        // one parallel copy, up to the block that fills the batch; else the
+        // one block that is left, of min(n1, n2) pairs
+        const bool whole = n1 == kPairBlock && n2 == kPairBlock;
+        size_t n = whole ? std::min(s1.recs.size() - s1.next, s2.recs.size() - s2.next) / kPairBlock * kPairBlock
+                         : std::min(n1, n2);
+        const size_t first = B->recs.size();
         auto text_len = [](const k2host::FlatRec &r) {
           return (uint64_t) r.id_len + r.cmt_len + (r.fastq ? r.seq_len : 0);
         };
@@ -792,6 +798,17 @@ int process_files(Ctx &c, const char *f1, const char *f2, Outputs &out) {
           const k2host::FlatRec &a = s1.recs[s1.next + k], &b = s2.recs[s2.next + k];
           tat[k + 1] = tat[k] + text_len(a) + text_len(b);
           sat[k + 1] = sat[k] + a.seq_len + b.seq_len;
+        }
+        size_t blocks = 1;
+        if (whole) {
+          blocks = 0;
+          uint64_t bases = B->bases;
+          while (blocks * kPairBlock < n) {
+            bases += sat[(blocks + 1) * kPairBlock] - sat[blocks * kPairBlock];
+            blocks++;
+            if (bases >= target || B->blocks.size() + blocks >= 4096) break;
+          }
+          n = blocks * kPairBlock;
         }
         B->recs.resize(first + 2 * n);
         B->text.resize(tat[n]);
@@ -815,10 +832,14 @@ int process_files(Ctx &c, const char *f1, const char *f2, Outputs &out) {
           }
         });
         B->seq.n = sat[n];
-        s1.next += n1;  // records past the shorter file's end are dropped
-        s2.next += n2;
+        // records past the shorter file's end are dropped with the last block
+        s1.next += whole ? n : n1;
+        s2.next += whole ? n : n2;
         c.t_rcopy += now_s() - t0;
-        close_block(first, 2 * n, n, sat[n] - sat[0]);
+        for (size_t bk = 0; bk < blocks; bk++) {
+          const size_t lo = whole ? bk * kPairBlock : 0, hi = whole ? lo + kPairBlock : n;
+          close_block(first + 2 * lo, 2 * (hi - lo), hi - lo, sat[hi] - sat[lo]);
+        }
         if (full()) next_batch(B->recs.size());
       }
     }
