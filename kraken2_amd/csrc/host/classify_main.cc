@@ -787,8 +787,9 @@ int process_files(Ctx &c, const char *f1, const char *f2, Outputs &out) {
         size_t n = whole ? std::min(s1.recs.size() - s1.next, s2.recs.size() - s2.next) / kPairBlock * kPairBlock
                          : std::min(n1, n2);
         const size_t first = B->recs.size();
-        auto text_len = [](const k2host::FlatRec &r) {
-          return (uint64_t) r.id_len + r.cmt_len + (r.fastq ? r.seq_len : 0);
+        // a FASTQ record's text holds its qualities only when the readers keep them
+        auto text_len = [quals](const k2host::FlatRec &r) {
+          return (uint64_t) r.id_len + r.cmt_len + (r.fastq && quals ? r.seq_len : 0);
         };
         tat.resize(n + 1);
         sat.resize(n + 1);

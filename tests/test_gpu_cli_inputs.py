@@ -129,3 +129,44 @@ def test_interleaved_pairs_with_an_odd_record(tmp_path):
     p.write_text("".join(f">p{i // 2}/{i % 2 + 1}\n{s}\n" for i, s in enumerate(reads)))
     ref, got = _both(["-S", str(p)], tmp_path, env={"K2G_BATCH_BASES": "100000"})
     assert got == ref and got.count(b"\n") == 1000
+
+
+def _fastq(reads, rng, prefix):
+    return "".join(f"@{prefix}{i} q{i % 5}\n{s}\n+\n{''.join(chr(33 + rng.randrange(41)) for _ in s)}\n"
+                   for i, s in enumerate(reads))
+
+
+@pytest.mark.parametrize("quals", ["dropped", "masked", "written"])
+def test_two_file_fastq_pairs(tmp_path, quals):
+    """Two-file FASTQ pairs over several 10 000-pair blocks, with uneven
+    files (the longer one's extra records are dropped) and batches small
+    enough to hold a few blocks: qualities dropped by the reader (no -Q, no
+    -C/-U), used for -Q masking, and written back in the -C/-U records."""
+    rng = random.Random(6)
+    g = _genomes()
+    m1 = _reads(rng, g, 25003, 60, 250)
+    m2 = _reads(rng, g, 25000, 60, 250)
+    f1, f2 = tmp_path / "a_1.fq", tmp_path / "a_2.fq"
+    f1.write_text(_fastq(m1, rng, "p"))
+    f2.write_text(_fastq(m2, rng, "p"))
+    args = ["-P", str(f1), str(f2)]
+    if quals == "masked":
+        args = ["-Q", "20"] + args
+    outs = []
+    for exe in (ORACLE_CLI, CLI):
+        extra = []
+        if quals == "written":
+            extra = ["-C", str(tmp_path / f"{os.path.basename(exe)}_c#.fq"),
+                     "-U", str(tmp_path / f"{os.path.basename(exe)}_u#.fq")]
+        base = ["-H", f"{TOY}/hash.k2d", "-t", f"{TOY}/taxo.k2d", "-o", f"{TOY}/opts.k2d", "-p", "4", "-g", "2"]
+        env = dict(os.environ, K2G_BATCH_BASES="4000000") if exe == CLI else None
+        r = subprocess.run([exe, *base, *extra, *args], capture_output=True, env=env, timeout=300)
+        assert r.returncode == 0, (exe, r.stderr[-500:])
+        outs.append(r.stdout)
+    assert outs[1] == outs[0] and outs[1].count(b"\n") == 25000
+    if quals == "written":
+        for kind in ("c", "u"):
+            for mate in ("1", "2"):
+                ref = (tmp_path / f"k2cpu_classify_{kind}_{mate}.fq").read_bytes()
+                got = (tmp_path / f"k2g_classify_{kind}_{mate}.fq").read_bytes()
+                assert got == ref
